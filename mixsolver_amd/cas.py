@@ -1,0 +1,145 @@
+"""Fluent ASCII ``.cas`` writer and reader for owner/neighbour face meshes.
+
+The layout is the subset that the reference reader consumes
+(mod_uns_cas_reader.f90:44-82 dispatch, :266-307 nodes, :315-410 cells,
+:412-526 faces): section 2 dimension, 10 nodes, 12 cells, 13 faces (hex
+indices, one zone per record, mixed face type 0 when triangles and quads
+share a zone), 45 zone names.  Face order in the file is the face order of
+the mesh, which is the order the solver's face loops walk.  Coordinates are
+written with 17 significant digits so that text round trips reproduce every
+double exactly.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .meshgen import Mesh
+
+# Fluent boundary condition codes for section-13 face zones
+_COND = {"interior": 2, "wall": 3, "pressure-outlet": 5, "symmetry": 7,
+         "velocity-inlet": 10, "outflow": 36, "pressure-far-field": 9}
+
+
+def write_cas(path: str, m: Mesh, zone_kind: dict | None = None) -> None:
+    """Write mesh m; zone_kind maps a face zone id to a Fluent kind name."""
+    zone_kind = dict(zone_kind or {})
+    nn, nf, nc = m.nnodes, m.nfaces, m.ncells
+    fnn = m.face_nn()
+    with open(path, "w") as f:
+        f.write('(0 "mixsolver_amd generated mesh")\n(2 3)\n')
+        f.write(f"(10 (0 1 {nn:x} 0 3))\n(13 (0 1 {nf:x} 0))\n(12 (0 1 {nc:x} 0))\n")
+        f.write(f"(10 (1 1 {nn:x} 1 3)(\n")
+        np.savetxt(f, m.x, fmt="%.17e")
+        f.write("))\n")
+        f.write(f"(12 ({m.cell_zone:x} 1 {nc:x} 1 {m.ctype:x}))\n")
+        f.write(f"(45 ({m.cell_zone} fluid fluid)())\n")
+        zones = m.zone
+        start = 0
+        while start < nf:
+            z = int(zones[start])
+            stop = start
+            while stop < nf and zones[stop] == z:
+                stop += 1
+            kinds = fnn[start:stop]
+            mixed = bool(np.any(kinds != kinds[0]))
+            ftype = 0 if mixed else int(kinds[0])
+            kind = zone_kind.get(z, "interior" if z == 3 else "wall")
+            cond = _COND.get(kind, 3)
+            f.write(f"(13 ({z:x} {start + 1:x} {stop:x} {cond:x} {ftype:x})(\n")
+            lines = []
+            for i in range(start, stop):
+                nodes = m.fnodes[m.fptr[i]:m.fptr[i + 1]]
+                head = f"{len(nodes):x} " if mixed else ""
+                lines.append(head + " ".join(f"{v:x}" for v in nodes)
+                             + f" {int(m.c0[i]):x} {int(m.c1[i]):x}")
+            f.write("\n".join(lines))
+            f.write("\n))\n")
+            start = stop
+
+
+def read_cas(path: str) -> Mesh:
+    """Read the subset written by ``write_cas`` (and by the reference's own
+    shipped meshes): returns faces in file order with 1-based ids."""
+    txt = open(path).read()
+    pos, n = 0, len(txt)
+    x = None
+    faces = {}            # first index -> (nodes list, c0, c1, zone)
+    ncells, ctype, czone = 0, 4, 2
+
+    def group(p):
+        # returns (tokens inside the parenthesised group starting at p, end)
+        assert txt[p] == "("
+        q = txt.index(")", p)
+        return txt[p + 1:q].split(), q + 1
+
+    while True:
+        p = txt.find("(", pos)
+        if p < 0:
+            break
+        q = p + 1
+        while q < n and txt[q] in " \t\r\n":
+            q += 1
+        e = q
+        while e < n and txt[e] not in " \t\r\n(":
+            e += 1
+        sec = txt[q:e]
+        if sec == "0":
+            pos = txt.index(")", txt.index('"', e) + 1) + 1 if '"' in txt[e:e + 4] else e
+            continue
+        while e < n and txt[e] in " \t\r\n":
+            e += 1
+        if sec in ("10", "12", "13") and txt[e] == "(":
+            hdr, e2 = group(e)
+            h = [int(t, 16) for t in hdr[:5]]
+            zid, first, last = h[0], h[1], h[2]
+            rest = txt[e2:].lstrip()
+            has_body = zid != 0 and rest.startswith("(")
+            if sec == "12":
+                if zid != 0:
+                    ncells = max(ncells, last)
+                    czone = zid
+                    if len(h) > 4:
+                        ctype = h[4]
+                else:
+                    ncells = max(ncells, last)
+            if not has_body:
+                pos = txt.index(")", e2) + 1
+                continue
+            b0 = txt.index("(", e2)
+            b1 = txt.index(")", b0)
+            body = txt[b0 + 1:b1].split()
+            pos = txt.index(")", b1 + 1) + 1
+            if sec == "10":
+                nd = h[4] if len(h) > 4 else 3
+                arr = np.array(body, dtype=np.float64).reshape(-1, nd)
+                if x is None:
+                    x = np.zeros((last, 3))
+                x[first - 1:last, :nd] = arr
+            elif sec == "13":
+                ftype = h[4]
+                vals = [int(t, 16) for t in body]
+                k, recs = 0, []
+                for _ in range(last - first + 1):
+                    if ftype == 0:
+                        nnf = vals[k]
+                        k += 1
+                    else:
+                        nnf = ftype
+                    nodes = vals[k:k + nnf]
+                    k += nnf
+                    recs.append((nodes, vals[k], vals[k + 1], zid))
+                    k += 2
+                faces[first] = recs
+        else:
+            pos = e
+    order = sorted(faces)
+    recs = [r for k in order for r in faces[k]]
+    nfn = np.array([len(r[0]) for r in recs], np.int64)
+    fptr = np.zeros(len(recs) + 1, np.int64)
+    fptr[1:] = np.cumsum(nfn)
+    return Mesh(x=x, fptr=fptr,
+                fnodes=np.array([v for r in recs for v in r[0]], np.int32),
+                c0=np.array([r[1] for r in recs], np.int32),
+                c1=np.array([r[2] for r in recs], np.int32),
+                zone=np.array([r[3] for r in recs], np.int32),
+                ncells=ncells, ctype=ctype, cell_zone=czone)

@@ -1,0 +1,55 @@
+#!/usr/bin/env bash
+# Build the reference's own serial FP64 Fortran path (test infrastructure).
+#
+# Compiles the reference modules where they lie under /root/reference with
+# amdflang -O2 (no -march: baseline x86-64, so no FMA contraction), links them
+# with oracle/ref_harness.f90 and writes ONE executable: oracle/_ref/ref_harness.
+# Intermediate objects/modules live in a temporary directory outside the
+# repository and are deleted afterwards.
+#
+# One build-time fix is applied to a temporary copy of
+# src/common/mod_vtu_writer.f90 (an output-only module pulled in through
+# mod_uns_output): flang rejects the second PUBLIC on `vtu_data` (line 78)
+# that gfortran tolerates; the token `, public` is dropped there.  Nothing
+# else is changed, stubbed or substituted.
+set -euo pipefail
+REF=${MIXSOLVER_REFERENCE:-/root/reference}/src
+HERE="$(cd "$(dirname "$0")" && pwd)"
+OUT="$HERE/_ref"
+FC=${FC:-amdflang}
+if [ ! -d "$REF/unstructured" ]; then
+  echo "reference sources not present; skipping reference build" >&2
+  exit 0
+fi
+mkdir -p "$OUT"
+TMP="$(mktemp -d /tmp/mixsolver_refbuild.XXXXXX)"
+trap 'rm -rf "$TMP"' EXIT
+sed 's/type, public :: vtu_data/type :: vtu_data/' \
+    "$REF/common/mod_vtu_writer.f90" > "$TMP/mod_vtu_writer.f90"
+SRCS=(
+  "$REF/common/mod_precision.f90"
+  "$REF/common/mod_constants.f90"
+  "$TMP/mod_vtu_writer.f90"
+  "$REF/common/mod_cell_types.f90"
+  "$REF/common/mod_interface.f90"
+  "$REF/unstructured/mod_uns_mesh.f90"
+  "$REF/unstructured/mod_uns_cas_reader.f90"
+  "$REF/unstructured/mod_uns_connectivity.f90"
+  "$REF/unstructured/mod_uns_control.f90"
+  "$REF/unstructured/mod_uns_geometry.f90"
+  "$REF/unstructured/mod_uns_bc.f90"
+  "$REF/unstructured/mod_uns_fields.f90"
+  "$REF/unstructured/mod_uns_linsolver.f90"
+  "$REF/unstructured/mod_uns_output.f90"
+  "$REF/unstructured/mod_uns_simple.f90"
+)
+OBJS=()
+cd "$TMP"
+for s in "${SRCS[@]}"; do
+  o="$TMP/$(basename "${s%.f90}").o"
+  "$FC" -cpp -O2 -c "$s" -o "$o"
+  OBJS+=("$o")
+done
+"$FC" -cpp -O2 -c "$HERE/ref_harness.f90" -o "$TMP/ref_harness.o"
+"$FC" -O2 -o "$OUT/ref_harness" "$TMP/ref_harness.o" "${OBJS[@]}"
+echo "built $OUT/ref_harness"

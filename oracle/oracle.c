@@ -1,0 +1,1410 @@
+/*
+ * oracle.c -- CPU FP64 restatement of the reference unstructured
+ * SIMPLE/PISO path.  TEST INFRASTRUCTURE ONLY (see oracle.h).
+ *
+ * Conventions: arrays are 0-based in C, face owner/neighbour ids stay
+ * 1-based as in mesh_t (c1 == 0 marks a boundary face).  Every expression is
+ * written in the reference's left-to-right evaluation order; the file must be
+ * compiled with -ffp-contract=off so no FMA changes a rounding.
+ *
+ * Reference files restated (paths relative to the reference root):
+ *   geometry      src/unstructured/mod_uns_geometry.f90:40-174
+ *   connectivity  src/unstructured/mod_uns_connectivity.f90:28-108
+ *   CSR pattern   src/unstructured/mod_uns_simple.f90:835-923
+ *   lf weights    src/unstructured/mod_uns_fields.f90:147-157
+ *   BC values     src/unstructured/mod_uns_bc.f90:155-265, 400-703, 879-893
+ *   gradients     src/unstructured/mod_uns_fields.f90:164-258, 355-374
+ *   momentum      src/unstructured/mod_uns_simple.f90:928-1338 (+ limiter :1346-1408)
+ *   Rhie-Chow     src/unstructured/mod_uns_simple.f90:1413-1450
+ *   outflow       src/unstructured/mod_uns_simple.f90:1463-1547
+ *   PPE           src/unstructured/mod_uns_simple.f90:1552-1644
+ *   correction    src/unstructured/mod_uns_simple.f90:1649-1739
+ *   energy        src/unstructured/mod_uns_simple.f90:1748-2037 (+ :2302-2361)
+ *   solvers       src/unstructured/mod_uns_linsolver.f90, mod_uns_linsolver_mpi.f90
+ *   drivers       src/unstructured/mod_uns_simple.f90:88-288 (SIMPLE),
+ *                 src/unstructured/mod_uns_simple_mpi.f90:298-571 (PISO, np = 1)
+ * Not restated (the constructor rejects them with ier = 50): coupling
+ * interface faces, fluid/porous interface faces (kink pressure, Beavers-Joseph
+ * slip), solid cell zones, thermal dispersion and the LTNE solid equation.
+ */
+#include "oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define TINY DBL_MIN
+
+struct or_case {
+  int nn, nf, nc, nbc, nnz;
+  const double *x;               /* borrowed during create only */
+  int *c0, *c1, *zone, *fgrp;
+  double *xf, *sf, *area, *vol, *xc, *lf;
+  int *cf_ptr, *cf, *c2c_ptr, *c2c;
+  int *row_ptr, *col_idx;        /* 1-based CSR (csr_t) */
+  or_ctrl_t ctrl;
+  or_bcgroup_t *gb;
+  double inlet_ramp, pval_ramp;  /* g_inlet_ramp_factor / g_pval_ramp_factor */
+  /* fields_t */
+  double *u, *p, *gp, *gu, *flux, *apc, *u_old, *u_old_old;
+  double *T, *T_old, *T_old_old, *gt, *T_s, *T_s_old, *T_s_old_old, *gts;
+  double *porosity, *perm_dir, *inertial, *k_s, *rho_s, *cp_s;
+  int ts_order;
+  double *hist;
+  int nhist, caphist;
+};
+
+/* ---------------------------------------------------------------- helpers */
+
+/* Fortran NORM2 as evaluated by the LLVM flang runtime (Norm2Accumulator):
+ * running maximum m and s = sum (x/m)^2 over the other entries, rescaled by
+ * (m_old/m_new)^2 when the maximum grows; result m*sqrt(1+s). */
+double or_norm2(const double *v, int n) {
+  double mx = 0.0, s = 0.0;
+  for (int i = 0; i < n; i++) {
+    double ax = fabs(v[i]);
+    if (mx == 0.0) {
+      mx = ax;
+    } else if (ax > mx) {
+      double t = mx / ax;
+      double tsq = t * t;
+      s = s * tsq;
+      s = s + tsq;
+      mx = ax;
+    } else {
+      double t = ax / mx;
+      s = s + t * t;
+    }
+  }
+  return mx * sqrt(1.0 + s);
+}
+
+static double norm3(const double a[3]) { return or_norm2(a, 3); }
+
+static double dot3(const double *a, const double *b) {
+  double s = 0.0;
+  s = s + a[0] * b[0];
+  s = s + a[1] * b[1];
+  s = s + a[2] * b[2];
+  return s;
+}
+
+static double dotn(const double *a, const double *b, int n) {
+  double s = 0.0;
+  for (int i = 0; i < n; i++) s = s + a[i] * b[i];
+  return s;
+}
+
+static double fmax0(double a, double b) { return a > b ? a : b; }
+static double fmin0(double a, double b) { return a < b ? a : b; }
+
+static void cross(const double *a, const double *b, double *c) {
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+static int pos_of(const or_case *cs, int i, int j) {  /* 1-based i,j -> 1-based k or 0 */
+  for (int k = cs->row_ptr[i - 1]; k <= cs->row_ptr[i] - 1; k++) {
+    int cj = cs->col_idx[k - 1];
+    if (cj == j) return k;
+    if (cj > j) return 0;
+  }
+  return 0;
+}
+
+#define U(c, d) cs->u[3 * ((c) - 1) + (d)]
+#define XC(c, d) cs->xc[3 * ((c) - 1) + (d)]
+#define XF(f, d) cs->xf[3 * (f) + (d)]
+#define SF(f, d) cs->sf[3 * (f) + (d)]
+#define GU(comp, dir, c) cs->gu[(comp) + 3 * (dir) + 9 * ((c) - 1)]
+
+/* ----------------------------------------------------------- BC helpers */
+
+/* bc_face_vel, mod_uns_bc.f90:400-509 (face f 0-based) */
+static void bc_face_vel(const or_case *cs, int f, const double *uP, double *uf) {
+  const or_bcgroup_t *g = &cs->gb[cs->fgrp[f] - 1];
+  const double *xf = &cs->xf[3 * f], *sf = &cs->sf[3 * f];
+  double nvec[3], un, xi, uspc, nrm;
+  switch (g->btype) {
+    case OR_BC_WALL:
+      if (g->has_lid && fabs(xf[g->lid_dir - 1] - g->lid_coord) <= g->lid_tol) {
+        uf[0] = g->lid_vel[0]; uf[1] = g->lid_vel[1]; uf[2] = g->lid_vel[2];
+      } else {
+        uf[0] = uf[1] = uf[2] = 0.0;
+      }
+      break;
+    case OR_BC_SYMMETRY:
+      nrm = norm3(sf);
+      for (int d = 0; d < 3; d++) nvec[d] = sf[d] / nrm;
+      un = dot3(uP, nvec);
+      for (int d = 0; d < 3; d++) uf[d] = uP[d] - 2.0 * un * nvec[d];
+      break;
+    case OR_BC_VINLET:
+      uf[0] = g->uvel[0]; uf[1] = g->uvel[1]; uf[2] = g->uvel[2];
+      break;
+    case OR_BC_VINLET_PARAB:
+      nrm = norm3(sf);
+      for (int d = 0; d < 3; d++) nvec[d] = sf[d] / nrm;
+      xi = (xf[g->prfl_axis - 1] - g->prfl_origin) / g->prfl_span;
+      xi = fmax0(0.0, fmin0(1.0, xi));
+      uspc = 6.0 * g->prfl_mean * xi * (1.0 - xi);
+      for (int d = 0; d < 3; d++) uf[d] = -uspc * nvec[d];
+      break;
+    case OR_BC_MASSINLET:
+      nrm = norm3(sf);
+      for (int d = 0; d < 3; d++) nvec[d] = sf[d] / nrm;
+      for (int d = 0; d < 3; d++) uf[d] = -g->uspeed * cs->inlet_ramp * nvec[d];
+      break;
+    case OR_BC_FARFIELD:
+      nrm = norm3(sf);
+      for (int d = 0; d < 3; d++) nvec[d] = sf[d] / nrm;
+      un = dot3(g->uvel, nvec);
+      if (un < 0.0) { uf[0] = g->uvel[0]; uf[1] = g->uvel[1]; uf[2] = g->uvel[2]; }
+      else { uf[0] = uP[0]; uf[1] = uP[1]; uf[2] = uP[2]; }
+      break;
+    case OR_BC_SLIPWALL:
+      nrm = norm3(sf);
+      for (int d = 0; d < 3; d++) nvec[d] = sf[d] / nrm;
+      un = dot3(uP, nvec);
+      for (int d = 0; d < 3; d++) uf[d] = uP[d] - un * nvec[d];
+      break;
+    default: /* POUTLET, OUTFLOW */
+      uf[0] = uP[0]; uf[1] = uP[1]; uf[2] = uP[2];
+  }
+}
+
+/* bc_face_p, mod_uns_bc.f90:514-549 */
+static double bc_face_p(const or_case *cs, int f, double pP) {
+  const or_bcgroup_t *g = &cs->gb[cs->fgrp[f] - 1];
+  if (g->btype == OR_BC_POUTLET || g->btype == OR_BC_FARFIELD) return g->pval * cs->pval_ramp;
+  return pP;
+}
+
+static int bc_p_dirichlet(const or_case *cs, int f) {  /* :879-893 */
+  int gi = cs->fgrp[f];
+  if (gi <= 0) return 0;
+  int bt = cs->gb[gi - 1].btype;
+  return bt == OR_BC_POUTLET || bt == OR_BC_FARFIELD;
+}
+
+/* bc_face_T, mod_uns_bc.f90:560-703 */
+static void bc_face_T(const or_case *cs, int f, double Tc, double *Tf, double *qf, int *neum) {
+  const or_bcgroup_t *g = &cs->gb[cs->fgrp[f] - 1];
+  const double *xf = &cs->xf[3 * f];
+  int tt;
+  double tv, qv;
+  *Tf = Tc; *qf = 0.0; *neum = 1;
+  switch (g->btype) {
+    case OR_BC_WALL:
+    case OR_BC_POUTLET:
+    case OR_BC_OUTFLOW:
+      tt = g->ttype; tv = g->tval; qv = g->qval;
+      for (int k = 0; k < g->ntbc_plane; k++) {
+        if (fabs(xf[g->tbc_pdir[k] - 1] - g->tbc_pcoord[k]) <= g->tbc_tol) {
+          tt = g->tbc_pttype[k]; tv = g->tbc_ptval[k]; qv = g->tbc_pqval[k];
+          break;
+        }
+      }
+      if (tt == 1) { *neum = 0; *Tf = tv; *qf = 0.0; }
+      else if (tt == 2) { *neum = 1; *qf = qv; *Tf = Tc; }
+      else { *neum = 1; *qf = 0.0; *Tf = Tc; }
+      break;
+    case OR_BC_VINLET:
+    case OR_BC_VINLET_PARAB:
+    case OR_BC_MASSINLET:
+      *neum = 0; *Tf = g->tval; *qf = 0.0;
+      break;
+    default: /* SYMMETRY, FARFIELD, SLIPWALL */
+      *neum = 1; *qf = 0.0; *Tf = Tc;
+  }
+}
+
+/* ---------------------------------------------------------- construction */
+
+static void *xcalloc(size_t n, size_t s) {
+  void *p = calloc(n ? n : 1, s);
+  if (!p) abort();
+  return p;
+}
+
+static void build_geometry(or_case *cs, const long long *fptr, const int *fnodes) {
+  int nf = cs->nf, nc = cs->nc;
+  const double *x = cs->x;
+  /* face_props, mod_uns_geometry.f90:97-131 */
+  for (int i = 0; i < nf; i++) {
+    int n = (int)(fptr[i + 1] - fptr[i]);
+    const int *nd = &fnodes[fptr[i]];
+    double s[3] = {0, 0, 0}, cen[3] = {0, 0, 0}, at = 0.0;
+    for (int j = 0; j < n; j++) {
+      int jn = (j + 1) % n;
+      double c[3];
+      cross(&x[3 * (nd[j] - 1)], &x[3 * (nd[jn] - 1)], c);
+      for (int d = 0; d < 3; d++) s[d] = s[d] + 0.5 * c[d];
+    }
+    double a = norm3(s);
+    const double *p1 = &x[3 * (nd[0] - 1)];
+    for (int j = 1; j < n - 1; j++) {
+      const double *p2 = &x[3 * (nd[j] - 1)], *p3 = &x[3 * (nd[j + 1] - 1)];
+      double e1[3], e2[3], st[3];
+      for (int d = 0; d < 3; d++) { e1[d] = p2[d] - p1[d]; e2[d] = p3[d] - p1[d]; }
+      cross(e1, e2, st);
+      for (int d = 0; d < 3; d++) st[d] = st[d] * 0.5;
+      double ns = norm3(st);
+      for (int d = 0; d < 3; d++) cen[d] = cen[d] + ns * (p1[d] + p2[d] + p3[d]) / 3.0;
+      at = at + ns;
+    }
+    if (at > 0.0)
+      for (int d = 0; d < 3; d++) cen[d] = cen[d] / at;
+    for (int d = 0; d < 3; d++) { XF(i, d) = cen[d]; SF(i, d) = s[d]; }
+    cs->area[i] = a;
+  }
+  /* prov_centers, :134-158 */
+  double *pcen = xcalloc(3 * (size_t)nc, sizeof(double));
+  int *cnt = xcalloc(nc, sizeof(int));
+  for (int i = 0; i < nf; i++) {
+    int c0 = cs->c0[i], c1 = cs->c1[i];
+    for (long long j = fptr[i]; j < fptr[i + 1]; j++) {
+      const double *xn = &x[3 * (fnodes[j] - 1)];
+      for (int d = 0; d < 3; d++) pcen[3 * (c0 - 1) + d] = pcen[3 * (c0 - 1) + d] + xn[d];
+      cnt[c0 - 1]++;
+      if (c1 > 0) {
+        for (int d = 0; d < 3; d++) pcen[3 * (c1 - 1) + d] = pcen[3 * (c1 - 1) + d] + xn[d];
+        cnt[c1 - 1]++;
+      }
+    }
+  }
+  for (int c = 0; c < nc; c++)
+    if (cnt[c] > 0)
+      for (int d = 0; d < 3; d++) pcen[3 * c + d] = pcen[3 * c + d] / (double)cnt[c];
+  /* orientation, :67-70 */
+  for (int i = 0; i < nf; i++) {
+    double r[3];
+    for (int d = 0; d < 3; d++) r[d] = XF(i, d) - pcen[3 * (cs->c0[i] - 1) + d];
+    double dotv = dot3(&cs->sf[3 * i], r);
+    if (dotv < 0.0)
+      for (int d = 0; d < 3; d++) SF(i, d) = -SF(i, d);
+  }
+  /* accumulate_cell, :161-174 */
+  const double THIRD = 1.0 / 3.0, QRTR = 1.0 / 4.0;
+  for (int i = 0; i < nf; i++) {
+    for (int side = 0; side < 2; side++) {
+      int cell = side == 0 ? cs->c0[i] : cs->c1[i];
+      if (cell <= 0) continue;
+      const double *p = &pcen[3 * (cell - 1)];
+      double r[3];
+      for (int d = 0; d < 3; d++) r[d] = XF(i, d) - p[d];
+      double dv = THIRD * dot3(&cs->sf[3 * i], r) * (side == 0 ? 1.0 : -1.0);
+      cs->vol[cell - 1] = cs->vol[cell - 1] + dv;
+      for (int d = 0; d < 3; d++)
+        XC(cell, d) = XC(cell, d) + dv * ((1.0 - QRTR) * XF(i, d) + QRTR * p[d]);
+    }
+  }
+  for (int c = 1; c <= nc; c++)
+    if (cs->vol[c - 1] > 0.0)
+      for (int d = 0; d < 3; d++) XC(c, d) = XC(c, d) / cs->vol[c - 1];
+  free(pcen);
+  free(cnt);
+}
+
+static void build_connectivity(or_case *cs) {
+  int nf = cs->nf, nc = cs->nc;
+  cs->cf_ptr = xcalloc(nc + 1, sizeof(int));
+  for (int i = 0; i < nf; i++) {
+    cs->cf_ptr[cs->c0[i]]++;
+    if (cs->c1[i] > 0) cs->cf_ptr[cs->c1[i]]++;
+  }
+  for (int k = 0; k < nc; k++) cs->cf_ptr[k + 1] += cs->cf_ptr[k];
+  cs->cf = xcalloc(cs->cf_ptr[nc], sizeof(int));
+  int *fill = xcalloc(nc, sizeof(int));
+  for (int k = 0; k < nc; k++) fill[k] = cs->cf_ptr[k];
+  for (int i = 0; i < nf; i++) {
+    cs->cf[fill[cs->c0[i] - 1]++] = i + 1;
+    if (cs->c1[i] > 0) cs->cf[fill[cs->c1[i] - 1]++] = i + 1;
+  }
+  cs->c2c_ptr = xcalloc(nc + 1, sizeof(int));
+  for (int i = 0; i < nf; i++)
+    if (cs->c1[i] > 0) { cs->c2c_ptr[cs->c0[i]]++; cs->c2c_ptr[cs->c1[i]]++; }
+  for (int k = 0; k < nc; k++) cs->c2c_ptr[k + 1] += cs->c2c_ptr[k];
+  cs->c2c = xcalloc(cs->c2c_ptr[nc], sizeof(int));
+  for (int k = 0; k < nc; k++) fill[k] = cs->c2c_ptr[k];
+  for (int i = 0; i < nf; i++)
+    if (cs->c1[i] > 0) {
+      cs->c2c[fill[cs->c0[i] - 1]++] = cs->c1[i];
+      cs->c2c[fill[cs->c1[i] - 1]++] = cs->c0[i];
+    }
+  free(fill);
+}
+
+/* build_cell_csr + sort_int, mod_uns_simple.f90:835-904 */
+static void build_csr(or_case *cs) {
+  int nc = cs->nc, maxn = 0;
+  for (int k = 0; k < nc; k++) {
+    int n = cs->c2c_ptr[k + 1] - cs->c2c_ptr[k];
+    if (n > maxn) maxn = n;
+  }
+  int *tmp = xcalloc(maxn + 1, sizeof(int)), *row = xcalloc(maxn + 1, sizeof(int));
+  cs->row_ptr = xcalloc(nc + 1, sizeof(int));
+  cs->row_ptr[0] = 1;
+  for (int pass = 1; pass <= 2; pass++) {
+    int cnt = 0;
+    for (int k = 1; k <= nc; k++) {
+      int n = cs->c2c_ptr[k] - cs->c2c_ptr[k - 1];
+      for (int j = 0; j < n; j++) tmp[j] = cs->c2c[cs->c2c_ptr[k - 1] + j];
+      tmp[n] = k;
+      for (int i = 1; i < n + 1; i++) {  /* insertion sort */
+        int key = tmp[i], j = i - 1;
+        while (j >= 0 && tmp[j] > key) { tmp[j + 1] = tmp[j]; j--; }
+        tmp[j + 1] = key;
+      }
+      int nu = 0;
+      for (int j = 0; j < n + 1; j++)
+        if (nu == 0 || tmp[j] != row[nu - 1]) row[nu++] = tmp[j];
+      if (pass == 1) cs->row_ptr[k] = cs->row_ptr[k - 1] + nu;
+      else for (int j = 0; j < nu; j++) cs->col_idx[cnt++] = row[j];
+    }
+    if (pass == 1) {
+      cs->nnz = cs->row_ptr[nc] - 1;
+      cs->col_idx = xcalloc(cs->nnz, sizeof(int));
+    }
+  }
+  free(tmp);
+  free(row);
+}
+
+or_case *or_create(int nn, const double *x, int nf, const long long *fptr,
+                   const int *fnodes, const int *c0, const int *c1,
+                   const int *zone, int nc, const or_ctrl_t *ctrl,
+                   const or_bcgroup_t *groups, int *ier) {
+  *ier = 0;
+  or_case *cs = xcalloc(1, sizeof(or_case));
+  cs->nn = nn; cs->nf = nf; cs->nc = nc; cs->x = x;
+  cs->ctrl = *ctrl;
+  cs->nbc = ctrl->nbc;
+  cs->inlet_ramp = 1.0; cs->pval_ramp = 1.0;
+  cs->c0 = xcalloc(nf, sizeof(int)); cs->c1 = xcalloc(nf, sizeof(int));
+  cs->zone = xcalloc(nf, sizeof(int)); cs->fgrp = xcalloc(nf, sizeof(int));
+  memcpy(cs->c0, c0, nf * sizeof(int));
+  memcpy(cs->c1, c1, nf * sizeof(int));
+  memcpy(cs->zone, zone, nf * sizeof(int));
+  cs->xf = xcalloc(3 * (size_t)nf, sizeof(double));
+  cs->sf = xcalloc(3 * (size_t)nf, sizeof(double));
+  cs->area = xcalloc(nf, sizeof(double));
+  cs->vol = xcalloc(nc, sizeof(double));
+  cs->xc = xcalloc(3 * (size_t)nc, sizeof(double));
+  cs->lf = xcalloc(nf, sizeof(double));
+  build_connectivity(cs);
+  build_geometry(cs, fptr, fnodes);
+  build_csr(cs);
+  for (int c = 0; c < nc; c++)
+    if (cs->vol[c] <= 0.0) *ier = 30;
+  /* build_bc, mod_uns_bc.f90:155-265 */
+  cs->gb = xcalloc(cs->nbc, sizeof(or_bcgroup_t));
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int k = 0; k < nn; k++)
+    for (int d = 0; d < 3; d++) {
+      if (x[3 * k + d] < lo[d]) lo[d] = x[3 * k + d];
+      if (x[3 * k + d] > hi[d]) hi[d] = x[3 * k + d];
+    }
+  double cr = hi[0] - lo[0];
+  if (hi[1] - lo[1] > cr) cr = hi[1] - lo[1];
+  if (hi[2] - lo[2] > cr) cr = hi[2] - lo[2];
+  for (int i = 0; i < cs->nbc; i++) {
+    cs->gb[i] = groups[i];
+    or_bcgroup_t *g = &cs->gb[i];
+    if (g->btype == OR_BC_MASSINLET) g->uspeed = g->mdot / ctrl->rho;
+    if (g->has_lid) g->lid_tol = 1.0e-4 * cr;
+    if (g->ntbc_plane > 0) g->tbc_tol = 1.0e-4 * cr;
+    if (g->btype == OR_BC_INTERFACE) *ier = 50;
+    int nb = 0;
+    for (int f = 0; f < nf; f++)
+      if (cs->c1[f] == 0 && cs->zone[f] == g->zone) { nb++; cs->fgrp[f] = i + 1; }
+    if (nb == 0) *ier = 11;
+  }
+  for (int f = 0; f < nf; f++)
+    if (cs->c1[f] == 0 && cs->fgrp[f] == 0) *ier = 12;
+  if (ctrl->thermal_model != 0) *ier = 50;
+  /* init_fields, mod_uns_fields.f90:86-159 */
+  size_t n3 = 3 * (size_t)nc;
+  cs->u = xcalloc(n3, 8); cs->p = xcalloc(nc, 8); cs->gp = xcalloc(n3, 8);
+  cs->gu = xcalloc(9 * (size_t)nc, 8); cs->flux = xcalloc(nf, 8);
+  cs->apc = xcalloc(nc, 8); cs->u_old = xcalloc(n3, 8); cs->u_old_old = xcalloc(n3, 8);
+  cs->T = xcalloc(nc, 8); cs->T_old = xcalloc(nc, 8); cs->T_old_old = xcalloc(nc, 8);
+  cs->gt = xcalloc(n3, 8); cs->T_s = xcalloc(nc, 8); cs->T_s_old = xcalloc(nc, 8);
+  cs->T_s_old_old = xcalloc(nc, 8); cs->gts = xcalloc(n3, 8);
+  cs->porosity = xcalloc(nc, 8); cs->perm_dir = xcalloc(n3, 8); cs->inertial = xcalloc(nc, 8);
+  cs->k_s = xcalloc(nc, 8); cs->rho_s = xcalloc(nc, 8); cs->cp_s = xcalloc(nc, 8);
+  for (int c = 0; c < nc; c++) { cs->apc[c] = 1.0; cs->porosity[c] = 1.0; }
+  for (int i = 0; i < nf; i++) {
+    if (cs->c1[i] == 0) { cs->lf[i] = 1.0; continue; }
+    double r0[3], r1[3];
+    for (int d = 0; d < 3; d++) {
+      r0[d] = XF(i, d) - XC(cs->c0[i], d);
+      r1[d] = XC(cs->c1[i], d) - XF(i, d);
+    }
+    double d0 = norm3(r0), d1 = norm3(r1);
+    cs->lf[i] = d1 / (d0 + d1);
+  }
+  cs->x = NULL;
+  if (*ier != 0 && *ier != 50) { or_destroy(cs); return NULL; }
+  if (*ier == 50) { or_destroy(cs); return NULL; }
+  return cs;
+}
+
+void or_destroy(or_case *cs) {
+  if (!cs) return;
+  void *ptrs[] = {cs->c0, cs->c1, cs->zone, cs->fgrp, cs->xf, cs->sf, cs->area, cs->vol,
+                  cs->xc, cs->lf, cs->cf_ptr, cs->cf, cs->c2c_ptr, cs->c2c, cs->row_ptr,
+                  cs->col_idx, cs->gb, cs->u, cs->p, cs->gp, cs->gu, cs->flux, cs->apc,
+                  cs->u_old, cs->u_old_old, cs->T, cs->T_old, cs->T_old_old, cs->gt, cs->T_s,
+                  cs->T_s_old, cs->T_s_old_old, cs->gts, cs->porosity, cs->perm_dir,
+                  cs->inertial, cs->k_s, cs->rho_s, cs->cp_s, cs->hist};
+  for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
+  free(cs);
+}
+
+const double *or_xf(or_case *c) { return c->xf; }
+const double *or_sf(or_case *c) { return c->sf; }
+const double *or_area(or_case *c) { return c->area; }
+const double *or_vol(or_case *c) { return c->vol; }
+const double *or_xc(or_case *c) { return c->xc; }
+const double *or_lf(or_case *c) { return c->lf; }
+const int *or_cf_ptr(or_case *c) { return c->cf_ptr; }
+const int *or_cf(or_case *c) { return c->cf; }
+const int *or_row_ptr(or_case *c) { return c->row_ptr; }
+const int *or_col_idx(or_case *c) { return c->col_idx; }
+int or_nnz(or_case *c) { return c->nnz; }
+const int *or_fgrp(or_case *c) { return c->fgrp; }
+void or_set_ts_order(or_case *c, int t) { c->ts_order = t; }
+int or_get_ts_order(or_case *c) { return c->ts_order; }
+
+double *or_field(or_case *cs, const char *nm) {
+  struct { const char *n; double *p; } t[] = {
+    {"u", cs->u}, {"p", cs->p}, {"gp", cs->gp}, {"gu", cs->gu}, {"flux", cs->flux},
+    {"apc", cs->apc}, {"u_old", cs->u_old}, {"u_old_old", cs->u_old_old}, {"T", cs->T},
+    {"T_old", cs->T_old}, {"T_old_old", cs->T_old_old}, {"gt", cs->gt}, {"T_s", cs->T_s},
+    {"T_s_old", cs->T_s_old}, {"T_s_old_old", cs->T_s_old_old}, {"gts", cs->gts},
+    {"porosity", cs->porosity}, {"perm_dir", cs->perm_dir}, {"inertial", cs->inertial},
+    {"k_s", cs->k_s}, {"rho_s", cs->rho_s}, {"cp_s", cs->cp_s}};
+  for (size_t i = 0; i < sizeof(t) / sizeof(t[0]); i++)
+    if (strcmp(t[i].n, nm) == 0) return t[i].p;
+  return NULL;
+}
+
+/* ------------------------------------------------------------ gradients */
+
+/* grad_scalar, mod_uns_fields.f90:355-374 (grad stride gs, dir stride ds) */
+static void grad_scalar(const or_case *cs, const double *phif, double *grad, int gs, int ds) {
+  for (int c = 0; c < cs->nc; c++)
+    for (int d = 0; d < 3; d++) grad[gs * c + ds * d] = 0.0;
+  for (int i = 0; i < cs->nf; i++) {
+    int c0 = cs->c0[i] - 1, c1 = cs->c1[i] - 1;
+    for (int d = 0; d < 3; d++) grad[gs * c0 + ds * d] = grad[gs * c0 + ds * d] + phif[i] * SF(i, d);
+    if (c1 >= 0)
+      for (int d = 0; d < 3; d++) grad[gs * c1 + ds * d] = grad[gs * c1 + ds * d] - phif[i] * SF(i, d);
+  }
+  for (int c = 0; c < cs->nc; c++)
+    for (int d = 0; d < 3; d++) grad[gs * c + ds * d] = grad[gs * c + ds * d] / cs->vol[c];
+}
+
+/* compute_gradients, mod_uns_fields.f90:164-258 (no porous interface faces) */
+void or_compute_gradients(or_case *cs) {
+  int nf = cs->nf;
+  double *phif = xcalloc(nf, sizeof(double));
+  for (int i = 0; i < nf; i++) {
+    int c0 = cs->c0[i], c1 = cs->c1[i];
+    if (c1 > 0) phif[i] = cs->lf[i] * cs->p[c0 - 1] + (1.0 - cs->lf[i]) * cs->p[c1 - 1];
+    else phif[i] = bc_face_p(cs, i, cs->p[c0 - 1]);
+  }
+  grad_scalar(cs, phif, cs->gp, 3, 1);
+  for (int comp = 0; comp < 3; comp++) {
+    for (int i = 0; i < nf; i++) {
+      int c0 = cs->c0[i], c1 = cs->c1[i];
+      if (c1 > 0) {
+        phif[i] = cs->lf[i] * U(c0, comp) + (1.0 - cs->lf[i]) * U(c1, comp);
+      } else {
+        double uf[3];
+        bc_face_vel(cs, i, &U(c0, 0), uf);
+        phif[i] = uf[comp];
+      }
+    }
+    grad_scalar(cs, phif, cs->gu + comp, 9, 3);
+  }
+  for (int pass = 0; pass < 2; pass++) {
+    const double *T = pass == 0 ? cs->T : cs->T_s;
+    for (int i = 0; i < nf; i++) {
+      int c0 = cs->c0[i], c1 = cs->c1[i];
+      if (c1 > 0) {
+        phif[i] = cs->lf[i] * T[c0 - 1] + (1.0 - cs->lf[i]) * T[c1 - 1];
+      } else {
+        double tf, qf; int ne;
+        bc_face_T(cs, i, T[c0 - 1], &tf, &qf, &ne);
+        phif[i] = tf;
+      }
+    }
+    grad_scalar(cs, phif, pass == 0 ? cs->gt : cs->gts, 3, 1);
+  }
+  free(phif);
+}
+
+/* --------------------------------------------------------- limiters */
+
+/* velocity_limiter (:1346-1408) when T == NULL, temperature_limiter
+ * (:2302-2361) otherwise */
+static void bj_limiter(const or_case *cs, int comp, const double *T, double *psi) {
+  int nc = cs->nc, nf = cs->nf;
+  double *pmin = xcalloc(nc, 8), *pmax = xcalloc(nc, 8);
+  for (int c = 1; c <= nc; c++) {
+    double v = T ? T[c - 1] : U(c, comp);
+    pmin[c - 1] = v; pmax[c - 1] = v;
+  }
+  for (int i = 0; i < nf; i++) {
+    int c0 = cs->c0[i], c1 = cs->c1[i];
+    if (c1 > 0) {
+      double v0 = T ? T[c0 - 1] : U(c0, comp), v1 = T ? T[c1 - 1] : U(c1, comp);
+      pmin[c0 - 1] = fmin0(pmin[c0 - 1], v1); pmax[c0 - 1] = fmax0(pmax[c0 - 1], v1);
+      pmin[c1 - 1] = fmin0(pmin[c1 - 1], v0); pmax[c1 - 1] = fmax0(pmax[c1 - 1], v0);
+    } else if (cs->fgrp[i] > 0) {
+      double vb;
+      if (T) { double qf; int ne; bc_face_T(cs, i, T[c0 - 1], &vb, &qf, &ne); }
+      else { double uf[3]; bc_face_vel(cs, i, &U(c0, 0), uf); vb = uf[comp]; }
+      pmin[c0 - 1] = fmin0(pmin[c0 - 1], vb); pmax[c0 - 1] = fmax0(pmax[c0 - 1], vb);
+    }
+  }
+  for (int c = 0; c < nc; c++) psi[c] = 1.0;
+  for (int i = 0; i < nf; i++) {
+    for (int side = 0; side < 2; side++) {
+      int c = side == 0 ? cs->c0[i] : cs->c1[i];
+      if (c <= 0) continue;
+      double uc = T ? T[c - 1] : U(c, comp);
+      double g[3], r[3];
+      for (int d = 0; d < 3; d++) {
+        g[d] = T ? cs->gt[3 * (c - 1) + d] : GU(comp, d, c);
+        r[d] = XF(i, d) - XC(c, d);
+      }
+      double fr = uc + dot3(g, r);
+      double df = fr - uc;
+      if (df > 0.0) psi[c - 1] = fmin0(psi[c - 1], (pmax[c - 1] - uc) / df);
+      else if (df < 0.0) psi[c - 1] = fmin0(psi[c - 1], (pmin[c - 1] - uc) / df);
+    }
+  }
+  for (int c = 0; c < nc; c++) psi[c] = fmax0(0.0, fmin0(1.0, psi[c]));
+  free(pmin);
+  free(pmax);
+}
+
+/* ------------------------------------------------------- momentum */
+
+void or_momentum_assembly(or_case *cs, int comp, double *val, double *rhs, double *ap) {
+  const or_ctrl_t *ct = &cs->ctrl;
+  int nc = cs->nc, nf = cs->nf;
+  const double POROSITY_FLOOR = 1.0e-8;
+  comp -= 1;  /* 0-based component */
+  for (int k = 0; k < cs->nnz; k++) val[k] = 0.0;
+  for (int c = 0; c < nc; c++) { rhs[c] = 0.0; ap[c] = 0.0; }
+  double sd = 0.0;
+  double *psi = NULL;
+  if (ct->conv_blend > 0.0) {
+    psi = xcalloc(nc, 8);
+    bj_limiter(cs, comp, NULL, psi);
+  }
+  /* interior faces :974-1120 */
+  for (int i = 0; i < nf; i++) {
+    int c1 = cs->c1[i];
+    if (c1 == 0) continue;
+    int c0 = cs->c0[i];
+    double F = cs->flux[i], lf = cs->lf[i];
+    double dvec[3];
+    for (int d = 0; d < 3; d++) dvec[d] = XC(c1, d) - XC(c0, d);
+    double dn = norm3(dvec);
+    double mu0 = ct->mu / fmax0(cs->porosity[c0 - 1], POROSITY_FLOOR);
+    double mu1 = ct->mu / fmax0(cs->porosity[c1 - 1], POROSITY_FLOOR);
+    double mu_f = lf * mu0 + (1.0 - lf) * mu1;
+    double D;
+    if (ct->nonorth_corr > 0.0) {
+      sd = dot3(&cs->sf[3 * i], dvec);
+      D = mu_f * (cs->area[i] * cs->area[i]) / sd;
+    } else {
+      D = mu_f * cs->area[i] / dn;
+    }
+    double D_face = D;
+    ap[c0 - 1] = ap[c0 - 1] + fmax0(F, 0.0) + D_face;
+    ap[c1 - 1] = ap[c1 - 1] + fmax0(-F, 0.0) + D_face;
+    int kd = pos_of(cs, c0, c1);
+    val[kd - 1] = val[kd - 1] - (D_face + fmax0(-F, 0.0));
+    kd = pos_of(cs, c1, c0);
+    val[kd - 1] = val[kd - 1] - (D_face + fmax0(F, 0.0));
+    if (ct->conv_blend > 0.0) {
+      double dc, r[3], g[3];
+      int cu = F > 0.0 ? c0 : c1;
+      for (int d = 0; d < 3; d++) { g[d] = GU(comp, d, cu); r[d] = XF(i, d) - XC(cu, d); }
+      dc = psi[cu - 1] * dot3(g, r);
+      dc = ct->conv_blend * F * dc;
+      rhs[c0 - 1] = rhs[c0 - 1] - dc;
+      rhs[c1 - 1] = rhs[c1 - 1] + dc;
+    }
+    if (ct->nonorth_corr > 0.0) {
+      double gf[3], t[3];
+      for (int d = 0; d < 3; d++) {
+        gf[d] = lf * GU(comp, d, c0) + (1.0 - lf) * GU(comp, d, c1);
+        t[d] = SF(i, d) - dvec[d] * ((cs->area[i] * cs->area[i]) / sd);
+      }
+      double noc = ct->nonorth_corr * mu_f * dot3(gf, t);
+      rhs[c0 - 1] = rhs[c0 - 1] + noc;
+      rhs[c1 - 1] = rhs[c1 - 1] - noc;
+    }
+    double pf = lf * cs->p[c0 - 1] + (1.0 - lf) * cs->p[c1 - 1];
+    double Sc = SF(i, comp);
+    rhs[c0 - 1] = rhs[c0 - 1] - pf * Sc;
+    rhs[c1 - 1] = rhs[c1 - 1] + pf * Sc;
+  }
+  /* boundary faces :1123-1227 */
+  for (int i = 0; i < nf; i++) {
+    int gi = cs->fgrp[i];
+    if (gi == 0) continue;
+    int c0 = cs->c0[i];
+    double uf[3], r[3];
+    bc_face_vel(cs, i, &U(c0, 0), uf);
+    double pf = bc_face_p(cs, i, cs->p[c0 - 1]);
+    for (int d = 0; d < 3; d++) r[d] = XF(i, d) - XC(c0, d);
+    double dw = norm3(r);
+    double D = (ct->mu / fmax0(cs->porosity[c0 - 1], POROSITY_FLOOR)) * cs->area[i] / dw;
+    double F = ct->rho * dot3(uf, &cs->sf[3 * i]);
+    switch (cs->gb[gi - 1].btype) {
+      case OR_BC_WALL: case OR_BC_SYMMETRY: case OR_BC_VINLET: case OR_BC_VINLET_PARAB:
+      case OR_BC_SLIPWALL: case OR_BC_MASSINLET:
+        ap[c0 - 1] = ap[c0 - 1] + D + fmax0(F, 0.0);
+        rhs[c0 - 1] = rhs[c0 - 1] + D * uf[comp] - F * uf[comp];
+        break;
+      case OR_BC_POUTLET: case OR_BC_OUTFLOW:
+        ap[c0 - 1] = ap[c0 - 1] + fmax0(F, 0.0);
+        rhs[c0 - 1] = rhs[c0 - 1] - fmin0(F, 0.0) * U(c0, comp);
+        break;
+      case OR_BC_FARFIELD:
+        if (F < 0.0) {
+          ap[c0 - 1] = ap[c0 - 1] + D + fmax0(F, 0.0);
+          rhs[c0 - 1] = rhs[c0 - 1] + D * uf[comp] - F * uf[comp];
+        } else {
+          ap[c0 - 1] = ap[c0 - 1] + F;
+        }
+        break;
+      default: break;
+    }
+    rhs[c0 - 1] = rhs[c0 - 1] - pf * SF(i, comp);
+  }
+  if (ct->boussinesq)
+    for (int c = 0; c < nc; c++)
+      rhs[c] = rhs[c] - ct->rho * ct->beta * (cs->T[c] - ct->tref) * ct->gravity[comp] * cs->vol[c];
+  if (ct->body_force[0] != 0.0 || ct->body_force[1] != 0.0 || ct->body_force[2] != 0.0)
+    for (int c = 0; c < nc; c++) rhs[c] = rhs[c] + ct->body_force[comp] * cs->vol[c];
+  for (int c = 1; c <= nc; c++) {  /* Darcy-Forchheimer :1268-1284 */
+    double K = cs->perm_dir[3 * (c - 1) + comp];
+    if (K > 0.0) ap[c - 1] = ap[c - 1] + ct->mu / K * cs->vol[c - 1];
+    if (cs->inertial[c - 1] > 0.0) {
+      double um = sqrt(U(c, 0) * U(c, 0) + U(c, 1) * U(c, 1) + U(c, 2) * U(c, 2));
+      rhs[c - 1] = rhs[c - 1] - ct->rho * cs->inertial[c - 1] * um * U(c, comp) * cs->vol[c - 1];
+    }
+  }
+  if (cs->ts_order == 1) {
+    for (int c = 0; c < nc; c++) {
+      double at = ct->rho * cs->vol[c] / ct->dt;
+      ap[c] = ap[c] + at;
+      rhs[c] = rhs[c] + at * cs->u_old[3 * c + comp];
+    }
+  } else if (cs->ts_order == 2) {
+    for (int c = 0; c < nc; c++) {
+      double at = ct->rho * cs->vol[c] / (2.0 * ct->dt);
+      ap[c] = ap[c] + 3.0 * at;
+      rhs[c] = rhs[c] + at * (4.0 * cs->u_old[3 * c + comp] - cs->u_old_old[3 * c + comp]);
+    }
+  }
+  for (int c = 1; c <= nc; c++) {
+    int kd = pos_of(cs, c, c);
+    if (cs->ts_order > 0 && !ct->pimple) {
+      val[kd - 1] = ap[c - 1];
+    } else {
+      val[kd - 1] = ap[c - 1] / ct->alpha_u;
+      rhs[c - 1] = rhs[c - 1] + (1.0 - ct->alpha_u) / ct->alpha_u * ap[c - 1] * U(c, comp);
+    }
+  }
+  free(psi);
+}
+
+/* ------------------------------------------------------- energy (LTE) */
+
+void or_temperature_assembly(or_case *cs, double *val, double *rhs, double *ap) {
+  const or_ctrl_t *ct = &cs->ctrl;
+  int nc = cs->nc, nf = cs->nf;
+  for (int k = 0; k < cs->nnz; k++) val[k] = 0.0;
+  for (int c = 0; c < nc; c++) { rhs[c] = 0.0; ap[c] = 0.0; }
+  double sd = 0.0, *psi = NULL;
+  if (ct->conv_blend > 0.0) {
+    psi = xcalloc(nc, 8);
+    bj_limiter(cs, 0, cs->T, psi);
+  }
+  for (int i = 0; i < nf; i++) {
+    int c1 = cs->c1[i];
+    if (c1 == 0) continue;
+    int c0 = cs->c0[i];
+    double FT = ct->cp * cs->flux[i], lf = cs->lf[i], dvec[3];
+    for (int d = 0; d < 3; d++) dvec[d] = XC(c1, d) - XC(c0, d);
+    double dn = norm3(dvec);
+    double k0 = cs->porosity[c0 - 1] * ct->k_cond + (1.0 - cs->porosity[c0 - 1]) * cs->k_s[c0 - 1];
+    double k1 = cs->porosity[c1 - 1] * ct->k_cond + (1.0 - cs->porosity[c1 - 1]) * cs->k_s[c1 - 1];
+    double kf = lf * k0 + (1.0 - lf) * k1;
+    kf = kf + (lf * 0.0 + (1.0 - lf) * 0.0);   /* zero dispersion term */
+    double DT;
+    if (ct->nonorth_corr > 0.0) {
+      sd = dot3(&cs->sf[3 * i], dvec);
+      DT = kf * (cs->area[i] * cs->area[i]) / sd;
+    } else {
+      DT = kf * cs->area[i] / dn;
+    }
+    ap[c0 - 1] = ap[c0 - 1] + fmax0(FT, 0.0) + DT;
+    ap[c1 - 1] = ap[c1 - 1] + fmax0(-FT, 0.0) + DT;
+    int kd = pos_of(cs, c0, c1);
+    val[kd - 1] = val[kd - 1] - (DT + fmax0(-FT, 0.0));
+    kd = pos_of(cs, c1, c0);
+    val[kd - 1] = val[kd - 1] - (DT + fmax0(FT, 0.0));
+    if (ct->conv_blend > 0.0) {
+      int cu = FT > 0.0 ? c0 : c1;
+      double r[3];
+      for (int d = 0; d < 3; d++) r[d] = XF(i, d) - XC(cu, d);
+      double dc = psi[cu - 1] * dot3(&cs->gt[3 * (cu - 1)], r);
+      dc = ct->conv_blend * FT * dc;
+      rhs[c0 - 1] = rhs[c0 - 1] - dc;
+      rhs[c1 - 1] = rhs[c1 - 1] + dc;
+    }
+    if (ct->nonorth_corr > 0.0) {
+      double gf[3], t[3];
+      for (int d = 0; d < 3; d++) {
+        gf[d] = lf * cs->gt[3 * (c0 - 1) + d] + (1.0 - lf) * cs->gt[3 * (c1 - 1) + d];
+        t[d] = SF(i, d) - dvec[d] * ((cs->area[i] * cs->area[i]) / sd);
+      }
+      double noc = ct->nonorth_corr * kf * dot3(gf, t);
+      rhs[c0 - 1] = rhs[c0 - 1] + noc;
+      rhs[c1 - 1] = rhs[c1 - 1] - noc;
+    }
+  }
+  for (int i = 0; i < nf; i++) {
+    int gi = cs->fgrp[i];
+    if (gi == 0) continue;
+    int c0 = cs->c0[i];
+    double Tf, qf, r[3];
+    int neum;
+    bc_face_T(cs, i, cs->T[c0 - 1], &Tf, &qf, &neum);
+    for (int d = 0; d < 3; d++) r[d] = XF(i, d) - XC(c0, d);
+    double dw = norm3(r);
+    double kf = cs->porosity[c0 - 1] * ct->k_cond + (1.0 - cs->porosity[c0 - 1]) * cs->k_s[c0 - 1];
+    kf = kf + 0.0;
+    double DT = kf * cs->area[i] / dw;
+    double FT = ct->cp * cs->flux[i];
+    if (!neum) {
+      ap[c0 - 1] = ap[c0 - 1] + DT + fmax0(FT, 0.0);
+      rhs[c0 - 1] = rhs[c0 - 1] + DT * Tf - fmin0(FT, 0.0) * Tf;
+    } else {
+      if (fabs(qf) > 0.0) rhs[c0 - 1] = rhs[c0 - 1] + qf * cs->area[i];
+      int bt = cs->gb[gi - 1].btype;
+      if (bt == OR_BC_POUTLET || bt == OR_BC_OUTFLOW) ap[c0 - 1] = ap[c0 - 1] + fmax0(FT, 0.0);
+    }
+  }
+  for (int c = 0; c < nc; c++) {
+    double cap = cs->porosity[c] * ct->rho * ct->cp + (1.0 - cs->porosity[c]) * cs->rho_s[c] * cs->cp_s[c];
+    if (cs->ts_order == 1) {
+      double at = cap * cs->vol[c] / ct->dt;
+      ap[c] = ap[c] + at;
+      rhs[c] = rhs[c] + at * cs->T_old[c];
+    } else if (cs->ts_order == 2) {
+      double at = cap * cs->vol[c] / (2.0 * ct->dt);
+      ap[c] = ap[c] + 3.0 * at;
+      rhs[c] = rhs[c] + at * (4.0 * cs->T_old[c] - cs->T_old_old[c]);
+    }
+  }
+  for (int c = 1; c <= nc; c++) {
+    int kd = pos_of(cs, c, c);
+    if (cs->ts_order > 0 && !ct->pimple) {
+      val[kd - 1] = ap[c - 1];
+    } else {
+      val[kd - 1] = ap[c - 1] / ct->alpha_u;
+      rhs[c - 1] = rhs[c - 1] + (1.0 - ct->alpha_u) / ct->alpha_u * ap[c - 1] * cs->T[c - 1];
+    }
+  }
+  free(psi);
+}
+
+/* --------------------------------------------------- Rhie-Chow, outflow */
+
+void or_flux_rhiechow(or_case *cs) {
+  const or_ctrl_t *ct = &cs->ctrl;
+  for (int i = 0; i < cs->nf; i++) {
+    int c0 = cs->c0[i], c1 = cs->c1[i];
+    if (c1 == 0) {
+      double uf[3];
+      bc_face_vel(cs, i, &U(c0, 0), uf);
+      cs->flux[i] = ct->rho * dot3(uf, &cs->sf[3 * i]);
+      continue;
+    }
+    double lf = cs->lf[i], ufl[3], gpl[3], dvec[3];
+    for (int d = 0; d < 3; d++) {
+      ufl[d] = lf * U(c0, d) + (1.0 - lf) * U(c1, d);
+      gpl[d] = lf * cs->gp[3 * (c0 - 1) + d] + (1.0 - lf) * cs->gp[3 * (c1 - 1) + d];
+      dvec[d] = XC(c1, d) - XC(c0, d);
+    }
+    double dn = norm3(dvec);
+    double pfn = (cs->p[c1 - 1] - cs->p[c0 - 1]) / dn;
+    double dbf = lf * cs->vol[c0 - 1] / cs->apc[c0 - 1] + (1.0 - lf) * cs->vol[c1 - 1] / cs->apc[c1 - 1];
+    const double *sf = &cs->sf[3 * i];
+    cs->flux[i] = ct->rho * (dot3(ufl, sf) + dbf * (dot3(gpl, sf) - pfn * dot3(dvec, sf) / dn));
+  }
+}
+
+void or_outflow_mass_rescale(or_case *cs) {
+  double m_req = 0.0, m_out = 0.0, a_out = 0.0;
+  for (int i = 0; i < cs->nf; i++) {
+    if (cs->c1[i] > 0 || cs->fgrp[i] == 0) continue;
+    int bt = cs->gb[cs->fgrp[i] - 1].btype;
+    if (bt == OR_BC_OUTFLOW) { m_out = m_out + cs->flux[i]; a_out = a_out + cs->area[i]; }
+    else if (bt != OR_BC_POUTLET && bt != OR_BC_FARFIELD) m_req = m_req - cs->flux[i];
+  }
+  if (a_out <= 0.0 || m_req <= 0.0) return;
+  if (m_out > 1.0e-2 * m_req) {
+    double beta = m_req / m_out;
+    for (int i = 0; i < cs->nf; i++)
+      if (cs->c1[i] == 0 && cs->fgrp[i] > 0 && cs->gb[cs->fgrp[i] - 1].btype == OR_BC_OUTFLOW)
+        cs->flux[i] = beta * cs->flux[i];
+  } else {
+    for (int i = 0; i < cs->nf; i++)
+      if (cs->c1[i] == 0 && cs->fgrp[i] > 0 && cs->gb[cs->fgrp[i] - 1].btype == OR_BC_OUTFLOW)
+        cs->flux[i] = m_req * cs->area[i] / a_out;
+  }
+}
+
+/* ------------------------------------------------------------------ PPE */
+
+void or_ppe_assembly(or_case *cs, double *val, double *rhs) {
+  const or_ctrl_t *ct = &cs->ctrl;
+  for (int k = 0; k < cs->nnz; k++) val[k] = 0.0;
+  for (int c = 0; c < cs->nc; c++) rhs[c] = 0.0;
+  int has_pdir = 0;
+  for (int i = 0; i < cs->nf; i++) {
+    int c0 = cs->c0[i], c1 = cs->c1[i];
+    if (c1 == 0) {
+      if (cs->fgrp[i] > 0 && bc_p_dirichlet(cs, i)) {
+        double r[3];
+        for (int d = 0; d < 3; d++) r[d] = XF(i, d) - XC(c0, d);
+        double dbf = cs->vol[c0 - 1] / cs->apc[c0 - 1];
+        double dn = norm3(r);
+        double af = ct->rho * dbf * cs->area[i] / dn;
+        int kd = pos_of(cs, c0, c0);
+        val[kd - 1] = val[kd - 1] + af;
+        has_pdir = 1;
+      }
+      rhs[c0 - 1] = rhs[c0 - 1] - cs->flux[i];
+      continue;
+    }
+    double lf = cs->lf[i], dvec[3];
+    double dbf = lf * cs->vol[c0 - 1] / cs->apc[c0 - 1] + (1.0 - lf) * cs->vol[c1 - 1] / cs->apc[c1 - 1];
+    for (int d = 0; d < 3; d++) dvec[d] = XC(c1, d) - XC(c0, d);
+    double dn = norm3(dvec);
+    double af = ct->rho * dbf * cs->area[i] / dn;
+    int kd;
+    kd = pos_of(cs, c0, c0); val[kd - 1] = val[kd - 1] + af;
+    kd = pos_of(cs, c0, c1); val[kd - 1] = val[kd - 1] - af;
+    kd = pos_of(cs, c1, c1); val[kd - 1] = val[kd - 1] + af;
+    kd = pos_of(cs, c1, c0); val[kd - 1] = val[kd - 1] - af;
+    rhs[c0 - 1] = rhs[c0 - 1] - cs->flux[i];
+    rhs[c1 - 1] = rhs[c1 - 1] + cs->flux[i];
+  }
+  if (!has_pdir) {
+    for (int k = cs->row_ptr[0]; k <= cs->row_ptr[1] - 1; k++)
+      val[k - 1] = cs->col_idx[k - 1] == 1 ? 1.0 : 0.0;
+    for (int r = 2; r <= cs->nc; r++) {
+      int kd = pos_of(cs, r, 1);
+      if (kd > 0) val[kd - 1] = 0.0;
+    }
+    rhs[0] = 0.0;
+  }
+}
+
+/* ------------------------------------------------------------ correction */
+
+void or_correct_fields(or_case *cs, const double *pp, double *dpp) {
+  const or_ctrl_t *ct = &cs->ctrl;
+  int nf = cs->nf, nc = cs->nc;
+  double *phif = xcalloc(nf, 8);
+  for (int i = 0; i < nf; i++) {
+    int c0 = cs->c0[i], c1 = cs->c1[i];
+    if (c1 > 0) phif[i] = cs->lf[i] * pp[c0 - 1] + (1.0 - cs->lf[i]) * pp[c1 - 1];
+    else if (cs->fgrp[i] > 0 && bc_p_dirichlet(cs, i)) phif[i] = 0.0;
+    else phif[i] = pp[c0 - 1];
+  }
+  grad_scalar(cs, phif, dpp, 3, 1);
+  for (int c = 0; c < nc; c++)
+    for (int d = 0; d < 3; d++) cs->u[3 * c + d] = cs->u[3 * c + d] - (cs->vol[c] / cs->apc[c]) * dpp[3 * c + d];
+  if (cs->ts_order > 0)
+    for (int c = 0; c < nc; c++) cs->p[c] = cs->p[c] + pp[c];
+  else
+    for (int c = 0; c < nc; c++) cs->p[c] = cs->p[c] + ct->alpha_p * pp[c];
+  for (int i = 0; i < nf; i++) {
+    int c0 = cs->c0[i], c1 = cs->c1[i];
+    if (c1 == 0) {
+      if (cs->fgrp[i] > 0 && bc_p_dirichlet(cs, i)) {
+        double r[3];
+        for (int d = 0; d < 3; d++) r[d] = XF(i, d) - XC(c0, d);
+        double dbf = cs->vol[c0 - 1] / cs->apc[c0 - 1];
+        double dn = norm3(r);
+        double af = ct->rho * dbf * cs->area[i] / dn;
+        cs->flux[i] = cs->flux[i] + af * pp[c0 - 1];
+      }
+      continue;
+    }
+    double lf = cs->lf[i], dvec[3];
+    double dbf = lf * cs->vol[c0 - 1] / cs->apc[c0 - 1] + (1.0 - lf) * cs->vol[c1 - 1] / cs->apc[c1 - 1];
+    for (int d = 0; d < 3; d++) dvec[d] = XC(c1, d) - XC(c0, d);
+    double dn = norm3(dvec);
+    double af = ct->rho * dbf * cs->area[i] / dn;
+    cs->flux[i] = cs->flux[i] - af * (pp[c1 - 1] - pp[c0 - 1]);
+  }
+  free(phif);
+}
+
+/* ------------------------------------------------------- linear algebra */
+
+void or_csr_matvec(int n, const int *rp, const int *ci, const double *v,
+                   const double *x, double *y) {
+  for (int i = 0; i < n; i++) {
+    y[i] = 0.0;
+    for (int k = rp[i]; k <= rp[i + 1] - 1; k++) y[i] = y[i] + v[k - 1] * x[ci[k - 1] - 1];
+  }
+}
+
+static int csr_find(const int *rp, const int *ci, int i, int j) {
+  for (int k = rp[i - 1]; k <= rp[i] - 1; k++) {
+    if (ci[k - 1] == j) return k;
+    if (ci[k - 1] > j) return 0;
+  }
+  return 0;
+}
+
+static double vnorm(const double *a, int n, int mpi) {
+  return mpi ? sqrt(dotn(a, a, n)) : or_norm2(a, n);
+}
+
+int or_ilu0_factor(int n, const int *rp, const int *ci, double *v) {
+  int *dp = xcalloc(n, sizeof(int));
+  for (int i = 1; i <= n; i++) {
+    dp[i - 1] = csr_find(rp, ci, i, i);
+    if (dp[i - 1] == 0) { free(dp); return 1; }
+  }
+  for (int i = 2; i <= n; i++) {
+    for (int kk = rp[i - 1]; kk <= dp[i - 1] - 1; kk++) {
+      int k = ci[kk - 1];
+      if (fabs(v[dp[k - 1] - 1]) < TINY) { free(dp); return 2; }
+      double aik = v[kk - 1] / v[dp[k - 1] - 1];
+      v[kk - 1] = aik;
+      for (int jj = dp[k - 1]; jj <= rp[k] - 1; jj++) {
+        int j = ci[jj - 1];
+        if (j <= k) continue;
+        int pos = csr_find(rp, ci, i, j);
+        if (pos > 0) v[pos - 1] = v[pos - 1] - aik * v[jj - 1];
+      }
+    }
+  }
+  free(dp);
+  return 0;
+}
+
+static void ilu0_apply(int n, const int *rp, const int *ci, const double *v, const int *dp,
+                       const double *r, double *z) {
+  for (int i = 0; i < n; i++) z[i] = r[i];
+  for (int i = 1; i <= n; i++) {
+    double s = z[i - 1];
+    for (int k = rp[i - 1]; k <= dp[i - 1] - 1; k++) s = s - v[k - 1] * z[ci[k - 1] - 1];
+    z[i - 1] = s;
+  }
+  for (int i = n; i >= 1; i--) {
+    double s = z[i - 1];
+    for (int k = dp[i - 1] + 1; k <= rp[i] - 1; k++) s = s - v[k - 1] * z[ci[k - 1] - 1];
+    z[i - 1] = s / v[dp[i - 1] - 1];
+  }
+}
+
+int or_ic0_factor(int n, const int *rp, const int *ci, double *v) {
+  int *dp = xcalloc(n, sizeof(int));
+  for (int i = 1; i <= n; i++) {
+    dp[i - 1] = csr_find(rp, ci, i, i);
+    if (dp[i - 1] == 0) { free(dp); return 1; }
+  }
+  for (int i = 1; i <= n; i++) {
+    int dpi = dp[i - 1];
+    for (int kp = rp[i - 1]; kp <= dpi - 1; kp++) {
+      int k = ci[kp - 1];
+      /* ic0_dot(A, row_ptr(i), kp-1, row_ptr(k), diag_pos(k)-1) */
+      int p = rp[i - 1], p2 = kp - 1, q = rp[k - 1], q2 = dp[k - 1] - 1;
+      double dsum = 0.0;
+      while (p <= p2 && q <= q2) {
+        if (ci[p - 1] < ci[q - 1]) p++;
+        else if (ci[p - 1] > ci[q - 1]) q++;
+        else { dsum = dsum + v[p - 1] * v[q - 1]; p++; q++; }
+      }
+      double s = v[kp - 1] - dsum;
+      v[kp - 1] = s / v[dp[k - 1] - 1];
+    }
+    double s = v[dpi - 1];
+    for (int kp = rp[i - 1]; kp <= dpi - 1; kp++) s = s - v[kp - 1] * v[kp - 1];
+    if (s <= 0.0) { free(dp); return 2; }
+    v[dpi - 1] = sqrt(s);
+  }
+  free(dp);
+  return 0;
+}
+
+static void ic0_apply(int n, const int *rp, const int *ci, const double *v, const int *dp,
+                      const double *r, double *z, double *w) {
+  for (int i = 1; i <= n; i++) {
+    double s = r[i - 1];
+    for (int kp = rp[i - 1]; kp <= dp[i - 1] - 1; kp++) s = s - v[kp - 1] * z[ci[kp - 1] - 1];
+    z[i - 1] = s / v[dp[i - 1] - 1];
+  }
+  for (int i = 0; i < n; i++) w[i] = z[i];
+  for (int i = n; i >= 1; i--) {
+    z[i - 1] = w[i - 1] / v[dp[i - 1] - 1];
+    for (int kp = rp[i - 1]; kp <= dp[i - 1] - 1; kp++) {
+      int k = ci[kp - 1];
+      w[k - 1] = w[k - 1] - v[kp - 1] * z[i - 1];
+    }
+  }
+}
+
+static int *diag_positions(int n, const int *rp, const int *ci) {
+  int *dp = xcalloc(n, sizeof(int));
+  for (int i = 1; i <= n; i++) dp[i - 1] = csr_find(rp, ci, i, i);
+  return dp;
+}
+
+/* bicgstab_ilu0 (:148-224) / bicgstab_ilu0_mpi (mod_uns_linsolver_mpi.f90:155-229) */
+void or_bicgstab_ilu0(int n, const int *rp, const int *ci, const double *v,
+                      const double *b, double *x, double tol, int maxit,
+                      int *it, double *res, int *ierr, int mpi) {
+  *it = 0; *ierr = 0; *res = 0.0;
+  double bnrm = vnorm(b, n, mpi);
+  if (bnrm == 0.0) { for (int i = 0; i < n; i++) x[i] = 0.0; return; }
+  int nnz = rp[n] - 1;
+  double *lu = xcalloc(nnz, 8);
+  memcpy(lu, v, nnz * sizeof(double));
+  int e = or_ilu0_factor(n, rp, ci, lu);
+  if (e) { *ierr = e + 10; free(lu); return; }
+  int *dp = diag_positions(n, rp, ci);
+  double *r0 = xcalloc(n, 8), *r = xcalloc(n, 8), *p = xcalloc(n, 8), *vv = xcalloc(n, 8);
+  double *s = xcalloc(n, 8), *t = xcalloc(n, 8), *ph = xcalloc(n, 8), *sh = xcalloc(n, 8);
+  or_csr_matvec(n, rp, ci, v, x, vv);
+  for (int i = 0; i < n; i++) { r0[i] = b[i] - vv[i]; r[i] = r0[i]; p[i] = 0.0; vv[i] = 0.0; }
+  double rho_old = 1.0, alpha = 1.0, omega = 1.0, rho, beta, tt, snrm;
+  *res = vnorm(r0, n, mpi) / bnrm;
+  int k;
+  if (*res <= tol) goto done;
+  for (k = 1; k <= maxit; k++) {
+    *it = k;
+    rho = dotn(r, r0, n);
+    if (fabs(rho) < TINY) { *ierr = 2; break; }
+    beta = (rho / rho_old) * (alpha / omega);
+    for (int i = 0; i < n; i++) p[i] = r[i] + beta * (p[i] - omega * vv[i]);
+    ilu0_apply(n, rp, ci, lu, dp, p, ph);
+    or_csr_matvec(n, rp, ci, v, ph, vv);
+    alpha = rho / dotn(r0, vv, n);
+    for (int i = 0; i < n; i++) s[i] = r[i] - alpha * vv[i];
+    snrm = vnorm(s, n, mpi);
+    if (snrm / bnrm <= tol) {
+      for (int i = 0; i < n; i++) x[i] = x[i] + alpha * ph[i];
+      *res = snrm / bnrm;
+      goto done;
+    }
+    ilu0_apply(n, rp, ci, lu, dp, s, sh);
+    or_csr_matvec(n, rp, ci, v, sh, t);
+    tt = dotn(t, t, n);
+    if (tt < TINY) { *ierr = 2; break; }
+    omega = dotn(t, s, n) / tt;
+    if (fabs(omega) < TINY) { *ierr = 2; break; }
+    for (int i = 0; i < n; i++) {
+      x[i] = x[i] + alpha * ph[i] + omega * sh[i];
+      r[i] = s[i] - omega * t[i];
+    }
+    *res = vnorm(r, n, mpi) / bnrm;
+    if (*res <= tol) goto done;
+    rho_old = rho;
+  }
+  if (*ierr == 0) { *ierr = 1; *it = maxit; }
+done:
+  free(lu); free(dp); free(r0); free(r); free(p); free(vv); free(s); free(t); free(ph); free(sh);
+}
+
+/* cg_jacobi (:230-291) / cg_jacobi_mpi and cg_ic0 (:405-461) / cg_ic0_mpi */
+static void pcg(int n, const int *rp, const int *ci, const double *v, const double *b, double *x,
+                double tol, int maxit, int *it, double *res, int *ierr, int mpi, int ic0) {
+  *it = 0; *ierr = 0; *res = 0.0;
+  double bnrm = vnorm(b, n, mpi);
+  if (bnrm == 0.0) { for (int i = 0; i < n; i++) x[i] = 0.0; return; }
+  int nnz = rp[n] - 1;
+  double *L = NULL, *diag = NULL, *w = NULL;
+  int *dp = NULL;
+  if (ic0) {
+    L = xcalloc(nnz, 8);
+    memcpy(L, v, nnz * sizeof(double));
+    int e = or_ic0_factor(n, rp, ci, L);
+    if (e) { *ierr = e + 10; free(L); return; }
+    dp = diag_positions(n, rp, ci);
+    w = xcalloc(n, 8);
+  } else {
+    diag = xcalloc(n, 8);
+    for (int i = 1; i <= n; i++) {
+      int k = csr_find(rp, ci, i, i);
+      if (k == 0 || (mpi && fabs(v[k - 1]) < TINY)) { *ierr = 2; free(diag); return; }
+      diag[i - 1] = v[k - 1];
+    }
+  }
+  double *r = xcalloc(n, 8), *z = xcalloc(n, 8), *p = xcalloc(n, 8), *ap = xcalloc(n, 8);
+  double rz, rz_new, pap, alpha, beta;
+  or_csr_matvec(n, rp, ci, v, x, ap);
+  for (int i = 0; i < n; i++) r[i] = b[i] - ap[i];
+  *res = vnorm(r, n, mpi) / bnrm;
+  if (*res <= tol) goto done;
+  if (ic0) ic0_apply(n, rp, ci, L, dp, r, z, w);
+  else for (int i = 0; i < n; i++) z[i] = r[i] / diag[i];
+  for (int i = 0; i < n; i++) p[i] = z[i];
+  rz = dotn(r, z, n);
+  int k;
+  for (k = 1; k <= maxit; k++) {
+    *it = k;
+    or_csr_matvec(n, rp, ci, v, p, ap);
+    pap = dotn(p, ap, n);
+    if (fabs(pap) < TINY) { *ierr = 2; break; }
+    alpha = rz / pap;
+    for (int i = 0; i < n; i++) { x[i] = x[i] + alpha * p[i]; r[i] = r[i] - alpha * ap[i]; }
+    *res = vnorm(r, n, mpi) / bnrm;
+    if (*res <= tol) goto done;
+    if (ic0) ic0_apply(n, rp, ci, L, dp, r, z, w);
+    else for (int i = 0; i < n; i++) z[i] = r[i] / diag[i];
+    rz_new = dotn(r, z, n);
+    if (fabs(rz) < TINY) { *ierr = 2; break; }
+    beta = rz_new / rz;
+    for (int i = 0; i < n; i++) p[i] = z[i] + beta * p[i];
+    rz = rz_new;
+  }
+  if (k > maxit) *it = maxit + 1;   /* Fortran DO-variable value on exhaustion */
+  if (*ierr == 0) *ierr = 1;
+done:
+  free(L); free(diag); free(w); free(dp); free(r); free(z); free(p); free(ap);
+}
+
+void or_cg_jacobi(int n, const int *rp, const int *ci, const double *v, const double *b,
+                  double *x, double tol, int maxit, int *it, double *res, int *ierr, int mpi) {
+  pcg(n, rp, ci, v, b, x, tol, maxit, it, res, ierr, mpi, 0);
+}
+
+void or_cg_ic0(int n, const int *rp, const int *ci, const double *v, const double *b,
+               double *x, double tol, int maxit, int *it, double *res, int *ierr, int mpi) {
+  pcg(n, rp, ci, v, b, x, tol, maxit, it, res, ierr, mpi, 1);
+}
+
+/* ---------------------------------------------------------------- drivers */
+
+static void push_hist(or_case *cs, double a, double b, double c, double d, double e, double f) {
+  if (cs->nhist == cs->caphist) {
+    cs->caphist = cs->caphist ? 2 * cs->caphist : 64;
+    cs->hist = realloc(cs->hist, 6 * sizeof(double) * cs->caphist);
+  }
+  double *h = &cs->hist[6 * cs->nhist++];
+  h[0] = a; h[1] = b; h[2] = c; h[3] = d; h[4] = e; h[5] = f;
+}
+
+const double *or_history(or_case *cs, int *nrows) { *nrows = cs->nhist; return cs->hist; }
+
+static double flux_reference(const or_case *cs) {
+  double usc = 0.0, ab = 0.0;
+  for (int i = 0; i < cs->nbc; i++) {
+    const or_bcgroup_t *g = &cs->gb[i];
+    double a = norm3(g->lid_vel), b = norm3(g->uvel);
+    double m = usc;
+    if (a > m) m = a;
+    if (b > m) m = b;
+    if (g->uspeed > m) m = g->uspeed;
+    usc = m;
+    for (int f = 0; f < cs->nf; f++)
+      if (cs->fgrp[f] == i + 1) ab = ab + cs->area[f];
+  }
+  if (usc <= 0.0) usc = 1.0;
+  if (ab <= 0.0) ab = 1.0;
+  return cs->ctrl.rho * usc * ab;
+}
+
+static double max_imbalance(const or_case *cs, double *imb) {
+  for (int c = 0; c < cs->nc; c++) imb[c] = 0.0;
+  for (int i = 0; i < cs->nf; i++) {
+    imb[cs->c0[i] - 1] = imb[cs->c0[i] - 1] + cs->flux[i];
+    if (cs->c1[i] > 0) imb[cs->c1[i] - 1] = imb[cs->c1[i] - 1] - cs->flux[i];
+  }
+  double m = 0.0;
+  for (int c = 0; c < cs->nc; c++) if (fabs(imb[c]) > m) m = fabs(imb[c]);
+  return m;
+}
+
+static void momentum_solves(or_case *cs, double *val, double *rhs, double *ap, double *ucomp,
+                            int mpi, int *itl_max, double *ressum, int *brk) {
+  const or_ctrl_t *ct = &cs->ctrl;
+  int nc = cs->nc;
+  *brk = 0;
+  for (int comp = 1; comp <= 3; comp++) {
+    or_momentum_assembly(cs, comp, val, rhs, ap);
+    for (int c = 0; c < nc; c++) ucomp[c] = cs->u[3 * c + comp - 1];
+    int itl, ierr; double resl;
+    or_bicgstab_ilu0(nc, cs->row_ptr, cs->col_idx, val, rhs, ucomp, ct->lin_tol, ct->lin_max,
+                     &itl, &resl, &ierr, mpi);
+    for (int c = 0; c < nc; c++) cs->u[3 * c + comp - 1] = ucomp[c];
+    if (ierr == 2) { *brk = comp; return; }
+    if (itl > *itl_max) *itl_max = itl;
+    *ressum = *ressum + resl;
+  }
+}
+
+static int ppe_solve(or_case *cs, double *val, double *rhs, double *pp, int mpi, int *itl) {
+  const or_ctrl_t *ct = &cs->ctrl;
+  int ierr; double resl;
+  or_ppe_assembly(cs, val, rhs);
+  for (int c = 0; c < cs->nc; c++) pp[c] = 0.0;
+  if (ct->ppe_precond == 1)
+    or_cg_ic0(cs->nc, cs->row_ptr, cs->col_idx, val, rhs, pp, ct->lin_tol, ct->lin_max, itl, &resl, &ierr, mpi);
+  else
+    or_cg_jacobi(cs->nc, cs->row_ptr, cs->col_idx, val, rhs, pp, ct->lin_tol, ct->lin_max, itl, &resl, &ierr, mpi);
+  return ierr;
+}
+
+static int temperature_solve(or_case *cs, double *val, double *rhs, double *ap, int mpi, int *itl) {
+  const or_ctrl_t *ct = &cs->ctrl;
+  int ierr; double resl;
+  or_temperature_assembly(cs, val, rhs, ap);
+  or_bicgstab_ilu0(cs->nc, cs->row_ptr, cs->col_idx, val, rhs, cs->T, ct->lin_tol, ct->lin_max,
+                   itl, &resl, &ierr, mpi);
+  return ierr;
+}
+
+static int nonfinite(const or_case *cs) {
+  for (int c = 0; c < cs->nc; c++) {
+    for (int d = 0; d < 3; d++) if (!isfinite(cs->u[3 * c + d])) return 1;
+    if (!isfinite(cs->p[c]) || !isfinite(cs->T[c])) return 1;
+  }
+  return 0;
+}
+
+/* simple_run (mod_uns_simple.f90:88-288); with mpi = 1 the solver norms of
+ * simple_run_mpi (mod_uns_simple_mpi.f90:57-289) */
+int or_simple_run(or_case *cs, int nsteps, int mpi, int *it_done) {
+  const or_ctrl_t *ct = &cs->ctrl;
+  int nc = cs->nc;
+  double *val = xcalloc(cs->nnz, 8), *rhs = xcalloc(nc, 8), *ap = xcalloc(nc, 8);
+  double *uo = xcalloc(3 * (size_t)nc, 8), *pp = xcalloc(nc, 8), *dpp = xcalloc(3 * (size_t)nc, 8);
+  double *imb = xcalloc(nc, 8), *Tprev = xcalloc(nc, 8), *ucomp = xcalloc(nc, 8);
+  int ier = 0;
+  cs->nhist = 0;
+  double flux_ref = flux_reference(cs);
+  for (int c = 0; c < nc; c++) cs->apc[c] = 1.0;
+  cs->ts_order = 0;
+  or_compute_gradients(cs);
+  int it_max = ct->outer_max;
+  if (nsteps > 0 && nsteps < it_max) it_max = nsteps;
+  int it;
+  double imbal = 0, du_max = 0, dT_max = 0;
+  for (it = 1; it <= it_max; it++) {
+    memcpy(uo, cs->u, 3 * (size_t)nc * 8);
+    int itl_max = 0, itl, brk;
+    double ressum = 0.0;
+    if (ct->inlet_ramp > 1) {
+      double f = (double)it / (double)ct->inlet_ramp;
+      cs->inlet_ramp = fmax0(0.0, fmin0(1.0, f));
+      cs->pval_ramp = fmax0(0.0, fmin0(1.0, f));
+    }
+    momentum_solves(cs, val, rhs, ap, ucomp, mpi, &itl_max, &ressum, &brk);
+    if (brk) { ier = 20; break; }
+    for (int c = 0; c < nc; c++) cs->apc[c] = ap[c] / ct->alpha_u;
+    or_compute_gradients(cs);
+    or_flux_rhiechow(cs);
+    or_outflow_mass_rescale(cs);
+    if (ppe_solve(cs, val, rhs, pp, mpi, &itl) == 2) { ier = 21; break; }
+    if (itl > itl_max) itl_max = itl;
+    or_correct_fields(cs, pp, dpp);
+    memcpy(Tprev, cs->T, nc * 8);
+    if (temperature_solve(cs, val, rhs, ap, mpi, &itl) == 2) { ier = 22; break; }
+    if (itl > itl_max) itl_max = itl;
+    if (nonfinite(cs)) { ier = 60; break; }
+    imbal = max_imbalance(cs, imb) / flux_ref;
+    du_max = 0.0; dT_max = 0.0;
+    for (size_t k = 0; k < 3 * (size_t)nc; k++) if (fabs(cs->u[k] - uo[k]) > du_max) du_max = fabs(cs->u[k] - uo[k]);
+    for (int c = 0; c < nc; c++) if (fabs(cs->T[c] - Tprev[c]) > dT_max) dT_max = fabs(cs->T[c] - Tprev[c]);
+    push_hist(cs, imbal, du_max, dT_max, 0.0, itl_max, ressum / 3.0);
+    if (imbal < ct->outer_tol && du_max < ct->outer_tol && dT_max < ct->outer_tol) break;
+  }
+  if (it_done) *it_done = it;
+  free(val); free(rhs); free(ap); free(uo); free(pp); free(dpp); free(imb); free(Tprev); free(ucomp);
+  return ier;
+}
+
+/* piso_run_mpi at one partition (mod_uns_simple_mpi.f90:298-571); with
+ * mpi = 0 the serial piso_run norms (mod_uns_simple.f90:303-547) */
+int or_piso_run(or_case *cs, int nsteps, int mpi) {
+  const or_ctrl_t *ct = &cs->ctrl;
+  int nc = cs->nc;
+  if (!ct->transient) return 30;
+  if (ct->dt <= 0.0) return 31;
+  double *val = xcalloc(cs->nnz, 8), *rhs = xcalloc(nc, 8), *ap = xcalloc(nc, 8);
+  double *pp = xcalloc(nc, 8), *dpp = xcalloc(3 * (size_t)nc, 8), *imb = xcalloc(nc, 8);
+  double *ucomp = xcalloc(nc, 8);
+  int ier = 0;
+  cs->nhist = 0;
+  int n_steps = ct->n_time_max > 1 ? ct->n_time_max : 1;
+  if (nsteps > 0) n_steps = nsteps;
+  double flux_ref = flux_reference(cs);
+  for (int c = 0; c < nc; c++) cs->apc[c] = 1.0;
+  or_compute_gradients(cs);
+  or_flux_rhiechow(cs);
+  size_t n3 = 3 * (size_t)nc;
+  for (int it = 1; it <= n_steps; it++) {
+    memcpy(cs->u_old_old, cs->u_old, n3 * 8);
+    memcpy(cs->u_old, cs->u, n3 * 8);
+    memcpy(cs->T_old_old, cs->T_old, nc * 8);
+    memcpy(cs->T_old, cs->T, nc * 8);
+    memcpy(cs->T_s_old_old, cs->T_s_old, nc * 8);
+    memcpy(cs->T_s_old, cs->T_s, nc * 8);
+    cs->ts_order = (it == 1 || ct->time_scheme == 1) ? 1 : 2;
+    int itl_max = 0, itl, brk;
+    double ressum = 0.0;
+    momentum_solves(cs, val, rhs, ap, ucomp, mpi, &itl_max, &ressum, &brk);
+    if (brk) { ier = 33; break; }
+    memcpy(cs->apc, ap, nc * 8);
+    or_compute_gradients(cs);
+    or_flux_rhiechow(cs);
+    or_outflow_mass_rescale(cs);
+    int bad = 0;
+    for (int ic = 1; ic <= ct->n_correct; ic++) {
+      if (ppe_solve(cs, val, rhs, pp, mpi, &itl) == 2) { bad = 1; break; }
+      if (itl > itl_max) itl_max = itl;
+      or_correct_fields(cs, pp, dpp);
+    }
+    if (bad) { ier = 34; break; }
+    if (temperature_solve(cs, val, rhs, ap, mpi, &itl) == 2) { ier = 35; break; }
+    if (itl > itl_max) itl_max = itl;
+    double imbal = max_imbalance(cs, imb) / flux_ref;
+    double du_max = 0.0, umax = 0.0;
+    for (size_t k = 0; k < n3; k++) if (fabs(cs->u[k] - cs->u_old[k]) > du_max) du_max = fabs(cs->u[k] - cs->u_old[k]);
+    du_max = du_max / ct->dt;
+    for (int c = 0; c < nc; c++) { double n = norm3(&cs->u[3 * c]); if (n > umax) umax = n; }
+    double cfl = 0.0;
+    for (int c = 0; c < nc; c++) {
+      double dtdx = umax * ct->dt / pow(cs->vol[c], 1.0 / 3.0);
+      if (dtdx > cfl) cfl = dtdx;
+    }
+    if (nonfinite(cs)) { ier = 60; break; }
+    push_hist(cs, imbal, du_max, umax, cfl, itl_max, ressum / 3.0);
+  }
+  free(val); free(rhs); free(ap); free(pp); free(dpp); free(imb); free(ucomp);
+  return ier;
+}

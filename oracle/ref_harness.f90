@@ -1,0 +1,212 @@
+!===============================================================================
+! ref_harness.f90 -- fixture writer driving the reference's own FP64 modules
+!
+! Test infrastructure only: this program is linked against the reference
+! sources compiled from /root/reference by oracle/build_ref.sh and is used to
+! write golden vectors for tests/golden.  It never ships in the product.
+!
+! usage: ref_harness <mesh.cas> <unMesh.control> <mode> <out.bin> [in.bin] [n]
+!   geom   : geometry, lf, connectivity and CSR pattern
+!   simple : seed fields from in.bin (u,p,T), run simple_run(nsteps=n)
+!   piso   : seed fields from in.bin (u,p,T), run piso_run (n_time_max steps)
+!   funcs  : seed fields from in.bin, then compute_gradients, momentum_assembly
+!            (comp 1..3), flux_rhiechow, temperature_assembly, correct_fields
+!   solve  : CSR system from in.bin; residual histories of cg_jacobi, cg_ic0,
+!            bicgstab_ilu0 for maxit = 1..n, IC0/ILU0 factors, final solutions
+! All arrays are written as raw little-endian stream records in the order the
+! Python reader (tests/golden/make_golden.py) expects.
+!===============================================================================
+program ref_harness
+   use mod_precision, only: dp
+   use mod_uns_mesh
+   use mod_uns_cas_reader, only: read_cas
+   use mod_uns_connectivity
+   use mod_uns_geometry
+   use mod_uns_control
+   use mod_uns_bc
+   use mod_uns_fields
+   use mod_uns_linsolver
+   use mod_uns_simple
+   implicit none
+
+   character(len=512) :: casf, ctlf, mode, outf, inf, narg
+   type(mesh_t)   :: m
+   type(conn_t)   :: c
+   type(geom_t)   :: g
+   type(ctrl_t)   :: ctrl
+   type(bc_t)     :: bcs
+   type(fields_t) :: fld
+   integer :: ier, nsteps, uo, ui, nargs
+
+   nargs = command_argument_count()
+   call get_command_argument(1, casf)
+   call get_command_argument(2, ctlf)
+   call get_command_argument(3, mode)
+   call get_command_argument(4, outf)
+   inf = ''
+   if ( nargs >= 5 ) call get_command_argument(5, inf)
+   nsteps = 1
+   if ( nargs >= 6 ) then
+      call get_command_argument(6, narg)
+      read(narg, *) nsteps
+   end if
+
+   if ( trim(mode) == 'solve' ) then
+      call run_solve()
+      stop
+   end if
+
+   call read_cas( trim(casf), m, ier );            call chk(ier, 'read_cas')
+   call build_connectivity( m, c, ier );           call chk(ier, 'connectivity')
+   call compute_geometry( m, g, ier );             call chk(ier, 'geometry')
+   call read_control( trim(ctlf), ctrl, ier );     call chk(ier, 'control')
+   call resolve_cell_zones( m, ctrl, ier );        call chk(ier, 'cell zones')
+   call build_bc( m, ctrl, bcs, ier );             call chk(ier, 'bc')
+   call init_fields( m, g, fld )
+   call setup_porous_fields( m, ctrl, fld )
+
+   open( newunit=uo, file=trim(outf), access='stream', form='unformatted', &
+         status='replace', action='write' )
+
+   select case ( trim(mode) )
+   case ( 'geom' )
+      call dump_geom()
+   case ( 'simple' )
+      call seed_fields()
+      call simple_run( m, c, g, ctrl, bcs, fld, ier, nsteps )
+      write(uo) ier
+      call dump_state()
+   case ( 'piso' )
+      call seed_fields()
+      call piso_run( m, c, g, ctrl, bcs, fld, '/tmp/ref_harness_piso', ier )
+      write(uo) ier
+      call dump_state()
+   case ( 'funcs' )
+      call run_funcs()
+   case default
+      write(*,*) 'unknown mode ', trim(mode)
+      stop 2
+   end select
+   close( uo )
+
+contains
+
+   subroutine chk( ierr, what )
+      integer, intent(in) :: ierr
+      character(len=*), intent(in) :: what
+      if ( ierr /= 0 ) then
+         write(*,'(a,a,i0)') 'ref_harness: failed in ', what, ier
+         stop 3
+      end if
+   end subroutine chk
+
+   subroutine dump_geom()
+      type(csr_t) :: A
+      write(uo) m%ncells, m%nfaces, size(c%cf)
+      write(uo) g%xf, g%sf, g%area, g%vol, g%xc, fld%lf
+      write(uo) c%cf_ptr, c%cf, c%c2c_ptr, c%c2c
+      call build_cell_csr( m, c, A )
+      write(uo) size(A%col_idx)
+      write(uo) A%row_ptr, A%col_idx
+      write(uo) bcs%fgrp
+   end subroutine dump_geom
+
+   ! in.bin: u(3,nc), p(nc), T(nc)
+   subroutine seed_fields()
+      if ( len_trim(inf) == 0 ) return
+      open( newunit=ui, file=trim(inf), access='stream', form='unformatted', &
+            status='old', action='read' )
+      read(ui) fld%u, fld%p, fld%T
+      close( ui )
+      fld%T_s = fld%T
+   end subroutine seed_fields
+
+   subroutine dump_state()
+      write(uo) fld%u, fld%p, fld%flux, fld%T, fld%gp, fld%gu, fld%apc, fld%gt
+   end subroutine dump_state
+
+   ! in.bin: u, p, T, flux(nf), u_old, u_old_old, T_old, T_old_old, apc(nc),
+   !         pp(nc), ts_order (int)
+   subroutine run_funcs()
+      type(csr_t) :: A
+      real(dp), allocatable :: rhs(:), ap(:), pp(:), phif(:), dpp(:,:)
+      integer :: comp, tso
+      allocate( rhs(m%ncells), ap(m%ncells), pp(m%ncells), phif(m%nfaces), &
+                dpp(3,m%ncells) )
+      open( newunit=ui, file=trim(inf), access='stream', form='unformatted', &
+            status='old', action='read' )
+      read(ui) fld%u, fld%p, fld%T, fld%flux, fld%u_old, fld%u_old_old, &
+               fld%T_old, fld%T_old_old, fld%apc, pp, tso
+      close( ui )
+      fld%ts_order = tso
+      fld%T_s = fld%T
+      call build_cell_csr( m, c, A )
+      call compute_gradients( m, g, bcs, fld )
+      write(uo) fld%gp, fld%gu, fld%gt
+      do comp = 1, 3
+         call momentum_assembly( m, g, ctrl, bcs, fld, comp, A, rhs, ap )
+         write(uo) A%val, rhs, ap
+      end do
+      call temperature_assembly( m, g, ctrl, bcs, fld, A, rhs, ap )
+      write(uo) A%val, rhs, ap
+      call flux_rhiechow( m, g, ctrl, bcs, fld )
+      write(uo) fld%flux
+      call outflow_mass_rescale( m, g, bcs, fld )
+      write(uo) fld%flux
+      call correct_fields( m, g, ctrl, bcs, fld, pp, phif, dpp )
+      write(uo) fld%u, fld%p, fld%flux, dpp
+   end subroutine run_funcs
+
+   ! in.bin: n, nnz, row_ptr(n+1), col_idx(nnz), val(nnz), b(n), x0(n),
+   !         tol, kmax ;   nsteps = maxit used for the final solves
+   subroutine run_solve()
+      type(csr_t) :: A, L
+      real(dp), allocatable :: b(:), x0(:), x(:)
+      real(dp) :: tol, res
+      integer :: n, nnz, kmax, k, it, ierr, isolver
+      open( newunit=ui, file=trim(inf), access='stream', form='unformatted', &
+            status='old', action='read' )
+      read(ui) n, nnz
+      A%nrows = n
+      allocate( A%row_ptr(n+1), A%col_idx(nnz), A%val(nnz), b(n), x0(n), x(n) )
+      read(ui) A%row_ptr, A%col_idx, A%val, b, x0, tol, kmax
+      close( ui )
+      open( newunit=uo, file=trim(outf), access='stream', form='unformatted', &
+            status='replace', action='write' )
+      L = A
+      call ic0_factor( L, ierr )
+      write(uo) ierr, L%val
+      L = A
+      call ilu0_factor( L, ierr )
+      write(uo) ierr, L%val
+      do isolver = 1, 3
+         do k = 1, kmax
+            x = x0
+            call run_one( isolver, A, b, x, tol, k, it, res, ierr )
+            write(uo) it, ierr, res
+         end do
+         x = x0
+         call run_one( isolver, A, b, x, tol, nsteps, it, res, ierr )
+         write(uo) it, ierr, res, x
+      end do
+      close( uo )
+   end subroutine run_solve
+
+   subroutine run_one( isolver, A, b, x, tol, maxit, it, res, ierr )
+      integer, intent(in) :: isolver, maxit
+      type(csr_t), intent(in) :: A
+      real(dp), intent(in) :: b(:), tol
+      real(dp), intent(inout) :: x(:)
+      integer, intent(out) :: it, ierr
+      real(dp), intent(out) :: res
+      select case ( isolver )
+      case ( 1 )
+         call cg_jacobi( A, b, x, tol, maxit, it, res, ierr )
+      case ( 2 )
+         call cg_ic0( A, b, x, tol, maxit, it, res, ierr )
+      case default
+         call bicgstab_ilu0( A, b, x, tol, maxit, it, res, ierr )
+      end select
+   end subroutine run_one
+
+end program ref_harness
