@@ -1,0 +1,370 @@
+"""Write golden vectors from the reference's own FP64 Fortran path.
+
+Runs oracle/_ref/ref_harness (built from /root/reference by
+oracle/build_ref.sh) on small generated meshes and stores inputs and outputs
+as compressed .npz fixtures next to this script.  Only needs to run where the
+reference sources exist; the fixtures are committed data.
+
+  python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from mixsolver_amd import cas, meshgen  # noqa: E402
+
+HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+
+CAVITY_CTL = """rho = 1.0
+mu = 0.001
+alpha_u = 0.7
+alpha_p = 0.3
+outer_max = 1000
+outer_tol = 1e-8
+lin_tol = 1e-8
+lin_max = 1000
+conv_blend = {blend}
+nonorth_corr = {nonorth}
+ppe_precond = {precond}
+bc = 5 wall
+bc = 4 symmetry
+lid = 5 2 1.0 1.0 0.0 0.0
+"""
+
+CHANNEL_CTL = """rho = 1.0
+mu = {mu}
+transient = true
+dt = {dt}
+n_time_max = {nsteps}
+n_correct = 2
+n_out_every = 1000000
+time_scheme = 2
+lin_tol = 1e-10
+lin_max = 2000
+conv_blend = {blend}
+nonorth_corr = 1.0
+ppe_precond = {precond}
+body_force = 1.0 0.0 0.0
+bc = 6 velocity-inlet-parabolic 1.0 2.0 2 -1.0 1.0
+bc = 7 outflow
+bc = 5 wall
+bc = 4 symmetry
+tbc = 5 1 0.5
+k_cond = 0.01
+cp = 1.0
+"""
+
+PRISM_CTL = """rho = 1.0
+mu = 0.02
+transient = true
+dt = 0.05
+n_time_max = 2
+n_correct = 2
+n_out_every = 1000000
+time_scheme = 2
+lin_tol = 1e-10
+lin_max = 2000
+conv_blend = 0.5
+nonorth_corr = 1.0
+ppe_precond = ic0
+bc = 6 pressure-far-field 0.0 1.0 0.0 0.0
+bc = 5 wall
+bc = 4 symmetry
+tbc = 5 1 1.0
+k_cond = 0.05
+cp = 1.0
+"""
+
+OUTLET_CTL = """rho = 1.2
+mu = 0.01
+alpha_u = 0.6
+alpha_p = 0.4
+outer_max = 50
+outer_tol = 1e-9
+lin_tol = 1e-9
+lin_max = 500
+conv_blend = 1.0
+nonorth_corr = 1.0
+ppe_precond = jacobi
+bc = 6 velocity-inlet 1.0 0.0 0.0 1.0
+bc = 7 pressure-outlet 0.0
+bc = 5 wall
+bc = 4 slip-wall
+boussinesq = true
+beta = 0.01
+gravity = 0.0 -9.81 0.0
+tref = 0.5
+k_cond = 0.1
+cp = 1.0
+tbc = 5 2 3.0
+"""
+
+
+def mesh_arrays(m):
+    return dict(x=m.x, fptr=m.fptr, fnodes=m.fnodes, c0=m.c0, c1=m.c1, zone=m.zone,
+                ncells=np.int64(m.ncells), ctype=np.int64(m.ctype))
+
+
+def run(args, cwd):
+    r = subprocess.run([HARNESS] + args, cwd=cwd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(r.stdout[-2000:] + r.stderr[-2000:])
+    return r.stdout
+
+
+class Reader:
+    def __init__(self, path):
+        self.b = open(path, "rb").read()
+        self.o = 0
+
+    def take(self, dt, n):
+        a = np.frombuffer(self.b, dtype=dt, count=n, offset=self.o).copy()
+        self.o += a.nbytes
+        return a
+
+
+def channel_box(nx, ny, nz):
+    return meshgen.channel(nx, ny, nz, lx=4.0, lz=1.0, ratio=1.1)
+
+
+def seeded_fields(m, seed, umag=1.0):
+    rng = np.random.default_rng(seed)
+    nc = m.ncells
+    u = rng.uniform(-umag, umag, size=(nc, 3))
+    p = rng.normal(size=nc)
+    T = rng.uniform(0.0, 1.0, size=nc)
+    return u, p, T
+
+
+def write_fields(path, *arrs):
+    with open(path, "wb") as f:
+        for a in arrs:
+            np.ascontiguousarray(a).tofile(f)
+
+
+def case_run(name, m, zone_kind, ctl, mode, nsteps, fields=None):
+    with tempfile.TemporaryDirectory() as td:
+        cas.write_cas(os.path.join(td, "m.cas"), m, zone_kind)
+        open(os.path.join(td, "c.control"), "w").write(ctl)
+        inf = ""
+        if fields is not None:
+            inf = os.path.join(td, "in.bin")
+            write_fields(inf, *fields)
+        log = run(["m.cas", "c.control", mode, "out.bin", inf, str(nsteps)], td)
+        r = Reader(os.path.join(td, "out.bin"))
+        nc, nf = m.ncells, m.nfaces
+        ier = r.take(np.int32, 1)[0]
+        out = dict(ier=np.int64(ier), u=r.take(np.float64, 3 * nc).reshape(nc, 3), p=r.take(np.float64, nc),
+                   flux=r.take(np.float64, nf), T=r.take(np.float64, nc),
+                   gp=r.take(np.float64, 3 * nc).reshape(nc, 3), gu=r.take(np.float64, 9 * nc).reshape(nc, 9),
+                   apc=r.take(np.float64, nc), gt=r.take(np.float64, 3 * nc).reshape(nc, 3))
+        return out, log
+
+
+def geom_run(m, zone_kind, ctl):
+    with tempfile.TemporaryDirectory() as td:
+        cas.write_cas(os.path.join(td, "m.cas"), m, zone_kind)
+        open(os.path.join(td, "c.control"), "w").write(ctl)
+        run(["m.cas", "c.control", "geom", "out.bin"], td)
+        r = Reader(os.path.join(td, "out.bin"))
+        nc, nf, ncf = r.take(np.int32, 3)
+        g = dict(xf=r.take(np.float64, 3 * nf).reshape(nf, 3), sf=r.take(np.float64, 3 * nf).reshape(nf, 3),
+                 area=r.take(np.float64, nf), vol=r.take(np.float64, nc), xc=r.take(np.float64, 3 * nc).reshape(nc, 3),
+                 lf=r.take(np.float64, nf))
+        g["cf_ptr"] = r.take(np.int32, nc + 1)
+        g["cf"] = r.take(np.int32, ncf)
+        g["c2c_ptr"] = r.take(np.int32, nc + 1)
+        g["c2c"] = r.take(np.int32, int(g["c2c_ptr"][-1]))
+        nnz = r.take(np.int32, 1)[0]
+        g["row_ptr"] = r.take(np.int32, nc + 1)
+        g["col_idx"] = r.take(np.int32, nnz)
+        g["fgrp"] = r.take(np.int32, nf)
+        return g
+
+
+def funcs_run(m, zone_kind, ctl, seed, ts_order):
+    rng = np.random.default_rng(seed)
+    nc, nf = m.ncells, m.nfaces
+    u, p, T = seeded_fields(m, seed)
+    flux = rng.normal(size=nf) * 0.1
+    u_old = u + 0.05 * rng.normal(size=u.shape)
+    u_oo = u + 0.05 * rng.normal(size=u.shape)
+    T_old = T + 0.05 * rng.normal(size=nc)
+    T_oo = T + 0.05 * rng.normal(size=nc)
+    apc = rng.uniform(1.0, 3.0, size=nc)
+    pp = rng.normal(size=nc) * 0.01
+    inputs = dict(u=u, p=p, T=T, flux=flux, u_old=u_old, u_old_old=u_oo, T_old=T_old, T_old_old=T_oo,
+                  apc=apc, pp=pp, ts_order=np.int64(ts_order))
+    with tempfile.TemporaryDirectory() as td:
+        cas.write_cas(os.path.join(td, "m.cas"), m, zone_kind)
+        open(os.path.join(td, "c.control"), "w").write(ctl)
+        inf = os.path.join(td, "in.bin")
+        write_fields(inf, u, p, T, flux, u_old, u_oo, T_old, T_oo, apc, pp, np.array([ts_order], np.int32))
+        run(["m.cas", "c.control", "funcs", "out.bin", inf], td)
+        r = Reader(os.path.join(td, "out.bin"))
+        g = geom_run(m, zone_kind, ctl)
+        nnz = len(g["col_idx"])
+        out = dict(gp=r.take(np.float64, 3 * nc).reshape(nc, 3), gu=r.take(np.float64, 9 * nc).reshape(nc, 9),
+                   gt=r.take(np.float64, 3 * nc).reshape(nc, 3))
+        for comp in (1, 2, 3):
+            out[f"mom{comp}_val"] = r.take(np.float64, nnz)
+            out[f"mom{comp}_rhs"] = r.take(np.float64, nc)
+            out[f"mom{comp}_ap"] = r.take(np.float64, nc)
+        out["tem_val"] = r.take(np.float64, nnz)
+        out["tem_rhs"] = r.take(np.float64, nc)
+        out["tem_ap"] = r.take(np.float64, nc)
+        out["flux_rc"] = r.take(np.float64, nf)
+        out["flux_out"] = r.take(np.float64, nf)
+        out["cor_u"] = r.take(np.float64, 3 * nc).reshape(nc, 3)
+        out["cor_p"] = r.take(np.float64, nc)
+        out["cor_flux"] = r.take(np.float64, nf)
+        out["cor_dpp"] = r.take(np.float64, 3 * nc).reshape(nc, 3)
+    return inputs, out
+
+
+def solve_run(row_ptr, col_idx, val, b, x0, tol, kmax, maxit):
+    n, nnz = len(b), len(val)
+    with tempfile.TemporaryDirectory() as td:
+        inf = os.path.join(td, "in.bin")
+        write_fields(inf, np.array([n, nnz], np.int32), row_ptr.astype(np.int32), col_idx.astype(np.int32),
+                     val, b, x0, np.array([tol]), np.array([kmax], np.int32))
+        run(["-", "-", "solve", "out.bin", inf, str(maxit)], td)
+        r = Reader(os.path.join(td, "out.bin"))
+        out = {}
+        out["ic0_ierr"] = np.int64(r.take(np.int32, 1)[0])
+        out["ic0_L"] = r.take(np.float64, nnz)
+        out["ilu0_ierr"] = np.int64(r.take(np.int32, 1)[0])
+        out["ilu0_LU"] = r.take(np.float64, nnz)
+        for s in ("cg_jacobi", "cg_ic0", "bicgstab_ilu0"):
+            its, errs, res = [], [], []
+            for _ in range(kmax):
+                its.append(r.take(np.int32, 1)[0])
+                errs.append(r.take(np.int32, 1)[0])
+                res.append(r.take(np.float64, 1)[0])
+            out[f"{s}_hist_it"] = np.array(its)
+            out[f"{s}_hist_ierr"] = np.array(errs)
+            out[f"{s}_hist_res"] = np.array(res)
+            out[f"{s}_it"] = np.int64(r.take(np.int32, 1)[0])
+            out[f"{s}_ierr"] = np.int64(r.take(np.int32, 1)[0])
+            out[f"{s}_res"] = np.float64(r.take(np.float64, 1)[0])
+            out[f"{s}_x"] = r.take(np.float64, n)
+    return out
+
+
+def save(name, **kw):
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, **kw)
+    print("wrote", path, os.path.getsize(path), "bytes")
+
+
+def main():
+    if not os.path.exists(HARNESS):
+        sys.exit("build the reference harness first: oracle/build_ref.sh")
+
+    # 1. cavity, Jacobi PCG, orthogonal-corrected diffusion, first-order upwind
+    m = meshgen.cavity(16)
+    zk = {4: "symmetry", 5: "wall"}
+    ctl = CAVITY_CTL.format(blend=0.0, nonorth=1.0, precond="jacobi")
+    g = geom_run(m, zk, ctl)
+    out, log = case_run("cav16", m, zk, ctl, "simple", 5)
+    save("cavity16_simple_jacobi", control=ctl, nsteps=np.int64(5), **mesh_arrays(m),
+         **{"geom_" + k: v for k, v in g.items()}, **{"out_" + k: v for k, v in out.items()})
+
+    # 2. cavity, IC0 PCG + limited second-order upwind (the shipped cavity controls)
+    ctl = CAVITY_CTL.format(blend=1.0, nonorth=0.0, precond="ic0")
+    out, log = case_run("cav16b", m, zk, ctl, "simple", 4)
+    save("cavity16_simple_ic0_blend", control=ctl, nsteps=np.int64(4), **mesh_arrays(m),
+         **{"out_" + k: v for k, v in out.items()})
+
+    # 3. channel, transient PISO (BDF2), parabolic inlet + outflow, seeded field
+    mc = channel_box(12, 8, 4)
+    zkc = {4: "symmetry", 5: "wall", 6: "velocity-inlet", 7: "outflow"}
+    ctl = CHANNEL_CTL.format(mu=0.05, dt=0.02, nsteps=3, blend=0.0, precond="jacobi")
+    u0, p0, T0 = seeded_fields(mc, 180, umag=0.3)
+    u0[:, 0] += 1.0
+    out, log = case_run("chan", mc, zkc, ctl, "piso", 3, (u0, p0, T0))
+    gch = geom_run(mc, zkc, ctl)
+    save("channel_piso_jacobi", control=ctl, nsteps=np.int64(3), **mesh_arrays(mc),
+         in_u=u0, in_p=p0, in_T=T0, **{"geom_" + k: v for k, v in gch.items()},
+         **{"out_" + k: v for k, v in out.items()})
+
+    # 4. jittered prism O-grid, PISO with IC0, blended convection, far field
+    mp = meshgen.cylinder_prisms(ntheta=24, nr=6, r_out=4.0, layers=2, lz=0.2, jitter=0.15, seed=200)
+    zkp = {4: "symmetry", 5: "wall", 6: "pressure-far-field"}
+    u0, p0, T0 = seeded_fields(mp, 200, umag=0.2)
+    u0[:, 0] += 1.0
+    out, log = case_run("prism", mp, zkp, PRISM_CTL, "piso", 2, (u0, p0, T0))
+    gp = geom_run(mp, zkp, PRISM_CTL)
+    save("prism_piso_ic0", control=PRISM_CTL, nsteps=np.int64(2), **mesh_arrays(mp),
+         in_u=u0, in_p=p0, in_T=T0, **{"geom_" + k: v for k, v in gp.items()},
+         **{"out_" + k: v for k, v in out.items()})
+
+    # 5. per-function vectors on the jittered prism mesh and on the outlet box
+    for tag, mm, zz, cc, tso in (("prism", mp, zkp, PRISM_CTL, 2),
+                                 ("outlet", None, None, OUTLET_CTL, 0),
+                                 ("channel", mc, zkc, CHANNEL_CTL.format(mu=0.05, dt=0.02, nsteps=3,
+                                                                        blend=1.0, precond="jacobi"), 1)):
+        if mm is None:
+            mm = meshgen.hex_box(np.linspace(0, 2, 9), np.linspace(0, 1, 6), np.linspace(0, 0.5, 4),
+                                 zone_ids=dict(xmin=6, xmax=7, ymin=5, ymax=5, zmin=4, zmax=4),
+                                 jitter=0.2, seed=7)
+            zz = {4: "wall", 5: "wall", 6: "velocity-inlet", 7: "pressure-outlet"}
+            mo = mm
+        inputs, out = funcs_run(mm, zz, cc, seed=11, ts_order=tso)
+        save(f"funcs_{tag}", control=cc, **mesh_arrays(mm), **{"in_" + k: v for k, v in inputs.items()},
+             **{"out_" + k: v for k, v in out.items()})
+
+    # 6. pressure-outlet box, steady SIMPLE (Dirichlet-p rows, no pin, Boussinesq, heat flux)
+    u0, p0, T0 = seeded_fields(mo, 5, umag=0.1)
+    out, log = case_run("outlet", mo, {4: "wall", 5: "wall", 6: "velocity-inlet", 7: "pressure-outlet"},
+                        OUTLET_CTL, "simple", 3, (u0, p0, T0))
+    save("outlet_simple_jacobi", control=OUTLET_CTL, nsteps=np.int64(3), **mesh_arrays(mo),
+         in_u=u0, in_p=p0, in_T=T0, **{"out_" + k: v for k, v in out.items()})
+
+    # 7. linear solver histories on a pressure-correction-like SPD matrix
+    rp, ci = g["row_ptr"], g["col_idx"]
+    n = len(rp) - 1
+    rng = np.random.default_rng(3)
+    val = np.zeros(len(ci))
+    for i in range(n):
+        for k in range(rp[i] - 1, rp[i + 1] - 1):
+            j = ci[k] - 1
+            val[k] = -rng.uniform(0.5, 1.5) if j != i else 0.0
+    # symmetrise and make diagonally dominant
+    A = np.zeros((n, n))
+    for i in range(n):
+        for k in range(rp[i] - 1, rp[i + 1] - 1):
+            A[i, ci[k] - 1] = val[k]
+    A = 0.5 * (A + A.T)
+    np.fill_diagonal(A, -A.sum(axis=1) + 0.05)
+    for i in range(n):
+        for k in range(rp[i] - 1, rp[i + 1] - 1):
+            val[k] = A[i, ci[k] - 1]
+    # non-symmetric twin for BiCGSTAB (convective skew)
+    b = rng.normal(size=n)
+    x0 = np.zeros(n)
+    out = solve_run(rp, ci, val, b, x0, 1e-10, 25, 1000)
+    save("solvers_spd", row_ptr=rp, col_idx=ci, val=val, b=b, x0=x0, tol=np.float64(1e-10),
+         kmax=np.int64(25), maxit=np.int64(1000), **out)
+    valn = val.copy()
+    for i in range(n):
+        for k in range(rp[i] - 1, rp[i + 1] - 1):
+            if ci[k] - 1 > i:
+                valn[k] *= 1.3
+    x0 = rng.normal(size=n) * 0.1
+    out = solve_run(rp, ci, valn, b, x0, 1e-10, 25, 1000)
+    save("solvers_nonsym", row_ptr=rp, col_idx=ci, val=valn, b=b, x0=x0, tol=np.float64(1e-10),
+         kmax=np.int64(25), maxit=np.int64(1000), **out)
+
+
+if __name__ == "__main__":
+    main()
