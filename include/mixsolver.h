@@ -1,0 +1,151 @@
+/*
+ * mixsolver.h -- C ABI of libmixsolver_hip.so, the MI355X (gfx950) device
+ * implementation of the unstructured SIMPLE/PISO pressure-velocity path.
+ *
+ * Plain pointers and sizes only.  Arrays use the reference's own layouts:
+ * Fortran column-major fields (u(3,ncells) = xyz interleaved per cell),
+ * 1-based owner/neighbour ids (c1 = 0 on boundary faces), 1-based CSR
+ * (csr_t: row_ptr(n+1), col_idx(nnz) sorted per row, val(nnz)).
+ * All host arrays are owned by the caller; device memory is owned by the
+ * library and keyed by the opaque ms_system handle.
+ *
+ * Return value of every entry point: 0 on success, a negative MS_E* code on
+ * an API/device error (message via ms_last_error()).  Solver/driver status
+ * follows the reference's ierr/ier conventions in out-parameters.
+ * Calls on one handle must not overlap; different handles are independent.
+ */
+#ifndef MIXSOLVER_H
+#define MIXSOLVER_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MS_OK 0
+#define MS_EINVAL (-1)     /* bad argument / size / unsupported feature */
+#define MS_EDEVICE (-2)    /* HIP runtime error */
+#define MS_ENOMEM (-3)     /* device allocation failed / over HBM budget */
+
+/* BC type codes: mod_uns_control.f90:32-49 */
+enum { MS_BC_NONE = 0, MS_BC_WALL = 1, MS_BC_SYMMETRY = 2, MS_BC_VINLET = 3,
+       MS_BC_POUTLET = 4, MS_BC_FARFIELD = 5, MS_BC_SLIPWALL = 6,
+       MS_BC_MASSINLET = 7, MS_BC_OUTFLOW = 8, MS_BC_INTERFACE = 9,
+       MS_BC_VINLET_PARAB = 10 };
+
+/* preconditioner / precision selectors for the device-path keys */
+enum { MS_PC_JACOBI = 0, MS_PC_NATURAL = 1, MS_PC_DILU = 2 };
+enum { MS_FP64 = 0, MS_MIXED = 1 };
+
+/* Control values: the fields of ctrl_t used on the path
+ * (mod_uns_control.f90:148-280) plus device-path keys. */
+typedef struct {
+  double rho, mu, alpha_u, alpha_p;
+  double outer_tol, lin_tol, conv_blend, nonorth_corr;
+  double dt, cp, k_cond, tref, beta;
+  double gravity[3], body_force[3];
+  double init_T, t_pert;
+  int outer_max, lin_max, ppe_precond, transient;   /* ppe_precond: 1 = 'ic0', else jacobi */
+  int time_scheme, n_time_max, n_correct, n_out_every;
+  int pimple, n_outer_iter, boussinesq, thermal_model;
+  int inlet_ramp, nbc;
+  int gpu_ppe_precond;   /* MS_PC_*: p' preconditioner on the device        */
+  int gpu_mom_precond;   /* MS_PC_*: momentum / energy preconditioner        */
+  int gpu_precision;     /* MS_FP64 or MS_MIXED (FP32 inner, FP64 refinement) */
+  int gpu_ppe_solver;    /* 0 = PCG (reference), 1 = BiCGSTAB                */
+  double gpu_inner_tol;  /* relative tolerance of each FP32 inner solve      */
+  int gpu_reorder;       /* 0 = natural order, 1 = RCM                       */
+  int gpu_reserved;
+} ms_ctrl_t;
+
+/* One boundary group (bcgroup_t, mod_uns_bc.f90:80-110), control order. */
+typedef struct {
+  int zone, btype;
+  double uvel[3], pval, mdot, uspeed;
+  double prfl_mean, prfl_span, prfl_origin;
+  int prfl_axis, has_lid;
+  int lid_dir, ttype;
+  double lid_coord, lid_tol, lid_vel[3];
+  double tval, qval;
+  int ntbc_plane, tbc_pdir[4], tbc_pttype[4];
+  double tbc_pcoord[4], tbc_tol, tbc_ptval[4], tbc_pqval[4];
+} ms_bcgroup_t;
+
+typedef struct ms_system ms_system;
+
+/* ---- library ---------------------------------------------------------- */
+int ms_init(int device);                 /* select device, create streams   */
+const char *ms_last_error(void);
+int ms_version(void);
+
+/* ---- system = one mesh + fields + matrices resident on the device ------
+ * Replaces the setup half of uns_run_mpi / uns_solver_init
+ * (mod_uns_driver_mpi.f90:114-228, mod_uns_driver.f90:60-239):
+ * build_connectivity, compute_geometry, build_bc, init_fields,
+ * build_cell_csr -- from the arrays read_cas fills in mesh_t
+ * (mod_uns_mesh.f90:17-54).  fptr: face-node offsets (0-based, nfaces+1),
+ * fnodes: 1-based node ids, x: (3,nnodes). */
+int ms_system_create(ms_system **out, int nnodes, const double *x, int nfaces,
+                     const long long *fptr, const int *fnodes, const int *c0,
+                     const int *c1, const int *zone, int ncells,
+                     const ms_ctrl_t *ctrl, const ms_bcgroup_t *groups);
+int ms_system_destroy(ms_system *s);
+int ms_system_info(ms_system *s, long long *out8);  /* ncells nfaces nnz ncolors nlevels bytes_device reserved reserved */
+
+/* fields_t components by name: "u"(3,nc) "p" "T" "T_s" "flux"(nf) "apc"
+ * "u_old" "u_old_old" "T_old" "T_old_old" "gp"(3,nc) "gu"(3,3,nc) "gt"
+ * "porosity" "perm_dir"(3,nc) "inertial" "k_s" "rho_s" "cp_s";
+ * geometry (read-only): "xf" "sf" "area" "vol" "xc" "lf". */
+int ms_set_field(ms_system *s, const char *name, const double *host, long long n);
+int ms_get_field(ms_system *s, const char *name, double *host, long long n);
+int ms_set_ts_order(ms_system *s, int ts_order);
+int ms_get_topology(ms_system *s, int *row_ptr, int *col_idx, int *fgrp); /* 1-based CSR */
+
+/* ---- loop level: keep everything resident across iterations ----------- */
+/* simple_run (mod_uns_simple.f90:88) / simple_run_mpi
+ * (mod_uns_simple_mpi.f90:57): nsteps caps outer_max (0 = outer_max).
+ * ier: 0 ok, 20-23 breakdowns, 60 non-finite (reference codes). */
+int ms_simple_run(ms_system *s, int nsteps, int *it_done, int *ier);
+/* piso_run_mpi at one partition (mod_uns_simple_mpi.f90:298); nsteps
+ * overrides n_time_max when > 0.  ier: 30-36 / 60 as the reference. */
+int ms_piso_run(ms_system *s, int nsteps, int *ier);
+/* per-iteration/step diagnostics of the last driver call, 8 doubles per row:
+ * imbal du_max dT_max|umax cfl lin_it_max lin_res_mean ppe_it_total ppe_res_last */
+int ms_get_history(ms_system *s, double *out, int maxrows, int *nrows);
+
+/* ---- per-call level (copy in/out through ms_set/get_field) -------------
+ * val/rhs/ap are host arrays in the reference CSR order (nnz / ncells). */
+int ms_compute_gradients(ms_system *s);                       /* mod_uns_fields.f90:164 */
+int ms_momentum_assembly(ms_system *s, int comp, double *val, double *rhs, double *ap); /* mod_uns_simple.f90:928 */
+int ms_temperature_assembly(ms_system *s, double *val, double *rhs, double *ap); /* :1748 */
+int ms_flux_rhiechow(ms_system *s);                           /* :1413 */
+int ms_outflow_mass_rescale(ms_system *s);                    /* :1536 */
+int ms_ppe_assembly(ms_system *s, double *val, double *rhs);  /* :1552 / mod_uns_simple_mpi.f90:882 */
+int ms_correct_fields(ms_system *s, const double *pp, double *dpp); /* :1649 */
+
+/* ---- linear solvers on a caller CSR (csr_t) ----------------------------
+ * Same argument meaning as cg_jacobi / cg_ic0 / bicgstab_ilu0
+ * (mod_uns_linsolver.f90:230, :405, :148) and their _mpi twins at one
+ * partition (mod_uns_linsolver_mpi.f90:235, :306, :155): x is the initial
+ * guess on entry, it/res/ierr as the reference (ierr 0 converged, 1 maxit,
+ * 2 breakdown, 11/12 factorization failure). */
+int ms_cg_jacobi(int n, const int *row_ptr, const int *col_idx, const double *val,
+                 const double *b, double *x, double tol, int maxit,
+                 int *it, double *res, int *ierr);
+int ms_cg_ic0(int n, const int *row_ptr, const int *col_idx, const double *val,
+              const double *b, double *x, double tol, int maxit,
+              int *it, double *res, int *ierr);
+int ms_bicgstab_ilu0(int n, const int *row_ptr, const int *col_idx, const double *val,
+                     const double *b, double *x, double tol, int maxit,
+                     int *it, double *res, int *ierr);
+/* Extended solver entry (no reference counterpart: the north-star solver
+ * options).  solver 0 PCG / 1 BiCGSTAB; precond MS_PC_*; precision
+ * MS_FP64 / MS_MIXED; inner_tol for MS_MIXED.  res is the FP64 true
+ * relative residual ||b-Ax||/||b|| for MS_MIXED. */
+int ms_solve(int n, const int *row_ptr, const int *col_idx, const double *val,
+             const double *b, double *x, double tol, int maxit, int solver,
+             int precond, int precision, double inner_tol,
+             int *it, double *res, int *ierr);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,88 @@
+"""Loader for libmixsolver_hip.so (the HIP/CDNA4 product library).
+
+There is no CPU fallback: if the library is missing or cannot be loaded the
+import raises, naming what to build.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmixsolver_hip.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "mixsolver.h")
+
+_D = C.POINTER(C.c_double)
+_I = C.POINTER(C.c_int)
+_L = C.POINTER(C.c_longlong)
+_V = C.c_void_p
+_lib = None
+
+
+class MixSolverError(RuntimeError):
+    pass
+
+
+def header_symbols():
+    """Entry points declared in include/mixsolver.h."""
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(ms_[a-z0-9_]+)\s*\(", txt)))
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise MixSolverError(f"{LIB_PATH} not built: run python -m mixsolver_amd.build")
+    L = C.CDLL(LIB_PATH)
+    sig = {
+        "ms_init": (C.c_int, [C.c_int]),
+        "ms_last_error": (C.c_char_p, []),
+        "ms_version": (C.c_int, []),
+        "ms_system_create": (C.c_int, [C.POINTER(_V), C.c_int, _D, C.c_int, _L, _I, _I, _I, _I, C.c_int, _V, _V]),
+        "ms_system_destroy": (C.c_int, [_V]),
+        "ms_system_info": (C.c_int, [_V, _L]),
+        "ms_set_field": (C.c_int, [_V, C.c_char_p, _D, C.c_longlong]),
+        "ms_get_field": (C.c_int, [_V, C.c_char_p, _D, C.c_longlong]),
+        "ms_set_ts_order": (C.c_int, [_V, C.c_int]),
+        "ms_get_topology": (C.c_int, [_V, _I, _I, _I]),
+        "ms_simple_run": (C.c_int, [_V, C.c_int, _I, _I]),
+        "ms_piso_run": (C.c_int, [_V, C.c_int, _I]),
+        "ms_get_history": (C.c_int, [_V, _D, C.c_int, _I]),
+        "ms_compute_gradients": (C.c_int, [_V]),
+        "ms_momentum_assembly": (C.c_int, [_V, C.c_int, _D, _D, _D]),
+        "ms_temperature_assembly": (C.c_int, [_V, _D, _D, _D]),
+        "ms_flux_rhiechow": (C.c_int, [_V]),
+        "ms_outflow_mass_rescale": (C.c_int, [_V]),
+        "ms_ppe_assembly": (C.c_int, [_V, _D, _D]),
+        "ms_correct_fields": (C.c_int, [_V, _D, _D]),
+        "ms_cg_jacobi": (C.c_int, [C.c_int, _I, _I, _D, _D, _D, C.c_double, C.c_int, _I, _D, _I]),
+        "ms_cg_ic0": (C.c_int, [C.c_int, _I, _I, _D, _D, _D, C.c_double, C.c_int, _I, _D, _I]),
+        "ms_bicgstab_ilu0": (C.c_int, [C.c_int, _I, _I, _D, _D, _D, C.c_double, C.c_int, _I, _D, _I]),
+        "ms_solve": (C.c_int, [C.c_int, _I, _I, _D, _D, _D, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int,
+                               C.c_double, _I, _D, _I]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != 0:
+        msg = lib().ms_last_error()
+        raise MixSolverError(f"libmixsolver_hip error {rc}: {msg.decode() if msg else ''}")
+    return rc
+
+
+def dp(a):
+    return a.ctypes.data_as(_D)
+
+
+def ip(a):
+    return a.ctypes.data_as(_I)

@@ -1,0 +1,357 @@
+// ms_capi.cpp -- extern "C" entry points of libmixsolver_hip.so.
+// Every call converts internal errors into MS_E* return codes; fields cross
+// the boundary in the reference's natural cell order and Fortran layouts.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+
+#include "ms_system.hpp"
+
+namespace ms {
+void ensure_runtime(int device);
+const char *last_error();
+int run_simple(System &s, int nsteps, int *it_done);
+int run_piso(System &s, int nsteps);
+void call_gradients(System &s);
+void call_momentum(System &s, int comp);
+void call_energy(System &s);
+void call_rhiechow(System &s);
+void call_outflow(System &s);
+void call_ppe(System &s);
+void call_correct(System &s, const double *d_pp);
+}  // namespace ms
+
+using namespace ms;
+
+struct ms_system {
+  std::unique_ptr<System> s;
+};
+
+template <class F>
+static int guarded(F f) {
+  try {
+    f();
+    return MS_OK;
+  } catch (const Error &e) {
+    set_error(e.msg);
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    set_error("host out of memory");
+    return MS_ENOMEM;
+  } catch (...) {
+    set_error("unknown error");
+    return MS_EINVAL;
+  }
+}
+
+namespace {
+enum Kind { CELL, FACE, PROP };
+struct FieldRef {
+  DBuf<double> *buf;
+  int width;
+  Kind kind;
+  std::vector<double> *host;   // PROP: natural-order host copy
+  bool writable;
+};
+
+FieldRef lookup(System &s, const char *name) {
+  std::string n(name ? name : "");
+  if (n == "u") return {&s.u, 3, CELL, nullptr, true};
+  if (n == "p") return {&s.p, 1, CELL, nullptr, true};
+  if (n == "T") return {&s.T, 1, CELL, nullptr, true};
+  if (n == "T_s") return {&s.T_s, 1, CELL, nullptr, true};
+  if (n == "apc") return {&s.apc, 1, CELL, nullptr, true};
+  if (n == "u_old") return {&s.u_old, 3, CELL, nullptr, true};
+  if (n == "u_old_old") return {&s.u_old_old, 3, CELL, nullptr, true};
+  if (n == "T_old") return {&s.T_old, 1, CELL, nullptr, true};
+  if (n == "T_old_old") return {&s.T_old_old, 1, CELL, nullptr, true};
+  if (n == "gp") return {&s.gp, 3, CELL, nullptr, true};
+  if (n == "gu") return {&s.gu, 9, CELL, nullptr, true};
+  if (n == "gt") return {&s.gt, 3, CELL, nullptr, true};
+  if (n == "gts") return {&s.gts, 3, CELL, nullptr, true};
+  if (n == "flux") return {&s.flux, 1, FACE, nullptr, true};
+  if (n == "porosity") return {&s.porosity_d, 1, PROP, &s.porosity, true};
+  if (n == "perm_dir") return {&s.perm_dir_d, 3, PROP, &s.perm_dir, true};
+  if (n == "inertial") return {&s.inertial_d, 1, PROP, &s.inertial, true};
+  if (n == "k_s") return {&s.k_s_d, 1, PROP, &s.k_s, true};
+  if (n == "rho_s") return {&s.rho_s_d, 1, PROP, &s.rho_s, true};
+  if (n == "cp_s") return {&s.cp_s_d, 1, PROP, &s.cp_s, true};
+  fail(MS_EINVAL, "unknown field name: " + n);
+}
+
+void to_device_order(const System &s, const double *h, int w, std::vector<double> &out) {
+  out.resize((size_t)s.nc * w);
+  for (int i = 0; i < s.nc; i++)
+    for (int j = 0; j < w; j++) out[(size_t)s.o2n[i] * w + j] = h[(size_t)i * w + j];
+}
+void from_device_order(const System &s, const std::vector<double> &d, int w, double *h) {
+  for (int i = 0; i < s.nc; i++)
+    for (int j = 0; j < w; j++) h[(size_t)i * w + j] = d[(size_t)s.o2n[i] * w + j];
+}
+void download(const DBuf<double> &b, std::vector<double> &h, size_t n) {
+  h.resize(n);
+  b.download(h.data(), n);
+  MS_HIP(hipStreamSynchronize(stream()));
+}
+System &sys(ms_system *s) {
+  if (!s || !s->s) fail(MS_EINVAL, "null system handle");
+  return *s->s;
+}
+}  // namespace
+
+extern "C" {
+
+int ms_init(int device) { return guarded([&] { ensure_runtime(device); }); }
+const char *ms_last_error(void) { return last_error(); }
+int ms_version(void) { return 100; }
+
+int ms_system_create(ms_system **out, int nnodes, const double *x, int nfaces, const long long *fptr,
+                     const int *fnodes, const int *c0, const int *c1, const int *zone, int ncells,
+                     const ms_ctrl_t *ctrl, const ms_bcgroup_t *groups) {
+  return guarded([&] {
+    if (!out) fail(MS_EINVAL, "null out handle");
+    ensure_runtime(-1);
+    std::unique_ptr<ms_system> h(new ms_system());
+    h->s.reset(create_system(nnodes, x, nfaces, fptr, fnodes, c0, c1, zone, ncells, ctrl, groups));
+    *out = h.release();
+  });
+}
+
+int ms_system_destroy(ms_system *s) {
+  return guarded([&] {
+    if (!s) return;
+    MS_HIP(hipStreamSynchronize(stream()));
+    delete s;
+  });
+}
+
+int ms_system_info(ms_system *h, long long *o) {
+  return guarded([&] {
+    System &s = sys(h);
+    o[0] = s.nc;
+    o[1] = s.nf;
+    o[2] = s.pat.nnz;
+    o[3] = s.ncolors;
+    o[4] = s.pat.has_levels ? (long long)s.pat.lev_ptr_f.size() - 1 : 0;
+    o[5] = (long long)s.device_bytes();
+    o[6] = s.pat.nstore;
+    o[7] = s.renumbered;
+  });
+}
+
+int ms_set_field(ms_system *h, const char *name, const double *host, long long n) {
+  return guarded([&] {
+    System &s = sys(h);
+    FieldRef f = lookup(s, name);
+    if (!host) fail(MS_EINVAL, "null host array");
+    if (f.kind == FACE) {
+      if (n != s.nf) fail(MS_EINVAL, "face field size mismatch");
+      f.buf->upload(host, n);
+    } else {
+      if (n != (long long)s.nc * f.width) fail(MS_EINVAL, "cell field size mismatch");
+      std::vector<double> d;
+      to_device_order(s, host, f.width, d);
+      f.buf->upload(d);
+      if (f.kind == PROP) {
+        f.host->assign(host, host + n);
+        s.props_dirty = true;
+      }
+      std::string nm(name);
+      if (nm == "T" || nm == "T_old" || nm == "T_old_old" || nm == "T_s")
+        for (long long i = 0; i < n; i++)
+          if (host[i] != 0.0) {
+            s.T_nonzero = true;
+            break;
+          }
+    }
+    MS_HIP(hipStreamSynchronize(stream()));
+  });
+}
+
+int ms_get_field(ms_system *h, const char *name, double *host, long long n) {
+  return guarded([&] {
+    System &s = sys(h);
+    if (!host) fail(MS_EINVAL, "null host array");
+    std::string nm(name ? name : "");
+    // geometry (natural order, read-only)
+    auto copy = [&](const std::vector<double> &v) {
+      if ((size_t)n != v.size()) fail(MS_EINVAL, "geometry size mismatch");
+      std::memcpy(host, v.data(), v.size() * sizeof(double));
+    };
+    if (nm == "xf") return copy(s.xf);
+    if (nm == "sf") return copy(s.sf);
+    if (nm == "area") return copy(s.area);
+    if (nm == "lf") return copy(s.lf);
+    if (nm == "vol") return copy(s.vol_nat);
+    if (nm == "xc") return copy(s.xc_nat);
+    FieldRef f = lookup(s, name);
+    std::vector<double> d;
+    if (f.kind == FACE) {
+      if (n != s.nf) fail(MS_EINVAL, "face field size mismatch");
+      download(*f.buf, d, s.nf);
+      std::memcpy(host, d.data(), s.nf * sizeof(double));
+    } else {
+      if (n != (long long)s.nc * f.width) fail(MS_EINVAL, "cell field size mismatch");
+      download(*f.buf, d, (size_t)s.nc * f.width);
+      from_device_order(s, d, f.width, host);
+    }
+  });
+}
+
+int ms_set_ts_order(ms_system *h, int t) {
+  return guarded([&] {
+    if (t < 0 || t > 2) fail(MS_EINVAL, "ts_order must be 0, 1 or 2");
+    sys(h).ts_order = t;
+  });
+}
+
+int ms_get_topology(ms_system *h, int *row_ptr, int *col_idx, int *fgrp) {
+  return guarded([&] {
+    System &s = sys(h);
+    if (row_ptr) std::copy(s.ref_row_ptr.begin(), s.ref_row_ptr.end(), row_ptr);
+    if (col_idx) std::copy(s.ref_col.begin(), s.ref_col.end(), col_idx);
+    if (fgrp) std::copy(s.fgrp.begin(), s.fgrp.end(), fgrp);
+  });
+}
+
+int ms_simple_run(ms_system *h, int nsteps, int *it_done, int *ier) {
+  return guarded([&] {
+    int r = run_simple(sys(h), nsteps, it_done);
+    if (ier) *ier = r;
+  });
+}
+
+int ms_piso_run(ms_system *h, int nsteps, int *ier) {
+  return guarded([&] {
+    int r = run_piso(sys(h), nsteps);
+    if (ier) *ier = r;
+  });
+}
+
+int ms_get_history(ms_system *h, double *out, int maxrows, int *nrows) {
+  return guarded([&] {
+    System &s = sys(h);
+    int rows = (int)(s.history.size() / 8);
+    if (nrows) *nrows = rows;
+    if (out) std::memcpy(out, s.history.data(), std::min(rows, maxrows) * 8 * sizeof(double));
+  });
+}
+
+// ---- per-call level --------------------------------------------------------
+static void matrix_out(System &s, double *val, double *rhs, double *ap) {
+  std::vector<double> d;
+  if (val) {
+    download(s.val, d, s.pat.nstore);
+    for (size_t k = 0; k < s.ref2dev.size(); k++) val[k] = d[s.ref2dev[k]];
+  }
+  if (rhs) {
+    download(s.rhs, d, s.nc);
+    from_device_order(s, d, 1, rhs);
+  }
+  if (ap) {
+    download(s.ap, d, s.nc);
+    from_device_order(s, d, 1, ap);
+  }
+}
+
+int ms_compute_gradients(ms_system *h) { return guarded([&] { call_gradients(sys(h)); }); }
+int ms_momentum_assembly(ms_system *h, int comp, double *val, double *rhs, double *ap) {
+  return guarded([&] {
+    if (comp < 1 || comp > 3) fail(MS_EINVAL, "comp must be 1..3");
+    call_momentum(sys(h), comp - 1);
+    matrix_out(sys(h), val, rhs, ap);
+  });
+}
+int ms_temperature_assembly(ms_system *h, double *val, double *rhs, double *ap) {
+  return guarded([&] {
+    call_energy(sys(h));
+    matrix_out(sys(h), val, rhs, ap);
+  });
+}
+int ms_flux_rhiechow(ms_system *h) { return guarded([&] { call_rhiechow(sys(h)); }); }
+int ms_outflow_mass_rescale(ms_system *h) { return guarded([&] { call_outflow(sys(h)); }); }
+int ms_ppe_assembly(ms_system *h, double *val, double *rhs) {
+  return guarded([&] {
+    call_ppe(sys(h));
+    matrix_out(sys(h), val, rhs, nullptr);
+  });
+}
+int ms_correct_fields(ms_system *h, const double *pp, double *dpp) {
+  return guarded([&] {
+    System &s = sys(h);
+    std::vector<double> d;
+    to_device_order(s, pp, 1, d);
+    s.pp.upload(d);
+    call_correct(s, s.pp.p);
+    if (dpp) {
+      download(s.dpp, d, 3 * (size_t)s.nc);
+      from_device_order(s, d, 3, dpp);
+    }
+  });
+}
+
+// ---- solvers on a caller CSR ---------------------------------------------------
+static void solve_csr(int n, const int *row_ptr, const int *col_idx, const double *val, const double *b, double *x,
+                      double tol, int maxit, int solver, int precond, int precision, double inner_tol, int *it,
+                      double *res, int *ierr) {
+  if (n <= 0 || !row_ptr || !col_idx || !val || !b || !x) fail(MS_EINVAL, "bad solver arguments");
+  ensure_runtime(-1);
+  std::vector<int> rp(n + 1), ci(row_ptr[n] - 1);
+  for (int i = 0; i <= n; i++) rp[i] = row_ptr[i] - 1;
+  for (size_t k = 0; k < ci.size(); k++) {
+    ci[k] = col_idx[k] - 1;
+    if (ci[k] < 0 || ci[k] >= n) fail(MS_EINVAL, "column index out of range");
+  }
+  for (int i = 0; i < n; i++)
+    for (int k = rp[i] + 1; k < rp[i + 1]; k++)
+      if (ci[k] <= ci[k - 1]) fail(MS_EINVAL, "CSR columns must be strictly ascending per row");
+  SellPattern pat;
+  csr_to_sell(pat, n, rp, ci);
+  if (precond == MS_PC_DILU) {
+    // colour classes in natural order (rows renumbering is the caller's choice)
+    fail(MS_EINVAL, "DILU needs a colour-blocked ordering; use a system with gpu_ppe_precond = dilu");
+  }
+  pat.upload();
+  if (precond == MS_PC_NATURAL) pat.build_levels();
+  std::vector<double> vs(pat.nstore, 0.0);
+  for (int r = 0; r < n; r++)
+    for (int k = 0; k < pat.rowlen[r]; k++) vs[pat.slot(r, k)] = val[rp[r] + k];
+  DBuf<double> dv, db, dx;
+  dv.upload(vs);
+  db.upload(b, n);
+  dx.upload(x, n);
+  Krylov K(&pat);
+  PcKind pc = precond == MS_PC_JACOBI ? PC_JACOBI : (solver == 0 ? PC_NATURAL_IC0 : PC_NATURAL_ILU0);
+  SolveResult R;
+  if (precision == MS_MIXED) R = K.mixed(dv.p, db.p, dx.p, tol, maxit, pc, solver, inner_tol);
+  else if (solver == 0) R = K.pcg(dv.p, db.p, dx.p, tol, maxit, pc);
+  else R = K.bicgstab(dv.p, db.p, dx.p, tol, maxit, pc);
+  dx.download(x, n);
+  MS_HIP(hipStreamSynchronize(stream()));
+  if (it) *it = R.it;
+  if (res) *res = R.res;
+  if (ierr) *ierr = R.ierr;
+}
+
+int ms_cg_jacobi(int n, const int *rp, const int *ci, const double *v, const double *b, double *x, double tol,
+                 int maxit, int *it, double *res, int *ierr) {
+  return guarded([&] { solve_csr(n, rp, ci, v, b, x, tol, maxit, 0, MS_PC_JACOBI, MS_FP64, 0.0, it, res, ierr); });
+}
+int ms_cg_ic0(int n, const int *rp, const int *ci, const double *v, const double *b, double *x, double tol,
+              int maxit, int *it, double *res, int *ierr) {
+  return guarded([&] { solve_csr(n, rp, ci, v, b, x, tol, maxit, 0, MS_PC_NATURAL, MS_FP64, 0.0, it, res, ierr); });
+}
+int ms_bicgstab_ilu0(int n, const int *rp, const int *ci, const double *v, const double *b, double *x, double tol,
+                     int maxit, int *it, double *res, int *ierr) {
+  return guarded([&] { solve_csr(n, rp, ci, v, b, x, tol, maxit, 1, MS_PC_NATURAL, MS_FP64, 0.0, it, res, ierr); });
+}
+int ms_solve(int n, const int *rp, const int *ci, const double *v, const double *b, double *x, double tol, int maxit,
+             int solver, int precond, int precision, double inner_tol, int *it, double *res, int *ierr) {
+  return guarded([&] {
+    if (solver < 0 || solver > 1 || precision < 0 || precision > 1) fail(MS_EINVAL, "bad solver selector");
+    solve_csr(n, rp, ci, v, b, x, tol, maxit, solver, precond, precision, inner_tol, it, res, ierr);
+  });
+}
+
+}  // extern "C"

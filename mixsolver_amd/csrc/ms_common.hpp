@@ -1,0 +1,115 @@
+// ms_common.hpp -- shared declarations of libmixsolver_hip (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/mixsolver.h"
+
+namespace ms {
+
+// ----------------------------------------------------------------- errors
+struct Error {
+  int code;
+  std::string msg;
+};
+void set_error(const std::string &m);
+[[noreturn]] void fail(int code, const std::string &m);
+
+#define MS_HIP(call)                                                            \
+  do {                                                                          \
+    hipError_t e_ = (call);                                                     \
+    if (e_ != hipSuccess)                                                       \
+      ::ms::fail(MS_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define MS_CHECK_LAUNCH() MS_HIP(hipGetLastError())
+
+hipStream_t stream();           // library stream (default priority)
+
+// --------------------------------------------------------- device buffers
+template <class T>
+struct DBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  explicit DBuf(size_t count) { alloc(count); }
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+  DBuf(DBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DBuf &operator=(DBuf &&o) noexcept {
+    if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+    return *this;
+  }
+  ~DBuf() { release(); }
+  void alloc(size_t count) {
+    release();
+    n = count;
+    if (count) {
+      hipError_t e = hipMalloc(&p, count * sizeof(T));
+      if (e != hipSuccess) { p = nullptr; n = 0; fail(MS_ENOMEM, "hipMalloc failed"); }
+    }
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void upload(const T *h, size_t count) {
+    if (count > n) alloc(count);
+    if (count) MS_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, stream()));
+  }
+  void upload(const std::vector<T> &v) { upload(v.data(), v.size()); }
+  void download(T *h, size_t count) const {
+    if (count) MS_HIP(hipMemcpyAsync(h, p, count * sizeof(T), hipMemcpyDeviceToHost, stream()));
+  }
+  void zero() {
+    if (n) MS_HIP(hipMemsetAsync(p, 0, n * sizeof(T), stream()));
+  }
+  size_t bytes() const { return n * sizeof(T); }
+};
+
+constexpr int kBlock = 256;   // 4 wave64 per workgroup
+constexpr int kSlice = 64;    // SELL slice height = one wavefront
+
+inline int nblocks(long long n, int b = kBlock) { return (int)((n + b - 1) / b); }
+
+// ------------------------------------------------- SELL-64 sparse pattern
+// Row r lives in slice s = r/64, lane l = r%64; its k-th entry (columns in
+// ascending order, as csr_t) sits at off[s] + k*64 + l.  Padding entries use
+// col = r, val = 0.  diagk[r] = k of the diagonal entry.
+struct SellPattern {
+  int n = 0, nslices = 0;
+  long long nnz = 0, nstore = 0;
+  std::vector<long long> off;       // nslices+1
+  std::vector<int> rowlen, col, diagk;
+  std::vector<long long> tpos;       // stored slot of (c,r) for each stored (r,c); -1 pad
+  DBuf<long long> d_off, d_tpos;
+  DBuf<int> d_rowlen, d_col, d_diagk;
+  // level schedules (natural-order triangular sweeps / factorizations)
+  std::vector<int> lev_ptr_f, lev_ptr_b;   // rows grouped per level
+  DBuf<int> d_lev_rows_f, d_lev_rows_b;
+  // colour classes: rows of colour c are [col_ptr[c], col_ptr[c+1])
+  std::vector<int> color_ptr;
+  bool has_levels = false;
+  bool symmetric = true;             // structurally symmetric (tpos complete)
+  long long slot(int r, int k) const { return off[r / kSlice] + (long long)k * kSlice + (r % kSlice); }
+  void build_from_csr(int n, const std::vector<int> &rp0, const std::vector<int> &ci0); // 0-based CSR
+  void build_levels();
+  void upload();
+  size_t device_bytes() const;
+};
+
+// --------------------------------------------------- Krylov solver state
+struct KState {
+  double bnrm, rz, pap, alpha, beta, res, rho, rho_old, omega, snrm, tt, rr;
+  double rnorm, tol, scale, spare;
+  int it, ierr, done, maxit;
+  unsigned int ticket;
+  int early, inner_it, outer_it;
+};
+
+}  // namespace ms

@@ -1,0 +1,787 @@
+// ms_setup.cpp -- host-side construction of a device system.
+//
+// Restates the setup chain of the reference for one partition:
+//   compute_geometry  (mod_uns_geometry.f90:40-174; Newell area vector,
+//                      fan-triangulated face centroid, pyramid volumes and
+//                      centroids about the node-average point, c0->c1
+//                      orientation)
+//   build_connectivity(mod_uns_connectivity.f90:28-108)
+//   build_cell_csr    (mod_uns_simple.f90:835-904)
+//   build_bc          (mod_uns_bc.f90:155-265)
+//   init_fields lf    (mod_uns_fields.f90:147-157)
+// with the same evaluation order, so every geometric coefficient handed to
+// the device equals the reference's bit for bit.  On top of that it builds
+// the device layout: optional colour-blocked / RCM renumbering of cells,
+// the SELL-64 matrix pattern, per-cell face lists with matrix slot maps,
+// level schedules for natural-order factorizations and per-face constants.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <queue>
+
+#include "ms_system.hpp"
+
+namespace ms {
+
+// ------------------------------------------------------------- runtime glue
+static thread_local std::string g_err;
+static hipStream_t g_stream = nullptr;
+static int g_device = -1;
+
+void set_error(const std::string &m) { g_err = m; }
+void fail(int code, const std::string &m) { throw Error{code, m}; }
+const char *last_error() { return g_err.c_str(); }
+
+void ensure_runtime(int device) {
+  if (g_stream && (device < 0 || device == g_device)) return;
+  int ndev = 0;
+  MS_HIP(hipGetDeviceCount(&ndev));
+  if (ndev <= 0) fail(MS_EDEVICE, "no HIP device visible");
+  if (device < 0) device = 0;
+  if (device >= ndev) fail(MS_EINVAL, "device index out of range");
+  MS_HIP(hipSetDevice(device));
+  if (g_stream) MS_HIP(hipStreamDestroy(g_stream));
+  MS_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+  g_device = device;
+}
+
+hipStream_t stream() {
+  if (!g_stream) ensure_runtime(-1);
+  return g_stream;
+}
+
+// ------------------------------------------------------------------- norms
+double norm2_ref(const double *v, int n) {
+  double mx = 0.0, s = 0.0;
+  for (int i = 0; i < n; i++) {
+    double ax = std::fabs(v[i]);
+    if (mx == 0.0) {
+      mx = ax;
+    } else if (ax > mx) {
+      double t = mx / ax;
+      double tsq = t * t;
+      s = s * tsq;
+      s = s + tsq;
+      mx = ax;
+    } else {
+      double t = ax / mx;
+      s = s + t * t;
+    }
+  }
+  return mx * std::sqrt(1.0 + s);
+}
+
+static inline double dot3(const double *a, const double *b) {
+  double s = 0.0;
+  s = s + a[0] * b[0];
+  s = s + a[1] * b[1];
+  s = s + a[2] * b[2];
+  return s;
+}
+
+// -------------------------------------------------------------- SELL pattern
+void csr_to_sell(SellPattern &P, int n, const std::vector<int> &rp, const std::vector<int> &ci) {
+  P.n = n;
+  P.nslices = (n + kSlice - 1) / kSlice;
+  P.nnz = rp[n];
+  P.rowlen.assign(n, 0);
+  P.diagk.assign(n, -1);
+  P.off.assign(P.nslices + 1, 0);
+  for (int r = 0; r < n; r++) P.rowlen[r] = rp[r + 1] - rp[r];
+  for (int s = 0; s < P.nslices; s++) {
+    int w = 0;
+    for (int r = s * kSlice; r < std::min(n, (s + 1) * kSlice); r++) w = std::max(w, P.rowlen[r]);
+    P.off[s + 1] = P.off[s] + (long long)w * kSlice;
+  }
+  P.nstore = P.off[P.nslices];
+  P.col.assign(P.nstore, 0);
+  for (int s = 0; s < P.nslices; s++) {   // padding: col = row itself
+    long long w = (P.off[s + 1] - P.off[s]) / kSlice;
+    for (long long k = 0; k < w; k++)
+      for (int l = 0; l < kSlice; l++) {
+        int r = s * kSlice + l;
+        P.col[P.off[s] + k * kSlice + l] = r < n ? r : 0;
+      }
+  }
+  for (int r = 0; r < n; r++)
+    for (int k = 0; k < P.rowlen[r]; k++) {
+      int c = ci[rp[r] + k];
+      P.col[P.slot(r, k)] = c;
+      if (c == r) P.diagk[r] = k;
+    }
+  for (int r = 0; r < n; r++)
+    if (P.diagk[r] < 0) fail(MS_EINVAL, "matrix row without diagonal entry");
+  // transpose slot map (pattern is structurally symmetric)
+  P.tpos.assign(P.nstore, -1);
+  P.symmetric = true;
+  for (int r = 0; r < n; r++)
+    for (int k = 0; k < P.rowlen[r]; k++) {
+      int c = P.col[P.slot(r, k)];
+      const int *b = &ci[rp[c]], *e = &ci[rp[c + 1]];
+      const int *f = std::lower_bound(b, e, r);
+      if (f == e || *f != r) {
+        P.symmetric = false;
+        continue;
+      }
+      P.tpos[P.slot(r, k)] = P.slot(c, (int)(f - b));
+    }
+  P.color_ptr = {0, n};
+}
+
+void SellPattern::build_levels() {
+  // forward: level(i) = 1 + max level of lower neighbours; backward on upper
+  std::vector<int> lf(n, 0), lb(n, 0);
+  int nlf = 0, nlb = 0;
+  for (int r = 0; r < n; r++) {
+    int l = 0;
+    for (int k = 0; k < diagk[r]; k++) l = std::max(l, lf[col[slot(r, k)]] + 1);
+    lf[r] = l;
+    nlf = std::max(nlf, l + 1);
+  }
+  for (int r = n - 1; r >= 0; r--) {
+    int l = 0;
+    for (int k = diagk[r] + 1; k < rowlen[r]; k++) l = std::max(l, lb[col[slot(r, k)]] + 1);
+    lb[r] = l;
+    nlb = std::max(nlb, l + 1);
+  }
+  auto group = [&](const std::vector<int> &lv, int nl, std::vector<int> &ptr, std::vector<int> &rows) {
+    ptr.assign(nl + 1, 0);
+    for (int r = 0; r < n; r++) ptr[lv[r] + 1]++;
+    for (int l = 0; l < nl; l++) ptr[l + 1] += ptr[l];
+    rows.assign(n, 0);
+    std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+    for (int r = 0; r < n; r++) rows[fill[lv[r]]++] = r;
+  };
+  std::vector<int> rf, rb;
+  group(lf, nlf, lev_ptr_f, rf);
+  group(lb, nlb, lev_ptr_b, rb);
+  d_lev_rows_f.upload(rf);
+  d_lev_rows_b.upload(rb);
+  has_levels = true;
+}
+
+void SellPattern::upload() {
+  d_off.upload(off);
+  d_rowlen.upload(rowlen);
+  d_col.upload(col);
+  d_diagk.upload(diagk);
+  d_tpos.upload(tpos);
+}
+
+size_t SellPattern::device_bytes() const {
+  return d_off.bytes() + d_rowlen.bytes() + d_col.bytes() + d_diagk.bytes() + d_tpos.bytes() +
+         d_lev_rows_f.bytes() + d_lev_rows_b.bytes();
+}
+
+// ------------------------------------------------------------ orderings
+// Reverse Cuthill-McKee on the cell graph, starting from a minimum-degree
+// cell of each component; ties broken by natural index (deterministic).
+static std::vector<int> rcm_order(int n, const std::vector<int> &ptr, const std::vector<int> &adj) {
+  std::vector<int> deg(n), order;
+  order.reserve(n);
+  for (int i = 0; i < n; i++) deg[i] = ptr[i + 1] - ptr[i];
+  std::vector<char> seen(n, 0);
+  std::vector<int> byDeg(n);
+  std::iota(byDeg.begin(), byDeg.end(), 0);
+  std::stable_sort(byDeg.begin(), byDeg.end(), [&](int a, int b) { return deg[a] < deg[b]; });
+  std::vector<int> nb;
+  for (int start : byDeg) {
+    if (seen[start]) continue;
+    std::queue<int> q;
+    q.push(start);
+    seen[start] = 1;
+    while (!q.empty()) {
+      int v = q.front();
+      q.pop();
+      order.push_back(v);
+      nb.clear();
+      for (int k = ptr[v]; k < ptr[v + 1]; k++)
+        if (!seen[adj[k]]) nb.push_back(adj[k]);
+      std::sort(nb.begin(), nb.end(), [&](int a, int b) { return deg[a] != deg[b] ? deg[a] < deg[b] : a < b; });
+      nb.erase(std::unique(nb.begin(), nb.end()), nb.end());
+      for (int w : nb) {
+        seen[w] = 1;
+        q.push(w);
+      }
+    }
+  }
+  std::reverse(order.begin(), order.end());
+  return order;   // order[new] = old
+}
+
+// Greedy colouring visiting cells in `visit` order; returns colour per cell.
+static std::vector<int> greedy_colors(int n, const std::vector<int> &ptr, const std::vector<int> &adj,
+                                      const std::vector<int> &visit, int &ncol) {
+  std::vector<int> color(n, -1);
+  std::vector<int> mark(64, -1);
+  ncol = 0;
+  for (int v : visit) {
+    for (int k = ptr[v]; k < ptr[v + 1]; k++) {
+      int c = color[adj[k]];
+      if (c >= 0) {
+        if (c >= (int)mark.size()) mark.resize(c + 1, -1);
+        mark[c] = v;
+      }
+    }
+    int c = 0;
+    while (c < (int)mark.size() && mark[c] == v) c++;
+    if (c >= (int)mark.size()) mark.resize(c + 1, -1);
+    color[v] = c;
+    ncol = std::max(ncol, c + 1);
+  }
+  return color;
+}
+
+// ---------------------------------------------------------------- system
+System *create_system(int nnodes, const double *x, int nfaces, const long long *fptr, const int *fnodes,
+                      const int *c0in, const int *c1in, const int *zone, int ncells, const ms_ctrl_t *ctrl,
+                      const ms_bcgroup_t *groups) {
+  if (nnodes <= 0 || nfaces <= 0 || ncells <= 0 || !x || !fptr || !fnodes || !c0in || !c1in || !zone || !ctrl)
+    fail(MS_EINVAL, "ms_system_create: empty mesh or null argument");
+  if (ctrl->nbc <= 0 || ctrl->nbc > 32 || !groups) fail(MS_EINVAL, "ms_system_create: need 1..32 bc groups");
+  std::unique_ptr<System> S(new System());
+  System &s = *S;
+  s.nc = ncells;
+  s.nf = nfaces;
+  s.nn = nnodes;
+  s.nbc = ctrl->nbc;
+  s.ctrl = *ctrl;
+  s.groups.assign(groups, groups + ctrl->nbc);
+  const int nc = ncells, nf = nfaces;
+  for (int f = 0; f < nf; f++) {
+    if (c0in[f] < 1 || c0in[f] > nc || c1in[f] < 0 || c1in[f] > nc) fail(MS_EINVAL, "face with invalid cell id");
+    for (long long j = fptr[f]; j < fptr[f + 1]; j++)
+      if (fnodes[j] < 1 || fnodes[j] > nnodes) fail(MS_EINVAL, "face with invalid node id");
+    if (fptr[f + 1] - fptr[f] < 3) fail(MS_EINVAL, "face with fewer than 3 nodes");
+  }
+  if (ctrl->thermal_model != 0) fail(MS_EINVAL, "thermal_model = ltne is not supported on the device path");
+  for (auto &g : s.groups)
+    if (g.btype == MS_BC_INTERFACE || g.btype < 1 || g.btype > 10)
+      fail(MS_EINVAL, "unsupported boundary type (coupling interface)");
+
+  // ---- natural-order geometry: compute_geometry -------------------------
+  s.xf.assign(3 * (size_t)nf, 0.0);
+  s.sf.assign(3 * (size_t)nf, 0.0);
+  s.area.assign(nf, 0.0);
+  s.vol_nat.assign(nc, 0.0);
+  s.xc_nat.assign(3 * (size_t)nc, 0.0);
+  auto X = [&](int node, int d) { return x[3 * (size_t)(node - 1) + d]; };
+  for (int f = 0; f < nf; f++) {
+    int m = (int)(fptr[f + 1] - fptr[f]);
+    const int *nd = &fnodes[fptr[f]];
+    double sv[3] = {0, 0, 0}, cen[3] = {0, 0, 0}, at = 0.0;
+    for (int j = 0; j < m; j++) {
+      int jn = (j + 1) % m;
+      double a[3] = {X(nd[j], 0), X(nd[j], 1), X(nd[j], 2)}, b[3] = {X(nd[jn], 0), X(nd[jn], 1), X(nd[jn], 2)};
+      double c[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+      for (int d = 0; d < 3; d++) sv[d] = sv[d] + 0.5 * c[d];
+    }
+    double ar = norm2_ref(sv, 3);
+    for (int j = 1; j < m - 1; j++) {
+      double e1[3], e2[3], st[3], p1[3], p2[3], p3[3];
+      for (int d = 0; d < 3; d++) {
+        p1[d] = X(nd[0], d);
+        p2[d] = X(nd[j], d);
+        p3[d] = X(nd[j + 1], d);
+        e1[d] = p2[d] - p1[d];
+        e2[d] = p3[d] - p1[d];
+      }
+      st[0] = e1[1] * e2[2] - e1[2] * e2[1];
+      st[1] = e1[2] * e2[0] - e1[0] * e2[2];
+      st[2] = e1[0] * e2[1] - e1[1] * e2[0];
+      for (int d = 0; d < 3; d++) st[d] = st[d] * 0.5;
+      double ns = norm2_ref(st, 3);
+      for (int d = 0; d < 3; d++) cen[d] = cen[d] + ns * (p1[d] + p2[d] + p3[d]) / 3.0;
+      at = at + ns;
+    }
+    if (at > 0.0)
+      for (int d = 0; d < 3; d++) cen[d] = cen[d] / at;
+    for (int d = 0; d < 3; d++) {
+      s.xf[3 * f + d] = cen[d];
+      s.sf[3 * f + d] = sv[d];
+    }
+    s.area[f] = ar;
+  }
+  {
+    std::vector<double> pcen(3 * (size_t)nc, 0.0);
+    std::vector<int> cnt(nc, 0);
+    for (int f = 0; f < nf; f++) {
+      int a = c0in[f] - 1, b = c1in[f] - 1;
+      for (long long j = fptr[f]; j < fptr[f + 1]; j++) {
+        for (int d = 0; d < 3; d++) pcen[3 * a + d] = pcen[3 * a + d] + X(fnodes[j], d);
+        cnt[a]++;
+        if (b >= 0) {
+          for (int d = 0; d < 3; d++) pcen[3 * b + d] = pcen[3 * b + d] + X(fnodes[j], d);
+          cnt[b]++;
+        }
+      }
+    }
+    for (int c = 0; c < nc; c++)
+      if (cnt[c] > 0)
+        for (int d = 0; d < 3; d++) pcen[3 * c + d] = pcen[3 * c + d] / (double)cnt[c];
+    for (int f = 0; f < nf; f++) {
+      double r[3];
+      int a = c0in[f] - 1;
+      for (int d = 0; d < 3; d++) r[d] = s.xf[3 * f + d] - pcen[3 * a + d];
+      if (dot3(&s.sf[3 * f], r) < 0.0)
+        for (int d = 0; d < 3; d++) s.sf[3 * f + d] = -s.sf[3 * f + d];
+    }
+    const double THIRD = 1.0 / 3.0, QRTR = 1.0 / 4.0;
+    for (int f = 0; f < nf; f++)
+      for (int side = 0; side < 2; side++) {
+        int cell = (side == 0 ? c0in[f] : c1in[f]) - 1;
+        if (cell < 0) continue;
+        const double *pc = &pcen[3 * cell];
+        double r[3];
+        for (int d = 0; d < 3; d++) r[d] = s.xf[3 * f + d] - pc[d];
+        double dv = THIRD * dot3(&s.sf[3 * f], r) * (side == 0 ? 1.0 : -1.0);
+        s.vol_nat[cell] = s.vol_nat[cell] + dv;
+        for (int d = 0; d < 3; d++)
+          s.xc_nat[3 * cell + d] = s.xc_nat[3 * cell + d] + dv * ((1.0 - QRTR) * s.xf[3 * f + d] + QRTR * pc[d]);
+      }
+    for (int c = 0; c < nc; c++) {
+      if (s.vol_nat[c] > 0.0)
+        for (int d = 0; d < 3; d++) s.xc_nat[3 * c + d] = s.xc_nat[3 * c + d] / s.vol_nat[c];
+      else
+        fail(MS_EINVAL, "non-positive cell volume");
+    }
+  }
+  // lf, mod_uns_fields.f90:147-157
+  s.lf.assign(nf, 1.0);
+  for (int f = 0; f < nf; f++) {
+    if (c1in[f] == 0) continue;
+    double r0[3], r1[3];
+    for (int d = 0; d < 3; d++) {
+      r0[d] = s.xf[3 * f + d] - s.xc_nat[3 * (c0in[f] - 1) + d];
+      r1[d] = s.xc_nat[3 * (c1in[f] - 1) + d] - s.xf[3 * f + d];
+    }
+    double d0 = norm2_ref(r0, 3), d1 = norm2_ref(r1, 3);
+    s.lf[f] = d1 / (d0 + d1);
+  }
+
+  // ---- bc groups: build_bc -------------------------------------------------
+  s.fgrp.assign(nf, 0);
+  {
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int k = 0; k < nnodes; k++)
+      for (int d = 0; d < 3; d++) {
+        lo[d] = std::min(lo[d], x[3 * (size_t)k + d]);
+        hi[d] = std::max(hi[d], x[3 * (size_t)k + d]);
+      }
+    double cr = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
+    for (int i = 0; i < s.nbc; i++) {
+      ms_bcgroup_t &g = s.groups[i];
+      if (g.btype == MS_BC_MASSINLET) g.uspeed = g.mdot / ctrl->rho;
+      if (g.has_lid) g.lid_tol = 1.0e-4 * cr;
+      if (g.ntbc_plane > 0) g.tbc_tol = 1.0e-4 * cr;
+      if (g.btype == MS_BC_VINLET_PARAB && (g.prfl_span <= 0.0 || g.prfl_axis < 1 || g.prfl_axis > 3))
+        fail(MS_EINVAL, "velocity-inlet-parabolic needs span > 0 and axis 1|2|3");
+      int nb = 0;
+      for (int f = 0; f < nf; f++)
+        if (c1in[f] == 0 && zone[f] == g.zone) {
+          nb++;
+          s.fgrp[f] = i + 1;
+        }
+      if (nb == 0) fail(MS_EINVAL, "bc zone has no boundary faces");
+    }
+    for (int f = 0; f < nf; f++)
+      if (c1in[f] == 0 && s.fgrp[f] == 0) fail(MS_EINVAL, "boundary face without bc specification");
+  }
+
+  // ---- natural connectivity and reference CSR ---------------------------
+  std::vector<int> cfp(nc + 1, 0), cfn, c2p(nc + 1, 0), c2c;
+  for (int f = 0; f < nf; f++) {
+    cfp[c0in[f]]++;
+    if (c1in[f] > 0) cfp[c1in[f]]++;
+  }
+  for (int c = 0; c < nc; c++) cfp[c + 1] += cfp[c];
+  cfn.resize(cfp[nc]);
+  {
+    std::vector<int> fill(cfp.begin(), cfp.end() - 1);
+    for (int f = 0; f < nf; f++) {
+      cfn[fill[c0in[f] - 1]++] = f;
+      if (c1in[f] > 0) cfn[fill[c1in[f] - 1]++] = f;
+    }
+  }
+  for (int f = 0; f < nf; f++)
+    if (c1in[f] > 0) {
+      c2p[c0in[f]]++;
+      c2p[c1in[f]]++;
+    }
+  for (int c = 0; c < nc; c++) c2p[c + 1] += c2p[c];
+  c2c.resize(c2p[nc]);
+  {
+    std::vector<int> fill(c2p.begin(), c2p.end() - 1);
+    for (int f = 0; f < nf; f++)
+      if (c1in[f] > 0) {
+        c2c[fill[c0in[f] - 1]++] = c1in[f] - 1;
+        c2c[fill[c1in[f] - 1]++] = c0in[f] - 1;
+      }
+  }
+  s.ref_row_ptr.assign(nc + 1, 1);
+  s.ref_col.clear();
+  {
+    std::vector<int> row;
+    for (int c = 0; c < nc; c++) {
+      row.assign(c2c.begin() + c2p[c], c2c.begin() + c2p[c + 1]);
+      row.push_back(c);
+      std::sort(row.begin(), row.end());
+      row.erase(std::unique(row.begin(), row.end()), row.end());
+      for (int v : row) s.ref_col.push_back(v + 1);
+      s.ref_row_ptr[c + 1] = s.ref_row_ptr[c] + (int)row.size();
+    }
+  }
+
+  // ---- device numbering -----------------------------------------------------
+  const int ppc = ctrl->gpu_ppe_precond, mpc = ctrl->gpu_mom_precond;
+  const bool want_color = ppc == MS_PC_DILU || mpc == MS_PC_DILU;
+  const bool want_rcm = ctrl->gpu_reorder == 1;
+  std::vector<int> visit(nc);
+  std::iota(visit.begin(), visit.end(), 0);
+  if (want_rcm) visit = rcm_order(nc, c2p, c2c);
+  std::vector<int> color(nc, 0);
+  s.ncolors = 1;
+  if (want_color) color = greedy_colors(nc, c2p, c2c, visit, s.ncolors);
+  s.n2o = visit;
+  if (want_color) {
+    std::vector<int> pos(nc);
+    for (int i = 0; i < nc; i++) pos[visit[i]] = i;
+    std::stable_sort(s.n2o.begin(), s.n2o.end(), [&](int a, int b) {
+      return color[a] != color[b] ? color[a] < color[b] : pos[a] < pos[b];
+    });
+  }
+  s.o2n.assign(nc, 0);
+  for (int i = 0; i < nc; i++) s.o2n[s.n2o[i]] = i;
+  s.renumbered = want_color || want_rcm;
+
+  // device-numbered topology
+  s.c0.resize(nf);
+  s.c1.resize(nf);
+  for (int f = 0; f < nf; f++) {
+    s.c0[f] = s.o2n[c0in[f] - 1];
+    s.c1[f] = c1in[f] > 0 ? s.o2n[c1in[f] - 1] : -1;
+  }
+  std::vector<int> rp(nc + 1, 0), ci;
+  {
+    std::vector<int> row;
+    for (int r = 0; r < nc; r++) {
+      int o = s.n2o[r];
+      row.clear();
+      for (int k = c2p[o]; k < c2p[o + 1]; k++) row.push_back(s.o2n[c2c[k]]);
+      row.push_back(r);
+      std::sort(row.begin(), row.end());
+      row.erase(std::unique(row.begin(), row.end()), row.end());
+      ci.insert(ci.end(), row.begin(), row.end());
+      rp[r + 1] = (int)ci.size();
+    }
+  }
+  csr_to_sell(s.pat, nc, rp, ci);
+  if (want_color) {
+    s.pat.color_ptr.assign(s.ncolors + 1, 0);
+    for (int r = 0; r < nc; r++) s.pat.color_ptr[color[s.n2o[r]] + 1]++;
+    for (int c = 0; c < s.ncolors; c++) s.pat.color_ptr[c + 1] += s.pat.color_ptr[c];
+  }
+  auto find_slot = [&](int r, int c) -> long long {
+    const int *b = &ci[rp[r]], *e = &ci[rp[r + 1]];
+    const int *f = std::lower_bound(b, e, c);
+    return s.pat.slot(r, (int)(f - b));
+  };
+  s.cf_ptr.assign(nc + 1, 0);
+  s.cf_ent.clear();
+  s.cf_slot.clear();
+  s.diag_slot.assign(nc, 0);
+  for (int r = 0; r < nc; r++) {
+    int o = s.n2o[r];
+    for (int k = cfp[o]; k < cfp[o + 1]; k++) {
+      int f = cfn[k];
+      int side = (c0in[f] - 1 == o) ? 0 : 1;
+      s.cf_ent.push_back(2 * f + side);
+      if (c1in[f] > 0) {
+        int other = side == 0 ? s.c1[f] : s.c0[f];
+        s.cf_slot.push_back(find_slot(r, other));
+      } else {
+        s.cf_slot.push_back(-1);
+      }
+    }
+    s.cf_ptr[r + 1] = (int)s.cf_ent.size();
+    s.diag_slot[r] = s.pat.slot(r, s.pat.diagk[r]);
+  }
+  s.ref2dev.resize(s.ref_col.size());
+  for (int i = 0; i < nc; i++)
+    for (int k = s.ref_row_ptr[i] - 1; k < s.ref_row_ptr[i + 1] - 1; k++)
+      s.ref2dev[k] = find_slot(s.o2n[i], s.o2n[s.ref_col[k] - 1]);
+  s.pat.upload();
+  if (ppc == MS_PC_NATURAL || mpc == MS_PC_NATURAL) s.pat.build_levels();
+
+  // pin cell and pressure-Dirichlet flag (ppe_assembly :1612-1642)
+  s.pin_cell = s.o2n[0];
+  s.has_pdir = 0;
+  s.has_outflow = 0;
+  for (int f = 0; f < nf; f++)
+    if (s.fgrp[f] > 0) {
+      int bt = s.groups[s.fgrp[f] - 1].btype;
+      if (bt == MS_BC_POUTLET || bt == MS_BC_FARFIELD) s.has_pdir = 1;
+      if (bt == MS_BC_OUTFLOW) s.has_outflow = 1;
+    }
+  // flux_ref (simple_run :118-130)
+  {
+    double usc = 0.0, ab = 0.0;
+    for (int i = 0; i < s.nbc; i++) {
+      const ms_bcgroup_t &g = s.groups[i];
+      usc = std::max(usc, std::max(norm2_ref(g.lid_vel, 3), std::max(norm2_ref(g.uvel, 3), g.uspeed)));
+      for (int f = 0; f < nf; f++)
+        if (s.fgrp[f] == i + 1) ab = ab + s.area[f];
+    }
+    if (usc <= 0.0) usc = 1.0;
+    if (ab <= 0.0) ab = 1.0;
+    s.flux_ref = ctrl->rho * usc * ab;
+  }
+
+  // ---- upload mesh data ---------------------------------------------------
+  std::vector<double> vol(nc), xc(3 * (size_t)nc);
+  for (int r = 0; r < nc; r++) {
+    vol[r] = s.vol_nat[s.n2o[r]];
+    for (int d = 0; d < 3; d++) xc[3 * r + d] = s.xc_nat[3 * (size_t)s.n2o[r] + d];
+  }
+  s.d_c0.upload(s.c0);
+  s.d_c1.upload(s.c1);
+  s.d_fgrp.upload(s.fgrp);
+  s.d_cf_ptr.upload(s.cf_ptr);
+  s.d_cf_ent.upload(s.cf_ent);
+  s.d_cf_slot.upload(s.cf_slot);
+  s.d_diag_slot.upload(s.diag_slot);
+  s.d_xf.upload(s.xf);
+  s.d_sf.upload(s.sf);
+  s.d_area.upload(s.area);
+  s.d_lf.upload(s.lf);
+  s.d_vol.upload(vol);
+  s.d_xc.upload(xc);
+  std::vector<DevBC> dbc(s.nbc);
+  for (int i = 0; i < s.nbc; i++) {
+    const ms_bcgroup_t &g = s.groups[i];
+    DevBC &b = dbc[i];
+    b.btype = g.btype;
+    b.prfl_axis = g.prfl_axis;
+    b.has_lid = g.has_lid;
+    b.lid_dir = g.lid_dir;
+    for (int d = 0; d < 3; d++) {
+      b.uvel[d] = g.uvel[d];
+      b.lid_vel[d] = g.lid_vel[d];
+    }
+    b.pval = g.pval;
+    b.uspeed = g.uspeed;
+    b.prfl_mean = g.prfl_mean;
+    b.prfl_span = g.prfl_span;
+    b.prfl_origin = g.prfl_origin;
+    b.lid_coord = g.lid_coord;
+    b.lid_tol = g.lid_tol;
+  }
+  s.d_bc.upload(dbc);
+
+  // ---- fields: init_fields (mod_uns_fields.f90:86-159) ---------------------
+  const size_t n3 = 3 * (size_t)nc;
+  for (DBuf<double> *b : {&s.u, &s.gp, &s.u_old, &s.u_old_old, &s.gt, &s.gts, &s.dpp, &s.perm_dir_d})
+    b->alloc(n3);
+  for (DBuf<double> *b : {&s.p, &s.T, &s.T_s, &s.apc, &s.T_old, &s.T_old_old, &s.T_s_old, &s.T_s_old_old,
+                          &s.porosity_d, &s.inertial_d, &s.k_s_d, &s.rho_s_d, &s.cp_s_d, &s.rhs, &s.ap, &s.pp,
+                          &s.ucomp, &s.diag8, &s.tmpc})
+    b->alloc(nc);
+  s.gu.alloc(9 * (size_t)nc);
+  s.psi.alloc(4 * (size_t)nc);
+  s.flux.alloc(nf);
+  s.af.alloc(nf);
+  s.val.alloc(s.pat.nstore);
+  s.part.alloc(8 * (size_t)nblocks(std::max(nc, nf)) + 64);
+  s.ticket.alloc(4);
+  for (DBuf<double> *b : {&s.u, &s.gp, &s.u_old, &s.u_old_old, &s.gt, &s.gts, &s.dpp, &s.perm_dir_d, &s.p, &s.T,
+                          &s.T_s, &s.T_old, &s.T_old_old, &s.T_s_old, &s.T_s_old_old, &s.inertial_d, &s.k_s_d,
+                          &s.rho_s_d, &s.cp_s_d, &s.rhs, &s.ap, &s.pp, &s.gu, &s.psi, &s.flux, &s.af, &s.val,
+                          &s.ucomp, &s.diag8, &s.tmpc})
+    b->zero();
+  s.ticket.zero();
+  std::vector<double> ones(nc, 1.0);
+  s.apc.upload(ones);
+  s.porosity_d.upload(ones);
+  s.porosity.assign(nc, 1.0);
+  s.perm_dir.assign(n3, 0.0);
+  s.inertial.assign(nc, 0.0);
+  s.k_s.assign(nc, 0.0);
+  s.rho_s.assign(nc, 0.0);
+  s.cp_s.assign(nc, 0.0);
+  s.props_dirty = true;
+  s.refresh_face_constants();
+  s.kry.reset(new Krylov(&s.pat));
+  MS_HIP(hipStreamSynchronize(stream()));
+  return S.release();
+}
+
+// Per-face constants that depend only on geometry, mu, k_cond and the cell
+// properties, evaluated exactly as the reference's face loops do.
+void System::refresh_face_constants() {
+  if (!props_dirty) return;
+  const double POROSITY_FLOOR = 1.0e-8;
+  const ms_ctrl_t &c = ctrl;
+  std::vector<double> dn(nf, 0.0), dvs(nf, 0.0), Dm(nf, 0.0), muf(nf, 0.0), tcor(3 * (size_t)nf, 0.0),
+      dw(nf, 0.0), Dmb(nf, 0.0), Dt(nf, 0.0), kfv(nf, 0.0), Dtb(nf, 0.0), tv(nf, 0.0), qv(nf, 0.0);
+  std::vector<signed char> tm(nf, 0);
+  energy_active = T_nonzero;
+  for (int f = 0; f < nf; f++) {
+    const int a = n2o[c0[f]];
+    const double *sfp = &sf[3 * (size_t)f];
+    if (c1[f] >= 0) {
+      const int b = n2o[c1[f]];
+      double dvec[3];
+      for (int d = 0; d < 3; d++) dvec[d] = xc_nat[3 * (size_t)b + d] - xc_nat[3 * (size_t)a + d];
+      dn[f] = norm2_ref(dvec, 3);
+      dvs[f] = dot3(dvec, sfp);
+      const double sd = dot3(sfp, dvec);
+      const double mu0 = c.mu / std::max(porosity[a], POROSITY_FLOOR);
+      const double mu1 = c.mu / std::max(porosity[b], POROSITY_FLOOR);
+      const double l = lf[f];
+      muf[f] = l * mu0 + (1.0 - l) * mu1;
+      const double k0 = porosity[a] * c.k_cond + (1.0 - porosity[a]) * k_s[a];
+      const double k1 = porosity[b] * c.k_cond + (1.0 - porosity[b]) * k_s[b];
+      double kf = l * k0 + (1.0 - l) * k1;
+      kf = kf + (l * 0.0 + (1.0 - l) * 0.0);
+      kfv[f] = kf;
+      if (c.nonorth_corr > 0.0) {
+        Dm[f] = muf[f] * (area[f] * area[f]) / sd;
+        Dt[f] = kf * (area[f] * area[f]) / sd;
+        for (int d = 0; d < 3; d++) tcor[3 * (size_t)f + d] = sfp[d] - dvec[d] * ((area[f] * area[f]) / sd);
+      } else {
+        Dm[f] = muf[f] * area[f] / dn[f];
+        Dt[f] = kf * area[f] / dn[f];
+      }
+    } else {
+      double r[3];
+      for (int d = 0; d < 3; d++) r[d] = xf[3 * (size_t)f + d] - xc_nat[3 * (size_t)a + d];
+      dw[f] = norm2_ref(r, 3);
+      Dmb[f] = (c.mu / std::max(porosity[a], POROSITY_FLOOR)) * area[f] / dw[f];
+      double kf = porosity[a] * c.k_cond + (1.0 - porosity[a]) * k_s[a];
+      kf = kf + 0.0;
+      Dtb[f] = kf * area[f] / dw[f];
+      // bc_face_T (mod_uns_bc.f90:560-703): static per face
+      const ms_bcgroup_t &g = groups[fgrp[f] - 1];
+      int tt = 0;
+      double tvv = 0.0, qvv = 0.0;
+      if (g.btype == MS_BC_WALL || g.btype == MS_BC_POUTLET || g.btype == MS_BC_OUTFLOW) {
+        tt = g.ttype;
+        tvv = g.tval;
+        qvv = g.qval;
+        for (int k = 0; k < g.ntbc_plane; k++)
+          if (std::fabs(xf[3 * (size_t)f + g.tbc_pdir[k] - 1] - g.tbc_pcoord[k]) <= g.tbc_tol) {
+            tt = g.tbc_pttype[k];
+            tvv = g.tbc_ptval[k];
+            qvv = g.tbc_pqval[k];
+            break;
+          }
+        if (tt != 1 && tt != 2) tt = 0;
+      } else if (g.btype == MS_BC_VINLET || g.btype == MS_BC_VINLET_PARAB || g.btype == MS_BC_MASSINLET) {
+        tt = 1;
+        tvv = g.tval;
+      }
+      tm[f] = (signed char)tt;
+      tv[f] = tt == 1 ? tvv : 0.0;
+      qv[f] = tt == 2 ? qvv : 0.0;
+      if ((tt == 1 && tvv != 0.0) || (tt == 2 && qvv != 0.0)) energy_active = true;
+    }
+  }
+  d_dn.upload(dn);
+  d_dvs.upload(dvs);
+  d_Dm.upload(Dm);
+  d_muf.upload(muf);
+  d_tcor.upload(tcor);
+  d_dw.upload(dw);
+  d_Dmb.upload(Dmb);
+  d_Dt.upload(Dt);
+  d_kf.upload(kfv);
+  d_Dtb.upload(Dtb);
+  d_tmode.upload(tm);
+  d_tval.upload(tv);
+  d_qval.upload(qv);
+  props_dirty = false;
+}
+
+DMesh System::dmesh() const {
+  DMesh m{};
+  m.nc = nc;
+  m.nf = nf;
+  m.nbc = nbc;
+  m.pin_cell = pin_cell;
+  m.has_pdir = has_pdir;
+  m.nonorth = ctrl.nonorth_corr > 0.0;
+  m.blend_on = ctrl.conv_blend > 0.0;
+  m.c0 = d_c0.p;
+  m.c1 = d_c1.p;
+  m.fgrp = d_fgrp.p;
+  m.xf = d_xf.p;
+  m.sf = d_sf.p;
+  m.area = d_area.p;
+  m.lf = d_lf.p;
+  m.vol = d_vol.p;
+  m.xc = d_xc.p;
+  m.cf_ptr = d_cf_ptr.p;
+  m.cf_ent = d_cf_ent.p;
+  m.cf_slot = d_cf_slot.p;
+  m.diag_slot = d_diag_slot.p;
+  m.dn = d_dn.p;
+  m.dvs = d_dvs.p;
+  m.Dm = d_Dm.p;
+  m.muf = d_muf.p;
+  m.tcor = d_tcor.p;
+  m.dw = d_dw.p;
+  m.Dmb = d_Dmb.p;
+  m.Dt = d_Dt.p;
+  m.kf = d_kf.p;
+  m.Dtb = d_Dtb.p;
+  m.tmode = d_tmode.p;
+  m.tval = d_tval.p;
+  m.qval = d_qval.p;
+  m.bc = d_bc.p;
+  return m;
+}
+
+DFields System::dfields() {
+  DFields F{};
+  F.u = u.p;
+  F.p = p.p;
+  F.T = T.p;
+  F.T_s = T_s.p;
+  F.flux = flux.p;
+  F.apc = apc.p;
+  F.u_old = u_old.p;
+  F.u_old_old = u_old_old.p;
+  F.T_old = T_old.p;
+  F.T_old_old = T_old_old.p;
+  F.T_s_old = T_s_old.p;
+  F.T_s_old_old = T_s_old_old.p;
+  F.gp = gp.p;
+  F.gu = gu.p;
+  F.gt = gt.p;
+  F.gts = gts.p;
+  F.psi = psi.p;
+  F.porosity = porosity_d.p;
+  F.perm_dir = perm_dir_d.p;
+  F.inertial = inertial_d.p;
+  F.k_s = k_s_d.p;
+  F.rho_s = rho_s_d.p;
+  F.cp_s = cp_s_d.p;
+  return F;
+}
+
+size_t System::device_bytes() const {
+  size_t b = pat.device_bytes();
+  for (const DBuf<double> *v : {&d_xf, &d_sf, &d_area, &d_lf, &d_vol, &d_xc, &d_dn, &d_dvs, &d_Dm, &d_muf, &d_tcor,
+                                &d_dw, &d_Dmb, &d_Dt, &d_kf, &d_Dtb, &d_tval, &d_qval, &u, &p, &T, &T_s, &flux, &apc,
+                                &u_old, &u_old_old, &T_old, &T_old_old, &T_s_old, &T_s_old_old, &gp, &gu, &gt, &gts,
+                                &psi, &porosity_d, &perm_dir_d, &inertial_d, &k_s_d, &rho_s_d, &cp_s_d, &val, &rhs,
+                                &ap, &pp, &dpp, &af, &ucomp, &part, &diag8, &tmpc})
+    b += v->bytes();
+  b += d_c0.bytes() + d_c1.bytes() + d_fgrp.bytes() + d_cf_ptr.bytes() + d_cf_ent.bytes() + d_cf_slot.bytes() +
+       d_diag_slot.bytes();
+  return b;
+}
+
+}  // namespace ms

@@ -1,0 +1,98 @@
+// ms_system.hpp -- one mesh + fields + matrices resident on the device.
+#pragma once
+#include <memory>
+
+#include "ms_common.hpp"
+#include "ms_linalg.hpp"
+
+namespace ms {
+
+// Boundary group as the kernels see it (subset of ms_bcgroup_t).
+struct DevBC {
+  int btype, prfl_axis, has_lid, lid_dir;
+  double uvel[3], pval, uspeed, prfl_mean, prfl_span, prfl_origin, lid_coord, lid_tol, lid_vel[3];
+};
+
+// Device views passed by value to the physics kernels.  Cell ids are in
+// the device numbering (0-based); face order is the mesh's face order.
+struct DMesh {
+  int nc, nf, nbc, pin_cell, has_pdir, nonorth, blend_on;
+  const int *c0, *c1, *fgrp;               // c1 < 0 on boundary faces
+  const double *xf, *sf, *area, *lf, *vol, *xc;
+  const int *cf_ptr, *cf_ent;              // cell -> faces ascending, ent = 2*face + side
+  const long long *cf_slot, *diag_slot;    // matrix slot of (cell, other) / (cell, cell)
+  const double *dn, *dvs, *Dm, *muf, *tcor, *dw, *Dmb;   // momentum / PPE face constants
+  const double *Dt, *kf, *Dtb;                           // energy face constants
+  const signed char *tmode;                // boundary thermal mode 0 neumann0 1 dirichlet 2 flux
+  const double *tval, *qval;
+  const DevBC *bc;
+};
+
+struct DFields {
+  double *u, *p, *T, *T_s, *flux, *apc, *u_old, *u_old_old, *T_old, *T_old_old, *T_s_old, *T_s_old_old;
+  double *gp, *gu, *gt, *gts, *psi;
+  const double *porosity, *perm_dir, *inertial, *k_s, *rho_s, *cp_s;
+};
+
+struct System {
+  // sizes and control
+  int nc = 0, nf = 0, nn = 0, nbc = 0;
+  ms_ctrl_t ctrl{};
+  std::vector<ms_bcgroup_t> groups;
+  // numbering: dev = o2n[natural], natural = n2o[dev]
+  std::vector<int> o2n, n2o;
+  bool renumbered = false;
+  int ncolors = 0;
+  // host topology in device numbering
+  std::vector<int> c0, c1, fgrp, cf_ptr, cf_ent;
+  std::vector<long long> cf_slot, diag_slot;
+  // natural-order geometry (face arrays: face order; cell arrays: natural order)
+  std::vector<double> xf, sf, area, lf, vol_nat, xc_nat;
+  // reference CSR (1-based, natural) and its slot map to the device matrix
+  std::vector<int> ref_row_ptr, ref_col;
+  std::vector<long long> ref2dev;
+  SellPattern pat;
+  std::unique_ptr<Krylov> kry;
+  // per-cell properties (natural order, host copy)
+  std::vector<double> porosity, perm_dir, inertial, k_s, rho_s, cp_s;
+  bool props_dirty = true;
+  bool energy_active = false;
+  bool T_nonzero = false;      // caller seeded a non-zero temperature field
+  int ts_order = 0;
+  double inlet_ramp = 1.0, pval_ramp = 1.0;
+  double flux_ref = 1.0;
+  // device mesh data
+  DBuf<int> d_c0, d_c1, d_fgrp, d_cf_ptr, d_cf_ent;
+  DBuf<long long> d_cf_slot, d_diag_slot;
+  DBuf<double> d_xf, d_sf, d_area, d_lf, d_vol, d_xc;
+  DBuf<double> d_dn, d_dvs, d_Dm, d_muf, d_tcor, d_dw, d_Dmb, d_Dt, d_kf, d_Dtb, d_tval, d_qval;
+  DBuf<signed char> d_tmode;
+  DBuf<DevBC> d_bc;
+  // device fields (device numbering)
+  DBuf<double> u, p, T, T_s, flux, apc, u_old, u_old_old, T_old, T_old_old, T_s_old, T_s_old_old;
+  DBuf<double> gp, gu, gt, gts, psi;
+  DBuf<double> porosity_d, perm_dir_d, inertial_d, k_s_d, rho_s_d, cp_s_d;
+  // work
+  DBuf<double> val, rhs, ap, pp, dpp, af, ucomp, part, diag8, tmpc;
+  DBuf<unsigned int> ticket;
+  std::vector<double> history;  // 8 per row
+  int pin_cell = 0, has_pdir = 0, has_outflow = 0;
+
+  DMesh dmesh() const;
+  DFields dfields();
+  void refresh_face_constants();
+  size_t device_bytes() const;
+};
+
+System *create_system(int nnodes, const double *x, int nfaces, const long long *fptr, const int *fnodes,
+                      const int *c0, const int *c1, const int *zone, int ncells, const ms_ctrl_t *ctrl,
+                      const ms_bcgroup_t *groups);
+
+// flang runtime NORM2 evaluation order (max-scaled accumulator), used for
+// every geometric length so the device coefficients equal the reference's.
+double norm2_ref(const double *v, int n);
+
+// build a SellPattern from a 0-based CSR (columns sorted per row)
+void csr_to_sell(SellPattern &pat, int n, const std::vector<int> &rp, const std::vector<int> &ci);
+
+}  // namespace ms

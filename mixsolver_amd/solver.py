@@ -1,0 +1,152 @@
+"""Host-side mirror of the reference's unstructured SIMPLE/PISO interface.
+
+``Case`` owns one device system (mesh, fields, matrices resident on the
+MI355X) and exposes the reference's routines under their names:
+
+  simple_run(nsteps)        mod_uns_simple.f90:88        (serial steady driver)
+  piso_run(nsteps)          mod_uns_simple_mpi.f90:298   (transient driver, np=1)
+  compute_gradients()       mod_uns_fields.f90:164
+  momentum_assembly(comp)   mod_uns_simple.f90:928
+  temperature_assembly()    mod_uns_simple.f90:1748
+  flux_rhiechow()           mod_uns_simple.f90:1413
+  outflow_mass_rescale()    mod_uns_simple.f90:1536
+  ppe_assembly()            mod_uns_simple.f90:1552 / mod_uns_simple_mpi.f90:882
+  correct_fields(pp)        mod_uns_simple.f90:1649
+
+Fields are exchanged in the reference's natural cell order and Fortran
+layouts (u(3,ncells) interleaved).  Driver status codes (ier) are the
+reference's: 20-23 SIMPLE breakdowns, 30-36 PISO, 60 non-finite.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, dp, ip
+from .control import groups_array, parse_control
+
+_WIDTH = dict(u=3, gp=3, gu=9, gt=3, gts=3, u_old=3, u_old_old=3, perm_dir=3, xf=3, sf=3, xc=3)
+
+
+class Case:
+    def __init__(self, mesh, ctrl, groups, device=0):
+        L = _lib.lib()
+        check(L.ms_init(device))
+        self.mesh = mesh
+        self.nc, self.nf = mesh.ncells, mesh.nfaces
+        self.ctrl = ctrl
+        self._garr = groups_array(groups)
+        h = C.c_void_p()
+        x = np.ascontiguousarray(mesh.x, np.float64)
+        fptr = np.ascontiguousarray(mesh.fptr, np.int64)
+        check(L.ms_system_create(C.byref(h), mesh.nnodes, dp(x), self.nf,
+                                 fptr.ctypes.data_as(C.POINTER(C.c_longlong)),
+                                 ip(np.ascontiguousarray(mesh.fnodes, np.int32)),
+                                 ip(np.ascontiguousarray(mesh.c0, np.int32)),
+                                 ip(np.ascontiguousarray(mesh.c1, np.int32)),
+                                 ip(np.ascontiguousarray(mesh.zone, np.int32)),
+                                 self.nc, C.byref(ctrl), self._garr))
+        self.h = h
+        info = np.zeros(8, np.int64)
+        check(L.ms_system_info(h, info.ctypes.data_as(C.POINTER(C.c_longlong))))
+        self.nnz = int(info[2])
+        self.info = dict(ncells=int(info[0]), nfaces=int(info[1]), nnz=int(info[2]), ncolors=int(info[3]),
+                         nlevels=int(info[4]), device_bytes=int(info[5]), nstore=int(info[6]),
+                         renumbered=bool(info[7]))
+
+    @classmethod
+    def from_control(cls, mesh, control_text, device=0, **overrides):
+        ctrl, groups = parse_control(control_text, warn=False)
+        for k, v in overrides.items():
+            setattr(ctrl, k, v)
+        return cls(mesh, ctrl, groups, device)
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.lib().ms_system_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # fields -----------------------------------------------------------------
+    def _size(self, name):
+        n = self.nf if name in ("flux", "xf", "sf", "area", "lf") else self.nc
+        return n * _WIDTH.get(name, 1)
+
+    def set_field(self, name, values):
+        a = np.ascontiguousarray(np.asarray(values, np.float64).ravel())
+        if a.size != self._size(name):
+            raise ValueError(f"{name}: expected {self._size(name)} values, got {a.size}")
+        check(_lib.lib().ms_set_field(self.h, name.encode(), dp(a), a.size))
+
+    def get(self, name):
+        n = self._size(name)
+        a = np.empty(n, np.float64)
+        check(_lib.lib().ms_get_field(self.h, name.encode(), dp(a), n))
+        w = _WIDTH.get(name, 1)
+        return a.reshape(-1, w) if w > 1 else a
+
+    def set_ts_order(self, t):
+        check(_lib.lib().ms_set_ts_order(self.h, int(t)))
+
+    def topology(self):
+        rp = np.zeros(self.nc + 1, np.int32)
+        ci = np.zeros(self.nnz, np.int32)
+        fg = np.zeros(self.nf, np.int32)
+        check(_lib.lib().ms_get_topology(self.h, ip(rp), ip(ci), ip(fg)))
+        return dict(row_ptr=rp, col_idx=ci, fgrp=fg)
+
+    # drivers ------------------------------------------------------------------
+    def simple_run(self, nsteps=0):
+        itd, ier = C.c_int(0), C.c_int(0)
+        check(_lib.lib().ms_simple_run(self.h, int(nsteps), C.byref(itd), C.byref(ier)))
+        return ier.value, itd.value
+
+    def piso_run(self, nsteps=0):
+        ier = C.c_int(0)
+        check(_lib.lib().ms_piso_run(self.h, int(nsteps), C.byref(ier)))
+        return ier.value
+
+    def history(self):
+        n = C.c_int(0)
+        check(_lib.lib().ms_get_history(self.h, None, 0, C.byref(n)))
+        out = np.zeros(max(1, n.value) * 8)
+        check(_lib.lib().ms_get_history(self.h, dp(out), n.value, C.byref(n)))
+        return out[: n.value * 8].reshape(-1, 8)
+
+    # per-call routines ---------------------------------------------------------
+    def compute_gradients(self):
+        check(_lib.lib().ms_compute_gradients(self.h))
+
+    def momentum_assembly(self, comp):
+        val, rhs, ap = np.zeros(self.nnz), np.zeros(self.nc), np.zeros(self.nc)
+        check(_lib.lib().ms_momentum_assembly(self.h, int(comp), dp(val), dp(rhs), dp(ap)))
+        return val, rhs, ap
+
+    def temperature_assembly(self):
+        val, rhs, ap = np.zeros(self.nnz), np.zeros(self.nc), np.zeros(self.nc)
+        check(_lib.lib().ms_temperature_assembly(self.h, dp(val), dp(rhs), dp(ap)))
+        return val, rhs, ap
+
+    def ppe_assembly(self):
+        val, rhs = np.zeros(self.nnz), np.zeros(self.nc)
+        check(_lib.lib().ms_ppe_assembly(self.h, dp(val), dp(rhs)))
+        return val, rhs
+
+    def flux_rhiechow(self):
+        check(_lib.lib().ms_flux_rhiechow(self.h))
+
+    def outflow_mass_rescale(self):
+        check(_lib.lib().ms_outflow_mass_rescale(self.h))
+
+    def correct_fields(self, pp):
+        pp = np.ascontiguousarray(pp, np.float64)
+        dpp = np.zeros(3 * self.nc)
+        check(_lib.lib().ms_correct_fields(self.h, dp(pp), dp(dpp)))
+        return dpp.reshape(-1, 3)
