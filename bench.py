@@ -1,0 +1,193 @@
+"""Benchmark: transient PISO on the 2M-cell channel, mixed precision vs FP64.
+
+Workload (one step = one PISO time step of piso_run_mpi at one partition):
+  channel 400 x 100 x 50 hex cells (4*pi x 2 x 2*pi, wall-normal tanh
+  clustering, adjacent-spacing ratio 1.1), Re_tau = 180 (mu = 1/180,
+  body force 1), parabolic inlet U_b = 15.63 / outflow in x, walls in y,
+  symmetry in z; initial field = laminar parabola + seeded uniform noise of
+  +-5% U_b (seed 180); dt from CFL 0.5; BDF2; 2 PISO correctors; p' by
+  PCG with multicolour DILU to a 1e-10 relative residual, momentum by
+  BiCGSTAB with multicolour DILU to 1e-10.
+  * mixed: FP32 matrix / preconditioner / Krylov vectors with FP32
+    accumulation inside FP64 reliable updates (p') and FP64 iterative
+    refinement (momentum); every solve ends at a 1e-10 FP64 true residual.
+  * fp64: the same algorithms entirely in FP64.
+value = mixed PISO steps/s; fp64 steps/s, the ratio and p50/p99 step
+latency are reported beside it.  Inputs are resident in HBM before timing.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "PISO steps/s @2M cells, mixed vs FP64 at 1e-10 residual; p50/p99 solve latency"
+HBM_PEAK = 8.0e12
+UB = 15.63
+
+
+def control_text(nx, lin_max=20000):
+    dx = 4 * np.pi / nx
+    dt = 0.5 * dx / (1.5 * UB * 1.05)
+    return f"""rho = 1.0
+mu = {1.0 / 180.0!r}
+transient = true
+dt = {dt!r}
+n_time_max = 1
+n_correct = 2
+time_scheme = 2
+lin_tol = 1e-10
+lin_max = {lin_max}
+conv_blend = 0.0
+nonorth_corr = 1.0
+ppe_precond = jacobi
+body_force = 1.0 0.0 0.0
+bc = 6 velocity-inlet-parabolic {UB!r} 2.0 2 -1.0
+bc = 7 outflow
+bc = 5 wall
+bc = 4 symmetry
+"""
+
+
+def initial_field(mesh, ny, nz, nx):
+    from mixsolver_amd.meshgen import tanh_stretch
+    rng = np.random.default_rng(180)
+    ys = tanh_stretch(ny, 1.1)
+    yc = 0.5 * (ys[1:] + ys[:-1])
+    Y = np.broadcast_to(yc[None, :, None], (nz, ny, nx)).ravel()
+    u0 = np.zeros((mesh.ncells, 3))
+    u0[:, 0] = 1.5 * UB * (1.0 - Y ** 2)
+    u0 += rng.uniform(-0.05 * UB, 0.05 * UB, size=u0.shape)
+    return u0
+
+
+def run_variant(Case, mesh, ctl, u0, over, steps, warmup):
+    cs = Case.from_control(mesh, ctl, **over)
+    cs.set_field("u", u0)
+    for _ in range(warmup):
+        if cs.piso_run(1) != 0:
+            raise RuntimeError("warmup step failed")
+    lat, hist = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        t1 = time.perf_counter()
+        ier = cs.piso_run(1)          # returns after the device is synchronised
+        lat.append(time.perf_counter() - t1)
+        if ier != 0:
+            raise RuntimeError(f"PISO step failed, ier={ier}")
+        hist.append(cs.history()[-1])
+    total = time.perf_counter() - t0
+    return cs, total, np.array(lat), np.array(hist)
+
+
+def cpu_baseline(mesh, ctl, u0, n_needed):
+    """Oracle (C port of the reference path, 1 thread) on a bounded sample:
+    one PISO step of the same 2M-cell case with the p' PCG-Jacobi solves
+    capped at 40 iterations; the full step is extrapolated with the measured
+    per-iteration p' cost to the iteration count the FP64 Jacobi device
+    solve needed for the same step."""
+    from mixsolver_amd.control import parse_control
+    from oracle import pyoracle
+    ctrl, groups = parse_control(control_text(400, lin_max=40), warn=False)
+    ctrl.ppe_precond = 0
+    oc = pyoracle.OracleCase(mesh, ctrl, groups)
+    oc.set_field("u", u0)
+    oc.piso_run(1, mpi=True)
+    t_total, t_ppe, iters = oc.timing()
+    oc.close()
+    per_it = t_ppe / max(iters, 1)
+    t_step = (t_total - t_ppe) + per_it * n_needed
+    return dict(value=1.0 / t_step, unit="steps/s", cores=1, kind="port",
+                sample=(f"oracle PISO step on the 2M-cell channel with p' PCG-Jacobi capped at 40 "
+                        f"iterations ({t_total:.1f} s, {iters} p' iterations at {per_it * 1e3:.1f} ms); "
+                        f"extrapolated to the {n_needed} p' iterations of the FP64 Jacobi device solve"))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--size", default="400x100x50")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU runs are not part "
+              "of this build", file=sys.stderr)
+        sys.exit(2)
+    from mixsolver_amd import meshgen
+    from mixsolver_amd.solver import Case
+    nx, ny, nz = (int(v) for v in args.size.split("x"))
+    mesh = meshgen.channel(nx, ny, nz)
+    ctl = control_text(nx)
+    u0 = initial_field(mesh, ny, nz, nx)
+    base = dict(gpu_ppe_precond=2, gpu_mom_precond=2)
+
+    mixed, t_mixed, lat_mixed, h_mixed = run_variant(Case, mesh, ctl, u0, dict(base, gpu_precision=1,
+                                                                             gpu_inner_tol=0.1),
+                                                     args.steps, args.warmup)
+    # roofline of the dominant kernel (FP32 SpMV of the mixed p' solve) on the last p' matrix
+    t_k, b_k = mixed.time_kernel(fp32=True, reps=100)
+    t_k64, b_k64 = mixed.time_kernel(fp32=False, reps=50)
+    mixed.close()
+    fp64, t_fp64, lat_fp64, h_fp64 = run_variant(Case, mesh, ctl, u0, dict(base, gpu_precision=0),
+                                                 args.steps, args.warmup)
+    fp64.close()
+
+    sps_mixed = args.steps / t_mixed
+    sps_fp64 = args.steps / t_fp64
+    cells = mesh.ncells
+    out = {
+        "metric": METRIC,
+        "value": sps_mixed,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * t_mixed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 inner + f64 refinement",
+        "data": "synthetic",
+        "config": {"workload": f"channel {nx}x{ny}x{nz} hex ({cells} cells), transient PISO, 2 correctors, "
+                               "p' PCG + multicolour DILU to 1e-10, momentum BiCGSTAB + DILU to 1e-10",
+                   "cells": cells, "faces": mesh.nfaces, "time_scheme": "BDF2", "parallelism": "1 GPU"},
+        "fp64_steps_per_s": sps_fp64,
+        "fp64_ms_per_step": 1e3 * t_fp64 / args.steps,
+        "speedup_mixed_vs_fp64": sps_mixed / sps_fp64,
+        "p50_step_ms": float(np.percentile(lat_mixed, 50) * 1e3),
+        "p99_step_ms": float(np.percentile(lat_mixed, 99) * 1e3),
+        "fp64_p50_step_ms": float(np.percentile(lat_fp64, 50) * 1e3),
+        "fp64_p99_step_ms": float(np.percentile(lat_fp64, 99) * 1e3),
+        "ppe_iters_per_step_mixed": float(np.mean(h_mixed[:, 6])),
+        "ppe_iters_per_step_fp64": float(np.mean(h_fp64[:, 6])),
+        "ppe_final_res_mixed_max": float(np.max(h_mixed[:, 7])),
+        "ppe_final_res_fp64_max": float(np.max(h_fp64[:, 7])),
+        "cell_iterations_per_s_mixed": cells * float(np.sum(h_mixed[:, 6])) / t_mixed,
+        "roofline": {"bound": "hbm", "kernel": "k_pcg_spmv<float> (SELL-64, difference form, fused p.q)",
+                     "achieved": b_k / (t_k * 1e-3) / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": b_k / (t_k * 1e-3) / HBM_PEAK, "traffic": None,
+                     "avg_launch_us": t_k * 1e3, "alg_bytes": b_k,
+                     "fp64_kernel_GBps": b_k64 / (t_k64 * 1e-3) / 1e9, "fp64_avg_launch_us": t_k64 * 1e3},
+    }
+    if not args.no_cpu:
+        # p' iterations the reference's Jacobi PCG needs for the first step (device FP64 Jacobi run)
+        jac, _, _, h_jac = run_variant(Case, mesh, ctl, u0, dict(gpu_ppe_precond=0, gpu_mom_precond=2,
+                                                                 gpu_precision=0), 1, 0)
+        jac.close()
+        out["cpu_baseline"] = cpu_baseline(mesh, ctl, u0, int(h_jac[0, 6]))
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -145,6 +145,13 @@ int ms_solve(int n, const int *row_ptr, const int *col_idx, const double *val,
              int precond, int precision, double inner_tol,
              int *it, double *res, int *ierr);
 
+/* ---- measurement (no reference counterpart) -----------------------------
+ * Average duration of the dominant solver kernel (PCG SpMV with its fused
+ * p.q partials) on the system's current matrix, timed with HIP events on the
+ * library stream over reps launches.  which: 0 = FP32 difference-form kernel
+ * of the mixed path, 1 = FP64 kernel.  alg_bytes: nnz*(s+4) + 4(n+1) + 2ns. */
+int ms_time_kernel(ms_system *s, int which, int reps, double *avg_ms, double *alg_bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -64,6 +64,7 @@ def lib():
         "ms_bicgstab_ilu0": (C.c_int, [C.c_int, _I, _I, _D, _D, _D, C.c_double, C.c_int, _I, _D, _I]),
         "ms_solve": (C.c_int, [C.c_int, _I, _I, _D, _D, _D, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int,
                                C.c_double, _I, _D, _I]),
+        "ms_time_kernel": (C.c_int, [_V, C.c_int, C.c_int, _D, _D]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -10,7 +10,9 @@ stay compatible with the reference reader:
   gpu_ppe_precond  jacobi | ic0 | dilu | ilu0     (default: follow ppe_precond)
   gpu_mom_precond  ilu0 | jacobi | dilu           (default: ilu0, the reference)
   gpu_precision    fp64 | mixed                   (default fp64)
-  gpu_inner_tol    relative tolerance of each FP32 inner solve (default 1e-5)
+  gpu_inner_tol    mixed PCG: reliable-update trigger, the FP32 recursive residual
+                   falls below this fraction of its value at the last FP64 update
+                   (default 0.1)
   gpu_reorder      none | rcm                     (default none)
 
 ``ilu0``/``ic0`` select natural-order factorizations (parity with the
@@ -98,7 +100,7 @@ def default_ctrl() -> MsCtrl:
     c.cp, c.k_cond, c.tref, c.beta = 1005.0, 0.026, 0.0, 0.0
     c.boussinesq, c.thermal_model, c.inlet_ramp = 0, 0, 1
     c.gpu_ppe_precond, c.gpu_mom_precond, c.gpu_precision = -1, 1, 0
-    c.gpu_ppe_solver, c.gpu_inner_tol, c.gpu_reorder = 0, 1.0e-5, 0
+    c.gpu_ppe_solver, c.gpu_inner_tol, c.gpu_reorder = 0, 0.1, 0
     return c
 
 

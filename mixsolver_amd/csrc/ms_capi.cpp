@@ -354,4 +354,17 @@ int ms_solve(int n, const int *rp, const int *ci, const double *v, const double 
   });
 }
 
+int ms_time_kernel(ms_system *h, int which, int reps, double *avg_ms, double *alg_bytes) {
+  return guarded([&] {
+    System &s = sys(h);
+    if (which < 0 || which > 1 || reps <= 0) fail(MS_EINVAL, "ms_time_kernel: which in {0,1}, reps > 0");
+    const bool fp32 = which == 0;
+    double t = s.kry->time_spmv(s.val.p, fp32, reps);
+    const double sz = fp32 ? 4.0 : 8.0;
+    if (avg_ms) *avg_ms = t;
+    // algorithmic bytes of one CSR SpMV: nnz (val + col) + row pointers + x and y
+    if (alg_bytes) *alg_bytes = (double)s.pat.nnz * (sz + 4.0) + 4.0 * (s.nc + 1) + 2.0 * s.nc * sz;
+  });
+}
+
 }  // extern "C"

@@ -5,14 +5,21 @@
 // (mod_uns_linsolver_mpi.f90: bicgstab_ilu0_mpi :155-229, cg_jacobi_mpi
 // :235-300, cg_ic0_mpi :306-363; factorizations mod_uns_linsolver.f90:79-142
 // and :328-399).  Scalars (alpha, beta, rho, omega, residuals, iteration
-// count, breakdown flags) live in device memory and are updated by the last
-// workgroup of each reduction kernel, so an iteration needs no host round
-// trip; the host enqueues iterations in chunks and reads the state once per
-// chunk.  Every iteration kernel returns at once when the state says done.
+// count, breakdown flags) live in device memory; they are updated by the
+// one-workgroup finisher of each two-stage reduction, so an iteration needs
+// no host round trip.  The host enqueues iterations in chunks and reads the
+// state once per chunk; every iteration kernel returns at once when the
+// state says done.
 //
-// Reductions are two-stage and deterministic: wave64 shuffles + LDS inside a
-// workgroup, per-workgroup partials, then the last workgroup (ticket counter)
-// sums the partials in a fixed order.
+// Kernel fusion per PCG iteration (FP64 path):
+//   Jacobi: [q = A p, p.q] [x,r update, r.r, r.(r/d)] [p = r/d + beta p]
+//   DILU  : [q = A p, p.q] [x,r update, r.r, z = r/d on colour 0]
+//           [forward sweeps colours 1..c-1, backward sweeps c-2..0, r.z]
+//           [p = z + beta p]
+// Vectors are templated on the value type: double for the FP64 path, float
+// for the inner iterations of mixed precision (FP32 products and FP32
+// accumulation of dot-product partials; scalar recurrences in double).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -22,33 +29,234 @@
 namespace ms {
 namespace {
 
-template <class T> struct Tiny;
-template <> struct Tiny<double> { static constexpr double v = 2.2250738585072014e-308; };
-template <> struct Tiny<float> { static constexpr float v = 1.17549435e-38f; };
-
-template <class T>
-__device__ __forceinline__ T row_dot(int r, const long long *off, const int *rowlen, const int *col,
-                                     const T *val, const T *x) {
-  const long long base = off[r >> 6] + (r & 63);
-  const int len = rowlen[r];
-  T s = T(0);
-  for (int k = 0; k < len; k++) {
-    const long long q = base + (long long)k * kSlice;
-    s = s + val[q] * x[col[q]];
-  }
-  return s;
-}
+constexpr double kTiny = 2.2250738585072014e-308;   // tiny(1.0_dp)
+constexpr double kRefineTol = 1.0e-5;               // FP32 inner BiCGSTAB reduction per refinement cycle
 
 struct Mat {
   const long long *off;
   const int *rowlen, *col, *diagk;
 };
 
+template <class T>
+__device__ __forceinline__ T row_dot(int r, const Mat &A, const T *val, const T *x) {
+  const long long base = A.off[r >> 6] + (r & 63);
+  const int len = A.rowlen[r];
+  T s = T(0);
+  for (int k = 0; k < len; k++) {
+    const long long q = base + (long long)k * kSlice;
+    s = s + val[q] * x[A.col[q]];
+  }
+  return s;
+}
+
+// FP32 rows are stored with the FP64 row sum in the diagonal slot and
+// applied in difference form, q_i = s_i p_i + sum_{j!=i} a_ij (p_j - p_i):
+// algebraically equal to A p, but constants are annihilated exactly, so the
+// near-null space of the pressure-correction Laplacian survives the rounding
+// of its coefficients to FP32.
+template <class T>
+__device__ __forceinline__ T row_apply(int r, const Mat &A, const T *val, const T *x) {
+  if constexpr (sizeof(T) == 8) {
+    return row_dot(r, A, val, x);
+  } else {
+    const long long base = A.off[r >> 6] + (r & 63);
+    const int len = A.rowlen[r], dk = A.diagk[r];
+    const T xi = x[r];
+    T s = val[base + (long long)dk * kSlice] * xi;
+    for (int k = 0; k < len; k++) {
+      if (k == dk) continue;
+      const long long q = base + (long long)k * kSlice;
+      s = s + val[q] * (x[A.col[q]] - xi);
+    }
+    return s;
+  }
+}
+
+// FP32 copy of an FP64 matrix for row_apply: off-diagonals rounded, diagonal
+// slot = FP64 row sum rounded.
+__global__ void k_mat32_diff(int n, Mat A, const double *v64, float *v32) {
+  int r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= n) return;
+  const long long base = A.off[r >> 6] + (r & 63);
+  const int len = A.rowlen[r], dk = A.diagk[r];
+  double s = 0.0;
+  for (int k = 0; k < len; k++) {
+    const long long q = base + (long long)k * kSlice;
+    s = s + v64[q];
+    if (k != dk) v32[q] = (float)v64[q];
+  }
+  v32[base + (long long)dk * kSlice] = (float)s;
+}
+
+// ------------------------------------------------------------- finishers
+struct FinBnorm {
+  KS *st;
+  __device__ bool skip() const { return false; }
+  template <class T> __device__ void operator()(const T *a) const { st->bnrm = sqrt((double)a[0]); }
+};
+struct FinResidual {
+  KS *st;
+  __device__ bool skip() const { return false; }
+  template <class T> __device__ void operator()(const T *a) const {
+    st->rr = a[0];
+    st->res = sqrt((double)a[0]) / st->bnrm;
+    st->rmax = sqrt((double)a[0]);
+    if (st->res <= st->tol) st->done = 1;
+  }
+};
+struct FinRz0 {
+  KS *st;
+  __device__ bool skip() const { return st->done != 0; }
+  template <class T> __device__ void operator()(const T *a) const { st->rz = a[0]; }
+};
+struct FinPap {
+  KS *st;
+  __device__ bool skip() const { return st->done != 0; }
+  template <class T> __device__ void operator()(const T *a) const {
+    st->it = st->it + 1;
+    st->pap = a[0];
+    if (fabs((double)a[0]) < kTiny) {
+      st->ierr = 2;
+      st->done = 1;
+    } else {
+      st->alpha = st->rz / (double)a[0];
+    }
+  }
+};
+struct FinUpdate {
+  KS *st;
+  int jac;
+  __device__ bool skip() const { return st->done != 0; }
+  template <class T> __device__ void operator()(const T *a) const {
+    st->rr = a[0];
+    st->res = sqrt((double)a[0]) / st->bnrm;
+    if (st->res <= st->tol) {
+      st->done = 1;
+      return;
+    }
+    if (jac) {
+      if (fabs(st->rz) < kTiny) {
+        st->ierr = 2;
+        st->done = 1;
+      } else {
+        st->beta = (double)a[1] / st->rz;
+        st->rz = a[1];
+      }
+    }
+  }
+};
+struct FinRz {
+  KS *st;
+  __device__ bool skip() const { return st->done != 0; }
+  template <class T> __device__ void operator()(const T *a) const {
+    if (st->ru == 2) st->ru = 0;           // reliable update fully applied
+    if (fabs(st->rz) < kTiny) {
+      st->ierr = 2;
+      st->done = 1;
+    } else {
+      st->beta = (double)a[0] / st->rz;
+      st->rz = a[0];
+    }
+  }
+};
+// Mixed PCG runs the FP32 iteration on r/sigma (sigma = |r| at the last
+// reliable update) so its vectors stay O(1): a p' right-hand side can be
+// ~1e-14, which would otherwise push FP32 products into the denormal range.
+struct FinUpdateRU {
+  KS *st;
+  __device__ bool skip() const { return st->done != 0; }
+  template <class T> __device__ void operator()(const T *a) const {
+    const double rn = sqrt((double)a[0]);              // scaled sloppy |r|
+    st->res = rn * st->sigma / st->bnrm;
+    if (st->res <= st->tol || rn < st->delta * st->rmax) st->ru = 1;
+    if (rn > st->rmax) st->rmax = rn;
+  }
+};
+struct FinRU {         // after x += sigma*y and r = b - A x in FP64
+  KS *st;
+  __device__ bool skip() const { return st->done != 0 || st->ru != 1; }
+  template <class T> __device__ void operator()(const T *a) const {
+    const double rt = sqrt((double)a[0]);
+    st->rr = a[0];
+    st->res = rt / st->bnrm;
+    st->nru = st->nru + 1;
+    if (st->res <= st->tol) {
+      st->done = 1;
+      return;
+    }
+    st->scale = st->sigma / rt;            // p and r.z move to the new scale
+    st->rz = st->rz * st->scale * st->scale;
+    st->sigma = rt;
+    st->rmax = 1.0;
+    st->ru = 2;                            // k_ru_rescale pending
+  }
+};
+struct FinBiRho {
+  KS *st;
+  __device__ bool skip() const { return st->done != 0; }
+  template <class T> __device__ void operator()(const T *a) const {
+    st->it = st->it + 1;
+    st->rho = a[0];
+    if (fabs((double)a[0]) < kTiny) {
+      st->ierr = 2;
+      st->done = 1;
+    } else {
+      st->beta = ((double)a[0] / st->rho_old) * (st->alpha / st->omega);
+    }
+  }
+};
+struct FinBiAlpha {
+  KS *st;
+  __device__ bool skip() const { return st->done != 0; }
+  template <class T> __device__ void operator()(const T *a) const { st->alpha = st->rho / (double)a[0]; }
+};
+struct FinBiS {
+  KS *st;
+  __device__ bool skip() const { return st->done != 0; }
+  template <class T> __device__ void operator()(const T *a) const {
+    st->snrm = sqrt((double)a[0]);
+    if (st->snrm / st->bnrm <= st->tol) {
+      st->res = st->snrm / st->bnrm;
+      st->done = 2;
+    }
+  }
+};
+struct FinBiOmega {
+  KS *st;
+  __device__ bool skip() const { return st->done != 0; }
+  template <class T> __device__ void operator()(const T *a) const {
+    st->tt = a[0];
+    if ((double)a[0] < kTiny) {
+      st->ierr = 2;
+      st->done = 1;
+      return;
+    }
+    st->omega = (double)a[1] / (double)a[0];
+    if (fabs(st->omega) < kTiny) {
+      st->ierr = 2;
+      st->done = 1;
+    }
+  }
+};
+struct FinBiXR {
+  KS *st;
+  __device__ bool skip() const { return st->done != 0; }
+  template <class T> __device__ void operator()(const T *a) const {
+    st->rr = a[0];
+    st->res = sqrt((double)a[0]) / st->bnrm;
+    if (st->res <= st->tol) {
+      st->done = 1;
+      return;
+    }
+    st->rho_old = st->rho;
+  }
+};
+
 // ------------------------------------------------------------ basic kernels
 template <class T>
 __global__ void k_spmv(int n, Mat A, const T *val, const T *x, T *y) {
   int r = blockIdx.x * kBlock + threadIdx.x;
-  if (r < n) y[r] = row_dot(r, A.off, A.rowlen, A.col, val, x);
+  if (r < n) y[r] = row_dot(r, A, val, x);
 }
 
 template <class T>
@@ -62,131 +270,150 @@ __global__ void k_to_float(long long n, const double *a, float *b) {
   if (i < n) b[i] = (float)a[i];
 }
 
-// bnrm = tsqrt(b.b)
 template <class T>
-__global__ void k_bnorm(int n, const T *b, KS<T> *st, T *part) {
+__global__ void k_sumsq(int n, const T *b, T *part) {
   int i = blockIdx.x * kBlock + threadIdx.x;
   T v[1] = {i < n ? b[i] * b[i] : T(0)};
-  grid_reduce<OpSum, 1>(v, part, &st->ticket, [&](T *a) { st->bnrm = tsqrt(a[0]); });
+  block_partials<OpSum, 1>(v, part);
 }
 
-// r = b - A x; res = ||r|| / bnrm; done if res <= tol
 template <class T>
-__global__ void k_residual(int n, Mat A, const T *val, const T *b, const T *x, T *r, KS<T> *st, T *part) {
+__global__ void k_residual(int n, Mat A, const T *val, const T *b, const T *x, T *r, T *part) {
   int i = blockIdx.x * kBlock + threadIdx.x;
   T v[1] = {T(0)};
   if (i < n) {
-    T ri = b[i] - row_dot(i, A.off, A.rowlen, A.col, val, x);
+    T ri = b[i] - row_dot(i, A, val, x);
     r[i] = ri;
     v[0] = ri * ri;
   }
-  grid_reduce<OpSum, 1>(v, part, &st->ticket, [&](T *a) {
-    st->rr = a[0];
-    st->res = tsqrt(a[0]) / st->bnrm;
-    if (st->res <= st->tol) st->done = 1;
-  });
+  block_partials<OpSum, 1>(v, part);
 }
 
 // ------------------------------------------------------------------- PCG
-// z = r/diag (Jacobi) fused variants carry a `jac` flag.
 template <class T>
-__global__ void k_pcg_start(int n, const T *r, const T *z, const T *diag, int jac, T *p, KS<T> *st, T *part) {
+__global__ void k_pcg_start(int n, const T *r, const T *z, const T *dg, int jac, T *p, const KS *st, T *part) {
   if (st->done) return;
   int i = blockIdx.x * kBlock + threadIdx.x;
   T v[1] = {T(0)};
   if (i < n) {
-    T zi = jac ? r[i] / diag[i] : z[i];
+    T zi = jac ? r[i] / dg[i] : z[i];
     p[i] = zi;
     v[0] = r[i] * zi;
   }
-  grid_reduce<OpSum, 1>(v, part, &st->ticket, [&](T *a) { st->rz = a[0]; });
+  block_partials<OpSum, 1>(v, part);
 }
 
 template <class T>
-__global__ void k_pcg_spmv(int n, Mat A, const T *val, const T *p, T *q, KS<T> *st, T *part) {
+__global__ void k_pcg_spmv(int n, Mat A, const T *val, const T *p, T *q, const KS *st, T *part) {
   if (st->done) return;
   int i = blockIdx.x * kBlock + threadIdx.x;
   T v[1] = {T(0)};
   if (i < n) {
-    T qi = row_dot(i, A.off, A.rowlen, A.col, val, p);
+    T qi = row_apply(i, A, val, p);
     q[i] = qi;
     v[0] = p[i] * qi;
   }
-  grid_reduce<OpSum, 1>(v, part, &st->ticket, [&](T *a) {
-    st->it = st->it + 1;
-    st->pap = a[0];
-    if (tabs(a[0]) < Tiny<T>::v) {
-      st->ierr = 2;
-      st->done = 1;
-    } else {
-      st->alpha = st->rz / a[0];
-    }
-  });
+  block_partials<OpSum, 1>(v, part);
 }
 
-// x += alpha p; r -= alpha q; res check; Jacobi: rz_new = r.(r/diag), beta
-template <class T>
-__global__ void k_pcg_update(int n, const T *p, const T *q, T *x, T *r, const T *diag, int jac, KS<T> *st,
-                             T *part) {
+// x += alpha p; r -= alpha q; partials r.r and (Jacobi) r.(r/d).
+// mode 0 plain, 1 Jacobi, 2 DILU: z = r/d on the first colour block [0,c0end)
+template <class T, class X>
+__global__ void k_pcg_update(int n, const T *p, const T *q, X *x, T *r, T *z, const T *dg, int mode, int c0end,
+                             const KS *st, T *part) {
   if (st->done) return;
   int i = blockIdx.x * kBlock + threadIdx.x;
   T v[2] = {T(0), T(0)};
   if (i < n) {
-    const T alpha = st->alpha;
-    x[i] = x[i] + alpha * p[i];
+    const T alpha = (T)st->alpha;
+    x[i] = x[i] + (X)(alpha * p[i]);
     T ri = r[i] - alpha * q[i];
     r[i] = ri;
     v[0] = ri * ri;
-    if (jac) v[1] = ri * (ri / diag[i]);
+    if (mode == 1) v[1] = ri * (ri / dg[i]);
+    else if (mode == 2 && i < c0end) z[i] = ri / dg[i];
   }
-  grid_reduce<OpSum, 2>(v, part, &st->ticket, [&](T *a) {
-    st->rr = a[0];
-    st->res = tsqrt(a[0]) / st->bnrm;
-    if (st->res <= st->tol) {
-      st->done = 1;
-      return;
-    }
-    if (jac) {
-      if (tabs(st->rz) < Tiny<T>::v) {
-        st->ierr = 2;
-        st->done = 1;
-      } else {
-        st->beta = a[1] / st->rz;
-        st->rz = a[1];
-      }
-    }
-  });
+  block_partials<OpSum, 2>(v, part);
 }
 
 template <class T>
-__global__ void k_pcg_rz(int n, const T *r, const T *z, KS<T> *st, T *part) {
-  if (st->done) return;
-  int i = blockIdx.x * kBlock + threadIdx.x;
-  T v[1] = {i < n ? r[i] * z[i] : T(0)};
-  grid_reduce<OpSum, 1>(v, part, &st->ticket, [&](T *a) {
-    if (tabs(st->rz) < Tiny<T>::v) {
-      st->ierr = 2;
-      st->done = 1;
-    } else {
-      st->beta = a[0] / st->rz;
-      st->rz = a[0];
-    }
-  });
-}
-
-template <class T>
-__global__ void k_pcg_p(int n, const T *r, const T *z, const T *diag, int jac, T *p, const KS<T> *st) {
+__global__ void k_pcg_p(int n, const T *r, const T *z, const T *dg, int jac, T *p, const KS *st) {
   if (st->done) return;
   int i = blockIdx.x * kBlock + threadIdx.x;
   if (i < n) {
-    T zi = jac ? r[i] / diag[i] : z[i];
-    p[i] = zi + st->beta * p[i];
+    T zi = jac ? r[i] / dg[i] : z[i];
+    p[i] = zi + (T)st->beta * p[i];
+  }
+}
+
+template <class T>
+__global__ void k_dot(int n, const T *a, const T *b, const KS *st, T *part) {
+  if (st->done) return;
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  T v[1] = {i < n ? a[i] * b[i] : T(0)};
+  block_partials<OpSum, 1>(v, part);
+}
+
+template <class T>
+__global__ void k_jac_rz(int n, const T *r, const T *dg, T *z, const KS *st, T *part) {
+  if (st->done) return;
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  T v[1] = {T(0)};
+  if (i < n) {
+    T zi = r[i] / dg[i];
+    z[i] = zi;
+    v[0] = r[i] * zi;
+  }
+  block_partials<OpSum, 1>(v, part);
+}
+
+// ------------------------------------------------------- reliable updates
+// x64 += sigma * y; y = 0   (only when a reliable update is pending)
+__global__ void k_fold(int n, float *y, double *x, const KS *st) {
+  if (st->done || st->ru != 1) return;
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) {
+    x[i] = x[i] + st->sigma * (double)y[i];
+    y[i] = 0.0f;
+  }
+}
+// r64 = b - A64 x and r64.r64 (FP64)
+__global__ void k_true_res(int n, Mat A, const double *val, const double *b, const double *x, double *r64,
+                           const KS *st, double *part) {
+  if (st->done || st->ru != 1) return;
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  double v[1] = {0.0};
+  if (i < n) {
+    double ri = b[i] - row_dot(i, A, val, x);
+    r64[i] = ri;
+    v[0] = ri * ri;
+  }
+  block_partials<OpSum, 1>(v, part);
+}
+// r32 = r64 / sigma; p *= scale   (after FinRU moved to the new scale)
+__global__ void k_ru_rescale(int n, const double *r64, float *r32, float *p, const KS *st) {
+  if (st->done || st->ru != 2) return;
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) {
+    r32[i] = (float)(r64[i] / st->sigma);
+    p[i] = p[i] * (float)st->scale;
+  }
+}
+// initial scaled residual r32 = r64 / sigma
+__global__ void k_scale_down(int n, const double *r64, float *r32, const KS *st) {
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) r32[i] = (float)(r64[i] / st->sigma);
+}
+__global__ void k_set_sigma(KS *st) {
+  if (threadIdx.x == 0) {
+    st->sigma = sqrt(st->rr);
+    st->rmax = 1.0;
   }
 }
 
 // -------------------------------------------------------------- BiCGSTAB
 template <class T>
-__global__ void k_bi_init(int n, const T *r, T *r0, T *p, T *v, KS<T> *st) {
+__global__ void k_bi_init(int n, const T *r, T *r0, T *p, T *v, KS *st) {
   int i = blockIdx.x * kBlock + threadIdx.x;
   if (i < n) {
     r0[i] = r[i];
@@ -194,142 +421,90 @@ __global__ void k_bi_init(int n, const T *r, T *r0, T *p, T *v, KS<T> *st) {
     v[i] = T(0);
   }
   if (i == 0) {
-    st->rho_old = T(1);
-    st->alpha = T(1);
-    st->omega = T(1);
+    st->rho_old = 1.0;
+    st->alpha = 1.0;
+    st->omega = 1.0;
   }
 }
 
 template <class T>
-__global__ void k_bi_rho(int n, const T *r, const T *r0, KS<T> *st, T *part) {
+__global__ void k_bi_p(int n, const T *r, const T *v, T *p, const KS *st) {
   if (st->done) return;
   int i = blockIdx.x * kBlock + threadIdx.x;
-  T v[1] = {i < n ? r[i] * r0[i] : T(0)};
-  grid_reduce<OpSum, 1>(v, part, &st->ticket, [&](T *a) {
-    st->it = st->it + 1;
-    st->rho = a[0];
-    if (tabs(a[0]) < Tiny<T>::v) {
-      st->ierr = 2;
-      st->done = 1;
-    } else {
-      st->beta = (a[0] / st->rho_old) * (st->alpha / st->omega);
-    }
-  });
+  if (i < n) p[i] = r[i] + (T)st->beta * (p[i] - (T)st->omega * v[i]);
 }
 
 template <class T>
-__global__ void k_bi_p(int n, const T *r, const T *v, T *p, const KS<T> *st) {
-  if (st->done) return;
-  int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i < n) p[i] = r[i] + st->beta * (p[i] - st->omega * v[i]);
-}
-
-// v = A ph ; alpha = rho / (r0.v)
-template <class T>
-__global__ void k_bi_spmv_alpha(int n, Mat A, const T *val, const T *ph, const T *r0, T *v, KS<T> *st, T *part) {
+__global__ void k_bi_spmv_dot(int n, Mat A, const T *val, const T *ph, const T *r0, T *v, const KS *st, T *part) {
   if (st->done) return;
   int i = blockIdx.x * kBlock + threadIdx.x;
   T w[1] = {T(0)};
   if (i < n) {
-    T vi = row_dot(i, A.off, A.rowlen, A.col, val, ph);
+    T vi = row_apply(i, A, val, ph);
     v[i] = vi;
     w[0] = r0[i] * vi;
   }
-  grid_reduce<OpSum, 1>(w, part, &st->ticket, [&](T *a) { st->alpha = st->rho / a[0]; });
+  block_partials<OpSum, 1>(w, part);
 }
 
-// s = r - alpha v ; early exit when ||s||/bnrm <= tol (done = 2: x += alpha ph pending)
 template <class T>
-__global__ void k_bi_s(int n, const T *r, const T *v, T *s, KS<T> *st, T *part) {
+__global__ void k_bi_s(int n, const T *r, const T *v, T *s, const KS *st, T *part) {
   if (st->done) return;
   int i = blockIdx.x * kBlock + threadIdx.x;
   T w[1] = {T(0)};
   if (i < n) {
-    T si = r[i] - st->alpha * v[i];
+    T si = r[i] - (T)st->alpha * v[i];
     s[i] = si;
     w[0] = si * si;
   }
-  grid_reduce<OpSum, 1>(w, part, &st->ticket, [&](T *a) {
-    st->snrm = tsqrt(a[0]);
-    if (st->snrm / st->bnrm <= st->tol) {
-      st->res = st->snrm / st->bnrm;
-      st->done = 2;
-    }
-  });
+  block_partials<OpSum, 1>(w, part);
 }
 
-// the pending early-exit update x += alpha ph (runs once: done 2 -> 1)
 template <class T>
-__global__ void k_bi_early(int n, const T *ph, T *x, KS<T> *st, T *part) {
+__global__ void k_bi_early(int n, const T *ph, T *x, const KS *st) {
   if (st->done != 2) return;
   int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i < n) x[i] = x[i] + st->alpha * ph[i];
-  T w[1] = {T(0)};
-  grid_reduce<OpSum, 1>(w, part, &st->ticket, [&](T *) { st->done = 1; });
+  if (i < n) x[i] = x[i] + (T)st->alpha * ph[i];
 }
 
-// t = A sh ; omega = (t.s)/(t.t) with breakdown checks
+__global__ void k_done2_to_1(KS *st) {
+  if (threadIdx.x == 0 && st->done == 2) st->done = 1;
+}
+
 template <class T>
-__global__ void k_bi_spmv_omega(int n, Mat A, const T *val, const T *sh, const T *s, T *t, KS<T> *st, T *part) {
+__global__ void k_bi_spmv_omega(int n, Mat A, const T *val, const T *sh, const T *s, T *t, const KS *st, T *part) {
   if (st->done) return;
   int i = blockIdx.x * kBlock + threadIdx.x;
   T w[2] = {T(0), T(0)};
   if (i < n) {
-    T ti = row_dot(i, A.off, A.rowlen, A.col, val, sh);
+    T ti = row_apply(i, A, val, sh);
     t[i] = ti;
     w[0] = ti * ti;
     w[1] = ti * s[i];
   }
-  grid_reduce<OpSum, 2>(w, part, &st->ticket, [&](T *a) {
-    st->tt = a[0];
-    if (a[0] < Tiny<T>::v) {
-      st->ierr = 2;
-      st->done = 1;
-      return;
-    }
-    st->omega = a[1] / a[0];
-    if (tabs(st->omega) < Tiny<T>::v) {
-      st->ierr = 2;
-      st->done = 1;
-    }
-  });
+  block_partials<OpSum, 2>(w, part);
 }
 
 template <class T>
-__global__ void k_bi_xr(int n, const T *ph, const T *sh, const T *s, const T *t, T *x, T *r, KS<T> *st, T *part) {
+__global__ void k_bi_xr(int n, const T *ph, const T *sh, const T *s, const T *t, T *x, T *r, const KS *st, T *part) {
   if (st->done) return;
   int i = blockIdx.x * kBlock + threadIdx.x;
   T w[1] = {T(0)};
   if (i < n) {
-    x[i] = x[i] + st->alpha * ph[i] + st->omega * sh[i];
-    T ri = s[i] - st->omega * t[i];
+    x[i] = x[i] + (T)st->alpha * ph[i] + (T)st->omega * sh[i];
+    T ri = s[i] - (T)st->omega * t[i];
     r[i] = ri;
     w[0] = ri * ri;
   }
-  grid_reduce<OpSum, 1>(w, part, &st->ticket, [&](T *a) {
-    st->rr = a[0];
-    st->res = tsqrt(a[0]) / st->bnrm;
-    if (st->res <= st->tol) {
-      st->done = 1;
-      return;
-    }
-    st->rho_old = st->rho;
-  });
+  block_partials<OpSum, 1>(w, part);
 }
 
 // ------------------------------------------------ preconditioner kernels
 template <class T>
-__global__ void k_jacobi(int n, const T *in, const T *diag, T *out, const int *done) {
-  if (done && *done) return;
+__global__ void k_jacobi(int n, const T *in, const T *dg, T *out, const int *done) {
+  if (*done) return;
   int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i < n) out[i] = in[i] / diag[i];
-}
-
-template <class T>
-__global__ void k_copy(int n, const T *in, T *out, const int *done) {
-  if (done && *done) return;
-  int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i < n) out[i] = in[i];
+  if (i < n) out[i] = in[i] / dg[i];
 }
 
 // natural IC(0), one level of rows (mod_uns_linsolver.f90:328-365)
@@ -387,7 +562,7 @@ __global__ void k_ilu0_factor(const int *rows, int nr, Mat A, T *F, int *flag) {
     const long long bk = A.off[k >> 6] + (k & 63);
     const int dpk = A.diagk[k], lenk = A.rowlen[k];
     const T piv = F[bk + (long long)dpk * kSlice];
-    if (tabs(piv) < Tiny<T>::v) {
+    if (tabs(piv) < (T)kTiny) {
       *flag = 2;
       return;
     }
@@ -409,10 +584,9 @@ __global__ void k_ilu0_factor(const int *rows, int nr, Mat A, T *F, int *flag) {
   }
 }
 
-// forward sweep rows of one level: ILU0 (unit L, in place) or IC0 (L, / diag)
 template <class T>
 __global__ void k_lower_level(const int *rows, int nr, Mat A, const T *F, const T *r, T *z, int ic0, const int *done) {
-  if (done && *done) return;
+  if (*done) return;
   int t = blockIdx.x * kBlock + threadIdx.x;
   if (t >= nr) return;
   const int i = rows[t];
@@ -426,13 +600,10 @@ __global__ void k_lower_level(const int *rows, int nr, Mat A, const T *F, const 
   z[i] = ic0 ? s / F[bi + (long long)dpi * kSlice] : s;
 }
 
-// backward sweep rows of one level.  ILU0: U z = y with U from row i.
-// IC0: L^T z = y, entries L(j,i) read through the transpose slot map in
-// descending j (the column-oriented order of ic0_apply, :390-397).
 template <class T>
 __global__ void k_upper_level(const int *rows, int nr, Mat A, const long long *tpos, const T *F, T *z, int ic0,
                               const int *done) {
-  if (done && *done) return;
+  if (*done) return;
   int t = blockIdx.x * kBlock + threadIdx.x;
   if (t >= nr) return;
   const int i = rows[t];
@@ -453,7 +624,6 @@ __global__ void k_upper_level(const int *rows, int nr, Mat A, const long long *t
   z[i] = s / F[bi + (long long)dpi * kSlice];
 }
 
-// multicolour DILU: d_i = a_ii - sum_{j<i} a_ij a_ji / d_j for one colour block
 template <class T>
 __global__ void k_dilu_setup(int r0, int r1, Mat A, const long long *tpos, const T *val, T *d) {
   int i = r0 + blockIdx.x * kBlock + threadIdx.x;
@@ -469,55 +639,57 @@ __global__ void k_dilu_setup(int r0, int r1, Mat A, const long long *tpos, const
 }
 
 template <class T>
-__global__ void k_dilu_fwd(int r0, int r1, Mat A, const T *val, const T *d, const T *r, T *z, const int *done) {
-  if (done && *done) return;
+__global__ void k_dilu_first(int r0, int r1, const T *dg, const T *r, T *z, const int *done) {
+  if (*done) return;
   int i = r0 + blockIdx.x * kBlock + threadIdx.x;
-  if (i >= r1) return;
-  const long long bi = A.off[i >> 6] + (i & 63);
-  const int dpi = A.diagk[i];
-  T s = r[i];
-  for (int k = 0; k < dpi; k++) {
-    const long long q = bi + (long long)k * kSlice;
-    s = s - val[q] * z[A.col[q]];
-  }
-  z[i] = s / d[i];
+  if (i < r1) z[i] = r[i] / dg[i];
 }
 
 template <class T>
-__global__ void k_dilu_bwd(int r0, int r1, Mat A, const T *val, const T *d, T *z, const int *done) {
-  if (done && *done) return;
+__global__ void k_dilu_fwd(int r0, int r1, Mat A, const T *val, const T *dg, const T *r, T *z, int last,
+                           const int *done, T *part) {
+  if (*done) return;
   int i = r0 + blockIdx.x * kBlock + threadIdx.x;
-  if (i >= r1) return;
-  const long long bi = A.off[i >> 6] + (i & 63);
-  const int dpi = A.diagk[i], len = A.rowlen[i];
-  T s = T(0);
-  for (int k = dpi + 1; k < len; k++) {
-    const long long q = bi + (long long)k * kSlice;
-    s = s + val[q] * z[A.col[q]];
+  T v[1] = {T(0)};
+  if (i < r1) {
+    const long long bi = A.off[i >> 6] + (i & 63);
+    const int dpi = A.diagk[i];
+    T s = r[i];
+    for (int k = 0; k < dpi; k++) {
+      const long long q = bi + (long long)k * kSlice;
+      s = s - val[q] * z[A.col[q]];
+    }
+    const T zi = s / dg[i];
+    z[i] = zi;
+    if (last) v[0] = r[i] * zi;
   }
-  z[i] = z[i] - s / d[i];
+  if (part) block_partials<OpSum, 1>(v, part);
+}
+
+template <class T>
+__global__ void k_dilu_bwd(int r0, int r1, Mat A, const T *val, const T *dg, const T *r, T *z, const int *done,
+                           T *part) {
+  if (*done) return;
+  int i = r0 + blockIdx.x * kBlock + threadIdx.x;
+  T v[1] = {T(0)};
+  if (i < r1) {
+    const long long bi = A.off[i >> 6] + (i & 63);
+    const int dpi = A.diagk[i], len = A.rowlen[i];
+    T s = T(0);
+    for (int k = dpi + 1; k < len; k++) {
+      const long long q = bi + (long long)k * kSlice;
+      s = s + val[q] * z[A.col[q]];
+    }
+    const T zi = z[i] - s / dg[i];
+    z[i] = zi;
+    v[0] = r[i] * zi;
+  }
+  if (part) block_partials<OpSum, 1>(v, part);
 }
 
 // --------------------------------------------------- mixed precision glue
-// r = b - A x (FP64), rr; then r32 = r / ||r|| and reset the inner state
-__global__ void k_mp_residual(int n, Mat A, const double *val, const double *b, const double *x, double *r,
-                              KS<double> *st, double *part) {
-  int i = blockIdx.x * kBlock + threadIdx.x;
-  double v[1] = {0.0};
-  if (i < n) {
-    double ri = b[i] - row_dot(i, A.off, A.rowlen, A.col, val, x);
-    r[i] = ri;
-    v[0] = ri * ri;
-  }
-  grid_reduce<OpSum, 1>(v, part, &st->ticket, [&](double *a) {
-    st->rr = a[0];
-    st->res = tsqrt(a[0]) / st->bnrm;
-    if (st->res <= st->tol) st->done = 1;
-  });
-}
-
-__global__ void k_mp_scale_down(int n, const double *r, const KS<double> *st, float *r32, float *x32,
-                                KS<float> *s32, float inner_tol, int inner_max) {
+__global__ void k_mp_scale_down(int n, const double *r, const KS *st, float *r32, float *x32, KS *s32,
+                                double inner_tol, int inner_max) {
   int i = blockIdx.x * kBlock + threadIdx.x;
   const double inv = 1.0 / sqrt(st->rr);
   if (i < n) {
@@ -528,14 +700,13 @@ __global__ void k_mp_scale_down(int n, const double *r, const KS<double> *st, fl
     s32->it = 0;
     s32->ierr = 0;
     s32->done = 0;
-    s32->bnrm = 1.0f;
+    s32->bnrm = 1.0;
     s32->tol = inner_tol;
     s32->maxit = inner_max;
-    s32->ticket = 0u;
   }
 }
 
-__global__ void k_mp_correct(int n, const float *x32, const KS<double> *st, double *x) {
+__global__ void k_mp_correct(int n, const float *x32, const KS *st, double *x) {
   int i = blockIdx.x * kBlock + threadIdx.x;
   if (i < n) x[i] = x[i] + sqrt(st->rr) * (double)x32[i];
 }
@@ -545,10 +716,15 @@ __global__ void k_mp_correct(int n, const float *x32, const KS<double> *st, doub
 // ============================================================ host side
 static Mat matof(const SellPattern *p) { return Mat{p->d_off.p, p->d_rowlen.p, p->d_col.p, p->d_diagk.p}; }
 
+template <class Op, int NV, class T, class Fin>
+static void finish(int nparts, const T *part, Fin fin) {
+  hipLaunchKernelGGL((k_finish<Op, NV, T, Fin>), dim3(1), dim3(kBlock), 0, stream(), nparts, part, fin);
+}
+
 Krylov::Krylov(SellPattern *pat_) : pat(pat_), n(pat_->n) {
   nblk_ = nblocks(n);
-  for (DBuf<double> *b : {&r_, &r0_, &p_, &q_, &z_, &s_, &t_, &ph_, &sh_, &dinv_}) b->alloc(n);
-  part_.alloc(2 * (size_t)nblk_ + 8);
+  for (DBuf<double> *b : {&r_, &r0_, &p_, &q_, &z_, &s_, &t_, &ph_, &sh_, &dg_}) b->alloc(n);
+  part_.alloc(4 * ((size_t)nblk_ + 64) * 2);
   st_.alloc(1);
   flag_.alloc(1);
   MS_HIP(hipHostMalloc(&hpin_, 4096, hipHostMallocDefault));
@@ -558,12 +734,20 @@ Krylov::~Krylov() {
   if (hpin_) (void)hipHostFree(hpin_);
 }
 
+void Krylov::ensure32() {
+  if (have32_) return;
+  for (DBuf<float> *v : {&r32_, &r032_, &p32_, &q32_, &z32_, &s32_, &t32_, &ph32_, &sh32_, &x32_, &dg32_}) v->alloc(n);
+  part32_.alloc(4 * ((size_t)nblk_ + 64) * 2);
+  st32_.alloc(1);
+  have32_ = true;
+}
+
 template <class T>
-KS<T> Krylov::read_state(KS<T> *d) {
-  MS_HIP(hipMemcpyAsync(hpin_, d, sizeof(KS<T>), hipMemcpyDeviceToHost, stream()));
+KS Krylov::read_state(KS *d) {
+  MS_HIP(hipMemcpyAsync(hpin_, d, sizeof(KS), hipMemcpyDeviceToHost, stream()));
   MS_HIP(hipStreamSynchronize(stream()));
-  KS<T> h;
-  std::memcpy(&h, hpin_, sizeof(KS<T>));
+  KS h;
+  std::memcpy(&h, hpin_, sizeof(KS));
   return h;
 }
 
@@ -572,38 +756,61 @@ void Krylov::spmv(const double *val, const double *x, double *y) {
   MS_CHECK_LAUNCH();
 }
 
-double Krylov::true_residual(const double *val, const double *b, const double *x) {
-  KS<double> h{};
-  h.tol = -1.0;
-  MS_HIP(hipMemcpyAsync(st_.p, &h, sizeof(h), hipMemcpyHostToDevice, stream()));
-  hipLaunchKernelGGL(k_bnorm<double>, dim3(nblk_), dim3(kBlock), 0, stream(), n, b, st_.p, part_.p);
-  hipLaunchKernelGGL(k_residual<double>, dim3(nblk_), dim3(kBlock), 0, stream(), n, matof(pat), val, b, x, r_.p,
-                     st_.p, part_.p);
+double Krylov::time_spmv(const double *val, bool fp32, int reps) {
+  Mat A = matof(pat);
+  const dim3 G(nblk_), B(kBlock);
+  hipStream_t S = stream();
+  KS h{};
+  MS_HIP(hipMemcpyAsync(st_.p, &h, sizeof(h), hipMemcpyHostToDevice, S));
+  if (fp32) {
+    ensure32();
+    if (val32_.n < (size_t)pat->nstore) val32_.alloc(pat->nstore);
+    hipLaunchKernelGGL(k_mat32_diff, G, B, 0, S, n, A, val, val32_.p);
+    MS_HIP(hipMemsetAsync(p32_.p, 0, n * sizeof(float), S));
+  } else {
+    MS_HIP(hipMemsetAsync(p_.p, 0, n * sizeof(double), S));
+  }
+  auto launch = [&]() {
+    if (fp32)
+      hipLaunchKernelGGL(k_pcg_spmv<float>, G, B, 0, S, n, A, val32_.p, p32_.p, q32_.p, st_.p, part32_.p);
+    else
+      hipLaunchKernelGGL(k_pcg_spmv<double>, G, B, 0, S, n, A, val, p_.p, q_.p, st_.p, part_.p);
+  };
+  for (int i = 0; i < 3; i++) launch();
+  hipEvent_t e0, e1;
+  MS_HIP(hipEventCreate(&e0));
+  MS_HIP(hipEventCreate(&e1));
+  MS_HIP(hipEventRecord(e0, S));
+  for (int i = 0; i < reps; i++) launch();
+  MS_HIP(hipEventRecord(e1, S));
+  MS_HIP(hipEventSynchronize(e1));
+  float ms = 0.0f;
+  MS_HIP(hipEventElapsedTime(&ms, e0, e1));
+  MS_HIP(hipEventDestroy(e0));
+  MS_HIP(hipEventDestroy(e1));
   MS_CHECK_LAUNCH();
-  return read_state(st_.p).res;
+  return (double)ms / reps;
 }
 
-// ---- preconditioners ------------------------------------------------------
 int Krylov::setup_pc64(PcKind pc, const double *val) {
   Mat A = matof(pat);
+  if ((pc == PC_DILU || pc == PC_NATURAL_IC0) && !pat->symmetric)
+    fail(MS_EINVAL, "IC0 / DILU need a structurally symmetric pattern");
   if (pc == PC_JACOBI) {
-    hipLaunchKernelGGL(k_extract_diag<double>, dim3(nblk_), dim3(kBlock), 0, stream(), n, A, val, dinv_.p);
+    hipLaunchKernelGGL(k_extract_diag<double>, dim3(nblk_), dim3(kBlock), 0, stream(), n, A, val, dg_.p);
     MS_CHECK_LAUNCH();
     return 0;
   }
-  if ((pc == PC_DILU || pc == PC_NATURAL_IC0) && !pat->symmetric)
-    fail(MS_EINVAL, "IC0 / DILU need a structurally symmetric pattern");
   if (pc == PC_DILU) {
     for (size_t c = 0; c + 1 < pat->color_ptr.size(); c++) {
       int r0 = pat->color_ptr[c], r1 = pat->color_ptr[c + 1];
       if (r1 > r0)
         hipLaunchKernelGGL(k_dilu_setup<double>, dim3(nblocks(r1 - r0)), dim3(kBlock), 0, stream(), r0, r1, A,
-                           pat->d_tpos.p, val, dinv_.p);
+                           pat->d_tpos.p, val, dg_.p);
     }
     MS_CHECK_LAUNCH();
     return 0;
   }
-  // natural-order IC0 / ILU0 on a private copy (the reference factors a copy per call)
   if (!pat->has_levels) fail(MS_EINVAL, "natural-order factorization needs level schedules");
   if (fac_.n < (size_t)pat->nstore) fac_.alloc(pat->nstore);
   MS_HIP(hipMemcpyAsync(fac_.p, val, pat->nstore * sizeof(double), hipMemcpyDeviceToDevice, stream()));
@@ -625,51 +832,14 @@ int Krylov::setup_pc64(PcKind pc, const double *val) {
   return h;
 }
 
-void Krylov::apply_pc64(PcKind pc, const double *val, const double *in, double *out) {
-  Mat A = matof(pat);
-  const int *done = &st_.p->done;
-  if (pc == PC_JACOBI) {
-    hipLaunchKernelGGL(k_jacobi<double>, dim3(nblk_), dim3(kBlock), 0, stream(), n, in, dinv_.p, out, done);
-  } else if (pc == PC_DILU) {
-    const auto &cp = pat->color_ptr;
-    int nc = (int)cp.size() - 1;
-    for (int c = 0; c < nc; c++)
-      if (cp[c + 1] > cp[c])
-        hipLaunchKernelGGL(k_dilu_fwd<double>, dim3(nblocks(cp[c + 1] - cp[c])), dim3(kBlock), 0, stream(), cp[c],
-                           cp[c + 1], A, val, dinv_.p, in, out, done);
-    for (int c = nc - 1; c >= 0; c--)
-      if (cp[c + 1] > cp[c])
-        hipLaunchKernelGGL(k_dilu_bwd<double>, dim3(nblocks(cp[c + 1] - cp[c])), dim3(kBlock), 0, stream(), cp[c],
-                           cp[c + 1], A, val, dinv_.p, out, done);
-  } else {
-    const int ic0 = pc == PC_NATURAL_IC0;
-    for (size_t l = 0; l + 1 < pat->lev_ptr_f.size(); l++) {
-      int a = pat->lev_ptr_f[l], b = pat->lev_ptr_f[l + 1];
-      hipLaunchKernelGGL(k_lower_level<double>, dim3(nblocks(b - a)), dim3(kBlock), 0, stream(),
-                         pat->d_lev_rows_f.p + a, b - a, A, fac_.p, ic0 ? in : in, out, ic0, done);
-    }
-    for (size_t l = 0; l + 1 < pat->lev_ptr_b.size(); l++) {
-      int a = pat->lev_ptr_b[l], b = pat->lev_ptr_b[l + 1];
-      hipLaunchKernelGGL(k_upper_level<double>, dim3(nblocks(b - a)), dim3(kBlock), 0, stream(),
-                         pat->d_lev_rows_b.p + a, b - a, A, pat->d_tpos.p, fac_.p, out, ic0, done);
-    }
-  }
-  MS_CHECK_LAUNCH();
-}
-
 void Krylov::setup_pc32(PcKind pc, const double *val) {
   Mat A = matof(pat);
-  if (pc == PC_JACOBI) {
-    hipLaunchKernelGGL(k_extract_diag<float>, dim3(nblk_), dim3(kBlock), 0, stream(), n, A, val32_.p, dinv32_.p);
-  } else if (pc == PC_DILU) {
-    for (size_t c = 0; c + 1 < pat->color_ptr.size(); c++) {
-      int r0 = pat->color_ptr[c], r1 = pat->color_ptr[c + 1];
-      if (r1 > r0)
-        hipLaunchKernelGGL(k_dilu_setup<float>, dim3(nblocks(r1 - r0)), dim3(kBlock), 0, stream(), r0, r1, A,
-                           pat->d_tpos.p, val32_.p, dinv32_.p);
-    }
+  ensure32();
+  if (pc == PC_JACOBI || pc == PC_DILU) {
+    // the diagonal slot of val32 holds row sums: take d from the FP64 matrix
+    setup_pc64(pc, val);
+    hipLaunchKernelGGL(k_to_float, dim3(nblk_), dim3(kBlock), 0, stream(), (long long)n, dg_.p, dg32_.p);
   } else {
-    // factor in FP64, keep the factor in FP32
     int e = setup_pc64(pc, val);
     if (e) fail(MS_EINVAL, "factorization failed in mixed-precision setup");
     if (fac32_.n < (size_t)pat->nstore) fac32_.alloc(pat->nstore);
@@ -679,51 +849,85 @@ void Krylov::setup_pc32(PcKind pc, const double *val) {
   MS_CHECK_LAUNCH();
 }
 
-void Krylov::apply_pc32(PcKind pc, const float *in, float *out) {
+template <class T>
+void Krylov::apply_pc(PcKind pc, const T *val, const T *fac, const T *dg, const T *in, T *out, const int *done) {
   Mat A = matof(pat);
-  const int *done = &st32_.p->done;
   if (pc == PC_JACOBI) {
-    hipLaunchKernelGGL(k_jacobi<float>, dim3(nblk_), dim3(kBlock), 0, stream(), n, in, dinv32_.p, out, done);
+    hipLaunchKernelGGL(k_jacobi<T>, dim3(nblk_), dim3(kBlock), 0, stream(), n, in, dg, out, done);
   } else if (pc == PC_DILU) {
     const auto &cp = pat->color_ptr;
-    int nc = (int)cp.size() - 1;
-    for (int c = 0; c < nc; c++)
+    const int nc = (int)cp.size() - 1;
+    if (cp[1] > cp[0])
+      hipLaunchKernelGGL(k_dilu_first<T>, dim3(nblocks(cp[1] - cp[0])), dim3(kBlock), 0, stream(), cp[0], cp[1], dg,
+                         in, out, done);
+    for (int c = 1; c < nc; c++)
       if (cp[c + 1] > cp[c])
-        hipLaunchKernelGGL(k_dilu_fwd<float>, dim3(nblocks(cp[c + 1] - cp[c])), dim3(kBlock), 0, stream(), cp[c],
-                           cp[c + 1], A, val32_.p, dinv32_.p, in, out, done);
-    for (int c = nc - 1; c >= 0; c--)
+        hipLaunchKernelGGL(k_dilu_fwd<T>, dim3(nblocks(cp[c + 1] - cp[c])), dim3(kBlock), 0, stream(), cp[c],
+                           cp[c + 1], A, val, dg, in, out, (int)(c == nc - 1), done, (T *)nullptr);
+    for (int c = nc - 2; c >= 0; c--)
       if (cp[c + 1] > cp[c])
-        hipLaunchKernelGGL(k_dilu_bwd<float>, dim3(nblocks(cp[c + 1] - cp[c])), dim3(kBlock), 0, stream(), cp[c],
-                           cp[c + 1], A, val32_.p, dinv32_.p, out, done);
+        hipLaunchKernelGGL(k_dilu_bwd<T>, dim3(nblocks(cp[c + 1] - cp[c])), dim3(kBlock), 0, stream(), cp[c],
+                           cp[c + 1], A, val, dg, in, out, done, (T *)nullptr);
   } else {
     const int ic0 = pc == PC_NATURAL_IC0;
     for (size_t l = 0; l + 1 < pat->lev_ptr_f.size(); l++) {
       int a = pat->lev_ptr_f[l], b = pat->lev_ptr_f[l + 1];
-      hipLaunchKernelGGL(k_lower_level<float>, dim3(nblocks(b - a)), dim3(kBlock), 0, stream(),
-                         pat->d_lev_rows_f.p + a, b - a, A, fac32_.p, in, out, ic0, done);
+      hipLaunchKernelGGL(k_lower_level<T>, dim3(nblocks(b - a)), dim3(kBlock), 0, stream(), pat->d_lev_rows_f.p + a,
+                         b - a, A, fac, in, out, ic0, done);
     }
     for (size_t l = 0; l + 1 < pat->lev_ptr_b.size(); l++) {
       int a = pat->lev_ptr_b[l], b = pat->lev_ptr_b[l + 1];
-      hipLaunchKernelGGL(k_upper_level<float>, dim3(nblocks(b - a)), dim3(kBlock), 0, stream(),
-                         pat->d_lev_rows_b.p + a, b - a, A, pat->d_tpos.p, fac32_.p, out, ic0, done);
+      hipLaunchKernelGGL(k_upper_level<T>, dim3(nblocks(b - a)), dim3(kBlock), 0, stream(), pat->d_lev_rows_b.p + a,
+                         b - a, A, pat->d_tpos.p, fac, out, ic0, done);
     }
   }
   MS_CHECK_LAUNCH();
 }
 
-// ---- PCG ---------------------------------------------------------------------
+// DILU sweeps of colours 1.. with fused r.z partials (one section per launch);
+// returns the number of partials written.
+template <class T>
+static int dilu_sweeps(const SellPattern *pat, Mat A, const T *val, const T *dg, const T *r, T *z, const int *done,
+                       T *part) {
+  const auto &cp = pat->color_ptr;
+  const int ncol = (int)cp.size() - 1;
+  T *sec = part;
+  int nparts = 0;
+  for (int c = 1; c < ncol; c++)
+    if (cp[c + 1] > cp[c]) {
+      int gb = nblocks(cp[c + 1] - cp[c]);
+      hipLaunchKernelGGL(k_dilu_fwd<T>, dim3(gb), dim3(kBlock), 0, stream(), cp[c], cp[c + 1], A, val, dg, r, z,
+                         (int)(c == ncol - 1), done, sec);
+      sec += gb;
+      nparts += gb;
+    }
+  for (int c = ncol - 2; c >= 0; c--)
+    if (cp[c + 1] > cp[c]) {
+      int gb = nblocks(cp[c + 1] - cp[c]);
+      hipLaunchKernelGGL(k_dilu_bwd<T>, dim3(gb), dim3(kBlock), 0, stream(), cp[c], cp[c + 1], A, val, dg, r, z, done,
+                         sec);
+      sec += gb;
+      nparts += gb;
+    }
+  return nparts;
+}
+
+// ---- PCG (FP64, reference recurrences) -------------------------------------
 SolveResult Krylov::pcg(const double *val, const double *b, double *x, double tol, int maxit, PcKind pc) {
   SolveResult R;
   Mat A = matof(pat);
   const dim3 G(nblk_), B(kBlock);
   hipStream_t S = stream();
-  KS<double> h{};
+  KS *st = st_.p;
+  double *part = part_.p;
+  KS h{};
   h.tol = tol;
   h.maxit = maxit;
-  MS_HIP(hipMemcpyAsync(st_.p, &h, sizeof(h), hipMemcpyHostToDevice, S));
-  hipLaunchKernelGGL(k_bnorm<double>, G, B, 0, S, n, b, st_.p, part_.p);
+  MS_HIP(hipMemcpyAsync(st, &h, sizeof(h), hipMemcpyHostToDevice, S));
+  hipLaunchKernelGGL(k_sumsq<double>, G, B, 0, S, n, b, part);
+  finish<OpSum, 1>(nblk_, part, FinBnorm{st});
   MS_CHECK_LAUNCH();
-  KS<double> hs = read_state(st_.p);
+  KS hs = read_state<double>(st);
   if (hs.bnrm == 0.0) {
     MS_HIP(hipMemsetAsync(x, 0, n * sizeof(double), S));
     return R;
@@ -733,31 +937,43 @@ SolveResult Krylov::pcg(const double *val, const double *b, double *x, double to
     R.ierr = fe + 10;
     return R;
   }
-  hipLaunchKernelGGL(k_residual<double>, G, B, 0, S, n, A, val, b, x, r_.p, st_.p, part_.p);
+  hipLaunchKernelGGL(k_residual<double>, G, B, 0, S, n, A, val, b, x, r_.p, part);
+  finish<OpSum, 1>(nblk_, part, FinResidual{st});
   MS_CHECK_LAUNCH();
-  hs = read_state(st_.p);
+  hs = read_state<double>(st);
   R.res = hs.res;
   if (hs.done) return R;
   const int jac = pc == PC_JACOBI;
-  if (!jac) apply_pc64(pc, val, r_.p, z_.p);
-  hipLaunchKernelGGL(k_pcg_start<double>, G, B, 0, S, n, r_.p, z_.p, dinv_.p, jac, p_.p, st_.p, part_.p);
-  int enq = 0, chunk = 4;
+  const int *done = &st->done;
+  if (!jac) apply_pc<double>(pc, val, fac_.p, dg_.p, r_.p, z_.p, done);
+  hipLaunchKernelGGL(k_pcg_start<double>, G, B, 0, S, n, r_.p, z_.p, dg_.p, jac, p_.p, st, part);
+  finish<OpSum, 1>(nblk_, part, FinRz0{st});
+  const int mode = jac ? 1 : (pc == PC_DILU ? 2 : 0);
+  const int c0end = pat->color_ptr[1];
+  int enq = 0, chunk = 8;
   while (true) {
     int m = std::min(chunk, maxit - enq);
-    for (int k = 0; k < m; k++) {
-      hipLaunchKernelGGL(k_pcg_spmv<double>, G, B, 0, S, n, A, val, p_.p, q_.p, st_.p, part_.p);
-      hipLaunchKernelGGL(k_pcg_update<double>, G, B, 0, S, n, p_.p, q_.p, x, r_.p, dinv_.p, jac, st_.p, part_.p);
-      if (!jac) {
-        apply_pc64(pc, val, r_.p, z_.p);
-        hipLaunchKernelGGL(k_pcg_rz<double>, G, B, 0, S, n, r_.p, z_.p, st_.p, part_.p);
+    for (int it = 0; it < m; it++) {
+      hipLaunchKernelGGL(k_pcg_spmv<double>, G, B, 0, S, n, A, val, p_.p, q_.p, st, part);
+      finish<OpSum, 1>(nblk_, part, FinPap{st});
+      hipLaunchKernelGGL((k_pcg_update<double, double>), G, B, 0, S, n, p_.p, q_.p, x, r_.p, z_.p, dg_.p, mode, c0end,
+                         st, part);
+      finish<OpSum, 2>(nblk_, part, FinUpdate{st, jac});
+      if (mode == 2) {
+        int np = dilu_sweeps<double>(pat, A, val, dg_.p, r_.p, z_.p, done, part);
+        finish<OpSum, 1>(np, part, FinRz{st});
+      } else if (mode == 0) {
+        apply_pc<double>(pc, val, fac_.p, dg_.p, r_.p, z_.p, done);
+        hipLaunchKernelGGL(k_dot<double>, G, B, 0, S, n, r_.p, z_.p, st, part);
+        finish<OpSum, 1>(nblk_, part, FinRz{st});
       }
-      hipLaunchKernelGGL(k_pcg_p<double>, G, B, 0, S, n, r_.p, z_.p, dinv_.p, jac, p_.p, st_.p);
+      hipLaunchKernelGGL(k_pcg_p<double>, G, B, 0, S, n, r_.p, z_.p, dg_.p, jac, p_.p, st);
     }
     MS_CHECK_LAUNCH();
     enq += m;
-    hs = read_state(st_.p);
+    hs = read_state<double>(st);
     if (hs.done || enq >= maxit) break;
-    chunk = std::min(chunk * 2, 64);
+    chunk = std::min(chunk * 2, 128);
   }
   R.it = hs.it;
   R.res = hs.res;
@@ -775,13 +991,16 @@ SolveResult Krylov::bicgstab(const double *val, const double *b, double *x, doub
   Mat A = matof(pat);
   const dim3 G(nblk_), B(kBlock);
   hipStream_t S = stream();
-  KS<double> h{};
+  KS *st = st_.p;
+  double *part = part_.p;
+  KS h{};
   h.tol = tol;
   h.maxit = maxit;
-  MS_HIP(hipMemcpyAsync(st_.p, &h, sizeof(h), hipMemcpyHostToDevice, S));
-  hipLaunchKernelGGL(k_bnorm<double>, G, B, 0, S, n, b, st_.p, part_.p);
+  MS_HIP(hipMemcpyAsync(st, &h, sizeof(h), hipMemcpyHostToDevice, S));
+  hipLaunchKernelGGL(k_sumsq<double>, G, B, 0, S, n, b, part);
+  finish<OpSum, 1>(nblk_, part, FinBnorm{st});
   MS_CHECK_LAUNCH();
-  KS<double> hs = read_state(st_.p);
+  KS hs = read_state<double>(st);
   if (hs.bnrm == 0.0) {
     MS_HIP(hipMemsetAsync(x, 0, n * sizeof(double), S));
     return R;
@@ -791,30 +1010,38 @@ SolveResult Krylov::bicgstab(const double *val, const double *b, double *x, doub
     R.ierr = fe + 10;
     return R;
   }
-  hipLaunchKernelGGL(k_residual<double>, G, B, 0, S, n, A, val, b, x, r_.p, st_.p, part_.p);
-  hipLaunchKernelGGL(k_bi_init<double>, G, B, 0, S, n, r_.p, r0_.p, p_.p, q_.p, st_.p);
+  hipLaunchKernelGGL(k_residual<double>, G, B, 0, S, n, A, val, b, x, r_.p, part);
+  finish<OpSum, 1>(nblk_, part, FinResidual{st});
+  hipLaunchKernelGGL(k_bi_init<double>, G, B, 0, S, n, r_.p, r0_.p, p_.p, q_.p, st);
   MS_CHECK_LAUNCH();
-  hs = read_state(st_.p);
+  hs = read_state<double>(st);
   R.res = hs.res;
   if (hs.done) return R;
   double *v = q_.p;
+  const int *done = &st->done;
   int enq = 0, chunk = 2;
   while (true) {
     int m = std::min(chunk, maxit - enq);
     for (int k = 0; k < m; k++) {
-      hipLaunchKernelGGL(k_bi_rho<double>, G, B, 0, S, n, r_.p, r0_.p, st_.p, part_.p);
-      hipLaunchKernelGGL(k_bi_p<double>, G, B, 0, S, n, r_.p, v, p_.p, st_.p);
-      apply_pc64(pc, val, p_.p, ph_.p);
-      hipLaunchKernelGGL(k_bi_spmv_alpha<double>, G, B, 0, S, n, A, val, ph_.p, r0_.p, v, st_.p, part_.p);
-      hipLaunchKernelGGL(k_bi_s<double>, G, B, 0, S, n, r_.p, v, s_.p, st_.p, part_.p);
-      hipLaunchKernelGGL(k_bi_early<double>, G, B, 0, S, n, ph_.p, x, st_.p, part_.p);
-      apply_pc64(pc, val, s_.p, sh_.p);
-      hipLaunchKernelGGL(k_bi_spmv_omega<double>, G, B, 0, S, n, A, val, sh_.p, s_.p, t_.p, st_.p, part_.p);
-      hipLaunchKernelGGL(k_bi_xr<double>, G, B, 0, S, n, ph_.p, sh_.p, s_.p, t_.p, x, r_.p, st_.p, part_.p);
+      hipLaunchKernelGGL(k_dot<double>, G, B, 0, S, n, r_.p, r0_.p, st, part);
+      finish<OpSum, 1>(nblk_, part, FinBiRho{st});
+      hipLaunchKernelGGL(k_bi_p<double>, G, B, 0, S, n, r_.p, v, p_.p, st);
+      apply_pc<double>(pc, val, fac_.p, dg_.p, p_.p, ph_.p, done);
+      hipLaunchKernelGGL(k_bi_spmv_dot<double>, G, B, 0, S, n, A, val, ph_.p, r0_.p, v, st, part);
+      finish<OpSum, 1>(nblk_, part, FinBiAlpha{st});
+      hipLaunchKernelGGL(k_bi_s<double>, G, B, 0, S, n, r_.p, v, s_.p, st, part);
+      finish<OpSum, 1>(nblk_, part, FinBiS{st});
+      hipLaunchKernelGGL(k_bi_early<double>, G, B, 0, S, n, ph_.p, x, st);
+      hipLaunchKernelGGL(k_done2_to_1, dim3(1), dim3(64), 0, S, st);
+      apply_pc<double>(pc, val, fac_.p, dg_.p, s_.p, sh_.p, done);
+      hipLaunchKernelGGL(k_bi_spmv_omega<double>, G, B, 0, S, n, A, val, sh_.p, s_.p, t_.p, st, part);
+      finish<OpSum, 2>(nblk_, part, FinBiOmega{st});
+      hipLaunchKernelGGL(k_bi_xr<double>, G, B, 0, S, n, ph_.p, sh_.p, s_.p, t_.p, x, r_.p, st, part);
+      finish<OpSum, 1>(nblk_, part, FinBiXR{st});
     }
     MS_CHECK_LAUNCH();
     enq += m;
-    hs = read_state(st_.p);
+    hs = read_state<double>(st);
     if (hs.done || enq >= maxit) break;
     chunk = std::min(chunk * 2, 32);
   }
@@ -828,62 +1055,42 @@ SolveResult Krylov::bicgstab(const double *val, const double *b, double *x, doub
   return R;
 }
 
-// ---- FP32 inner solves ------------------------------------------------------------
-SolveResult Krylov::inner32(int solver, PcKind pc, double inner_tol, int maxit) {
+SolveResult Krylov::bicg32(PcKind pc, int maxit) {
   SolveResult R;
   Mat A = matof(pat);
   const dim3 G(nblk_), B(kBlock);
   hipStream_t S = stream();
-  KS<float> *st = st32_.p;
+  KS *st = st32_.p;
   float *part = part32_.p;
-  const int jac = pc == PC_JACOBI;
-  int enq = 0, chunk = 8;
-  KS<float> hs{};
-  if (solver == 0) {
-    // x32 = 0, r32 = scaled residual: z = M^-1 r, p = z, rz
-    if (!jac) apply_pc32(pc, r32_.p, z32_.p);
-    hipLaunchKernelGGL(k_pcg_start<float>, G, B, 0, S, n, r32_.p, z32_.p, dinv32_.p, jac, p32_.p, st, part);
-    while (true) {
-      int m = std::min(chunk, maxit - enq);
-      for (int k = 0; k < m; k++) {
-        hipLaunchKernelGGL(k_pcg_spmv<float>, G, B, 0, S, n, A, val32_.p, p32_.p, q32_.p, st, part);
-        hipLaunchKernelGGL(k_pcg_update<float>, G, B, 0, S, n, p32_.p, q32_.p, x32_.p, r32_.p, dinv32_.p, jac, st,
-                           part);
-        if (!jac) {
-          apply_pc32(pc, r32_.p, z32_.p);
-          hipLaunchKernelGGL(k_pcg_rz<float>, G, B, 0, S, n, r32_.p, z32_.p, st, part);
-        }
-        hipLaunchKernelGGL(k_pcg_p<float>, G, B, 0, S, n, r32_.p, z32_.p, dinv32_.p, jac, p32_.p, st);
-      }
-      MS_CHECK_LAUNCH();
-      enq += m;
-      hs = read_state(st);
-      if (hs.done || enq >= maxit) break;
-      chunk = std::min(chunk * 2, 64);
+  float *v = q32_.p;
+  const int *done = &st->done;
+  hipLaunchKernelGGL(k_bi_init<float>, G, B, 0, S, n, r32_.p, r032_.p, p32_.p, v, st);
+  int enq = 0, chunk = 4;
+  KS hs{};
+  while (true) {
+    int m = std::min(chunk, maxit - enq);
+    for (int k = 0; k < m; k++) {
+      hipLaunchKernelGGL(k_dot<float>, G, B, 0, S, n, r32_.p, r032_.p, st, part);
+      finish<OpSum, 1>(nblk_, part, FinBiRho{st});
+      hipLaunchKernelGGL(k_bi_p<float>, G, B, 0, S, n, r32_.p, v, p32_.p, st);
+      apply_pc<float>(pc, val32_.p, fac32_.p, dg32_.p, p32_.p, ph32_.p, done);
+      hipLaunchKernelGGL(k_bi_spmv_dot<float>, G, B, 0, S, n, A, val32_.p, ph32_.p, r032_.p, v, st, part);
+      finish<OpSum, 1>(nblk_, part, FinBiAlpha{st});
+      hipLaunchKernelGGL(k_bi_s<float>, G, B, 0, S, n, r32_.p, v, s32_.p, st, part);
+      finish<OpSum, 1>(nblk_, part, FinBiS{st});
+      hipLaunchKernelGGL(k_bi_early<float>, G, B, 0, S, n, ph32_.p, x32_.p, st);
+      hipLaunchKernelGGL(k_done2_to_1, dim3(1), dim3(64), 0, S, st);
+      apply_pc<float>(pc, val32_.p, fac32_.p, dg32_.p, s32_.p, sh32_.p, done);
+      hipLaunchKernelGGL(k_bi_spmv_omega<float>, G, B, 0, S, n, A, val32_.p, sh32_.p, s32_.p, t32_.p, st, part);
+      finish<OpSum, 2>(nblk_, part, FinBiOmega{st});
+      hipLaunchKernelGGL(k_bi_xr<float>, G, B, 0, S, n, ph32_.p, sh32_.p, s32_.p, t32_.p, x32_.p, r32_.p, st, part);
+      finish<OpSum, 1>(nblk_, part, FinBiXR{st});
     }
-  } else {
-    float *v = q32_.p;
-    hipLaunchKernelGGL(k_bi_init<float>, G, B, 0, S, n, r32_.p, r032_.p, p32_.p, v, st);
-    while (true) {
-      int m = std::min(chunk, maxit - enq);
-      for (int k = 0; k < m; k++) {
-        hipLaunchKernelGGL(k_bi_rho<float>, G, B, 0, S, n, r32_.p, r032_.p, st, part);
-        hipLaunchKernelGGL(k_bi_p<float>, G, B, 0, S, n, r32_.p, v, p32_.p, st);
-        apply_pc32(pc, p32_.p, ph32_.p);
-        hipLaunchKernelGGL(k_bi_spmv_alpha<float>, G, B, 0, S, n, A, val32_.p, ph32_.p, r032_.p, v, st, part);
-        hipLaunchKernelGGL(k_bi_s<float>, G, B, 0, S, n, r32_.p, v, s32_.p, st, part);
-        hipLaunchKernelGGL(k_bi_early<float>, G, B, 0, S, n, ph32_.p, x32_.p, st, part);
-        apply_pc32(pc, s32_.p, sh32_.p);
-        hipLaunchKernelGGL(k_bi_spmv_omega<float>, G, B, 0, S, n, A, val32_.p, sh32_.p, s32_.p, t32_.p, st, part);
-        hipLaunchKernelGGL(k_bi_xr<float>, G, B, 0, S, n, ph32_.p, sh32_.p, s32_.p, t32_.p, x32_.p, r32_.p, st,
-                           part);
-      }
-      MS_CHECK_LAUNCH();
-      enq += m;
-      hs = read_state(st);
-      if (hs.done || enq >= maxit) break;
-      chunk = std::min(chunk * 2, 32);
-    }
+    MS_CHECK_LAUNCH();
+    enq += m;
+    hs = read_state<float>(st);
+    if (hs.done || enq >= maxit) break;
+    chunk = std::min(chunk * 2, 32);
   }
   R.it = hs.it;
   R.ierr = hs.ierr;
@@ -891,53 +1098,125 @@ SolveResult Krylov::inner32(int solver, PcKind pc, double inner_tol, int maxit) 
   return R;
 }
 
+// ---- mixed precision ------------------------------------------------------------
 SolveResult Krylov::mixed(const double *val, const double *b, double *x, double tol, int maxit, PcKind pc,
                           int solver, double inner_tol) {
   SolveResult R;
   Mat A = matof(pat);
   const dim3 G(nblk_), B(kBlock);
   hipStream_t S = stream();
-  if (!have32_) {
-    for (DBuf<float> *v : {&r32_, &r032_, &p32_, &q32_, &z32_, &s32_, &t32_, &ph32_, &sh32_, &x32_, &dinv32_})
-      v->alloc(n);
-    part32_.alloc(2 * (size_t)nblk_ + 8);
-    st32_.alloc(1);
-    have32_ = true;
-  }
+  ensure32();
   if (val32_.n < (size_t)pat->nstore) val32_.alloc(pat->nstore);
-  KS<double> h{};
+  KS *st = st_.p;
+  double *part = part_.p;
+  KS h{};
   h.tol = tol;
   h.maxit = maxit;
-  MS_HIP(hipMemcpyAsync(st_.p, &h, sizeof(h), hipMemcpyHostToDevice, S));
-  hipLaunchKernelGGL(k_bnorm<double>, G, B, 0, S, n, b, st_.p, part_.p);
+  h.delta = inner_tol;
+  MS_HIP(hipMemcpyAsync(st, &h, sizeof(h), hipMemcpyHostToDevice, S));
+  hipLaunchKernelGGL(k_sumsq<double>, G, B, 0, S, n, b, part);
+  finish<OpSum, 1>(nblk_, part, FinBnorm{st});
   MS_CHECK_LAUNCH();
-  KS<double> hs = read_state(st_.p);
+  KS hs = read_state<double>(st);
   if (hs.bnrm == 0.0) {
     MS_HIP(hipMemsetAsync(x, 0, n * sizeof(double), S));
     return R;
   }
-  hipLaunchKernelGGL(k_to_float, dim3(nblocks(pat->nstore)), dim3(kBlock), 0, S, pat->nstore, val, val32_.p);
+  hipLaunchKernelGGL(k_mat32_diff, G, B, 0, S, n, A, val, val32_.p);
   setup_pc32(pc, val);
-  int total = 0;
-  while (true) {
-    hipLaunchKernelGGL(k_mp_residual, G, B, 0, S, n, A, val, b, x, r_.p, st_.p, part_.p);
-    MS_CHECK_LAUNCH();
-    hs = read_state(st_.p);
-    R.res = hs.res;
-    if (hs.done || total >= maxit) break;
-    int imax = maxit - total;
-    hipLaunchKernelGGL(k_mp_scale_down, G, B, 0, S, n, r_.p, st_.p, r32_.p, x32_.p, st32_.p, (float)inner_tol, imax);
-    SolveResult in = inner32(solver, pc, inner_tol, imax);
-    total += std::max(in.it, 1);
-    R.outer++;
-    hipLaunchKernelGGL(k_mp_correct, G, B, 0, S, n, x32_.p, st_.p, x);
-    if (in.ierr == 2 && in.it <= 1) {   // inner breakdown without progress
-      R.ierr = 2;
-      break;
+  hipLaunchKernelGGL(k_residual<double>, G, B, 0, S, n, A, val, b, x, r_.p, part);
+  finish<OpSum, 1>(nblk_, part, FinResidual{st});
+  MS_CHECK_LAUNCH();
+  hs = read_state<double>(st);
+  R.res = hs.res;
+  if (hs.done) return R;
+
+  if (solver == 1) {
+    // BiCGSTAB: FP32 inner solves inside FP64 iterative refinement
+    int total = 0;
+    while (true) {
+      int imax = maxit - total;
+      hipLaunchKernelGGL(k_mp_scale_down, G, B, 0, S, n, r_.p, st, r32_.p, x32_.p, st32_.p, kRefineTol, imax);
+      SolveResult in = bicg32(pc, imax);
+      total += std::max(in.it, 1);
+      R.outer++;
+      hipLaunchKernelGGL(k_mp_correct, G, B, 0, S, n, x32_.p, st, x);
+      hipLaunchKernelGGL(k_residual<double>, G, B, 0, S, n, A, val, b, x, r_.p, part);
+      finish<OpSum, 1>(nblk_, part, FinResidual{st});
+      MS_CHECK_LAUNCH();
+      hs = read_state<double>(st);
+      R.res = hs.res;
+      if (hs.done || total >= maxit || R.outer >= 200) break;
+      if (in.ierr == 2 && in.it <= 1) {
+        R.ierr = 2;
+        break;
+      }
     }
+    R.it = total;
+    if (R.ierr == 0 && !(R.res <= tol)) R.ierr = 1;
+    return R;
   }
-  R.it = total;
-  if (R.ierr == 0 && !(R.res <= tol)) R.ierr = 1;
+
+  // PCG with reliable updates: y accumulates alpha p in FP32 between updates
+  float *r = r32_.p, *z = z32_.p, *p = p32_.p, *q = q32_.p, *y = x32_.p;
+  float *part32 = part32_.p;
+  const float *v32 = val32_.p, *dg = dg32_.p, *fac = fac32_.p;
+  const int *done = &st->done;
+  const int jac = pc == PC_JACOBI;
+  MS_HIP(hipMemsetAsync(y, 0, n * sizeof(float), S));
+  hipLaunchKernelGGL(k_set_sigma, dim3(1), dim3(64), 0, S, st);
+  hipLaunchKernelGGL(k_scale_down, G, B, 0, S, n, r_.p, r, st);
+  if (!jac) apply_pc<float>(pc, v32, fac, dg, r, z, done);
+  hipLaunchKernelGGL(k_pcg_start<float>, G, B, 0, S, n, r, z, dg, jac, p, st, part32);
+  finish<OpSum, 1>(nblk_, part32, FinRz0{st});
+  const auto &cp = pat->color_ptr;
+  int enq = 0, chunk = 8;
+  while (true) {
+    int m = std::min(chunk, maxit - enq);
+    for (int it = 0; it < m; it++) {
+      hipLaunchKernelGGL(k_pcg_spmv<float>, G, B, 0, S, n, A, v32, p, q, st, part32);
+      finish<OpSum, 1>(nblk_, part32, FinPap{st});
+      hipLaunchKernelGGL((k_pcg_update<float, float>), G, B, 0, S, n, p, q, y, r, z, dg, 0, 0, st, part32);
+      finish<OpSum, 2>(nblk_, part32, FinUpdateRU{st});
+      hipLaunchKernelGGL(k_fold, G, B, 0, S, n, y, x, st);
+      hipLaunchKernelGGL(k_true_res, G, B, 0, S, n, A, val, b, x, r_.p, st, part);
+      finish<OpSum, 1>(nblk_, part, FinRU{st});
+      hipLaunchKernelGGL(k_ru_rescale, G, B, 0, S, n, r_.p, r, p, st);
+      if (jac) {
+        hipLaunchKernelGGL(k_jac_rz<float>, G, B, 0, S, n, r, dg, z, st, part32);
+        finish<OpSum, 1>(nblk_, part32, FinRz{st});
+      } else if (pc == PC_DILU) {
+        hipLaunchKernelGGL(k_dilu_first<float>, dim3(nblocks(cp[1] - cp[0])), B, 0, S, cp[0], cp[1], dg, r, z, done);
+        int np = dilu_sweeps<float>(pat, A, v32, dg, r, z, done, part32);
+        finish<OpSum, 1>(np, part32, FinRz{st});
+      } else {
+        apply_pc<float>(pc, v32, fac, dg, r, z, done);
+        hipLaunchKernelGGL(k_dot<float>, G, B, 0, S, n, r, z, st, part32);
+        finish<OpSum, 1>(nblk_, part32, FinRz{st});
+      }
+      hipLaunchKernelGGL(k_pcg_p<float>, G, B, 0, S, n, r, z, dg, 0, p, st);
+    }
+    MS_CHECK_LAUNCH();
+    enq += m;
+    hs = read_state<double>(st);
+    if (hs.done || enq >= maxit) break;
+    chunk = std::min(chunk * 2, 128);
+  }
+  if (!hs.done) {
+    KS force = hs;
+    force.ru = 1;
+    MS_HIP(hipMemcpyAsync(st, &force, sizeof(KS), hipMemcpyHostToDevice, S));
+    hipLaunchKernelGGL(k_fold, G, B, 0, S, n, y, x, st);
+    hipLaunchKernelGGL(k_true_res, G, B, 0, S, n, A, val, b, x, r_.p, st, part);
+    finish<OpSum, 1>(nblk_, part, FinRU{st});
+    MS_CHECK_LAUNCH();
+    hs = read_state<double>(st);
+  }
+  R.it = hs.it;
+  R.res = hs.res;
+  R.ierr = hs.ierr;
+  R.outer = hs.nru;
+  if (R.ierr == 0 && !hs.done) R.ierr = 1;
   return R;
 }
 

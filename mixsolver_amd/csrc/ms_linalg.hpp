@@ -4,19 +4,20 @@
 
 namespace ms {
 
-template <class T>
 struct KS {              // device-resident solver scalars (one per solve)
-  T bnrm, rz, pap, alpha, beta, res, rho, rho_old, omega, snrm, tt, rr;
+  double bnrm, rz, pap, alpha, beta, res, rho, rho_old, omega, snrm, tt, rr;
+  double rmax, delta;    // reliable updates: largest sloppy |r| since the last update, trigger ratio
+  double sigma, scale;   // FP32 iteration runs on r/sigma; scale = sigma_old/sigma_new at an update
+  double tol;
   int it, ierr, done, maxit;
-  T tol;
-  unsigned int ticket;
-  int pad;
+  int ru;                // reliable update requested for this iteration
+  int nru;               // reliable updates performed
 };
 
 struct SolveResult {
   int it = 0, ierr = 0;
   double res = 0.0;
-  int outer = 0;          // refinement cycles (mixed precision)
+  int outer = 0;          // refinement cycles / reliable updates (mixed precision)
 };
 
 // Preconditioner kinds used internally.
@@ -30,32 +31,36 @@ class Krylov {
   // x is the initial guess on entry.  Returns reference it/res/ierr.
   SolveResult pcg(const double *val, const double *b, double *x, double tol, int maxit, PcKind pc);
   SolveResult bicgstab(const double *val, const double *b, double *x, double tol, int maxit, PcKind pc);
-  // FP32 inner Krylov inside FP64 iterative refinement; res = FP64 true
-  // relative residual.  solver 0 = PCG, 1 = BiCGSTAB.
+  // Mixed precision.  PCG: FP32 matrix, preconditioner and Krylov vectors
+  // with FP32 accumulation, FP64 solution and reliable updates of the true
+  // FP64 residual.  BiCGSTAB: FP32 inner solves inside FP64 iterative
+  // refinement.  res is the FP64 true relative residual.
   SolveResult mixed(const double *val, const double *b, double *x, double tol, int maxit, PcKind pc,
                     int solver, double inner_tol);
-  // y = A x in FP64 (for tests / residual checks)
   void spmv(const double *val, const double *x, double *y);
-  double true_residual(const double *val, const double *b, const double *x);  // ||b-Ax||/||b||
-  long long launches = 0;  // kernel launches issued (diagnostics)
+  // average duration (ms) of the PCG SpMV+dot kernel on matrix val, measured
+  // with HIP events on the library stream over `reps` launches; fp32 selects
+  // the FP32 difference-form kernel of the mixed path.
+  double time_spmv(const double *val, bool fp32, int reps);
+  long long launches = 0;
 
   SellPattern *pat;
   int n;
 
  private:
-  int setup_pc64(PcKind pc, const double *val);           // returns factorization ierr
-  void apply_pc64(PcKind pc, const double *val, const double *in, double *out);
+  int setup_pc64(PcKind pc, const double *val);
   void setup_pc32(PcKind pc, const double *val);
-  void apply_pc32(PcKind pc, const float *in, float *out);
-  template <class T> KS<T> read_state(KS<T> *d);
-  SolveResult inner32(int solver, PcKind pc, double inner_tol, int maxit);
+  template <class T> void apply_pc(PcKind pc, const T *val, const T *fac, const T *dg, const T *in, T *out,
+                                   const int *done);
+  template <class T> KS read_state(KS *d);
+  SolveResult bicg32(PcKind pc, int maxit);
+  void ensure32();
 
-  DBuf<double> r_, r0_, p_, q_, z_, s_, t_, ph_, sh_, fac_, dinv_, part_;
-  DBuf<float> val32_, b32_, x32_, r32_, r032_, p32_, q32_, z32_, s32_, t32_, ph32_, sh32_, fac32_, dinv32_, part32_;
-  DBuf<KS<double>> st_;
-  DBuf<KS<float>> st32_;
+  DBuf<double> r_, r0_, p_, q_, z_, s_, t_, ph_, sh_, fac_, dg_, part_;
+  DBuf<float> val32_, r32_, r032_, p32_, q32_, z32_, s32_, t32_, ph32_, sh32_, x32_, fac32_, dg32_, part32_;
+  DBuf<KS> st_, st32_;
   DBuf<int> flag_;
-  void *hpin_ = nullptr;   // pinned host staging for state reads
+  void *hpin_ = nullptr;
   int nblk_;
   bool have32_ = false;
 };

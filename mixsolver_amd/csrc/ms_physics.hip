@@ -448,7 +448,7 @@ __global__ void k_rhiechow(DMesh m, DFields F, Ramp rp, double rho) {
 }
 
 // ----------------------------------------------------------- outflow rescale
-__global__ void k_outflow_sums(DMesh m, const double *flux, double *sums, double *part, unsigned *ticket) {
+__global__ void k_outflow_sums(DMesh m, const double *flux, double *part) {
   const int f = blockIdx.x * kBlock + threadIdx.x;
   double v[3] = {0.0, 0.0, 0.0};
   if (f < m.nf && m.c1[f] < 0 && m.fgrp[f] > 0) {
@@ -460,12 +460,17 @@ __global__ void k_outflow_sums(DMesh m, const double *flux, double *sums, double
       v[0] = -flux[f];
     }
   }
-  grid_reduce<OpSum, 3>(v, part, ticket, [&](double *a) {
-    sums[0] = a[0];
-    sums[1] = a[1];
-    sums[2] = a[2];
-  });
+  block_partials<OpSum, 3>(v, part);
 }
+
+struct FinStore {      // copy the reduced values into out[0..nv)
+  double *out;
+  int nv;
+  __device__ bool skip() const { return false; }
+  __device__ void operator()(const double *a) const {
+    for (int j = 0; j < nv; j++) out[j] = a[j];
+  }
+};
 
 __global__ void k_outflow_scale(DMesh m, double *flux, const double *sums) {
   const int f = blockIdx.x * kBlock + threadIdx.x;
@@ -596,8 +601,7 @@ __global__ void k_fill(long long n, double *x, double v) {
 
 // ------------------------------------------------------------ diagnostics
 // out: [0] max|imbalance| [1] max|u - uref| [2] max|T - Tref| [3] max norm2(u) [4] non-finite
-__global__ void k_diagnostics(DMesh m, DFields F, const double *uref, const double *Tref, double *out, double *part,
-                              unsigned *ticket) {
+__global__ void k_diagnostics(DMesh m, DFields F, const double *uref, const double *Tref, double *part) {
   const int r = blockIdx.x * kBlock + threadIdx.x;
   double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   if (r < m.nc) {
@@ -614,9 +618,7 @@ __global__ void k_diagnostics(DMesh m, DFields F, const double *uref, const doub
     bool bad = !isfinite(u[0]) || !isfinite(u[1]) || !isfinite(u[2]) || !isfinite(F.p[r]) || !isfinite(F.T[r]);
     v[4] = bad ? 1.0 : 0.0;
   }
-  grid_reduce<OpMax, 5>(v, part, ticket, [&](double *a) {
-    for (int j = 0; j < 5; j++) out[j] = a[j];
-  });
+  block_partials<OpMax, 5>(v, part);
 }
 
 }  // namespace
@@ -648,8 +650,9 @@ struct Driver {
   }
   void outflow() {
     if (!s.has_outflow) return;
-    hipLaunchKernelGGL(k_outflow_sums, gc(s.nf), dim3(kBlock), 0, stream(), s.dmesh(), s.flux.p, s.tmpc.p,
-                       s.part.p, s.ticket.p);
+    hipLaunchKernelGGL(k_outflow_sums, gc(s.nf), dim3(kBlock), 0, stream(), s.dmesh(), s.flux.p, s.part.p);
+    hipLaunchKernelGGL((k_finish<OpSum, 3, double, FinStore>), dim3(1), dim3(kBlock), 0, stream(), nblocks(s.nf),
+                       s.part.p, FinStore{s.tmpc.p, 3});
     hipLaunchKernelGGL(k_outflow_scale, gc(s.nf), dim3(kBlock), 0, stream(), s.dmesh(), s.flux.p, s.tmpc.p);
     MS_CHECK_LAUNCH();
   }
@@ -724,7 +727,9 @@ struct Driver {
   // diagnostics: returns out[5]
   void diagnostics(const double *uref, const double *Tref, double *out) {
     hipLaunchKernelGGL(k_diagnostics, gc(s.nc), dim3(kBlock), 0, stream(), s.dmesh(), s.dfields(), uref, Tref,
-                       s.tmpc.p, s.part.p, s.ticket.p);
+                       s.part.p);
+    hipLaunchKernelGGL((k_finish<OpMax, 5, double, FinStore>), dim3(1), dim3(kBlock), 0, stream(), nblocks(s.nc),
+                       s.part.p, FinStore{s.tmpc.p, 5});
     MS_CHECK_LAUNCH();
     MS_HIP(hipMemcpyAsync(out, s.tmpc.p, 5 * sizeof(double), hipMemcpyDeviceToHost, stream()));
     MS_HIP(hipStreamSynchronize(stream()));

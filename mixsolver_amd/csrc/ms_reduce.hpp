@@ -1,9 +1,13 @@
-// ms_reduce.hpp -- deterministic two-stage workgroup reductions (device code).
+// ms_reduce.hpp -- deterministic two-stage reductions (device code).
 //
-// Stage 1: wave64 shuffles, then LDS across the 4 waves of a workgroup.
-// Stage 2: every workgroup publishes its partials; the workgroup that takes
-// the last ticket sums all partials in a fixed order and runs the finisher on
-// one thread.  Results depend only on the grid size, never on scheduling.
+// Stage 1 (inside the producing kernel): wave64 shuffles, then LDS across
+// the 4 waves of a workgroup; thread 0 writes the workgroup's partials.
+// Stage 2 (k_finish, one workgroup): sums the partials in a fixed order and
+// applies the finisher functor (scalar recurrences of the solver) on one
+// thread.  The kernel boundary between the two stages provides the memory
+// ordering, so no device-scope fence or atomic is needed: on a multi-XCD
+// CDNA device an agent-scope release fence in every workgroup would write
+// back the XCD's L2 each time.  Results depend only on the grid size.
 #pragma once
 #include "ms_common.hpp"
 
@@ -39,41 +43,34 @@ __device__ __forceinline__ void block_reduce(T (&v)[NV], T *lds) {
   __syncthreads();
 }
 
-template <class Op, int NV, class T, class Fin>
-__device__ __forceinline__ void grid_reduce(T (&v)[NV], T *partial, unsigned *ticket, Fin fin) {
+// Stage 1: one partial per workgroup (NV values at partial[blockIdx*NV]).
+template <class Op, int NV, class T>
+__device__ __forceinline__ void block_partials(T (&v)[NV], T *partial) {
   __shared__ T lds[NV * (kBlock / 64)];
-  __shared__ int last;
   block_reduce<Op, NV, T>(v, lds);
-  if (threadIdx.x == 0) {
+  if (threadIdx.x == 0)
 #pragma unroll
     for (int j = 0; j < NV; j++) partial[blockIdx.x * NV + j] = v[j];
-    __threadfence();
-    unsigned t = atomicAdd(ticket, 1u);
-    last = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
+}
+
+// Stage 2: launched with one workgroup of kBlock threads.
+template <class Op, int NV, class T, class Fin>
+__global__ void k_finish(int nparts, const T *partial, Fin fin) {
+  if (fin.skip()) return;
   T acc[NV];
 #pragma unroll
   for (int j = 0; j < NV; j++) acc[j] = Op::template init<T>();
-  const volatile T *vp = partial;
-  for (int b = threadIdx.x; b < (int)gridDim.x; b += kBlock)
+  for (int b = threadIdx.x; b < nparts; b += kBlock)
 #pragma unroll
-    for (int j = 0; j < NV; j++) acc[j] = Op::apply(acc[j], (T)vp[b * NV + j]);
+    for (int j = 0; j < NV; j++) acc[j] = Op::apply(acc[j], partial[b * NV + j]);
+  __shared__ T lds[NV * (kBlock / 64)];
   block_reduce<Op, NV, T>(acc, lds);
-  if (threadIdx.x == 0) {
-    fin(acc);
-    *ticket = 0u;
-    __threadfence();
-  }
+  if (threadIdx.x == 0) fin(acc);
 }
 
-}  // namespace ms
-
-namespace ms {
 __device__ __forceinline__ double tabs(double x) { return fabs(x); }
 __device__ __forceinline__ float tabs(float x) { return fabsf(x); }
 __device__ __forceinline__ double tsqrt(double x) { return sqrt(x); }
 __device__ __forceinline__ float tsqrt(float x) { return sqrtf(x); }
+
 }  // namespace ms

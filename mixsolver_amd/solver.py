@@ -120,6 +120,13 @@ class Case:
         check(_lib.lib().ms_get_history(self.h, dp(out), n.value, C.byref(n)))
         return out[: n.value * 8].reshape(-1, 8)
 
+    def time_kernel(self, fp32=True, reps=50):
+        """(average ms, algorithmic bytes) of the PCG SpMV kernel on the
+        current matrix (HIP events on the library stream)."""
+        t, b = C.c_double(0.0), C.c_double(0.0)
+        check(_lib.lib().ms_time_kernel(self.h, 0 if fp32 else 1, int(reps), C.byref(t), C.byref(b)))
+        return t.value, b.value
+
     # per-call routines ---------------------------------------------------------
     def compute_gradients(self):
         check(_lib.lib().ms_compute_gradients(self.h))

@@ -31,6 +31,7 @@
 
 #include <float.h>
 #include <math.h>
+#include <time.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -53,7 +54,21 @@ struct or_case {
   int ts_order;
   double *hist;
   int nhist, caphist;
+  double t_total, t_ppe;      /* seconds in the last driver call / in its p' solves */
+  long long ppe_iters;        /* PCG iterations in the last driver call */
 };
+
+static double now_s(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+void or_timing(or_case *cs, double *out3) {
+  out3[0] = cs->t_total;
+  out3[1] = cs->t_ppe;
+  out3[2] = (double)cs->ppe_iters;
+}
 
 /* ---------------------------------------------------------------- helpers */
 
@@ -1272,10 +1287,13 @@ static int ppe_solve(or_case *cs, double *val, double *rhs, double *pp, int mpi,
   int ierr; double resl;
   or_ppe_assembly(cs, val, rhs);
   for (int c = 0; c < cs->nc; c++) pp[c] = 0.0;
+  double t0 = now_s();
   if (ct->ppe_precond == 1)
     or_cg_ic0(cs->nc, cs->row_ptr, cs->col_idx, val, rhs, pp, ct->lin_tol, ct->lin_max, itl, &resl, &ierr, mpi);
   else
     or_cg_jacobi(cs->nc, cs->row_ptr, cs->col_idx, val, rhs, pp, ct->lin_tol, ct->lin_max, itl, &resl, &ierr, mpi);
+  cs->t_ppe += now_s() - t0;
+  cs->ppe_iters += *itl > ct->lin_max ? ct->lin_max : *itl;
   return ierr;
 }
 
@@ -1360,6 +1378,9 @@ int or_piso_run(or_case *cs, int nsteps, int mpi) {
   double *ucomp = xcalloc(nc, 8);
   int ier = 0;
   cs->nhist = 0;
+  cs->t_ppe = 0.0;
+  cs->ppe_iters = 0;
+  double t_start = now_s();
   int n_steps = ct->n_time_max > 1 ? ct->n_time_max : 1;
   if (nsteps > 0) n_steps = nsteps;
   double flux_ref = flux_reference(cs);
@@ -1405,6 +1426,7 @@ int or_piso_run(or_case *cs, int nsteps, int mpi) {
     if (nonfinite(cs)) { ier = 60; break; }
     push_hist(cs, imbal, du_max, umax, cfl, itl_max, ressum / 3.0);
   }
+  cs->t_total = now_s() - t_start;
   free(val); free(rhs); free(ap); free(pp); free(dpp); free(imb); free(ucomp);
   return ier;
 }

@@ -102,6 +102,9 @@ int or_piso_run(or_case *, int nsteps, int mpi);
 /* per-step diagnostics of the last driver call: n_steps x 6
  * (imbal, du_max, umax, cfl_max, lin_it_max, lin_res_mean) */
 const double *or_history(or_case *, int *nrows);
+/* wall time of the last PISO call, time inside its p' solves, and the
+ * number of p' PCG iterations it performed (CPU-baseline extrapolation) */
+void or_timing(or_case *, double *out3);
 
 /* linear solvers on a 1-based CSR (csr_t), mod_uns_linsolver(_mpi).f90 */
 void or_cg_jacobi(int n, const int *rp, const int *ci, const double *v,

