@@ -119,6 +119,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--size", default="400x100x50")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--coarse", type=int, default=256, help="aggregates of the p' coarse space")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU runs are not part "
@@ -131,21 +132,30 @@ def main():
     ctl = control_text(nx)
     u0 = initial_field(mesh, ny, nz, nx)
     base = dict(gpu_ppe_precond=2, gpu_mom_precond=2)
+    two = dict(base, gpu_coarse=args.coarse)
 
-    mixed, t_mixed, lat_mixed, h_mixed = run_variant(Case, mesh, ctl, u0, dict(base, gpu_precision=1,
+    mixed, t_mixed, lat_mixed, h_mixed = run_variant(Case, mesh, ctl, u0, dict(two, gpu_precision=1,
                                                                              gpu_inner_tol=0.1),
                                                      args.steps, args.warmup)
     # roofline of the dominant kernel (FP32 SpMV of the mixed p' solve) on the last p' matrix
     t_k, b_k = mixed.time_kernel(fp32=True, reps=100)
     t_k64, b_k64 = mixed.time_kernel(fp32=False, reps=50)
     mixed.close()
+    fp64c, t_fp64c, lat_fp64c, h_fp64c = run_variant(Case, mesh, ctl, u0, dict(two, gpu_precision=0),
+                                                     args.steps, args.warmup)
+    fp64c.close()
     fp64, t_fp64, lat_fp64, h_fp64 = run_variant(Case, mesh, ctl, u0, dict(base, gpu_precision=0),
                                                  args.steps, args.warmup)
     fp64.close()
 
     sps_mixed = args.steps / t_mixed
     sps_fp64 = args.steps / t_fp64
+    sps_fp64c = args.steps / t_fp64c
     cells = mesh.ncells
+
+    def pct(lat, q):
+        return float(np.percentile(lat, q) * 1e3)
+
     out = {
         "metric": METRIC,
         "value": sps_mixed,
@@ -160,19 +170,28 @@ def main():
         "dtype": "f32 inner + f64 refinement",
         "data": "synthetic",
         "config": {"workload": f"channel {nx}x{ny}x{nz} hex ({cells} cells), transient PISO, 2 correctors, "
-                               "p' PCG + multicolour DILU to 1e-10, momentum BiCGSTAB + DILU to 1e-10",
+                               f"p' PCG + multicolour DILU + {args.coarse}-aggregate coarse correction to "
+                               "1e-10, momentum BiCGSTAB + DILU to 1e-10",
                    "cells": cells, "faces": mesh.nfaces, "time_scheme": "BDF2", "parallelism": "1 GPU"},
+        # the plain FP64 device path: reference recurrences, one-level DILU
         "fp64_steps_per_s": sps_fp64,
         "fp64_ms_per_step": 1e3 * t_fp64 / args.steps,
         "speedup_mixed_vs_fp64": sps_mixed / sps_fp64,
-        "p50_step_ms": float(np.percentile(lat_mixed, 50) * 1e3),
-        "p99_step_ms": float(np.percentile(lat_mixed, 99) * 1e3),
-        "fp64_p50_step_ms": float(np.percentile(lat_fp64, 50) * 1e3),
-        "fp64_p99_step_ms": float(np.percentile(lat_fp64, 99) * 1e3),
+        # the same two-level solver in FP64 (isolates the precision gain)
+        "fp64_two_level_steps_per_s": sps_fp64c,
+        "speedup_mixed_vs_fp64_two_level": sps_mixed / sps_fp64c,
+        "p50_step_ms": pct(lat_mixed, 50),
+        "p99_step_ms": pct(lat_mixed, 99),
+        "fp64_p50_step_ms": pct(lat_fp64, 50),
+        "fp64_p99_step_ms": pct(lat_fp64, 99),
+        "fp64_two_level_p50_step_ms": pct(lat_fp64c, 50),
+        "fp64_two_level_p99_step_ms": pct(lat_fp64c, 99),
         "ppe_iters_per_step_mixed": float(np.mean(h_mixed[:, 6])),
         "ppe_iters_per_step_fp64": float(np.mean(h_fp64[:, 6])),
+        "ppe_iters_per_step_fp64_two_level": float(np.mean(h_fp64c[:, 6])),
         "ppe_final_res_mixed_max": float(np.max(h_mixed[:, 7])),
         "ppe_final_res_fp64_max": float(np.max(h_fp64[:, 7])),
+        "ppe_final_res_fp64_two_level_max": float(np.max(h_fp64c[:, 7])),
         "cell_iterations_per_s_mixed": cells * float(np.sum(h_mixed[:, 6])) / t_mixed,
         "roofline": {"bound": "hbm", "kernel": "k_pcg_spmv<float> (SELL-64, difference form, fused p.q)",
                      "achieved": b_k / (t_k * 1e-3) / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",

@@ -53,7 +53,7 @@ typedef struct {
   int gpu_ppe_solver;    /* 0 = PCG (reference), 1 = BiCGSTAB                */
   double gpu_inner_tol;  /* relative tolerance of each FP32 inner solve      */
   int gpu_reorder;       /* 0 = natural order, 1 = RCM                       */
-  int gpu_reserved;
+  int gpu_coarse;        /* p' PCG coarse space: number of aggregates, 0 = off (max 256) */
 } ms_ctrl_t;
 
 /* One boundary group (bcgroup_t, mod_uns_bc.f90:80-110), control order. */
@@ -88,7 +88,17 @@ int ms_system_create(ms_system **out, int nnodes, const double *x, int nfaces,
                      const int *c1, const int *zone, int ncells,
                      const ms_ctrl_t *ctrl, const ms_bcgroup_t *groups);
 int ms_system_destroy(ms_system *s);
-int ms_system_info(ms_system *s, long long *out8);  /* ncells nfaces nnz ncolors nlevels bytes_device reserved reserved */
+int ms_system_info(ms_system *s, long long *out8);  /* ncells nfaces nnz ncolors nlevels bytes_device
+                                                       nstore renumbered */
+/* p' coarse space of a system: k aggregates, nchunks row chunks; agg (may be
+ * NULL) receives the aggregate of every cell in the caller's numbering. */
+int ms_coarse_info(ms_system *s, int *k, int *nchunks, int *agg);
+
+/* Aggregates of the p' coarse space (gpu_coarse > 0): recursive coordinate
+ * bisection of the cell centres xc (3,n) into k groups of equal size (+-1),
+ * splitting each group along the longest side of its bounding box.  agg[i]
+ * in 0..k-1.  Host only (no device needed); no reference counterpart. */
+int ms_coarse_aggregates(int n, const double *xc, int k, int *agg);
 
 /* fields_t components by name: "u"(3,nc) "p" "T" "T_s" "flux"(nf) "apc"
  * "u_old" "u_old_old" "T_old" "T_old_old" "gp"(3,nc) "gu"(3,3,nc) "gt"

@@ -65,6 +65,8 @@ def lib():
         "ms_solve": (C.c_int, [C.c_int, _I, _I, _D, _D, _D, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int,
                                C.c_double, _I, _D, _I]),
         "ms_time_kernel": (C.c_int, [_V, C.c_int, C.c_int, _D, _D]),
+        "ms_coarse_info": (C.c_int, [_V, _I, _I, _I]),
+        "ms_coarse_aggregates": (C.c_int, [C.c_int, _D, C.c_int, _I]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -14,6 +14,9 @@ stay compatible with the reference reader:
                    falls below this fraction of its value at the last FP64 update
                    (default 0.1)
   gpu_reorder      none | rcm                     (default none)
+  gpu_coarse       p' PCG two-level preconditioner: number of cell aggregates
+                   (recursive coordinate bisection), 0 = off, at most 256
+                   (default 0)
 
 ``ilu0``/``ic0`` select natural-order factorizations (parity with the
 reference); ``dilu`` selects the multicolour DILU preconditioner.
@@ -63,7 +66,7 @@ class MsCtrl(C.Structure):
         ("gpu_ppe_precond", C.c_int), ("gpu_mom_precond", C.c_int), ("gpu_precision", C.c_int),
         ("gpu_ppe_solver", C.c_int),
         ("gpu_inner_tol", C.c_double),
-        ("gpu_reorder", C.c_int), ("gpu_reserved", C.c_int),
+        ("gpu_reorder", C.c_int), ("gpu_coarse", C.c_int),
     ]
 
 
@@ -100,7 +103,7 @@ def default_ctrl() -> MsCtrl:
     c.cp, c.k_cond, c.tref, c.beta = 1005.0, 0.026, 0.0, 0.0
     c.boussinesq, c.thermal_model, c.inlet_ramp = 0, 0, 1
     c.gpu_ppe_precond, c.gpu_mom_precond, c.gpu_precision = -1, 1, 0
-    c.gpu_ppe_solver, c.gpu_inner_tol, c.gpu_reorder = 0, 0.1, 0
+    c.gpu_ppe_solver, c.gpu_inner_tol, c.gpu_reorder, c.gpu_coarse = 0, 0.1, 0, 0
     return c
 
 
@@ -268,6 +271,10 @@ def parse_control(text: str, warn=True):
                 ctrl.gpu_precision = PRECISION[val.lower()]
             elif key == "gpu_reorder":
                 ctrl.gpu_reorder = {"none": 0, "rcm": 1}[val.lower()]
+            elif key == "gpu_coarse":
+                ctrl.gpu_coarse = int(val)
+                if not 0 <= ctrl.gpu_coarse <= 256:
+                    raise ValueError(val)
             elif key in ("out_format", "out_fmt", "out_cells", "out_cell", "porous_model",
                          "restart", "restart_file", "partition_file", "dump_file",
                          "mesh_scale", "cell_zone"):

@@ -140,6 +140,30 @@ int ms_system_info(ms_system *h, long long *o) {
   });
 }
 
+int ms_coarse_info(ms_system *h, int *k, int *nchunks, int *agg) {
+  return guarded([&] {
+    System &s = sys(h);
+    const Coarse &C = s.pat.coarse;
+    if (k) *k = C.k;
+    if (nchunks) *nchunks = C.nchunks;
+    if (agg) {
+      std::vector<int> dev(s.nc, 0);
+      for (int c = 0; c < C.nchunks; c++)
+        for (int r = C.chunk_ptr[c]; r < C.chunk_ptr[c + 1]; r++) dev[r] = C.chunk_agg[c];
+      for (int i = 0; i < s.nc; i++) agg[i] = dev[s.o2n[i]];
+    }
+  });
+}
+
+int ms_coarse_aggregates(int n, const double *xc, int k, int *agg) {
+  return guarded([&] {
+    if (n <= 0 || !xc || !agg || k < 1 || k > n) fail(MS_EINVAL, "ms_coarse_aggregates: need 1 <= k <= n");
+    std::vector<double> v(xc, xc + 3 * (size_t)n);
+    std::vector<int> a = rcb_aggregates(n, v, k);
+    std::copy(a.begin(), a.end(), agg);
+  });
+}
+
 int ms_set_field(ms_system *h, const char *name, const double *host, long long n) {
   return guarded([&] {
     System &s = sys(h);

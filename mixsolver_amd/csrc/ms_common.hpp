@@ -77,6 +77,31 @@ constexpr int kSlice = 64;    // SELL slice height = one wavefront
 
 inline int nblocks(long long n, int b = kBlock) { return (int)((n + b - 1) / b); }
 
+// ------------------------------------------------------ coarse space
+// Piecewise-constant coarse space for the pressure-correction PCG: cells are
+// grouped into k compact aggregates (recursive coordinate bisection of the
+// cell centres) and the preconditioner becomes M^-1 r + Z E^-1 Z^T r with
+// Z the aggregate indicator matrix and E = Z^T A Z.  The device numbering
+// keeps every aggregate contiguous inside each colour, so the rows are cut
+// into chunks of at most kBlock rows that each lie in one aggregate; the
+// cell-wise PCG kernels run one workgroup per chunk, which turns Z^T r into
+// per-workgroup partial sums and Z nu into a broadcast.
+constexpr int kMaxAgg = 256;
+struct Coarse {
+  int k = 0;                          // number of aggregates (0 = none)
+  int nchunks = 0;
+  std::vector<int> chunk_ptr;         // nchunks+1 row boundaries
+  std::vector<int> chunk_agg;         // aggregate of each chunk
+  std::vector<int> agg_cptr, agg_cidx; // chunks of each aggregate (CSR)
+  std::vector<int> e_ptr, e_key;      // E gather map: entry key a*k+b, slots [e_ptr[i], e_ptr[i+1])
+  std::vector<int> e_slot;            // matrix slots, grouped by key, ascending
+  DBuf<int> d_chunk_ptr, d_chunk_agg, d_agg_cptr, d_agg_cidx, d_e_ptr, d_e_key, d_e_slot;
+  DBuf<double> d_E, d_nu;             // dense E (k x k), coarse solution nu (k)
+  DBuf<float> d_Einv;                 // dense E^-1 (k x k), FP32
+  void upload();
+  size_t device_bytes() const;
+};
+
 // ------------------------------------------------- SELL-64 sparse pattern
 // Row r lives in slice s = r/64, lane l = r%64; its k-th entry (columns in
 // ascending order, as csr_t) sits at off[s] + k*64 + l.  Padding entries use
@@ -96,6 +121,7 @@ struct SellPattern {
   std::vector<int> color_ptr;
   bool has_levels = false;
   bool symmetric = true;             // structurally symmetric (tpos complete)
+  Coarse coarse;                     // coarse space of the pressure-correction solves
   long long slot(int r, int k) const { return off[r / kSlice] + (long long)k * kSlice + (r % kSlice); }
   void build_from_csr(int n, const std::vector<int> &rp0, const std::vector<int> &ci0); // 0-based CSR
   void build_levels();
@@ -103,13 +129,6 @@ struct SellPattern {
   size_t device_bytes() const;
 };
 
-// --------------------------------------------------- Krylov solver state
-struct KState {
-  double bnrm, rz, pap, alpha, beta, res, rho, rho_old, omega, snrm, tt, rr;
-  double rnorm, tol, scale, spare;
-  int it, ierr, done, maxit;
-  unsigned int ticket;
-  int early, inner_it, outer_it;
-};
+
 
 }  // namespace ms

@@ -37,6 +37,35 @@ struct Mat {
   const int *rowlen, *col, *diagk;
 };
 
+// Row set of a cell-wise kernel: plain blocks of kBlock rows, or (coarse
+// space active) one chunk of rows of a single aggregate per workgroup.
+struct Rows {
+  const int *cptr;   // chunk row boundaries, nullptr = plain blocks
+  const int *cagg;   // aggregate of each chunk
+  int n;
+  __device__ __forceinline__ bool get(int &i, int &a) const {
+    if (cptr) {
+      const int r0 = cptr[blockIdx.x], r1 = cptr[blockIdx.x + 1];
+      i = r0 + (int)threadIdx.x;
+      a = cagg[blockIdx.x];
+      return i < r1;
+    }
+    i = blockIdx.x * kBlock + threadIdx.x;
+    a = 0;
+    return i < n;
+  }
+};
+
+// Coarse solve inside a finisher: c = Z^T r from the chunk partials (last
+// of the NV values), nu = E^-1 c, st->cnu = c.nu.
+struct CoDev {
+  int k;             // 0 = no coarse space
+  const int *agg_cptr, *agg_cidx;
+  const float *Einv;
+  double *nu;
+  KS *st;
+};
+
 template <class T>
 __device__ __forceinline__ T row_dot(int r, const Mat &A, const T *val, const T *x) {
   const long long base = A.off[r >> 6] + (r & 63);
@@ -104,10 +133,20 @@ struct FinResidual {
     if (st->res <= st->tol) st->done = 1;
   }
 };
+// r.z of the two-level preconditioner: the coarse part c.nu is added when
+// the coarse space is active (kept separate so the one-level path is
+// unchanged bit for bit).
+__device__ __forceinline__ double rz_total(const KS *st, double a) { return st->coarse ? a + st->cnu : a; }
+
+struct FinNop {
+  KS *st;
+  __device__ bool skip() const { return st->done == 1; }
+  template <class T> __device__ void operator()(const T *) const {}
+};
 struct FinRz0 {
   KS *st;
   __device__ bool skip() const { return st->done != 0; }
-  template <class T> __device__ void operator()(const T *a) const { st->rz = a[0]; }
+  template <class T> __device__ void operator()(const T *a) const { st->rz = rz_total(st, (double)a[0]); }
 };
 struct FinPap {
   KS *st;
@@ -139,8 +178,9 @@ struct FinUpdate {
         st->ierr = 2;
         st->done = 1;
       } else {
-        st->beta = (double)a[1] / st->rz;
-        st->rz = a[1];
+        const double rzn = rz_total(st, (double)a[1]);
+        st->beta = rzn / st->rz;
+        st->rz = rzn;
       }
     }
   }
@@ -154,27 +194,53 @@ struct FinRz {
       st->ierr = 2;
       st->done = 1;
     } else {
-      st->beta = (double)a[0] / st->rz;
-      st->rz = a[0];
+      const double rzn = rz_total(st, (double)a[0]);
+      st->beta = rzn / st->rz;
+      st->rz = rzn;
     }
   }
 };
 // Mixed PCG runs the FP32 iteration on r/sigma (sigma = |r| at the last
 // reliable update) so its vectors stay O(1): a p' right-hand side can be
 // ~1e-14, which would otherwise push FP32 products into the denormal range.
+// Sloppy residual check of the FP32 iteration.  When the recursive residual
+// meets the tolerance the iteration pauses (done = 2) and the host runs a
+// reliable update at once; when it has dropped below delta times its largest
+// value since the last reliable update, an update is flagged (ru = 1) and
+// the host applies it at the end of the chunk.  `trigger` is 0 on the pass
+// that follows a paused update.  jac: Jacobi r.z from the second partial.
 struct FinUpdateRU {
   KS *st;
+  int jac, trigger;
   __device__ bool skip() const { return st->done != 0; }
   template <class T> __device__ void operator()(const T *a) const {
     const double rn = sqrt((double)a[0]);              // scaled sloppy |r|
     st->res = rn * st->sigma / st->bnrm;
-    if (st->res <= st->tol || rn < st->delta * st->rmax) st->ru = 1;
+    if (trigger) {
+      if (st->res <= st->tol) {            // pause: the host verifies convergence in FP64
+        st->ru = 1;
+        st->done = 2;
+        return;
+      }
+      if (rn < st->delta * st->rmax) st->ru = 1;   // applied at the end of the chunk
+    }
     if (rn > st->rmax) st->rmax = rn;
+    if (!trigger) st->ru = 0;              // rescale pass done
+    if (jac) {
+      if (fabs(st->rz) < kTiny) {
+        st->ierr = 2;
+        st->done = 1;
+      } else {
+        const double rzn = rz_total(st, (double)a[1]);
+        st->beta = rzn / st->rz;
+        st->rz = rzn;
+      }
+    }
   }
 };
 struct FinRU {         // after x += sigma*y and r = b - A x in FP64
   KS *st;
-  __device__ bool skip() const { return st->done != 0 || st->ru != 1; }
+  __device__ bool skip() const { return st->done == 1 || st->ru != 1; }
   template <class T> __device__ void operator()(const T *a) const {
     const double rt = sqrt((double)a[0]);
     st->rr = a[0];
@@ -189,6 +255,7 @@ struct FinRU {         // after x += sigma*y and r = b - A x in FP64
     st->sigma = rt;
     st->rmax = 1.0;
     st->ru = 2;                            // k_ru_rescale pending
+    st->done = 0;                          // resume the iteration
   }
 };
 struct FinBiRho {
@@ -290,17 +357,29 @@ __global__ void k_residual(int n, Mat A, const T *val, const T *b, const T *x, T
 }
 
 // ------------------------------------------------------------------- PCG
+// p = z (+ nu of the row's aggregate); partial r.z of the one-level part
 template <class T>
-__global__ void k_pcg_start(int n, const T *r, const T *z, const T *dg, int jac, T *p, const KS *st, T *part) {
+__global__ void k_pcg_start(Rows R, const T *r, const T *z, const T *dg, int jac, T *p, const double *nu,
+                            const KS *st, T *part) {
   if (st->done) return;
-  int i = blockIdx.x * kBlock + threadIdx.x;
+  int i, a;
   T v[1] = {T(0)};
-  if (i < n) {
+  if (R.get(i, a)) {
     T zi = jac ? r[i] / dg[i] : z[i];
-    p[i] = zi;
+    p[i] = nu ? zi + (T)nu[a] : zi;
     v[0] = r[i] * zi;
   }
   block_partials<OpSum, 1>(v, part);
+}
+
+// partials (0, 0, sum r) per row set: Z^T r of the initial residual
+template <class T>
+__global__ void k_rsum(Rows R, const T *r, const KS *st, T *part) {
+  if (st->done == 1) return;
+  int i, a;
+  T v[3] = {T(0), T(0), T(0)};
+  if (R.get(i, a)) v[2] = r[i];
+  block_partials<OpSum, 3>(v, part);
 }
 
 template <class T>
@@ -316,32 +395,37 @@ __global__ void k_pcg_spmv(int n, Mat A, const T *val, const T *p, T *q, const K
   block_partials<OpSum, 1>(v, part);
 }
 
-// x += alpha p; r -= alpha q; partials r.r and (Jacobi) r.(r/d).
+// x += alpha p; r -= alpha q; partials r.r, (Jacobi) r.(r/d) and sum r
+// (Z^T r when the rows are aggregate chunks).
 // mode 0 plain, 1 Jacobi, 2 DILU: z = r/d on the first colour block [0,c0end)
 template <class T, class X>
-__global__ void k_pcg_update(int n, const T *p, const T *q, X *x, T *r, T *z, const T *dg, int mode, int c0end,
+__global__ void k_pcg_update(Rows R, const T *p, const T *q, X *x, T *r, T *z, const T *dg, int mode, int c0end,
                              const KS *st, T *part) {
   if (st->done) return;
-  int i = blockIdx.x * kBlock + threadIdx.x;
-  T v[2] = {T(0), T(0)};
-  if (i < n) {
+  int i, a;
+  T v[3] = {T(0), T(0), T(0)};
+  if (R.get(i, a)) {
     const T alpha = (T)st->alpha;
     x[i] = x[i] + (X)(alpha * p[i]);
     T ri = r[i] - alpha * q[i];
     r[i] = ri;
     v[0] = ri * ri;
+    v[2] = ri;
     if (mode == 1) v[1] = ri * (ri / dg[i]);
     else if (mode == 2 && i < c0end) z[i] = ri / dg[i];
   }
-  block_partials<OpSum, 2>(v, part);
+  block_partials<OpSum, 3>(v, part);
 }
 
+// p = z (+ nu of the row's aggregate) + beta p
 template <class T>
-__global__ void k_pcg_p(int n, const T *r, const T *z, const T *dg, int jac, T *p, const KS *st) {
+__global__ void k_pcg_p(Rows R, const T *r, const T *z, const T *dg, int jac, T *p, const double *nu,
+                        const KS *st) {
   if (st->done) return;
-  int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i < n) {
+  int i, a;
+  if (R.get(i, a)) {
     T zi = jac ? r[i] / dg[i] : z[i];
+    if (nu) zi = zi + (T)nu[a];
     p[i] = zi + (T)st->beta * p[i];
   }
 }
@@ -370,7 +454,7 @@ __global__ void k_jac_rz(int n, const T *r, const T *dg, T *z, const KS *st, T *
 // ------------------------------------------------------- reliable updates
 // x64 += sigma * y; y = 0   (only when a reliable update is pending)
 __global__ void k_fold(int n, float *y, double *x, const KS *st) {
-  if (st->done || st->ru != 1) return;
+  if (st->done == 1 || st->ru != 1) return;
   int i = blockIdx.x * kBlock + threadIdx.x;
   if (i < n) {
     x[i] = x[i] + st->sigma * (double)y[i];
@@ -380,7 +464,7 @@ __global__ void k_fold(int n, float *y, double *x, const KS *st) {
 // r64 = b - A64 x and r64.r64 (FP64)
 __global__ void k_true_res(int n, Mat A, const double *val, const double *b, const double *x, double *r64,
                            const KS *st, double *part) {
-  if (st->done || st->ru != 1) return;
+  if (st->done == 1 || st->ru != 1) return;
   int i = blockIdx.x * kBlock + threadIdx.x;
   double v[1] = {0.0};
   if (i < n) {
@@ -390,14 +474,23 @@ __global__ void k_true_res(int n, Mat A, const double *val, const double *b, con
   }
   block_partials<OpSum, 1>(v, part);
 }
-// r32 = r64 / sigma; p *= scale   (after FinRU moved to the new scale)
-__global__ void k_ru_rescale(int n, const double *r64, float *r32, float *p, const KS *st) {
+// r32 = r64 / sigma; p *= scale (after FinRU moved to the new scale), with
+// the partials and colour-0 DILU values of k_pcg_update for the new r
+__global__ void k_ru_rescale(Rows R, const double *r64, float *r32, float *p, float *z, const float *dg, int mode,
+                             int c0end, const KS *st, float *part) {
   if (st->done || st->ru != 2) return;
-  int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i < n) {
-    r32[i] = (float)(r64[i] / st->sigma);
+  int i, a;
+  float v[3] = {0.0f, 0.0f, 0.0f};
+  if (R.get(i, a)) {
+    const float ri = (float)(r64[i] / st->sigma);
+    r32[i] = ri;
     p[i] = p[i] * (float)st->scale;
+    v[0] = ri * ri;
+    v[2] = ri;
+    if (mode == 1) v[1] = ri * (ri / dg[i]);
+    else if (mode == 2 && i < c0end) z[i] = ri / dg[i];
   }
+  block_partials<OpSum, 3>(v, part);
 }
 // initial scaled residual r32 = r64 / sigma
 __global__ void k_scale_down(int n, const double *r64, float *r32, const KS *st) {
@@ -687,6 +780,107 @@ __global__ void k_dilu_bwd(int r0, int r1, Mat A, const T *val, const T *dg, con
   if (part) block_partials<OpSum, 1>(v, part);
 }
 
+// ------------------------------------------------------------ coarse space
+// Finisher of the update reduction with the coarse solve (one workgroup):
+// the scalar sums as k_finish, then c_a = sum of the last partial over the
+// chunks of aggregate a (fixed order), nu = E^-1 c and c.nu, before the
+// functor runs on thread 0.
+template <int NV, class T, class Fin>
+__global__ void k_finish_co(int nparts, const T *partial, Fin fin, CoDev co) {
+  if (fin.skip()) return;
+  T acc[NV];
+#pragma unroll
+  for (int j = 0; j < NV; j++) acc[j] = T(0);
+  for (int b = threadIdx.x; b < nparts; b += kBlock)
+#pragma unroll
+    for (int j = 0; j < NV; j++) acc[j] = acc[j] + partial[b * NV + j];
+  __shared__ T lds[NV * (kBlock / 64)];
+  block_reduce<OpSum, NV, T>(acc, lds);
+  if (co.k > 0) {
+    __shared__ double c[kMaxAgg];
+    for (int a = threadIdx.x; a < co.k; a += kBlock) {
+      double s = 0.0;
+      for (int j = co.agg_cptr[a]; j < co.agg_cptr[a + 1]; j++) s = s + (double)partial[co.agg_cidx[j] * NV + NV - 1];
+      c[a] = s;
+    }
+    __syncthreads();
+    double prod[1] = {0.0};
+    for (int a = threadIdx.x; a < co.k; a += kBlock) {
+      double s = 0.0;
+      for (int b = 0; b < co.k; b++) s = s + (double)co.Einv[(size_t)b * co.k + a] * c[b];   // E^-1 symmetric
+      co.nu[a] = s;
+      prod[0] = prod[0] + s * c[a];
+    }
+    __shared__ double lds2[kBlock / 64];
+    block_reduce<OpSum, 1, double>(prod, lds2);
+    if (threadIdx.x == 0) co.st->cnu = prod[0];
+  }
+  if (threadIdx.x == 0) fin(acc);
+}
+
+__global__ void k_fill64(long long n, double *v, double x) {
+  long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) v[i] = x;
+}
+
+// E = Z^T A Z: one wavefront per nonzero (a,b) sums its matrix slots (listed
+// in ascending order) and writes E[a*k+b]; absent entries stay zero.
+__global__ void k_coarse_E(int nE, const int *e_ptr, const int *e_key, const int *e_slot, const double *val,
+                           double *E) {
+  const int e = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (e >= nE) return;
+  double s = 0.0;
+  for (int j = e_ptr[e] + lane; j < e_ptr[e + 1]; j += 64) s = s + val[e_slot[j]];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s = s + __shfl_down(s, o, 64);
+  if (lane == 0) E[e_key[e]] = s;
+}
+
+// In-place Gauss-Jordan inversion of the SPD matrix E (k <= 256, no
+// pivoting needed) in FP32, one workgroup of 1024 threads holding the
+// padded 256 x 256 matrix in registers: thread t owns column t % 256 of rows
+// t/256 + 4m, m = 0..63.  Pivot step j with pivot p = a_jj:
+//   a_jj <- 1/p,  a_jc <- a_jc/p,  a_rj <- -a_rj/p,  a_rc <- a_rc - a_rj a_jc/p.
+constexpr int kInvThreads = 1024;
+__global__ __launch_bounds__(kInvThreads) void k_coarse_inv(int k, const double *E, float *Einv) {
+  const int t = threadIdx.x, c = t & 255, rb = t >> 8;
+  float a[64];
+#pragma unroll
+  for (int m = 0; m < 64; m++) {
+    const int r = rb + 4 * m;
+    a[m] = (r < k && c < k) ? (float)E[(size_t)r * k + c] : (r == c ? 1.0f : 0.0f);
+  }
+  __shared__ float prow[256], pcol[256];
+  for (int j = 0; j < k; j++) {
+    const int mj = j >> 2, rj = j & 3;
+    if (rb == rj) {
+#pragma unroll
+      for (int m = 0; m < 64; m++)
+        if (m == mj) prow[c] = a[m];
+    }
+    if (c == j) {
+#pragma unroll
+      for (int m = 0; m < 64; m++) pcol[rb + 4 * m] = a[m];
+    }
+    __syncthreads();
+    const float inv = 1.0f / prow[j];
+    const float rowc = prow[c] * inv;
+#pragma unroll
+    for (int m = 0; m < 64; m++) {
+      const int r = rb + 4 * m;
+      if (r == j) a[m] = (c == j) ? inv : rowc;
+      else if (c == j) a[m] = -pcol[r] * inv;
+      else a[m] = a[m] - pcol[r] * rowc;
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int m = 0; m < 64; m++) {
+    const int r = rb + 4 * m;
+    if (r < k && c < k) Einv[(size_t)r * k + c] = a[m];
+  }
+}
+
 // --------------------------------------------------- mixed precision glue
 __global__ void k_mp_scale_down(int n, const double *r, const KS *st, float *r32, float *x32, KS *s32,
                                 double inner_tol, int inner_max) {
@@ -724,7 +918,7 @@ static void finish(int nparts, const T *part, Fin fin) {
 Krylov::Krylov(SellPattern *pat_) : pat(pat_), n(pat_->n) {
   nblk_ = nblocks(n);
   for (DBuf<double> *b : {&r_, &r0_, &p_, &q_, &z_, &s_, &t_, &ph_, &sh_, &dg_}) b->alloc(n);
-  part_.alloc(4 * ((size_t)nblk_ + 64) * 2);
+  part_.alloc(4 * ((size_t)std::max(nblk_, pat_->coarse.nchunks) + 64) * 2);
   st_.alloc(1);
   flag_.alloc(1);
   MS_HIP(hipHostMalloc(&hpin_, 4096, hipHostMallocDefault));
@@ -737,7 +931,7 @@ Krylov::~Krylov() {
 void Krylov::ensure32() {
   if (have32_) return;
   for (DBuf<float> *v : {&r32_, &r032_, &p32_, &q32_, &z32_, &s32_, &t32_, &ph32_, &sh32_, &x32_, &dg32_}) v->alloc(n);
-  part32_.alloc(4 * ((size_t)nblk_ + 64) * 2);
+  part32_.alloc(4 * ((size_t)std::max(nblk_, pat->coarse.nchunks) + 64) * 2);
   st32_.alloc(1);
   have32_ = true;
 }
@@ -913,16 +1107,44 @@ static int dilu_sweeps(const SellPattern *pat, Mat A, const T *val, const T *dg,
 }
 
 // ---- PCG (FP64, reference recurrences) -------------------------------------
-SolveResult Krylov::pcg(const double *val, const double *b, double *x, double tol, int maxit, PcKind pc) {
+static Rows rows_of(const SellPattern *pat, bool coarse) {
+  if (!coarse) return Rows{nullptr, nullptr, pat->n};
+  return Rows{pat->coarse.d_chunk_ptr.p, pat->coarse.d_chunk_agg.p, pat->n};
+}
+static CoDev codev_of(const SellPattern *pat, bool coarse, KS *st) {
+  if (!coarse) return CoDev{0, nullptr, nullptr, nullptr, nullptr, st};
+  const Coarse &C = pat->coarse;
+  return CoDev{C.k, C.d_agg_cptr.p, C.d_agg_cidx.p, C.d_Einv.p, C.d_nu.p, st};
+}
+template <int NV, class T, class Fin>
+static void finish_co(int nparts, const T *part, Fin fin, const CoDev &co) {
+  hipLaunchKernelGGL((k_finish_co<NV, T, Fin>), dim3(1), dim3(kBlock), 0, stream(), nparts, part, fin, co);
+}
+
+void Krylov::coarse_setup(const double *val) {
+  Coarse &C = pat->coarse;
+  const long long kk = (long long)C.k * C.k;
+  const int nE = (int)C.e_key.size();
+  hipLaunchKernelGGL(k_fill64, dim3(nblocks(kk)), dim3(kBlock), 0, stream(), kk, C.d_E.p, 0.0);
+  hipLaunchKernelGGL(k_coarse_E, dim3(nblocks(nE, kBlock / 64)), dim3(kBlock), 0, stream(), nE, C.d_e_ptr.p,
+                     C.d_e_key.p, C.d_e_slot.p, val, C.d_E.p);
+  hipLaunchKernelGGL(k_coarse_inv, dim3(1), dim3(kInvThreads), 0, stream(), C.k, C.d_E.p, C.d_Einv.p);
+  MS_CHECK_LAUNCH();
+}
+
+SolveResult Krylov::pcg(const double *val, const double *b, double *x, double tol, int maxit, PcKind pc,
+                        bool coarse) {
   SolveResult R;
   Mat A = matof(pat);
   const dim3 G(nblk_), B(kBlock);
   hipStream_t S = stream();
   KS *st = st_.p;
   double *part = part_.p;
+  coarse = coarse && pat->coarse.k > 0;
   KS h{};
   h.tol = tol;
   h.maxit = maxit;
+  h.coarse = coarse;
   MS_HIP(hipMemcpyAsync(st, &h, sizeof(h), hipMemcpyHostToDevice, S));
   hipLaunchKernelGGL(k_sumsq<double>, G, B, 0, S, n, b, part);
   finish<OpSum, 1>(nblk_, part, FinBnorm{st});
@@ -937,6 +1159,7 @@ SolveResult Krylov::pcg(const double *val, const double *b, double *x, double to
     R.ierr = fe + 10;
     return R;
   }
+  if (coarse) coarse_setup(val);
   hipLaunchKernelGGL(k_residual<double>, G, B, 0, S, n, A, val, b, x, r_.p, part);
   finish<OpSum, 1>(nblk_, part, FinResidual{st});
   MS_CHECK_LAUNCH();
@@ -945,9 +1168,17 @@ SolveResult Krylov::pcg(const double *val, const double *b, double *x, double to
   if (hs.done) return R;
   const int jac = pc == PC_JACOBI;
   const int *done = &st->done;
+  const Rows RW = rows_of(pat, coarse);
+  const dim3 GR(coarse ? pat->coarse.nchunks : nblk_);
+  const CoDev co = codev_of(pat, coarse, st);
+  const double *nu = coarse ? pat->coarse.d_nu.p : nullptr;
+  if (coarse) {
+    hipLaunchKernelGGL(k_rsum<double>, GR, B, 0, S, RW, r_.p, st, part);
+    finish_co<3>((int)GR.x, part, FinNop{st}, co);
+  }
   if (!jac) apply_pc<double>(pc, val, fac_.p, dg_.p, r_.p, z_.p, done);
-  hipLaunchKernelGGL(k_pcg_start<double>, G, B, 0, S, n, r_.p, z_.p, dg_.p, jac, p_.p, st, part);
-  finish<OpSum, 1>(nblk_, part, FinRz0{st});
+  hipLaunchKernelGGL(k_pcg_start<double>, GR, B, 0, S, RW, r_.p, z_.p, dg_.p, jac, p_.p, nu, st, part);
+  finish<OpSum, 1>((int)GR.x, part, FinRz0{st});
   const int mode = jac ? 1 : (pc == PC_DILU ? 2 : 0);
   const int c0end = pat->color_ptr[1];
   int enq = 0, chunk = 8;
@@ -956,9 +1187,9 @@ SolveResult Krylov::pcg(const double *val, const double *b, double *x, double to
     for (int it = 0; it < m; it++) {
       hipLaunchKernelGGL(k_pcg_spmv<double>, G, B, 0, S, n, A, val, p_.p, q_.p, st, part);
       finish<OpSum, 1>(nblk_, part, FinPap{st});
-      hipLaunchKernelGGL((k_pcg_update<double, double>), G, B, 0, S, n, p_.p, q_.p, x, r_.p, z_.p, dg_.p, mode, c0end,
-                         st, part);
-      finish<OpSum, 2>(nblk_, part, FinUpdate{st, jac});
+      hipLaunchKernelGGL((k_pcg_update<double, double>), GR, B, 0, S, RW, p_.p, q_.p, x, r_.p, z_.p, dg_.p, mode,
+                         c0end, st, part);
+      finish_co<3>((int)GR.x, part, FinUpdate{st, jac}, co);
       if (mode == 2) {
         int np = dilu_sweeps<double>(pat, A, val, dg_.p, r_.p, z_.p, done, part);
         finish<OpSum, 1>(np, part, FinRz{st});
@@ -967,7 +1198,7 @@ SolveResult Krylov::pcg(const double *val, const double *b, double *x, double to
         hipLaunchKernelGGL(k_dot<double>, G, B, 0, S, n, r_.p, z_.p, st, part);
         finish<OpSum, 1>(nblk_, part, FinRz{st});
       }
-      hipLaunchKernelGGL(k_pcg_p<double>, G, B, 0, S, n, r_.p, z_.p, dg_.p, jac, p_.p, st);
+      hipLaunchKernelGGL(k_pcg_p<double>, GR, B, 0, S, RW, r_.p, z_.p, dg_.p, jac, p_.p, nu, st);
     }
     MS_CHECK_LAUNCH();
     enq += m;
@@ -1100,7 +1331,7 @@ SolveResult Krylov::bicg32(PcKind pc, int maxit) {
 
 // ---- mixed precision ------------------------------------------------------------
 SolveResult Krylov::mixed(const double *val, const double *b, double *x, double tol, int maxit, PcKind pc,
-                          int solver, double inner_tol) {
+                          int solver, double inner_tol, bool coarse) {
   SolveResult R;
   Mat A = matof(pat);
   const dim3 G(nblk_), B(kBlock);
@@ -1157,54 +1388,85 @@ SolveResult Krylov::mixed(const double *val, const double *b, double *x, double 
     return R;
   }
 
-  // PCG with reliable updates: y accumulates alpha p in FP32 between updates
+  // PCG with reliable updates: y accumulates alpha p in FP32 between updates.
+  // The FP32 iteration pauses when its recursive residual meets the
+  // tolerance; a drop below delta times its running maximum only flags an
+  // update, which the host applies at the end of the chunk (after the
+  // p update: r and p are rescaled, the recurrence continues).
   float *r = r32_.p, *z = z32_.p, *p = p32_.p, *q = q32_.p, *y = x32_.p;
   float *part32 = part32_.p;
   const float *v32 = val32_.p, *dg = dg32_.p, *fac = fac32_.p;
   const int *done = &st->done;
   const int jac = pc == PC_JACOBI;
+  const int mode = jac ? 1 : (pc == PC_DILU ? 2 : 0);
+  const int c0end = pat->color_ptr[1];
+  coarse = coarse && pat->coarse.k > 0;
+  if (coarse) {
+    h.coarse = 1;
+    MS_HIP(hipMemcpyAsync(&st->coarse, &h.coarse, sizeof(int), hipMemcpyHostToDevice, S));
+    coarse_setup(val);
+  }
+  const Rows RW = rows_of(pat, coarse);
+  const dim3 GR(coarse ? pat->coarse.nchunks : nblk_);
+  const CoDev co = codev_of(pat, coarse, st);
+  const double *nu = coarse ? pat->coarse.d_nu.p : nullptr;
   MS_HIP(hipMemsetAsync(y, 0, n * sizeof(float), S));
   hipLaunchKernelGGL(k_set_sigma, dim3(1), dim3(64), 0, S, st);
   hipLaunchKernelGGL(k_scale_down, G, B, 0, S, n, r_.p, r, st);
+  if (coarse) {
+    hipLaunchKernelGGL(k_rsum<float>, GR, B, 0, S, RW, r, st, part32);
+    finish_co<3>((int)GR.x, part32, FinNop{st}, co);
+  }
   if (!jac) apply_pc<float>(pc, v32, fac, dg, r, z, done);
-  hipLaunchKernelGGL(k_pcg_start<float>, G, B, 0, S, n, r, z, dg, jac, p, st, part32);
-  finish<OpSum, 1>(nblk_, part32, FinRz0{st});
-  const auto &cp = pat->color_ptr;
-  int enq = 0, chunk = 8;
-  while (true) {
-    int m = std::min(chunk, maxit - enq);
+  hipLaunchKernelGGL(k_pcg_start<float>, GR, B, 0, S, RW, r, z, dg, jac, p, nu, st, part32);
+  finish<OpSum, 1>((int)GR.x, part32, FinRz0{st});
+  // preconditioner, r.z and p update: the part of an iteration after the
+  // update finisher
+  auto tail = [&]() {
+    if (mode == 2) {
+      int np = dilu_sweeps<float>(pat, A, v32, dg, r, z, done, part32);
+      finish<OpSum, 1>(np, part32, FinRz{st});
+    } else if (mode == 0) {
+      apply_pc<float>(pc, v32, fac, dg, r, z, done);
+      hipLaunchKernelGGL(k_dot<float>, G, B, 0, S, n, r, z, st, part32);
+      finish<OpSum, 1>(nblk_, part32, FinRz{st});
+    }
+    hipLaunchKernelGGL(k_pcg_p<float>, GR, B, 0, S, RW, r, z, dg, jac, p, nu, st);
+  };
+  auto reliable_update = [&](bool paused) {
+    hipLaunchKernelGGL(k_fold, G, B, 0, S, n, y, x, st);
+    hipLaunchKernelGGL(k_true_res, G, B, 0, S, n, A, val, b, x, r_.p, st, part);
+    finish<OpSum, 1>(nblk_, part, FinRU{st});
+    hipLaunchKernelGGL(k_ru_rescale, GR, B, 0, S, RW, r_.p, r, p, z, dg, mode, c0end, st, part32);
+    if (paused) {        // complete the interrupted iteration on the new r
+      finish_co<3>((int)GR.x, part32, FinUpdateRU{st, jac, 0}, co);
+      tail();
+    }
+    MS_CHECK_LAUNCH();
+  };
+  constexpr int kChunk = 16;   // reliable updates are applied at most this many iterations late
+  hs = read_state<double>(st);
+  while (hs.done != 1 && hs.it < maxit) {
+    const int m = std::min(kChunk, maxit - hs.it);
     for (int it = 0; it < m; it++) {
       hipLaunchKernelGGL(k_pcg_spmv<float>, G, B, 0, S, n, A, v32, p, q, st, part32);
       finish<OpSum, 1>(nblk_, part32, FinPap{st});
-      hipLaunchKernelGGL((k_pcg_update<float, float>), G, B, 0, S, n, p, q, y, r, z, dg, 0, 0, st, part32);
-      finish<OpSum, 2>(nblk_, part32, FinUpdateRU{st});
-      hipLaunchKernelGGL(k_fold, G, B, 0, S, n, y, x, st);
-      hipLaunchKernelGGL(k_true_res, G, B, 0, S, n, A, val, b, x, r_.p, st, part);
-      finish<OpSum, 1>(nblk_, part, FinRU{st});
-      hipLaunchKernelGGL(k_ru_rescale, G, B, 0, S, n, r_.p, r, p, st);
-      if (jac) {
-        hipLaunchKernelGGL(k_jac_rz<float>, G, B, 0, S, n, r, dg, z, st, part32);
-        finish<OpSum, 1>(nblk_, part32, FinRz{st});
-      } else if (pc == PC_DILU) {
-        hipLaunchKernelGGL(k_dilu_first<float>, dim3(nblocks(cp[1] - cp[0])), B, 0, S, cp[0], cp[1], dg, r, z, done);
-        int np = dilu_sweeps<float>(pat, A, v32, dg, r, z, done, part32);
-        finish<OpSum, 1>(np, part32, FinRz{st});
-      } else {
-        apply_pc<float>(pc, v32, fac, dg, r, z, done);
-        hipLaunchKernelGGL(k_dot<float>, G, B, 0, S, n, r, z, st, part32);
-        finish<OpSum, 1>(nblk_, part32, FinRz{st});
-      }
-      hipLaunchKernelGGL(k_pcg_p<float>, G, B, 0, S, n, r, z, dg, 0, p, st);
+      hipLaunchKernelGGL((k_pcg_update<float, float>), GR, B, 0, S, RW, p, q, y, r, z, dg, mode, c0end, st,
+                         part32);
+      finish_co<3>((int)GR.x, part32, FinUpdateRU{st, jac, 1}, co);
+      tail();
     }
     MS_CHECK_LAUNCH();
-    enq += m;
     hs = read_state<double>(st);
-    if (hs.done || enq >= maxit) break;
-    chunk = std::min(chunk * 2, 128);
+    if (hs.ru == 1 && hs.done != 1) {
+      reliable_update(hs.done == 2);
+      hs = read_state<double>(st);
+    }
   }
-  if (!hs.done) {
+  if (hs.done != 1) {    // exhaustion: fold y and report the true residual
     KS force = hs;
     force.ru = 1;
+    force.done = 0;
     MS_HIP(hipMemcpyAsync(st, &force, sizeof(KS), hipMemcpyHostToDevice, S));
     hipLaunchKernelGGL(k_fold, G, B, 0, S, n, y, x, st);
     hipLaunchKernelGGL(k_true_res, G, B, 0, S, n, A, val, b, x, r_.p, st, part);
@@ -1216,7 +1478,7 @@ SolveResult Krylov::mixed(const double *val, const double *b, double *x, double 
   R.res = hs.res;
   R.ierr = hs.ierr;
   R.outer = hs.nru;
-  if (R.ierr == 0 && !hs.done) R.ierr = 1;
+  if (R.ierr == 0 && hs.done != 1) R.ierr = 1;
   return R;
 }
 

@@ -8,8 +8,10 @@ struct KS {              // device-resident solver scalars (one per solve)
   double bnrm, rz, pap, alpha, beta, res, rho, rho_old, omega, snrm, tt, rr;
   double rmax, delta;    // reliable updates: largest sloppy |r| since the last update, trigger ratio
   double sigma, scale;   // FP32 iteration runs on r/sigma; scale = sigma_old/sigma_new at an update
+  double cnu;            // coarse part of r.z: (Z^T r) . E^-1 (Z^T r)
   double tol;
-  int it, ierr, done, maxit;
+  int it, ierr, done, maxit;  // done: 0 running, 1 finished, 2 paused for a reliable update
+  int coarse;            // coarse correction active
   int ru;                // reliable update requested for this iteration
   int nru;               // reliable updates performed
 };
@@ -29,14 +31,17 @@ class Krylov {
   ~Krylov();
   // FP64 solvers with the reference recurrences (mod_uns_linsolver_mpi.f90):
   // x is the initial guess on entry.  Returns reference it/res/ierr.
-  SolveResult pcg(const double *val, const double *b, double *x, double tol, int maxit, PcKind pc);
+  // coarse: add the aggregate coarse correction of pat->coarse to the
+  // preconditioner (PCG only).
+  SolveResult pcg(const double *val, const double *b, double *x, double tol, int maxit, PcKind pc,
+                  bool coarse = false);
   SolveResult bicgstab(const double *val, const double *b, double *x, double tol, int maxit, PcKind pc);
   // Mixed precision.  PCG: FP32 matrix, preconditioner and Krylov vectors
   // with FP32 accumulation, FP64 solution and reliable updates of the true
   // FP64 residual.  BiCGSTAB: FP32 inner solves inside FP64 iterative
   // refinement.  res is the FP64 true relative residual.
   SolveResult mixed(const double *val, const double *b, double *x, double tol, int maxit, PcKind pc,
-                    int solver, double inner_tol);
+                    int solver, double inner_tol, bool coarse = false);
   void spmv(const double *val, const double *x, double *y);
   // average duration (ms) of the PCG SpMV+dot kernel on matrix val, measured
   // with HIP events on the library stream over `reps` launches; fp32 selects
@@ -55,6 +60,7 @@ class Krylov {
   template <class T> KS read_state(KS *d);
   SolveResult bicg32(PcKind pc, int maxit);
   void ensure32();
+  void coarse_setup(const double *val);   // E = Z^T A Z and its inverse
 
   DBuf<double> r_, r0_, p_, q_, z_, s_, t_, ph_, sh_, fac_, dg_, part_;
   DBuf<float> val32_, r32_, r032_, p32_, q32_, z32_, s32_, t32_, ph32_, sh32_, x32_, fac32_, dg32_, part32_;

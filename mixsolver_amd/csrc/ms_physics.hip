@@ -705,9 +705,10 @@ struct Driver {
     const ms_ctrl_t &c = s.ctrl;
     PcKind pc = pc_of(ppe_eq ? c.gpu_ppe_precond : c.gpu_mom_precond, ppe_eq);
     int solver = ppe_eq ? c.gpu_ppe_solver : 1;
+    const bool coarse = ppe_eq && s.pat.coarse.k > 0;   // two-level preconditioner for p'
     if (c.gpu_precision == MS_MIXED)
-      return s.kry->mixed(s.val.p, s.rhs.p, x, c.lin_tol, c.lin_max, pc, solver, c.gpu_inner_tol);
-    if (solver == 0) return s.kry->pcg(s.val.p, s.rhs.p, x, c.lin_tol, c.lin_max, pc);
+      return s.kry->mixed(s.val.p, s.rhs.p, x, c.lin_tol, c.lin_max, pc, solver, c.gpu_inner_tol, coarse);
+    if (solver == 0) return s.kry->pcg(s.val.p, s.rhs.p, x, c.lin_tol, c.lin_max, pc, coarse);
     return s.kry->bicgstab(s.val.p, s.rhs.p, x, c.lin_tol, c.lin_max, pc);
   }
   // momentum predictor over the 3 components; returns failing comp (1..3) or 0

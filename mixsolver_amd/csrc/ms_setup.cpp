@@ -172,7 +172,7 @@ void SellPattern::upload() {
 
 size_t SellPattern::device_bytes() const {
   return d_off.bytes() + d_rowlen.bytes() + d_col.bytes() + d_diagk.bytes() + d_tpos.bytes() +
-         d_lev_rows_f.bytes() + d_lev_rows_b.bytes();
+         d_lev_rows_f.bytes() + d_lev_rows_b.bytes() + coarse.device_bytes();
 }
 
 // ------------------------------------------------------------ orderings
@@ -232,6 +232,115 @@ static std::vector<int> greedy_colors(int n, const std::vector<int> &ptr, const 
     ncol = std::max(ncol, c + 1);
   }
   return color;
+}
+
+// Recursive coordinate bisection of the cell centres into k aggregates of
+// equal size (+-1): each step splits the cell set at the count-weighted
+// position along the longest side of its bounding box.  The order used for
+// the split is (coordinate, cell index), a strict total order, so the result
+// does not depend on the selection algorithm.  Returns the aggregate of each
+// cell (natural numbering).
+std::vector<int> rcb_aggregates(int n, const std::vector<double> &xc, int k) {
+  std::vector<int> agg(n, 0), ids(n);
+  std::iota(ids.begin(), ids.end(), 0);
+  struct Task { int lo, hi, a0, cnt; };
+  std::vector<Task> todo{{0, n, 0, k}};
+  while (!todo.empty()) {
+    Task t = todo.back();
+    todo.pop_back();
+    if (t.cnt <= 1) {
+      for (int i = t.lo; i < t.hi; i++) agg[ids[i]] = t.a0;
+      continue;
+    }
+    double lo[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, hi[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+    for (int i = t.lo; i < t.hi; i++)
+      for (int d = 0; d < 3; d++) {
+        lo[d] = std::min(lo[d], xc[3 * (size_t)ids[i] + d]);
+        hi[d] = std::max(hi[d], xc[3 * (size_t)ids[i] + d]);
+      }
+    int dir = 0;
+    for (int d = 1; d < 3; d++)
+      if (hi[d] - lo[d] > hi[dir] - lo[dir]) dir = d;
+    const int h = t.cnt / 2;
+    const int cut = t.lo + (int)((long long)(t.hi - t.lo) * h / t.cnt);
+    std::nth_element(ids.begin() + t.lo, ids.begin() + cut, ids.begin() + t.hi, [&](int a, int b) {
+      double xa = xc[3 * (size_t)a + dir], xb = xc[3 * (size_t)b + dir];
+      return xa != xb ? xa < xb : a < b;
+    });
+    todo.push_back({cut, t.hi, t.a0 + h, t.cnt - h});
+    todo.push_back({t.lo, cut, t.a0, h});
+  }
+  return agg;
+}
+
+// Chunks, chunk lists per aggregate and the gather map of E = Z^T A Z over
+// the device numbering (aggregate of device row r = agg_dev[r]).
+static void build_coarse(SellPattern &P, const std::vector<int> &agg_dev, int k) {
+  Coarse &C = P.coarse;
+  const int n = P.n;
+  C.k = k;
+  C.chunk_ptr.assign(1, 0);
+  C.chunk_agg.clear();
+  for (int r = 0; r < n; r++) {
+    const int start = C.chunk_ptr.back();
+    if (r > start && (agg_dev[r] != agg_dev[start] || r - start == kBlock)) {
+      C.chunk_agg.push_back(agg_dev[start]);
+      C.chunk_ptr.push_back(r);
+    }
+  }
+  C.chunk_agg.push_back(agg_dev[C.chunk_ptr.back()]);
+  C.chunk_ptr.push_back(n);
+  C.nchunks = (int)C.chunk_agg.size();
+  C.agg_cptr.assign(k + 1, 0);
+  for (int c = 0; c < C.nchunks; c++) C.agg_cptr[C.chunk_agg[c] + 1]++;
+  for (int a = 0; a < k; a++) C.agg_cptr[a + 1] += C.agg_cptr[a];
+  C.agg_cidx.assign(C.nchunks, 0);
+  {
+    std::vector<int> fill(C.agg_cptr.begin(), C.agg_cptr.end() - 1);
+    for (int c = 0; c < C.nchunks; c++) C.agg_cidx[fill[C.chunk_agg[c]]++] = c;
+  }
+  // counting sort of the stored entries by key agg(row)*k + agg(col)
+  if (P.nstore > 0x7fffffffLL) fail(MS_EINVAL, "coarse space: matrix too large for 32-bit slots");
+  std::vector<int> cnt((size_t)k * k + 1, 0);
+  for (int r = 0; r < n; r++)
+    for (int j = 0; j < P.rowlen[r]; j++) cnt[(size_t)agg_dev[r] * k + agg_dev[P.col[P.slot(r, j)]] + 1]++;
+  for (size_t i = 0; i < (size_t)k * k; i++) cnt[i + 1] += cnt[i];
+  C.e_slot.assign(cnt.back(), 0);
+  {
+    std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+    for (int r = 0; r < n; r++)
+      for (int j = 0; j < P.rowlen[r]; j++) {
+        const long long q = P.slot(r, j);
+        C.e_slot[fill[(size_t)agg_dev[r] * k + agg_dev[P.col[q]]]++] = (int)q;
+      }
+  }
+  C.e_ptr.assign(1, 0);
+  C.e_key.clear();
+  for (size_t key = 0; key < (size_t)k * k; key++)
+    if (cnt[key + 1] > cnt[key]) {
+      std::sort(C.e_slot.begin() + cnt[key], C.e_slot.begin() + cnt[key + 1]);
+      C.e_key.push_back((int)key);
+      C.e_ptr.push_back(cnt[key + 1]);
+    }
+  C.upload();
+}
+
+void Coarse::upload() {
+  d_chunk_ptr.upload(chunk_ptr);
+  d_chunk_agg.upload(chunk_agg);
+  d_agg_cptr.upload(agg_cptr);
+  d_agg_cidx.upload(agg_cidx);
+  d_e_ptr.upload(e_ptr);
+  d_e_key.upload(e_key);
+  d_e_slot.upload(e_slot);
+  d_E.alloc((size_t)k * k);
+  d_Einv.alloc((size_t)k * k);
+  d_nu.alloc(k);
+}
+
+size_t Coarse::device_bytes() const {
+  return d_chunk_ptr.bytes() + d_chunk_agg.bytes() + d_agg_cptr.bytes() + d_agg_cidx.bytes() + d_e_ptr.bytes() +
+         d_e_key.bytes() + d_e_slot.bytes() + d_E.bytes() + d_nu.bytes() + d_Einv.bytes();
 }
 
 // ---------------------------------------------------------------- system
@@ -438,23 +547,30 @@ System *create_system(int nnodes, const double *x, int nfaces, const long long *
   const int ppc = ctrl->gpu_ppe_precond, mpc = ctrl->gpu_mom_precond;
   const bool want_color = ppc == MS_PC_DILU || mpc == MS_PC_DILU;
   const bool want_rcm = ctrl->gpu_reorder == 1;
+  if (ctrl->gpu_coarse < 0) fail(MS_EINVAL, "gpu_coarse must be >= 0");
+  const int nagg = std::min(std::min(ctrl->gpu_coarse, kMaxAgg), nc);
   std::vector<int> visit(nc);
   std::iota(visit.begin(), visit.end(), 0);
   if (want_rcm) visit = rcm_order(nc, c2p, c2c);
   std::vector<int> color(nc, 0);
   s.ncolors = 1;
   if (want_color) color = greedy_colors(nc, c2p, c2c, visit, s.ncolors);
+  std::vector<int> agg(nc, 0);
+  if (nagg > 0) agg = rcb_aggregates(nc, s.xc_nat, nagg);
   s.n2o = visit;
-  if (want_color) {
+  if (want_color || nagg > 0) {
+    // colour classes, aggregates contiguous inside each class
     std::vector<int> pos(nc);
     for (int i = 0; i < nc; i++) pos[visit[i]] = i;
     std::stable_sort(s.n2o.begin(), s.n2o.end(), [&](int a, int b) {
-      return color[a] != color[b] ? color[a] < color[b] : pos[a] < pos[b];
+      if (color[a] != color[b]) return color[a] < color[b];
+      if (agg[a] != agg[b]) return agg[a] < agg[b];
+      return pos[a] < pos[b];
     });
   }
   s.o2n.assign(nc, 0);
   for (int i = 0; i < nc; i++) s.o2n[s.n2o[i]] = i;
-  s.renumbered = want_color || want_rcm;
+  s.renumbered = want_color || want_rcm || nagg > 0;
 
   // device-numbered topology
   s.c0.resize(nf);
@@ -514,6 +630,11 @@ System *create_system(int nnodes, const double *x, int nfaces, const long long *
       s.ref2dev[k] = find_slot(s.o2n[i], s.o2n[s.ref_col[k] - 1]);
   s.pat.upload();
   if (ppc == MS_PC_NATURAL || mpc == MS_PC_NATURAL) s.pat.build_levels();
+  if (nagg > 0) {
+    std::vector<int> agg_dev(nc);
+    for (int r = 0; r < nc; r++) agg_dev[r] = agg[s.n2o[r]];
+    build_coarse(s.pat, agg_dev, nagg);
+  }
 
   // pin cell and pressure-Dirichlet flag (ppe_assembly :1612-1642)
   s.pin_cell = s.o2n[0];

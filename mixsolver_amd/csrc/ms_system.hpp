@@ -91,6 +91,7 @@ System *create_system(int nnodes, const double *x, int nfaces, const long long *
 // flang runtime NORM2 evaluation order (max-scaled accumulator), used for
 // every geometric length so the device coefficients equal the reference's.
 double norm2_ref(const double *v, int n);
+std::vector<int> rcb_aggregates(int n, const std::vector<double> &xc, int k);
 
 // build a SellPattern from a 0-based CSR (columns sorted per row)
 void csr_to_sell(SellPattern &pat, int n, const std::vector<int> &rp, const std::vector<int> &ci);

@@ -127,6 +127,13 @@ class Case:
         check(_lib.lib().ms_time_kernel(self.h, 0 if fp32 else 1, int(reps), C.byref(t), C.byref(b)))
         return t.value, b.value
 
+    def coarse_info(self):
+        """(k, nchunks, aggregate of every cell) of the p' coarse space."""
+        k, nch = C.c_int(0), C.c_int(0)
+        agg = np.zeros(self.nc, np.int32)
+        check(_lib.lib().ms_coarse_info(self.h, C.byref(k), C.byref(nch), ip(agg)))
+        return k.value, nch.value, agg
+
     # per-call routines ---------------------------------------------------------
     def compute_gradients(self):
         check(_lib.lib().ms_compute_gradients(self.h))
@@ -157,3 +164,13 @@ class Case:
         dpp = np.zeros(3 * self.nc)
         check(_lib.lib().ms_correct_fields(self.h, dp(pp), dp(dpp)))
         return dpp.reshape(-1, 3)
+
+
+def coarse_aggregates(xc, k):
+    """Recursive coordinate bisection of cell centres xc (n,3) into k
+    aggregates of equal size (+-1): the aggregation of the p' coarse space
+    (host only, no device needed)."""
+    xc = np.ascontiguousarray(xc, np.float64).reshape(-1, 3)
+    agg = np.zeros(len(xc), np.int32)
+    check(_lib.lib().ms_coarse_aggregates(len(xc), dp(xc), int(k), ip(agg)))
+    return agg

@@ -38,7 +38,7 @@ typedef struct {
   /* device-path extension keys (ignored by the oracle) */
   int gpu_ppe_precond, gpu_mom_precond, gpu_precision, gpu_ppe_solver;
   double gpu_inner_tol;
-  int gpu_reorder, gpu_reserved;
+  int gpu_reorder, gpu_coarse;
 } or_ctrl_t;
 
 /* one boundary group (bcgroup_t, mod_uns_bc.f90:80-110) */

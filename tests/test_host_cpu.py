@@ -93,3 +93,51 @@ def test_tanh_stretch_ratio():
     assert abs(y[0] + 1) < 1e-15 and abs(y[-1] - 1) < 1e-15
     r = np.maximum(d[1:] / d[:-1], d[:-1] / d[1:])
     assert abs(r.max() - 1.1) < 1e-6
+
+
+def _grid_centres(nx, ny, nz, h=(1.0, 1.0, 1.0)):
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    return np.stack([(i.ravel() + 0.5) * h[0], (j.ravel() + 0.5) * h[1], (k.ravel() + 0.5) * h[2]], 1)
+
+
+def test_coarse_aggregates_closed_form():
+    """On a uniform grid, bisection along the longest side with count-weighted
+    cuts gives boxes: 8x4x2 cells into 4 -> x-quarters (x is split twice, the
+    x/y tie goes to x); 8x8x8 into 8 -> octants."""
+    from mixsolver_amd.solver import coarse_aggregates
+    xc = _grid_centres(8, 4, 2)
+    i = np.arange(len(xc)) % 8
+    a = coarse_aggregates(xc, 4)
+    assert np.array_equal(a, i // 2)
+    xc = _grid_centres(8, 8, 8)
+    a = coarse_aggregates(xc, 8)
+    i, j, k = np.arange(512) % 8, (np.arange(512) // 8) % 8, np.arange(512) // 64
+    octant = (i // 4) * 4 + (j // 4) * 2 + (k // 4)   # split order x, then y, then z
+    # aggregates are numbered by the split tree: x halves, then y, then z
+    assert np.array_equal(a, octant)
+
+
+def test_coarse_aggregates_balanced_deterministic():
+    from mixsolver_amd.solver import coarse_aggregates
+    rng = np.random.default_rng(5)
+    xc = rng.uniform(size=(5000, 3)) * [4.0, 1.0, 2.0]
+    for k in (1, 3, 37, 256):
+        a = coarse_aggregates(xc, k)
+        cnt = np.bincount(a, minlength=k)
+        assert len(cnt) == k and cnt.max() - cnt.min() <= 1, k
+        assert np.array_equal(a, coarse_aggregates(xc, k))
+    # every cell its own aggregate
+    assert np.array_equal(np.sort(coarse_aggregates(xc[:50], 50)), np.arange(50))
+    # compact: 64 aggregates of uniform points have small boxes
+    a = coarse_aggregates(xc, 64)
+    ext = np.array([np.ptp(xc[a == g], axis=0).prod() for g in range(64)])
+    assert ext.max() < 4.0 * (8.0 / 64)
+
+
+def test_coarse_aggregates_rejects_bad_k():
+    from mixsolver_amd._lib import MixSolverError
+    from mixsolver_amd.solver import coarse_aggregates
+    xc = np.zeros((10, 3))
+    for k in (0, 11):
+        with pytest.raises(MixSolverError):
+            coarse_aggregates(xc, k)
