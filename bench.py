@@ -120,6 +120,8 @@ def main():
     ap.add_argument("--size", default="400x100x50")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--coarse", type=int, default=256, help="aggregates of the p' coarse space")
+    ap.add_argument("--variants", default="mixed,fp64_two_level,fp64",
+                    help="comma list of variants to run (profiling runs may select one)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU runs are not part "
@@ -134,71 +136,81 @@ def main():
     base = dict(gpu_ppe_precond=2, gpu_mom_precond=2)
     two = dict(base, gpu_coarse=args.coarse)
 
-    mixed, t_mixed, lat_mixed, h_mixed = run_variant(Case, mesh, ctl, u0, dict(two, gpu_precision=1,
-                                                                             gpu_inner_tol=0.1),
-                                                     args.steps, args.warmup)
-    # roofline of the dominant kernel (FP32 SpMV of the mixed p' solve) on the last p' matrix
-    t_k, b_k = mixed.time_kernel(fp32=True, reps=100)
-    t_k64, b_k64 = mixed.time_kernel(fp32=False, reps=50)
-    mixed.close()
-    fp64c, t_fp64c, lat_fp64c, h_fp64c = run_variant(Case, mesh, ctl, u0, dict(two, gpu_precision=0),
-                                                     args.steps, args.warmup)
-    fp64c.close()
-    fp64, t_fp64, lat_fp64, h_fp64 = run_variant(Case, mesh, ctl, u0, dict(base, gpu_precision=0),
-                                                 args.steps, args.warmup)
-    fp64.close()
-
-    sps_mixed = args.steps / t_mixed
-    sps_fp64 = args.steps / t_fp64
-    sps_fp64c = args.steps / t_fp64c
+    variants = set(args.variants.split(","))
+    runs = {}
+    for name, over in (("mixed", dict(two, gpu_precision=1, gpu_inner_tol=0.1)),
+                       ("fp64_two_level", dict(two, gpu_precision=0)),
+                       ("fp64", dict(base, gpu_precision=0))):
+        if name not in variants:
+            continue
+        cs, t, lat, h = run_variant(Case, mesh, ctl, u0, over, args.steps, args.warmup)
+        if name == "mixed":
+            # roofline of the dominant kernel (FP32 SpMV of the mixed p' solve) on the last p' matrix
+            runs["kernel"] = cs.time_kernel(fp32=True, reps=100) + cs.time_kernel(fp32=False, reps=50)
+        cs.close()
+        runs[name] = (args.steps / t, t, lat, h)
     cells = mesh.ncells
+    def pct(name, q):
+        return float(np.percentile(runs[name][2], q) * 1e3) if name in runs else None
 
-    def pct(lat, q):
-        return float(np.percentile(lat, q) * 1e3)
+    def sps(name):
+        return runs[name][0] if name in runs else None
 
+    def ratio(a, b):
+        return sps(a) / sps(b) if a in runs and b in runs else None
+
+    def iters(name):
+        return float(np.mean(runs[name][3][:, 6])) if name in runs else None
+
+    def resmax(name):
+        return float(np.max(runs[name][3][:, 7])) if name in runs else None
+
+    lead = "mixed" if "mixed" in runs else sorted(runs.keys() - {"kernel"})[0]
     out = {
         "metric": METRIC,
-        "value": sps_mixed,
+        "value": sps(lead),
         "unit": "steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * t_mixed / args.steps,
+        "ms_per_step": 1e3 * runs[lead][1] / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 inner + f64 refinement",
+        "dtype": "f32 inner + f64 refinement" if lead == "mixed" else "f64",
         "data": "synthetic",
         "config": {"workload": f"channel {nx}x{ny}x{nz} hex ({cells} cells), transient PISO, 2 correctors, "
                                f"p' PCG + multicolour DILU + {args.coarse}-aggregate coarse correction to "
                                "1e-10, momentum BiCGSTAB + DILU to 1e-10",
-                   "cells": cells, "faces": mesh.nfaces, "time_scheme": "BDF2", "parallelism": "1 GPU"},
+                   "cells": cells, "faces": mesh.nfaces, "time_scheme": "BDF2", "parallelism": "1 GPU",
+                   "variant": lead},
         # the plain FP64 device path: reference recurrences, one-level DILU
-        "fp64_steps_per_s": sps_fp64,
-        "fp64_ms_per_step": 1e3 * t_fp64 / args.steps,
-        "speedup_mixed_vs_fp64": sps_mixed / sps_fp64,
+        "fp64_steps_per_s": sps("fp64"),
+        "speedup_mixed_vs_fp64": ratio("mixed", "fp64"),
         # the same two-level solver in FP64 (isolates the precision gain)
-        "fp64_two_level_steps_per_s": sps_fp64c,
-        "speedup_mixed_vs_fp64_two_level": sps_mixed / sps_fp64c,
-        "p50_step_ms": pct(lat_mixed, 50),
-        "p99_step_ms": pct(lat_mixed, 99),
-        "fp64_p50_step_ms": pct(lat_fp64, 50),
-        "fp64_p99_step_ms": pct(lat_fp64, 99),
-        "fp64_two_level_p50_step_ms": pct(lat_fp64c, 50),
-        "fp64_two_level_p99_step_ms": pct(lat_fp64c, 99),
-        "ppe_iters_per_step_mixed": float(np.mean(h_mixed[:, 6])),
-        "ppe_iters_per_step_fp64": float(np.mean(h_fp64[:, 6])),
-        "ppe_iters_per_step_fp64_two_level": float(np.mean(h_fp64c[:, 6])),
-        "ppe_final_res_mixed_max": float(np.max(h_mixed[:, 7])),
-        "ppe_final_res_fp64_max": float(np.max(h_fp64[:, 7])),
-        "ppe_final_res_fp64_two_level_max": float(np.max(h_fp64c[:, 7])),
-        "cell_iterations_per_s_mixed": cells * float(np.sum(h_mixed[:, 6])) / t_mixed,
-        "roofline": {"bound": "hbm", "kernel": "k_pcg_spmv<float> (SELL-64, difference form, fused p.q)",
-                     "achieved": b_k / (t_k * 1e-3) / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": b_k / (t_k * 1e-3) / HBM_PEAK, "traffic": None,
-                     "avg_launch_us": t_k * 1e3, "alg_bytes": b_k,
-                     "fp64_kernel_GBps": b_k64 / (t_k64 * 1e-3) / 1e9, "fp64_avg_launch_us": t_k64 * 1e3},
+        "fp64_two_level_steps_per_s": sps("fp64_two_level"),
+        "speedup_mixed_vs_fp64_two_level": ratio("mixed", "fp64_two_level"),
+        "p50_step_ms": pct(lead, 50),
+        "p99_step_ms": pct(lead, 99),
+        "fp64_p50_step_ms": pct("fp64", 50),
+        "fp64_p99_step_ms": pct("fp64", 99),
+        "fp64_two_level_p50_step_ms": pct("fp64_two_level", 50),
+        "fp64_two_level_p99_step_ms": pct("fp64_two_level", 99),
+        "ppe_iters_per_step_mixed": iters("mixed"),
+        "ppe_iters_per_step_fp64": iters("fp64"),
+        "ppe_iters_per_step_fp64_two_level": iters("fp64_two_level"),
+        "ppe_final_res_mixed_max": resmax("mixed"),
+        "ppe_final_res_fp64_max": resmax("fp64"),
+        "ppe_final_res_fp64_two_level_max": resmax("fp64_two_level"),
+        "cell_iterations_per_s": cells * float(np.sum(runs[lead][3][:, 6])) / runs[lead][1],
     }
+    if "kernel" in runs:
+        t_k, b_k, t_k64, b_k64 = runs["kernel"]
+        out["roofline"] = {"bound": "hbm", "kernel": "k_pcg_spmv<float> (SELL-64, difference form, fused p.q)",
+                           "achieved": b_k / (t_k * 1e-3) / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                           "frac": b_k / (t_k * 1e-3) / HBM_PEAK, "traffic": None,
+                           "avg_launch_us": t_k * 1e3, "alg_bytes": b_k,
+                           "fp64_kernel_GBps": b_k64 / (t_k64 * 1e-3) / 1e9, "fp64_avg_launch_us": t_k64 * 1e3}
     if not args.no_cpu:
         # p' iterations the reference's Jacobi PCG needs for the first step (device FP64 Jacobi run)
         jac, _, _, h_jac = run_variant(Case, mesh, ctl, u0, dict(gpu_ppe_precond=0, gpu_mom_precond=2,

@@ -88,16 +88,21 @@ __device__ __forceinline__ T row_apply(int r, const Mat &A, const T *val, const 
   if constexpr (sizeof(T) == 8) {
     return row_dot(r, A, val, x);
   } else {
-    const long long base = A.off[r >> 6] + (r & 63);
-    const int len = A.rowlen[r], dk = A.diagk[r];
+    // every slot of the slice width contributes a_k (x_col - x_i): padding
+    // (col = r, a = 0) and the diagonal slot (col = r) give zero, so the loop
+    // runs the slice width, uniform across the wavefront, without branches
+    const int s = r >> 6;
+    const long long o0 = A.off[s];
+    const int w = (int)((A.off[s + 1] - o0) >> 6);
+    const long long base = o0 + (r & 63);
     const T xi = x[r];
-    T s = val[base + (long long)dk * kSlice] * xi;
-    for (int k = 0; k < len; k++) {
-      if (k == dk) continue;
+    T acc = T(0);
+#pragma unroll 4
+    for (int k = 0; k < w; k++) {
       const long long q = base + (long long)k * kSlice;
-      s = s + val[q] * (x[A.col[q]] - xi);
+      acc = acc + val[q] * (x[A.col[q]] - xi);
     }
-    return s;
+    return val[base + (long long)A.diagk[r] * kSlice] * xi + acc;
   }
 }
 
@@ -115,6 +120,9 @@ __global__ void k_mat32_diff(int n, Mat A, const double *v64, float *v32) {
     if (k != dk) v32[q] = (float)v64[q];
   }
   v32[base + (long long)dk * kSlice] = (float)s;
+  // padding slots are read by row_apply: zero them
+  const int w = (int)((A.off[(r >> 6) + 1] - A.off[r >> 6]) >> 6);
+  for (int k = len; k < w; k++) v32[base + (long long)k * kSlice] = 0.0f;
 }
 
 // ------------------------------------------------------------- finishers
@@ -786,33 +794,51 @@ __global__ void k_dilu_bwd(int r0, int r1, Mat A, const T *val, const T *dg, con
 // chunks of aggregate a (fixed order), nu = E^-1 c and c.nu, before the
 // functor runs on thread 0.
 template <int NV, class T, class Fin>
-__global__ void k_finish_co(int nparts, const T *partial, Fin fin, CoDev co) {
+__global__ __launch_bounds__(kFinThreads) void k_finish_co(int nparts, const T *partial, Fin fin, CoDev co) {
   if (fin.skip()) return;
   T acc[NV];
-#pragma unroll
-  for (int j = 0; j < NV; j++) acc[j] = T(0);
-  for (int b = threadIdx.x; b < nparts; b += kBlock)
-#pragma unroll
-    for (int j = 0; j < NV; j++) acc[j] = acc[j] + partial[b * NV + j];
-  __shared__ T lds[NV * (kBlock / 64)];
-  block_reduce<OpSum, NV, T>(acc, lds);
+  fin_gather<OpSum, NV, T>(nparts, partial, acc);
+  __shared__ T lds[NV * (kFinThreads / 64)];
+  block_reduce<OpSum, NV, T, kFinThreads>(acc, lds);
   if (co.k > 0) {
-    __shared__ double c[kMaxAgg];
-    for (int a = threadIdx.x; a < co.k; a += kBlock) {
+    // four threads per aggregate (quarter q = t / 256) each sum every fourth
+    // chunk partial and every fourth block of E^-1 columns; quarters are
+    // combined in q order
+    __shared__ double c[kMaxAgg], part4[4][kMaxAgg];
+    const int a = threadIdx.x & (kMaxAgg - 1), q = threadIdx.x / kMaxAgg;
+    if (a < co.k) {
       double s = 0.0;
-      for (int j = co.agg_cptr[a]; j < co.agg_cptr[a + 1]; j++) s = s + (double)partial[co.agg_cidx[j] * NV + NV - 1];
-      c[a] = s;
+      for (int j = co.agg_cptr[a] + q; j < co.agg_cptr[a + 1]; j += 4)
+        s = s + (double)partial[co.agg_cidx[j] * NV + NV - 1];
+      part4[q][a] = s;
+    }
+    __syncthreads();
+    if (q == 0 && a < co.k) c[a] = ((part4[0][a] + part4[1][a]) + part4[2][a]) + part4[3][a];
+    __syncthreads();
+    const int kq = (co.k + 3) / 4, b0 = q * kq, b1 = min(co.k, b0 + kq);
+    if (a < co.k) {
+      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;    // E^-1 is symmetric: column a
+      int b = b0;
+      for (; b + 3 < b1; b += 4) {
+        const float e0 = co.Einv[(size_t)b * co.k + a], e1 = co.Einv[(size_t)(b + 1) * co.k + a];
+        const float e2 = co.Einv[(size_t)(b + 2) * co.k + a], e3 = co.Einv[(size_t)(b + 3) * co.k + a];
+        s0 = s0 + (double)e0 * c[b];
+        s1 = s1 + (double)e1 * c[b + 1];
+        s2 = s2 + (double)e2 * c[b + 2];
+        s3 = s3 + (double)e3 * c[b + 3];
+      }
+      for (; b < b1; b++) s0 = s0 + (double)co.Einv[(size_t)b * co.k + a] * c[b];
+      part4[q][a] = ((s0 + s1) + s2) + s3;
     }
     __syncthreads();
     double prod[1] = {0.0};
-    for (int a = threadIdx.x; a < co.k; a += kBlock) {
-      double s = 0.0;
-      for (int b = 0; b < co.k; b++) s = s + (double)co.Einv[(size_t)b * co.k + a] * c[b];   // E^-1 symmetric
-      co.nu[a] = s;
-      prod[0] = prod[0] + s * c[a];
+    if (q == 0 && a < co.k) {
+      const double nu = ((part4[0][a] + part4[1][a]) + part4[2][a]) + part4[3][a];
+      co.nu[a] = nu;
+      prod[0] = nu * c[a];
     }
-    __shared__ double lds2[kBlock / 64];
-    block_reduce<OpSum, 1, double>(prod, lds2);
+    __shared__ double lds2[kFinThreads / 64];
+    block_reduce<OpSum, 1, double, kFinThreads>(prod, lds2);
     if (threadIdx.x == 0) co.st->cnu = prod[0];
   }
   if (threadIdx.x == 0) fin(acc);
@@ -912,7 +938,7 @@ static Mat matof(const SellPattern *p) { return Mat{p->d_off.p, p->d_rowlen.p, p
 
 template <class Op, int NV, class T, class Fin>
 static void finish(int nparts, const T *part, Fin fin) {
-  hipLaunchKernelGGL((k_finish<Op, NV, T, Fin>), dim3(1), dim3(kBlock), 0, stream(), nparts, part, fin);
+  hipLaunchKernelGGL((k_finish<Op, NV, T, Fin>), dim3(1), dim3(kFinThreads), 0, stream(), nparts, part, fin);
 }
 
 Krylov::Krylov(SellPattern *pat_) : pat(pat_), n(pat_->n) {
@@ -1118,7 +1144,7 @@ static CoDev codev_of(const SellPattern *pat, bool coarse, KS *st) {
 }
 template <int NV, class T, class Fin>
 static void finish_co(int nparts, const T *part, Fin fin, const CoDev &co) {
-  hipLaunchKernelGGL((k_finish_co<NV, T, Fin>), dim3(1), dim3(kBlock), 0, stream(), nparts, part, fin, co);
+  hipLaunchKernelGGL((k_finish_co<NV, T, Fin>), dim3(1), dim3(kFinThreads), 0, stream(), nparts, part, fin, co);
 }
 
 void Krylov::coarse_setup(const double *val) {

@@ -651,7 +651,7 @@ struct Driver {
   void outflow() {
     if (!s.has_outflow) return;
     hipLaunchKernelGGL(k_outflow_sums, gc(s.nf), dim3(kBlock), 0, stream(), s.dmesh(), s.flux.p, s.part.p);
-    hipLaunchKernelGGL((k_finish<OpSum, 3, double, FinStore>), dim3(1), dim3(kBlock), 0, stream(), nblocks(s.nf),
+    hipLaunchKernelGGL((k_finish<OpSum, 3, double, FinStore>), dim3(1), dim3(kFinThreads), 0, stream(), nblocks(s.nf),
                        s.part.p, FinStore{s.tmpc.p, 3});
     hipLaunchKernelGGL(k_outflow_scale, gc(s.nf), dim3(kBlock), 0, stream(), s.dmesh(), s.flux.p, s.tmpc.p);
     MS_CHECK_LAUNCH();
@@ -729,7 +729,7 @@ struct Driver {
   void diagnostics(const double *uref, const double *Tref, double *out) {
     hipLaunchKernelGGL(k_diagnostics, gc(s.nc), dim3(kBlock), 0, stream(), s.dmesh(), s.dfields(), uref, Tref,
                        s.part.p);
-    hipLaunchKernelGGL((k_finish<OpMax, 5, double, FinStore>), dim3(1), dim3(kBlock), 0, stream(), nblocks(s.nc),
+    hipLaunchKernelGGL((k_finish<OpMax, 5, double, FinStore>), dim3(1), dim3(kFinThreads), 0, stream(), nblocks(s.nc),
                        s.part.p, FinStore{s.tmpc.p, 5});
     MS_CHECK_LAUNCH();
     MS_HIP(hipMemcpyAsync(out, s.tmpc.p, 5 * sizeof(double), hipMemcpyDeviceToHost, stream()));

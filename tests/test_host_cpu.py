@@ -4,6 +4,7 @@ grammar, the mesh generators produce valid owner/neighbour meshes and the
 .cas writer/reader round-trips them."""
 import ctypes as C
 import os
+import shutil
 
 import numpy as np
 import pytest
@@ -141,3 +142,31 @@ def test_coarse_aggregates_rejects_bad_k():
     for k in (0, 11):
         with pytest.raises(MixSolverError):
             coarse_aggregates(xc, k)
+
+
+FLANG = shutil.which("amdflang") or shutil.which("flang")
+
+
+@pytest.mark.skipif(FLANG is None, reason="no Fortran compiler in this image")
+def test_fortran_binding_layout_and_link(tmp_path):
+    """mixsolver_c.f90 (the ISO_C_BINDING interface a Fortran caller uses)
+    compiles, links against the library, has the C struct sizes, and calls a
+    host-only entry point."""
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    fdir = os.path.join(root, "mixsolver_amd", "fortran")
+    libdir = os.path.dirname(_lib.LIB_PATH)
+    subprocess.run([FLANG, "-c", os.path.join(fdir, "mixsolver_c.f90"), "-o", "mixsolver_c.o"],
+                   cwd=tmp_path, check=True)
+    subprocess.run([FLANG, os.path.join(fdir, "check_binding.f90"), "mixsolver_c.o", f"-L{libdir}",
+                    "-lmixsolver_hip", f"-Wl,-rpath,{libdir}", "-o", "check_binding"], cwd=tmp_path, check=True)
+    csz = (tmp_path / "sz.c")
+    csz.write_text('#include <stdio.h>\n#include "%s"\nint main(){printf("%%zu %%zu", sizeof(ms_ctrl_t), '
+                   'sizeof(ms_bcgroup_t));return 0;}\n' % os.path.join(root, "include", "mixsolver.h"))
+    subprocess.run(["gcc", str(csz), "-o", str(tmp_path / "sz")], check=True)
+    sizes = subprocess.run([str(tmp_path / "sz")], capture_output=True, text=True, check=True).stdout.split()
+    # the ctypes mirror has the same sizes
+    from mixsolver_amd.control import MsBcGroup, MsCtrl
+    assert [C.sizeof(MsCtrl), C.sizeof(MsBcGroup)] == [int(v) for v in sizes]
+    r = subprocess.run([str(tmp_path / "check_binding")] + sizes, capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
