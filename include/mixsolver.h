@@ -117,6 +117,9 @@ int ms_simple_run(ms_system *s, int nsteps, int *it_done, int *ier);
 /* piso_run_mpi at one partition (mod_uns_simple_mpi.f90:298); nsteps
  * overrides n_time_max when > 0.  ier: 30-36 / 60 as the reference. */
 int ms_piso_run(ms_system *s, int nsteps, int *ier);
+/* pimple_run_mpi at one partition (mod_uns_simple_mpi.f90:580): n_outer_iter
+ * outer iterations per step; ier: 50-56 / 60 as the reference. */
+int ms_pimple_run(ms_system *s, int nsteps, int *ier);
 /* per-iteration/step diagnostics of the last driver call, 8 doubles per row:
  * imbal du_max dT_max|umax cfl lin_it_max lin_res_mean ppe_it_total ppe_res_last */
 int ms_get_history(ms_system *s, double *out, int maxrows, int *nrows);

@@ -51,6 +51,7 @@ def lib():
         "ms_get_topology": (C.c_int, [_V, _I, _I, _I]),
         "ms_simple_run": (C.c_int, [_V, C.c_int, _I, _I]),
         "ms_piso_run": (C.c_int, [_V, C.c_int, _I]),
+        "ms_pimple_run": (C.c_int, [_V, C.c_int, _I]),
         "ms_get_history": (C.c_int, [_V, _D, C.c_int, _I]),
         "ms_compute_gradients": (C.c_int, [_V]),
         "ms_momentum_assembly": (C.c_int, [_V, C.c_int, _D, _D, _D]),

@@ -13,6 +13,7 @@ void ensure_runtime(int device);
 const char *last_error();
 int run_simple(System &s, int nsteps, int *it_done);
 int run_piso(System &s, int nsteps);
+int run_pimple(System &s, int nsteps);
 void call_gradients(System &s);
 void call_momentum(System &s, int comp);
 void call_energy(System &s);
@@ -249,6 +250,13 @@ int ms_simple_run(ms_system *h, int nsteps, int *it_done, int *ier) {
 int ms_piso_run(ms_system *h, int nsteps, int *ier) {
   return guarded([&] {
     int r = run_piso(sys(h), nsteps);
+    if (ier) *ier = r;
+  });
+}
+
+int ms_pimple_run(ms_system *h, int nsteps, int *ier) {
+  return guarded([&] {
+    int r = run_pimple(sys(h), nsteps);
     if (ier) *ier = r;
   });
 }

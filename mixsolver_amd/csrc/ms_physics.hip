@@ -599,6 +599,12 @@ __global__ void k_fill(long long n, double *x, double v) {
   if (i < n) x[i] = v;
 }
 
+// y = x / d (the relaxed momentum diagonal apc = ap / alpha_u)
+__global__ void k_div(int n, const double *x, double d, double *y) {
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) y[i] = x[i] / d;
+}
+
 // ------------------------------------------------------------ diagnostics
 // out: [0] max|imbalance| [1] max|u - uref| [2] max|T - Tref| [3] max norm2(u) [4] non-finite
 __global__ void k_diagnostics(DMesh m, DFields F, const double *uref, const double *Tref, double *part) {
@@ -863,6 +869,78 @@ int run_piso(System &s, int nsteps) {
     const double cfl = umax * c.dt / minv13;
     if (dg[4] != 0.0) { ier = 60; break; }
     push_history(s, imbal, du, umax, cfl, itl_max, ressum / 3.0, ppe_it, ppe_res);
+  }
+  MS_HIP(hipStreamSynchronize(stream()));
+  return ier;
+}
+
+// pimple_run_mpi at one partition (mod_uns_simple_mpi.f90:580-850): outer
+// iterations per time step, apc = ap / alpha_u (:706), corrections inside
+// each outer iteration, energy once per step (:751-760), ts_order kept at 2
+// on the first step when the state on entry is a BDF2 restart (:664-682).
+int run_pimple(System &s, int nsteps) {
+  const ms_ctrl_t &c = s.ctrl;
+  if (!c.transient) return 50;
+  if (c.dt <= 0.0) return 51;
+  s.refresh_face_constants();
+  Driver D(s);
+  const size_t n3 = 3 * (size_t)s.nc;
+  int n_steps = std::max(1, c.n_time_max);
+  if (nsteps > 0) n_steps = nsteps;
+  const int n_outer = std::max(1, c.n_outer_iter);
+  s.history.clear();
+  fill(s.apc.p, s.nc, 1.0);
+  D.gradients();
+  D.rhiechow();
+  const int ts_order_in = s.ts_order;
+  double minv13 = INFINITY;
+  for (double v : s.vol_nat) minv13 = std::min(minv13, std::pow(v, 1.0 / 3.0));
+  int ier = 0;
+  for (int it = 1; it <= n_steps; it++) {
+    dcopy(s.u_old_old.p, s.u_old.p, n3);
+    dcopy(s.u_old.p, s.u.p, n3);
+    dcopy(s.T_old_old.p, s.T_old.p, s.nc);
+    dcopy(s.T_old.p, s.T.p, s.nc);
+    dcopy(s.T_s_old_old.p, s.T_s_old.p, s.nc);
+    dcopy(s.T_s_old.p, s.T_s.p, s.nc);
+    if (c.time_scheme == 1) s.ts_order = 1;
+    else if (it == 1 && ts_order_in < 2) s.ts_order = 1;
+    else s.ts_order = 2;
+    int itl_max = 0, ppe_it = 0;
+    double ressum = 0.0, ppe_res = 0.0;
+    bool brk = false, bad = false;
+    for (int io = 1; io <= n_outer && !brk && !bad; io++) {
+      if (D.momentum_solves(itl_max, ressum)) { brk = true; break; }
+      hipLaunchKernelGGL(k_div, gc(s.nc), dim3(kBlock), 0, stream(), s.nc, s.ap.p, c.alpha_u, s.apc.p);
+      MS_CHECK_LAUNCH();
+      D.gradients();
+      D.rhiechow();
+      D.outflow();
+      for (int ic = 1; ic <= c.n_correct; ic++) {
+        D.ppe();
+        fill(s.pp.p, s.nc, 0.0);
+        SolveResult R = D.solve(true, s.pp.p);
+        if (R.ierr == 2) { bad = true; break; }
+        itl_max = std::max(itl_max, R.it);
+        ppe_it += R.it;
+        ppe_res = R.res;
+        D.correct(s.pp.p);
+      }
+    }
+    if (brk) { ier = 53; break; }
+    if (bad) { ier = 54; break; }
+    if (s.energy_active) {
+      D.energy();
+      SolveResult RT = D.solve(false, s.T.p);
+      if (RT.ierr == 2) { ier = 55; break; }
+      itl_max = std::max(itl_max, RT.it);
+    }
+    double dg[5];
+    D.diagnostics(s.u_old.p, s.T_old.p, dg);
+    const double imbal = dg[0] / s.flux_ref, du = dg[1] / c.dt, umax = dg[3];
+    const double cfl = umax * c.dt / minv13;
+    if (dg[4] != 0.0) { ier = 60; break; }
+    push_history(s, imbal, du, umax, cfl, itl_max, ressum / (3.0 * n_outer), ppe_it, ppe_res);
   }
   MS_HIP(hipStreamSynchronize(stream()));
   return ier;

@@ -16,7 +16,7 @@ module mixsolver_c
    public :: ms_init, ms_version, ms_system_create, ms_system_destroy, ms_system_info
    public :: ms_coarse_info, ms_coarse_aggregates
    public :: ms_set_field, ms_get_field, ms_set_ts_order, ms_get_topology
-   public :: ms_simple_run, ms_piso_run, ms_get_history
+   public :: ms_simple_run, ms_piso_run, ms_pimple_run, ms_get_history
    public :: ms_compute_gradients, ms_momentum_assembly, ms_temperature_assembly
    public :: ms_flux_rhiechow, ms_outflow_mass_rescale, ms_ppe_assembly, ms_correct_fields
    public :: ms_cg_jacobi, ms_cg_ic0, ms_bicgstab_ilu0, ms_solve, ms_time_kernel
@@ -123,6 +123,12 @@ module mixsolver_c
          integer(c_int) :: it_done, ier
       end function
       integer(c_int) function ms_piso_run(s, nsteps, ier) bind(c, name='ms_piso_run')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: s
+         integer(c_int), value :: nsteps
+         integer(c_int) :: ier
+      end function
+      integer(c_int) function ms_pimple_run(s, nsteps, ier) bind(c, name='ms_pimple_run')
          import :: c_int, c_ptr
          type(c_ptr), value :: s
          integer(c_int), value :: nsteps

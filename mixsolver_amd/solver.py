@@ -5,6 +5,7 @@ MI355X) and exposes the reference's routines under their names:
 
   simple_run(nsteps)        mod_uns_simple.f90:88        (serial steady driver)
   piso_run(nsteps)          mod_uns_simple_mpi.f90:298   (transient driver, np=1)
+  pimple_run(nsteps)        mod_uns_simple_mpi.f90:580   (PIMPLE driver, np=1)
   compute_gradients()       mod_uns_fields.f90:164
   momentum_assembly(comp)   mod_uns_simple.f90:928
   temperature_assembly()    mod_uns_simple.f90:1748
@@ -111,6 +112,11 @@ class Case:
     def piso_run(self, nsteps=0):
         ier = C.c_int(0)
         check(_lib.lib().ms_piso_run(self.h, int(nsteps), C.byref(ier)))
+        return ier.value
+
+    def pimple_run(self, nsteps=0):
+        ier = C.c_int(0)
+        check(_lib.lib().ms_pimple_run(self.h, int(nsteps), C.byref(ier)))
         return ier.value
 
     def history(self):

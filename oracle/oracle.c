@@ -1430,3 +1430,78 @@ int or_piso_run(or_case *cs, int nsteps, int mpi) {
   free(val); free(rhs); free(ap); free(pp); free(dpp); free(imb); free(ucomp);
   return ier;
 }
+
+/* pimple_run_mpi at one partition, mod_uns_simple_mpi.f90:580-850: outer
+ * iterations per time step with the relaxed momentum diagonal apc = ap/alpha_u
+ * (:706), corrections inside each outer iteration, energy once per step with
+ * the final velocity (:751-760); ts_order keeps 2 on the first step when the
+ * state on entry is a BDF2 restart (:664, :676-682). */
+int or_pimple_run(or_case *cs, int nsteps, int mpi) {
+  const or_ctrl_t *ct = &cs->ctrl;
+  int nc = cs->nc;
+  if (!ct->transient) return 50;
+  if (ct->dt <= 0.0) return 51;
+  double *val = xcalloc(cs->nnz, 8), *rhs = xcalloc(nc, 8), *ap = xcalloc(nc, 8);
+  double *pp = xcalloc(nc, 8), *dpp = xcalloc(3 * (size_t)nc, 8), *imb = xcalloc(nc, 8);
+  double *ucomp = xcalloc(nc, 8);
+  int ier = 0;
+  cs->nhist = 0;
+  cs->t_ppe = 0.0;
+  cs->ppe_iters = 0;
+  double t_start = now_s();
+  int n_steps = ct->n_time_max > 1 ? ct->n_time_max : 1;
+  if (nsteps > 0) n_steps = nsteps;
+  const int n_outer = ct->n_outer_iter > 1 ? ct->n_outer_iter : 1;
+  double flux_ref = flux_reference(cs);
+  for (int c = 0; c < nc; c++) cs->apc[c] = 1.0;
+  or_compute_gradients(cs);
+  or_flux_rhiechow(cs);
+  const int ts_order_in = cs->ts_order;
+  size_t n3 = 3 * (size_t)nc;
+  for (int it = 1; it <= n_steps; it++) {
+    memcpy(cs->u_old_old, cs->u_old, n3 * 8);
+    memcpy(cs->u_old, cs->u, n3 * 8);
+    memcpy(cs->T_old_old, cs->T_old, nc * 8);
+    memcpy(cs->T_old, cs->T, nc * 8);
+    memcpy(cs->T_s_old_old, cs->T_s_old, nc * 8);
+    memcpy(cs->T_s_old, cs->T_s, nc * 8);
+    if (ct->time_scheme == 1) cs->ts_order = 1;
+    else if (it == 1 && ts_order_in < 2) cs->ts_order = 1;
+    else cs->ts_order = 2;
+    int itl_max = 0, itl, brk = 0, bad = 0;
+    double ressum = 0.0;
+    for (int io = 1; io <= n_outer; io++) {
+      momentum_solves(cs, val, rhs, ap, ucomp, mpi, &itl_max, &ressum, &brk);
+      if (brk) break;
+      for (int c = 0; c < nc; c++) cs->apc[c] = ap[c] / ct->alpha_u;
+      or_compute_gradients(cs);
+      or_flux_rhiechow(cs);
+      or_outflow_mass_rescale(cs);
+      for (int ic = 1; ic <= ct->n_correct; ic++) {
+        if (ppe_solve(cs, val, rhs, pp, mpi, &itl) == 2) { bad = 1; break; }
+        if (itl > itl_max) itl_max = itl;
+        or_correct_fields(cs, pp, dpp);
+      }
+      if (bad) break;
+    }
+    if (brk) { ier = 53; break; }
+    if (bad) { ier = 54; break; }
+    if (temperature_solve(cs, val, rhs, ap, mpi, &itl) == 2) { ier = 55; break; }
+    if (itl > itl_max) itl_max = itl;
+    double imbal = max_imbalance(cs, imb) / flux_ref;
+    double du_max = 0.0, umax = 0.0;
+    for (size_t k = 0; k < n3; k++) if (fabs(cs->u[k] - cs->u_old[k]) > du_max) du_max = fabs(cs->u[k] - cs->u_old[k]);
+    du_max = du_max / ct->dt;
+    for (int c = 0; c < nc; c++) { double n = norm3(&cs->u[3 * c]); if (n > umax) umax = n; }
+    double cfl = 0.0;
+    for (int c = 0; c < nc; c++) {
+      double dtdx = umax * ct->dt / pow(cs->vol[c], 1.0 / 3.0);
+      if (dtdx > cfl) cfl = dtdx;
+    }
+    if (nonfinite(cs)) { ier = 60; break; }
+    push_hist(cs, imbal, du_max, umax, cfl, itl_max, ressum / (3.0 * n_outer));
+  }
+  cs->t_total = now_s() - t_start;
+  free(val); free(rhs); free(ap); free(pp); free(dpp); free(imb); free(ucomp);
+  return ier;
+}

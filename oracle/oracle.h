@@ -99,6 +99,7 @@ void or_correct_fields(or_case *, const double *pp, double *dpp);
  * ordering of mod_uns_simple_mpi.f90, mpi = 0 the serial norm2 variants. */
 int or_simple_run(or_case *, int nsteps, int mpi, int *it_done);
 int or_piso_run(or_case *, int nsteps, int mpi);
+int or_pimple_run(or_case *, int nsteps, int mpi);
 /* per-step diagnostics of the last driver call: n_steps x 6
  * (imbal, du_max, umax, cfl_max, lin_it_max, lin_res_mean) */
 const double *or_history(or_case *, int *nrows);

@@ -47,6 +47,7 @@ def lib():
         L.or_correct_fields.argtypes = [C.c_void_p, _D, _D]
         L.or_simple_run.argtypes = [C.c_void_p, C.c_int, C.c_int, _I]
         L.or_piso_run.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.or_pimple_run.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.or_timing.argtypes = [C.c_void_p, _D]
         L.or_history.restype = _D
         L.or_history.argtypes = [C.c_void_p, _I]
@@ -188,6 +189,9 @@ class OracleCase:
 
     def piso_run(self, nsteps=0, mpi=True):
         return lib().or_piso_run(self.h, int(nsteps), int(mpi))
+
+    def pimple_run(self, nsteps=0, mpi=True):
+        return lib().or_pimple_run(self.h, int(nsteps), int(mpi))
 
     def timing(self):
         """(seconds of the last PISO call, seconds in its p' solves, p' iterations)"""

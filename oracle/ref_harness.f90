@@ -9,6 +9,7 @@
 !   geom   : geometry, lf, connectivity and CSR pattern
 !   simple : seed fields from in.bin (u,p,T), run simple_run(nsteps=n)
 !   piso   : seed fields from in.bin (u,p,T), run piso_run (n_time_max steps)
+!   pimple : seed fields from in.bin (u,p,T), run pimple_run (n_time_max steps)
 !   funcs  : seed fields from in.bin, then compute_gradients, momentum_assembly
 !            (comp 1..3), flux_rhiechow, temperature_assembly, correct_fields
 !   solve  : CSR system from in.bin; residual histories of cg_jacobi, cg_ic0,
@@ -79,6 +80,11 @@ program ref_harness
    case ( 'piso' )
       call seed_fields()
       call piso_run( m, c, g, ctrl, bcs, fld, '/tmp/ref_harness_piso', ier )
+      write(uo) ier
+      call dump_state()
+   case ( 'pimple' )
+      call seed_fields()
+      call pimple_run( m, c, g, ctrl, bcs, fld, '/tmp/ref_harness_pimple', ier )
       write(uo) ier
       call dump_state()
    case ( 'funcs' )

@@ -63,6 +63,32 @@ k_cond = 0.01
 cp = 1.0
 """
 
+PIMPLE_CTL = """rho = 1.0
+mu = 0.02
+transient = true
+pimple = true
+n_outer_iter = 3
+alpha_u = 0.7
+dt = 0.08
+n_time_max = 3
+n_correct = 1
+n_out_every = 1000000
+time_scheme = 2
+lin_tol = 1e-10
+lin_max = 2000
+conv_blend = 0.5
+nonorth_corr = 1.0
+ppe_precond = ic0
+body_force = 1.0 0.0 0.0
+bc = 6 velocity-inlet-parabolic 1.0 2.0 2 -1.0 1.0
+bc = 7 outflow
+bc = 5 wall
+bc = 4 symmetry
+tbc = 5 1 0.5
+k_cond = 0.01
+cp = 1.0
+"""
+
 PRISM_CTL = """rho = 1.0
 mu = 0.02
 transient = true
@@ -296,6 +322,11 @@ def main():
     save("channel_piso_jacobi", control=ctl, nsteps=np.int64(3), **mesh_arrays(mc),
          in_u=u0, in_p=p0, in_T=T0, **{"geom_" + k: v for k, v in gch.items()},
          **{"out_" + k: v for k, v in out.items()})
+
+    # 3b. the same channel under PIMPLE: 3 outer iterations, relaxed momentum, IC0, blending
+    out, log = case_run("chanpim", mc, zkc, PIMPLE_CTL, "pimple", 3, (u0, p0, T0))
+    save("channel_pimple_ic0", control=PIMPLE_CTL, nsteps=np.int64(3), **mesh_arrays(mc),
+         in_u=u0, in_p=p0, in_T=T0, **{"out_" + k: v for k, v in out.items()})
 
     # 4. jittered prism O-grid, PISO with IC0, blended convection, far field
     mp = meshgen.cylinder_prisms(ntheta=24, nr=6, r_out=4.0, layers=2, lz=0.2, jitter=0.15, seed=200)

@@ -154,7 +154,8 @@ def test_mixed_precision_reaches_fp64_tolerance():
                                        ("cavity16_simple_ic0_blend", "simple"),
                                        ("outlet_simple_jacobi", "simple"),
                                        ("channel_piso_jacobi", "piso"),
-                                       ("prism_piso_ic0", "piso")])
+                                       ("prism_piso_ic0", "piso"),
+                                       ("channel_pimple_ic0", "pimple")])
 def test_drivers_match_reference(name, mode):
     d = load_golden(name)
     cs = make(d)
@@ -165,6 +166,8 @@ def test_drivers_match_reference(name, mode):
         cs.set_field("T_s", d["in_T"])
     if mode == "simple":
         ier, _ = cs.simple_run(int(d["nsteps"]))
+    elif mode == "pimple":
+        ier = cs.pimple_run(int(d["nsteps"]))
     else:
         ier = cs.piso_run(int(d["nsteps"]))
     assert ier == int(d["out_ier"])

@@ -33,7 +33,8 @@ def test_geometry_topology_bitexact(name):
                                        ("cavity16_simple_ic0_blend", "simple"),
                                        ("outlet_simple_jacobi", "simple"),
                                        ("channel_piso_jacobi", "piso"),
-                                       ("prism_piso_ic0", "piso")])
+                                       ("prism_piso_ic0", "piso"),
+                                       ("channel_pimple_ic0", "pimple")])
 def test_driver_bitexact(name, mode):
     d = load_golden(name)
     oc = make_case(d)
@@ -44,6 +45,8 @@ def test_driver_bitexact(name, mode):
         oc.set_field("T_s", d["in_T"])
     if mode == "simple":
         ier, _ = oc.simple_run(int(d["nsteps"]), mpi=False)
+    elif mode == "pimple":
+        ier = oc.pimple_run(int(d["nsteps"]), mpi=False)
     else:
         ier = oc.piso_run(int(d["nsteps"]), mpi=False)
     assert ier == int(d["out_ier"])
