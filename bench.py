@@ -112,6 +112,60 @@ def cpu_baseline(mesh, ctl, u0, n_needed):
                         f"extrapolated to the {n_needed} p' iterations of the FP64 Jacobi device solve"))
 
 
+def serve_bench(rate, seconds, pool, coarse, seed=4):
+    """Serving workload: Poisson arrivals of SPD pressure-correction solves on a
+    pool of seeded jittered hex/prism meshes with log-uniform sizes in
+    [1e4, 2e6] cells, log-normal face diffusivity (sigma 1), Dirichlet
+    boundaries, zero-mean Gaussian RHS, FP32 inner / FP64 reliable updates to
+    1e-10.  Each mesh carries two pre-drawn coefficient sets and right-hand
+    sides; a request picks a mesh and a set at random.  Latency = completion
+    - arrival (submission at the arrival time)."""
+    from mixsolver_amd.serve import LaplacianPattern, Server, request_mesh
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(1e4), np.log(2e6), pool))
+    kinds = np.where(rng.uniform(size=pool) < 0.5, "hex", "prism")
+    pats, data = [], []
+    for i in range(pool):
+        P = LaplacianPattern(request_mesh(sizes[i], kinds[i], seed=100 + i))
+        pats.append(P)
+        r2 = np.random.default_rng(1000 + i)
+        data.append([(P.values(r2), P.rhs(r2)) for _ in range(2)])
+    srv = Server(precond="dilu", coarse=coarse, round_iters=16)
+    pids = [srv.add_pattern(P.row_ptr, P.col_idx, P.xc) for P in pats]
+    for i in range(pool):            # pattern uploads and first-use set-up, untimed
+        srv.result(srv.submit(pids[i], *data[i][0]), timeout_ms=-1)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, int(rate * seconds * 2) + 10))
+    arrivals = arrivals[arrivals < seconds]
+    picks = rng.integers(0, pool, len(arrivals))
+    sets = rng.integers(0, 2, len(arrivals))
+    tickets = []
+    t0 = time.perf_counter()
+    for ta, pi, si in zip(arrivals, picks, sets):
+        dt = ta - (time.perf_counter() - t0)
+        if dt > 0:
+            time.sleep(dt)
+        tickets.append((srv.submit(pids[pi], *data[pi][si], tol=1e-10, maxit=20000), pi))
+    res = [(srv.result(t, timeout_ms=-1), pi) for t, pi in tickets]
+    span = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    lat = np.array([r["t_done"] for r, _ in res])
+    wait = np.array([r["t_admit"] for r, _ in res])
+    cells_it = float(sum(pats[pi].n * r["it"] for r, pi in res))
+    ok = sum(1 for r, _ in res if r["ierr"] == 0 and r["res"] <= 1e-10)
+    return {
+        "workload": f"Poisson {rate:g} req/s for {seconds:g} s; {pool} seeded meshes, log-uniform 1e4..2e6 cells, "
+                    "hex/prism, log-normal diffusivity, Dirichlet, DILU"
+                    + (f" + {coarse}-aggregate coarse" if coarse else "") + ", FP32 inner / FP64 to 1e-10",
+        "requests": len(res), "converged": ok, "offered_rate": rate,
+        "throughput_req_per_s": len(res) / span, "makespan_s": span,
+        "p50_latency_ms": float(np.percentile(lat, 50) * 1e3), "p99_latency_ms": float(np.percentile(lat, 99) * 1e3),
+        "p50_queue_ms": float(np.percentile(wait, 50) * 1e3), "p99_queue_ms": float(np.percentile(wait, 99) * 1e3),
+        "cell_iterations_per_s": cells_it / span, "mean_iterations": float(np.mean([r["it"] for r, _ in res])),
+        "mesh_cells": [int(P.n) for P in pats], "rounds": st["rounds"], "launches": st["launches"],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,6 +176,9 @@ def main():
     ap.add_argument("--coarse", type=int, default=256, help="aggregates of the p' coarse space")
     ap.add_argument("--variants", default="mixed,fp64_two_level,fp64",
                     help="comma list of variants to run (profiling runs may select one)")
+    ap.add_argument("--serve-seconds", type=float, default=3.0, help="serving run length (0 = skip)")
+    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-pool", type=int, default=10)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU runs are not part "
@@ -211,6 +268,8 @@ def main():
                            "frac": b_k / (t_k * 1e-3) / HBM_PEAK, "traffic": None,
                            "avg_launch_us": t_k * 1e3, "alg_bytes": b_k,
                            "fp64_kernel_GBps": b_k64 / (t_k64 * 1e-3) / 1e9, "fp64_avg_launch_us": t_k64 * 1e3}
+    if args.serve_seconds > 0:
+        out["serving"] = serve_bench(args.serve_rate, args.serve_seconds, args.serve_pool, args.coarse)
     if not args.no_cpu:
         # p' iterations the reference's Jacobi PCG needs for the first step (device FP64 Jacobi run)
         jac, _, _, h_jac = run_variant(Case, mesh, ctl, u0, dict(gpu_ppe_precond=0, gpu_mom_precond=2,

@@ -158,6 +158,42 @@ int ms_solve(int n, const int *row_ptr, const int *col_idx, const double *val,
              int precond, int precision, double inner_tol,
              int *it, double *res, int *ierr);
 
+/* ---- serving (no reference counterpart) ---------------------------------
+ * Continuous-batching server for SPD pressure-correction solves A x = b,
+ * x0 = 0, FP32 inner iterations with FP64 reliable updates to tol (the FP64
+ * true relative residual).  Patterns (one per mesh) are registered once and
+ * stay resident; a request brings values in the pattern's CSR order and a
+ * right-hand side.  The val and b arrays are read when the request is
+ * admitted: they must stay valid until its result has been fetched.  All
+ * admitted requests are packed into every launch; a result does not depend on
+ * which other requests share the device (bit for bit).  The server's device
+ * work runs on the library stream from its own worker thread. */
+typedef struct ms_server ms_server;
+typedef struct {
+  long long hbm_budget;  /* bytes admitted requests may hold; 0 = half of free memory */
+  int max_active;        /* requests in flight (default 64 when 0) */
+  int precond;           /* MS_PC_JACOBI or MS_PC_DILU */
+  int coarse;            /* aggregates per pattern for the coarse correction (0..256; needs xc) */
+  int round_iters;       /* iterations between scheduling decisions (default 16 when 0) */
+  double delta;          /* reliable-update trigger (default 0.1 when 0) */
+} ms_server_opts;
+int ms_server_create(ms_server **out, const ms_server_opts *opts);
+int ms_server_destroy(ms_server *s);
+/* 1-based CSR pattern (csr_t layout, sorted columns, diagonal present);
+ * xc (3,n) cell centres or NULL.  Returns the pattern id. */
+int ms_server_add_pattern(ms_server *s, int n, const int *row_ptr, const int *col_idx,
+                          const double *xc, int *pattern);
+int ms_server_submit(ms_server *s, int pattern, const double *val, const double *b,
+                     double tol, int maxit, long long *ticket);
+/* timeout_ms < 0 waits, 0 polls.  *ready = 1 when x, it, res (FP64 true
+ * relative residual), ierr (0 converged, 1 maxit, 2 breakdown) and the times
+ * from submission to admission and to completion (seconds) were written; the
+ * ticket is then released. */
+int ms_server_result(ms_server *s, long long ticket, double timeout_ms, double *x, int *it,
+                     double *res, int *ierr, double *t_admit, double *t_done, int *ready);
+/* submitted completed active pending rounds launches peak_budget_bytes patterns */
+int ms_server_stats(ms_server *s, long long *out8);
+
 /* ---- measurement (no reference counterpart) -----------------------------
  * Average duration of the dominant solver kernel (PCG SpMV with its fused
  * p.q partials) on the system's current matrix, timed with HIP events on the

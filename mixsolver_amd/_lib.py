@@ -68,6 +68,12 @@ def lib():
         "ms_time_kernel": (C.c_int, [_V, C.c_int, C.c_int, _D, _D]),
         "ms_coarse_info": (C.c_int, [_V, _I, _I, _I]),
         "ms_coarse_aggregates": (C.c_int, [C.c_int, _D, C.c_int, _I]),
+        "ms_server_create": (C.c_int, [C.POINTER(_V), _V]),
+        "ms_server_destroy": (C.c_int, [_V]),
+        "ms_server_add_pattern": (C.c_int, [_V, C.c_int, _I, _I, _D, _I]),
+        "ms_server_submit": (C.c_int, [_V, C.c_int, _D, _D, C.c_double, C.c_int, _L]),
+        "ms_server_result": (C.c_int, [_V, C.c_longlong, C.c_double, _D, _I, _D, _I, _D, _D, _I]),
+        "ms_server_stats": (C.c_int, [_V, _L]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -6,6 +6,7 @@
 #include <cstring>
 #include <memory>
 
+#include "ms_serve.hpp"
 #include "ms_system.hpp"
 
 namespace ms {
@@ -396,6 +397,77 @@ int ms_time_kernel(ms_system *h, int which, int reps, double *avg_ms, double *al
     if (avg_ms) *avg_ms = t;
     // algorithmic bytes of one CSR SpMV: nnz (val + col) + row pointers + x and y
     if (alg_bytes) *alg_bytes = (double)s.pat.nnz * (sz + 4.0) + 4.0 * (s.nc + 1) + 2.0 * s.nc * sz;
+  });
+}
+
+// ---- serving -------------------------------------------------------------
+struct ms_server {
+  ms::Server *s = nullptr;
+};
+
+int ms_server_create(ms_server **out, const ms_server_opts *o) {
+  return guarded([&] {
+    if (!out || !o) fail(MS_EINVAL, "ms_server_create: null argument");
+    ServeOpts so;
+    so.budget = o->hbm_budget;
+    so.max_active = o->max_active > 0 ? o->max_active : 64;
+    so.precond = o->precond;
+    so.coarse = o->coarse;
+    so.round_iters = o->round_iters > 0 ? o->round_iters : 16;
+    so.delta = o->delta > 0.0 ? o->delta : 0.1;
+    std::unique_ptr<ms_server> h(new ms_server());
+    h->s = server_create(so);
+    *out = h.release();
+  });
+}
+
+int ms_server_destroy(ms_server *h) {
+  return guarded([&] {
+    if (!h) return;
+    server_destroy(h->s);
+    delete h;
+  });
+}
+
+static ms::Server *srv(ms_server *h) {
+  if (!h || !h->s) fail(MS_EINVAL, "null server handle");
+  return h->s;
+}
+
+int ms_server_add_pattern(ms_server *h, int n, const int *rp, const int *ci, const double *xc, int *pattern) {
+  return guarded([&] {
+    int id = server_add_pattern(srv(h), n, rp, ci, xc);
+    if (pattern) *pattern = id;
+  });
+}
+
+int ms_server_submit(ms_server *h, int pattern, const double *val, const double *b, double tol, int maxit,
+                     long long *ticket) {
+  return guarded([&] {
+    long long t = server_submit(srv(h), pattern, val, b, tol, maxit);
+    if (ticket) *ticket = t;
+  });
+}
+
+int ms_server_result(ms_server *h, long long ticket, double timeout_ms, double *x, int *it, double *res, int *ierr,
+                     double *t_admit, double *t_done, int *ready) {
+  return guarded([&] {
+    ServeResult r;
+    const bool ok = server_result(srv(h), ticket, timeout_ms, x, &r);
+    if (ready) *ready = ok;
+    if (!ok) return;
+    if (it) *it = r.it;
+    if (res) *res = r.res;
+    if (ierr) *ierr = r.ierr;
+    if (t_admit) *t_admit = r.t_admit;
+    if (t_done) *t_done = r.t_done;
+  });
+}
+
+int ms_server_stats(ms_server *h, long long *out8) {
+  return guarded([&] {
+    if (!out8) fail(MS_EINVAL, "null output");
+    server_stats(srv(h), out8);
   });
 }
 

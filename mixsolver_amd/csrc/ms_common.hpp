@@ -29,6 +29,7 @@ void set_error(const std::string &m);
 #define MS_CHECK_LAUNCH() MS_HIP(hipGetLastError())
 
 hipStream_t stream();           // library stream (default priority)
+void ensure_runtime(int device); // select the device, create the library stream
 
 // --------------------------------------------------------- device buffers
 template <class T>

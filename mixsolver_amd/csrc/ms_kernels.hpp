@@ -1,0 +1,489 @@
+// ms_kernels.hpp -- device code shared by the single-system Krylov solvers
+// (ms_linalg.hip) and the multi-system request server (ms_serve.hip): SELL-64
+// row kernels, the finisher functors of the solver recurrences, the FP32
+// matrix copy, preconditioner set-up and the coarse-space kernels.
+#pragma once
+#include "ms_linalg.hpp"
+#include "ms_reduce.hpp"
+
+namespace ms {
+namespace {
+
+constexpr double kTiny = 2.2250738585072014e-308;   // tiny(1.0_dp)
+constexpr double kRefineTol = 1.0e-5;               // FP32 inner BiCGSTAB reduction per refinement cycle
+
+struct Mat {
+  const long long *off;
+  const int *rowlen, *col, *diagk;
+};
+
+// Row set of a cell-wise kernel: plain blocks of kBlock rows, or (coarse
+// space active) one chunk of rows of a single aggregate per workgroup.
+struct Rows {
+  const int *cptr;   // chunk row boundaries, nullptr = plain blocks
+  const int *cagg;   // aggregate of each chunk
+  int n;
+  __device__ __forceinline__ bool get(int &i, int &a) const {
+    if (cptr) {
+      const int r0 = cptr[blockIdx.x], r1 = cptr[blockIdx.x + 1];
+      i = r0 + (int)threadIdx.x;
+      a = cagg[blockIdx.x];
+      return i < r1;
+    }
+    i = blockIdx.x * kBlock + threadIdx.x;
+    a = 0;
+    return i < n;
+  }
+};
+
+// Coarse solve inside a finisher: c = Z^T r from the chunk partials (last
+// of the NV values), nu = E^-1 c, st->cnu = c.nu.
+struct CoDev {
+  int k;             // 0 = no coarse space
+  const int *agg_cptr, *agg_cidx;
+  const float *Einv;
+  double *nu;
+  KS *st;
+};
+
+template <class T>
+__device__ __forceinline__ T row_dot(int r, const Mat &A, const T *val, const T *x) {
+  const long long base = A.off[r >> 6] + (r & 63);
+  const int len = A.rowlen[r];
+  T s = T(0);
+  for (int k = 0; k < len; k++) {
+    const long long q = base + (long long)k * kSlice;
+    s = s + val[q] * x[A.col[q]];
+  }
+  return s;
+}
+
+// FP32 rows are stored with the FP64 row sum in the diagonal slot and
+// applied in difference form, q_i = s_i p_i + sum_{j!=i} a_ij (p_j - p_i):
+// algebraically equal to A p, but constants are annihilated exactly, so the
+// near-null space of the pressure-correction Laplacian survives the rounding
+// of its coefficients to FP32.
+template <class T>
+__device__ __forceinline__ T row_apply(int r, const Mat &A, const T *val, const T *x) {
+  if constexpr (sizeof(T) == 8) {
+    return row_dot(r, A, val, x);
+  } else {
+    // every slot of the slice width contributes a_k (x_col - x_i): padding
+    // (col = r, a = 0) and the diagonal slot (col = r) give zero, so the loop
+    // runs the slice width, uniform across the wavefront, without branches
+    const int s = r >> 6;
+    const long long o0 = A.off[s];
+    const int w = (int)((A.off[s + 1] - o0) >> 6);
+    const long long base = o0 + (r & 63);
+    const T xi = x[r];
+    T acc = T(0);
+#pragma unroll 4
+    for (int k = 0; k < w; k++) {
+      const long long q = base + (long long)k * kSlice;
+      acc = acc + val[q] * (x[A.col[q]] - xi);
+    }
+    return val[base + (long long)A.diagk[r] * kSlice] * xi + acc;
+  }
+}
+
+// FP32 copy of an FP64 matrix for row_apply: off-diagonals rounded, diagonal
+// slot = FP64 row sum rounded.
+__global__ void k_mat32_diff(int n, Mat A, const double *v64, float *v32) {
+  int r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= n) return;
+  const long long base = A.off[r >> 6] + (r & 63);
+  const int len = A.rowlen[r], dk = A.diagk[r];
+  double s = 0.0;
+  for (int k = 0; k < len; k++) {
+    const long long q = base + (long long)k * kSlice;
+    s = s + v64[q];
+    if (k != dk) v32[q] = (float)v64[q];
+  }
+  v32[base + (long long)dk * kSlice] = (float)s;
+  // padding slots are read by row_apply: zero them
+  const int w = (int)((A.off[(r >> 6) + 1] - A.off[r >> 6]) >> 6);
+  for (int k = len; k < w; k++) v32[base + (long long)k * kSlice] = 0.0f;
+}
+
+// ------------------------------------------------------------- finishers
+struct FinBnorm {
+  KS *st;
+  __device__ bool skip() const { return false; }
+  template <class T> __device__ void operator()(const T *a) const { st->bnrm = sqrt((double)a[0]); }
+};
+struct FinResidual {
+  KS *st;
+  __device__ bool skip() const { return false; }
+  template <class T> __device__ void operator()(const T *a) const {
+    st->rr = a[0];
+    st->res = sqrt((double)a[0]) / st->bnrm;
+    st->rmax = sqrt((double)a[0]);
+    if (st->res <= st->tol) st->done = 1;
+  }
+};
+// r.z of the two-level preconditioner: the coarse part c.nu is added when
+// the coarse space is active (kept separate so the one-level path is
+// unchanged bit for bit).
+__device__ __forceinline__ double rz_total(const KS *st, double a) { return st->coarse ? a + st->cnu : a; }
+
+struct FinNop {
+  KS *st;
+  __device__ bool skip() const { return st->done == 1; }
+  template <class T> __device__ void operator()(const T *) const {}
+};
+struct FinRz0 {
+  KS *st;
+  __device__ bool skip() const { return st->done != 0; }
+  template <class T> __device__ void operator()(const T *a) const { st->rz = rz_total(st, (double)a[0]); }
+};
+struct FinPap {
+  KS *st;
+  __device__ bool skip() const { return st->done != 0; }
+  template <class T> __device__ void operator()(const T *a) const {
+    st->it = st->it + 1;
+    st->pap = a[0];
+    if (fabs((double)a[0]) < kTiny) {
+      st->ierr = 2;
+      st->done = 1;
+    } else {
+      st->alpha = st->rz / (double)a[0];
+    }
+  }
+};
+struct FinUpdate {
+  KS *st;
+  int jac;
+  __device__ bool skip() const { return st->done != 0; }
+  template <class T> __device__ void operator()(const T *a) const {
+    st->rr = a[0];
+    st->res = sqrt((double)a[0]) / st->bnrm;
+    if (st->res <= st->tol) {
+      st->done = 1;
+      return;
+    }
+    if (jac) {
+      if (fabs(st->rz) < kTiny) {
+        st->ierr = 2;
+        st->done = 1;
+      } else {
+        const double rzn = rz_total(st, (double)a[1]);
+        st->beta = rzn / st->rz;
+        st->rz = rzn;
+      }
+    }
+  }
+};
+struct FinRz {
+  KS *st;
+  __device__ bool skip() const { return st->done != 0; }
+  template <class T> __device__ void operator()(const T *a) const {
+    if (st->ru == 2) st->ru = 0;           // reliable update fully applied
+    if (fabs(st->rz) < kTiny) {
+      st->ierr = 2;
+      st->done = 1;
+    } else {
+      const double rzn = rz_total(st, (double)a[0]);
+      st->beta = rzn / st->rz;
+      st->rz = rzn;
+    }
+  }
+};
+// Mixed PCG runs the FP32 iteration on r/sigma (sigma = |r| at the last
+// reliable update) so its vectors stay O(1): a p' right-hand side can be
+// ~1e-14, which would otherwise push FP32 products into the denormal range.
+// Sloppy residual check of the FP32 iteration.  When the recursive residual
+// meets the tolerance the iteration pauses (done = 2) and the host runs a
+// reliable update at once; when it has dropped below delta times its largest
+// value since the last reliable update, an update is flagged (ru = 1) and
+// the host applies it at the end of the chunk.  `trigger` is 0 on the pass
+// that follows a paused update.  jac: Jacobi r.z from the second partial.
+struct FinUpdateRU {
+  KS *st;
+  int jac, trigger;
+  __device__ bool skip() const { return st->done != 0; }
+  template <class T> __device__ void operator()(const T *a) const {
+    const double rn = sqrt((double)a[0]);              // scaled sloppy |r|
+    st->res = rn * st->sigma / st->bnrm;
+    if (trigger) {
+      if (st->res <= st->tol) {            // pause: the host verifies convergence in FP64
+        st->ru = 1;
+        st->done = 2;
+        return;
+      }
+      if (rn < st->delta * st->rmax) st->ru = 1;   // applied at the end of the chunk
+    }
+    if (rn > st->rmax) st->rmax = rn;
+    if (!trigger) st->ru = 0;              // rescale pass done
+    if (jac) {
+      if (fabs(st->rz) < kTiny) {
+        st->ierr = 2;
+        st->done = 1;
+      } else {
+        const double rzn = rz_total(st, (double)a[1]);
+        st->beta = rzn / st->rz;
+        st->rz = rzn;
+      }
+    }
+  }
+};
+struct FinRU {         // after x += sigma*y and r = b - A x in FP64
+  KS *st;
+  __device__ bool skip() const { return st->done == 1 || st->ru != 1; }
+  template <class T> __device__ void operator()(const T *a) const {
+    const double rt = sqrt((double)a[0]);
+    st->rr = a[0];
+    st->res = rt / st->bnrm;
+    st->nru = st->nru + 1;
+    if (st->res <= st->tol) {
+      st->done = 1;
+      return;
+    }
+    st->scale = st->sigma / rt;            // p and r.z move to the new scale
+    st->rz = st->rz * st->scale * st->scale;
+    st->sigma = rt;
+    st->rmax = 1.0;
+    st->ru = 2;                            // k_ru_rescale pending
+    st->done = 0;                          // resume the iteration
+  }
+};
+struct FinBiRho {
+  KS *st;
+  __device__ bool skip() const { return st->done != 0; }
+  template <class T> __device__ void operator()(const T *a) const {
+    st->it = st->it + 1;
+    st->rho = a[0];
+    if (fabs((double)a[0]) < kTiny) {
+      st->ierr = 2;
+      st->done = 1;
+    } else {
+      st->beta = ((double)a[0] / st->rho_old) * (st->alpha / st->omega);
+    }
+  }
+};
+struct FinBiAlpha {
+  KS *st;
+  __device__ bool skip() const { return st->done != 0; }
+  template <class T> __device__ void operator()(const T *a) const { st->alpha = st->rho / (double)a[0]; }
+};
+struct FinBiS {
+  KS *st;
+  __device__ bool skip() const { return st->done != 0; }
+  template <class T> __device__ void operator()(const T *a) const {
+    st->snrm = sqrt((double)a[0]);
+    if (st->snrm / st->bnrm <= st->tol) {
+      st->res = st->snrm / st->bnrm;
+      st->done = 2;
+    }
+  }
+};
+struct FinBiOmega {
+  KS *st;
+  __device__ bool skip() const { return st->done != 0; }
+  template <class T> __device__ void operator()(const T *a) const {
+    st->tt = a[0];
+    if ((double)a[0] < kTiny) {
+      st->ierr = 2;
+      st->done = 1;
+      return;
+    }
+    st->omega = (double)a[1] / (double)a[0];
+    if (fabs(st->omega) < kTiny) {
+      st->ierr = 2;
+      st->done = 1;
+    }
+  }
+};
+struct FinBiXR {
+  KS *st;
+  __device__ bool skip() const { return st->done != 0; }
+  template <class T> __device__ void operator()(const T *a) const {
+    st->rr = a[0];
+    st->res = sqrt((double)a[0]) / st->bnrm;
+    if (st->res <= st->tol) {
+      st->done = 1;
+      return;
+    }
+    st->rho_old = st->rho;
+  }
+};
+
+// ------------------------------------------------------------ basic kernels
+template <class T>
+__global__ void k_spmv(int n, Mat A, const T *val, const T *x, T *y) {
+  int r = blockIdx.x * kBlock + threadIdx.x;
+  if (r < n) y[r] = row_dot(r, A, val, x);
+}
+
+template <class T>
+__global__ void k_extract_diag(int n, Mat A, const T *val, T *d) {
+  int r = blockIdx.x * kBlock + threadIdx.x;
+  if (r < n) d[r] = val[A.off[r >> 6] + (r & 63) + (long long)A.diagk[r] * kSlice];
+}
+
+__global__ void k_to_float(long long n, const double *a, float *b) {
+  long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) b[i] = (float)a[i];
+}
+
+template <class T>
+__global__ void k_sumsq(int n, const T *b, T *part) {
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  T v[1] = {i < n ? b[i] * b[i] : T(0)};
+  block_partials<OpSum, 1>(v, part);
+}
+
+template <class T>
+__global__ void k_residual(int n, Mat A, const T *val, const T *b, const T *x, T *r, T *part) {
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  T v[1] = {T(0)};
+  if (i < n) {
+    T ri = b[i] - row_dot(i, A, val, x);
+    r[i] = ri;
+    v[0] = ri * ri;
+  }
+  block_partials<OpSum, 1>(v, part);
+}
+
+// ------------------------------------------------ preconditioner set-up
+template <class T>
+__global__ void k_dilu_setup(int r0, int r1, Mat A, const long long *tpos, const T *val, T *d) {
+  int i = r0 + blockIdx.x * kBlock + threadIdx.x;
+  if (i >= r1) return;
+  const long long bi = A.off[i >> 6] + (i & 63);
+  const int dpi = A.diagk[i];
+  T di = val[bi + (long long)dpi * kSlice];
+  for (int k = 0; k < dpi; k++) {
+    const long long q = bi + (long long)k * kSlice;
+    di = di - val[q] * val[tpos[q]] / d[A.col[q]];
+  }
+  d[i] = di;
+}
+
+// ------------------------------------------------------------ coarse space
+// Finisher of the update reduction with the coarse solve (one workgroup):
+// the scalar sums as k_finish, then c_a = sum of the last partial over the
+// chunks of aggregate a (fixed order), nu = E^-1 c and c.nu, before the
+// functor runs on thread 0.
+// Coarse solve by the whole finisher workgroup (kFinThreads threads): four
+// threads per aggregate (quarter q = t / 256) each sum every fourth chunk
+// partial (the last of the NV values) and every fourth block of E^-1
+// columns; quarters are combined in q order.  Writes nu and co.st->cnu.
+template <int NV, class T>
+__device__ void coarse_solve(const T *partial, const CoDev &co) {
+  __shared__ double c[kMaxAgg], part4[4][kMaxAgg];
+  const int a = threadIdx.x & (kMaxAgg - 1), q = threadIdx.x / kMaxAgg;
+  if (a < co.k) {
+    double s = 0.0;
+    for (int j = co.agg_cptr[a] + q; j < co.agg_cptr[a + 1]; j += 4)
+      s = s + (double)partial[co.agg_cidx[j] * NV + NV - 1];
+    part4[q][a] = s;
+  }
+  __syncthreads();
+  if (q == 0 && a < co.k) c[a] = ((part4[0][a] + part4[1][a]) + part4[2][a]) + part4[3][a];
+  __syncthreads();
+  const int kq = (co.k + 3) / 4, b0 = q * kq, b1 = min(co.k, b0 + kq);
+  if (a < co.k) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;    // E^-1 is symmetric: column a
+    int b = b0;
+    for (; b + 3 < b1; b += 4) {
+      const float e0 = co.Einv[(size_t)b * co.k + a], e1 = co.Einv[(size_t)(b + 1) * co.k + a];
+      const float e2 = co.Einv[(size_t)(b + 2) * co.k + a], e3 = co.Einv[(size_t)(b + 3) * co.k + a];
+      s0 = s0 + (double)e0 * c[b];
+      s1 = s1 + (double)e1 * c[b + 1];
+      s2 = s2 + (double)e2 * c[b + 2];
+      s3 = s3 + (double)e3 * c[b + 3];
+    }
+    for (; b < b1; b++) s0 = s0 + (double)co.Einv[(size_t)b * co.k + a] * c[b];
+    part4[q][a] = ((s0 + s1) + s2) + s3;
+  }
+  __syncthreads();
+  double prod[1] = {0.0};
+  if (q == 0 && a < co.k) {
+    const double nu = ((part4[0][a] + part4[1][a]) + part4[2][a]) + part4[3][a];
+    co.nu[a] = nu;
+    prod[0] = nu * c[a];
+  }
+  __shared__ double lds2[kFinThreads / 64];
+  block_reduce<OpSum, 1, double, kFinThreads>(prod, lds2);
+  if (threadIdx.x == 0) co.st->cnu = prod[0];
+}
+
+template <int NV, class T, class Fin>
+__global__ __launch_bounds__(kFinThreads) void k_finish_co(int nparts, const T *partial, Fin fin, CoDev co) {
+  if (fin.skip()) return;
+  T acc[NV];
+  fin_gather<OpSum, NV, T>(nparts, partial, acc);
+  __shared__ T lds[NV * (kFinThreads / 64)];
+  block_reduce<OpSum, NV, T, kFinThreads>(acc, lds);
+  if (co.k > 0) coarse_solve<NV, T>(partial, co);
+  if (threadIdx.x == 0) fin(acc);
+}
+
+__global__ void k_fill64(long long n, double *v, double x) {
+  long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) v[i] = x;
+}
+
+// E = Z^T A Z: one wavefront per nonzero (a,b) sums its matrix slots (listed
+// in ascending order) and writes E[a*k+b]; absent entries stay zero.
+__global__ void k_coarse_E(int nE, const int *e_ptr, const int *e_key, const int *e_slot, const double *val,
+                           double *E) {
+  const int e = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (e >= nE) return;
+  double s = 0.0;
+  for (int j = e_ptr[e] + lane; j < e_ptr[e + 1]; j += 64) s = s + val[e_slot[j]];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s = s + __shfl_down(s, o, 64);
+  if (lane == 0) E[e_key[e]] = s;
+}
+
+// In-place Gauss-Jordan inversion of the SPD matrix E (k <= 256, no
+// pivoting needed) in FP32, one workgroup of 1024 threads holding the
+// padded 256 x 256 matrix in registers: thread t owns column t % 256 of rows
+// t/256 + 4m, m = 0..63.  Pivot step j with pivot p = a_jj:
+//   a_jj <- 1/p,  a_jc <- a_jc/p,  a_rj <- -a_rj/p,  a_rc <- a_rc - a_rj a_jc/p.
+constexpr int kInvThreads = 1024;
+__global__ __launch_bounds__(kInvThreads) void k_coarse_inv(int k, const double *E, float *Einv) {
+  const int t = threadIdx.x, c = t & 255, rb = t >> 8;
+  float a[64];
+#pragma unroll
+  for (int m = 0; m < 64; m++) {
+    const int r = rb + 4 * m;
+    a[m] = (r < k && c < k) ? (float)E[(size_t)r * k + c] : (r == c ? 1.0f : 0.0f);
+  }
+  __shared__ float prow[256], pcol[256];
+  for (int j = 0; j < k; j++) {
+    const int mj = j >> 2, rj = j & 3;
+    if (rb == rj) {
+#pragma unroll
+      for (int m = 0; m < 64; m++)
+        if (m == mj) prow[c] = a[m];
+    }
+    if (c == j) {
+#pragma unroll
+      for (int m = 0; m < 64; m++) pcol[rb + 4 * m] = a[m];
+    }
+    __syncthreads();
+    const float inv = 1.0f / prow[j];
+    const float rowc = prow[c] * inv;
+#pragma unroll
+    for (int m = 0; m < 64; m++) {
+      const int r = rb + 4 * m;
+      if (r == j) a[m] = (c == j) ? inv : rowc;
+      else if (c == j) a[m] = -pcol[r] * inv;
+      else a[m] = a[m] - pcol[r] * rowc;
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int m = 0; m < 64; m++) {
+    const int r = rb + 4 * m;
+    if (r < k && c < k) Einv[(size_t)r * k + c] = a[m];
+  }
+}
+
+}  // namespace
+
+static inline Mat matof(const SellPattern *p) { return Mat{p->d_off.p, p->d_rowlen.p, p->d_col.p, p->d_diagk.p}; }
+
+}  // namespace ms

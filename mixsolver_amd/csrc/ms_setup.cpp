@@ -212,7 +212,7 @@ static std::vector<int> rcm_order(int n, const std::vector<int> &ptr, const std:
 }
 
 // Greedy colouring visiting cells in `visit` order; returns colour per cell.
-static std::vector<int> greedy_colors(int n, const std::vector<int> &ptr, const std::vector<int> &adj,
+std::vector<int> greedy_colors(int n, const std::vector<int> &ptr, const std::vector<int> &adj,
                                       const std::vector<int> &visit, int &ncol) {
   std::vector<int> color(n, -1);
   std::vector<int> mark(64, -1);
@@ -273,17 +273,22 @@ std::vector<int> rcb_aggregates(int n, const std::vector<double> &xc, int k) {
   return agg;
 }
 
-// Chunks, chunk lists per aggregate and the gather map of E = Z^T A Z over
-// the device numbering (aggregate of device row r = agg_dev[r]).
-static void build_coarse(SellPattern &P, const std::vector<int> &agg_dev, int k) {
+// Chunks (rows of one aggregate and one colour, at most kBlock), chunk lists
+// per aggregate and the gather map of E = Z^T A Z over the device numbering
+// (aggregate of device row r = agg_dev[r]).  k = 0 builds the chunks only.
+void build_coarse(SellPattern &P, const std::vector<int> &agg_dev, int k) {
   Coarse &C = P.coarse;
   const int n = P.n;
   C.k = k;
   C.chunk_ptr.assign(1, 0);
   C.chunk_agg.clear();
+  // colour of each row, so that no chunk straddles a colour boundary
+  std::vector<int> col_of(n, 0);
+  for (size_t c = 0; c + 1 < P.color_ptr.size(); c++)
+    for (int r = P.color_ptr[c]; r < P.color_ptr[c + 1]; r++) col_of[r] = (int)c;
   for (int r = 0; r < n; r++) {
     const int start = C.chunk_ptr.back();
-    if (r > start && (agg_dev[r] != agg_dev[start] || r - start == kBlock)) {
+    if (r > start && (agg_dev[r] != agg_dev[start] || col_of[r] != col_of[start] || r - start == kBlock)) {
       C.chunk_agg.push_back(agg_dev[start]);
       C.chunk_ptr.push_back(r);
     }
@@ -291,6 +296,11 @@ static void build_coarse(SellPattern &P, const std::vector<int> &agg_dev, int k)
   C.chunk_agg.push_back(agg_dev[C.chunk_ptr.back()]);
   C.chunk_ptr.push_back(n);
   C.nchunks = (int)C.chunk_agg.size();
+  if (k == 0) {          // chunks only (aggregate ids all 0): no coarse space
+    C.d_chunk_ptr.upload(C.chunk_ptr);
+    C.d_chunk_agg.upload(C.chunk_agg);
+    return;
+  }
   C.agg_cptr.assign(k + 1, 0);
   for (int c = 0; c < C.nchunks; c++) C.agg_cptr[C.chunk_agg[c] + 1]++;
   for (int a = 0; a < k; a++) C.agg_cptr[a + 1] += C.agg_cptr[a];

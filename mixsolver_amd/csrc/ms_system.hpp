@@ -92,6 +92,9 @@ System *create_system(int nnodes, const double *x, int nfaces, const long long *
 // every geometric length so the device coefficients equal the reference's.
 double norm2_ref(const double *v, int n);
 std::vector<int> rcb_aggregates(int n, const std::vector<double> &xc, int k);
+std::vector<int> greedy_colors(int n, const std::vector<int> &ptr, const std::vector<int> &adj,
+                               const std::vector<int> &visit, int &ncol);
+void build_coarse(SellPattern &P, const std::vector<int> &agg_dev, int k);
 
 // build a SellPattern from a 0-based CSR (columns sorted per row)
 void csr_to_sell(SellPattern &pat, int n, const std::vector<int> &rp, const std::vector<int> &ci);

@@ -170,3 +170,26 @@ def test_fortran_binding_layout_and_link(tmp_path):
     assert [C.sizeof(MsCtrl), C.sizeof(MsBcGroup)] == [int(v) for v in sizes]
     r = subprocess.run([str(tmp_path / "check_binding")] + sizes, capture_output=True, text=True)
     assert r.returncode == 0, r.stdout + r.stderr
+
+
+@pytest.mark.parametrize("kind", ["hex", "prism"])
+def test_serving_workload_systems(kind):
+    """The serving workload's systems: 1-based sorted CSR with the diagonal,
+    symmetric, zero row sums on interior cells and positive ones on cells with
+    a Dirichlet face (weak diagonal dominance), SPD; RHS zero-mean."""
+    from mixsolver_amd.serve import LaplacianPattern, request_mesh
+    P = LaplacianPattern(request_mesh(700, kind, 4))
+    rng = np.random.default_rng(2)
+    val, b = P.values(rng), P.rhs(rng)
+    A = P.dense(val)
+    assert np.array_equal(A, A.T)
+    rs = A.sum(axis=1)
+    bnd = np.zeros(P.n, bool)
+    bnd[P.mesh.c0[P.mesh.c1 == 0] - 1] = True
+    assert np.all(np.abs(rs[~bnd]) <= 1e-12 * np.abs(np.diag(A)[~bnd]))
+    assert np.all(rs[bnd] > 0)
+    assert np.linalg.eigvalsh(A).min() > 0
+    assert abs(b.sum()) < 1e-10
+    for i in range(P.n):
+        cols = P.col_idx[P.row_ptr[i] - 1:P.row_ptr[i + 1] - 1]
+        assert np.all(np.diff(cols) > 0) and (i + 1) in cols
