@@ -261,11 +261,23 @@ def main():
         "ppe_final_res_fp64_two_level_max": resmax("fp64_two_level"),
         "cell_iterations_per_s": cells * float(np.sum(runs[lead][3][:, 6])) / runs[lead][1],
     }
+    # calibration kernel of the traffic counters (rocprofv3 --pmc runs of this
+    # script read its FETCH_SIZE / WRITE_SIZE beside the SpMV's)
+    from mixsolver_amd import _lib
+    import ctypes
+    cp_ms = ctypes.c_double(0.0)
+    _lib.check(_lib.lib().ms_time_copy(1 << 30, 20, ctypes.byref(cp_ms)))
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "pcg_spmv_traffic.json")
+    if os.path.exists(tpath):      # HBM bytes per launch measured by the counter runs (committed summary)
+        with open(tpath) as fh:
+            traffic = json.load(fh)["traffic_bytes"].get("k_pcg_spmv<float>")
     if "kernel" in runs:
         t_k, b_k, t_k64, b_k64 = runs["kernel"]
         out["roofline"] = {"bound": "hbm", "kernel": "k_pcg_spmv<float> (SELL-64, difference form, fused p.q)",
                            "achieved": b_k / (t_k * 1e-3) / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                           "frac": b_k / (t_k * 1e-3) / HBM_PEAK, "traffic": None,
+                           "frac": b_k / (t_k * 1e-3) / HBM_PEAK, "traffic": traffic,
+                           "traffic_source": "profiles/pcg_mixed_channel2M.md (FETCH_SIZE, WRITE_SIZE, copy-calibrated)",
                            "avg_launch_us": t_k * 1e3, "alg_bytes": b_k,
                            "fp64_kernel_GBps": b_k64 / (t_k64 * 1e-3) / 1e9, "fp64_avg_launch_us": t_k64 * 1e3}
     if args.serve_seconds > 0:

@@ -200,6 +200,10 @@ int ms_server_stats(ms_server *s, long long *out8);
  * library stream over reps launches.  which: 0 = FP32 difference-form kernel
  * of the mixed path, 1 = FP64 kernel.  alg_bytes: nnz*(s+4) + 4(n+1) + 2ns. */
 int ms_time_kernel(ms_system *s, int which, int reps, double *avg_ms, double *alg_bytes);
+/* Average duration of a plain device-to-device copy kernel (k_copy4) moving
+ * `bytes` (read once, written once): the calibration of traffic counters and
+ * the achievable-bandwidth reference beside the published peak. */
+int ms_time_copy(long long bytes, int reps, double *avg_ms);
 
 #ifdef __cplusplus
 }

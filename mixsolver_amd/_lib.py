@@ -66,6 +66,7 @@ def lib():
         "ms_solve": (C.c_int, [C.c_int, _I, _I, _D, _D, _D, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int,
                                C.c_double, _I, _D, _I]),
         "ms_time_kernel": (C.c_int, [_V, C.c_int, C.c_int, _D, _D]),
+        "ms_time_copy": (C.c_int, [C.c_longlong, C.c_int, _D]),
         "ms_coarse_info": (C.c_int, [_V, _I, _I, _I]),
         "ms_coarse_aggregates": (C.c_int, [C.c_int, _D, C.c_int, _I]),
         "ms_server_create": (C.c_int, [C.POINTER(_V), _V]),

@@ -387,6 +387,15 @@ int ms_solve(int n, const int *rp, const int *ci, const double *v, const double 
   });
 }
 
+int ms_time_copy(long long bytes, int reps, double *avg_ms) {
+  return guarded([&] {
+    if (bytes < 16 || reps <= 0) fail(MS_EINVAL, "ms_time_copy: bytes >= 16, reps > 0");
+    ensure_runtime(-1);
+    const double t = time_copy(bytes, reps);
+    if (avg_ms) *avg_ms = t;
+  });
+}
+
 int ms_time_kernel(ms_system *h, int which, int reps, double *avg_ms, double *alg_bytes) {
   return guarded([&] {
     System &s = sys(h);

@@ -364,6 +364,68 @@ __global__ void k_dilu_setup(int r0, int r1, Mat A, const long long *tpos, const
 // the scalar sums as k_finish, then c_a = sum of the last partial over the
 // chunks of aggregate a (fixed order), nu = E^-1 c and c.nu, before the
 // functor runs on thread 0.
+// Workgroups are dispatched round-robin over the 8 XCDs (blockIdx % 8).
+// xcd_block maps them so that each XCD works on one contiguous range of
+// logical blocks; the neighbour gathers of its rows then mostly hit its own
+// L2 instead of being fetched again by every XCD.  A bijection of [0, nb).
+constexpr int kXcd = 8;
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+  const int q = nb / kXcd, r = nb % kXcd, x = b % kXcd;
+  return x * q + min(x, r) + b / kXcd;
+}
+
+// Coarse solve riding along a DILU forward-sweep launch: nwg extra
+// workgroups of kBlock threads, each recomputing c = Z^T r from the chunk
+// partials (third of three values per chunk) and four rows of nu = E^-1 c;
+// c.nu partials go to cnu_part[g], summed with the sweep's r.z partials.
+template <class T>
+struct CoWork {
+  int k, nwg;
+  const int *agg_cptr, *agg_cidx;
+  const float *Einv;
+  double *nu;
+  const T *cpart;
+  T *cnu_part;
+};
+
+template <class T>
+__device__ void coarse_ride(const CoWork<T> &cw, int g) {
+  __shared__ double c[kMaxAgg];
+  const int t = threadIdx.x;
+  for (int b = t; b < cw.k; b += kBlock) {
+    const int j0 = cw.agg_cptr[b], j1 = cw.agg_cptr[b + 1];
+    double s = 0.0;
+    int j = j0;
+    for (; j + 7 < j1; j += 8) {       // eight independent far loads in flight, summed in index order
+      T x[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) x[u] = cw.cpart[cw.agg_cidx[j + u] * 3 + 2];
+#pragma unroll
+      for (int u = 0; u < 8; u++) s = s + (double)x[u];
+    }
+    for (; j < j1; j++) s = s + (double)cw.cpart[cw.agg_cidx[j] * 3 + 2];
+    c[b] = s;
+  }
+  __syncthreads();
+  const int w = t >> 6, lane = t & 63, a = g * (kBlock / 64) + w;
+  double acc = 0.0;
+  if (a < cw.k)
+    for (int b = lane; b < cw.k; b += 64) acc = acc + (double)cw.Einv[(size_t)a * cw.k + b] * c[b];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) acc = acc + __shfl_down(acc, o, 64);
+  __shared__ double pr[kBlock / 64];
+  if (lane == 0) {
+    double cn = 0.0;
+    if (a < cw.k) {
+      cw.nu[a] = acc;
+      cn = acc * c[a];
+    }
+    pr[w] = cn;
+  }
+  __syncthreads();
+  if (t == 0) cw.cnu_part[g] = (T)(((pr[0] + pr[1]) + pr[2]) + pr[3]);
+}
+
 // Coarse solve by the whole finisher workgroup (kFinThreads threads): four
 // threads per aggregate (quarter q = t / 256) each sum every fourth chunk
 // partial (the last of the NV values) and every fourth block of E^-1

@@ -58,7 +58,7 @@ __global__ void k_rsum(Rows R, const T *r, const KS *st, T *part) {
 template <class T>
 __global__ void k_pcg_spmv(int n, Mat A, const T *val, const T *p, T *q, const KS *st, T *part) {
   if (st->done) return;
-  int i = blockIdx.x * kBlock + threadIdx.x;
+  int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   T v[1] = {T(0)};
   if (i < n) {
     T qi = row_apply(i, A, val, p);
@@ -397,11 +397,19 @@ __global__ void k_dilu_first(int r0, int r1, const T *dg, const T *r, T *z, cons
   if (i < r1) z[i] = r[i] / dg[i];
 }
 
+// forward sweep of one colour [r0, r1) over nbr row blocks; the first
+// cw.nwg blocks carry the coarse solve when a coarse space rides along (they
+// start first and overlap the sweep)
 template <class T>
-__global__ void k_dilu_fwd(int r0, int r1, Mat A, const T *val, const T *dg, const T *r, T *z, int last,
-                           const int *done, T *part) {
+__global__ void k_dilu_fwd(int r0, int r1, int nbr, Mat A, const T *val, const T *dg, const T *r, T *z, int last,
+                           const int *done, T *part, CoWork<T> cw) {
   if (*done) return;
-  int i = r0 + blockIdx.x * kBlock + threadIdx.x;
+  if ((int)blockIdx.x < cw.nwg) {
+    coarse_ride<T>(cw, blockIdx.x);
+    return;
+  }
+  const int rb = blockIdx.x - cw.nwg;
+  int i = r0 + xcd_block(rb, nbr) * kBlock + threadIdx.x;
   T v[1] = {T(0)};
   if (i < r1) {
     const long long bi = A.off[i >> 6] + (i & 63);
@@ -415,14 +423,14 @@ __global__ void k_dilu_fwd(int r0, int r1, Mat A, const T *val, const T *dg, con
     z[i] = zi;
     if (last) v[0] = r[i] * zi;
   }
-  if (part) block_partials<OpSum, 1>(v, part);
+  if (part) block_partials<OpSum, 1>(v, part, rb);
 }
 
 template <class T>
 __global__ void k_dilu_bwd(int r0, int r1, Mat A, const T *val, const T *dg, const T *r, T *z, const int *done,
                            T *part) {
   if (*done) return;
-  int i = r0 + blockIdx.x * kBlock + threadIdx.x;
+  int i = r0 + xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   T v[1] = {T(0)};
   if (i < r1) {
     const long long bi = A.off[i >> 6] + (i & 63);
@@ -463,7 +471,40 @@ __global__ void k_mp_correct(int n, const float *x32, const KS *st, double *x) {
   if (i < n) x[i] = x[i] + sqrt(st->rr) * (double)x32[i];
 }
 
+// streaming copy (calibration of traffic counters and achievable bandwidth)
+__global__ void k_copy4(long long n4, const float4 *a, float4 *b) {
+  const long long base = (long long)blockIdx.x * kBlock * 4 + threadIdx.x;
+  float4 v[4];
+#pragma unroll
+  for (int u = 0; u < 4; u++)
+    if (base + u * kBlock < n4) v[u] = a[base + u * kBlock];
+#pragma unroll
+  for (int u = 0; u < 4; u++)
+    if (base + u * kBlock < n4) b[base + u * kBlock] = v[u];
+}
+
 }  // namespace
+
+double time_copy(long long bytes, int reps) {
+  const long long n4 = bytes / 16;
+  DBuf<float4> a(n4), b(n4);
+  MS_HIP(hipMemsetAsync(a.p, 0, n4 * 16, stream()));
+  const int grid = (int)((n4 + 4 * kBlock - 1) / (4 * kBlock));
+  for (int i = 0; i < 2; i++) hipLaunchKernelGGL(k_copy4, dim3(grid), dim3(kBlock), 0, stream(), n4, a.p, b.p);
+  hipEvent_t e0, e1;
+  MS_HIP(hipEventCreate(&e0));
+  MS_HIP(hipEventCreate(&e1));
+  MS_HIP(hipEventRecord(e0, stream()));
+  for (int i = 0; i < reps; i++) hipLaunchKernelGGL(k_copy4, dim3(grid), dim3(kBlock), 0, stream(), n4, a.p, b.p);
+  MS_HIP(hipEventRecord(e1, stream()));
+  MS_HIP(hipEventSynchronize(e1));
+  float ms = 0.0f;
+  MS_HIP(hipEventElapsedTime(&ms, e0, e1));
+  MS_HIP(hipEventDestroy(e0));
+  MS_HIP(hipEventDestroy(e1));
+  MS_CHECK_LAUNCH();
+  return (double)ms / reps;
+}
 
 // ============================================================ host side
 
@@ -476,6 +517,7 @@ Krylov::Krylov(SellPattern *pat_) : pat(pat_), n(pat_->n) {
   nblk_ = nblocks(n);
   for (DBuf<double> *b : {&r_, &r0_, &p_, &q_, &z_, &s_, &t_, &ph_, &sh_, &dg_}) b->alloc(n);
   part_.alloc(4 * ((size_t)std::max(nblk_, pat_->coarse.nchunks) + 64) * 2);
+  prz_.alloc(2 * ((size_t)nblk_ + 64) + kMaxAgg);
   st_.alloc(1);
   flag_.alloc(1);
   MS_HIP(hipHostMalloc(&hpin_, 4096, hipHostMallocDefault));
@@ -489,6 +531,7 @@ void Krylov::ensure32() {
   if (have32_) return;
   for (DBuf<float> *v : {&r32_, &r032_, &p32_, &q32_, &z32_, &s32_, &t32_, &ph32_, &sh32_, &x32_, &dg32_}) v->alloc(n);
   part32_.alloc(4 * ((size_t)std::max(nblk_, pat->coarse.nchunks) + 64) * 2);
+  prz32_.alloc(2 * ((size_t)nblk_ + 64) + kMaxAgg);
   st32_.alloc(1);
   have32_ = true;
 }
@@ -612,9 +655,11 @@ void Krylov::apply_pc(PcKind pc, const T *val, const T *fac, const T *dg, const 
       hipLaunchKernelGGL(k_dilu_first<T>, dim3(nblocks(cp[1] - cp[0])), dim3(kBlock), 0, stream(), cp[0], cp[1], dg,
                          in, out, done);
     for (int c = 1; c < nc; c++)
-      if (cp[c + 1] > cp[c])
-        hipLaunchKernelGGL(k_dilu_fwd<T>, dim3(nblocks(cp[c + 1] - cp[c])), dim3(kBlock), 0, stream(), cp[c],
-                           cp[c + 1], A, val, dg, in, out, (int)(c == nc - 1), done, (T *)nullptr);
+      if (cp[c + 1] > cp[c]) {
+        const int gb = nblocks(cp[c + 1] - cp[c]);
+        hipLaunchKernelGGL(k_dilu_fwd<T>, dim3(gb), dim3(kBlock), 0, stream(), cp[c], cp[c + 1], gb, A, val, dg, in,
+                           out, (int)(c == nc - 1), done, (T *)nullptr, CoWork<T>{});
+      }
     for (int c = nc - 2; c >= 0; c--)
       if (cp[c + 1] > cp[c])
         hipLaunchKernelGGL(k_dilu_bwd<T>, dim3(nblocks(cp[c + 1] - cp[c])), dim3(kBlock), 0, stream(), cp[c],
@@ -636,31 +681,39 @@ void Krylov::apply_pc(PcKind pc, const T *val, const T *fac, const T *dg, const 
 }
 
 // DILU sweeps of colours 1.. with fused r.z partials (one section per launch);
-// returns the number of partials written.
+// returns the number of partials written.  With cw.nwg > 0 the coarse solve
+// rides along the first forward launch and its c.nu partials follow the
+// sweep sections, so the sum of all partials is r.z of the two-level
+// preconditioner.
 template <class T>
 static int dilu_sweeps(const SellPattern *pat, Mat A, const T *val, const T *dg, const T *r, T *z, const int *done,
-                       T *part) {
+                       T *part, CoWork<T> cw = CoWork<T>{}) {
   const auto &cp = pat->color_ptr;
   const int ncol = (int)cp.size() - 1;
+  int total = 0;
+  for (int c = 1; c < ncol; c++) total += nblocks(cp[c + 1] - cp[c]);
+  for (int c = ncol - 2; c >= 0; c--) total += nblocks(cp[c + 1] - cp[c]);
+  cw.cnu_part = part + total;
   T *sec = part;
-  int nparts = 0;
+  bool first = true;
   for (int c = 1; c < ncol; c++)
     if (cp[c + 1] > cp[c]) {
-      int gb = nblocks(cp[c + 1] - cp[c]);
-      hipLaunchKernelGGL(k_dilu_fwd<T>, dim3(gb), dim3(kBlock), 0, stream(), cp[c], cp[c + 1], A, val, dg, r, z,
-                         (int)(c == ncol - 1), done, sec);
+      const int gb = nblocks(cp[c + 1] - cp[c]);
+      CoWork<T> cwl = cw;
+      if (!first) cwl.nwg = 0;
+      hipLaunchKernelGGL(k_dilu_fwd<T>, dim3(gb + cwl.nwg), dim3(kBlock), 0, stream(), cp[c], cp[c + 1], gb, A, val,
+                         dg, r, z, (int)(c == ncol - 1), done, sec, cwl);
+      first = false;
       sec += gb;
-      nparts += gb;
     }
   for (int c = ncol - 2; c >= 0; c--)
     if (cp[c + 1] > cp[c]) {
-      int gb = nblocks(cp[c + 1] - cp[c]);
+      const int gb = nblocks(cp[c + 1] - cp[c]);
       hipLaunchKernelGGL(k_dilu_bwd<T>, dim3(gb), dim3(kBlock), 0, stream(), cp[c], cp[c + 1], A, val, dg, r, z, done,
                          sec);
       sec += gb;
-      nparts += gb;
     }
-  return nparts;
+  return total + (first ? 0 : cw.nwg);
 }
 
 // ---- PCG (FP64, reference recurrences) -------------------------------------
@@ -673,6 +726,21 @@ static CoDev codev_of(const SellPattern *pat, bool coarse, KS *st) {
   const Coarse &C = pat->coarse;
   return CoDev{C.k, C.d_agg_cptr.p, C.d_agg_cidx.p, C.d_Einv.p, C.d_nu.p, st};
 }
+template <class T>
+static CoWork<T> cowork_of(const SellPattern *pat, bool fused, const T *cpart) {
+  CoWork<T> cw{};
+  if (!fused) return cw;
+  const Coarse &C = pat->coarse;
+  cw.k = C.k;
+  cw.nwg = (C.k + kBlock / 64 - 1) / (kBlock / 64);
+  cw.agg_cptr = C.d_agg_cptr.p;
+  cw.agg_cidx = C.d_agg_cidx.p;
+  cw.Einv = C.d_Einv.p;
+  cw.nu = C.d_nu.p;
+  cw.cpart = cpart;
+  return cw;
+}
+
 template <int NV, class T, class Fin>
 static void finish_co(int nparts, const T *part, Fin fin, const CoDev &co) {
   hipLaunchKernelGGL((k_finish_co<NV, T, Fin>), dim3(1), dim3(kFinThreads), 0, stream(), nparts, part, fin, co);
@@ -698,10 +766,12 @@ SolveResult Krylov::pcg(const double *val, const double *b, double *x, double to
   KS *st = st_.p;
   double *part = part_.p;
   coarse = coarse && pat->coarse.k > 0;
+  // DILU with a coarse space: the coarse solve rides along the forward sweep
+  const bool fused = coarse && pc == PC_DILU && pat->color_ptr.size() > 2;
   KS h{};
   h.tol = tol;
   h.maxit = maxit;
-  h.coarse = coarse;
+  h.coarse = coarse && !fused;
   MS_HIP(hipMemcpyAsync(st, &h, sizeof(h), hipMemcpyHostToDevice, S));
   hipLaunchKernelGGL(k_sumsq<double>, G, B, 0, S, n, b, part);
   finish<OpSum, 1>(nblk_, part, FinBnorm{st});
@@ -729,15 +799,29 @@ SolveResult Krylov::pcg(const double *val, const double *b, double *x, double to
   const dim3 GR(coarse ? pat->coarse.nchunks : nblk_);
   const CoDev co = codev_of(pat, coarse, st);
   const double *nu = coarse ? pat->coarse.d_nu.p : nullptr;
-  if (coarse) {
+  const CoWork<double> cw = cowork_of<double>(pat, fused, part);
+  double *prz = prz_.p;
+  if (fused) {
     hipLaunchKernelGGL(k_rsum<double>, GR, B, 0, S, RW, r_.p, st, part);
-    finish_co<3>((int)GR.x, part, FinNop{st}, co);
+    const auto &cp = pat->color_ptr;
+    hipLaunchKernelGGL(k_dilu_first<double>, dim3(nblocks(cp[1] - cp[0])), B, 0, S, cp[0], cp[1], dg_.p, r_.p, z_.p,
+                       done);
+    const int np = dilu_sweeps<double>(pat, A, val, dg_.p, r_.p, z_.p, done, prz, cw);
+    finish<OpSum, 1>(np, prz, FinRz0{st});
+    MS_HIP(hipMemsetAsync(p_.p, 0, n * sizeof(double), S));
+    hipLaunchKernelGGL(k_pcg_p<double>, GR, B, 0, S, RW, r_.p, z_.p, dg_.p, 0, p_.p, nu, st);   // beta = 0
+  } else {
+    if (coarse) {
+      hipLaunchKernelGGL(k_rsum<double>, GR, B, 0, S, RW, r_.p, st, part);
+      finish_co<3>((int)GR.x, part, FinNop{st}, co);
+    }
+    if (!jac) apply_pc<double>(pc, val, fac_.p, dg_.p, r_.p, z_.p, done);
+    hipLaunchKernelGGL(k_pcg_start<double>, GR, B, 0, S, RW, r_.p, z_.p, dg_.p, jac, p_.p, nu, st, part);
+    finish<OpSum, 1>((int)GR.x, part, FinRz0{st});
   }
-  if (!jac) apply_pc<double>(pc, val, fac_.p, dg_.p, r_.p, z_.p, done);
-  hipLaunchKernelGGL(k_pcg_start<double>, GR, B, 0, S, RW, r_.p, z_.p, dg_.p, jac, p_.p, nu, st, part);
-  finish<OpSum, 1>((int)GR.x, part, FinRz0{st});
   const int mode = jac ? 1 : (pc == PC_DILU ? 2 : 0);
   const int c0end = pat->color_ptr[1];
+  const CoDev co_upd = fused ? CoDev{0, nullptr, nullptr, nullptr, nullptr, st} : co;
   int enq = 0, chunk = 8;
   while (true) {
     int m = std::min(chunk, maxit - enq);
@@ -746,10 +830,10 @@ SolveResult Krylov::pcg(const double *val, const double *b, double *x, double to
       finish<OpSum, 1>(nblk_, part, FinPap{st});
       hipLaunchKernelGGL((k_pcg_update<double, double>), GR, B, 0, S, RW, p_.p, q_.p, x, r_.p, z_.p, dg_.p, mode,
                          c0end, st, part);
-      finish_co<3>((int)GR.x, part, FinUpdate{st, jac}, co);
+      finish_co<3>((int)GR.x, part, FinUpdate{st, jac}, co_upd);
       if (mode == 2) {
-        int np = dilu_sweeps<double>(pat, A, val, dg_.p, r_.p, z_.p, done, part);
-        finish<OpSum, 1>(np, part, FinRz{st});
+        int np = dilu_sweeps<double>(pat, A, val, dg_.p, r_.p, z_.p, done, prz, cw);
+        finish<OpSum, 1>(np, prz, FinRz{st});
       } else if (mode == 0) {
         apply_pc<double>(pc, val, fac_.p, dg_.p, r_.p, z_.p, done);
         hipLaunchKernelGGL(k_dot<double>, G, B, 0, S, n, r_.p, z_.p, st, part);
@@ -958,31 +1042,45 @@ SolveResult Krylov::mixed(const double *val, const double *b, double *x, double 
   const int mode = jac ? 1 : (pc == PC_DILU ? 2 : 0);
   const int c0end = pat->color_ptr[1];
   coarse = coarse && pat->coarse.k > 0;
+  // DILU with a coarse space: the coarse solve rides along the forward sweep
+  const bool fused = coarse && pc == PC_DILU && pat->color_ptr.size() > 2;
   if (coarse) {
-    h.coarse = 1;
+    h.coarse = fused ? 0 : 1;
     MS_HIP(hipMemcpyAsync(&st->coarse, &h.coarse, sizeof(int), hipMemcpyHostToDevice, S));
     coarse_setup(val);
   }
   const Rows RW = rows_of(pat, coarse);
   const dim3 GR(coarse ? pat->coarse.nchunks : nblk_);
-  const CoDev co = codev_of(pat, coarse, st);
+  const CoDev co = codev_of(pat, coarse && !fused, st);
   const double *nu = coarse ? pat->coarse.d_nu.p : nullptr;
+  const CoWork<float> cw = cowork_of<float>(pat, fused, part32);
+  float *prz = prz32_.p;
   MS_HIP(hipMemsetAsync(y, 0, n * sizeof(float), S));
   hipLaunchKernelGGL(k_set_sigma, dim3(1), dim3(64), 0, S, st);
   hipLaunchKernelGGL(k_scale_down, G, B, 0, S, n, r_.p, r, st);
-  if (coarse) {
+  if (fused) {
     hipLaunchKernelGGL(k_rsum<float>, GR, B, 0, S, RW, r, st, part32);
-    finish_co<3>((int)GR.x, part32, FinNop{st}, co);
+    const auto &cp = pat->color_ptr;
+    hipLaunchKernelGGL(k_dilu_first<float>, dim3(nblocks(cp[1] - cp[0])), B, 0, S, cp[0], cp[1], dg, r, z, done);
+    const int np = dilu_sweeps<float>(pat, A, v32, dg, r, z, done, prz, cw);
+    finish<OpSum, 1>(np, prz, FinRz0{st});
+    MS_HIP(hipMemsetAsync(p, 0, n * sizeof(float), S));
+    hipLaunchKernelGGL(k_pcg_p<float>, GR, B, 0, S, RW, r, z, dg, 0, p, nu, st);   // beta = 0
+  } else {
+    if (coarse) {
+      hipLaunchKernelGGL(k_rsum<float>, GR, B, 0, S, RW, r, st, part32);
+      finish_co<3>((int)GR.x, part32, FinNop{st}, co);
+    }
+    if (!jac) apply_pc<float>(pc, v32, fac, dg, r, z, done);
+    hipLaunchKernelGGL(k_pcg_start<float>, GR, B, 0, S, RW, r, z, dg, jac, p, nu, st, part32);
+    finish<OpSum, 1>((int)GR.x, part32, FinRz0{st});
   }
-  if (!jac) apply_pc<float>(pc, v32, fac, dg, r, z, done);
-  hipLaunchKernelGGL(k_pcg_start<float>, GR, B, 0, S, RW, r, z, dg, jac, p, nu, st, part32);
-  finish<OpSum, 1>((int)GR.x, part32, FinRz0{st});
   // preconditioner, r.z and p update: the part of an iteration after the
   // update finisher
   auto tail = [&]() {
     if (mode == 2) {
-      int np = dilu_sweeps<float>(pat, A, v32, dg, r, z, done, part32);
-      finish<OpSum, 1>(np, part32, FinRz{st});
+      int np = dilu_sweeps<float>(pat, A, v32, dg, r, z, done, prz, cw);
+      finish<OpSum, 1>(np, prz, FinRz{st});
     } else if (mode == 0) {
       apply_pc<float>(pc, v32, fac, dg, r, z, done);
       hipLaunchKernelGGL(k_dot<float>, G, B, 0, S, n, r, z, st, part32);

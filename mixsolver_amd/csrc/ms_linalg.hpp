@@ -22,6 +22,9 @@ struct SolveResult {
   int outer = 0;          // refinement cycles / reliable updates (mixed precision)
 };
 
+// average duration (ms) of a device-to-device copy of `bytes` (read + write)
+double time_copy(long long bytes, int reps);
+
 // Preconditioner kinds used internally.
 enum PcKind { PC_JACOBI = 0, PC_NATURAL_IC0 = 1, PC_NATURAL_ILU0 = 2, PC_DILU = 3 };
 
@@ -62,8 +65,8 @@ class Krylov {
   void ensure32();
   void coarse_setup(const double *val);   // E = Z^T A Z and its inverse
 
-  DBuf<double> r_, r0_, p_, q_, z_, s_, t_, ph_, sh_, fac_, dg_, part_;
-  DBuf<float> val32_, r32_, r032_, p32_, q32_, z32_, s32_, t32_, ph32_, sh32_, x32_, fac32_, dg32_, part32_;
+  DBuf<double> r_, r0_, p_, q_, z_, s_, t_, ph_, sh_, fac_, dg_, part_, prz_;   // prz_: DILU sweep partials
+  DBuf<float> val32_, r32_, r032_, p32_, q32_, z32_, s32_, t32_, ph32_, sh32_, x32_, fac32_, dg32_, part32_, prz32_;
   DBuf<KS> st_, st32_;
   DBuf<int> flag_;
   void *hpin_ = nullptr;

@@ -46,14 +46,16 @@ __device__ __forceinline__ void block_reduce(T (&v)[NV], T *lds) {
   __syncthreads();
 }
 
-// Stage 1: one partial per workgroup (NV values at partial[blockIdx*NV]).
+// Stage 1: one partial per workgroup (NV values at partial[idx*NV], idx =
+// blockIdx by default).
 template <class Op, int NV, class T>
-__device__ __forceinline__ void block_partials(T (&v)[NV], T *partial) {
+__device__ __forceinline__ void block_partials(T (&v)[NV], T *partial, int idx = -1) {
   __shared__ T lds[NV * (kBlock / 64)];
   block_reduce<Op, NV, T>(v, lds);
+  if (idx < 0) idx = blockIdx.x;
   if (threadIdx.x == 0)
 #pragma unroll
-    for (int j = 0; j < NV; j++) partial[blockIdx.x * NV + j] = v[j];
+    for (int j = 0; j < NV; j++) partial[idx * NV + j] = v[j];
 }
 
 // Stage 2 runs in one workgroup of kFinThreads threads.  The partials were

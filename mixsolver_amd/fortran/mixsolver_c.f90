@@ -19,7 +19,7 @@ module mixsolver_c
    public :: ms_simple_run, ms_piso_run, ms_pimple_run, ms_get_history
    public :: ms_compute_gradients, ms_momentum_assembly, ms_temperature_assembly
    public :: ms_flux_rhiechow, ms_outflow_mass_rescale, ms_ppe_assembly, ms_correct_fields
-   public :: ms_cg_jacobi, ms_cg_ic0, ms_bicgstab_ilu0, ms_solve, ms_time_kernel
+   public :: ms_cg_jacobi, ms_cg_ic0, ms_bicgstab_ilu0, ms_solve, ms_time_kernel, ms_time_copy
    public :: c_str
 
    integer(c_int), parameter :: MS_OK = 0, MS_EINVAL = -1, MS_EDEVICE = -2, MS_ENOMEM = -3
@@ -211,6 +211,12 @@ module mixsolver_c
          type(c_ptr), value :: s
          integer(c_int), value :: which, reps
          real(c_double) :: avg_ms, alg_bytes
+      end function
+      integer(c_int) function ms_time_copy(bytes, reps, avg_ms) bind(c, name='ms_time_copy')
+         import :: c_int, c_long_long, c_double
+         integer(c_long_long), value :: bytes
+         integer(c_int), value :: reps
+         real(c_double) :: avg_ms
       end function
    end interface
 
