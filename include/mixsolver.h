@@ -173,7 +173,7 @@ typedef struct {
   long long hbm_budget;  /* bytes admitted requests may hold; 0 = half of free memory */
   int max_active;        /* requests in flight (default 64 when 0) */
   int precond;           /* MS_PC_JACOBI or MS_PC_DILU */
-  int coarse;            /* aggregates per pattern for the coarse correction (0..256; needs xc) */
+  int coarse;            /* max aggregates per pattern (0..256; needs xc): min(coarse, n/4096) are used */
   int round_iters;       /* iterations between scheduling decisions (default 16 when 0) */
   double delta;          /* reliable-update trigger (default 0.1 when 0) */
 } ms_server_opts;
@@ -198,7 +198,9 @@ int ms_server_stats(ms_server *s, long long *out8);
  * Average duration of the dominant solver kernel (PCG SpMV with its fused
  * p.q partials) on the system's current matrix, timed with HIP events on the
  * library stream over reps launches.  which: 0 = FP32 difference-form kernel
- * of the mixed path, 1 = FP64 kernel.  alg_bytes: nnz*(s+4) + 4(n+1) + 2ns. */
+ * of the mixed path, 1 = FP64 kernel.  alg_bytes: nnz*(s+i) + 4(n+1) + 2ns with
+ * s the value size and i the column-index size of the format in use (2 for the
+ * FP32 kernel's 16-bit slice offsets, else 4). */
 int ms_time_kernel(ms_system *s, int which, int reps, double *avg_ms, double *alg_bytes);
 /* Average duration of a plain device-to-device copy kernel (k_copy4) moving
  * `bytes` (read once, written once): the calibration of traffic counters and

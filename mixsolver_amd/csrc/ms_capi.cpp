@@ -403,9 +403,10 @@ int ms_time_kernel(ms_system *h, int which, int reps, double *avg_ms, double *al
     const bool fp32 = which == 0;
     double t = s.kry->time_spmv(s.val.p, fp32, reps);
     const double sz = fp32 ? 4.0 : 8.0;
+    const double idx = (fp32 && s.pat.has16) ? 2.0 : 4.0;   // column index bytes of the format in use
     if (avg_ms) *avg_ms = t;
-    // algorithmic bytes of one CSR SpMV: nnz (val + col) + row pointers + x and y
-    if (alg_bytes) *alg_bytes = (double)s.pat.nnz * (sz + 4.0) + 4.0 * (s.nc + 1) + 2.0 * s.nc * sz;
+    // algorithmic bytes of one SpMV: nnz (value + column index) + one int per row + x and y
+    if (alg_bytes) *alg_bytes = (double)s.pat.nnz * (sz + idx) + 4.0 * (s.nc + 1) + 2.0 * s.nc * sz;
   });
 }
 

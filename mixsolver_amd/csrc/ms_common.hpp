@@ -115,6 +115,14 @@ struct SellPattern {
   std::vector<long long> tpos;       // stored slot of (c,r) for each stored (r,c); -1 pad
   DBuf<long long> d_off, d_tpos;
   DBuf<int> d_rowlen, d_col, d_diagk;
+  // 16-bit column offsets for the FP32 kernels: col = cbase[slice] + col16,
+  // 0xFFFF = the row itself (diagonal and padding); used when every slice's
+  // off-diagonal columns span fewer than 65535 (has16)
+  std::vector<unsigned short> col16;
+  std::vector<int> cbase;
+  DBuf<unsigned short> d_col16;
+  DBuf<int> d_cbase;
+  bool has16 = false;
   // level schedules (natural-order triangular sweeps / factorizations)
   std::vector<int> lev_ptr_f, lev_ptr_b;   // rows grouped per level
   DBuf<int> d_lev_rows_f, d_lev_rows_b;

@@ -15,7 +15,18 @@ constexpr double kRefineTol = 1.0e-5;               // FP32 inner BiCGSTAB reduc
 struct Mat {
   const long long *off;
   const int *rowlen, *col, *diagk;
+  const unsigned short *col16;   // 16-bit offsets for the FP32 kernels (nullptr: use col)
+  const int *cbase;
 };
+
+// column of stored slot q of row r (slice s) in the FP32 kernels
+__device__ __forceinline__ int col_f32(const Mat &A, long long q, int r, int s) {
+  if (A.col16) {
+    const unsigned c = A.col16[q];
+    return c == 0xFFFFu ? r : A.cbase[s] + (int)c;
+  }
+  return A.col[q];
+}
 
 // Row set of a cell-wise kernel: plain blocks of kBlock rows, or (coarse
 // space active) one chunk of rows of a single aggregate per workgroup.
@@ -80,7 +91,7 @@ __device__ __forceinline__ T row_apply(int r, const Mat &A, const T *val, const 
 #pragma unroll 4
     for (int k = 0; k < w; k++) {
       const long long q = base + (long long)k * kSlice;
-      acc = acc + val[q] * (x[A.col[q]] - xi);
+      acc = acc + val[q] * (x[col_f32(A, q, r, s)] - xi);
     }
     return val[base + (long long)A.diagk[r] * kSlice] * xi + acc;
   }
@@ -504,36 +515,36 @@ __global__ void k_coarse_E(int nE, const int *e_ptr, const int *e_key, const int
 // padded 256 x 256 matrix in registers: thread t owns column t % 256 of rows
 // t/256 + 4m, m = 0..63.  Pivot step j with pivot p = a_jj:
 //   a_jj <- 1/p,  a_jc <- a_jc/p,  a_rj <- -a_rj/p,  a_rc <- a_rc - a_rj a_jc/p.
+// While updating, the owners of row j+1 and column j+1 publish them to a
+// second LDS buffer (static register indices only), so each step needs one
+// barrier and no dynamically indexed register array.
 constexpr int kInvThreads = 1024;
 __global__ __launch_bounds__(kInvThreads) void k_coarse_inv(int k, const double *E, float *Einv) {
   const int t = threadIdx.x, c = t & 255, rb = t >> 8;
   float a[64];
+  __shared__ float prow[2][256], pcol[2][256];
 #pragma unroll
   for (int m = 0; m < 64; m++) {
     const int r = rb + 4 * m;
     a[m] = (r < k && c < k) ? (float)E[(size_t)r * k + c] : (r == c ? 1.0f : 0.0f);
+    if (r == 0) prow[0][c] = a[m];
+    if (c == 0) pcol[0][r] = a[m];
   }
-  __shared__ float prow[256], pcol[256];
+  __syncthreads();
   for (int j = 0; j < k; j++) {
-    const int mj = j >> 2, rj = j & 3;
-    if (rb == rj) {
-#pragma unroll
-      for (int m = 0; m < 64; m++)
-        if (m == mj) prow[c] = a[m];
-    }
-    if (c == j) {
-#pragma unroll
-      for (int m = 0; m < 64; m++) pcol[rb + 4 * m] = a[m];
-    }
-    __syncthreads();
-    const float inv = 1.0f / prow[j];
-    const float rowc = prow[c] * inv;
+    const int cur = j & 1, nxt = cur ^ 1, jn = j + 1;
+    const float inv = 1.0f / prow[cur][j];
+    const float rowc = prow[cur][c] * inv;
 #pragma unroll
     for (int m = 0; m < 64; m++) {
       const int r = rb + 4 * m;
-      if (r == j) a[m] = (c == j) ? inv : rowc;
-      else if (c == j) a[m] = -pcol[r] * inv;
-      else a[m] = a[m] - pcol[r] * rowc;
+      float v;
+      if (r == j) v = (c == j) ? inv : rowc;
+      else if (c == j) v = -pcol[cur][r] * inv;
+      else v = a[m] - pcol[cur][r] * rowc;
+      a[m] = v;
+      if (r == jn) prow[nxt][c] = v;
+      if (c == jn) pcol[nxt][r] = v;
     }
     __syncthreads();
   }
@@ -546,6 +557,9 @@ __global__ __launch_bounds__(kInvThreads) void k_coarse_inv(int k, const double 
 
 }  // namespace
 
-static inline Mat matof(const SellPattern *p) { return Mat{p->d_off.p, p->d_rowlen.p, p->d_col.p, p->d_diagk.p}; }
+static inline Mat matof(const SellPattern *p) {
+  return Mat{p->d_off.p, p->d_rowlen.p, p->d_col.p, p->d_diagk.p, p->has16 ? p->d_col16.p : nullptr,
+             p->has16 ? p->d_cbase.p : nullptr};
+}
 
 }  // namespace ms

@@ -417,7 +417,8 @@ __global__ void k_dilu_fwd(int r0, int r1, int nbr, Mat A, const T *val, const T
     T s = r[i];
     for (int k = 0; k < dpi; k++) {
       const long long q = bi + (long long)k * kSlice;
-      s = s - val[q] * z[A.col[q]];
+      const int c = sizeof(T) == 4 ? col_f32(A, q, i, i >> 6) : A.col[q];
+      s = s - val[q] * z[c];
     }
     const T zi = s / dg[i];
     z[i] = zi;
@@ -438,7 +439,8 @@ __global__ void k_dilu_bwd(int r0, int r1, Mat A, const T *val, const T *dg, con
     T s = T(0);
     for (int k = dpi + 1; k < len; k++) {
       const long long q = bi + (long long)k * kSlice;
-      s = s + val[q] * z[A.col[q]];
+      const int c = sizeof(T) == 4 ? col_f32(A, q, i, i >> 6) : A.col[q];
+      s = s + val[q] * z[c];
     }
     const T zi = z[i] - s / dg[i];
     z[i] = zi;
@@ -557,6 +559,7 @@ double Krylov::time_spmv(const double *val, bool fp32, int reps) {
   KS h{};
   MS_HIP(hipMemcpyAsync(st_.p, &h, sizeof(h), hipMemcpyHostToDevice, S));
   if (fp32) {
+    last_ = SetupKey{};
     ensure32();
     if (val32_.n < (size_t)pat->nstore) val32_.alloc(pat->nstore);
     hipLaunchKernelGGL(k_mat32_diff, G, B, 0, S, n, A, val, val32_.p);
@@ -758,7 +761,7 @@ void Krylov::coarse_setup(const double *val) {
 }
 
 SolveResult Krylov::pcg(const double *val, const double *b, double *x, double tol, int maxit, PcKind pc,
-                        bool coarse) {
+                        bool coarse, bool reuse) {
   SolveResult R;
   Mat A = matof(pat);
   const dim3 G(nblk_), B(kBlock);
@@ -781,12 +784,17 @@ SolveResult Krylov::pcg(const double *val, const double *b, double *x, double to
     MS_HIP(hipMemsetAsync(x, 0, n * sizeof(double), S));
     return R;
   }
-  int fe = setup_pc64(pc, val);
-  if (fe) {
-    R.ierr = fe + 10;
-    return R;
+  const SetupKey key{val, (int)pc, 0, (int)coarse};
+  if (!(reuse && last_ == key)) {
+    last_ = SetupKey{};
+    int fe = setup_pc64(pc, val);
+    if (fe) {
+      R.ierr = fe + 10;
+      return R;
+    }
+    if (coarse) coarse_setup(val);
+    last_ = key;
   }
-  if (coarse) coarse_setup(val);
   hipLaunchKernelGGL(k_residual<double>, G, B, 0, S, n, A, val, b, x, r_.p, part);
   finish<OpSum, 1>(nblk_, part, FinResidual{st});
   MS_CHECK_LAUNCH();
@@ -877,6 +885,7 @@ SolveResult Krylov::bicgstab(const double *val, const double *b, double *x, doub
     MS_HIP(hipMemsetAsync(x, 0, n * sizeof(double), S));
     return R;
   }
+  last_ = SetupKey{};
   int fe = setup_pc64(pc, val);
   if (fe) {
     R.ierr = fe + 10;
@@ -972,7 +981,7 @@ SolveResult Krylov::bicg32(PcKind pc, int maxit) {
 
 // ---- mixed precision ------------------------------------------------------------
 SolveResult Krylov::mixed(const double *val, const double *b, double *x, double tol, int maxit, PcKind pc,
-                          int solver, double inner_tol, bool coarse) {
+                          int solver, double inner_tol, bool coarse, bool reuse) {
   SolveResult R;
   Mat A = matof(pat);
   const dim3 G(nblk_), B(kBlock);
@@ -994,8 +1003,14 @@ SolveResult Krylov::mixed(const double *val, const double *b, double *x, double 
     MS_HIP(hipMemsetAsync(x, 0, n * sizeof(double), S));
     return R;
   }
-  hipLaunchKernelGGL(k_mat32_diff, G, B, 0, S, n, A, val, val32_.p);
-  setup_pc32(pc, val);
+  coarse = coarse && pat->coarse.k > 0 && solver == 0;
+  const SetupKey key{val, (int)pc, 1, (int)coarse};
+  const bool reused = reuse && last_ == key;
+  if (!reused) {
+    last_ = SetupKey{};
+    hipLaunchKernelGGL(k_mat32_diff, G, B, 0, S, n, A, val, val32_.p);
+    setup_pc32(pc, val);
+  }
   hipLaunchKernelGGL(k_residual<double>, G, B, 0, S, n, A, val, b, x, r_.p, part);
   finish<OpSum, 1>(nblk_, part, FinResidual{st});
   MS_CHECK_LAUNCH();
@@ -1041,14 +1056,14 @@ SolveResult Krylov::mixed(const double *val, const double *b, double *x, double 
   const int jac = pc == PC_JACOBI;
   const int mode = jac ? 1 : (pc == PC_DILU ? 2 : 0);
   const int c0end = pat->color_ptr[1];
-  coarse = coarse && pat->coarse.k > 0;
   // DILU with a coarse space: the coarse solve rides along the forward sweep
   const bool fused = coarse && pc == PC_DILU && pat->color_ptr.size() > 2;
   if (coarse) {
     h.coarse = fused ? 0 : 1;
     MS_HIP(hipMemcpyAsync(&st->coarse, &h.coarse, sizeof(int), hipMemcpyHostToDevice, S));
-    coarse_setup(val);
+    if (!reused) coarse_setup(val);
   }
+  last_ = key;
   const Rows RW = rows_of(pat, coarse);
   const dim3 GR(coarse ? pat->coarse.nchunks : nblk_);
   const CoDev co = codev_of(pat, coarse && !fused, st);

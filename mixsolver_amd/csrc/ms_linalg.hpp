@@ -36,15 +36,18 @@ class Krylov {
   // x is the initial guess on entry.  Returns reference it/res/ierr.
   // coarse: add the aggregate coarse correction of pat->coarse to the
   // preconditioner (PCG only).
+  // reuse: the matrix values are those of the previous solve with the same
+  // preconditioner (e.g. the second PISO corrector): its set-up (FP32 copy,
+  // preconditioner, coarse inverse) is kept.
   SolveResult pcg(const double *val, const double *b, double *x, double tol, int maxit, PcKind pc,
-                  bool coarse = false);
+                  bool coarse = false, bool reuse = false);
   SolveResult bicgstab(const double *val, const double *b, double *x, double tol, int maxit, PcKind pc);
   // Mixed precision.  PCG: FP32 matrix, preconditioner and Krylov vectors
   // with FP32 accumulation, FP64 solution and reliable updates of the true
   // FP64 residual.  BiCGSTAB: FP32 inner solves inside FP64 iterative
   // refinement.  res is the FP64 true relative residual.
   SolveResult mixed(const double *val, const double *b, double *x, double tol, int maxit, PcKind pc,
-                    int solver, double inner_tol, bool coarse = false);
+                    int solver, double inner_tol, bool coarse = false, bool reuse = false);
   void spmv(const double *val, const double *x, double *y);
   // average duration (ms) of the PCG SpMV+dot kernel on matrix val, measured
   // with HIP events on the library stream over `reps` launches; fp32 selects
@@ -72,6 +75,13 @@ class Krylov {
   void *hpin_ = nullptr;
   int nblk_;
   bool have32_ = false;
+  struct SetupKey {
+    const double *val = nullptr;
+    int pc = -1, fp32 = -1, coarse = -1;
+    bool operator==(const SetupKey &o) const {
+      return val == o.val && pc == o.pc && fp32 == o.fp32 && coarse == o.coarse;
+    }
+  } last_;   // the set-up the preconditioner buffers currently hold
 };
 
 }  // namespace ms

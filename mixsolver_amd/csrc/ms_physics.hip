@@ -707,14 +707,16 @@ struct Driver {
     hipLaunchKernelGGL(k_correct_flux, gc(s.nf), dim3(kBlock), 0, stream(), s.dmesh(), pp, s.af.p, s.flux.p);
     MS_CHECK_LAUNCH();
   }
-  SolveResult solve(bool ppe_eq, double *x) {
+  // reuse: the matrix equals the one of the previous solve (a later p'
+  // corrector with unchanged apc), so the solver keeps its set-up
+  SolveResult solve(bool ppe_eq, double *x, bool reuse = false) {
     const ms_ctrl_t &c = s.ctrl;
     PcKind pc = pc_of(ppe_eq ? c.gpu_ppe_precond : c.gpu_mom_precond, ppe_eq);
     int solver = ppe_eq ? c.gpu_ppe_solver : 1;
     const bool coarse = ppe_eq && s.pat.coarse.k > 0;   // two-level preconditioner for p'
     if (c.gpu_precision == MS_MIXED)
-      return s.kry->mixed(s.val.p, s.rhs.p, x, c.lin_tol, c.lin_max, pc, solver, c.gpu_inner_tol, coarse);
-    if (solver == 0) return s.kry->pcg(s.val.p, s.rhs.p, x, c.lin_tol, c.lin_max, pc, coarse);
+      return s.kry->mixed(s.val.p, s.rhs.p, x, c.lin_tol, c.lin_max, pc, solver, c.gpu_inner_tol, coarse, reuse);
+    if (solver == 0) return s.kry->pcg(s.val.p, s.rhs.p, x, c.lin_tol, c.lin_max, pc, coarse, reuse);
     return s.kry->bicgstab(s.val.p, s.rhs.p, x, c.lin_tol, c.lin_max, pc);
   }
   // momentum predictor over the 3 components; returns failing comp (1..3) or 0
@@ -849,7 +851,7 @@ int run_piso(System &s, int nsteps) {
     for (int ic = 1; ic <= c.n_correct; ic++) {
       D.ppe();
       fill(s.pp.p, s.nc, 0.0);
-      SolveResult R = D.solve(true, s.pp.p);
+      SolveResult R = D.solve(true, s.pp.p, ic > 1);
       if (R.ierr == 2) { bad = true; break; }
       itl_max = std::max(itl_max, R.it);
       ppe_it += R.it;
@@ -919,7 +921,7 @@ int run_pimple(System &s, int nsteps) {
       for (int ic = 1; ic <= c.n_correct; ic++) {
         D.ppe();
         fill(s.pp.p, s.nc, 0.0);
-        SolveResult R = D.solve(true, s.pp.p);
+        SolveResult R = D.solve(true, s.pp.p, ic > 1);
         if (R.ierr == 2) { bad = true; break; }
         itl_max = std::max(itl_max, R.it);
         ppe_it += R.it;

@@ -113,17 +113,19 @@ __global__ void k_s_unperm(int n, const int *o2n, const double *src, double *dst
 
 // ------------------------------------------------------ batched iteration
 __global__ void k_b_sumsq(const Item *L, const BSys *S, double *part) {
-  const Item it = L[blockIdx.x];
+  const int ii = xcd_block(blockIdx.x, gridDim.x);   // item of this workgroup
+  const Item it = L[ii];
   const BSys &s = S[it.sys];
   const int i = it.row0 + (int)threadIdx.x;
   double v[1] = {0.0};
   if (i < it.row1) v[0] = s.b[i] * s.b[i];
-  block_partials<OpSum, 1>(v, part);
+  block_partials<OpSum, 1>(v, part, ii);
 }
 
 // x = y = p = 0, r = b / sigma; partials (r.r, Jacobi r.(r/d), sum r); DILU: z = r/d on colour 0
 __global__ void k_b_init(const Item *L, const BSys *S, int mode, float *part) {
-  const Item it = L[blockIdx.x];
+  const int ii = xcd_block(blockIdx.x, gridDim.x);   // item of this workgroup
+  const Item it = L[ii];
   const BSys &s = S[it.sys];
   if (s.st->done) return;
   const int i = it.row0 + (int)threadIdx.x;
@@ -138,11 +140,12 @@ __global__ void k_b_init(const Item *L, const BSys *S, int mode, float *part) {
     if (mode == 1) v[1] = ri * (ri / s.dg[i]);
     else if (mode == 2 && i < s.c0end) s.z[i] = ri / s.dg[i];
   }
-  block_partials<OpSum, 3>(v, part);
+  block_partials<OpSum, 3>(v, part, ii);
 }
 
 __global__ void k_b_spmv(const Item *L, const BSys *S, float *part) {
-  const Item it = L[blockIdx.x];
+  const int ii = xcd_block(blockIdx.x, gridDim.x);   // item of this workgroup
+  const Item it = L[ii];
   const BSys &s = S[it.sys];
   if (s.st->done) return;
   const int i = it.row0 + (int)threadIdx.x;
@@ -152,11 +155,12 @@ __global__ void k_b_spmv(const Item *L, const BSys *S, float *part) {
     s.q[i] = qi;
     v[0] = s.p[i] * qi;
   }
-  block_partials<OpSum, 1>(v, part);
+  block_partials<OpSum, 1>(v, part, ii);
 }
 
 __global__ void k_b_update(const Item *L, const BSys *S, int mode, float *part) {
-  const Item it = L[blockIdx.x];
+  const int ii = xcd_block(blockIdx.x, gridDim.x);   // item of this workgroup
+  const Item it = L[ii];
   const BSys &s = S[it.sys];
   if (s.st->done) return;
   const int i = it.row0 + (int)threadIdx.x;
@@ -171,12 +175,13 @@ __global__ void k_b_update(const Item *L, const BSys *S, int mode, float *part) 
     if (mode == 1) v[1] = ri * (ri / s.dg[i]);
     else if (mode == 2 && i < s.c0end) s.z[i] = ri / s.dg[i];
   }
-  block_partials<OpSum, 3>(v, part);
+  block_partials<OpSum, 3>(v, part, ii);
 }
 
 // p = z (Jacobi: r/d) + nu of the chunk's aggregate + beta p
 __global__ void k_b_p(const Item *L, const BSys *S, int jac) {
-  const Item it = L[blockIdx.x];
+  const int ii = xcd_block(blockIdx.x, gridDim.x);   // item of this workgroup
+  const Item it = L[ii];
   const BSys &s = S[it.sys];
   if (s.st->done) return;
   const int i = it.row0 + (int)threadIdx.x;
@@ -190,7 +195,8 @@ __global__ void k_b_p(const Item *L, const BSys *S, int jac) {
 // DILU sweeps over the chunks of one colour; r.z partial at prz[slot] for the
 // chunks whose z is final after this sweep
 __global__ void k_b_fwd(const Item *L, const BSys *S, float *prz) {
-  const Item it = L[blockIdx.x];
+  const int ii = xcd_block(blockIdx.x, gridDim.x);   // item of this workgroup
+  const Item it = L[ii];
   const BSys &s = S[it.sys];
   if (s.st->done) return;
   const int i = it.row0 + (int)threadIdx.x;
@@ -201,7 +207,7 @@ __global__ void k_b_fwd(const Item *L, const BSys *S, float *prz) {
     float acc = s.r[i];
     for (int k = 0; k < dpi; k++) {
       const long long q = bi + (long long)k * kSlice;
-      acc = acc - s.v32[q] * s.z[s.A.col[q]];
+      acc = acc - s.v32[q] * s.z[col_f32(s.A, q, i, i >> 6)];
     }
     const float zi = acc / s.dg[i];
     s.z[i] = zi;
@@ -213,7 +219,8 @@ __global__ void k_b_fwd(const Item *L, const BSys *S, float *prz) {
 }
 
 __global__ void k_b_bwd(const Item *L, const BSys *S, float *prz) {
-  const Item it = L[blockIdx.x];
+  const int ii = xcd_block(blockIdx.x, gridDim.x);   // item of this workgroup
+  const Item it = L[ii];
   const BSys &s = S[it.sys];
   if (s.st->done) return;
   const int i = it.row0 + (int)threadIdx.x;
@@ -224,7 +231,7 @@ __global__ void k_b_bwd(const Item *L, const BSys *S, float *prz) {
     float acc = 0.0f;
     for (int k = dpi + 1; k < len; k++) {
       const long long q = bi + (long long)k * kSlice;
-      acc = acc + s.v32[q] * s.z[s.A.col[q]];
+      acc = acc + s.v32[q] * s.z[col_f32(s.A, q, i, i >> 6)];
     }
     const float zi = s.z[i] - acc / s.dg[i];
     s.z[i] = zi;
@@ -238,7 +245,8 @@ __global__ void k_b_bwd(const Item *L, const BSys *S, float *prz) {
 // reliable update: x += sigma y, y = 0; r64 = b - A64 x with its r.r partials;
 // then r = r64 / sigma_new, p *= scale with the partials of k_b_update
 __global__ void k_b_fold(const Item *L, const BSys *S) {
-  const Item it = L[blockIdx.x];
+  const int ii = xcd_block(blockIdx.x, gridDim.x);   // item of this workgroup
+  const Item it = L[ii];
   const BSys &s = S[it.sys];
   if (s.st->done == 1 || s.st->ru != 1) return;
   const int i = it.row0 + (int)threadIdx.x;
@@ -248,7 +256,8 @@ __global__ void k_b_fold(const Item *L, const BSys *S) {
   }
 }
 __global__ void k_b_true_res(const Item *L, const BSys *S, double *part) {
-  const Item it = L[blockIdx.x];
+  const int ii = xcd_block(blockIdx.x, gridDim.x);   // item of this workgroup
+  const Item it = L[ii];
   const BSys &s = S[it.sys];
   if (s.st->done == 1 || s.st->ru != 1) return;
   const int i = it.row0 + (int)threadIdx.x;
@@ -258,10 +267,11 @@ __global__ void k_b_true_res(const Item *L, const BSys *S, double *part) {
     s.r64[i] = ri;
     v[0] = ri * ri;
   }
-  block_partials<OpSum, 1>(v, part);
+  block_partials<OpSum, 1>(v, part, ii);
 }
 __global__ void k_b_rescale(const Item *L, const BSys *S, int mode, float *part) {
-  const Item it = L[blockIdx.x];
+  const int ii = xcd_block(blockIdx.x, gridDim.x);   // item of this workgroup
+  const Item it = L[ii];
   const BSys &s = S[it.sys];
   if (s.st->done || s.st->ru != 2) return;
   const int i = it.row0 + (int)threadIdx.x;
@@ -275,7 +285,7 @@ __global__ void k_b_rescale(const Item *L, const BSys *S, int mode, float *part)
     if (mode == 1) v[1] = ri * (ri / s.dg[i]);
     else if (mode == 2 && i < s.c0end) s.z[i] = ri / s.dg[i];
   }
-  block_partials<OpSum, 3>(v, part);
+  block_partials<OpSum, 3>(v, part, ii);
 }
 
 // one finisher workgroup per request: its partials [start, end) in a fixed
@@ -477,7 +487,7 @@ class Server {
     if (o_.precond == MS_PC_DILU) color = greedy_colors(n, adjp, adj, visit, P->ncol);
     std::vector<int> agg(n, 0);
     if (o_.coarse > 0 && xc) {
-      P->k = std::max(1, std::min(o_.coarse, n / 64));
+      P->k = std::max(1, std::min(o_.coarse, n / 4096));   // about 4096 cells per aggregate, at most `coarse`
       std::vector<double> v(xc, xc + 3 * (size_t)n);
       agg = rcb_aggregates(n, v, P->k);
     }

@@ -15,6 +15,7 @@
 // the SELL-64 matrix pattern, per-cell face lists with matrix slot maps,
 // level schedules for natural-order factorizations and per-face constants.
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -128,6 +129,32 @@ void csr_to_sell(SellPattern &P, int n, const std::vector<int> &rp, const std::v
       P.tpos[P.slot(r, k)] = P.slot(c, (int)(f - b));
     }
   P.color_ptr = {0, n};
+  // 16-bit offsets per slice
+  P.cbase.assign(P.nslices, 0);
+  P.has16 = true;
+  for (int s = 0; s < P.nslices && P.has16; s++) {
+    int lo = INT32_MAX, hi = -1;
+    for (int r = s * kSlice; r < std::min(n, (s + 1) * kSlice); r++)
+      for (int k = 0; k < P.rowlen[r]; k++) {
+        const int c = P.col[P.slot(r, k)];
+        if (c == r) continue;
+        lo = std::min(lo, c);
+        hi = std::max(hi, c);
+      }
+    if (hi < 0) lo = hi = 0;
+    if (hi - lo >= 0xFFFF) P.has16 = false;
+    P.cbase[s] = lo;
+  }
+  P.col16.clear();
+  if (P.has16) {
+    P.col16.assign(P.nstore, 0xFFFF);
+    for (int r = 0; r < n; r++)
+      for (int k = 0; k < P.rowlen[r]; k++) {
+        const long long q = P.slot(r, k);
+        const int c = P.col[q];
+        if (c != r) P.col16[q] = (unsigned short)(c - P.cbase[r / kSlice]);
+      }
+  }
 }
 
 void SellPattern::build_levels() {
@@ -168,11 +195,15 @@ void SellPattern::upload() {
   d_col.upload(col);
   d_diagk.upload(diagk);
   d_tpos.upload(tpos);
+  if (has16) {
+    d_col16.upload(col16);
+    d_cbase.upload(cbase);
+  }
 }
 
 size_t SellPattern::device_bytes() const {
   return d_off.bytes() + d_rowlen.bytes() + d_col.bytes() + d_diagk.bytes() + d_tpos.bytes() +
-         d_lev_rows_f.bytes() + d_lev_rows_b.bytes() + coarse.device_bytes();
+         d_lev_rows_f.bytes() + d_lev_rows_b.bytes() + coarse.device_bytes() + d_col16.bytes() + d_cbase.bytes();
 }
 
 // ------------------------------------------------------------ orderings
