@@ -257,6 +257,70 @@ struct FinRU {         // after x += sigma*y and r = b - A x in FP64
     st->done = 0;                          // resume the iteration
   }
 };
+// DILU iterations finish the update's r.r (first of the three chunk partials)
+// and the sweeps' r.z in one kernel after the sweeps: the convergence and
+// reliable-update checks of r.r move behind the preconditioner, which leaves
+// the iterates unchanged (one unneeded sweep on the last iteration).
+struct FinUpdRz {        // FP64 PCG: FinUpdate then FinRz
+  KS *st;
+  __device__ bool skip() const { return st->done != 0; }
+  template <class T> __device__ void operator()(const T *a1, const T *a2) const {
+    st->rr = a1[0];
+    st->res = sqrt((double)a1[0]) / st->bnrm;
+    if (st->res <= st->tol) {
+      st->done = 1;
+      return;
+    }
+    if (fabs(st->rz) < kTiny) {
+      st->ierr = 2;
+      st->done = 1;
+    } else {
+      const double rzn = rz_total(st, (double)a2[0]);
+      st->beta = rzn / st->rz;
+      st->rz = rzn;
+    }
+  }
+};
+struct FinUpdRzRU {      // mixed PCG: FinUpdateRU then FinRz
+  KS *st;
+  int trigger;
+  __device__ bool skip() const { return st->done != 0; }
+  template <class T> __device__ void operator()(const T *a1, const T *a2) const {
+    const double rn = sqrt((double)a1[0]);
+    st->res = rn * st->sigma / st->bnrm;
+    if (trigger) {
+      if (st->res <= st->tol) {
+        st->ru = 1;
+        st->done = 2;
+        return;
+      }
+      if (rn < st->delta * st->rmax) st->ru = 1;
+    }
+    if (rn > st->rmax) st->rmax = rn;
+    if (st->ru == 2) st->ru = 0;
+    if (fabs(st->rz) < kTiny) {
+      st->ierr = 2;
+      st->done = 1;
+    } else {
+      const double rzn = rz_total(st, (double)a2[0]);
+      st->beta = rzn / st->rz;
+      st->rz = rzn;
+    }
+  }
+};
+
+template <class T, class Fin>
+__global__ __launch_bounds__(kFinThreads) void k_finish2(int n1, const T *p1, int n2, const T *p2, Fin fin) {
+  if (fin.skip()) return;
+  T a1[3], a2[1];
+  fin_gather<OpSum, 3, T>(n1, p1, a1);
+  fin_gather<OpSum, 1, T>(n2, p2, a2);
+  __shared__ T l1[3 * (kFinThreads / 64)], l2[kFinThreads / 64];
+  block_reduce<OpSum, 3, T, kFinThreads>(a1, l1);
+  block_reduce<OpSum, 1, T, kFinThreads>(a2, l2);
+  if (threadIdx.x == 0) fin(a1, a2);
+}
+
 struct FinBiRho {
   KS *st;
   __device__ bool skip() const { return st->done != 0; }

@@ -830,6 +830,7 @@ SolveResult Krylov::pcg(const double *val, const double *b, double *x, double to
   const int mode = jac ? 1 : (pc == PC_DILU ? 2 : 0);
   const int c0end = pat->color_ptr[1];
   const CoDev co_upd = fused ? CoDev{0, nullptr, nullptr, nullptr, nullptr, st} : co;
+  const bool merged = mode == 2 && (!coarse || fused);
   int enq = 0, chunk = 8;
   while (true) {
     int m = std::min(chunk, maxit - enq);
@@ -838,8 +839,14 @@ SolveResult Krylov::pcg(const double *val, const double *b, double *x, double to
       finish<OpSum, 1>(nblk_, part, FinPap{st});
       hipLaunchKernelGGL((k_pcg_update<double, double>), GR, B, 0, S, RW, p_.p, q_.p, x, r_.p, z_.p, dg_.p, mode,
                          c0end, st, part);
-      finish_co<3>((int)GR.x, part, FinUpdate{st, jac}, co_upd);
-      if (mode == 2) {
+      if (merged) {      // convergence check and r.z in one finisher after the sweeps
+        const int np = dilu_sweeps<double>(pat, A, val, dg_.p, r_.p, z_.p, done, prz, cw);
+        hipLaunchKernelGGL((k_finish2<double, FinUpdRz>), dim3(1), dim3(kFinThreads), 0, S, (int)GR.x, part, np, prz,
+                           FinUpdRz{st});
+      } else {
+        finish_co<3>((int)GR.x, part, FinUpdate{st, jac}, co_upd);
+      }
+      if (mode == 2 && !merged) {
         int np = dilu_sweeps<double>(pat, A, val, dg_.p, r_.p, z_.p, done, prz, cw);
         finish<OpSum, 1>(np, prz, FinRz{st});
       } else if (mode == 0) {
@@ -1090,16 +1097,25 @@ SolveResult Krylov::mixed(const double *val, const double *b, double *x, double 
     hipLaunchKernelGGL(k_pcg_start<float>, GR, B, 0, S, RW, r, z, dg, jac, p, nu, st, part32);
     finish<OpSum, 1>((int)GR.x, part32, FinRz0{st});
   }
-  // preconditioner, r.z and p update: the part of an iteration after the
-  // update finisher
-  auto tail = [&]() {
-    if (mode == 2) {
-      int np = dilu_sweeps<float>(pat, A, v32, dg, r, z, done, prz, cw);
-      finish<OpSum, 1>(np, prz, FinRz{st});
-    } else if (mode == 0) {
-      apply_pc<float>(pc, v32, fac, dg, r, z, done);
-      hipLaunchKernelGGL(k_dot<float>, G, B, 0, S, n, r, z, st, part32);
-      finish<OpSum, 1>(nblk_, part32, FinRz{st});
+  // the part of an iteration after the update kernel: residual check,
+  // preconditioner, r.z and p update.  DILU finishes r.r and r.z in one
+  // kernel after the sweeps (trigger: reliable-update checks active).
+  const bool merged = mode == 2 && (!coarse || fused);
+  auto tail = [&](int trigger) {
+    if (merged) {
+      const int np = dilu_sweeps<float>(pat, A, v32, dg, r, z, done, prz, cw);
+      hipLaunchKernelGGL((k_finish2<float, FinUpdRzRU>), dim3(1), dim3(kFinThreads), 0, S, (int)GR.x, part32, np,
+                         prz, FinUpdRzRU{st, trigger});
+    } else {
+      finish_co<3>((int)GR.x, part32, FinUpdateRU{st, jac, trigger}, co);
+      if (mode == 2) {
+        int np = dilu_sweeps<float>(pat, A, v32, dg, r, z, done, prz, cw);
+        finish<OpSum, 1>(np, prz, FinRz{st});
+      } else if (mode == 0) {
+        apply_pc<float>(pc, v32, fac, dg, r, z, done);
+        hipLaunchKernelGGL(k_dot<float>, G, B, 0, S, n, r, z, st, part32);
+        finish<OpSum, 1>(nblk_, part32, FinRz{st});
+      }
     }
     hipLaunchKernelGGL(k_pcg_p<float>, GR, B, 0, S, RW, r, z, dg, jac, p, nu, st);
   };
@@ -1108,10 +1124,7 @@ SolveResult Krylov::mixed(const double *val, const double *b, double *x, double 
     hipLaunchKernelGGL(k_true_res, G, B, 0, S, n, A, val, b, x, r_.p, st, part);
     finish<OpSum, 1>(nblk_, part, FinRU{st});
     hipLaunchKernelGGL(k_ru_rescale, GR, B, 0, S, RW, r_.p, r, p, z, dg, mode, c0end, st, part32);
-    if (paused) {        // complete the interrupted iteration on the new r
-      finish_co<3>((int)GR.x, part32, FinUpdateRU{st, jac, 0}, co);
-      tail();
-    }
+    if (paused) tail(0);   // complete the interrupted iteration on the new r
     MS_CHECK_LAUNCH();
   };
   constexpr int kChunk = 16;   // reliable updates are applied at most this many iterations late
@@ -1123,8 +1136,7 @@ SolveResult Krylov::mixed(const double *val, const double *b, double *x, double 
       finish<OpSum, 1>(nblk_, part32, FinPap{st});
       hipLaunchKernelGGL((k_pcg_update<float, float>), GR, B, 0, S, RW, p, q, y, r, z, dg, mode, c0end, st,
                          part32);
-      finish_co<3>((int)GR.x, part32, FinUpdateRU{st, jac, 1}, co);
-      tail();
+      tail(1);
     }
     MS_CHECK_LAUNCH();
     hs = read_state<double>(st);

@@ -292,6 +292,36 @@ def save(name, **kw):
     print("wrote", path, os.path.getsize(path), "bytes")
 
 
+REF_CAVITY = "/root/reference/cluster_cases/cavity_Re1000"
+
+
+def cavity_known_answer():
+    """The reference's shipped Re=1000 lid-driven cavity (80x80x1 hex mesh,
+    unMesh.cas + unMesh.control of cluster_cases/cavity_Re1000): its README
+    (lines 179-180) records max|u| = 9.11512E-01 and the Ghia centreline
+    relative L2 error 2.89706E-02 for this control (outer_max = 1000, no
+    early exit).  The harness re-runs the reference's simple_run for 1000
+    iterations and the fixture keeps both the published and the full-precision
+    numbers."""
+    from mixsolver_amd.post import ghia_compare
+    from mixsolver_amd.control import parse_control
+    from oracle import pyoracle
+    m = cas.read_cas(os.path.join(REF_CAVITY, "unMesh.cas"))
+    ctl = "\n".join(l for l in open(os.path.join(REF_CAVITY, "unMesh.control")).read().splitlines()
+                    if not l.strip().startswith("out_format")) + "\nn_out_every = 1000000\n"
+    with tempfile.TemporaryDirectory() as td:
+        cas.write_cas(os.path.join(td, "m.cas"), m, {4: "symmetry", 5: "wall"})
+        open(os.path.join(td, "c.control"), "w").write(ctl)
+        run(["m.cas", "c.control", "simple", "out.bin", "", "1000"], td)
+        r = Reader(os.path.join(td, "out.bin"))
+        ier = r.take(np.int32, 1)[0]
+        u = r.take(np.float64, 3 * m.ncells).reshape(m.ncells, 3)
+    ctrl, groups = parse_control(ctl, warn=False)
+    xc = pyoracle.OracleCase(m, ctrl, groups).geometry()["xc"].reshape(-1, 3)
+    l2, _ = ghia_compare(xc, u, 1000)
+    return m, ctl, int(ier), float(np.linalg.norm(u, axis=1).max()), float(l2)
+
+
 def main():
     if not os.path.exists(HARNESS):
         sys.exit("build the reference harness first: oracle/build_ref.sh")
@@ -327,6 +357,13 @@ def main():
     out, log = case_run("chanpim", mc, zkc, PIMPLE_CTL, "pimple", 3, (u0, p0, T0))
     save("channel_pimple_ic0", control=PIMPLE_CTL, nsteps=np.int64(3), **mesh_arrays(mc),
          in_u=u0, in_p=p0, in_T=T0, **{"out_" + k: v for k, v in out.items()})
+
+    # 3c. the reference's shipped Re=1000 cavity and its published answer
+    if os.path.isdir(REF_CAVITY):
+        mk, ctlk, ierk, umax, l2 = cavity_known_answer()
+        save("cavity_re1000_known_answer", control=ctlk, nsteps=np.int64(1000), **mesh_arrays(mk),
+             out_ier=np.int64(ierk), ref_umax=umax, ref_ghia_l2=l2,
+             readme_umax=0.911512, readme_ghia_l2=0.0289706)
 
     # 4. jittered prism O-grid, PISO with IC0, blended convection, far field
     mp = meshgen.cylinder_prisms(ntheta=24, nr=6, r_out=4.0, layers=2, lz=0.2, jitter=0.15, seed=200)
