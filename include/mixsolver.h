@@ -54,6 +54,17 @@ typedef struct {
   double gpu_inner_tol;  /* relative tolerance of each FP32 inner solve      */
   int gpu_reorder;       /* 0 = natural order, 1 = RCM                       */
   int gpu_coarse;        /* p' PCG coarse space: number of aggregates, 0 = off (max 256) */
+  /* Periodic face pairs (no reference counterpart; the reference's BC list
+   * has no periodic type).  Pair i joins every boundary face of zone
+   * periodic_zone[i][0] to the face of zone periodic_zone[i][1] whose
+   * centroid translated by periodic_shift[i] coincides with it; the two
+   * become one interior face.  periodic_auto[i] != 0 takes the shift from the
+   * difference of the two zones' mean face centroids.  Periodic zones have
+   * no bc group. */
+  int nperiodic;                    /* 0..4 */
+  int periodic_zone[4][2];
+  int periodic_auto[4];
+  double periodic_shift[4][3];
 } ms_ctrl_t;
 
 /* One boundary group (bcgroup_t, mod_uns_bc.f90:80-110), control order. */
@@ -99,6 +110,15 @@ int ms_coarse_info(ms_system *s, int *k, int *nchunks, int *agg);
  * splitting each group along the longest side of its bounding box.  agg[i]
  * in 0..k-1.  Host only (no device needed); no reference counterpart. */
 int ms_coarse_aggregates(int n, const double *xc, int k, int *agg);
+
+/* Periodic face matching of ctrl's periodic pairs on a mesh given as for
+ * ms_system_create: partner[f] (nfaces) = the face joined with boundary face
+ * f (0-based, both directions), -1 elsewhere.  Host only (no device
+ * needed); the same matching and errors as ms_system_create.  No reference
+ * counterpart (the reference has no periodic boundary). */
+int ms_periodic_match(int nnodes, const double *x, int nfaces, const long long *fptr, const int *fnodes,
+                      const int *c0, const int *c1, const int *zone, int ncells, const ms_ctrl_t *ctrl,
+                      int *partner);
 
 /* fields_t components by name: "u"(3,nc) "p" "T" "T_s" "flux"(nf) "apc"
  * "u_old" "u_old_old" "T_old" "T_old_old" "gp"(3,nc) "gu"(3,3,nc) "gt"

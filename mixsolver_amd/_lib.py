@@ -69,6 +69,7 @@ def lib():
         "ms_time_copy": (C.c_int, [C.c_longlong, C.c_int, _D]),
         "ms_coarse_info": (C.c_int, [_V, _I, _I, _I]),
         "ms_coarse_aggregates": (C.c_int, [C.c_int, _D, C.c_int, _I]),
+        "ms_periodic_match": (C.c_int, [C.c_int, _D, C.c_int, _L, _I, _I, _I, _I, C.c_int, _V, _I]),
         "ms_server_create": (C.c_int, [C.POINTER(_V), _V]),
         "ms_server_destroy": (C.c_int, [_V]),
         "ms_server_add_pattern": (C.c_int, [_V, C.c_int, _I, _I, _D, _I]),

@@ -17,6 +17,12 @@ stay compatible with the reference reader:
   gpu_coarse       p' PCG two-level preconditioner: number of cell aggregates
                    (recursive coordinate bisection), 0 = off, at most 256
                    (default 0)
+  periodic         zoneA zoneB [tx ty tz]: join the faces of zoneA to the faces
+                   of zoneB translated by (tx, ty, tz) (default: the difference
+                   of the two zones' mean face centroids); up to 4 lines; the
+                   two zones take no bc line.  The reference has no periodic
+                   boundary, so a control file using it runs on the device path
+                   only.
 
 ``ilu0``/``ic0`` select natural-order factorizations (parity with the
 reference); ``dilu`` selects the multicolour DILU preconditioner.
@@ -67,6 +73,8 @@ class MsCtrl(C.Structure):
         ("gpu_ppe_solver", C.c_int),
         ("gpu_inner_tol", C.c_double),
         ("gpu_reorder", C.c_int), ("gpu_coarse", C.c_int),
+        ("nperiodic", C.c_int), ("periodic_zone", (C.c_int * 2) * 4), ("periodic_auto", C.c_int * 4),
+        ("periodic_shift", (C.c_double * 3) * 4),
     ]
 
 
@@ -275,6 +283,18 @@ def parse_control(text: str, warn=True):
                 ctrl.gpu_coarse = int(val)
                 if not 0 <= ctrl.gpu_coarse <= 256:
                     raise ValueError(val)
+            elif key == "periodic":
+                k = ctrl.nperiodic
+                if k >= 4 or len(toks) not in (2, 5):
+                    raise ControlError(2, f"bad periodic value (zoneA zoneB [tx ty tz], max 4 lines): {val}")
+                za, zb = int(toks[0]), int(toks[1])
+                if za == zb:
+                    raise ControlError(2, f"periodic zones must differ: {val}")
+                ctrl.periodic_zone[k][0], ctrl.periodic_zone[k][1] = za, zb
+                ctrl.periodic_auto[k] = 1 if len(toks) == 2 else 0
+                if len(toks) == 5:
+                    ctrl.periodic_shift[k][:] = [float(t) for t in toks[2:5]]
+                ctrl.nperiodic = k + 1
             elif key in ("out_format", "out_fmt", "out_cells", "out_cell", "porous_model",
                          "restart", "restart_file", "partition_file", "dump_file",
                          "mesh_scale", "cell_zone"):
@@ -288,6 +308,9 @@ def parse_control(text: str, warn=True):
             raise ControlError(2, f"bad value for {key}: {val}") from None
     if not groups:
         raise ControlError(5, "control file defines no boundary conditions")
+    pz = {ctrl.periodic_zone[k][j] for k in range(ctrl.nperiodic) for j in range(2)}
+    if any(g.zone in pz for g in groups):
+        raise ControlError(2, "a periodic zone also has a bc line")
     ctrl.nbc = len(groups)
     ctrl.gpu_ppe_precond = ctrl.ppe_precond if ppe_gpu is None else ppe_gpu
     return ctrl, groups

@@ -157,6 +157,12 @@ int ms_coarse_info(ms_system *h, int *k, int *nchunks, int *agg) {
   });
 }
 
+int ms_periodic_match(int nnodes, const double *x, int nfaces, const long long *fptr, const int *fnodes,
+                      const int *c0, const int *c1, const int *zone, int ncells, const ms_ctrl_t *ctrl,
+                      int *partner) {
+  return guarded([&] { periodic_match(nnodes, x, nfaces, fptr, fnodes, c0, c1, zone, ncells, ctrl, partner); });
+}
+
 int ms_coarse_aggregates(int n, const double *xc, int k, int *agg) {
   return guarded([&] {
     if (n <= 0 || !xc || !agg || k < 1 || k > n) fail(MS_EINVAL, "ms_coarse_aggregates: need 1 <= k <= n");
@@ -216,6 +222,9 @@ int ms_get_field(ms_system *h, const char *name, double *host, long long n) {
     if (f.kind == FACE) {
       if (n != s.nf) fail(MS_EINVAL, "face field size mismatch");
       download(*f.buf, d, s.nf);
+      // a periodic shadow face carries its joined face's flux, oriented out of its own cell
+      for (int g = 0; g < s.nf; g++)
+        if (s.shadow[g] >= 0) d[g] = -d[s.shadow[g]];
       std::memcpy(host, d.data(), s.nf * sizeof(double));
     } else {
       if (n != (long long)s.nc * f.width) fail(MS_EINVAL, "cell field size mismatch");

@@ -136,6 +136,18 @@ __device__ __forceinline__ double bc_face_T(const DMesh &m, int f, double Tc) {
   return m.tmode[f] == 1 ? m.tval[f] : Tc;
 }
 
+// rr = xf[f] - xc[cell] for the cell on side `far` (0: c0, 1: c1) of face f;
+// the c1 cell of a joined periodic face sees the face at xf - shift.
+__device__ __forceinline__ void face_offset(const DMesh &m, int f, int cell, bool far, double *rr) {
+  const double *xf = &m.xf[3 * (size_t)f], *xc = &m.xc[3 * (size_t)cell];
+  const int k = (far && m.nper) ? m.pidx[f] : 0;
+  if (k == 0) {
+    for (int d = 0; d < 3; d++) rr[d] = xf[d] - xc[d];
+  } else {
+    for (int d = 0; d < 3; d++) rr[d] = (xf[d] - m.psh[k - 1][d]) - xc[d];
+  }
+}
+
 // ------------------------------------------------------------- gradients
 __global__ void k_gradients(DMesh m, DFields F, Ramp rp, int do_T) {
   const int r = blockIdx.x * kBlock + threadIdx.x;
@@ -218,9 +230,9 @@ __global__ void k_limiter(DMesh m, DFields F, Ramp rp, int comp, double *psi) {
   double ps = 1.0, g[3];
   for (int d = 0; d < 3; d++) g[d] = isT ? F.gt[3 * (size_t)r + d] : F.gu[9 * (size_t)r + comp + 3 * d];
   for (int k = k0; k < k1; k++) {
-    const int f = m.cf_ent[k] >> 1;
+    const int e = m.cf_ent[k];
     double rr[3];
-    for (int d = 0; d < 3; d++) rr[d] = m.xf[3 * (size_t)f + d] - m.xc[3 * (size_t)r + d];
+    face_offset(m, e >> 1, r, (e & 1) != 0, rr);
     const double fr = uc + ddot3(g, rr);
     const double df = fr - uc;
     if (df > 0.0) ps = dmin(ps, (pmax - uc) / df);
@@ -262,10 +274,8 @@ __global__ void k_momentum(DMesh m, DFields F, Ramp rp, MomParams q, const doubl
     if (m.blend_on) {
       const int cu = Ff > 0.0 ? a : b;
       double g[3], rr[3];
-      for (int d = 0; d < 3; d++) {
-        g[d] = F.gu[9 * (size_t)cu + comp + 3 * d];
-        rr[d] = m.xf[3 * (size_t)f + d] - m.xc[3 * (size_t)cu + d];
-      }
+      for (int d = 0; d < 3; d++) g[d] = F.gu[9 * (size_t)cu + comp + 3 * d];
+      face_offset(m, f, cu, !(Ff > 0.0), rr);
       double dc = psi[cu] * ddot3(g, rr);
       dc = q.blend * Ff * dc;
       rhs = side == 0 ? rhs - dc : rhs + dc;
@@ -373,7 +383,7 @@ __global__ void k_energy(DMesh m, DFields F, EnParams q, const double *psi, doub
     if (m.blend_on) {
       const int cu = FT > 0.0 ? a : b;
       double rr[3];
-      for (int d = 0; d < 3; d++) rr[d] = m.xf[3 * (size_t)f + d] - m.xc[3 * (size_t)cu + d];
+      face_offset(m, f, cu, !(FT > 0.0), rr);
       double dc = psi[cu] * ddot3(&F.gt[3 * (size_t)cu], rr);
       dc = q.blend * FT * dc;
       rhs = side == 0 ? rhs - dc : rhs + dc;
@@ -430,6 +440,7 @@ __global__ void k_rhiechow(DMesh m, DFields F, Ramp rp, double rho) {
   const int a = m.c0[f], b = m.c1[f];
   const double *sf = &m.sf[3 * (size_t)f];
   if (b < 0) {
+    if (m.fgrp[f] <= 0) return;   // shadow of a joined periodic face
     double uf[3];
     bc_face_vel(m, f, &F.u[3 * (size_t)a], uf, rp);
     F.flux[f] = rho * ddot3(uf, sf);

@@ -385,33 +385,9 @@ size_t Coarse::device_bytes() const {
 }
 
 // ---------------------------------------------------------------- system
-System *create_system(int nnodes, const double *x, int nfaces, const long long *fptr, const int *fnodes,
-                      const int *c0in, const int *c1in, const int *zone, int ncells, const ms_ctrl_t *ctrl,
-                      const ms_bcgroup_t *groups) {
-  if (nnodes <= 0 || nfaces <= 0 || ncells <= 0 || !x || !fptr || !fnodes || !c0in || !c1in || !zone || !ctrl)
-    fail(MS_EINVAL, "ms_system_create: empty mesh or null argument");
-  if (ctrl->nbc <= 0 || ctrl->nbc > 32 || !groups) fail(MS_EINVAL, "ms_system_create: need 1..32 bc groups");
-  std::unique_ptr<System> S(new System());
-  System &s = *S;
-  s.nc = ncells;
-  s.nf = nfaces;
-  s.nn = nnodes;
-  s.nbc = ctrl->nbc;
-  s.ctrl = *ctrl;
-  s.groups.assign(groups, groups + ctrl->nbc);
-  const int nc = ncells, nf = nfaces;
-  for (int f = 0; f < nf; f++) {
-    if (c0in[f] < 1 || c0in[f] > nc || c1in[f] < 0 || c1in[f] > nc) fail(MS_EINVAL, "face with invalid cell id");
-    for (long long j = fptr[f]; j < fptr[f + 1]; j++)
-      if (fnodes[j] < 1 || fnodes[j] > nnodes) fail(MS_EINVAL, "face with invalid node id");
-    if (fptr[f + 1] - fptr[f] < 3) fail(MS_EINVAL, "face with fewer than 3 nodes");
-  }
-  if (ctrl->thermal_model != 0) fail(MS_EINVAL, "thermal_model = ltne is not supported on the device path");
-  for (auto &g : s.groups)
-    if (g.btype == MS_BC_INTERFACE || g.btype < 1 || g.btype > 10)
-      fail(MS_EINVAL, "unsupported boundary type (coupling interface)");
-
-  // ---- natural-order geometry: compute_geometry -------------------------
+// ---- natural-order geometry: compute_geometry (mod_uns_geometry.f90:40-174)
+static void compute_geometry(System &s, const double *x, int nf, const long long *fptr, const int *fnodes,
+                             const int *c0in, const int *c1orig, int nc) {
   s.xf.assign(3 * (size_t)nf, 0.0);
   s.sf.assign(3 * (size_t)nf, 0.0);
   s.area.assign(nf, 0.0);
@@ -458,7 +434,7 @@ System *create_system(int nnodes, const double *x, int nfaces, const long long *
     std::vector<double> pcen(3 * (size_t)nc, 0.0);
     std::vector<int> cnt(nc, 0);
     for (int f = 0; f < nf; f++) {
-      int a = c0in[f] - 1, b = c1in[f] - 1;
+      int a = c0in[f] - 1, b = c1orig[f] - 1;
       for (long long j = fptr[f]; j < fptr[f + 1]; j++) {
         for (int d = 0; d < 3; d++) pcen[3 * a + d] = pcen[3 * a + d] + X(fnodes[j], d);
         cnt[a]++;
@@ -481,7 +457,7 @@ System *create_system(int nnodes, const double *x, int nfaces, const long long *
     const double THIRD = 1.0 / 3.0, QRTR = 1.0 / 4.0;
     for (int f = 0; f < nf; f++)
       for (int side = 0; side < 2; side++) {
-        int cell = (side == 0 ? c0in[f] : c1in[f]) - 1;
+        int cell = (side == 0 ? c0in[f] : c1orig[f]) - 1;
         if (cell < 0) continue;
         const double *pc = &pcen[3 * cell];
         double r[3];
@@ -498,7 +474,161 @@ System *create_system(int nnodes, const double *x, int nfaces, const long long *
         fail(MS_EINVAL, "non-positive cell volume");
     }
   }
-  // lf, mod_uns_fields.f90:147-157
+}
+
+// Periodic pairs (no reference counterpart).  Every boundary face of zone A
+// is matched with the boundary face of zone B whose centroid, translated by
+// the pair's shift, lies on its own (hash of the centroids on a grid of the
+// smallest face length, search of the 27 neighbouring bins, tolerance 1e-3 of
+// the face length).  The A face becomes an interior face whose c1 is B's
+// cell, seen at xc + shift; the B face becomes its shadow and leaves every
+// face list.  c1v receives the joined topology (1-based, 0 = boundary).
+static void join_periodic(System &s, const ms_ctrl_t *ctrl, int nf, const int *c0in, const int *c1in,
+                          const int *zone, std::vector<int> &c1v) {
+  s.nper = ctrl->nperiodic;
+  s.pidx.assign(nf, 0);
+  s.shadow.assign(nf, -1);
+  c1v.assign(c1in, c1in + nf);
+  if (s.nper == 0) return;
+  if (s.nper < 0 || s.nper > 4) fail(MS_EINVAL, "nperiodic must be 0..4");
+  for (int k = 0; k < s.nper; k++) {
+    const int zA = ctrl->periodic_zone[k][0], zB = ctrl->periodic_zone[k][1];
+    const std::string tag = "periodic pair " + std::to_string(zA) + " " + std::to_string(zB) + ": ";
+    if (zA == zB) fail(MS_EINVAL, tag + "the two zones must differ");
+    std::vector<int> fa, fb;
+    for (int f = 0; f < nf; f++)
+      if (c1in[f] == 0) {
+        if (zone[f] == zA) fa.push_back(f);
+        else if (zone[f] == zB) fb.push_back(f);
+      }
+    if (fa.empty() || fa.size() != fb.size()) fail(MS_EINVAL, tag + "zones need equal, non-zero face counts");
+    double sh[3];
+    if (ctrl->periodic_auto[k]) {
+      double ma[3] = {0, 0, 0}, mb[3] = {0, 0, 0};
+      for (int f : fa)
+        for (int d = 0; d < 3; d++) ma[d] += s.xf[3 * (size_t)f + d];
+      for (int f : fb)
+        for (int d = 0; d < 3; d++) mb[d] += s.xf[3 * (size_t)f + d];
+      for (int d = 0; d < 3; d++) sh[d] = ma[d] / (double)fa.size() - mb[d] / (double)fb.size();
+    } else {
+      for (int d = 0; d < 3; d++) sh[d] = ctrl->periodic_shift[k][d];
+    }
+    double h = INFINITY;
+    for (int f : fa) h = std::min(h, std::sqrt(s.area[f]));
+    if (!(h > 0.0) || !std::isfinite(h)) fail(MS_EINVAL, tag + "degenerate faces");
+    struct Bin {
+      long long i, j, k;
+      int f;
+      bool operator<(const Bin &o) const {
+        return i != o.i ? i < o.i : j != o.j ? j < o.j : k != o.k ? k < o.k : f < o.f;
+      }
+    };
+    auto bin = [&](const double *p, int f) {
+      return Bin{(long long)std::floor(p[0] / h), (long long)std::floor(p[1] / h), (long long)std::floor(p[2] / h), f};
+    };
+    std::vector<Bin> bins;
+    bins.reserve(fa.size());
+    for (int f : fa) bins.push_back(bin(&s.xf[3 * (size_t)f], f));
+    std::sort(bins.begin(), bins.end());
+    for (int g : fb) {
+      double t[3];
+      for (int d = 0; d < 3; d++) t[d] = s.xf[3 * (size_t)g + d] + sh[d];
+      const Bin c = bin(t, -1);
+      int best = -1;
+      double bd = INFINITY;
+      for (long long di = -1; di <= 1; di++)
+        for (long long dj = -1; dj <= 1; dj++)
+          for (long long dk = -1; dk <= 1; dk++) {
+            Bin lo{c.i + di, c.j + dj, c.k + dk, -1};
+            for (auto it = std::lower_bound(bins.begin(), bins.end(), lo);
+                 it != bins.end() && it->i == lo.i && it->j == lo.j && it->k == lo.k; ++it) {
+              double r[3];
+              for (int d = 0; d < 3; d++) r[d] = s.xf[3 * (size_t)it->f + d] - t[d];
+              const double dd = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+              if (dd < bd) bd = dd, best = it->f;
+            }
+          }
+      const double lg = std::sqrt(s.area[g]);
+      if (best < 0 || bd > 1.0e-3 * lg) fail(MS_EINVAL, tag + "a face of the second zone has no translated partner");
+      if (s.pidx[best] != 0 || s.shadow[best] >= 0 || s.shadow[g] >= 0 || s.pidx[g] != 0)
+        fail(MS_EINVAL, tag + "a face is matched twice (zones shared between pairs?)");
+      double sum[3];
+      for (int d = 0; d < 3; d++) sum[d] = s.sf[3 * (size_t)best + d] + s.sf[3 * (size_t)g + d];
+      if (std::fabs(s.area[best] - s.area[g]) > 1.0e-6 * s.area[g] ||
+          std::sqrt(sum[0] * sum[0] + sum[1] * sum[1] + sum[2] * sum[2]) > 1.0e-6 * s.area[g])
+        fail(MS_EINVAL, tag + "partner faces differ in area or are not opposite");
+      if (c0in[best] == c0in[g])
+        fail(MS_EINVAL, tag + "a cell would neighbour itself (need two cells across the periodic direction)");
+      c1v[best] = c0in[g];
+      s.pidx[best] = (signed char)(k + 1);
+      s.shadow[g] = best;
+    }
+    for (int d = 0; d < 3; d++) s.psh[k][d] = sh[d];
+  }
+}
+
+static void check_faces(int nnodes, int nf, const long long *fptr, const int *fnodes, const int *c0in,
+                        const int *c1orig, int nc) {
+  for (int f = 0; f < nf; f++) {
+    if (c0in[f] < 1 || c0in[f] > nc || c1orig[f] < 0 || c1orig[f] > nc) fail(MS_EINVAL, "face with invalid cell id");
+    for (long long j = fptr[f]; j < fptr[f + 1]; j++)
+      if (fnodes[j] < 1 || fnodes[j] > nnodes) fail(MS_EINVAL, "face with invalid node id");
+    if (fptr[f + 1] - fptr[f] < 3) fail(MS_EINVAL, "face with fewer than 3 nodes");
+  }
+}
+
+void periodic_match(int nnodes, const double *x, int nfaces, const long long *fptr, const int *fnodes,
+                    const int *c0in, const int *c1orig, const int *zone, int ncells, const ms_ctrl_t *ctrl,
+                    int *partner) {
+  if (nnodes <= 0 || nfaces <= 0 || ncells <= 0 || !x || !fptr || !fnodes || !c0in || !c1orig || !zone || !ctrl ||
+      !partner)
+    fail(MS_EINVAL, "ms_periodic_match: empty mesh or null argument");
+  check_faces(nnodes, nfaces, fptr, fnodes, c0in, c1orig, ncells);
+  System s;
+  s.nc = ncells;
+  s.nf = nfaces;
+  compute_geometry(s, x, nfaces, fptr, fnodes, c0in, c1orig, ncells);
+  std::vector<int> c1v;
+  join_periodic(s, ctrl, nfaces, c0in, c1orig, zone, c1v);
+  for (int f = 0; f < nfaces; f++) partner[f] = -1;
+  for (int f = 0; f < nfaces; f++)
+    if (s.shadow[f] >= 0) {
+      partner[f] = s.shadow[f];
+      partner[s.shadow[f]] = f;
+    }
+}
+
+System *create_system(int nnodes, const double *x, int nfaces, const long long *fptr, const int *fnodes,
+                      const int *c0in, const int *c1orig, const int *zone, int ncells, const ms_ctrl_t *ctrl,
+                      const ms_bcgroup_t *groups) {
+  if (nnodes <= 0 || nfaces <= 0 || ncells <= 0 || !x || !fptr || !fnodes || !c0in || !c1orig || !zone || !ctrl)
+    fail(MS_EINVAL, "ms_system_create: empty mesh or null argument");
+  if (ctrl->nbc <= 0 || ctrl->nbc > 32 || !groups) fail(MS_EINVAL, "ms_system_create: need 1..32 bc groups");
+  std::unique_ptr<System> S(new System());
+  System &s = *S;
+  s.nc = ncells;
+  s.nf = nfaces;
+  s.nn = nnodes;
+  s.nbc = ctrl->nbc;
+  s.ctrl = *ctrl;
+  s.groups.assign(groups, groups + ctrl->nbc);
+  const int nc = ncells, nf = nfaces;
+  check_faces(nnodes, nf, fptr, fnodes, c0in, c1orig, nc);
+  if (ctrl->thermal_model != 0) fail(MS_EINVAL, "thermal_model = ltne is not supported on the device path");
+  for (auto &g : s.groups)
+    if (g.btype == MS_BC_INTERFACE || g.btype < 1 || g.btype > 10)
+      fail(MS_EINVAL, "unsupported boundary type (coupling interface)");
+
+  compute_geometry(s, x, nf, fptr, fnodes, c0in, c1orig, nc);
+
+  // periodic pairs join faces; from here on c1in is the joined topology
+  std::vector<int> c1v;
+  join_periodic(s, ctrl, nf, c0in, c1orig, zone, c1v);
+  const int *c1in = c1v.data();
+  auto is_shadow = [&](int f) { return s.shadow[f] >= 0; };
+
+  // lf, mod_uns_fields.f90:147-157 (the c1 cell of a joined periodic face
+  // is seen at its centre + shift)
   s.lf.assign(nf, 1.0);
   for (int f = 0; f < nf; f++) {
     if (c1in[f] == 0) continue;
@@ -506,6 +636,7 @@ System *create_system(int nnodes, const double *x, int nfaces, const long long *
     for (int d = 0; d < 3; d++) {
       r0[d] = s.xf[3 * f + d] - s.xc_nat[3 * (c0in[f] - 1) + d];
       r1[d] = s.xc_nat[3 * (c1in[f] - 1) + d] - s.xf[3 * f + d];
+      if (s.pidx[f]) r1[d] = (s.xc_nat[3 * (c1in[f] - 1) + d] + s.psh[s.pidx[f] - 1][d]) - s.xf[3 * f + d];
     }
     double d0 = norm2_ref(r0, 3), d1 = norm2_ref(r1, 3);
     s.lf[f] = d1 / (d0 + d1);
@@ -530,19 +661,20 @@ System *create_system(int nnodes, const double *x, int nfaces, const long long *
         fail(MS_EINVAL, "velocity-inlet-parabolic needs span > 0 and axis 1|2|3");
       int nb = 0;
       for (int f = 0; f < nf; f++)
-        if (c1in[f] == 0 && zone[f] == g.zone) {
+        if (c1in[f] == 0 && !is_shadow(f) && zone[f] == g.zone) {
           nb++;
           s.fgrp[f] = i + 1;
         }
       if (nb == 0) fail(MS_EINVAL, "bc zone has no boundary faces");
     }
     for (int f = 0; f < nf; f++)
-      if (c1in[f] == 0 && s.fgrp[f] == 0) fail(MS_EINVAL, "boundary face without bc specification");
+      if (c1in[f] == 0 && !is_shadow(f) && s.fgrp[f] == 0) fail(MS_EINVAL, "boundary face without bc specification");
   }
 
   // ---- natural connectivity and reference CSR ---------------------------
   std::vector<int> cfp(nc + 1, 0), cfn, c2p(nc + 1, 0), c2c;
   for (int f = 0; f < nf; f++) {
+    if (is_shadow(f)) continue;
     cfp[c0in[f]]++;
     if (c1in[f] > 0) cfp[c1in[f]]++;
   }
@@ -551,6 +683,7 @@ System *create_system(int nnodes, const double *x, int nfaces, const long long *
   {
     std::vector<int> fill(cfp.begin(), cfp.end() - 1);
     for (int f = 0; f < nf; f++) {
+      if (is_shadow(f)) continue;
       cfn[fill[c0in[f] - 1]++] = f;
       if (c1in[f] > 0) cfn[fill[c1in[f] - 1]++] = f;
     }
@@ -710,6 +843,7 @@ System *create_system(int nnodes, const double *x, int nfaces, const long long *
   s.d_c0.upload(s.c0);
   s.d_c1.upload(s.c1);
   s.d_fgrp.upload(s.fgrp);
+  if (s.nper > 0) s.d_pidx.upload(s.pidx);
   s.d_cf_ptr.upload(s.cf_ptr);
   s.d_cf_ent.upload(s.cf_ent);
   s.d_cf_slot.upload(s.cf_slot);
@@ -792,10 +926,14 @@ void System::refresh_face_constants() {
   for (int f = 0; f < nf; f++) {
     const int a = n2o[c0[f]];
     const double *sfp = &sf[3 * (size_t)f];
+    if (shadow[f] >= 0) continue;   // replaced by its joined periodic face
     if (c1[f] >= 0) {
       const int b = n2o[c1[f]];
       double dvec[3];
       for (int d = 0; d < 3; d++) dvec[d] = xc_nat[3 * (size_t)b + d] - xc_nat[3 * (size_t)a + d];
+      if (pidx[f])
+        for (int d = 0; d < 3; d++)
+          dvec[d] = (xc_nat[3 * (size_t)b + d] + psh[pidx[f] - 1][d]) - xc_nat[3 * (size_t)a + d];
       dn[f] = norm2_ref(dvec, 3);
       dvs[f] = dot3(dvec, sfp);
       const double sd = dot3(sfp, dvec);
@@ -902,6 +1040,10 @@ DMesh System::dmesh() const {
   m.tval = d_tval.p;
   m.qval = d_qval.p;
   m.bc = d_bc.p;
+  m.nper = nper;
+  m.pidx = d_pidx.p;
+  for (int k = 0; k < 4; k++)
+    for (int d = 0; d < 3; d++) m.psh[k][d] = psh[k][d];
   return m;
 }
 
@@ -941,7 +1083,7 @@ size_t System::device_bytes() const {
                                 &psi, &porosity_d, &perm_dir_d, &inertial_d, &k_s_d, &rho_s_d, &cp_s_d, &val, &rhs,
                                 &ap, &pp, &dpp, &af, &ucomp, &part, &diag8, &tmpc})
     b += v->bytes();
-  b += d_c0.bytes() + d_c1.bytes() + d_fgrp.bytes() + d_cf_ptr.bytes() + d_cf_ent.bytes() + d_cf_slot.bytes() +
+  b += d_c0.bytes() + d_c1.bytes() + d_fgrp.bytes() + d_pidx.bytes() + d_cf_ptr.bytes() + d_cf_ent.bytes() + d_cf_slot.bytes() +
        d_diag_slot.bytes();
   return b;
 }

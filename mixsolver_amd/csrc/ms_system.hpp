@@ -26,6 +26,11 @@ struct DMesh {
   const signed char *tmode;                // boundary thermal mode 0 neumann0 1 dirichlet 2 flux
   const double *tval, *qval;
   const DevBC *bc;
+  // periodic pairs: pidx[f] = k + 1 on a joined face of pair k (0 elsewhere);
+  // its c1 cell sits at xc + psh[k] as seen from the face
+  int nper;
+  const signed char *pidx;
+  double psh[4][3];
 };
 
 struct DFields {
@@ -45,6 +50,12 @@ struct System {
   int ncolors = 0;
   // host topology in device numbering
   std::vector<int> c0, c1, fgrp, cf_ptr, cf_ent;
+  // periodic pairs: per face 0 or pair + 1 on joined faces; shadow[f] is the
+  // joined face that replaces boundary face f (-1 if f is not a shadow)
+  int nper = 0;
+  double psh[4][3] = {};
+  std::vector<signed char> pidx;
+  std::vector<int> shadow;
   std::vector<long long> cf_slot, diag_slot;
   // natural-order geometry (face arrays: face order; cell arrays: natural order)
   std::vector<double> xf, sf, area, lf, vol_nat, xc_nat;
@@ -66,7 +77,7 @@ struct System {
   DBuf<long long> d_cf_slot, d_diag_slot;
   DBuf<double> d_xf, d_sf, d_area, d_lf, d_vol, d_xc;
   DBuf<double> d_dn, d_dvs, d_Dm, d_muf, d_tcor, d_dw, d_Dmb, d_Dt, d_kf, d_Dtb, d_tval, d_qval;
-  DBuf<signed char> d_tmode;
+  DBuf<signed char> d_tmode, d_pidx;
   DBuf<DevBC> d_bc;
   // device fields (device numbering)
   DBuf<double> u, p, T, T_s, flux, apc, u_old, u_old_old, T_old, T_old_old, T_s_old, T_s_old_old;
@@ -87,6 +98,12 @@ struct System {
 System *create_system(int nnodes, const double *x, int nfaces, const long long *fptr, const int *fnodes,
                       const int *c0, const int *c1, const int *zone, int ncells, const ms_ctrl_t *ctrl,
                       const ms_bcgroup_t *groups);
+
+// Host only: partner[f] = the face joined with boundary face f by the
+// periodic pairs of ctrl (both directions), -1 elsewhere; same matching and
+// errors as create_system.
+void periodic_match(int nnodes, const double *x, int nfaces, const long long *fptr, const int *fnodes,
+                    const int *c0, const int *c1, const int *zone, int ncells, const ms_ctrl_t *ctrl, int *partner);
 
 // flang runtime NORM2 evaluation order (max-scaled accumulator), used for
 // every geometric length so the device coefficients equal the reference's.

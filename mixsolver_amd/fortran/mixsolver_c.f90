@@ -39,6 +39,10 @@ module mixsolver_c
       integer(c_int) :: gpu_ppe_precond, gpu_mom_precond, gpu_precision, gpu_ppe_solver
       real(c_double) :: gpu_inner_tol
       integer(c_int) :: gpu_reorder, gpu_coarse
+      integer(c_int) :: nperiodic
+      integer(c_int) :: periodic_zone(2, 4)
+      integer(c_int) :: periodic_auto(4)
+      real(c_double) :: periodic_shift(3, 4)
    end type ms_ctrl_t
 
    type, bind(c) :: ms_bcgroup_t

@@ -66,7 +66,7 @@ def _hex_nodes(xs, ys, zs):
                      Z.transpose(2, 1, 0).ravel()], axis=1).astype(np.float64)
 
 
-def hex_box(xs, ys, zs, zone_ids=None, jitter=0.0, seed=0):
+def hex_box(xs, ys, zs, zone_ids=None, jitter=0.0, seed=0, periodic=(), tile_x=None):
     """Hexahedral box on the tensor grid xs x ys x zs.
 
     zone_ids maps side names ('xmin','xmax','ymin','ymax','zmin','zmax') to
@@ -74,7 +74,11 @@ def hex_box(xs, ys, zs, zone_ids=None, jitter=0.0, seed=0):
     reference's shipped cavity mesh.  Interior faces are emitted per owner
     cell in natural cell order (+x, +y, +z), then one block per boundary side.
     ``jitter`` moves interior nodes by a seeded uniform fraction of the local
-    spacing in every direction.
+    spacing in every direction.  Axes named in ``periodic`` ('x', 'z') get
+    the same jitter on their two end planes, so the end faces are exact
+    translates of each other.  ``tile_x`` = m repeats the jitter of an m-cell
+    wide box along x (the jitter of node i is that of node i mod m), so a box
+    of k*m uniform x cells is k exact copies of the m-cell box.
     """
     xs = np.asarray(xs, np.float64)
     ys = np.asarray(ys, np.float64)
@@ -94,8 +98,13 @@ def hex_box(xs, ys, zs, zone_ids=None, jitter=0.0, seed=0):
         dx = np.minimum(np.diff(xs)[np.clip(i - 1, 0, nx - 1)], np.diff(xs)[np.clip(i, 0, nx - 1)])
         dy = np.minimum(np.diff(ys)[np.clip(j - 1, 0, ny - 1)], np.diff(ys)[np.clip(j, 0, ny - 1)])
         dz = np.minimum(np.diff(zs)[np.clip(k - 1, 0, nz - 1)], np.diff(zs)[np.clip(k, 0, nz - 1)])
-        r = rng.uniform(-jitter, jitter, size=(x.shape[0], 3))
-        inner_x = (i > 0) & (i < nx)
+        m1 = (tile_x + 1) if tile_x else N1
+        r = rng.uniform(-jitter, jitter, size=(m1 * N2 * (nz + 1), 3))
+        src_i = i % tile_x if tile_x else i
+        src_i = np.where(i == nx, 0, src_i) if "x" in periodic else src_i
+        src_k = np.where(k == nz, 0, k) if "z" in periodic else k
+        r = r[src_i + m1 * (j + N2 * src_k)]
+        inner_x = (i > 0) & (i < nx) & ((i % tile_x > 0) if tile_x else True)
         inner_y = (j > 0) & (j < ny)
         inner_z = (k > 0) & (k < nz)
         x[:, 0] += np.where(inner_x, r[:, 0] * dx, 0.0)
@@ -207,16 +216,19 @@ def tanh_stretch(n, ratio=1.1, lo=-1.0, hi=1.0):
     return lo + (y + 1.0) * 0.5 * (hi - lo)
 
 
-def channel(nx=400, ny=100, nz=50, lx=4 * np.pi, lz=2 * np.pi, ratio=1.1):
+def channel(nx=400, ny=100, nz=50, lx=4 * np.pi, lz=2 * np.pi, ratio=1.1, periodic=False, jitter=0.0, seed=0):
     """Plane channel [0,lx] x [-1,1] x [0,lz] with tanh wall clustering.
 
     Zones: 3 interior, 6 inlet (x=0), 7 outlet (x=lx), 5 walls (y=+-1),
-    4 side planes (z=0, z=lz).
+    4 side planes (z=0, z=lz).  periodic=True gives the z planes their own
+    zones, 8 (z=0) and 9 (z=lz), for periodic pairs 7-6 and 9-8.
     """
     xs = np.linspace(0.0, lx, nx + 1)
     ys = tanh_stretch(ny, ratio)
     zs = np.linspace(0.0, lz, nz + 1)
-    return hex_box(xs, ys, zs, zone_ids=dict(xmin=6, xmax=7, ymin=5, ymax=5, zmin=4, zmax=4))
+    z = dict(xmin=6, xmax=7, ymin=5, ymax=5, zmin=8, zmax=9) if periodic else \
+        dict(xmin=6, xmax=7, ymin=5, ymax=5, zmin=4, zmax=4)
+    return hex_box(xs, ys, zs, zone_ids=z, jitter=jitter, seed=seed, periodic=("x", "z") if periodic else ())
 
 
 # ---------------------------------------------------------------------------

@@ -180,3 +180,22 @@ def coarse_aggregates(xc, k):
     agg = np.zeros(len(xc), np.int32)
     check(_lib.lib().ms_coarse_aggregates(len(xc), dp(xc), int(k), ip(agg)))
     return agg
+
+
+def periodic_match(mesh, control_text):
+    """Face pairing of the control text's ``periodic`` lines on ``mesh``:
+    partner[f] = the face joined with boundary face f (0-based, both
+    directions), -1 elsewhere.  Host only (no device needed); raises on the
+    errors the device system create would raise."""
+    ctrl, _ = parse_control(control_text, warn=False)
+    x = np.ascontiguousarray(mesh.x, np.float64)
+    fptr = np.ascontiguousarray(mesh.fptr, np.int64)
+    partner = np.zeros(mesh.nfaces, np.int32)
+    check(_lib.lib().ms_periodic_match(mesh.nnodes, dp(x), mesh.nfaces,
+                                       fptr.ctypes.data_as(C.POINTER(C.c_longlong)),
+                                       ip(np.ascontiguousarray(mesh.fnodes, np.int32)),
+                                       ip(np.ascontiguousarray(mesh.c0, np.int32)),
+                                       ip(np.ascontiguousarray(mesh.c1, np.int32)),
+                                       ip(np.ascontiguousarray(mesh.zone, np.int32)),
+                                       mesh.ncells, C.byref(ctrl), ip(partner)))
+    return partner

@@ -440,6 +440,7 @@ or_case *or_create(int nn, const double *x, int nf, const long long *fptr,
   for (int f = 0; f < nf; f++)
     if (cs->c1[f] == 0 && cs->fgrp[f] == 0) *ier = 12;
   if (ctrl->thermal_model != 0) *ier = 50;
+  if (ctrl->nperiodic != 0) *ier = 50;   /* the reference has no periodic boundary */
   /* init_fields, mod_uns_fields.f90:86-159 */
   size_t n3 = 3 * (size_t)nc;
   cs->u = xcalloc(n3, 8); cs->p = xcalloc(nc, 8); cs->gp = xcalloc(n3, 8);

@@ -39,6 +39,11 @@ typedef struct {
   int gpu_ppe_precond, gpu_mom_precond, gpu_precision, gpu_ppe_solver;
   double gpu_inner_tol;
   int gpu_reorder, gpu_coarse;
+  /* periodic pairs: the reference has none; or_create rejects nperiodic > 0 */
+  int nperiodic;
+  int periodic_zone[4][2];
+  int periodic_auto[4];
+  double periodic_shift[4][3];
 } or_ctrl_t;
 
 /* one boundary group (bcgroup_t, mod_uns_bc.f90:80-110) */

@@ -144,6 +144,60 @@ def test_coarse_aggregates_rejects_bad_k():
             coarse_aggregates(xc, k)
 
 
+PER_CTL = "bc = 5 wall\nperiodic = 7 6\nperiodic = 9 8\n"
+
+
+def test_periodic_control_grammar():
+    ctrl, groups = parse_control(PER_CTL + "periodic = 11 10 0.5 0 -2.25\n", warn=False)
+    assert ctrl.nperiodic == 3
+    assert [list(ctrl.periodic_zone[k]) for k in range(3)] == [[7, 6], [9, 8], [11, 10]]
+    assert list(ctrl.periodic_auto)[:3] == [1, 1, 0]
+    assert list(ctrl.periodic_shift[2]) == [0.5, 0.0, -2.25]
+    for bad in ("periodic = 7\n", "periodic = 7 7\n", "periodic = 7 6 1 2\n", "periodic = 1 2\n" * 5,
+                "bc = 7 wall\n"):
+        with pytest.raises(ControlError):
+            parse_control(PER_CTL + bad, warn=False)
+
+
+def test_periodic_match_closed_form():
+    """hex_box emits the x-min and x-max faces (and z-min / z-max) in the same
+    (j, k) order, so face i of one zone pairs with face i of the other; the
+    jittered end planes are exact translates."""
+    from mixsolver_amd.solver import periodic_match
+    nx, ny, nz, lx, lz = 6, 4, 5, 2.0, 1.5
+    m = meshgen.channel(nx, ny, nz, lx=lx, lz=lz, periodic=True, jitter=0.15, seed=1)
+    auto = periodic_match(m, PER_CTL)
+    given = periodic_match(m, f"bc = 5 wall\nperiodic = 7 6 {lx!r} 0 0\nperiodic = 9 8 0 0 {lz!r}\n")
+    assert np.array_equal(auto, given)
+    z = m.zone
+    for za, zb, cnt in ((7, 6, ny * nz), (9, 8, nx * ny)):
+        fa, fb = np.nonzero(z == za)[0], np.nonzero(z == zb)[0]
+        assert len(fa) == len(fb) == cnt
+        assert np.array_equal(auto[fa], fb) and np.array_equal(auto[fb], fa)
+    assert np.all(auto[np.isin(z, [3, 5])] == -1)
+
+
+def test_periodic_match_rejects_bad_pairs():
+    from mixsolver_amd._lib import MixSolverError
+    from mixsolver_amd.solver import periodic_match
+    m = meshgen.channel(6, 4, 5, lx=2.0, lz=1.5, periodic=True, jitter=0.15, seed=1)
+    for ctl in ("bc = 5 wall\nperiodic = 7 6 1.0 0 0\nperiodic = 9 8\n",     # wrong shift
+                "bc = 5 wall\nperiodic = 7 9\n",                               # face counts differ
+                "bc = 5 wall\nperiodic = 7 6\nperiodic = 7 8\n"):             # zone in two pairs
+        with pytest.raises(MixSolverError):
+            periodic_match(m, ctl)
+    # end planes jittered independently are not translates of each other
+    zid = dict(xmin=6, xmax=7, ymin=5, ymax=5, zmin=8, zmax=9)
+    j = meshgen.hex_box(np.linspace(0, 2, 7), np.linspace(-1, 1, 5), np.linspace(0, 1.5, 6), zone_ids=zid,
+                        jitter=0.15, seed=1)
+    with pytest.raises(MixSolverError):
+        periodic_match(j, "bc = 5 wall\nperiodic = 7 6\n")
+    # one cell across: a cell would be its own neighbour
+    one = meshgen.channel(1, 4, 3, lx=1.0, lz=1.0, periodic=True)
+    with pytest.raises(MixSolverError, match="itself"):
+        periodic_match(one, PER_CTL)
+
+
 FLANG = shutil.which("amdflang") or shutil.which("flang")
 
 
