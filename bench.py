@@ -3,11 +3,13 @@
 Workload (one step = one PISO time step of piso_run_mpi at one partition):
   channel 400 x 100 x 50 hex cells (4*pi x 2 x 2*pi, wall-normal tanh
   clustering, adjacent-spacing ratio 1.1), Re_tau = 180 (mu = 1/180,
-  body force 1), parabolic inlet U_b = 15.63 / outflow in x, walls in y,
-  symmetry in z; initial field = laminar parabola + seeded uniform noise of
-  +-5% U_b (seed 180); dt from CFL 0.5; BDF2; 2 PISO correctors; p' by
-  PCG with multicolour DILU to a 1e-10 relative residual, momentum by
-  BiCGSTAB with multicolour DILU to 1e-10.
+  body force 1), periodic in x and z, walls in y; initial field = laminar
+  parabola of bulk velocity U_b = 15.63 + seeded uniform noise of +-5% U_b
+  (seed 180); dt from CFL 0.5; BDF2; 2 PISO correctors; p' by PCG with
+  multicolour DILU to a 1e-10 relative residual, momentum by BiCGSTAB with
+  multicolour DILU to 1e-10.  --bc twin runs the non-periodic twin instead
+  (parabolic inlet / outflow in x, symmetry in z), the case the CPU oracle
+  times, since the reference has no periodic boundary.
   * mixed: FP32 matrix / preconditioner / Krylov vectors with FP32
     accumulation inside FP64 reliable updates (p') and FP64 iterative
     refinement (momentum); every solve ends at a 1e-10 FP64 true residual.
@@ -35,9 +37,11 @@ HBM_PEAK = 8.0e12
 UB = 15.63
 
 
-def control_text(nx, lin_max=20000):
+def control_text(nx, lin_max=20000, periodic=True):
     dx = 4 * np.pi / nx
     dt = 0.5 * dx / (1.5 * UB * 1.05)
+    bcs = ("bc = 5 wall\nperiodic = 7 6\nperiodic = 9 8\n" if periodic else
+           f"bc = 6 velocity-inlet-parabolic {UB!r} 2.0 2 -1.0\nbc = 7 outflow\nbc = 5 wall\nbc = 4 symmetry\n")
     return f"""rho = 1.0
 mu = {1.0 / 180.0!r}
 transient = true
@@ -51,11 +55,7 @@ conv_blend = 0.0
 nonorth_corr = 1.0
 ppe_precond = jacobi
 body_force = 1.0 0.0 0.0
-bc = 6 velocity-inlet-parabolic {UB!r} 2.0 2 -1.0
-bc = 7 outflow
-bc = 5 wall
-bc = 4 symmetry
-"""
+{bcs}"""
 
 
 def initial_field(mesh, ny, nz, nx):
@@ -89,15 +89,15 @@ def run_variant(Case, mesh, ctl, u0, over, steps, warmup):
     return cs, total, np.array(lat), np.array(hist)
 
 
-def cpu_baseline(mesh, ctl, u0, n_needed):
+def cpu_baseline(mesh, u0, n_needed):
     """Oracle (C port of the reference path, 1 thread) on a bounded sample:
-    one PISO step of the same 2M-cell case with the p' PCG-Jacobi solves
-    capped at 40 iterations; the full step is extrapolated with the measured
-    per-iteration p' cost to the iteration count the FP64 Jacobi device
-    solve needed for the same step."""
+    one PISO step of the 2M-cell non-periodic twin channel with the p'
+    PCG-Jacobi solves capped at 40 iterations; the full step is extrapolated
+    with the measured per-iteration p' cost to the iteration count the FP64
+    Jacobi device solve needed for the same step."""
     from mixsolver_amd.control import parse_control
     from oracle import pyoracle
-    ctrl, groups = parse_control(control_text(400, lin_max=40), warn=False)
+    ctrl, groups = parse_control(control_text(400, lin_max=40, periodic=False), warn=False)
     ctrl.ppe_precond = 0
     oc = pyoracle.OracleCase(mesh, ctrl, groups)
     oc.set_field("u", u0)
@@ -107,7 +107,8 @@ def cpu_baseline(mesh, ctl, u0, n_needed):
     per_it = t_ppe / max(iters, 1)
     t_step = (t_total - t_ppe) + per_it * n_needed
     return dict(value=1.0 / t_step, unit="steps/s", cores=1, kind="port",
-                sample=(f"oracle PISO step on the 2M-cell channel with p' PCG-Jacobi capped at 40 "
+                sample=(f"oracle PISO step on the 2M-cell non-periodic twin channel (parabolic inlet, outflow, "
+                        f"z symmetry) with p' PCG-Jacobi capped at 40 "
                         f"iterations ({t_total:.1f} s, {iters} p' iterations at {per_it * 1e3:.1f} ms); "
                         f"extrapolated to the {n_needed} p' iterations of the FP64 Jacobi device solve"))
 
@@ -173,6 +174,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--size", default="400x100x50")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--bc", choices=("periodic", "twin"), default="periodic",
+                    help="periodic x/z channel, or its non-periodic twin")
     ap.add_argument("--coarse", type=int, default=256, help="aggregates of the p' coarse space")
     ap.add_argument("--variants", default="mixed,fp64_two_level,fp64",
                     help="comma list of variants to run (profiling runs may select one)")
@@ -187,8 +190,9 @@ def main():
     from mixsolver_amd import meshgen
     from mixsolver_amd.solver import Case
     nx, ny, nz = (int(v) for v in args.size.split("x"))
-    mesh = meshgen.channel(nx, ny, nz)
-    ctl = control_text(nx)
+    periodic = args.bc == "periodic"
+    mesh = meshgen.channel(nx, ny, nz, periodic=periodic)
+    ctl = control_text(nx, periodic=periodic)
     u0 = initial_field(mesh, ny, nz, nx)
     base = dict(gpu_ppe_precond=2, gpu_mom_precond=2)
     two = dict(base, gpu_coarse=args.coarse)
@@ -236,7 +240,9 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 inner + f64 refinement" if lead == "mixed" else "f64",
         "data": "synthetic",
-        "config": {"workload": f"channel {nx}x{ny}x{nz} hex ({cells} cells), transient PISO, 2 correctors, "
+        "config": {"workload": f"channel {nx}x{ny}x{nz} hex ({cells} cells), "
+                               + ("periodic x/z" if periodic else "non-periodic twin (inlet/outflow, z symmetry)")
+                               + ", transient PISO, 2 correctors, "
                                f"p' PCG + multicolour DILU + {args.coarse}-aggregate coarse correction to "
                                "1e-10, momentum BiCGSTAB + DILU to 1e-10",
                    "cells": cells, "faces": mesh.nfaces, "time_scheme": "BDF2", "parallelism": "1 GPU",
@@ -283,11 +289,14 @@ def main():
     if args.serve_seconds > 0:
         out["serving"] = serve_bench(args.serve_rate, args.serve_seconds, args.serve_pool, args.coarse)
     if not args.no_cpu:
-        # p' iterations the reference's Jacobi PCG needs for the first step (device FP64 Jacobi run)
-        jac, _, _, h_jac = run_variant(Case, mesh, ctl, u0, dict(gpu_ppe_precond=0, gpu_mom_precond=2,
-                                                                 gpu_precision=0), 1, 0)
+        # the oracle runs the non-periodic twin (the reference has no periodic
+        # boundary); p' iterations its Jacobi PCG needs for the first step come
+        # from the device FP64 Jacobi run of the same twin
+        twin = mesh if not periodic else meshgen.channel(nx, ny, nz)
+        jac, _, _, h_jac = run_variant(Case, twin, control_text(nx, periodic=False), u0,
+                                       dict(gpu_ppe_precond=0, gpu_mom_precond=2, gpu_precision=0), 1, 0)
         jac.close()
-        out["cpu_baseline"] = cpu_baseline(mesh, ctl, u0, int(h_jac[0, 6]))
+        out["cpu_baseline"] = cpu_baseline(twin, u0, int(h_jac[0, 6]))
     print(json.dumps(out))
 
 
