@@ -65,6 +65,10 @@ typedef struct {
   int periodic_zone[4][2];
   int periodic_auto[4];
   double periodic_shift[4][3];
+  /* porous_model (mod_uns_control.f90:488-499): 0 = partitioned (kink face
+   * pressure and Beavers-Joseph slip at fluid/porous faces), 1 = unified
+   * (harmonic face viscosity, no interface treatment) */
+  int porous_unified;
 } ms_ctrl_t;
 
 /* One boundary group (bcgroup_t, mod_uns_bc.f90:80-110), control order. */
@@ -121,12 +125,19 @@ int ms_periodic_match(int nnodes, const double *x, int nfaces, const long long *
                       int *partner);
 
 /* fields_t components by name: "u"(3,nc) "p" "T" "T_s" "flux"(nf) "apc"
- * "u_old" "u_old_old" "T_old" "T_old_old" "gp"(3,nc) "gu"(3,3,nc) "gt"
- * "porosity" "perm_dir"(3,nc) "inertial" "k_s" "rho_s" "cp_s";
+ * "u_old" "u_old_old" "T_old" "T_old_old" "T_s_old" "T_s_old_old" "gp"(3,nc)
+ * "gu"(3,3,nc) "gt" "gts"; per-cell porous-medium coefficients
+ * (mod_uns_fields.f90:50-71, filled by setup_porous_fields :418-504)
+ * "porosity" "perm_dir"(3,nc) "inertial" "disp_l" "disp_t" "bj_alpha" "k_s"
+ * "rho_s" "cp_s" "h_sf" "a_sf" "q_src";
  * geometry (read-only): "xf" "sf" "area" "vol" "xc" "lf". */
 int ms_set_field(ms_system *s, const char *name, const double *host, long long n);
 int ms_get_field(ms_system *s, const char *name, double *host, long long n);
 int ms_set_ts_order(ms_system *s, int ts_order);
+/* Block type of every cell (m%cztype after resolve_cell_zones,
+ * mod_uns_control.f90:575-733): 1 fluid, 2 porous, 3 solid; NULL = all
+ * fluid.  Porous coefficients come through ms_set_field. */
+int ms_set_cell_types(ms_system *s, const int *cztype);
 int ms_get_topology(ms_system *s, int *row_ptr, int *col_idx, int *fgrp); /* 1-based CSR */
 
 /* ---- loop level: keep everything resident across iterations ----------- */
@@ -149,6 +160,7 @@ int ms_get_history(ms_system *s, double *out, int maxrows, int *nrows);
 int ms_compute_gradients(ms_system *s);                       /* mod_uns_fields.f90:164 */
 int ms_momentum_assembly(ms_system *s, int comp, double *val, double *rhs, double *ap); /* mod_uns_simple.f90:928 */
 int ms_temperature_assembly(ms_system *s, double *val, double *rhs, double *ap); /* :1748 */
+int ms_solid_temperature_assembly(ms_system *s, double *val, double *rhs, double *ap); /* :2109 */
 int ms_flux_rhiechow(ms_system *s);                           /* :1413 */
 int ms_outflow_mass_rescale(ms_system *s);                    /* :1536 */
 int ms_ppe_assembly(ms_system *s, double *val, double *rhs);  /* :1552 / mod_uns_simple_mpi.f90:882 */

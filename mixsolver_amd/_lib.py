@@ -48,6 +48,8 @@ def lib():
         "ms_set_field": (C.c_int, [_V, C.c_char_p, _D, C.c_longlong]),
         "ms_get_field": (C.c_int, [_V, C.c_char_p, _D, C.c_longlong]),
         "ms_set_ts_order": (C.c_int, [_V, C.c_int]),
+        "ms_set_cell_types": (C.c_int, [_V, _I]),
+        "ms_solid_temperature_assembly": (C.c_int, [_V, _D, _D, _D]),
         "ms_get_topology": (C.c_int, [_V, _I, _I, _I]),
         "ms_simple_run": (C.c_int, [_V, C.c_int, _I, _I]),
         "ms_piso_run": (C.c_int, [_V, C.c_int, _I]),

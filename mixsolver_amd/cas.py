@@ -31,8 +31,17 @@ def write_cas(path: str, m: Mesh, zone_kind: dict | None = None) -> None:
         f.write(f"(10 (1 1 {nn:x} 1 3)(\n")
         np.savetxt(f, m.x, fmt="%.17e")
         f.write("))\n")
-        f.write(f"(12 ({m.cell_zone:x} 1 {nc:x} 1 {m.ctype:x}))\n")
-        f.write(f"(45 ({m.cell_zone} fluid fluid)())\n")
+        cz = m.cell_zones()
+        ids = sorted(set(int(v) for v in cz))
+        for z in ids:
+            rows = np.nonzero(cz == z)[0]
+            first, last = int(rows[0]) + 1, int(rows[-1]) + 1
+            if last - first + 1 != len(rows):
+                raise ValueError(f"cell zone {z} is not one contiguous cell range (see meshgen.with_cell_zones)")
+            f.write(f"(12 ({z:x} {first:x} {last:x} 1 {m.ctype:x}))\n")
+        for z in ids:
+            cond, name = m.czone_names.get(z, ("fluid", "fluid"))
+            f.write(f"(45 ({z} {cond} {name})())\n")
         zones = m.zone
         start = 0
         while start < nf:
@@ -65,6 +74,7 @@ def read_cas(path: str) -> Mesh:
     x = None
     faces = {}            # first index -> (nodes list, c0, c1, zone)
     ncells, ctype, czone = 0, 4, 2
+    cranges, names = [], {}
 
     def group(p):
         # returns (tokens inside the parenthesised group starting at p, end)
@@ -98,6 +108,7 @@ def read_cas(path: str) -> Mesh:
                 if zid != 0:
                     ncells = max(ncells, last)
                     czone = zid
+                    cranges.append((zid, first, last))
                     if len(h) > 4:
                         ctype = h[4]
                 else:
@@ -130,8 +141,18 @@ def read_cas(path: str) -> Mesh:
                     recs.append((nodes, vals[k], vals[k + 1], zid))
                     k += 2
                 faces[first] = recs
+        elif sec == "45" and txt[e] == "(":
+            hdr, e2 = group(e)
+            if len(hdr) >= 3:
+                names[int(hdr[0])] = (hdr[1], hdr[2])
+            pos = e2
         else:
             pos = e
+    czarr = None
+    if len(cranges) > 1:
+        czarr = np.zeros(ncells, np.int32)
+        for zid, first, last in cranges:
+            czarr[first - 1:last] = zid
     order = sorted(faces)
     recs = [r for k in order for r in faces[k]]
     nfn = np.array([len(r[0]) for r in recs], np.int64)
@@ -142,4 +163,5 @@ def read_cas(path: str) -> Mesh:
                 c0=np.array([r[1] for r in recs], np.int32),
                 c1=np.array([r[2] for r in recs], np.int32),
                 zone=np.array([r[3] for r in recs], np.int32),
-                ncells=ncells, ctype=ctype, cell_zone=czone)
+                ncells=ncells, ctype=ctype, cell_zone=czone, czone=czarr,
+                czone_names={z: v for z, v in names.items() if any(z == c[0] for c in cranges)})

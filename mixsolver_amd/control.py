@@ -75,6 +75,7 @@ class MsCtrl(C.Structure):
         ("gpu_reorder", C.c_int), ("gpu_coarse", C.c_int),
         ("nperiodic", C.c_int), ("periodic_zone", (C.c_int * 2) * 4), ("periodic_auto", C.c_int * 4),
         ("periodic_shift", (C.c_double * 3) * 4),
+        ("porous_unified", C.c_int),
     ]
 
 
@@ -112,7 +113,47 @@ def default_ctrl() -> MsCtrl:
     c.boussinesq, c.thermal_model, c.inlet_ramp = 0, 0, 1
     c.gpu_ppe_precond, c.gpu_mom_precond, c.gpu_precision = -1, 1, 0
     c.gpu_ppe_solver, c.gpu_inner_tol, c.gpu_reorder, c.gpu_coarse = 0, 0.1, 0, 0
+    c.cell_zones = []          # parsed cell_zone lines (host side, see zones.py)
     return c
+
+
+CZ_AUTO, CZ_FLUID, CZ_POROUS, CZ_SOLID = 0, 1, 2, 3
+_CZ_KEYS = ("perm", "perm_xx", "perm_yy", "perm_zz", "disp_l", "disp_t", "bj_alpha", "inertial",
+            "porosity", "k_s", "cp_s", "rho_s", "h_sf", "a_sf", "q_src")
+
+
+def _parse_cell_zone(val):
+    """parse_cell_zone (mod_uns_control.f90:811-1005):
+    ``<id|name> [fluid|porous|solid] [key=value ...]``.  Without a type token
+    the line supplies coefficients only (CZ_AUTO)."""
+    toks = val.split()
+    if not toks:
+        raise ControlError(9, f"bad cell_zone line: {val}")
+    cz = dict(id=0, name="", ztype=CZ_AUTO, **{k: 0.0 for k in _CZ_KEYS})
+    cz["porosity"] = 1.0
+    try:
+        cz["id"] = int(toks[0])
+    except ValueError:
+        cz["name"] = toks[0][:32]
+    rest = toks[1:]
+    if rest and "=" not in rest[0]:
+        t = rest[0].lower()
+        if t not in ("fluid", "porous", "solid"):
+            raise ControlError(9, f"cell_zone type must be fluid, porous or solid: {rest[0]}")
+        cz["ztype"] = {"fluid": CZ_FLUID, "porous": CZ_POROUS, "solid": CZ_SOLID}[t]
+        rest = rest[1:]
+    for tok in rest:
+        if "=" not in tok:
+            raise ControlError(9, f"bad cell_zone option (need key=value): {tok}")
+        k, v = tok.split("=", 1)
+        k = k.strip().lower()
+        if k not in _CZ_KEYS:
+            raise ControlError(9, f"unknown cell_zone option: {k}")
+        try:
+            cz[k] = float(v.replace("d", "e").replace("D", "e"))
+        except ValueError:
+            raise ControlError(9, f"bad {k} value: {v}") from None
+    return cz
 
 
 def _floats(toks, n, code, val):
@@ -283,6 +324,18 @@ def parse_control(text: str, warn=True):
                 ctrl.gpu_coarse = int(val)
                 if not 0 <= ctrl.gpu_coarse <= 256:
                     raise ValueError(val)
+            elif key == "porous_model":
+                v = val.lower()
+                if v in ("partitioned", "partition", "block"):
+                    ctrl.porous_unified = 0
+                elif v in ("unified", "one-domain", "onedomain", "single-domain"):
+                    ctrl.porous_unified = 1
+                else:
+                    raise ControlError(2, f"porous_model must be partitioned or unified: {val}")
+            elif key == "cell_zone":
+                if len(ctrl.cell_zones) >= 32:
+                    raise ControlError(9, "too many cell_zone lines (max 32)")
+                ctrl.cell_zones.append(_parse_cell_zone(val))
             elif key == "periodic":
                 k = ctrl.nperiodic
                 if k >= 4 or len(toks) not in (2, 5):
@@ -295,9 +348,9 @@ def parse_control(text: str, warn=True):
                 if len(toks) == 5:
                     ctrl.periodic_shift[k][:] = [float(t) for t in toks[2:5]]
                 ctrl.nperiodic = k + 1
-            elif key in ("out_format", "out_fmt", "out_cells", "out_cell", "porous_model",
+            elif key in ("out_format", "out_fmt", "out_cells", "out_cell",
                          "restart", "restart_file", "partition_file", "dump_file",
-                         "mesh_scale", "cell_zone"):
+                         "mesh_scale"):
                 pass          # output / restart / zone keys: host-side, no device effect
             else:
                 if warn:

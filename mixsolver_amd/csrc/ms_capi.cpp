@@ -18,6 +18,7 @@ int run_pimple(System &s, int nsteps);
 void call_gradients(System &s);
 void call_momentum(System &s, int comp);
 void call_energy(System &s);
+void call_solid_energy(System &s);
 void call_rhiechow(System &s);
 void call_outflow(System &s);
 void call_ppe(System &s);
@@ -68,6 +69,8 @@ FieldRef lookup(System &s, const char *name) {
   if (n == "u_old_old") return {&s.u_old_old, 3, CELL, nullptr, true};
   if (n == "T_old") return {&s.T_old, 1, CELL, nullptr, true};
   if (n == "T_old_old") return {&s.T_old_old, 1, CELL, nullptr, true};
+  if (n == "T_s_old") return {&s.T_s_old, 1, CELL, nullptr, true};
+  if (n == "T_s_old_old") return {&s.T_s_old_old, 1, CELL, nullptr, true};
   if (n == "gp") return {&s.gp, 3, CELL, nullptr, true};
   if (n == "gu") return {&s.gu, 9, CELL, nullptr, true};
   if (n == "gt") return {&s.gt, 3, CELL, nullptr, true};
@@ -79,6 +82,12 @@ FieldRef lookup(System &s, const char *name) {
   if (n == "k_s") return {&s.k_s_d, 1, PROP, &s.k_s, true};
   if (n == "rho_s") return {&s.rho_s_d, 1, PROP, &s.rho_s, true};
   if (n == "cp_s") return {&s.cp_s_d, 1, PROP, &s.cp_s, true};
+  if (n == "disp_l") return {&s.disp_l_d, 1, PROP, &s.disp_l, true};
+  if (n == "disp_t") return {&s.disp_t_d, 1, PROP, &s.disp_t, true};
+  if (n == "bj_alpha") return {nullptr, 1, PROP, &s.bj_alpha, true};
+  if (n == "h_sf") return {&s.h_sf_d, 1, PROP, &s.h_sf, true};
+  if (n == "a_sf") return {&s.a_sf_d, 1, PROP, &s.a_sf, true};
+  if (n == "q_src") return {&s.q_src_d, 1, PROP, &s.q_src, true};
   fail(MS_EINVAL, "unknown field name: " + n);
 }
 
@@ -184,13 +193,13 @@ int ms_set_field(ms_system *h, const char *name, const double *host, long long n
       if (n != (long long)s.nc * f.width) fail(MS_EINVAL, "cell field size mismatch");
       std::vector<double> d;
       to_device_order(s, host, f.width, d);
-      f.buf->upload(d);
+      if (f.buf) f.buf->upload(d);
       if (f.kind == PROP) {
         f.host->assign(host, host + n);
         s.props_dirty = true;
       }
       std::string nm(name);
-      if (nm == "T" || nm == "T_old" || nm == "T_old_old" || nm == "T_s")
+      if (nm == "T" || nm == "T_old" || nm == "T_old_old" || nm == "T_s" || nm == "T_s_old" || nm == "T_s_old_old")
         for (long long i = 0; i < n; i++)
           if (host[i] != 0.0) {
             s.T_nonzero = true;
@@ -219,7 +228,10 @@ int ms_get_field(ms_system *h, const char *name, double *host, long long n) {
     if (nm == "xc") return copy(s.xc_nat);
     FieldRef f = lookup(s, name);
     std::vector<double> d;
-    if (f.kind == FACE) {
+    if (f.kind == PROP) {
+      if (n != (long long)s.nc * f.width) fail(MS_EINVAL, "cell field size mismatch");
+      std::memcpy(host, f.host->data(), (size_t)n * sizeof(double));
+    } else if (f.kind == FACE) {
       if (n != s.nf) fail(MS_EINVAL, "face field size mismatch");
       download(*f.buf, d, s.nf);
       // a periodic shadow face carries its joined face's flux, oriented out of its own cell
@@ -231,6 +243,20 @@ int ms_get_field(ms_system *h, const char *name, double *host, long long n) {
       download(*f.buf, d, (size_t)s.nc * f.width);
       from_device_order(s, d, f.width, host);
     }
+  });
+}
+
+int ms_set_cell_types(ms_system *h, const int *cztype) {
+  return guarded([&] {
+    System &s = sys(h);
+    if (!cztype) {
+      s.cztype.clear();
+    } else {
+      for (int i = 0; i < s.nc; i++)
+        if (cztype[i] < 1 || cztype[i] > 3) fail(MS_EINVAL, "cell type must be 1 (fluid), 2 (porous) or 3 (solid)");
+      s.cztype.assign(cztype, cztype + s.nc);
+    }
+    s.props_dirty = true;
   });
 }
 
@@ -308,6 +334,12 @@ int ms_momentum_assembly(ms_system *h, int comp, double *val, double *rhs, doubl
 int ms_temperature_assembly(ms_system *h, double *val, double *rhs, double *ap) {
   return guarded([&] {
     call_energy(sys(h));
+    matrix_out(sys(h), val, rhs, ap);
+  });
+}
+int ms_solid_temperature_assembly(ms_system *h, double *val, double *rhs, double *ap) {
+  return guarded([&] {
+    call_solid_energy(sys(h));
     matrix_out(sys(h), val, rhs, ap);
   });
 }

@@ -148,6 +148,23 @@ __device__ __forceinline__ void face_offset(const DMesh &m, int f, int cell, boo
   }
 }
 
+// kink_face_pressure (mod_uns_fields.f90:328-349): one-sided quadratic
+// reconstruction at a fluid/porous face from the cell gradients gp
+__device__ __forceinline__ double kink_pf(const DMesh &m, const double *p, const double *gp, int f, int a, int b) {
+  double dvec[3], nhat[3];
+  const int k = m.nper ? m.pidx[f] : 0;
+  for (int d = 0; d < 3; d++)
+    dvec[d] = (k ? m.xc[3 * (size_t)b + d] + m.psh[k - 1][d] : m.xc[3 * (size_t)b + d]) - m.xc[3 * (size_t)a + d];
+  const double dn = m.dn[f];
+  for (int d = 0; d < 3; d++) nhat[d] = dvec[d] / dn;
+  const double l = m.lf[f];
+  const double d0 = (1.0 - l) * dn, d1 = -l * dn;
+  return 0.5 * (p[a] + d0 * ddot3(&gp[3 * (size_t)a], nhat) + p[b] + d1 * ddot3(&gp[3 * (size_t)b], nhat));
+}
+
+__device__ __forceinline__ int face_flags(const DMesh &m, int f) { return m.has_face_flags ? m.fflag[f] : 0; }
+__device__ __forceinline__ bool solid_cell(const DFields &F, int r) { return F.cztype && F.cztype[r] == CZ_SOLID; }
+
 // ------------------------------------------------------------- gradients
 __global__ void k_gradients(DMesh m, DFields F, Ramp rp, int do_T) {
   const int r = blockIdx.x * kBlock + threadIdx.x;
@@ -161,7 +178,8 @@ __global__ void k_gradients(DMesh m, DFields F, Ramp rp, int do_T) {
     double phip, phiu[3], phit = 0.0, phis = 0.0;
     if (b >= 0) {
       const double l = m.lf[f];
-      phip = l * F.p[a] + (1.0 - l) * F.p[b];
+      if (face_flags(m, f) & FF_KINK) phip = kink_pf(m, F.p, F.gp_prev, f, a, b);
+      else phip = l * F.p[a] + (1.0 - l) * F.p[b];
       for (int c = 0; c < 3; c++) phiu[c] = l * F.u[3 * (size_t)a + c] + (1.0 - l) * F.u[3 * (size_t)b + c];
       if (do_T) {
         phit = l * F.T[a] + (1.0 - l) * F.T[b];
@@ -264,12 +282,25 @@ __global__ void k_momentum(DMesh m, DFields F, Ramp rp, MomParams q, const doubl
     const int a = m.c0[f];
     const double Ff = F.flux[f], D = m.Dm[f], l = m.lf[f];
     const long long sl = m.cf_slot[k];
+    const int fl = face_flags(m, f);
+    double Dface = D;
+    if (fl & FF_BJ) {   // Beavers-Joseph slip at a fluid/porous face (:1031-1059)
+      const double Cb = m.Cbj[f], ar = m.area[f];
+      double nh[3];
+      for (int d = 0; d < 3; d++) nh[d] = m.sf[3 * (size_t)f + d] / ar;
+      Dface = Cb + (D - Cb) * (nh[comp] * nh[comp]);
+      const double *u0 = &F.u[3 * (size_t)a], *u1 = &F.u[3 * (size_t)b];
+      const double rx = (D - Cb) * nh[comp] *
+                        ((u0[0] - u1[0]) * nh[0] + (u0[1] - u1[1]) * nh[1] + (u0[2] - u1[2]) * nh[2] -
+                         (u0[comp] - u1[comp]) * nh[comp]);
+      rhs = side == 0 ? rhs + rx : rhs - rx;
+    }
     if (side == 0) {
-      ap = ap + dmax(Ff, 0.0) + D;
-      val[sl] = val[sl] - (D + dmax(-Ff, 0.0));
+      ap = ap + dmax(Ff, 0.0) + Dface;
+      val[sl] = val[sl] - (Dface + dmax(-Ff, 0.0));
     } else {
-      ap = ap + dmax(-Ff, 0.0) + D;
-      val[sl] = val[sl] - (D + dmax(Ff, 0.0));
+      ap = ap + dmax(-Ff, 0.0) + Dface;
+      val[sl] = val[sl] - (Dface + dmax(Ff, 0.0));
     }
     if (m.blend_on) {
       const int cu = Ff > 0.0 ? a : b;
@@ -280,14 +311,14 @@ __global__ void k_momentum(DMesh m, DFields F, Ramp rp, MomParams q, const doubl
       dc = q.blend * Ff * dc;
       rhs = side == 0 ? rhs - dc : rhs + dc;
     }
-    if (m.nonorth) {
+    if (m.nonorth && !(fl & FF_BJ)) {
       double gf[3];
       for (int d = 0; d < 3; d++)
         gf[d] = l * F.gu[9 * (size_t)a + comp + 3 * d] + (1.0 - l) * F.gu[9 * (size_t)b + comp + 3 * d];
       const double noc = q.nonorth * m.muf[f] * ddot3(gf, &m.tcor[3 * (size_t)f]);
       rhs = side == 0 ? rhs + noc : rhs - noc;
     }
-    const double pf = l * F.p[a] + (1.0 - l) * F.p[b];
+    const double pf = (fl & FF_KINK) ? kink_pf(m, F.p, F.gp, f, a, b) : l * F.p[a] + (1.0 - l) * F.p[b];
     const double Sc = m.sf[3 * (size_t)f + comp];
     rhs = side == 0 ? rhs - pf * Sc : rhs + pf * Sc;
   }
@@ -333,6 +364,7 @@ __global__ void k_momentum(DMesh m, DFields F, Ramp rp, MomParams q, const doubl
     const double um = sqrt(uP[0] * uP[0] + uP[1] * uP[1] + uP[2] * uP[2]);
     rhs = rhs - q.rho * F.inertial[r] * um * uP[comp] * vol;
   }
+  if (solid_cell(F, r)) ap = ap + 1.0e15 * vol;   // stationary solid (:1282-1283)
   if (q.ts_order == 1) {
     const double at = q.rho * vol / q.dt;
     ap = ap + at;
@@ -355,9 +387,37 @@ __global__ void k_momentum(DMesh m, DFields F, Ramp rp, MomParams q, const doubl
 // ----------------------------------------------------------------- energy
 struct EnParams {
   int ts_order, pimple;
-  double rho, cp, alpha_u, dt, blend, nonorth;
+  double rho, cp, alpha_u, dt, blend, nonorth, k_cond;
 };
 
+// thermal dispersion conductivity (Bear model, :1781-1799): kdisp(3,n)
+__global__ void k_kdisp(DFields F, int n, double rho, double cp) {
+  const int r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= n) return;
+  double kd[3] = {0.0, 0.0, 0.0};
+  const double dl = F.disp_l[r], dt = F.disp_t[r];
+  if (dl > 0.0 || dt > 0.0) {
+    const double *u = &F.u[3 * (size_t)r];
+    const double umag = sqrt(ddot3(u, u));
+    if (!(umag < 1.0e-14)) {
+      const double umag2 = umag * umag;
+      for (int d = 0; d < 3; d++) kd[d] = rho * cp / umag * (dl * (u[d] * u[d]) + dt * (umag2 - u[d] * u[d]));
+    }
+  }
+  for (int d = 0; d < 3; d++) F.kdisp[3 * (size_t)r + d] = kd[d];
+}
+
+// n.D.n of a face with the dispersion tensor of cell c: sum (S_d/|S|)^2 kdisp_d
+__device__ __forceinline__ double disp_n(const DMesh &m, const DFields &F, int f, int c) {
+  double nd[3];
+  for (int d = 0; d < 3; d++) {
+    const double t = m.sf[3 * (size_t)f + d] / m.area[f];
+    nd[d] = t * t;
+  }
+  return ddot3(nd, &F.kdisp[3 * (size_t)c]);
+}
+
+// temperature_assembly (:1748-2037): LTE or the fluid phase of LTNE
 __global__ void k_energy(DMesh m, DFields F, EnParams q, const double *psi, double *val, double *rhs_out,
                          double *ap_out) {
   const int r = blockIdx.x * kBlock + threadIdx.x;
@@ -371,7 +431,22 @@ __global__ void k_energy(DMesh m, DFields F, EnParams q, const double *psi, doub
     const int b = m.c1[f];
     if (b < 0) continue;
     const int a = m.c0[f];
-    const double FT = q.cp * F.flux[f], DT = m.Dt[f], l = m.lf[f];
+    const int fl = face_flags(m, f);
+    if (fl & FF_SOLID2) continue;                  // solid-solid: not in the fluid equation
+    if (fl & FF_CHT) {                             // fluid side of a solid face (:1812-1834)
+      const int o = side == 0 ? b : a;
+      if (!solid_cell(F, r)) {
+        ap = ap + m.kch[f];
+        rhs = rhs + m.kch[f] * F.T_s[o];
+      }
+      continue;
+    }
+    const double FT = q.cp * F.flux[f], l = m.lf[f];
+    double DT = m.Dt[f], kf = m.kf[f];
+    if (m.has_disp) {
+      kf = kf + (l * disp_n(m, F, f, a) + (1.0 - l) * disp_n(m, F, f, b));
+      DT = m.nonorth ? kf * (m.area[f] * m.area[f]) / m.dvs[f] : kf * m.area[f] / m.dn[f];
+    }
     const long long sl = m.cf_slot[k];
     if (side == 0) {
       ap = ap + dmax(FT, 0.0) + DT;
@@ -391,14 +466,21 @@ __global__ void k_energy(DMesh m, DFields F, EnParams q, const double *psi, doub
     if (m.nonorth) {
       double gf[3];
       for (int d = 0; d < 3; d++) gf[d] = l * F.gt[3 * (size_t)a + d] + (1.0 - l) * F.gt[3 * (size_t)b + d];
-      const double noc = q.nonorth * m.kf[f] * ddot3(gf, &m.tcor[3 * (size_t)f]);
+      const double noc = q.nonorth * kf * ddot3(gf, &m.tcor[3 * (size_t)f]);
       rhs = side == 0 ? rhs + noc : rhs - noc;
     }
   }
+  const double por = F.porosity[r];
   for (int k = k0; k < k1; k++) {
     const int f = m.cf_ent[k] >> 1;
     if (m.c1[f] >= 0) continue;
-    const double DT = m.Dtb[f], FT = q.cp * F.flux[f];
+    const double FT = q.cp * F.flux[f];
+    double DT = m.Dtb[f];
+    if (m.has_disp) {
+      double kf = m.ltne ? por * q.k_cond : por * q.k_cond + (1.0 - por) * F.k_s[r];
+      kf = kf + disp_n(m, F, f, r);
+      DT = kf * m.area[f] / m.dw[f];
+    }
     const int tm = m.tmode[f];
     if (tm == 1) {
       const double Tf = m.tval[f];
@@ -406,14 +488,27 @@ __global__ void k_energy(DMesh m, DFields F, EnParams q, const double *psi, doub
       rhs = rhs + DT * Tf - dmin(FT, 0.0) * Tf;
     } else {
       const double qf = tm == 2 ? m.qval[f] : 0.0;
-      if (fabs(qf) > 0.0) rhs = rhs + qf * m.area[f];
+      if (fabs(qf) > 0.0) {
+        if (m.ltne) {   // the fluid's conductivity share of the wall flux (:1918-1924)
+          const double kft = por * q.k_cond, kst = (1.0 - por) * F.k_s[r];
+          rhs = rhs + qf * m.area[f] * kft / (kft + kst);
+        } else {
+          rhs = rhs + qf * m.area[f];
+        }
+      }
       const int bt = m.bc[m.fgrp[f] - 1].btype;
       if (bt == MS_BC_POUTLET || bt == MS_BC_OUTFLOW) ap = ap + dmax(FT, 0.0);
     }
   }
   const double vol = m.vol[r];
-  const double por = F.porosity[r];
-  const double cap = por * q.rho * q.cp + (1.0 - por) * F.rho_s[r] * F.cp_s[r];
+  if (m.ltne) {    // fluid-solid exchange (:1959-1967)
+    const double hasf = F.h_sf[r] * F.a_sf[r];
+    if (hasf > 0.0) {
+      ap = ap + hasf * vol;
+      rhs = rhs + hasf * F.T_s[r] * vol;
+    }
+  }
+  const double cap = m.ltne ? por * q.rho * q.cp : por * q.rho * q.cp + (1.0 - por) * F.rho_s[r] * F.cp_s[r];
   if (q.ts_order == 1) {
     const double at = cap * vol / q.dt;
     ap = ap + at;
@@ -428,6 +523,94 @@ __global__ void k_energy(DMesh m, DFields F, EnParams q, const double *psi, doub
   } else {
     val[m.diag_slot[r]] = ap / q.alpha_u;
     rhs = rhs + (1.0 - q.alpha_u) / q.alpha_u * ap * F.T[r];
+  }
+  if (solid_cell(F, r)) {    // solid cells keep T = T_s (:2024-2033)
+    val[m.diag_slot[r]] = 1.0;
+    rhs = F.T_s[r];
+  }
+  rhs_out[r] = rhs;
+  ap_out[r] = ap;
+}
+
+// solid_temperature_assembly (:2109-2296): the solid phase of LTNE and the
+// conduction in solid zones
+__global__ void k_solid_energy(DMesh m, DFields F, EnParams q, double *val, double *rhs_out, double *ap_out) {
+  const int r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= m.nc) return;
+  const int k0 = m.cf_ptr[r], k1 = m.cf_ptr[r + 1];
+  double ap = 0.0, rhs = 0.0;
+  for (int k = k0; k < k1; k++)
+    if (m.cf_slot[k] >= 0) val[m.cf_slot[k]] = 0.0;
+  for (int k = k0; k < k1; k++) {
+    const int e = m.cf_ent[k], f = e >> 1, side = e & 1;
+    const int b = m.c1[f];
+    if (b < 0) continue;
+    const int a = m.c0[f];
+    if (face_flags(m, f) & FF_CHT) {               // solid side of a fluid face (:2142-2160)
+      const int o = side == 0 ? b : a;
+      if (solid_cell(F, r)) {
+        ap = ap + m.kch[f];
+        rhs = rhs + m.kch[f] * F.T[o];
+      }
+      continue;
+    }
+    const double Ds = m.Ds[f];
+    const long long sl = m.cf_slot[k];
+    ap = ap + Ds;
+    val[sl] = val[sl] - Ds;
+    if (m.nonorth) {
+      const double l = m.lf[f];
+      double gf[3];
+      for (int d = 0; d < 3; d++) gf[d] = l * F.gts[3 * (size_t)a + d] + (1.0 - l) * F.gts[3 * (size_t)b + d];
+      const double noc = q.nonorth * m.ksf[f] * ddot3(gf, &m.tcor[3 * (size_t)f]);
+      rhs = side == 0 ? rhs + noc : rhs - noc;
+    }
+  }
+  const double por = F.porosity[r];
+  for (int k = k0; k < k1; k++) {
+    const int f = m.cf_ent[k] >> 1;
+    if (m.c1[f] >= 0) continue;
+    const int tm = m.tmode_s[f];
+    if (tm == 1) {
+      ap = ap + m.Dsb[f];
+      rhs = rhs + m.Dsb[f] * m.tval[f];
+    } else {
+      const double qf = tm == 2 ? m.qval[f] : 0.0;
+      if (fabs(qf) > 0.0) {
+        const double ksf = (1.0 - por) * F.k_s[r], kft = por * q.k_cond;
+        rhs = rhs + qf * m.area[f] * ksf / (kft + ksf);
+      }
+    }
+  }
+  const double vol = m.vol[r];
+  const double hasf = F.h_sf[r] * F.a_sf[r];
+  if (hasf > 0.0) {
+    ap = ap + hasf * vol;
+    rhs = rhs + hasf * F.T[r] * vol;
+  }
+  if (F.q_src[r] != 0.0) rhs = rhs + F.q_src[r] * vol;
+  if (q.ts_order == 1) {
+    const double at = (1.0 - por) * F.rho_s[r] * F.cp_s[r] * vol / q.dt;
+    ap = ap + at;
+    rhs = rhs + at * F.T_s_old[r];
+  } else if (q.ts_order == 2) {
+    const double at = (1.0 - por) * F.rho_s[r] * F.cp_s[r] * vol / (2.0 * q.dt);
+    ap = ap + 3.0 * at;
+    rhs = rhs + at * (4.0 * F.T_s_old[r] - F.T_s_old_old[r]);
+  }
+  if (q.ts_order > 0 && !q.pimple) {
+    val[m.diag_slot[r]] = ap;
+  } else {
+    val[m.diag_slot[r]] = ap / q.alpha_u;
+    rhs = rhs + (1.0 - q.alpha_u) / q.alpha_u * ap * F.T_s[r];
+  }
+  if (m.has_solid && !solid_cell(F, r)) {   // pure fluid cells keep T_s (:2280-2292)
+    const double *K = &F.perm_dir[3 * (size_t)r];
+    const bool porous = K[0] > 0.0 || K[1] > 0.0 || K[2] > 0.0 || F.inertial[r] > 0.0;
+    if (!porous) {
+      val[m.diag_slot[r]] = 1.0;
+      rhs = F.T_s[r];
+    }
   }
   rhs_out[r] = rhs;
   ap_out[r] = ap;
@@ -579,6 +762,14 @@ __global__ void k_correct_cells(DMesh m, DFields F, const double *pp, double *dp
     dpp_out[3 * (size_t)r + d] = g[d];
   }
   F.p[r] = palpha < 0.0 ? F.p[r] + pp[r] : F.p[r] + palpha * pp[r];
+  if (solid_cell(F, r))   // zero_solid_velocity (:2064-2076)
+    for (int d = 0; d < 3; d++) F.u[3 * (size_t)r + d] = F.u_old[3 * (size_t)r + d] = 0.0;
+}
+
+__global__ void k_zero_solid(DFields F, int n) {
+  const int r = blockIdx.x * kBlock + threadIdx.x;
+  if (r < n && solid_cell(F, r))
+    for (int d = 0; d < 3; d++) F.u[3 * (size_t)r + d] = F.u_old[3 * (size_t)r + d] = 0.0;
 }
 
 __global__ void k_correct_flux(DMesh m, const double *pp, const double *af, double *flux) {
@@ -657,8 +848,11 @@ struct Driver {
   explicit Driver(System &sys) : s(sys) { rp = Ramp{s.inlet_ramp, s.pval_ramp}; }
 
   void gradients() {
+    if (s.has_kink)   // kink faces reconstruct with the gradient of the previous pass
+      MS_HIP(hipMemcpyAsync(s.gp_prev.p, s.gp.p, 3 * (size_t)s.nc * sizeof(double), hipMemcpyDeviceToDevice,
+                            stream()));
     hipLaunchKernelGGL(k_gradients, gc(s.nc), dim3(kBlock), 0, stream(), s.dmesh(), s.dfields(), rp,
-                       (int)s.energy_active);
+                       (int)s.energy_on());
     MS_CHECK_LAUNCH();
   }
   void rhiechow() {
@@ -701,10 +895,31 @@ struct Driver {
     const ms_ctrl_t &c = s.ctrl;
     if (c.conv_blend > 0.0)
       hipLaunchKernelGGL(k_limiter, gc(s.nc), dim3(kBlock), 0, stream(), s.dmesh(), s.dfields(), rp, 3, s.psi.p);
-    EnParams q{s.ts_order, c.pimple, c.rho, c.cp, c.alpha_u, c.dt, c.conv_blend, c.nonorth_corr};
+    EnParams q{s.ts_order, c.pimple, c.rho, c.cp, c.alpha_u, c.dt, c.conv_blend, c.nonorth_corr, c.k_cond};
+    if (s.has_disp)
+      hipLaunchKernelGGL(k_kdisp, gc(s.nc), dim3(kBlock), 0, stream(), s.dfields(), s.nc, c.rho, c.cp);
     hipLaunchKernelGGL(k_energy, gc(s.nc), dim3(kBlock), 0, stream(), s.dmesh(), s.dfields(), q, s.psi.p, s.val.p,
                        s.rhs.p, s.ap.p);
     MS_CHECK_LAUNCH();
+  }
+  void solid_energy() {
+    const ms_ctrl_t &c = s.ctrl;
+    EnParams q{s.ts_order, c.pimple, c.rho, c.cp, c.alpha_u, c.dt, c.conv_blend, c.nonorth_corr, c.k_cond};
+    hipLaunchKernelGGL(k_solid_energy, gc(s.nc), dim3(kBlock), 0, stream(), s.dmesh(), s.dfields(), q, s.val.p,
+                       s.rhs.p, s.ap.p);
+    MS_CHECK_LAUNCH();
+  }
+  void zero_solid() {
+    if (!s.has_solid) return;
+    hipLaunchKernelGGL(k_zero_solid, gc(s.nc), dim3(kBlock), 0, stream(), s.dfields(), s.nc);
+    MS_CHECK_LAUNCH();
+  }
+  // solid-phase temperature solve (BiCGSTAB with the momentum preconditioner)
+  int solid_solve(int &itl_max) {
+    solid_energy();
+    SolveResult R = solve(false, s.T_s.p);
+    itl_max = std::max(itl_max, R.it);
+    return R.ierr;
   }
   void ppe() {
     hipLaunchKernelGGL(k_ppe_af, gc(s.nf), dim3(kBlock), 0, stream(), s.dmesh(), s.apc.p, s.ctrl.rho, s.af.p);
@@ -796,6 +1011,7 @@ int run_simple(System &s, int nsteps, int *it_done) {
     }
     if (D.momentum_solves(itl_max, ressum)) { ier = 20; break; }
     hipLaunchKernelGGL(k_apc, gc(s.nc), dim3(kBlock), 0, stream(), s.nc, s.ap.p, c.alpha_u, s.apc.p);
+    D.zero_solid();
     D.gradients();
     D.rhiechow();
     D.outflow();
@@ -806,11 +1022,13 @@ int run_simple(System &s, int nsteps, int *it_done) {
     itl_max = std::max(itl_max, R.it);
     D.correct(s.pp.p);
     dcopy(s.T_old.p, s.T.p, s.nc);    // Tprev
-    if (s.energy_active) {
+    if (s.energy_on()) {
       D.energy();
       SolveResult RT = D.solve(false, s.T.p);
       if (RT.ierr == 2) { ier = 22; break; }
       itl_max = std::max(itl_max, RT.it);
+      // solid-phase temperature: LTNE or solid zones (mod_uns_simple.f90:214-222)
+      if (s.solid_phase() && D.solid_solve(itl_max) == 2) { ier = 23; break; }
     }
     double dg[5];
     D.diagnostics(s.u_old.p, s.T_old.p, dg);
@@ -870,11 +1088,13 @@ int run_piso(System &s, int nsteps) {
       D.correct(s.pp.p);
     }
     if (bad) { ier = 34; break; }
-    if (s.energy_active) {
+    if (s.energy_on()) {
       D.energy();
       SolveResult RT = D.solve(false, s.T.p);
       if (RT.ierr == 2) { ier = 35; break; }
       itl_max = std::max(itl_max, RT.it);
+      // solid-phase temperature: LTNE (mod_uns_simple_mpi.f90:483-495)
+      if (c.thermal_model != 0 && D.solid_solve(itl_max) == 2) { ier = 36; break; }
     }
     double dg[5];
     D.diagnostics(s.u_old.p, s.T_old.p, dg);
@@ -942,11 +1162,13 @@ int run_pimple(System &s, int nsteps) {
     }
     if (brk) { ier = 53; break; }
     if (bad) { ier = 54; break; }
-    if (s.energy_active) {
+    if (s.energy_on()) {
       D.energy();
       SolveResult RT = D.solve(false, s.T.p);
       if (RT.ierr == 2) { ier = 55; break; }
       itl_max = std::max(itl_max, RT.it);
+      // solid-phase temperature: LTNE (mod_uns_simple_mpi.f90:765-777)
+      if (c.thermal_model != 0 && D.solid_solve(itl_max) == 2) { ier = 56; break; }
     }
     double dg[5];
     D.diagnostics(s.u_old.p, s.T_old.p, dg);
@@ -972,10 +1194,25 @@ void call_energy(System &s) {
   s.refresh_face_constants();
   Driver(s).energy();
 }
-void call_rhiechow(System &s) { Driver(s).rhiechow(); }
+void call_solid_energy(System &s) {
+  if (!s.solid_consts) {   // the per-call routine also works without an active solid phase
+    s.solid_consts = true;
+    s.props_dirty = true;
+  }
+  s.refresh_face_constants();
+  Driver(s).solid_energy();
+}
+void call_rhiechow(System &s) {
+  s.refresh_face_constants();
+  Driver(s).rhiechow();
+}
 void call_outflow(System &s) { Driver(s).outflow(); }
-void call_ppe(System &s) { Driver(s).ppe(); }
+void call_ppe(System &s) {
+  s.refresh_face_constants();
+  Driver(s).ppe();
+}
 void call_correct(System &s, const double *d_pp) {
+  s.refresh_face_constants();
   Driver D(s);
   hipLaunchKernelGGL(k_ppe_af, gc(s.nf), dim3(kBlock), 0, stream(), s.dmesh(), s.apc.p, s.ctrl.rho, s.af.p);
   D.correct(d_pp);

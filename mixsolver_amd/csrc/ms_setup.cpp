@@ -614,7 +614,6 @@ System *create_system(int nnodes, const double *x, int nfaces, const long long *
   s.groups.assign(groups, groups + ctrl->nbc);
   const int nc = ncells, nf = nfaces;
   check_faces(nnodes, nf, fptr, fnodes, c0in, c1orig, nc);
-  if (ctrl->thermal_model != 0) fail(MS_EINVAL, "thermal_model = ltne is not supported on the device path");
   for (auto &g : s.groups)
     if (g.btype == MS_BC_INTERFACE || g.btype < 1 || g.btype > 10)
       fail(MS_EINVAL, "unsupported boundary type (coupling interface)");
@@ -906,6 +905,7 @@ System *create_system(int nnodes, const double *x, int nfaces, const long long *
   s.k_s.assign(nc, 0.0);
   s.rho_s.assign(nc, 0.0);
   s.cp_s.assign(nc, 0.0);
+  for (std::vector<double> *v : {&s.disp_l, &s.disp_t, &s.bj_alpha, &s.h_sf, &s.a_sf, &s.q_src}) v->assign(nc, 0.0);
   s.props_dirty = true;
   s.refresh_face_constants();
   s.kry.reset(new Krylov(&s.pat));
@@ -914,15 +914,42 @@ System *create_system(int nnodes, const double *x, int nfaces, const long long *
 }
 
 // Per-face constants that depend only on geometry, mu, k_cond and the cell
-// properties, evaluated exactly as the reference's face loops do.
+// properties, evaluated exactly as the reference's face loops do
+// (momentum_assembly :974-1060, temperature_assembly :1803-1940,
+// solid_temperature_assembly :2125-2225).  Porous and solid zones add
+// per-face flags: fluid/porous faces (kink face pressure, Beavers-Joseph
+// slip), solid/fluid faces (conjugate heat transfer) and solid/solid faces.
 void System::refresh_face_constants() {
   if (!props_dirty) return;
   const double POROSITY_FLOOR = 1.0e-8;
   const ms_ctrl_t &c = ctrl;
+  const bool ltne = c.thermal_model != 0, unified = c.porous_unified != 0;
+  auto is_porous = [&](int o) {
+    return perm_dir[3 * (size_t)o] > 0.0 || perm_dir[3 * (size_t)o + 1] > 0.0 || perm_dir[3 * (size_t)o + 2] > 0.0 ||
+           inertial[o] > 0.0;
+  };
+  auto ztype = [&](int o) { return cztype.empty() ? (int)CZ_FLUID : cztype[o]; };
+  auto is_solid = [&](int o) { return ztype(o) == CZ_SOLID; };
+  has_solid = false;
+  for (int o = 0; o < nc && !cztype.empty(); o++) has_solid = has_solid || cztype[o] == CZ_SOLID;
+  has_disp = false;
+  has_qsrc = false;
+  for (int o = 0; o < nc; o++) {
+    has_disp = has_disp || disp_l[o] > 0.0 || disp_t[o] > 0.0;
+    has_qsrc = has_qsrc || q_src[o] != 0.0;
+  }
+  const bool solid_phase = ltne || has_solid || solid_consts;
+  // the c1 cell of a joined periodic face sits at xc + shift
+  auto xc1 = [&](int f, int o, int d) {
+    return pidx[f] ? xc_nat[3 * (size_t)o + d] + psh[pidx[f] - 1][d] : xc_nat[3 * (size_t)o + d];
+  };
   std::vector<double> dn(nf, 0.0), dvs(nf, 0.0), Dm(nf, 0.0), muf(nf, 0.0), tcor(3 * (size_t)nf, 0.0),
       dw(nf, 0.0), Dmb(nf, 0.0), Dt(nf, 0.0), kfv(nf, 0.0), Dtb(nf, 0.0), tv(nf, 0.0), qv(nf, 0.0);
-  std::vector<signed char> tm(nf, 0);
-  energy_active = T_nonzero;
+  std::vector<signed char> tm(nf, 0), tms(nf, 0), ff(nf, 0);
+  std::vector<double> Cbj(nf, 0.0), kch(nf, 0.0), Ds(nf, 0.0), ksfv(nf, 0.0), Dsb(nf, 0.0);
+  bc_energy = false;
+  has_kink = false;
+  bool any_flag = false;
   for (int f = 0; f < nf; f++) {
     const int a = n2o[c0[f]];
     const double *sfp = &sf[3 * (size_t)f];
@@ -930,22 +957,25 @@ void System::refresh_face_constants() {
     if (c1[f] >= 0) {
       const int b = n2o[c1[f]];
       double dvec[3];
-      for (int d = 0; d < 3; d++) dvec[d] = xc_nat[3 * (size_t)b + d] - xc_nat[3 * (size_t)a + d];
-      if (pidx[f])
-        for (int d = 0; d < 3; d++)
-          dvec[d] = (xc_nat[3 * (size_t)b + d] + psh[pidx[f] - 1][d]) - xc_nat[3 * (size_t)a + d];
+      for (int d = 0; d < 3; d++) dvec[d] = xc1(f, b, d) - xc_nat[3 * (size_t)a + d];
       dn[f] = norm2_ref(dvec, 3);
       dvs[f] = dot3(dvec, sfp);
       const double sd = dot3(sfp, dvec);
       const double mu0 = c.mu / std::max(porosity[a], POROSITY_FLOOR);
       const double mu1 = c.mu / std::max(porosity[b], POROSITY_FLOOR);
       const double l = lf[f];
-      muf[f] = l * mu0 + (1.0 - l) * mu1;
-      const double k0 = porosity[a] * c.k_cond + (1.0 - porosity[a]) * k_s[a];
-      const double k1 = porosity[b] * c.k_cond + (1.0 - porosity[b]) * k_s[b];
+      muf[f] = unified ? 1.0 / ((1.0 - l) / mu0 + l / mu1) : l * mu0 + (1.0 - l) * mu1;
+      double k0, k1;
+      if (ltne) {
+        k0 = porosity[a] * c.k_cond;
+        k1 = porosity[b] * c.k_cond;
+      } else {
+        k0 = porosity[a] * c.k_cond + (1.0 - porosity[a]) * k_s[a];
+        k1 = porosity[b] * c.k_cond + (1.0 - porosity[b]) * k_s[b];
+      }
       double kf = l * k0 + (1.0 - l) * k1;
-      kf = kf + (l * 0.0 + (1.0 - l) * 0.0);
       kfv[f] = kf;
+      kf = kf + (l * 0.0 + (1.0 - l) * 0.0);   // + dispersion, 0 without dispersivities
       if (c.nonorth_corr > 0.0) {
         Dm[f] = muf[f] * (area[f] * area[f]) / sd;
         Dt[f] = kf * (area[f] * area[f]) / sd;
@@ -954,12 +984,59 @@ void System::refresh_face_constants() {
         Dm[f] = muf[f] * area[f] / dn[f];
         Dt[f] = kf * area[f] / dn[f];
       }
+      // kink face pressure at fluid/porous faces (:1111-1113, fields :191-193)
+      if (!unified && is_porous(a) != is_porous(b)) {
+        ff[f] |= FF_KINK;
+        has_kink = true;
+      }
+      // Beavers-Joseph slip (:1031-1059): static coefficient C per face
+      if (!unified && ztype(a) != ztype(b)) {
+        const int cp = ztype(a) == CZ_POROUS ? a : b, cf = cp == a ? b : a;
+        const double bja = bj_alpha[cp];
+        if (bja > 0.0) {
+          double nh[3];
+          for (int d = 0; d < 3; d++) nh[d] = sfp[d] / area[f];
+          double Knn = 0.0;
+          for (int d = 0; d < 3; d++) Knn = Knn + perm_dir[3 * (size_t)cp + d] * (nh[d] * nh[d]);
+          if (Knn > 0.0) {
+            double r[3];
+            for (int d = 0; d < 3; d++) r[d] = xf[3 * (size_t)f + d] - (cf == b ? xc1(f, b, d) : xc_nat[3 * (size_t)a + d]);
+            const double dPf = norm2_ref(r, 3);
+            Cbj[f] = c.mu * area[f] * (bja / std::sqrt(Knn)) / (1.0 + bja * dPf / std::sqrt(Knn));
+            ff[f] |= FF_BJ;
+          }
+        }
+      }
+      // solid zones: conjugate heat transfer faces and solid/solid faces
+      const bool s0 = is_solid(a), s1 = is_solid(b);
+      if (s0 || s1) {
+        if (s0 && s1) {
+          ff[f] |= FF_SOLID2;
+        } else {
+          const int cs = s0 ? a : b, cfl = s0 ? b : a;
+          double rf[3], rs[3];
+          for (int d = 0; d < 3; d++) {
+            rf[d] = xf[3 * (size_t)f + d] - (cfl == b ? xc1(f, b, d) : xc_nat[3 * (size_t)a + d]);
+            rs[d] = xf[3 * (size_t)f + d] - (cs == b ? xc1(f, b, d) : xc_nat[3 * (size_t)a + d]);
+          }
+          const double k1c = porosity[cfl] * c.k_cond, k2c = k_s[cs];
+          kch[f] = area[f] / (norm2_ref(rf, 3) / std::max(k1c, 1.0e-30) + norm2_ref(rs, 3) / std::max(k2c, 1.0e-30));
+          ff[f] |= FF_CHT;
+        }
+      }
+      // solid-phase diffusion (:2161-2175)
+      if (solid_phase) {
+        const double ks0 = (1.0 - porosity[a]) * k_s[a], ks1 = (1.0 - porosity[b]) * k_s[b];
+        ksfv[f] = l * ks0 + (1.0 - l) * ks1;
+        Ds[f] = c.nonorth_corr > 0.0 ? ksfv[f] * (area[f] * area[f]) / sd : ksfv[f] * area[f] / dn[f];
+      }
+      if (ff[f]) any_flag = true;
     } else {
       double r[3];
       for (int d = 0; d < 3; d++) r[d] = xf[3 * (size_t)f + d] - xc_nat[3 * (size_t)a + d];
       dw[f] = norm2_ref(r, 3);
       Dmb[f] = (c.mu / std::max(porosity[a], POROSITY_FLOOR)) * area[f] / dw[f];
-      double kf = porosity[a] * c.k_cond + (1.0 - porosity[a]) * k_s[a];
+      double kf = ltne ? porosity[a] * c.k_cond : porosity[a] * c.k_cond + (1.0 - porosity[a]) * k_s[a];
       kf = kf + 0.0;
       Dtb[f] = kf * area[f] / dw[f];
       // bc_face_T (mod_uns_bc.f90:560-703): static per face
@@ -985,9 +1062,14 @@ void System::refresh_face_constants() {
       tm[f] = (signed char)tt;
       tv[f] = tt == 1 ? tvv : 0.0;
       qv[f] = tt == 2 ? qvv : 0.0;
-      if ((tt == 1 && tvv != 0.0) || (tt == 2 && qvv != 0.0)) energy_active = true;
+      if ((tt == 1 && tvv != 0.0) || (tt == 2 && qvv != 0.0)) bc_energy = true;
+      // the solid phase sees inlets as adiabatic (:2187-2193)
+      const bool inlet = g.btype == MS_BC_VINLET || g.btype == MS_BC_VINLET_PARAB || g.btype == MS_BC_MASSINLET;
+      tms[f] = inlet ? 0 : (signed char)tt;
+      if (solid_phase) Dsb[f] = ((1.0 - porosity[a]) * k_s[a]) * area[f] / dw[f];
     }
   }
+  has_face_flags = any_flag;
   d_dn.upload(dn);
   d_dvs.upload(dvs);
   d_Dm.upload(Dm);
@@ -1001,6 +1083,37 @@ void System::refresh_face_constants() {
   d_tmode.upload(tm);
   d_tval.upload(tv);
   d_qval.upload(qv);
+  if (any_flag) {
+    d_fflag.upload(ff);
+    d_Cbj.upload(Cbj);
+    d_kch.upload(kch);
+  }
+  // per-cell arrays the thermal kernels read, in device order
+  auto dev_upload = [&](const std::vector<double> &h, DBuf<double> &d) {
+    std::vector<double> v(nc);
+    for (int r = 0; r < nc; r++) v[r] = h[n2o[r]];
+    d.upload(v);
+  };
+  if (solid_phase || has_disp) {
+    dev_upload(h_sf, h_sf_d);
+    dev_upload(a_sf, a_sf_d);
+    dev_upload(q_src, q_src_d);
+    dev_upload(disp_l, disp_l_d);
+    dev_upload(disp_t, disp_t_d);
+  }
+  if (has_disp) kdisp.alloc(3 * (size_t)nc);
+  if (has_kink) gp_prev.alloc(3 * (size_t)nc);
+  if (!cztype.empty()) {
+    std::vector<signed char> t(nc);
+    for (int r = 0; r < nc; r++) t[r] = (signed char)cztype[n2o[r]];
+    d_cztype.upload(t);
+  }
+  if (solid_phase) {
+    d_Ds.upload(Ds);
+    d_ksf.upload(ksfv);
+    d_Dsb.upload(Dsb);
+    d_tmode_s.upload(tms);
+  }
   props_dirty = false;
 }
 
@@ -1042,6 +1155,18 @@ DMesh System::dmesh() const {
   m.bc = d_bc.p;
   m.nper = nper;
   m.pidx = d_pidx.p;
+  m.has_face_flags = has_face_flags;
+  m.ltne = ctrl.thermal_model != 0;
+  m.has_solid = has_solid;
+  m.has_disp = has_disp;
+  m.solid_phase = solid_phase();
+  m.fflag = d_fflag.p;
+  m.Cbj = d_Cbj.p;
+  m.kch = d_kch.p;
+  m.Ds = d_Ds.p;
+  m.ksf = d_ksf.p;
+  m.Dsb = d_Dsb.p;
+  m.tmode_s = d_tmode_s.p;
   for (int k = 0; k < 4; k++)
     for (int d = 0; d < 3; d++) m.psh[k][d] = psh[k][d];
   return m;
@@ -1072,6 +1197,14 @@ DFields System::dfields() {
   F.k_s = k_s_d.p;
   F.rho_s = rho_s_d.p;
   F.cp_s = cp_s_d.p;
+  F.disp_l = disp_l_d.p;
+  F.disp_t = disp_t_d.p;
+  F.h_sf = h_sf_d.p;
+  F.a_sf = a_sf_d.p;
+  F.q_src = q_src_d.p;
+  F.cztype = has_solid ? d_cztype.p : nullptr;
+  F.kdisp = kdisp.p;
+  F.gp_prev = gp_prev.p;
   return F;
 }
 
@@ -1081,8 +1214,10 @@ size_t System::device_bytes() const {
                                 &d_dw, &d_Dmb, &d_Dt, &d_kf, &d_Dtb, &d_tval, &d_qval, &u, &p, &T, &T_s, &flux, &apc,
                                 &u_old, &u_old_old, &T_old, &T_old_old, &T_s_old, &T_s_old_old, &gp, &gu, &gt, &gts,
                                 &psi, &porosity_d, &perm_dir_d, &inertial_d, &k_s_d, &rho_s_d, &cp_s_d, &val, &rhs,
-                                &ap, &pp, &dpp, &af, &ucomp, &part, &diag8, &tmpc})
+                                &ap, &pp, &dpp, &af, &ucomp, &part, &diag8, &tmpc, &disp_l_d, &disp_t_d, &h_sf_d,
+                                &a_sf_d, &q_src_d, &kdisp, &gp_prev, &d_Cbj, &d_kch, &d_Ds, &d_ksf, &d_Dsb})
     b += v->bytes();
+  b += d_fflag.bytes() + d_tmode_s.bytes() + d_cztype.bytes();
   b += d_c0.bytes() + d_c1.bytes() + d_fgrp.bytes() + d_pidx.bytes() + d_cf_ptr.bytes() + d_cf_ent.bytes() + d_cf_slot.bytes() +
        d_diag_slot.bytes();
   return b;

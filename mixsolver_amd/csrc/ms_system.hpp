@@ -31,12 +31,26 @@ struct DMesh {
   int nper;
   const signed char *pidx;
   double psh[4][3];
+  // porous / solid zones (null unless the system has them): per-face flags
+  // FF_KINK | FF_BJ | FF_CHT | FF_SOLID2, Beavers-Joseph coefficient, CHT
+  // conductance, solid-phase face constants
+  int has_face_flags, ltne, has_solid, has_disp, solid_phase;
+  const signed char *fflag;
+  const double *Cbj, *kch, *Ds, *ksf, *Dsb;
+  const signed char *tmode_s;             // solid-phase boundary thermal mode
 };
+
+enum { FF_KINK = 1, FF_BJ = 2, FF_CHT = 4, FF_SOLID2 = 8 };
+enum { CZ_FLUID = 1, CZ_POROUS = 2, CZ_SOLID = 3 };
 
 struct DFields {
   double *u, *p, *T, *T_s, *flux, *apc, *u_old, *u_old_old, *T_old, *T_old_old, *T_s_old, *T_s_old_old;
   double *gp, *gu, *gt, *gts, *psi;
   const double *porosity, *perm_dir, *inertial, *k_s, *rho_s, *cp_s;
+  const double *disp_l, *disp_t, *h_sf, *a_sf, *q_src;
+  const signed char *cztype;               // block type per cell (null: all fluid)
+  double *kdisp;                           // (3,nc) dispersion conductivity work array
+  const double *gp_prev;                   // gp before the current gradient pass (kink faces)
 };
 
 struct System {
@@ -66,8 +80,13 @@ struct System {
   std::unique_ptr<Krylov> kry;
   // per-cell properties (natural order, host copy)
   std::vector<double> porosity, perm_dir, inertial, k_s, rho_s, cp_s;
+  std::vector<double> disp_l, disp_t, bj_alpha, h_sf, a_sf, q_src;
+  std::vector<int> cztype;                 // natural order, CZ_* (empty: all fluid)
   bool props_dirty = true;
-  bool energy_active = false;
+  // porous / solid flags derived from the properties (refresh_face_constants)
+  bool has_face_flags = false, has_solid = false, has_disp = false, has_kink = false;
+  bool solid_consts = false;   // keep solid-phase face constants (per-call assembly)
+  bool bc_energy = false, has_qsrc = false;   // boundary / solid sources make T non-trivial
   bool T_nonzero = false;      // caller seeded a non-zero temperature field
   int ts_order = 0;
   double inlet_ramp = 1.0, pval_ramp = 1.0;
@@ -77,18 +96,24 @@ struct System {
   DBuf<long long> d_cf_slot, d_diag_slot;
   DBuf<double> d_xf, d_sf, d_area, d_lf, d_vol, d_xc;
   DBuf<double> d_dn, d_dvs, d_Dm, d_muf, d_tcor, d_dw, d_Dmb, d_Dt, d_kf, d_Dtb, d_tval, d_qval;
-  DBuf<signed char> d_tmode, d_pidx;
+  DBuf<signed char> d_tmode, d_pidx, d_fflag, d_tmode_s, d_cztype;
+  DBuf<double> d_Cbj, d_kch, d_Ds, d_ksf, d_Dsb;
   DBuf<DevBC> d_bc;
   // device fields (device numbering)
   DBuf<double> u, p, T, T_s, flux, apc, u_old, u_old_old, T_old, T_old_old, T_s_old, T_s_old_old;
   DBuf<double> gp, gu, gt, gts, psi;
   DBuf<double> porosity_d, perm_dir_d, inertial_d, k_s_d, rho_s_d, cp_s_d;
+  DBuf<double> disp_l_d, disp_t_d, h_sf_d, a_sf_d, q_src_d, kdisp, gp_prev;
   // work
   DBuf<double> val, rhs, ap, pp, dpp, af, ucomp, part, diag8, tmpc;
   DBuf<unsigned int> ticket;
   std::vector<double> history;  // 8 per row
   int pin_cell = 0, has_pdir = 0, has_outflow = 0;
 
+  // the energy equations can have a non-zero solution (otherwise their
+  // right-hand sides are zero and the solves return at once)
+  bool energy_on() const { return T_nonzero || bc_energy || has_qsrc; }
+  bool solid_phase() const { return ctrl.thermal_model != 0 || has_solid; }
   DMesh dmesh() const;
   DFields dfields();
   void refresh_face_constants();

@@ -43,6 +43,7 @@ module mixsolver_c
       integer(c_int) :: periodic_zone(2, 4)
       integer(c_int) :: periodic_auto(4)
       real(c_double) :: periodic_shift(3, 4)
+      integer(c_int) :: porous_unified
    end type ms_ctrl_t
 
    type, bind(c) :: ms_bcgroup_t

@@ -38,6 +38,8 @@ class Mesh:
     ctype: int = HEX
     zones: dict = field(default_factory=dict)   # id -> (kind, name)
     cell_zone: int = 2
+    czone: np.ndarray | None = None    # (ncells,) cell zone id (None: all cell_zone)
+    czone_names: dict = field(default_factory=dict)   # cell zone id -> (condition, name)
 
     @property
     def nnodes(self) -> int:
@@ -53,6 +55,31 @@ class Mesh:
 
     def face_nn(self) -> np.ndarray:
         return np.diff(self.fptr).astype(np.int32)
+
+
+    def cell_zones(self) -> np.ndarray:
+        if self.czone is None:
+            return np.full(self.ncells, self.cell_zone, np.int32)
+        return np.asarray(self.czone, np.int32)
+
+
+def with_cell_zones(m: Mesh, czone, names=None) -> Mesh:
+    """Copy of m with cell zones: cells are renumbered so that every zone is
+    one contiguous id range (zones in ascending id order, the previous order
+    inside each), as the cell records of a .cas file require.  czone gives
+    the zone id of every cell of m; names maps zone ids to (condition, name).
+    The copy's ``cell_perm`` holds the old 0-based id of every new cell."""
+    czone = np.asarray(czone, np.int32)
+    order = np.argsort(czone, kind="stable")          # new -> old
+    o2n = np.empty_like(order)
+    o2n[order] = np.arange(len(order))
+    c0 = (o2n[m.c0 - 1] + 1).astype(np.int32)
+    c1 = np.where(m.c1 > 0, o2n[np.maximum(m.c1, 1) - 1] + 1, 0).astype(np.int32)
+    out = Mesh(x=m.x, fptr=m.fptr, fnodes=m.fnodes, c0=c0, c1=c1, zone=m.zone, ncells=m.ncells,
+               ctype=m.ctype, zones=dict(m.zones), cell_zone=int(czone[order][0]), czone=czone[order],
+               czone_names=dict(names or {}))
+    out.cell_perm = order
+    return out
 
 
 # ---------------------------------------------------------------------------

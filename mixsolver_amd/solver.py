@@ -56,6 +56,15 @@ class Case:
         self.info = dict(ncells=int(info[0]), nfaces=int(info[1]), nnz=int(info[2]), ncolors=int(info[3]),
                          nlevels=int(info[4]), device_bytes=int(info[5]), nstore=int(info[6]),
                          renumbered=bool(info[7]))
+        from .zones import apply_zones
+        apply_zones(self, mesh, ctrl)
+
+    def set_cell_types(self, cztype):
+        """Block type of every cell (1 fluid, 2 porous, 3 solid), natural order."""
+        t = None if cztype is None else np.ascontiguousarray(cztype, np.int32)
+        if t is not None and t.size != self.nc:
+            raise ValueError(f"cztype: expected {self.nc} values, got {t.size}")
+        check(_lib.lib().ms_set_cell_types(self.h, None if t is None else ip(t)))
 
     @classmethod
     def from_control(cls, mesh, control_text, device=0, **overrides):
@@ -147,6 +156,11 @@ class Case:
     def momentum_assembly(self, comp):
         val, rhs, ap = np.zeros(self.nnz), np.zeros(self.nc), np.zeros(self.nc)
         check(_lib.lib().ms_momentum_assembly(self.h, int(comp), dp(val), dp(rhs), dp(ap)))
+        return val, rhs, ap
+
+    def solid_temperature_assembly(self):
+        val, rhs, ap = np.zeros(self.nnz), np.zeros(self.nc), np.zeros(self.nc)
+        check(_lib.lib().ms_solid_temperature_assembly(self.h, dp(val), dp(rhs), dp(ap)))
         return val, rhs, ap
 
     def temperature_assembly(self):

@@ -23,9 +23,19 @@
  *   solvers       src/unstructured/mod_uns_linsolver.f90, mod_uns_linsolver_mpi.f90
  *   drivers       src/unstructured/mod_uns_simple.f90:88-288 (SIMPLE),
  *                 src/unstructured/mod_uns_simple_mpi.f90:298-571 (PISO, np = 1)
+ *   porous / solid zones: src/unstructured/mod_uns_fields.f90:267-349
+ *                 (is_porous_cell, is_solid_cell, kink_face_pressure),
+ *                 mod_uns_simple.f90:986-1113 (Brinkman / unified face
+ *                 viscosity, Beavers-Joseph slip, kink pressure force),
+ *                 :1252-1284 (Darcy-Forchheimer, solid source), :1781-2033
+ *                 (dispersion, conjugate heat transfer, LTNE fluid phase,
+ *                 solid pin), :2046-2296 (solid_phase_active,
+ *                 zero_solid_velocity, ch_face_U, solid_temperature_assembly)
+ * The block type of every cell (cztype) and the per-cell coefficients come
+ * from the caller (resolve_cell_zones / setup_porous_fields are restated on
+ * the host in mixsolver_amd/zones.py).
  * Not restated (the constructor rejects them with ier = 50): coupling
- * interface faces, fluid/porous interface faces (kink pressure, Beavers-Joseph
- * slip), solid cell zones, thermal dispersion and the LTNE solid equation.
+ * interface faces and periodic pairs (the reference has none).
  */
 #include "oracle.h"
 
@@ -51,6 +61,8 @@ struct or_case {
   double *u, *p, *gp, *gu, *flux, *apc, *u_old, *u_old_old;
   double *T, *T_old, *T_old_old, *gt, *T_s, *T_s_old, *T_s_old_old, *gts;
   double *porosity, *perm_dir, *inertial, *k_s, *rho_s, *cp_s;
+  double *disp_l, *disp_t, *bj_alpha, *h_sf, *a_sf, *q_src;
+  int *cztype;                   /* m%cztype: 1 fluid, 2 porous, 3 solid */
   int ts_order;
   double *hist;
   int nhist, caphist;
@@ -439,7 +451,6 @@ or_case *or_create(int nn, const double *x, int nf, const long long *fptr,
   }
   for (int f = 0; f < nf; f++)
     if (cs->c1[f] == 0 && cs->fgrp[f] == 0) *ier = 12;
-  if (ctrl->thermal_model != 0) *ier = 50;
   if (ctrl->nperiodic != 0) *ier = 50;   /* the reference has no periodic boundary */
   /* init_fields, mod_uns_fields.f90:86-159 */
   size_t n3 = 3 * (size_t)nc;
@@ -451,7 +462,10 @@ or_case *or_create(int nn, const double *x, int nf, const long long *fptr,
   cs->T_s_old_old = xcalloc(nc, 8); cs->gts = xcalloc(n3, 8);
   cs->porosity = xcalloc(nc, 8); cs->perm_dir = xcalloc(n3, 8); cs->inertial = xcalloc(nc, 8);
   cs->k_s = xcalloc(nc, 8); cs->rho_s = xcalloc(nc, 8); cs->cp_s = xcalloc(nc, 8);
-  for (int c = 0; c < nc; c++) { cs->apc[c] = 1.0; cs->porosity[c] = 1.0; }
+  cs->disp_l = xcalloc(nc, 8); cs->disp_t = xcalloc(nc, 8); cs->bj_alpha = xcalloc(nc, 8);
+  cs->h_sf = xcalloc(nc, 8); cs->a_sf = xcalloc(nc, 8); cs->q_src = xcalloc(nc, 8);
+  cs->cztype = xcalloc(nc, sizeof(int));
+  for (int c = 0; c < nc; c++) { cs->apc[c] = 1.0; cs->porosity[c] = 1.0; cs->cztype[c] = 1; }
   for (int i = 0; i < nf; i++) {
     if (cs->c1[i] == 0) { cs->lf[i] = 1.0; continue; }
     double r0[3], r1[3];
@@ -475,7 +489,8 @@ void or_destroy(or_case *cs) {
                   cs->col_idx, cs->gb, cs->u, cs->p, cs->gp, cs->gu, cs->flux, cs->apc,
                   cs->u_old, cs->u_old_old, cs->T, cs->T_old, cs->T_old_old, cs->gt, cs->T_s,
                   cs->T_s_old, cs->T_s_old_old, cs->gts, cs->porosity, cs->perm_dir,
-                  cs->inertial, cs->k_s, cs->rho_s, cs->cp_s, cs->hist};
+                  cs->inertial, cs->k_s, cs->rho_s, cs->cp_s, cs->hist, cs->disp_l, cs->disp_t,
+                  cs->bj_alpha, cs->h_sf, cs->a_sf, cs->q_src, cs->cztype};
   for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
   free(cs);
 }
@@ -502,10 +517,54 @@ double *or_field(or_case *cs, const char *nm) {
     {"T_old", cs->T_old}, {"T_old_old", cs->T_old_old}, {"gt", cs->gt}, {"T_s", cs->T_s},
     {"T_s_old", cs->T_s_old}, {"T_s_old_old", cs->T_s_old_old}, {"gts", cs->gts},
     {"porosity", cs->porosity}, {"perm_dir", cs->perm_dir}, {"inertial", cs->inertial},
-    {"k_s", cs->k_s}, {"rho_s", cs->rho_s}, {"cp_s", cs->cp_s}};
+    {"k_s", cs->k_s}, {"rho_s", cs->rho_s}, {"cp_s", cs->cp_s}, {"disp_l", cs->disp_l},
+    {"disp_t", cs->disp_t}, {"bj_alpha", cs->bj_alpha}, {"h_sf", cs->h_sf}, {"a_sf", cs->a_sf},
+    {"q_src", cs->q_src}};
   for (size_t i = 0; i < sizeof(t) / sizeof(t[0]); i++)
     if (strcmp(t[i].n, nm) == 0) return t[i].p;
   return NULL;
+}
+
+void or_set_cztype(or_case *cs, const int *t) {
+  for (int c = 0; c < cs->nc; c++) cs->cztype[c] = t ? t[c] : 1;
+}
+
+/* is_porous_cell / is_solid_cell, mod_uns_fields.f90:267-291 (c 1-based) */
+static int is_porous(const or_case *cs, int c) {
+  const double *K = &cs->perm_dir[3 * (c - 1)];
+  return K[0] > 0.0 || K[1] > 0.0 || K[2] > 0.0 || cs->inertial[c - 1] > 0.0;
+}
+static int is_solid(const or_case *cs, int c) { return cs->cztype[c - 1] == 3; }
+static int any_solid(const or_case *cs) {
+  for (int c = 0; c < cs->nc; c++) if (cs->cztype[c] == 3) return 1;
+  return 0;
+}
+/* solid_phase_active, mod_uns_simple.f90:2046-2056 */
+static int solid_phase_active(const or_case *cs) { return cs->ctrl.thermal_model != 0 || any_solid(cs); }
+/* kink_face_pressure, mod_uns_fields.f90:328-349 (face i 0-based) */
+static double kink_face_pressure(const or_case *cs, int i) {
+  int c0 = cs->c0[i], c1 = cs->c1[i];
+  double dvec[3], nhat[3];
+  for (int d = 0; d < 3; d++) dvec[d] = XC(c1, d) - XC(c0, d);
+  double dn = norm3(dvec);
+  for (int d = 0; d < 3; d++) nhat[d] = dvec[d] / dn;
+  double d0 = (1.0 - cs->lf[i]) * dn, d1 = -cs->lf[i] * dn;
+  return 0.5 * (cs->p[c0 - 1] + d0 * dot3(&cs->gp[3 * (c0 - 1)], nhat) + cs->p[c1 - 1] +
+                d1 * dot3(&cs->gp[3 * (c1 - 1)], nhat));
+}
+static int kink_face(const or_case *cs, int i) {
+  return !cs->ctrl.porous_unified && (is_porous(cs, cs->c0[i]) != is_porous(cs, cs->c1[i]));
+}
+/* zero_solid_velocity, mod_uns_simple.f90:2064-2076 */
+static void zero_solid_velocity(or_case *cs) {
+  for (int c = 1; c <= cs->nc; c++)
+    if (is_solid(cs, c)) for (int d = 0; d < 3; d++) cs->u[3 * (c - 1) + d] = 0.0;
+  for (int c = 1; c <= cs->nc; c++)
+    if (is_solid(cs, c)) for (int d = 0; d < 3; d++) cs->u_old[3 * (c - 1) + d] = 0.0;
+}
+/* ch_face_U, mod_uns_simple.f90:2093-2097 */
+static double ch_face_U(double A, double d1, double d2, double k1, double k2) {
+  return A / (d1 / fmax0(k1, 1.0e-30) + d2 / fmax0(k2, 1.0e-30));
 }
 
 /* ------------------------------------------------------------ gradients */
@@ -530,8 +589,12 @@ void or_compute_gradients(or_case *cs) {
   double *phif = xcalloc(nf, sizeof(double));
   for (int i = 0; i < nf; i++) {
     int c0 = cs->c0[i], c1 = cs->c1[i];
-    if (c1 > 0) phif[i] = cs->lf[i] * cs->p[c0 - 1] + (1.0 - cs->lf[i]) * cs->p[c1 - 1];
-    else phif[i] = bc_face_p(cs, i, cs->p[c0 - 1]);
+    if (c1 > 0) {
+      if (kink_face(cs, i)) phif[i] = kink_face_pressure(cs, i);
+      else phif[i] = cs->lf[i] * cs->p[c0 - 1] + (1.0 - cs->lf[i]) * cs->p[c1 - 1];
+    } else {
+      phif[i] = bc_face_p(cs, i, cs->p[c0 - 1]);
+    }
   }
   grad_scalar(cs, phif, cs->gp, 3, 1);
   for (int comp = 0; comp < 3; comp++) {
@@ -636,7 +699,7 @@ void or_momentum_assembly(or_case *cs, int comp, double *val, double *rhs, doubl
     double dn = norm3(dvec);
     double mu0 = ct->mu / fmax0(cs->porosity[c0 - 1], POROSITY_FLOOR);
     double mu1 = ct->mu / fmax0(cs->porosity[c1 - 1], POROSITY_FLOOR);
-    double mu_f = lf * mu0 + (1.0 - lf) * mu1;
+    double mu_f = ct->porous_unified ? 1.0 / ((1.0 - lf) / mu0 + lf / mu1) : lf * mu0 + (1.0 - lf) * mu1;
     double D;
     if (ct->nonorth_corr > 0.0) {
       sd = dot3(&cs->sf[3 * i], dvec);
@@ -645,6 +708,30 @@ void or_momentum_assembly(or_case *cs, int comp, double *val, double *rhs, doubl
       D = mu_f * cs->area[i] / dn;
     }
     double D_face = D;
+    int is_bj = 0;
+    if (!ct->porous_unified && cs->cztype[c0 - 1] != cs->cztype[c1 - 1]) {   /* :1031-1059 */
+      int cp, cf;
+      if (cs->cztype[c0 - 1] == 2) { cp = c0; cf = c1; } else { cp = c1; cf = c0; }
+      double bja = cs->bj_alpha[cp - 1];
+      if (bja > 0.0) {
+        double nh[3], nh2[3];
+        for (int d = 0; d < 3; d++) { nh[d] = SF(i, d) / cs->area[i]; nh2[d] = nh[d] * nh[d]; }
+        double Knn = dot3(&cs->perm_dir[3 * (cp - 1)], nh2);
+        if (Knn > 0.0) {
+          double r[3];
+          for (int d = 0; d < 3; d++) r[d] = XF(i, d) - XC(cf, d);
+          double dPf = norm3(r);
+          double Cbj = ct->mu * cs->area[i] * (bja / sqrt(Knn)) / (1.0 + bja * dPf / sqrt(Knn));
+          D_face = Cbj + (D - Cbj) * (nh[comp] * nh[comp]);
+          double rhs_x = (D - Cbj) * nh[comp] *
+                         ((U(c0, 0) - U(c1, 0)) * nh[0] + (U(c0, 1) - U(c1, 1)) * nh[1] +
+                          (U(c0, 2) - U(c1, 2)) * nh[2] - (U(c0, comp) - U(c1, comp)) * nh[comp]);
+          rhs[c0 - 1] = rhs[c0 - 1] + rhs_x;
+          rhs[c1 - 1] = rhs[c1 - 1] - rhs_x;
+          is_bj = 1;
+        }
+      }
+    }
     ap[c0 - 1] = ap[c0 - 1] + fmax0(F, 0.0) + D_face;
     ap[c1 - 1] = ap[c1 - 1] + fmax0(-F, 0.0) + D_face;
     int kd = pos_of(cs, c0, c1);
@@ -660,7 +747,7 @@ void or_momentum_assembly(or_case *cs, int comp, double *val, double *rhs, doubl
       rhs[c0 - 1] = rhs[c0 - 1] - dc;
       rhs[c1 - 1] = rhs[c1 - 1] + dc;
     }
-    if (ct->nonorth_corr > 0.0) {
+    if (ct->nonorth_corr > 0.0 && !is_bj) {
       double gf[3], t[3];
       for (int d = 0; d < 3; d++) {
         gf[d] = lf * GU(comp, d, c0) + (1.0 - lf) * GU(comp, d, c1);
@@ -670,7 +757,7 @@ void or_momentum_assembly(or_case *cs, int comp, double *val, double *rhs, doubl
       rhs[c0 - 1] = rhs[c0 - 1] + noc;
       rhs[c1 - 1] = rhs[c1 - 1] - noc;
     }
-    double pf = lf * cs->p[c0 - 1] + (1.0 - lf) * cs->p[c1 - 1];
+    double pf = kink_face(cs, i) ? kink_face_pressure(cs, i) : lf * cs->p[c0 - 1] + (1.0 - lf) * cs->p[c1 - 1];
     double Sc = SF(i, comp);
     rhs[c0 - 1] = rhs[c0 - 1] - pf * Sc;
     rhs[c1 - 1] = rhs[c1 - 1] + pf * Sc;
@@ -721,6 +808,7 @@ void or_momentum_assembly(or_case *cs, int comp, double *val, double *rhs, doubl
       double um = sqrt(U(c, 0) * U(c, 0) + U(c, 1) * U(c, 1) + U(c, 2) * U(c, 2));
       rhs[c - 1] = rhs[c - 1] - ct->rho * cs->inertial[c - 1] * um * U(c, comp) * cs->vol[c - 1];
     }
+    if (is_solid(cs, c)) ap[c - 1] = ap[c - 1] + 1.0e15 * cs->vol[c - 1];
   }
   if (cs->ts_order == 1) {
     for (int c = 0; c < nc; c++) {
@@ -747,11 +835,14 @@ void or_momentum_assembly(or_case *cs, int comp, double *val, double *rhs, doubl
   free(psi);
 }
 
-/* ------------------------------------------------------- energy (LTE) */
+/* ------------------------------------------------------------ energy */
 
+/* temperature_assembly, mod_uns_simple.f90:1748-2037 (LTE, or the fluid
+ * phase of LTNE) */
 void or_temperature_assembly(or_case *cs, double *val, double *rhs, double *ap) {
   const or_ctrl_t *ct = &cs->ctrl;
   int nc = cs->nc, nf = cs->nf;
+  const int ltne = ct->thermal_model != 0;
   for (int k = 0; k < cs->nnz; k++) val[k] = 0.0;
   for (int c = 0; c < nc; c++) { rhs[c] = 0.0; ap[c] = 0.0; }
   double sd = 0.0, *psi = NULL;
@@ -759,17 +850,56 @@ void or_temperature_assembly(or_case *cs, double *val, double *rhs, double *ap) 
     psi = xcalloc(nc, 8);
     bj_limiter(cs, 0, cs->T, psi);
   }
+  /* thermal dispersion, Bear model (:1781-1799) */
+  double *kdisp = xcalloc(3 * (size_t)nc, 8);
+  for (int c = 0; c < nc; c++) {
+    if (cs->disp_l[c] <= 0.0 && cs->disp_t[c] <= 0.0) continue;
+    double umag = sqrt(dot3(&cs->u[3 * c], &cs->u[3 * c]));
+    if (umag < 1.0e-14) continue;
+    double umag2 = umag * umag;
+    for (int d = 0; d < 3; d++) {
+      double ud = cs->u[3 * c + d];
+      kdisp[3 * c + d] = ct->rho * ct->cp / umag * (cs->disp_l[c] * (ud * ud) + cs->disp_t[c] * (umag2 - ud * ud));
+    }
+  }
   for (int i = 0; i < nf; i++) {
     int c1 = cs->c1[i];
     if (c1 == 0) continue;
     int c0 = cs->c0[i];
+    int s0 = is_solid(cs, c0), s1 = is_solid(cs, c1);
+    if (s0 || s1) {                                         /* :1812-1834 */
+      if (s0 && s1) continue;
+      double r0[3], r1[3];
+      for (int d = 0; d < 3; d++) { r0[d] = XF(i, d) - XC(c0, d); r1[d] = XF(i, d) - XC(c1, d); }
+      if (s0) {
+        double kf_ch = ch_face_U(cs->area[i], norm3(r1), norm3(r0), cs->porosity[c1 - 1] * ct->k_cond,
+                                 cs->k_s[c0 - 1]);
+        ap[c1 - 1] = ap[c1 - 1] + kf_ch;
+        rhs[c1 - 1] = rhs[c1 - 1] + kf_ch * cs->T_s[c0 - 1];
+      } else {
+        double kf_ch = ch_face_U(cs->area[i], norm3(r0), norm3(r1), cs->porosity[c0 - 1] * ct->k_cond,
+                                 cs->k_s[c1 - 1]);
+        ap[c0 - 1] = ap[c0 - 1] + kf_ch;
+        rhs[c0 - 1] = rhs[c0 - 1] + kf_ch * cs->T_s[c1 - 1];
+      }
+      continue;
+    }
     double FT = ct->cp * cs->flux[i], lf = cs->lf[i], dvec[3];
     for (int d = 0; d < 3; d++) dvec[d] = XC(c1, d) - XC(c0, d);
     double dn = norm3(dvec);
-    double k0 = cs->porosity[c0 - 1] * ct->k_cond + (1.0 - cs->porosity[c0 - 1]) * cs->k_s[c0 - 1];
-    double k1 = cs->porosity[c1 - 1] * ct->k_cond + (1.0 - cs->porosity[c1 - 1]) * cs->k_s[c1 - 1];
+    double k0, k1;
+    if (ltne) {
+      k0 = cs->porosity[c0 - 1] * ct->k_cond;
+      k1 = cs->porosity[c1 - 1] * ct->k_cond;
+    } else {
+      k0 = cs->porosity[c0 - 1] * ct->k_cond + (1.0 - cs->porosity[c0 - 1]) * cs->k_s[c0 - 1];
+      k1 = cs->porosity[c1 - 1] * ct->k_cond + (1.0 - cs->porosity[c1 - 1]) * cs->k_s[c1 - 1];
+    }
     double kf = lf * k0 + (1.0 - lf) * k1;
-    kf = kf + (lf * 0.0 + (1.0 - lf) * 0.0);   /* zero dispersion term */
+    double ndsf[3];
+    for (int d = 0; d < 3; d++) { double t = SF(i, d) / cs->area[i]; ndsf[d] = t * t; }
+    double kd_f = lf * dot3(ndsf, &kdisp[3 * (c0 - 1)]) + (1.0 - lf) * dot3(ndsf, &kdisp[3 * (c1 - 1)]);
+    kf = kf + kd_f;
     double DT;
     if (ct->nonorth_corr > 0.0) {
       sd = dot3(&cs->sf[3 * i], dvec);
@@ -812,21 +942,40 @@ void or_temperature_assembly(or_case *cs, double *val, double *rhs, double *ap) 
     bc_face_T(cs, i, cs->T[c0 - 1], &Tf, &qf, &neum);
     for (int d = 0; d < 3; d++) r[d] = XF(i, d) - XC(c0, d);
     double dw = norm3(r);
-    double kf = cs->porosity[c0 - 1] * ct->k_cond + (1.0 - cs->porosity[c0 - 1]) * cs->k_s[c0 - 1];
-    kf = kf + 0.0;
+    double por = cs->porosity[c0 - 1];
+    double kf = ltne ? por * ct->k_cond : por * ct->k_cond + (1.0 - por) * cs->k_s[c0 - 1];
+    double ndsf[3];
+    for (int d = 0; d < 3; d++) { double t = SF(i, d) / cs->area[i]; ndsf[d] = t * t; }
+    kf = kf + dot3(ndsf, &kdisp[3 * (c0 - 1)]);
     double DT = kf * cs->area[i] / dw;
     double FT = ct->cp * cs->flux[i];
     if (!neum) {
       ap[c0 - 1] = ap[c0 - 1] + DT + fmax0(FT, 0.0);
       rhs[c0 - 1] = rhs[c0 - 1] + DT * Tf - fmin0(FT, 0.0) * Tf;
     } else {
-      if (fabs(qf) > 0.0) rhs[c0 - 1] = rhs[c0 - 1] + qf * cs->area[i];
+      if (fabs(qf) > 0.0) {
+        if (ltne) {
+          double kft = por * ct->k_cond, kst = (1.0 - por) * cs->k_s[c0 - 1];
+          rhs[c0 - 1] = rhs[c0 - 1] + qf * cs->area[i] * kft / (kft + kst);
+        } else {
+          rhs[c0 - 1] = rhs[c0 - 1] + qf * cs->area[i];
+        }
+      }
       int bt = cs->gb[gi - 1].btype;
       if (bt == OR_BC_POUTLET || bt == OR_BC_OUTFLOW) ap[c0 - 1] = ap[c0 - 1] + fmax0(FT, 0.0);
     }
   }
+  if (ltne)                                                  /* :1959-1967 */
+    for (int c = 0; c < nc; c++) {
+      double hasf = cs->h_sf[c] * cs->a_sf[c];
+      if (hasf > 0.0) {
+        ap[c] = ap[c] + hasf * cs->vol[c];
+        rhs[c] = rhs[c] + hasf * cs->T_s[c] * cs->vol[c];
+      }
+    }
   for (int c = 0; c < nc; c++) {
-    double cap = cs->porosity[c] * ct->rho * ct->cp + (1.0 - cs->porosity[c]) * cs->rho_s[c] * cs->cp_s[c];
+    double cap = ltne ? cs->porosity[c] * ct->rho * ct->cp
+                      : cs->porosity[c] * ct->rho * ct->cp + (1.0 - cs->porosity[c]) * cs->rho_s[c] * cs->cp_s[c];
     if (cs->ts_order == 1) {
       double at = cap * cs->vol[c] / ct->dt;
       ap[c] = ap[c] + at;
@@ -846,7 +995,131 @@ void or_temperature_assembly(or_case *cs, double *val, double *rhs, double *ap) 
       rhs[c - 1] = rhs[c - 1] + (1.0 - ct->alpha_u) / ct->alpha_u * ap[c - 1] * cs->T[c - 1];
     }
   }
+  if (any_solid(cs))                                         /* :2024-2033 */
+    for (int c = 1; c <= nc; c++)
+      if (is_solid(cs, c)) {
+        int kd = pos_of(cs, c, c);
+        val[kd - 1] = 1.0;
+        rhs[c - 1] = cs->T_s[c - 1];
+      }
+  free(kdisp);
   free(psi);
+}
+
+/* solid_temperature_assembly, mod_uns_simple.f90:2109-2296 */
+void or_solid_temperature_assembly(or_case *cs, double *val, double *rhs, double *ap) {
+  const or_ctrl_t *ct = &cs->ctrl;
+  int nc = cs->nc, nf = cs->nf;
+  for (int k = 0; k < cs->nnz; k++) val[k] = 0.0;
+  for (int c = 0; c < nc; c++) { rhs[c] = 0.0; ap[c] = 0.0; }
+  double sd = 0.0;
+  for (int i = 0; i < nf; i++) {
+    int c1 = cs->c1[i];
+    if (c1 == 0) continue;
+    int c0 = cs->c0[i];
+    int s0 = is_solid(cs, c0), s1 = is_solid(cs, c1);
+    if (s0 != s1) {
+      double r0[3], r1[3];
+      for (int d = 0; d < 3; d++) { r0[d] = XF(i, d) - XC(c0, d); r1[d] = XF(i, d) - XC(c1, d); }
+      if (s0) {
+        double kf_ch = ch_face_U(cs->area[i], norm3(r0), norm3(r1), cs->k_s[c0 - 1],
+                                 cs->porosity[c1 - 1] * ct->k_cond);
+        ap[c0 - 1] = ap[c0 - 1] + kf_ch;
+        rhs[c0 - 1] = rhs[c0 - 1] + kf_ch * cs->T[c1 - 1];
+      } else {
+        double kf_ch = ch_face_U(cs->area[i], norm3(r1), norm3(r0), cs->k_s[c1 - 1],
+                                 cs->porosity[c0 - 1] * ct->k_cond);
+        ap[c1 - 1] = ap[c1 - 1] + kf_ch;
+        rhs[c1 - 1] = rhs[c1 - 1] + kf_ch * cs->T[c0 - 1];
+      }
+      continue;
+    }
+    double dvec[3];
+    for (int d = 0; d < 3; d++) dvec[d] = XC(c1, d) - XC(c0, d);
+    double dn = norm3(dvec);
+    double ks0 = (1.0 - cs->porosity[c0 - 1]) * cs->k_s[c0 - 1];
+    double ks1 = (1.0 - cs->porosity[c1 - 1]) * cs->k_s[c1 - 1];
+    double ksf = cs->lf[i] * ks0 + (1.0 - cs->lf[i]) * ks1, Ds;
+    if (ct->nonorth_corr > 0.0) {
+      sd = dot3(&cs->sf[3 * i], dvec);
+      Ds = ksf * (cs->area[i] * cs->area[i]) / sd;
+    } else {
+      Ds = ksf * cs->area[i] / dn;
+    }
+    ap[c0 - 1] = ap[c0 - 1] + Ds;
+    ap[c1 - 1] = ap[c1 - 1] + Ds;
+    int kd = pos_of(cs, c0, c1);
+    val[kd - 1] = val[kd - 1] - Ds;
+    kd = pos_of(cs, c1, c0);
+    val[kd - 1] = val[kd - 1] - Ds;
+    if (ct->nonorth_corr > 0.0) {
+      double gf[3], t[3];
+      for (int d = 0; d < 3; d++) {
+        gf[d] = cs->lf[i] * cs->gts[3 * (c0 - 1) + d] + (1.0 - cs->lf[i]) * cs->gts[3 * (c1 - 1) + d];
+        t[d] = SF(i, d) - dvec[d] * ((cs->area[i] * cs->area[i]) / sd);
+      }
+      double noc = ct->nonorth_corr * ksf * dot3(gf, t);
+      rhs[c0 - 1] = rhs[c0 - 1] + noc;
+      rhs[c1 - 1] = rhs[c1 - 1] - noc;
+    }
+  }
+  for (int i = 0; i < nf; i++) {
+    int gi = cs->fgrp[i];
+    if (gi == 0) continue;
+    int c0 = cs->c0[i];
+    double Tf, qf, r[3];
+    int neum;
+    bc_face_T(cs, i, cs->T_s[c0 - 1], &Tf, &qf, &neum);
+    int bt = cs->gb[gi - 1].btype;
+    if (bt == OR_BC_VINLET || bt == OR_BC_VINLET_PARAB || bt == OR_BC_MASSINLET) { neum = 1; qf = 0.0; }
+    for (int d = 0; d < 3; d++) r[d] = XF(i, d) - XC(c0, d);
+    double dw = norm3(r);
+    double ksf = (1.0 - cs->porosity[c0 - 1]) * cs->k_s[c0 - 1];
+    double kft = cs->porosity[c0 - 1] * ct->k_cond;
+    double Ds = ksf * cs->area[i] / dw;
+    if (!neum) {
+      ap[c0 - 1] = ap[c0 - 1] + Ds;
+      rhs[c0 - 1] = rhs[c0 - 1] + Ds * Tf;
+    } else if (fabs(qf) > 0.0) {
+      rhs[c0 - 1] = rhs[c0 - 1] + qf * cs->area[i] * ksf / (kft + ksf);
+    }
+  }
+  for (int c = 0; c < nc; c++) {
+    double hasf = cs->h_sf[c] * cs->a_sf[c];
+    if (hasf > 0.0) {
+      ap[c] = ap[c] + hasf * cs->vol[c];
+      rhs[c] = rhs[c] + hasf * cs->T[c] * cs->vol[c];
+    }
+  }
+  for (int c = 0; c < nc; c++)
+    if (cs->q_src[c] != 0.0) rhs[c] = rhs[c] + cs->q_src[c] * cs->vol[c];
+  for (int c = 0; c < nc; c++) {
+    if (cs->ts_order == 1) {
+      double at = (1.0 - cs->porosity[c]) * cs->rho_s[c] * cs->cp_s[c] * cs->vol[c] / ct->dt;
+      ap[c] = ap[c] + at;
+      rhs[c] = rhs[c] + at * cs->T_s_old[c];
+    } else if (cs->ts_order == 2) {
+      double at = (1.0 - cs->porosity[c]) * cs->rho_s[c] * cs->cp_s[c] * cs->vol[c] / (2.0 * ct->dt);
+      ap[c] = ap[c] + 3.0 * at;
+      rhs[c] = rhs[c] + at * (4.0 * cs->T_s_old[c] - cs->T_s_old_old[c]);
+    }
+  }
+  for (int c = 1; c <= nc; c++) {
+    int kd = pos_of(cs, c, c);
+    if (cs->ts_order > 0 && !ct->pimple) {
+      val[kd - 1] = ap[c - 1];
+    } else {
+      val[kd - 1] = ap[c - 1] / ct->alpha_u;
+      rhs[c - 1] = rhs[c - 1] + (1.0 - ct->alpha_u) / ct->alpha_u * ap[c - 1] * cs->T_s[c - 1];
+    }
+  }
+  if (any_solid(cs))
+    for (int c = 1; c <= nc; c++) {
+      if (is_solid(cs, c) || is_porous(cs, c)) continue;
+      int kd = pos_of(cs, c, c);
+      val[kd - 1] = 1.0;
+      rhs[c - 1] = cs->T_s[c - 1];
+    }
 }
 
 /* --------------------------------------------------- Rhie-Chow, outflow */
@@ -983,6 +1256,7 @@ void or_correct_fields(or_case *cs, const double *pp, double *dpp) {
     cs->flux[i] = cs->flux[i] - af * (pp[c1 - 1] - pp[c0 - 1]);
   }
   free(phif);
+  zero_solid_velocity(cs);
 }
 
 /* ------------------------------------------------------- linear algebra */
@@ -1307,6 +1581,16 @@ static int temperature_solve(or_case *cs, double *val, double *rhs, double *ap, 
   return ierr;
 }
 
+/* solid-phase temperature solve (solid_temperature_assembly + BiCGSTAB) */
+static int solid_temperature_solve(or_case *cs, double *val, double *rhs, double *ap, int mpi, int *itl) {
+  const or_ctrl_t *ct = &cs->ctrl;
+  int ierr; double resl;
+  or_solid_temperature_assembly(cs, val, rhs, ap);
+  or_bicgstab_ilu0(cs->nc, cs->row_ptr, cs->col_idx, val, rhs, cs->T_s, ct->lin_tol, ct->lin_max,
+                   itl, &resl, &ierr, mpi);
+  return ierr;
+}
+
 static int nonfinite(const or_case *cs) {
   for (int c = 0; c < cs->nc; c++) {
     for (int d = 0; d < 3; d++) if (!isfinite(cs->u[3 * c + d])) return 1;
@@ -1345,6 +1629,7 @@ int or_simple_run(or_case *cs, int nsteps, int mpi, int *it_done) {
     momentum_solves(cs, val, rhs, ap, ucomp, mpi, &itl_max, &ressum, &brk);
     if (brk) { ier = 20; break; }
     for (int c = 0; c < nc; c++) cs->apc[c] = ap[c] / ct->alpha_u;
+    zero_solid_velocity(cs);
     or_compute_gradients(cs);
     or_flux_rhiechow(cs);
     or_outflow_mass_rescale(cs);
@@ -1354,6 +1639,10 @@ int or_simple_run(or_case *cs, int nsteps, int mpi, int *it_done) {
     memcpy(Tprev, cs->T, nc * 8);
     if (temperature_solve(cs, val, rhs, ap, mpi, &itl) == 2) { ier = 22; break; }
     if (itl > itl_max) itl_max = itl;
+    if (solid_phase_active(cs)) {                       /* :214-223 */
+      if (solid_temperature_solve(cs, val, rhs, ap, mpi, &itl) == 2) { ier = 23; break; }
+      if (itl > itl_max) itl_max = itl;
+    }
     if (nonfinite(cs)) { ier = 60; break; }
     imbal = max_imbalance(cs, imb) / flux_ref;
     du_max = 0.0; dT_max = 0.0;
@@ -1402,6 +1691,7 @@ int or_piso_run(or_case *cs, int nsteps, int mpi) {
     momentum_solves(cs, val, rhs, ap, ucomp, mpi, &itl_max, &ressum, &brk);
     if (brk) { ier = 33; break; }
     memcpy(cs->apc, ap, nc * 8);
+    if (!mpi) zero_solid_velocity(cs);                  /* serial piso_run only (:432) */
     or_compute_gradients(cs);
     or_flux_rhiechow(cs);
     or_outflow_mass_rescale(cs);
@@ -1414,6 +1704,12 @@ int or_piso_run(or_case *cs, int nsteps, int mpi) {
     if (bad) { ier = 34; break; }
     if (temperature_solve(cs, val, rhs, ap, mpi, &itl) == 2) { ier = 35; break; }
     if (itl > itl_max) itl_max = itl;
+    /* solid phase: LTNE on the partitioned path (mpi :483-495), LTNE or
+     * solid zones in the serial piso_run (:478-487) */
+    if (mpi ? ct->thermal_model != 0 : solid_phase_active(cs)) {
+      if (solid_temperature_solve(cs, val, rhs, ap, mpi, &itl) == 2) { ier = 36; break; }
+      if (itl > itl_max) itl_max = itl;
+    }
     double imbal = max_imbalance(cs, imb) / flux_ref;
     double du_max = 0.0, umax = 0.0;
     for (size_t k = 0; k < n3; k++) if (fabs(cs->u[k] - cs->u_old[k]) > du_max) du_max = fabs(cs->u[k] - cs->u_old[k]);
@@ -1440,8 +1736,8 @@ int or_piso_run(or_case *cs, int nsteps, int mpi) {
 int or_pimple_run(or_case *cs, int nsteps, int mpi) {
   const or_ctrl_t *ct = &cs->ctrl;
   int nc = cs->nc;
-  if (!ct->transient) return 50;
-  if (ct->dt <= 0.0) return 51;
+  if (!ct->transient) return mpi ? 50 : 40;
+  if (ct->dt <= 0.0) return mpi ? 51 : 41;
   double *val = xcalloc(cs->nnz, 8), *rhs = xcalloc(nc, 8), *ap = xcalloc(nc, 8);
   double *pp = xcalloc(nc, 8), *dpp = xcalloc(3 * (size_t)nc, 8), *imb = xcalloc(nc, 8);
   double *ucomp = xcalloc(nc, 8);
@@ -1475,6 +1771,7 @@ int or_pimple_run(or_case *cs, int nsteps, int mpi) {
       momentum_solves(cs, val, rhs, ap, ucomp, mpi, &itl_max, &ressum, &brk);
       if (brk) break;
       for (int c = 0; c < nc; c++) cs->apc[c] = ap[c] / ct->alpha_u;
+      if (!mpi) zero_solid_velocity(cs);                /* serial pimple_run only (:689) */
       or_compute_gradients(cs);
       or_flux_rhiechow(cs);
       or_outflow_mass_rescale(cs);
@@ -1485,10 +1782,15 @@ int or_pimple_run(or_case *cs, int nsteps, int mpi) {
       }
       if (bad) break;
     }
-    if (brk) { ier = 53; break; }
-    if (bad) { ier = 54; break; }
-    if (temperature_solve(cs, val, rhs, ap, mpi, &itl) == 2) { ier = 55; break; }
+    const int base = mpi ? 50 : 40;                      /* pimple_run codes 40-46 (:595-737) */
+    if (brk) { ier = base + 3; break; }
+    if (bad) { ier = base + 4; break; }
+    if (temperature_solve(cs, val, rhs, ap, mpi, &itl) == 2) { ier = base + 5; break; }
     if (itl > itl_max) itl_max = itl;
+    if (mpi ? ct->thermal_model != 0 : solid_phase_active(cs)) {
+      if (solid_temperature_solve(cs, val, rhs, ap, mpi, &itl) == 2) { ier = base + 6; break; }
+      if (itl > itl_max) itl_max = itl;
+    }
     double imbal = max_imbalance(cs, imb) / flux_ref;
     double du_max = 0.0, umax = 0.0;
     for (size_t k = 0; k < n3; k++) if (fabs(cs->u[k] - cs->u_old[k]) > du_max) du_max = fabs(cs->u[k] - cs->u_old[k]);

@@ -44,6 +44,7 @@ typedef struct {
   int periodic_zone[4][2];
   int periodic_auto[4];
   double periodic_shift[4][3];
+  int porous_unified;   /* porous_model: 0 partitioned, 1 unified */
 } or_ctrl_t;
 
 /* one boundary group (bcgroup_t, mod_uns_bc.f90:80-110) */
@@ -95,6 +96,9 @@ int or_get_ts_order(or_case *);
 void or_compute_gradients(or_case *);
 void or_momentum_assembly(or_case *, int comp, double *val, double *rhs, double *ap);
 void or_temperature_assembly(or_case *, double *val, double *rhs, double *ap);
+void or_solid_temperature_assembly(or_case *, double *val, double *rhs, double *ap);
+/* block type per cell (1 fluid, 2 porous, 3 solid; NULL = all fluid) */
+void or_set_cztype(or_case *, const int *cztype);
 void or_flux_rhiechow(or_case *);
 void or_outflow_mass_rescale(or_case *);
 void or_ppe_assembly(or_case *, double *val, double *rhs);

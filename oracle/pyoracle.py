@@ -43,6 +43,8 @@ def lib():
             getattr(L, nm).argtypes = [C.c_void_p]
         L.or_momentum_assembly.argtypes = [C.c_void_p, C.c_int, _D, _D, _D]
         L.or_temperature_assembly.argtypes = [C.c_void_p, _D, _D, _D]
+        L.or_solid_temperature_assembly.argtypes = [C.c_void_p, _D, _D, _D]
+        L.or_set_cztype.argtypes = [C.c_void_p, _I]
         L.or_ppe_assembly.argtypes = [C.c_void_p, _D, _D]
         L.or_correct_fields.argtypes = [C.c_void_p, _D, _D]
         L.or_simple_run.argtypes = [C.c_void_p, C.c_int, C.c_int, _I]
@@ -94,6 +96,13 @@ class OracleCase:
         if not self.h:
             raise RuntimeError(f"oracle case creation failed, ier={ier.value}")
         self.nnz = L.or_nnz(self.h)
+        from mixsolver_amd.zones import apply_zones
+        apply_zones(self, mesh, ctrl)
+
+    def set_cell_types(self, cztype):
+        t = np.ascontiguousarray(cztype, np.int32)
+        self._cztype = t
+        lib().or_set_cztype(self.h, _ip(t))
 
     def close(self):
         if self.h:
@@ -163,6 +172,11 @@ class OracleCase:
     def temperature_assembly(self):
         val, rhs, ap = np.zeros(self.nnz), np.zeros(self.nc), np.zeros(self.nc)
         lib().or_temperature_assembly(self.h, _dp(val), _dp(rhs), _dp(ap))
+        return val, rhs, ap
+
+    def solid_temperature_assembly(self):
+        val, rhs, ap = np.zeros(self.nnz), np.zeros(self.nc), np.zeros(self.nc)
+        lib().or_solid_temperature_assembly(self.h, _dp(val), _dp(rhs), _dp(ap))
         return val, rhs, ap
 
     def ppe_assembly(self):

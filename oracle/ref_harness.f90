@@ -12,6 +12,9 @@
 !   pimple : seed fields from in.bin (u,p,T), run pimple_run (n_time_max steps)
 !   funcs  : seed fields from in.bin, then compute_gradients, momentum_assembly
 !            (comp 1..3), flux_rhiechow, temperature_assembly, correct_fields
+!   funcs2 : funcs with T_s / T_s_old / T_s_old_old seeded too, preceded by the
+!            resolved cell types and porous coefficients, with the solid-phase
+!            gradient and solid_temperature_assembly added
 !   solve  : CSR system from in.bin; residual histories of cg_jacobi, cg_ic0,
 !            bicgstab_ilu0 for maxit = 1..n, IC0/ILU0 factors, final solutions
 ! All arrays are written as raw little-endian stream records in the order the
@@ -89,6 +92,8 @@ program ref_harness
       call dump_state()
    case ( 'funcs' )
       call run_funcs()
+   case ( 'funcs2' )
+      call run_funcs2()
    case default
       write(*,*) 'unknown mode ', trim(mode)
       stop 2
@@ -129,6 +134,7 @@ contains
 
    subroutine dump_state()
       write(uo) fld%u, fld%p, fld%flux, fld%T, fld%gp, fld%gu, fld%apc, fld%gt
+      write(uo) fld%T_s
    end subroutine dump_state
 
    ! in.bin: u, p, T, flux(nf), u_old, u_old_old, T_old, T_old_old, apc(nc),
@@ -162,6 +168,42 @@ contains
       call correct_fields( m, g, ctrl, bcs, fld, pp, phif, dpp )
       write(uo) fld%u, fld%p, fld%flux, dpp
    end subroutine run_funcs
+
+   ! in.bin: as run_funcs, then T_s(nc), T_s_old(nc), T_s_old_old(nc)
+   subroutine run_funcs2()
+      type(csr_t) :: A
+      real(dp), allocatable :: rhs(:), ap(:), pp(:), phif(:), dpp(:,:)
+      integer :: comp, tso
+      allocate( rhs(m%ncells), ap(m%ncells), pp(m%ncells), phif(m%nfaces), &
+                dpp(3,m%ncells) )
+      open( newunit=ui, file=trim(inf), access='stream', form='unformatted', &
+            status='old', action='read' )
+      read(ui) fld%u, fld%p, fld%T, fld%flux, fld%u_old, fld%u_old_old, &
+               fld%T_old, fld%T_old_old, fld%apc, pp, tso
+      read(ui) fld%T_s, fld%T_s_old, fld%T_s_old_old
+      close( ui )
+      fld%ts_order = tso
+      write(uo) m%cztype
+      write(uo) fld%perm_dir, fld%inertial, fld%porosity, fld%disp_l, fld%disp_t, &
+                fld%bj_alpha, fld%k_s, fld%cp_s, fld%rho_s, fld%h_sf, fld%a_sf, fld%q_src
+      call build_cell_csr( m, c, A )
+      call compute_gradients( m, g, bcs, fld )
+      write(uo) fld%gp, fld%gu, fld%gt, fld%gts
+      do comp = 1, 3
+         call momentum_assembly( m, g, ctrl, bcs, fld, comp, A, rhs, ap )
+         write(uo) A%val, rhs, ap
+      end do
+      call temperature_assembly( m, g, ctrl, bcs, fld, A, rhs, ap )
+      write(uo) A%val, rhs, ap
+      call solid_temperature_assembly( m, g, ctrl, bcs, fld, A, rhs, ap )
+      write(uo) A%val, rhs, ap
+      call flux_rhiechow( m, g, ctrl, bcs, fld )
+      write(uo) fld%flux
+      call outflow_mass_rescale( m, g, bcs, fld )
+      write(uo) fld%flux
+      call correct_fields( m, g, ctrl, bcs, fld, pp, phif, dpp )
+      write(uo) fld%u, fld%p, fld%flux, dpp, fld%u_old
+   end subroutine run_funcs2
 
    ! in.bin: n, nnz, row_ptr(n+1), col_idx(nnz), val(nnz), b(n), x0(n),
    !         tol, kmax ;   nsteps = maxit used for the final solves

@@ -19,8 +19,45 @@ def load_golden(name):
 
 def mesh_from(d):
     from mixsolver_amd.meshgen import Mesh
-    return Mesh(x=d["x"], fptr=d["fptr"], fnodes=d["fnodes"], c0=d["c0"], c1=d["c1"],
-                zone=d["zone"], ncells=int(d["ncells"]), ctype=int(d["ctype"]))
+    m = Mesh(x=d["x"], fptr=d["fptr"], fnodes=d["fnodes"], c0=d["c0"], c1=d["c1"],
+             zone=d["zone"], ncells=int(d["ncells"]), ctype=int(d["ctype"]))
+    if "czone" in d:
+        m.czone = d["czone"]
+        m.czone_names = {int(i): (str(c), str(n))
+                         for i, c, n in zip(d["czone_ids"], d["czone_cond"], d["czone_name"])}
+    return m
+
+
+POROUS_CASES = ("porous_plug_ltne", "porous_plug_unified", "porous_solid_cht")
+FUNCS2_FIELDS = ("u", "p", "T", "flux", "u_old", "u_old_old", "T_old", "T_old_old", "apc", "T_s", "T_s_old",
+                 "T_s_old_old")
+
+
+def run_funcs2(case, d, tso):
+    """Seed case (device Case or OracleCase) with the inputs of a funcs2
+    fixture and return its per-function outputs under the fixture's keys."""
+    for k in FUNCS2_FIELDS:
+        case.set_field(k, d[f"ts{tso}_in_{k}"])
+    if hasattr(case, "set_ts_order"):
+        case.set_ts_order(tso)
+    else:
+        case.ts_order = tso
+    out = {}
+    case.compute_gradients()
+    for k in ("gp", "gu", "gt", "gts"):
+        out[k] = case.get(k)
+    for comp in (1, 2, 3):
+        out[f"mom{comp}_val"], out[f"mom{comp}_rhs"], out[f"mom{comp}_ap"] = case.momentum_assembly(comp)
+    out["tem_val"], out["tem_rhs"], out["tem_ap"] = case.temperature_assembly()
+    out["sol_val"], out["sol_rhs"], out["sol_ap"] = case.solid_temperature_assembly()
+    case.flux_rhiechow()
+    out["flux_rc"] = case.get("flux")
+    case.outflow_mass_rescale()
+    out["flux_out"] = case.get("flux")
+    out["cor_dpp"] = case.correct_fields(d[f"ts{tso}_in_pp"])
+    for k, f in (("cor_u", "u"), ("cor_p", "p"), ("cor_flux", "flux"), ("cor_u_old", "u_old")):
+        out[k] = case.get(f)
+    return out
 
 
 @pytest.fixture

@@ -136,8 +136,14 @@ tbc = 5 2 3.0
 
 
 def mesh_arrays(m):
-    return dict(x=m.x, fptr=m.fptr, fnodes=m.fnodes, c0=m.c0, c1=m.c1, zone=m.zone,
-                ncells=np.int64(m.ncells), ctype=np.int64(m.ctype))
+    out = dict(x=m.x, fptr=m.fptr, fnodes=m.fnodes, c0=m.c0, c1=m.c1, zone=m.zone,
+               ncells=np.int64(m.ncells), ctype=np.int64(m.ctype))
+    if m.czone is not None:
+        ids = sorted(m.czone_names)
+        out.update(czone=m.czone, czone_ids=np.array(ids, np.int64),
+                   czone_cond=np.array([m.czone_names[i][0] for i in ids]),
+                   czone_name=np.array([m.czone_names[i][1] for i in ids]))
+    return out
 
 
 def run(args, cwd):
@@ -177,7 +183,7 @@ def write_fields(path, *arrs):
             np.ascontiguousarray(a).tofile(f)
 
 
-def case_run(name, m, zone_kind, ctl, mode, nsteps, fields=None):
+def case_run(name, m, zone_kind, ctl, mode, nsteps, fields=None, want_ts=False):
     with tempfile.TemporaryDirectory() as td:
         cas.write_cas(os.path.join(td, "m.cas"), m, zone_kind)
         open(os.path.join(td, "c.control"), "w").write(ctl)
@@ -193,6 +199,8 @@ def case_run(name, m, zone_kind, ctl, mode, nsteps, fields=None):
                    flux=r.take(np.float64, nf), T=r.take(np.float64, nc),
                    gp=r.take(np.float64, 3 * nc).reshape(nc, 3), gu=r.take(np.float64, 9 * nc).reshape(nc, 9),
                    apc=r.take(np.float64, nc), gt=r.take(np.float64, 3 * nc).reshape(nc, 3))
+        if want_ts:
+            out["T_s"] = r.take(np.float64, nc)
         return out, log
 
 
@@ -255,6 +263,132 @@ def funcs_run(m, zone_kind, ctl, seed, ts_order):
         out["cor_flux"] = r.take(np.float64, nf)
         out["cor_dpp"] = r.take(np.float64, 3 * nc).reshape(nc, 3)
     return inputs, out
+
+
+def funcs2_run(m, zone_kind, ctl, seed, ts_order):
+    """Per-function vectors with porous / solid zones (harness mode funcs2)."""
+    rng = np.random.default_rng(seed)
+    nc, nf = m.ncells, m.nfaces
+    u, p, T = seeded_fields(m, seed)
+    flux = rng.normal(size=nf) * 0.1
+    u_old = u + 0.05 * rng.normal(size=u.shape)
+    u_oo = u + 0.05 * rng.normal(size=u.shape)
+    T_old = T + 0.05 * rng.normal(size=nc)
+    T_oo = T + 0.05 * rng.normal(size=nc)
+    apc = rng.uniform(1.0, 3.0, size=nc)
+    pp = rng.normal(size=nc) * 0.01
+    Ts = T + 0.2 * rng.normal(size=nc)
+    Ts_old = Ts + 0.05 * rng.normal(size=nc)
+    Ts_oo = Ts + 0.05 * rng.normal(size=nc)
+    inputs = dict(u=u, p=p, T=T, flux=flux, u_old=u_old, u_old_old=u_oo, T_old=T_old, T_old_old=T_oo,
+                  apc=apc, pp=pp, ts_order=np.int64(ts_order), T_s=Ts, T_s_old=Ts_old, T_s_old_old=Ts_oo)
+    with tempfile.TemporaryDirectory() as td:
+        cas.write_cas(os.path.join(td, "m.cas"), m, zone_kind)
+        open(os.path.join(td, "c.control"), "w").write(ctl)
+        inf = os.path.join(td, "in.bin")
+        write_fields(inf, u, p, T, flux, u_old, u_oo, T_old, T_oo, apc, pp, np.array([ts_order], np.int32),
+                     Ts, Ts_old, Ts_oo)
+        run(["m.cas", "c.control", "funcs2", "out.bin", inf], td)
+        r = Reader(os.path.join(td, "out.bin"))
+        g = geom_run(m, zone_kind, ctl)
+        nnz = len(g["col_idx"])
+        out = dict(cztype=r.take(np.int32, nc), perm_dir=r.take(np.float64, 3 * nc).reshape(nc, 3))
+        for k in ("inertial", "porosity", "disp_l", "disp_t", "bj_alpha", "k_s", "cp_s", "rho_s", "h_sf", "a_sf",
+                  "q_src"):
+            out[k] = r.take(np.float64, nc)
+        out.update(gp=r.take(np.float64, 3 * nc).reshape(nc, 3), gu=r.take(np.float64, 9 * nc).reshape(nc, 9),
+                   gt=r.take(np.float64, 3 * nc).reshape(nc, 3), gts=r.take(np.float64, 3 * nc).reshape(nc, 3))
+        for comp in (1, 2, 3):
+            out[f"mom{comp}_val"] = r.take(np.float64, nnz)
+            out[f"mom{comp}_rhs"] = r.take(np.float64, nc)
+            out[f"mom{comp}_ap"] = r.take(np.float64, nc)
+        for t in ("tem", "sol"):
+            out[f"{t}_val"] = r.take(np.float64, nnz)
+            out[f"{t}_rhs"] = r.take(np.float64, nc)
+            out[f"{t}_ap"] = r.take(np.float64, nc)
+        out["flux_rc"] = r.take(np.float64, nf)
+        out["flux_out"] = r.take(np.float64, nf)
+        out["cor_u"] = r.take(np.float64, 3 * nc).reshape(nc, 3)
+        out["cor_p"] = r.take(np.float64, nc)
+        out["cor_flux"] = r.take(np.float64, nf)
+        out["cor_dpp"] = r.take(np.float64, 3 * nc).reshape(nc, 3)
+        out["cor_u_old"] = r.take(np.float64, 3 * nc).reshape(nc, 3)
+    return inputs, out
+
+
+POROUS_CTL = """rho = 1.0
+mu = 0.05
+alpha_u = 0.7
+alpha_p = 0.3
+transient = true
+dt = 0.02
+n_time_max = 2
+n_correct = 2
+n_out_every = 1000000
+time_scheme = 2
+outer_max = 3
+lin_tol = 1e-10
+lin_max = 2000
+conv_blend = 0.5
+nonorth_corr = 1.0
+ppe_precond = jacobi
+k_cond = 0.1
+cp = 1.0
+{model}
+bc = 6 velocity-inlet-parabolic 1.0 2.0 2 -1.0 1.0
+bc = 7 outflow
+bc = 5 wall
+bc = 4 symmetry
+tbc = 5 1 0.5
+tbc_plane = 5 2 -1.0 2 3.0
+{zones}
+"""
+
+PLUG_LTNE = ("thermal_model = ltne",
+             "cell_zone = 8 porous perm=0.01 perm_yy=0.02 inertial=0.5 porosity=0.4 disp_l=0.01 disp_t=0.002 "
+             "bj_alpha=0.3 k_s=2.0 cp_s=0.8 rho_s=3.0 h_sf=5.0 a_sf=10.0")
+PLUG_UNIFIED = ("porous_model = unified",
+                "cell_zone = 8 porous perm=0.01 inertial=0.5 porosity=0.4 disp_l=0.01 disp_t=0.002 k_s=2.0 "
+                "cp_s=0.8 rho_s=3.0")
+# porous zone typed by a VC tag in its name, coefficients matched by name;
+# a solid block with a heat source (conjugate heat transfer)
+SOLID_CHT = ("", "cell_zone = plug perm=0.02 inertial=0.2 porosity=0.5 k_s=1.5 cp_s=0.9 rho_s=2.5\n"
+                  "cell_zone = 9 solid k_s=5.0 cp_s=0.5 rho_s=2.0 q_src=4.0")
+
+
+def zoned_channel(solid=False):
+    """10x6x3 jittered channel with a porous plug (cells 3 <= i < 6) and,
+    with solid=True, a solid block (i >= 7, j < 2) -- zones contiguous."""
+    m = meshgen.hex_box(np.linspace(0, 2.0, 11), meshgen.tanh_stretch(6, 1.1), np.linspace(0, 0.6, 4),
+                        zone_ids=dict(xmin=6, xmax=7, ymin=5, ymax=5, zmin=4, zmax=4), jitter=0.15, seed=21)
+    idx = np.arange(m.ncells)
+    I, J = idx % 10, (idx // 10) % 6
+    cz = np.full(m.ncells, 2, np.int32)
+    cz[(I >= 3) & (I < 6)] = 8
+    names = {2: ("fluid", "free"), 8: ("fluid", "plug")}
+    if solid:
+        cz[(I >= 7) & (J < 2)] = 9
+        names = {2: ("fluid", "free"), 8: ("vc:porous", "plug"), 9: ("solid", "block")}
+    return meshgen.with_cell_zones(m, cz, names)
+
+
+def porous_fixtures():
+    zk = {4: "symmetry", 5: "wall", 6: "velocity-inlet", 7: "outflow"}
+    for tag, (model, zones), solid in (("plug_ltne", PLUG_LTNE, False), ("plug_unified", PLUG_UNIFIED, False),
+                                       ("solid_cht", SOLID_CHT, True)):
+        m = zoned_channel(solid)
+        ctl = POROUS_CTL.format(model=model, zones=zones)
+        fx = {}
+        for tso in (0, 2):
+            inputs, out = funcs2_run(m, zk, ctl, seed=31 + tso, ts_order=tso)
+            fx.update({f"ts{tso}_in_" + k: v for k, v in inputs.items()})
+            fx.update({f"ts{tso}_out_" + k: v for k, v in out.items()})
+        u0, p0, T0 = seeded_fields(m, 41, umag=0.3)
+        u0[:, 0] += 1.0
+        simple, _ = case_run(tag, m, zk, ctl, "simple", 3, (u0, p0, T0), want_ts=True)
+        piso, _ = case_run(tag, m, zk, ctl, "piso", 2, (u0, p0, T0), want_ts=True)
+        save(f"porous_{tag}", control=ctl, **mesh_arrays(m), in_u=u0, in_p=p0, in_T=T0,
+             **{"simple_" + k: v for k, v in simple.items()}, **{"piso_" + k: v for k, v in piso.items()}, **fx)
 
 
 def solve_run(row_ptr, col_idx, val, b, x0, tol, kmax, maxit):
@@ -437,4 +571,8 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["--porous"]:
+        porous_fixtures()
+    else:
+        main()
+        porous_fixtures()

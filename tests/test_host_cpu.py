@@ -247,3 +247,30 @@ def test_serving_workload_systems(kind):
     for i in range(P.n):
         cols = P.col_idx[P.row_ptr[i] - 1:P.row_ptr[i + 1] - 1]
         assert np.all(np.diff(cols) > 0) and (i + 1) in cols
+
+
+def test_packed_bed_closures():
+    """Kozeny-Carman written with the particle diameter (constant 180) equals
+    the Kozeny form with the specific surface S0 = 6/dp (Kozeny constant 5):
+    K = eps^3 / (5 S0^2 (1-eps)^2) = eps^3 / (5 a_sf^2) with a_sf = S0 (1-eps);
+    the Ergun sink at a superficial speed U reproduces the Ergun pressure
+    gradient 150 mu (1-eps)^2 U / (eps^3 dp^2) + 1.75 rho (1-eps) U^2 / (eps^3 dp)
+    once the viscous constant 150 is used (ratio 180/150 on the Darcy part)."""
+    from mixsolver_amd import zones
+    eps = np.array([0.35, 0.4, 0.6, 0.9])
+    dp = 1.0e-3
+    K = zones.kozeny_carman_permeability(eps, dp)
+    a = zones.specific_surface(eps, dp)
+    np.testing.assert_allclose(K, eps ** 3 / (5.0 * a ** 2), rtol=1e-14)
+    S0 = 6.0 / dp
+    np.testing.assert_allclose(K, eps ** 3 / (5.0 * S0 ** 2 * (1 - eps) ** 2), rtol=1e-14)
+    mu, rho, U = 1.8e-5, 1.2, 0.05
+    C = zones.ergun_inertial(eps, dp)
+    sink = mu / K * U * (150.0 / 180.0) + rho * C * U * U
+    ergun = 150 * mu * (1 - eps) ** 2 * U / (eps ** 3 * dp ** 2) + 1.75 * rho * (1 - eps) * U ** 2 / (eps ** 3 * dp)
+    np.testing.assert_allclose(sink, ergun, rtol=1e-13)
+    # K grows with dp^2 and falls as the bed closes
+    assert np.all(np.diff(K) > 0)
+    np.testing.assert_allclose(zones.kozeny_carman_permeability(eps, 2 * dp), 4 * K, rtol=1e-14)
+    f = zones.packed_bed_fields(eps, dp, k_s=1.0)
+    assert f["perm_dir"].shape == (4, 3) and np.all(f["perm_dir"][:, 0] == K)

@@ -8,7 +8,7 @@ order, so every comparison here is bit-exact (np.array_equal).
 import numpy as np
 import pytest
 
-from conftest import load_golden, mesh_from
+from conftest import POROUS_CASES, load_golden, mesh_from, run_funcs2
 from mixsolver_amd.control import parse_control
 from oracle import pyoracle
 
@@ -139,3 +139,52 @@ def test_ghia_probe_closed_form():
     assert np.allclose(probe_velocity(xc, u, xc[37]), u[37], atol=1e-9)
     l2, uq = ghia_compare(xc, u, 1000)
     assert np.isfinite(l2) and uq.shape == (17,)
+
+
+# ---------------------------------------------------------------- porous zones
+@pytest.mark.parametrize("name", POROUS_CASES)
+def test_zone_resolution_bitexact(name):
+    """Host restatement of resolve_cell_zones + setup_porous_fields
+    (mixsolver_amd/zones.py) against the reference's resolved block types and
+    per-cell coefficient arrays: explicit types, a VC tag in a zone name,
+    name-matched coefficient lines, solid zones."""
+    from mixsolver_amd import zones
+    d = load_golden(name)
+    m = mesh_from(d)
+    ctrl, _ = parse_control(str(d["control"]), warn=False)
+    cz = zones.resolve_cell_zones(m, ctrl.cell_zones)
+    f = zones.setup_porous_fields(m, ctrl.cell_zones, cz)
+    assert np.array_equal(cz, d["ts0_out_cztype"])
+    for k in zones.FIELDS:
+        assert np.array_equal(f[k], d["ts0_out_" + k]), k
+
+
+@pytest.mark.parametrize("name", POROUS_CASES)
+@pytest.mark.parametrize("tso", [0, 2])
+def test_porous_per_function_bitexact(name, tso):
+    """Kink face pressure (gradients, pressure force), Beavers-Joseph slip,
+    unified harmonic viscosity, Darcy-Forchheimer, solid momentum source,
+    dispersion, LTNE fluid / solid phases, conjugate heat transfer, solid
+    pins and zero_solid_velocity: bit-exact against the reference."""
+    d = load_golden(name)
+    ctrl, groups = parse_control(str(d["control"]), warn=False)
+    oc = pyoracle.OracleCase(mesh_from(d), ctrl, groups)
+    out = run_funcs2(oc, d, tso)
+    for k, v in out.items():
+        assert np.array_equal(np.ravel(v), np.ravel(d[f"ts{tso}_out_{k}"])), k
+
+
+@pytest.mark.parametrize("name", POROUS_CASES)
+@pytest.mark.parametrize("drv", ["simple", "piso"])
+def test_porous_driver_bitexact(name, drv):
+    """Serial simple_run / piso_run with porous and solid zones, including the
+    solid-phase temperature solve."""
+    d = load_golden(name)
+    ctrl, groups = parse_control(str(d["control"]), warn=False)
+    oc = pyoracle.OracleCase(mesh_from(d), ctrl, groups)
+    for k, f in (("u", "in_u"), ("p", "in_p"), ("T", "in_T"), ("T_s", "in_T")):
+        oc.set_field(k, d[f])
+    ier = oc.simple_run(3, mpi=False)[0] if drv == "simple" else oc.piso_run(2, mpi=False)
+    assert ier == int(d[f"{drv}_ier"]) == 0
+    for k in ("u", "p", "flux", "T", "T_s"):
+        assert np.array_equal(oc.get(k).ravel(), d[f"{drv}_{k}"].ravel()), k
