@@ -53,3 +53,26 @@ done
 "$FC" -cpp -O2 -c "$HERE/ref_harness.f90" -o "$TMP/ref_harness.o"
 "$FC" -O2 -o "$OUT/ref_harness" "$TMP/ref_harness.o" "${OBJS[@]}"
 echo "built $OUT/ref_harness"
+
+# Drop-in check: the same chain with the reference's mod_uns_linsolver.f90
+# replaced by mixsolver_amd/fortran/mod_uns_linsolver.f90, whose solvers call
+# libmixsolver_hip.so: oracle/_ref/ref_dropin runs the reference's own
+# drivers with the device Krylov solvers.
+LIBDIR="$(cd "$HERE/../mixsolver_amd" && pwd)"
+if [ -f "$LIBDIR/libmixsolver_hip.so" ]; then
+  D="$TMP/dropin"
+  mkdir -p "$D"
+  cd "$D"
+  "$FC" -cpp -O2 -c "$LIBDIR/fortran/mixsolver_c.f90" -o "$D/mixsolver_c.o"
+  DOBJS=("$D/mixsolver_c.o")
+  for s in "${SRCS[@]}"; do
+    [ "$(basename "$s")" = mod_uns_linsolver.f90 ] && s="$LIBDIR/fortran/mod_uns_linsolver.f90"
+    o="$D/$(basename "${s%.f90}").o"
+    "$FC" -cpp -O2 -c "$s" -o "$o"
+    DOBJS+=("$o")
+  done
+  "$FC" -cpp -O2 -c "$HERE/ref_harness.f90" -o "$D/ref_harness.o"
+  "$FC" -O2 -o "$OUT/ref_dropin" "$D/ref_harness.o" "${DOBJS[@]}" -L"$LIBDIR" -lmixsolver_hip \
+    -Wl,-rpath,'$ORIGIN/../../mixsolver_amd'
+  echo "built $OUT/ref_dropin"
+fi

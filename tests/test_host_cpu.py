@@ -274,3 +274,89 @@ def test_packed_bed_closures():
     np.testing.assert_allclose(zones.kozeny_carman_permeability(eps, 2 * dp), 4 * K, rtol=1e-14)
     f = zones.packed_bed_fields(eps, dp, k_s=1.0)
     assert f["perm_dir"].shape == (4, 3) and np.all(f["perm_dir"][:, 0] == K)
+
+
+REF_PRECISION = "/root/reference/src/common/mod_precision.f90"
+
+
+@pytest.mark.skipif(FLANG is None or not os.path.exists(REF_PRECISION),
+                    reason="needs a Fortran compiler and the reference's mod_precision")
+def test_dropin_module_host_factorizations(tmp_path):
+    """The drop-in mod_uns_linsolver keeps the reference's host helpers:
+    its ilu0_factor / ic0_factor reproduce the reference's factors bit for bit
+    (solvers_spd fixture), and csr_matvec / ilu0_apply / ic0_apply invert
+    them (host only, no device call)."""
+    import subprocess
+    d = np.load(os.path.join(os.path.dirname(__file__), "golden", "solvers_spd.npz"))
+    rp, ci, val = d["row_ptr"].astype(np.int32), d["col_idx"].astype(np.int32), d["val"]
+    n, nnz = len(rp) - 1, len(ci)
+    with open(tmp_path / "in.bin", "wb") as f:
+        np.array([n, nnz], np.int32).tofile(f)
+        rp.tofile(f)
+        ci.tofile(f)
+        val.tofile(f)
+    prog = tmp_path / "t.f90"
+    prog.write_text("""program t
+   use mod_precision, only: dp
+   use mod_uns_linsolver
+   implicit none
+   type(csr_t) :: A, L, U
+   integer :: n, nnz, ierr, u1, u2
+   real(dp), allocatable :: x(:), y(:), z(:)
+   open(newunit=u1, file='in.bin', access='stream', form='unformatted', status='old')
+   read(u1) n, nnz
+   A%nrows = n
+   allocate(A%row_ptr(n+1), A%col_idx(nnz), A%val(nnz), x(n), y(n), z(n))
+   read(u1) A%row_ptr, A%col_idx, A%val
+   close(u1)
+   L = A; call ic0_factor(L, ierr)
+   open(newunit=u2, file='out.bin', access='stream', form='unformatted', status='replace')
+   write(u2) ierr, L%val
+   U = A; call ilu0_factor(U, ierr)
+   write(u2) ierr, U%val
+   x = 1.0_dp
+   call csr_matvec(A, x, y)
+   call ilu0_apply(U, y, z)
+   write(u2) z
+   call ic0_apply(L, y, z)
+   write(u2) z
+   close(u2)
+end program t
+""")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    fdir = os.path.join(root, "mixsolver_amd", "fortran")
+    libdir = os.path.join(root, "mixsolver_amd")
+    for src in (REF_PRECISION, os.path.join(fdir, "mixsolver_c.f90"), os.path.join(fdir, "mod_uns_linsolver.f90")):
+        subprocess.run([FLANG, "-O2", "-c", src], cwd=tmp_path, check=True)
+    subprocess.run([FLANG, "-O2", "t.f90", "mod_precision.o", "mixsolver_c.o", "mod_uns_linsolver.o",
+                    f"-L{libdir}", "-lmixsolver_hip", f"-Wl,-rpath,{libdir}", "-o", "t"], cwd=tmp_path, check=True)
+    subprocess.run(["./t"], cwd=tmp_path, check=True)
+    b = np.fromfile(tmp_path / "out.bin", dtype=np.uint8)
+    o = 0
+
+    def take(dt, k):
+        nonlocal o
+        a = np.frombuffer(b, dtype=dt, count=k, offset=o)
+        o += a.nbytes
+        return a
+    assert take(np.int32, 1)[0] == d["ic0_ierr"] == 0
+    assert np.array_equal(take(np.float64, nnz), d["ic0_L"])
+    assert take(np.int32, 1)[0] == d["ilu0_ierr"] == 0
+    assert np.array_equal(take(np.float64, nnz), d["ilu0_LU"])
+    # for this symmetric matrix (LU)^-1 A 1 is 1 only if the factors are exact; ILU(0) and
+    # IC(0) of a matrix with fill differ from A, so check the solves against dense algebra
+    z_ilu, z_ic = take(np.float64, n), take(np.float64, n)
+    A = np.zeros((n, n))
+    for i in range(n):
+        A[i, ci[rp[i] - 1:rp[i + 1] - 1] - 1] = val[rp[i] - 1:rp[i + 1] - 1]
+    Lu = np.zeros((n, n))
+    for i in range(n):
+        Lu[i, ci[rp[i] - 1:rp[i + 1] - 1] - 1] = d["ilu0_LU"][rp[i] - 1:rp[i + 1] - 1]
+    Lo = np.tril(Lu, -1) + np.eye(n)
+    Up = np.triu(Lu)
+    np.testing.assert_allclose(Lo @ Up @ z_ilu, A @ np.ones(n), rtol=1e-12, atol=1e-12)
+    Lc = np.zeros((n, n))
+    for i in range(n):
+        Lc[i, ci[rp[i] - 1:rp[i + 1] - 1] - 1] = d["ic0_L"][rp[i] - 1:rp[i + 1] - 1]
+    Lc = np.tril(Lc)
+    np.testing.assert_allclose(Lc @ Lc.T @ z_ic, A @ np.ones(n), rtol=1e-12, atol=1e-12)
