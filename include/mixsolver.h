@@ -195,11 +195,14 @@ int ms_solve(int n, const int *row_ptr, const int *col_idx, const double *val,
  * x0 = 0, FP32 inner iterations with FP64 reliable updates to tol (the FP64
  * true relative residual).  Patterns (one per mesh) are registered once and
  * stay resident; a request brings values in the pattern's CSR order and a
- * right-hand side.  The val and b arrays are read when the request is
- * admitted: they must stay valid until its result has been fetched.  All
- * admitted requests are packed into every launch; a result does not depend on
- * which other requests share the device (bit for bit).  The server's device
- * work runs on the library stream from its own worker thread. */
+ * right-hand side, copied into pinned host memory before ms_server_submit
+ * returns.  All admitted requests are packed into every launch; a result
+ * does not depend on which other requests share the device (bit for bit).
+ * The server's worker thread runs the iterations on the library stream and
+ * the set-up of admitted requests (uploads, FP32 copy, preconditioner, coarse
+ * inverse) on a second stream, so set-ups overlap the rounds.  Device and
+ * pinned buffers of finished requests are kept per pattern and reused; the
+ * pool is released when the budget needs room for another pattern. */
 typedef struct ms_server ms_server;
 typedef struct {
   long long hbm_budget;  /* bytes admitted requests may hold; 0 = half of free memory */

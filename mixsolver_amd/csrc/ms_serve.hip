@@ -14,6 +14,12 @@
 // a request's iterates do not depend on which other requests share the
 // launches: a result is the same alone and in a mix, bit for bit.
 //
+// Set-up of admitted requests runs on its own stream and overlaps the
+// rounds; a request joins the iterating set at the first round boundary
+// after its set-up event fired.  Values and right-hand sides are copied into
+// pinned memory at submission; device buffers of finished requests and the
+// pinned stages are pooled per pattern.
+//
 // The iteration is the mixed-precision PCG of ms_linalg.hip (FP32 matrix in
 // difference form, FP32 preconditioner and vectors on r/sigma, FP64 solution
 // and reliable updates) with Jacobi or multicolour DILU and the optional
@@ -21,7 +27,7 @@
 // of iterations; between rounds it applies reliable updates, retires finished
 // requests and admits waiting ones while the device-memory budget and the
 // slot count allow (continuous batching).  All device work of the server is
-// issued by the worker thread on the library stream.
+// issued by the worker thread.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -313,6 +319,15 @@ double now_s() {
 }  // namespace
 
 // ================================================================ host side
+struct Slot;
+
+// A request's values and right-hand side, copied at submission into pinned
+// host memory (val at [0, nnz), b at [nnz, nnz + n)).
+struct Stage {
+  double *p = nullptr;
+  size_t n = 0;
+};
+
 struct Pattern {
   int n = 0;
   long long nnz = 0;
@@ -321,12 +336,16 @@ struct Pattern {
   std::vector<int> o2n, slot_of, col_of_chunk, agg_dev;
   DBuf<int> d_o2n, d_slot_of;
   bool uploaded = false;
+  // device buffers of retired requests and pinned stages, reused by the
+  // next requests on this pattern
+  std::vector<std::unique_ptr<Slot>> pool;
+  std::vector<Stage> stages;
 };
 
 struct Req {
   long long ticket = 0;
   Pattern *P = nullptr;
-  const double *val = nullptr, *b = nullptr;
+  Stage stage;
   double tol = 0.0;
   int maxit = 0;
   double t_submit = 0.0, t_admit = -1.0, t_done = -1.0;
@@ -345,7 +364,12 @@ struct Slot {
   DBuf<float> v32, dg, r, z, p, q, y, Einv;
   BSys hsys{};
   KS hks{};
-  std::vector<double> xdev;
+};
+
+// requests whose set-up runs on the set-up stream; ready when `done` fires
+struct Setup {
+  std::vector<Slot *> slots;
+  hipEvent_t done = nullptr;
 };
 
 struct List {
@@ -445,6 +469,9 @@ class Server {
     d_ks_.alloc(o_.max_active);
     d_sys_.alloc(o_.max_active);
     MS_HIP(hipHostMalloc((void **)&h_ks_, sizeof(KS) * o_.max_active, hipHostMallocDefault));
+    // request set-up (uploads, FP32 copy, preconditioner, coarse inverse)
+    // overlaps the rounds on a second stream of the default priority
+    MS_HIP(hipStreamCreateWithFlags(&setup_stream_, hipStreamNonBlocking));
     for (int i = o_.max_active - 1; i >= 0; i--) free_ids_.push_back(i);
     worker_ = std::thread([this] { loop(); });
   }
@@ -456,6 +483,13 @@ class Server {
     }
     cv_work_.notify_all();
     if (worker_.joinable()) worker_.join();
+    (void)hipStreamSynchronize(setup_stream_);
+    for (Setup &su : setups_) (void)hipEventDestroy(su.done);
+    (void)hipStreamDestroy(setup_stream_);
+    for (auto &P : patterns_)
+      for (Stage &st : P->stages) (void)hipHostFree(st.p);
+    for (auto &kv : reqs_)
+      if (kv.second->stage.p) (void)hipHostFree(kv.second->stage.p);
     if (h_ks_) (void)hipHostFree(h_ks_);
   }
 
@@ -532,20 +566,36 @@ class Server {
     return (int)patterns_.size() - 1;
   }
 
+  // The values and the right-hand side are copied into pinned host memory
+  // here, so the caller may reuse its arrays as soon as submit returns.
   long long submit(int pattern, const double *val, const double *b, double tol, int maxit) {
     if (!val || !b || maxit < 1 || !(tol > 0.0)) fail(MS_EINVAL, "server submit: bad arguments");
     std::unique_ptr<Req> r(new Req());
-    r->val = val;
-    r->b = b;
     r->tol = tol;
     r->maxit = maxit;
     r->t_submit = now_s();
-    long long t;
+    Pattern *P;
     {
       std::lock_guard<std::mutex> lk(mu_);
       if (!err_.empty()) fail(MS_EDEVICE, "server stopped: " + err_);
       if (pattern < 0 || pattern >= (int)patterns_.size()) fail(MS_EINVAL, "server submit: unknown pattern");
-      r->P = patterns_[pattern].get();
+      P = patterns_[pattern].get();
+      if (!P->stages.empty()) {
+        r->stage = P->stages.back();
+        P->stages.pop_back();
+      }
+    }
+    const size_t need = (size_t)P->nnz + (size_t)P->n;
+    if (!r->stage.p) {
+      MS_HIP(hipHostMalloc((void **)&r->stage.p, need * sizeof(double), hipHostMallocDefault));
+      r->stage.n = need;
+    }
+    std::memcpy(r->stage.p, val, (size_t)P->nnz * sizeof(double));
+    std::memcpy(r->stage.p + P->nnz, b, (size_t)P->n * sizeof(double));
+    r->P = P;
+    long long t;
+    {
+      std::lock_guard<std::mutex> lk(mu_);
       t = r->ticket = next_ticket_++;
       pending_.push_back(r.get());
       reqs_[t] = std::move(r);
@@ -602,9 +652,12 @@ class Server {
   std::map<long long, std::unique_ptr<Req>> reqs_;
   long long next_ticket_ = 1, n_sub_ = 0, n_done_ = 0;
   std::atomic<long long> rounds_{0}, launches_{0};
-  std::vector<std::unique_ptr<Slot>> active_;
+  std::vector<std::unique_ptr<Slot>> active_;   // iterating
+  std::vector<std::unique_ptr<Slot>> staged_;   // set-up in flight or done, not yet started
+  std::vector<Setup> setups_;
   std::vector<int> free_ids_;
-  size_t used_ = 0, peak_ = 0;
+  size_t used_ = 0, pooled_ = 0, peak_ = 0;     // bytes held by requests / kept in pattern pools
+  hipStream_t setup_stream_ = nullptr;
   DBuf<KS> d_ks_;
   DBuf<BSys> d_sys_;
   KS *h_ks_ = nullptr;
@@ -636,6 +689,10 @@ class Server {
     }
     P->d_o2n.upload(P->o2n);
     P->d_slot_of.upload(P->slot_of);
+    // these uploads run on the library stream, but the set-up kernels that
+    // read them (scatter, FP32 copy, DILU set-up, coarse operator) run on the
+    // set-up stream: complete them before any set-up is enqueued
+    MS_HIP(hipStreamSynchronize(stream()));
     P->uploaded = true;
   }
 
@@ -689,37 +746,67 @@ class Server {
     return v;
   }
 
-  void start(const std::vector<Req *> &reqs) {
-    std::vector<Slot *> fresh;
-    for (Req *r : reqs) {
-      Pattern *P = r->P;
+  // A slot for a request on P: a pooled one, or new device buffers.  Called
+  // with mu_ held; returns null when the budget does not allow it now.
+  std::unique_ptr<Slot> take_slot(Pattern *P, bool nothing_running) {
+    const size_t need = request_bytes(P);
+    if (!P->pool.empty()) {
+      std::unique_ptr<Slot> S = std::move(P->pool.back());
+      P->pool.pop_back();
+      pooled_ -= S->bytes;
+      used_ += S->bytes;
+      return S;
+    }
+    // make room by releasing pooled buffers of other patterns
+    for (auto &Q : patterns_) {
+      while (used_ + pooled_ + need > (size_t)o_.budget && !Q->pool.empty()) {
+        pooled_ -= Q->pool.back()->bytes;
+        Q->pool.pop_back();
+      }
+    }
+    if (used_ + pooled_ + need > (size_t)o_.budget && !nothing_running) return nullptr;
+    std::unique_ptr<Slot> S(new Slot());
+    S->P = P;
+    S->bytes = need;
+    const int n = P->n;
+    const long long ns = P->pat.nstore;
+    S->v64.alloc(ns);
+    S->v32.alloc(ns);
+    S->tmp.alloc(std::max<long long>(P->nnz + n, n));
+    for (DBuf<double> *d : {&S->x64, &S->b64, &S->r64}) d->alloc(n);
+    for (DBuf<float> *d : {&S->dg, &S->r, &S->z, &S->p, &S->q, &S->y}) d->alloc(n);
+    if (P->k > 0) {
+      const long long kk = (long long)P->k * P->k;
+      S->E.alloc(kk);
+      S->Einv.alloc(kk);
+      S->nu.alloc(P->k);
+    }
+    used_ += need;
+    return S;
+  }
+
+  // Enqueue the set-up of the slots on the set-up stream: values and b from
+  // the pinned stage into the device layout, FP32 copy, preconditioner
+  // diagonal, coarse operator and its inverse, solver state.
+  void begin_setup(std::vector<std::unique_ptr<Slot>> &slots) {
+    hipStream_t st = setup_stream_;
+    Setup su;
+    for (auto &S : slots) {
+      Req *r = S->req;
+      Pattern *P = S->P;
       upload_pattern(P);
-      std::unique_ptr<Slot> S(new Slot());
-      S->req = r;
-      S->P = P;
-      S->id = free_ids_.back();
-      free_ids_.pop_back();
       const int n = P->n;
       const long long ns = P->pat.nstore;
-      S->v64.alloc(ns);
-      S->v32.alloc(ns);
-      S->tmp.alloc(std::max<long long>(P->nnz, n));
-      for (DBuf<double> *d : {&S->x64, &S->b64, &S->r64}) d->alloc(n);
-      for (DBuf<float> *d : {&S->dg, &S->r, &S->z, &S->p, &S->q, &S->y}) d->alloc(n);
-      S->bytes = request_bytes(P);
-      hipStream_t st = stream();
       const Mat A = matof(&P->pat);
-      // values into the device layout; padding slots zero
-      MS_HIP(hipMemsetAsync(S->v64.p, 0, ns * sizeof(double), st));
-      MS_HIP(hipMemcpyAsync(S->tmp.p, r->val, P->nnz * sizeof(double), hipMemcpyHostToDevice, st));
+      MS_HIP(hipMemcpyAsync(S->tmp.p, r->stage.p, ((size_t)P->nnz + n) * sizeof(double), hipMemcpyHostToDevice,
+                            st));
+      hipLaunchKernelGGL(k_fill64, dim3(nblocks(ns)), dim3(kBlock), 0, st, ns, S->v64.p, 0.0);   // padding
       hipLaunchKernelGGL(k_s_scatter, dim3(nblocks(P->nnz)), dim3(kBlock), 0, st, P->nnz, P->d_slot_of.p, S->tmp.p,
                          S->v64.p);
-      MS_HIP(hipStreamSynchronize(st));   // tmp is reused for b
-      MS_HIP(hipMemcpyAsync(S->tmp.p, r->b, n * sizeof(double), hipMemcpyHostToDevice, st));
-      hipLaunchKernelGGL(k_s_perm, dim3(nblocks(n)), dim3(kBlock), 0, st, n, P->d_o2n.p, S->tmp.p, S->b64.p);
-      MS_HIP(hipMemsetAsync(S->x64.p, 0, n * sizeof(double), st));
+      hipLaunchKernelGGL(k_s_perm, dim3(nblocks(n)), dim3(kBlock), 0, st, n, P->d_o2n.p, S->tmp.p + P->nnz,
+                         S->b64.p);
+      hipLaunchKernelGGL(k_fill64, dim3(nblocks(n)), dim3(kBlock), 0, st, (long long)n, S->x64.p, 0.0);
       hipLaunchKernelGGL(k_mat32_diff, dim3(nblocks(n)), dim3(kBlock), 0, st, n, A, S->v64.p, S->v32.p);
-      // preconditioner diagonal (FP64, rounded once)
       if (jac()) {
         hipLaunchKernelGGL(k_extract_diag<double>, dim3(nblocks(n)), dim3(kBlock), 0, st, n, A, S->v64.p,
                            S->r64.p);
@@ -735,9 +822,6 @@ class Server {
       const Coarse &C = P->pat.coarse;
       if (P->k > 0) {
         const long long kk = (long long)P->k * P->k;
-        S->E.alloc(kk);
-        S->Einv.alloc(kk);
-        S->nu.alloc(P->k);
         const int nE = (int)C.e_key.size();
         hipLaunchKernelGGL(k_fill64, dim3(nblocks(kk)), dim3(kBlock), 0, st, kk, S->E.p, 0.0);
         hipLaunchKernelGGL(k_coarse_E, dim3(nblocks(nE, kBlock / 64)), dim3(kBlock), 0, st, nE, C.d_e_ptr.p,
@@ -758,11 +842,52 @@ class Server {
                      P->k > 0 ? S->Einv.p : nullptr, P->k > 0 ? S->nu.p : nullptr};
       MS_HIP(hipMemcpyAsync(d_sys_.p + S->id, &S->hsys, sizeof(BSys), hipMemcpyHostToDevice, st));
       r->t_admit = now_s();
-      fresh.push_back(S.get());
-      std::lock_guard<std::mutex> lk(mu_);
-      active_.push_back(std::move(S));
+      su.slots.push_back(S.get());
+      staged_.push_back(std::move(S));
     }
-    // batched start of the admitted requests
+    MS_HIP(hipEventCreateWithFlags(&su.done, hipEventDisableTiming));
+    MS_HIP(hipEventRecord(su.done, st));
+    setups_.push_back(su);
+  }
+
+  // Set-ups whose event fired (or, with wait, the oldest one): their slots
+  // are returned and their pinned stages go back to the pattern pools.
+  std::vector<Slot *> finished_setups(bool wait) {
+    std::vector<Slot *> ready;
+    for (size_t i = 0; i < setups_.size();) {
+      hipError_t e = (wait && i == 0) ? hipEventSynchronize(setups_[i].done) : hipEventQuery(setups_[i].done);
+      if (e == hipErrorNotReady) {
+        i++;
+        continue;
+      }
+      MS_HIP(e);
+      MS_HIP(hipEventDestroy(setups_[i].done));
+      for (Slot *S : setups_[i].slots) ready.push_back(S);
+      setups_.erase(setups_.begin() + (long)i);
+      wait = false;
+    }
+    if (!ready.empty()) {
+      std::lock_guard<std::mutex> lk(mu_);
+      for (Slot *S : ready) {
+        S->P->stages.push_back(S->req->stage);
+        S->req->stage = Stage{};
+      }
+    }
+    return ready;
+  }
+
+  // batched start of set-up requests on the compute stream
+  void start(const std::vector<Slot *> &fresh) {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      for (Slot *S : fresh)
+        for (size_t i = 0; i < staged_.size(); i++)
+          if (staged_[i].get() == S) {
+            active_.push_back(std::move(staged_[i]));
+            staged_.erase(staged_.begin() + (long)i);
+            break;
+          }
+    }
     aux_.build(fresh);
     ensure_partials(aux_.items.size());
     launch(k_b_sumsq, aux_, part64_.p);
@@ -775,7 +900,6 @@ class Server {
     }
     launch(k_b_p, aux_, jac());
     MS_CHECK_LAUNCH();
-    MS_HIP(hipStreamSynchronize(stream()));
     dirty_ = true;
   }
 
@@ -829,7 +953,7 @@ class Server {
       MS_CHECK_LAUNCH();
       read_states();
     }
-    // retire finished requests
+    // retire finished requests; their buffers go to the pattern pool
     std::vector<Slot *> fin_slots;
     for (Slot *s : slots) {
       const KS &h = h_ks_[s->id];
@@ -857,15 +981,20 @@ class Server {
         r->t_done = t;
         r->done = true;
         n_done_++;
-        used_ -= s->bytes;
         free_ids_.push_back(s->id);
       }
-      active_.erase(std::remove_if(active_.begin(), active_.end(),
-                                   [&](const std::unique_ptr<Slot> &p) {
-                                     return std::find(fin_slots.begin(), fin_slots.end(), p.get()) !=
-                                            fin_slots.end();
-                                   }),
-                    active_.end());
+      for (size_t i = 0; i < active_.size();) {
+        Slot *s = active_[i].get();
+        if (std::find(fin_slots.begin(), fin_slots.end(), s) == fin_slots.end()) {
+          i++;
+          continue;
+        }
+        used_ -= s->bytes;
+        pooled_ += s->bytes;
+        s->req = nullptr;
+        s->P->pool.push_back(std::move(active_[i]));
+        active_.erase(active_.begin() + (long)i);
+      }
     }
     cv_done_.notify_all();
     dirty_ = true;
@@ -875,22 +1004,30 @@ class Server {
     try {
       MS_HIP(hipSetDevice(device_));
       while (true) {
-        std::vector<Req *> admit;
+        std::vector<std::unique_ptr<Slot>> admit;
         {
           std::unique_lock<std::mutex> lk(mu_);
-          cv_work_.wait(lk, [&] { return stop_ || !pending_.empty() || !active_.empty(); });
+          cv_work_.wait(lk, [&] { return stop_ || !pending_.empty() || !active_.empty() || !staged_.empty(); });
           if (stop_) break;
-          while (!pending_.empty() && (int)(active_.size() + admit.size()) < o_.max_active) {
+          while (!pending_.empty() && (int)(active_.size() + staged_.size() + admit.size()) < o_.max_active) {
             Req *r = pending_.front();
-            const size_t need = request_bytes(r->P);
-            if (used_ + need > (size_t)o_.budget && !(active_.empty() && admit.empty())) break;
+            const bool idle = active_.empty() && staged_.empty() && admit.empty();
+            std::unique_ptr<Slot> S = take_slot(r->P, idle);
+            if (!S) break;
             pending_.pop_front();
-            used_ += need;
             peak_ = std::max(peak_, used_);
-            admit.push_back(r);
+            S->req = r;
+            S->P = r->P;
+            S->id = free_ids_.back();
+            free_ids_.pop_back();
+            admit.push_back(std::move(S));
           }
         }
-        if (!admit.empty()) start(admit);
+        if (!admit.empty()) begin_setup(admit);
+        // requests whose set-up finished join at this round boundary; with
+        // nothing iterating, wait for the oldest set-up instead of spinning
+        std::vector<Slot *> ready = finished_setups(active_.empty());
+        if (!ready.empty()) start(ready);
         if (active_.empty()) continue;
         if (dirty_) {
           all_.build(active_ptrs());

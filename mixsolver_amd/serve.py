@@ -59,14 +59,14 @@ class Server:
         return pid.value
 
     def submit(self, pattern, val, b, tol=1e-10, maxit=20000):
-        """Queue a solve; val and b are read at admission, so they are kept
-        referenced here until the result is fetched."""
+        """Queue a solve; val (pattern CSR order) and b are copied before
+        this returns."""
         val = np.ascontiguousarray(val, np.float64)
         b = np.ascontiguousarray(b, np.float64)
         t = C.c_longlong(0)
         check(_lib.lib().ms_server_submit(self.h, int(pattern), dp(val), dp(b), float(tol), int(maxit),
                                           C.byref(t)))
-        self._keep[t.value] = (val, b, self._n[pattern])
+        self._keep[t.value] = (None, None, self._n[pattern])
         return t.value
 
     def result(self, ticket, timeout_ms=-1.0):
