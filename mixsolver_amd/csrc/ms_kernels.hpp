@@ -79,9 +79,10 @@ __device__ __forceinline__ T row_apply(int r, const Mat &A, const T *val, const 
   if constexpr (sizeof(T) == 8) {
     return row_dot(r, A, val, x);
   } else {
-    // every slot of the slice width contributes a_k (x_col - x_i): padding
-    // (col = r, a = 0) and the diagonal slot (col = r) give zero, so the loop
-    // runs the slice width, uniform across the wavefront, without branches
+    // every slot of the slice width contributes: a_k (x_col - x_i) off the
+    // diagonal, s_i x_i in the diagonal slot (col = r, holding the row sum)
+    // and 0 in padding (col = r, a = 0); the loop runs the slice width,
+    // uniform across the wavefront, and needs no diagonal position
     const int s = r >> 6;
     const long long o0 = A.off[s];
     const int w = (int)((A.off[s + 1] - o0) >> 6);
@@ -91,9 +92,10 @@ __device__ __forceinline__ T row_apply(int r, const Mat &A, const T *val, const 
 #pragma unroll 4
     for (int k = 0; k < w; k++) {
       const long long q = base + (long long)k * kSlice;
-      acc = acc + val[q] * (x[col_f32(A, q, r, s)] - xi);
+      const int c = col_f32(A, q, r, s);
+      acc = acc + val[q] * (c == r ? xi : x[c] - xi);
     }
-    return val[base + (long long)A.diagk[r] * kSlice] * xi + acc;
+    return acc;
   }
 }
 
