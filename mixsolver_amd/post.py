@@ -7,6 +7,22 @@ cell centroids (weight 1/(d^2 + 1e-12)); the u-velocity on x = 0.5, z = 0.5
 at the 17 stations of Ghia, Ghia & Shin (1982, JCP 48:387-411, Tables I/II)
 is compared with the published values, and the relative L2 error over the
 15 interior stations is returned.
+
+``bj_two_layer`` and ``bj_profile`` restate the reference's check of its
+Beavers-Joseph fluid/porous channel (cluster_cases/bj_porous/plot_bj.py:
+reference() :62-92, extraction :128-146): a fluid layer over a Brinkman
+porous layer of equal height H, driven by a body force fb, has the closed
+form
+
+  fluid  (0 <= y <= H):  u_f = -fb y^2 / (2 mu) + a1 y + a2,   u_f(H)  = 0
+  porous (-H <= y <= 0): u_p = u_D + E e^(y/lam) + F e^(-y/lam), u_p(-H) = 0
+
+with lam = sqrt(K), u_D = K fb / mu, equal shear stress at the interface
+(a1 = (E - F) / lam) and, for alpha > 0, the stress-jump condition
+a1 = (alpha / lam)(u_f(0) - u_p(0)), else continuity u_f(0) = u_p(0).
+The numerical check takes the u_x column at the middle x station: the first
+fluid cell above the interface (uF1) and the mean over the porous rows at
+least lam away from the bed wall and the interface (uD).
 """
 from __future__ import annotations
 
@@ -38,3 +54,49 @@ def ghia_compare(xc, u, re):
     err = uq - uref
     l2 = np.sqrt((err[1:-1] ** 2).sum() / max((uref[1:-1] ** 2).sum(), np.finfo(float).tiny))
     return l2, uq
+
+
+def bj_two_layer(alpha, H, K, mu, fb):
+    """(u_f(y), u_p(y), u_f(0)) of the two-layer closed form, y measured
+    from the interface."""
+    lam = np.sqrt(K)
+    uD = K * fb / mu
+    A = np.array([[H, 1.0, 0.0, 0.0],
+                  [0.0, 0.0, np.exp(-H / lam), np.exp(H / lam)],
+                  [1.0, 0.0, -1.0 / lam, 1.0 / lam],
+                  [1.0, -alpha / lam, alpha / lam, alpha / lam] if alpha > 0.0 else [0.0, 1.0, -1.0, -1.0]])
+    b = np.array([fb * H * H / (2.0 * mu), -uD, 0.0, -alpha / lam * uD if alpha > 0.0 else uD])
+    a1, a2, E, F = np.linalg.solve(A, b)
+
+    def uf(y):
+        y = np.asarray(y, np.float64)
+        return -fb * y * y / (2.0 * mu) + a1 * y + a2
+
+    def up(y):
+        y = np.asarray(y, np.float64)
+        return uD + E * np.exp(y / lam) + F * np.exp(-y / lam)
+
+    return uf, up, a2
+
+
+def bj_classical_slip(alpha, H, K, mu, fb):
+    """Interface speed of the classical Beavers-Joseph slip condition over a
+    pure-Darcy porous side (for comparison only)."""
+    lam = np.sqrt(K)
+    uD = K * fb / mu
+    return (fb * H * H / (2.0 * mu) + alpha * (H / lam) * uD) / (1.0 + alpha * H / lam)
+
+
+def bj_profile(xc, u, lx, ly, nx, ny, K):
+    """u_x column at x station nx // 2 of an nx x ny channel of size lx x ly
+    (bed in the lower half): (column, uF1, uD)."""
+    xc = np.asarray(xc).reshape(-1, 3)
+    ux = np.asarray(u).reshape(-1, 3)[:, 0]
+    dx, dy = lx / nx, ly / ny
+    ix = np.round((xc[:, 0] - dx / 2) / dx).astype(int)
+    iy = np.round((xc[:, 1] - dy / 2) / dy).astype(int)
+    sel = ix == nx // 2
+    col = np.array([ux[sel & (iy == j)].mean() for j in range(ny)])
+    lam = np.sqrt(K)
+    r0, r1 = int(np.ceil(lam / dy)), ny // 2 - int(np.ceil(lam / dy))
+    return col, float(col[ny // 2]), float(col[r0:r1].mean())

@@ -456,6 +456,50 @@ def cavity_known_answer():
     return m, ctl, int(ier), float(np.linalg.norm(u, axis=1).max()), float(l2)
 
 
+REF_BJ = "/root/reference/cluster_cases/bj_porous"
+BJ_README = dict(  # cluster_cases/bj_porous/README.md:51-54 (table), :65 (iterations), :69 (it=1 line)
+    readme_alpha=np.array([0.0, 1.0, 2.0]),
+    readme_uF1=np.array([8.5512e-03, 1.1768e-02, 1.0038e-02]),
+    readme_uF1_ref=np.array([8.4977e-03, 1.2117e-02, 1.0447e-02]),
+    readme_uD=np.array([2.9946e-03, 2.9180e-03, 2.9592e-03]),
+    readme_uD_ref=np.float64(2.7086e-03),
+    readme_uB_class=np.array([3.3857e-02, 7.9000e-03, 5.5402e-03]),
+    readme_rms_pct=np.array([0.22, 0.94, 1.20]),
+    readme_iters=np.array([754, 882, 809]),
+    readme_it1_a1=np.array([3.110e-18, 2.120e-04, 1.36e-09]))  # mass-imbal, du_max, lin-res
+
+
+def bj_known_answer():
+    """The reference's shipped Beavers-Joseph fluid/porous channel (100x40x1
+    hex mesh, upper half fluid, lower half a porous bed tagged 'VC:porous' in
+    the mesh; controls variants/bj_a{0,1,2}.control: continuity and stress
+    jump alpha = 1, 2).  Its README publishes, per alpha, the converged
+    iteration count, the first-fluid-cell and bed velocities uF1 / uD and
+    the two-layer closed form; the harness re-runs the reference's
+    simple_run to convergence (outer_max 2000) and the fixture keeps its
+    velocity field, iteration count and it=1 log line beside the published
+    numbers."""
+    import re
+    m = cas.read_cas(os.path.join(REF_BJ, "unMesh.cas"))
+    zk = {5: "symmetry", 6: "wall", 7: "wall", 8: "pressure-outlet", 9: "pressure-outlet"}
+    out = {}
+    for a in range(3):
+        ctl = open(os.path.join(REF_BJ, "variants", f"bj_a{a}.control")).read()
+        with tempfile.TemporaryDirectory() as td:
+            cas.write_cas(os.path.join(td, "m.cas"), m, zk)
+            open(os.path.join(td, "c.control"), "w").write(ctl)
+            log = run(["m.cas", "c.control", "simple", "out.bin", "", "2000"], td)
+            r = Reader(os.path.join(td, "out.bin"))
+            ier = r.take(np.int32, 1)[0]
+            u = r.take(np.float64, 3 * m.ncells).reshape(m.ncells, 3)
+        it = int(re.search(r"CONVERGED after\s+(\d+)", log).group(1))
+        l1 = re.search(r"it=\s+1\s+mass-imbal=\s*(\S+)\s+du_max=\s*(\S+).*lin-it=\s*(\d+)\s+lin-res=\s*(\S+)", log)
+        out.update({f"control_a{a}": ctl, f"ref_ier_a{a}": np.int64(ier), f"ref_u_a{a}": u,
+                    f"ref_iters_a{a}": np.int64(it),
+                    f"ref_it1_a{a}": np.array([float(l1.group(1)), float(l1.group(2)), float(l1.group(4))])})
+    return m, out
+
+
 def main():
     if not os.path.exists(HARNESS):
         sys.exit("build the reference harness first: oracle/build_ref.sh")
@@ -498,6 +542,11 @@ def main():
         save("cavity_re1000_known_answer", control=ctlk, nsteps=np.int64(1000), **mesh_arrays(mk),
              out_ier=np.int64(ierk), ref_umax=umax, ref_ghia_l2=l2,
              readme_umax=0.911512, readme_ghia_l2=0.0289706)
+
+    # 3d. the reference's shipped Beavers-Joseph porous channel and its published answers
+    if os.path.isdir(REF_BJ):
+        mb, ob = bj_known_answer()
+        save("bj_porous_known_answer", **mesh_arrays(mb), **ob, **BJ_README)
 
     # 4. jittered prism O-grid, PISO with IC0, blended convection, far field
     mp = meshgen.cylinder_prisms(ntheta=24, nr=6, r_out=4.0, layers=2, lz=0.2, jitter=0.15, seed=200)
@@ -573,6 +622,9 @@ def main():
 if __name__ == "__main__":
     if sys.argv[1:] == ["--porous"]:
         porous_fixtures()
+    elif sys.argv[1:] == ["--bj"]:
+        mb, ob = bj_known_answer()
+        save("bj_porous_known_answer", **mesh_arrays(mb), **ob, **BJ_README)
     else:
         main()
         porous_fixtures()

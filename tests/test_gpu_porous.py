@@ -139,3 +139,45 @@ def test_per_cell_fields_equal_zone_entries():
             assert np.array_equal(x, y)
     for x, y in zip(a.solid_temperature_assembly(), b.solid_temperature_assembly()):
         assert np.array_equal(x, y)
+
+
+# ------------------------------------------- Beavers-Joseph published answers
+BJ_GEOM = dict(lx=0.10, ly=0.010, nx=100, ny=40, K=1.0e-6)
+
+
+@pytest.mark.parametrize("alpha", [0, 1, 2])
+@pytest.mark.parametrize("path", ["fp64", "fast"])
+def test_bj_published_answer(alpha, path):
+    """The reference's shipped Beavers-Joseph channel (100x40x1, bed typed by
+    its 'VC:porous' tag; continuity and stress jump alpha = 1, 2) solved to
+    convergence by device SIMPLE.  FP64 path (the control's IC0 PCG): same
+    iteration count as the reference to +-1 (dot-product order), velocity
+    rel-L2 <= 1e-8 and the README's uF1 / uD to the printed digits.  Fast
+    path (multicolour DILU, FP32 inner iterations with FP64 refinement,
+    coarse space): the same steady state within the outer tolerance's reach,
+    rel-L2 <= 2e-6 and uF1 / uD within 1e-6 relative of the reference's."""
+    from mixsolver_amd.post import bj_profile
+    d = load_golden("bj_porous_known_answer")
+    ctrl, groups = parse_control(str(d[f"control_a{alpha}"]), warn=False)
+    if path == "fast":
+        for k, v in dict(gpu_ppe_precond=2, gpu_mom_precond=2, gpu_precision=1, gpu_coarse=16).items():
+            setattr(ctrl, k, v)
+    cs = Case(mesh_from(d), ctrl, groups)
+    ier, it = cs.simple_run(2000)
+    assert ier == 0
+    u = cs.get("u").reshape(-1, 3)
+    ref = d[f"ref_u_a{alpha}"]
+    xc = cs.get("xc").reshape(-1, 3)
+    _, uF1, uD = bj_profile(xc, u, **BJ_GEOM)
+    _, rF1, rD = bj_profile(xc, ref, **BJ_GEOM)
+    err = rel_l2(u, ref)
+    print(f"bj alpha={alpha} {path}: it {it} (ref {int(d[f'ref_iters_a{alpha}'])}) rel-L2 {err:.3e} "
+          f"uF1 {uF1:.8e} ({rF1:.8e}) uD {uD:.8e} ({rD:.8e})", flush=True)
+    if path == "fp64":
+        assert abs(it - int(d[f"ref_iters_a{alpha}"])) <= 1
+        assert err <= 1e-8
+        assert float("%.4e" % uF1) == d["readme_uF1"][alpha]
+        assert float("%.4e" % uD) == d["readme_uD"][alpha]
+    else:
+        assert err <= 2e-6
+        assert abs(uF1 - rF1) <= 1e-6 * abs(rF1) and abs(uD - rD) <= 1e-6 * abs(rD)

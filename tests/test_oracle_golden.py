@@ -188,3 +188,90 @@ def test_porous_driver_bitexact(name, drv):
     assert ier == int(d[f"{drv}_ier"]) == 0
     for k in ("u", "p", "flux", "T", "T_s"):
         assert np.array_equal(oc.get(k).ravel(), d[f"{drv}_{k}"].ravel()), k
+
+
+# ------------------------------------------- Beavers-Joseph published answers
+BJ_GEOM = dict(lx=0.10, ly=0.010, nx=100, ny=40, K=1.0e-6)
+BJ_PHYS = dict(H=0.005, K=1.0e-6, mu=1.846e-5, fb=0.05)
+
+
+def _bj_xc(d):
+    ctrl, groups = parse_control(str(d["control_a1"]), warn=False)
+    return pyoracle.OracleCase(mesh_from(d), ctrl, groups).geometry()["xc"].reshape(-1, 3)
+
+
+def _printed(v, fmt="%.4e"):
+    return float(fmt % v)
+
+
+def test_bj_two_layer_closed_form_published():
+    """The two-layer closed form (fluid over Brinkman bed, stress jump or
+    continuity at the interface) reproduces the reference case's published
+    uF1_ref, uD_ref and classical-slip columns to the printed digits, and
+    satisfies its own ODEs and interface conditions: wall no-slip on both
+    sides, equal shear stress, the alpha-weighted velocity jump."""
+    from mixsolver_amd.post import bj_classical_slip, bj_two_layer
+    d = load_golden("bj_porous_known_answer")
+    H, K, mu, fb = (BJ_PHYS[k] for k in ("H", "K", "mu", "fb"))
+    lam, dy = np.sqrt(K), 0.010 / 40
+    assert _printed(K * fb / mu) == float(d["readme_uD_ref"])
+    for i, a in enumerate(d["readme_alpha"]):
+        uf, up, uB = bj_two_layer(a, H, K, mu, fb)
+        assert _printed(uf(dy / 2)) == d["readme_uF1_ref"][i]
+        assert _printed(bj_classical_slip(a, H, K, mu, fb)) == d["readme_uB_class"][i]
+        h = 1e-5
+        # fluid: mu u'' + fb = 0; bed: mu u'' - mu/K u + fb = 0 (central differences)
+        y = 0.3 * H
+        assert abs(mu * (uf(y + h) - 2 * uf(y) + uf(y - h)) / h ** 2 + fb) <= 1e-6 * fb
+        y = -0.3 * lam
+        assert abs(mu * (up(y + h) - 2 * up(y) + up(y - h)) / h ** 2 - mu / K * up(y) + fb) <= 1e-4 * fb
+        assert abs(uf(H)) <= 1e-15 and abs(up(-H)) <= 1e-15
+        h = 1e-7
+        duf = (uf(h) - uf(-h)) / (2 * h)
+        dup = (up(h) - up(-h)) / (2 * h)
+        assert abs(duf - dup) <= 1e-6 * abs(duf)
+        assert uB == pytest.approx(float(uf(0.0)), rel=1e-14)
+        if a > 0:
+            assert duf == pytest.approx(a / lam * (uf(0.0) - up(0.0)), rel=1e-6)
+        else:
+            assert float(uf(0.0)) == pytest.approx(float(up(0.0)), rel=1e-12)
+
+
+def test_bj_known_answer_fixture_matches_published():
+    """The reference's shipped Beavers-Joseph channel, re-run to convergence
+    by the reference's own simple_run, gives its README's numbers: iteration
+    counts 754 / 882 / 809, the it=1 log line of alpha = 1, and per alpha the
+    uF1 / uD columns and the profile RMS error against the closed form."""
+    from mixsolver_amd.post import bj_profile, bj_two_layer
+    d = load_golden("bj_porous_known_answer")
+    xc = _bj_xc(d)
+    assert np.array_equal(d["readme_it1_a1"], d["ref_it1_a1"])
+    dy = 0.010 / 40
+    yc = (np.arange(40) + 0.5) * dy
+    for i in range(3):
+        assert int(d[f"ref_ier_a{i}"]) == 0
+        assert int(d[f"ref_iters_a{i}"]) == int(d["readme_iters"][i])
+        col, uF1, uD = bj_profile(xc, d[f"ref_u_a{i}"], **BJ_GEOM)
+        assert _printed(uF1) == d["readme_uF1"][i]
+        assert _printed(uD) == d["readme_uD"][i]
+        uf, up, _ = bj_two_layer(d["readme_alpha"][i], **BJ_PHYS)
+        yrel = yc - BJ_PHYS["H"]
+        uref = np.where(yrel >= 0.0, uf(np.maximum(yrel, 0.0)), up(np.minimum(yrel, 0.0)))
+        rms = np.sqrt(np.mean((col - uref) ** 2)) / uref.max() * 100
+        assert float("%.2f" % rms) == d["readme_rms_pct"][i]
+
+
+def test_bj_oracle_reproduces_reference_run():
+    """End to end on the shipped mesh: the oracle's serial SIMPLE driver with
+    the stress-jump control (alpha = 1, IC0 PCG, limited second-order upwind,
+    zone typed by its 'VC:porous' tag) stops at the same iteration as the
+    reference and leaves the same velocity field, bit for bit."""
+    d = load_golden("bj_porous_known_answer")
+    ctrl, groups = parse_control(str(d["control_a1"]), warn=False)
+    oc = pyoracle.OracleCase(mesh_from(d), ctrl, groups)
+    ier, it = oc.simple_run(2000, mpi=False)
+    assert ier == 0 and it == int(d["ref_iters_a1"])
+    assert np.array_equal(oc.get("u").reshape(-1, 3), d["ref_u_a1"])
+    h = oc.history()
+    assert "%.3E" % h[0, 0] == "%.3E" % d["ref_it1_a1"][0]
+    assert "%.2E" % h[0, 5] == "%.2E" % d["ref_it1_a1"][2]
