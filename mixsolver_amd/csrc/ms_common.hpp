@@ -75,6 +75,8 @@ struct DBuf {
 
 constexpr int kBlock = 256;   // 4 wave64 per workgroup
 constexpr int kSlice = 64;    // SELL slice height = one wavefront
+constexpr int kSegThreads = 1024;   // workgroup of a run of narrow levels
+constexpr int kNarrowRows = 2048;   // widest level such a run takes
 
 inline int nblocks(long long n, int b = kBlock) { return (int)((n + b - 1) / b); }
 
@@ -125,7 +127,15 @@ struct SellPattern {
   bool has16 = false;
   // level schedules (natural-order triangular sweeps / factorizations)
   std::vector<int> lev_ptr_f, lev_ptr_b;   // rows grouped per level
-  DBuf<int> d_lev_rows_f, d_lev_rows_b;
+  DBuf<int> d_lev_rows_f, d_lev_rows_b, d_lev_ptr_f, d_lev_ptr_b;
+  // launch plan of a sweep: runs of consecutive narrow levels (at most
+  // kNarrowRows rows each) go to one workgroup that steps through them with
+  // a barrier between levels; every wider level is a launch of its own
+  struct LevSeg {
+    int l0, l1;
+    bool narrow;
+  };
+  std::vector<LevSeg> seg_f, seg_b;
   // colour classes: rows of colour c are [col_ptr[c], col_ptr[c+1])
   std::vector<int> color_ptr;
   bool has_levels = false;

@@ -273,12 +273,9 @@ __global__ void k_jacobi(int n, const T *in, const T *dg, T *out, const int *don
   if (i < n) out[i] = in[i] / dg[i];
 }
 
-// natural IC(0), one level of rows (mod_uns_linsolver.f90:328-365)
+// natural IC(0) of row i, its lower neighbours final (mod_uns_linsolver.f90:328-365)
 template <class T>
-__global__ void k_ic0_factor(const int *rows, int nr, Mat A, T *F, int *flag) {
-  int t = blockIdx.x * kBlock + threadIdx.x;
-  if (t >= nr) return;
-  const int i = rows[t];
+__device__ void ic0_factor_row(int i, const Mat &A, T *F, int *flag) {
   const long long bi = A.off[i >> 6] + (i & 63);
   const int dpi = A.diagk[i];
   for (int kp = 0; kp < dpi; kp++) {
@@ -314,12 +311,9 @@ __global__ void k_ic0_factor(const int *rows, int nr, Mat A, T *F, int *flag) {
   F[qd] = tsqrt(s);
 }
 
-// natural ILU(0), IKJ variant, one level of rows (mod_uns_linsolver.f90:79-115)
+// natural ILU(0) of row i, IKJ variant (mod_uns_linsolver.f90:79-115)
 template <class T>
-__global__ void k_ilu0_factor(const int *rows, int nr, Mat A, T *F, int *flag) {
-  int t = blockIdx.x * kBlock + threadIdx.x;
-  if (t >= nr) return;
-  const int i = rows[t];
+__device__ void ilu0_factor_row(int i, const Mat &A, T *F, int *flag) {
   const long long bi = A.off[i >> 6] + (i & 63);
   const int dpi = A.diagk[i], leni = A.rowlen[i];
   for (int kk = 0; kk < dpi; kk++) {
@@ -350,12 +344,9 @@ __global__ void k_ilu0_factor(const int *rows, int nr, Mat A, T *F, int *flag) {
   }
 }
 
+// forward substitution of row i (unit lower for ILU(0), L of IC(0))
 template <class T>
-__global__ void k_lower_level(const int *rows, int nr, Mat A, const T *F, const T *r, T *z, int ic0, const int *done) {
-  if (*done) return;
-  int t = blockIdx.x * kBlock + threadIdx.x;
-  if (t >= nr) return;
-  const int i = rows[t];
+__device__ void lower_row(int i, const Mat &A, const T *F, const T *r, T *z, int ic0) {
   const long long bi = A.off[i >> 6] + (i & 63);
   const int dpi = A.diagk[i];
   T s = r[i];
@@ -366,13 +357,9 @@ __global__ void k_lower_level(const int *rows, int nr, Mat A, const T *F, const 
   z[i] = ic0 ? s / F[bi + (long long)dpi * kSlice] : s;
 }
 
+// backward substitution of row i (U of ILU(0), L^T of IC(0) via tpos)
 template <class T>
-__global__ void k_upper_level(const int *rows, int nr, Mat A, const long long *tpos, const T *F, T *z, int ic0,
-                              const int *done) {
-  if (*done) return;
-  int t = blockIdx.x * kBlock + threadIdx.x;
-  if (t >= nr) return;
-  const int i = rows[t];
+__device__ void upper_row(int i, const Mat &A, const long long *tpos, const T *F, T *z, int ic0) {
   const long long bi = A.off[i >> 6] + (i & 63);
   const int dpi = A.diagk[i], len = A.rowlen[i];
   T s = z[i];
@@ -388,6 +375,71 @@ __global__ void k_upper_level(const int *rows, int nr, Mat A, const long long *t
     }
   }
   z[i] = s / F[bi + (long long)dpi * kSlice];
+}
+
+// one wide level: a thread per row
+template <class T>
+__global__ void k_factor_level(const int *rows, int nr, Mat A, T *F, int ic0, int *flag) {
+  int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= nr) return;
+  if (ic0) ic0_factor_row(rows[t], A, F, flag);
+  else ilu0_factor_row(rows[t], A, F, flag);
+}
+
+template <class T>
+__global__ void k_lower_level(const int *rows, int nr, Mat A, const T *F, const T *r, T *z, int ic0, const int *done) {
+  if (*done) return;
+  int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t < nr) lower_row(rows[t], A, F, r, z, ic0);
+}
+
+template <class T>
+__global__ void k_upper_level(const int *rows, int nr, Mat A, const long long *tpos, const T *F, T *z, int ic0,
+                              const int *done) {
+  if (*done) return;
+  int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t < nr) upper_row(rows[t], A, tpos, F, z, ic0);
+}
+
+// a run of narrow levels [l0, l1) in one workgroup: rows of a level are
+// spread over the threads, a barrier orders the levels (the workgroup shares
+// one L1, so rows of the next level see the values just stored).  Every
+// thread reaches every barrier: the level loop is uniform, only the row loop
+// inside a level depends on the thread.
+template <class T>
+__global__ __launch_bounds__(kSegThreads) void k_factor_seg(const int *lev_ptr, int l0, int l1, const int *rows,
+                                                             Mat A, T *F, int ic0, int *flag) {
+  for (int l = l0; l < l1; l++) {
+    const int a = lev_ptr[l], b = lev_ptr[l + 1];
+    for (int t = a + (int)threadIdx.x; t < b; t += kSegThreads) {
+      if (ic0) ic0_factor_row(rows[t], A, F, flag);
+      else ilu0_factor_row(rows[t], A, F, flag);
+    }
+    __syncthreads();
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(kSegThreads) void k_lower_seg(const int *lev_ptr, int l0, int l1, const int *rows, Mat A,
+                                                            const T *F, const T *r, T *z, int ic0, const int *done) {
+  if (*done) return;
+  for (int l = l0; l < l1; l++) {
+    const int a = lev_ptr[l], b = lev_ptr[l + 1];
+    for (int t = a + (int)threadIdx.x; t < b; t += kSegThreads) lower_row(rows[t], A, F, r, z, ic0);
+    __syncthreads();
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(kSegThreads) void k_upper_seg(const int *lev_ptr, int l0, int l1, const int *rows, Mat A,
+                                                            const long long *tpos, const T *F, T *z, int ic0,
+                                                            const int *done) {
+  if (*done) return;
+  for (int l = l0; l < l1; l++) {
+    const int a = lev_ptr[l], b = lev_ptr[l + 1];
+    for (int t = a + (int)threadIdx.x; t < b; t += kSegThreads) upper_row(rows[t], A, tpos, F, z, ic0);
+    __syncthreads();
+  }
 }
 
 template <class T>
@@ -612,14 +664,16 @@ int Krylov::setup_pc64(PcKind pc, const double *val) {
   if (fac_.n < (size_t)pat->nstore) fac_.alloc(pat->nstore);
   MS_HIP(hipMemcpyAsync(fac_.p, val, pat->nstore * sizeof(double), hipMemcpyDeviceToDevice, stream()));
   MS_HIP(hipMemsetAsync(flag_.p, 0, sizeof(int), stream()));
-  for (size_t l = 0; l + 1 < pat->lev_ptr_f.size(); l++) {
-    int a = pat->lev_ptr_f[l], b = pat->lev_ptr_f[l + 1];
-    if (pc == PC_NATURAL_IC0)
-      hipLaunchKernelGGL(k_ic0_factor<double>, dim3(nblocks(b - a)), dim3(kBlock), 0, stream(),
-                         pat->d_lev_rows_f.p + a, b - a, A, fac_.p, flag_.p);
-    else
-      hipLaunchKernelGGL(k_ilu0_factor<double>, dim3(nblocks(b - a)), dim3(kBlock), 0, stream(),
-                         pat->d_lev_rows_f.p + a, b - a, A, fac_.p, flag_.p);
+  const int ic0 = pc == PC_NATURAL_IC0;
+  for (const auto &g : pat->seg_f) {
+    if (g.narrow) {
+      hipLaunchKernelGGL(k_factor_seg<double>, dim3(1), dim3(kSegThreads), 0, stream(), pat->d_lev_ptr_f.p, g.l0,
+                         g.l1, pat->d_lev_rows_f.p, A, fac_.p, ic0, flag_.p);
+      continue;
+    }
+    const int a = pat->lev_ptr_f[g.l0], b = pat->lev_ptr_f[g.l1];
+    hipLaunchKernelGGL(k_factor_level<double>, dim3(nblocks(b - a)), dim3(kBlock), 0, stream(),
+                       pat->d_lev_rows_f.p + a, b - a, A, fac_.p, ic0, flag_.p);
   }
   MS_CHECK_LAUNCH();
   int h = 0;
@@ -669,13 +723,23 @@ void Krylov::apply_pc(PcKind pc, const T *val, const T *fac, const T *dg, const 
                            cp[c + 1], A, val, dg, in, out, done, (T *)nullptr);
   } else {
     const int ic0 = pc == PC_NATURAL_IC0;
-    for (size_t l = 0; l + 1 < pat->lev_ptr_f.size(); l++) {
-      int a = pat->lev_ptr_f[l], b = pat->lev_ptr_f[l + 1];
+    for (const auto &g : pat->seg_f) {
+      if (g.narrow) {
+        hipLaunchKernelGGL(k_lower_seg<T>, dim3(1), dim3(kSegThreads), 0, stream(), pat->d_lev_ptr_f.p, g.l0, g.l1,
+                           pat->d_lev_rows_f.p, A, fac, in, out, ic0, done);
+        continue;
+      }
+      const int a = pat->lev_ptr_f[g.l0], b = pat->lev_ptr_f[g.l1];
       hipLaunchKernelGGL(k_lower_level<T>, dim3(nblocks(b - a)), dim3(kBlock), 0, stream(), pat->d_lev_rows_f.p + a,
                          b - a, A, fac, in, out, ic0, done);
     }
-    for (size_t l = 0; l + 1 < pat->lev_ptr_b.size(); l++) {
-      int a = pat->lev_ptr_b[l], b = pat->lev_ptr_b[l + 1];
+    for (const auto &g : pat->seg_b) {
+      if (g.narrow) {
+        hipLaunchKernelGGL(k_upper_seg<T>, dim3(1), dim3(kSegThreads), 0, stream(), pat->d_lev_ptr_b.p, g.l0, g.l1,
+                           pat->d_lev_rows_b.p, A, pat->d_tpos.p, fac, out, ic0, done);
+        continue;
+      }
+      const int a = pat->lev_ptr_b[g.l0], b = pat->lev_ptr_b[g.l1];
       hipLaunchKernelGGL(k_upper_level<T>, dim3(nblocks(b - a)), dim3(kBlock), 0, stream(), pat->d_lev_rows_b.p + a,
                          b - a, A, pat->d_tpos.p, fac, out, ic0, done);
     }

@@ -184,8 +184,27 @@ void SellPattern::build_levels() {
   std::vector<int> rf, rb;
   group(lf, nlf, lev_ptr_f, rf);
   group(lb, nlb, lev_ptr_b, rb);
+  auto plan = [](const std::vector<int> &ptr, std::vector<LevSeg> &seg) {
+    seg.clear();
+    const int nl = (int)ptr.size() - 1;
+    for (int l = 0; l < nl;) {
+      if (ptr[l + 1] - ptr[l] > kNarrowRows) {
+        seg.push_back({l, l + 1, false});
+        l++;
+        continue;
+      }
+      int e = l;
+      while (e < nl && ptr[e + 1] - ptr[e] <= kNarrowRows) e++;
+      seg.push_back({l, e, true});
+      l = e;
+    }
+  };
+  plan(lev_ptr_f, seg_f);
+  plan(lev_ptr_b, seg_b);
   d_lev_rows_f.upload(rf);
   d_lev_rows_b.upload(rb);
+  d_lev_ptr_f.upload(lev_ptr_f);
+  d_lev_ptr_b.upload(lev_ptr_b);
   has_levels = true;
 }
 
@@ -203,7 +222,8 @@ void SellPattern::upload() {
 
 size_t SellPattern::device_bytes() const {
   return d_off.bytes() + d_rowlen.bytes() + d_col.bytes() + d_diagk.bytes() + d_tpos.bytes() +
-         d_lev_rows_f.bytes() + d_lev_rows_b.bytes() + coarse.device_bytes() + d_col16.bytes() + d_cbase.bytes();
+         d_lev_rows_f.bytes() + d_lev_rows_b.bytes() + d_lev_ptr_f.bytes() + d_lev_ptr_b.bytes() +
+         coarse.device_bytes() + d_col16.bytes() + d_cbase.bytes();
 }
 
 // ------------------------------------------------------------ orderings

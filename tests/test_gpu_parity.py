@@ -263,3 +263,53 @@ def test_cavity_re1000_published_answer():
     l2, _ = ghia_compare(cs.get("xc").reshape(-1, 3), u, 1000)
     assert abs(umax - float(d["readme_umax"])) <= 2e-5, umax
     assert abs(l2 - float(d["readme_ghia_l2"])) <= 2e-5, l2
+
+
+def _grid_matrix(nx, ny, nz, skew, seed):
+    """1-based CSR of a 7-point operator on an nx x ny x nz grid in natural
+    order: diagonally dominant, symmetric for skew = 0 (else the upper
+    couplings are scaled by 1 - skew, the lower by 1 + skew)."""
+    rng = np.random.default_rng(seed)
+    n = nx * ny * nz
+    idx = np.arange(n).reshape(nz, ny, nx)
+    rows, cols, vals = [], [], []
+    for sl_a, sl_b in (((slice(None), slice(None), slice(1, None)), (slice(None), slice(None), slice(None, -1))),
+                       ((slice(None), slice(1, None), slice(None)), (slice(None), slice(None, -1), slice(None))),
+                       ((slice(1, None), slice(None), slice(None)), (slice(None, -1), slice(None), slice(None)))):
+        a, b = idx[sl_a].ravel(), idx[sl_b].ravel()      # a > b: b is the lower neighbour of a
+        w = rng.uniform(0.5, 1.5, a.size)
+        rows += [a, b]
+        cols += [b, a]
+        vals += [-w * (1 + skew), -w * (1 - skew)]
+    rows, cols, vals = np.concatenate(rows), np.concatenate(cols), np.concatenate(vals)
+    diag = np.zeros(n)
+    np.add.at(diag, rows, np.abs(vals))
+    rows = np.concatenate([rows, np.arange(n)])
+    cols = np.concatenate([cols, np.arange(n)])
+    vals = np.concatenate([vals, diag * rng.uniform(1.01, 1.2, n)])
+    o = np.lexsort((cols, rows))
+    rp = np.zeros(n + 1, np.int64)
+    np.add.at(rp, rows + 1, 1)
+    return np.cumsum(rp) + 1, cols[o] + 1, vals[o]
+
+
+@pytest.mark.parametrize("kind", ["cg_ic0", "bicgstab_ilu0"])
+def test_natural_sweeps_wide_and_narrow_levels(kind):
+    """Natural-order IC(0) / ILU(0) on a 64^3 grid: the level schedule has
+    diagonal planes of up to ~3000 rows (one launch per level) and narrow
+    levels at both corners (runs of levels in one workgroup), so factor and
+    sweeps take both launch forms.  Residual histories against the oracle
+    (bar as the reference's solver histories: CG 1e-10, BiCGSTAB 1e-9)."""
+    from oracle import pyoracle
+    rp, ci, val = _grid_matrix(64, 64, 64, 0.0 if kind == "cg_ic0" else 0.3, 7)
+    n = len(rp) - 1
+    rng = np.random.default_rng(8)
+    b, x0 = rng.normal(size=n), np.zeros(n)
+    fn = dict(cg_ic0=linsolver.cg_ic0, bicgstab_ilu0=linsolver.bicgstab_ilu0)[kind]
+    bar = 1e-10 if kind == "cg_ic0" else 1e-9
+    for k in (1, 2, 3, 5):
+        x, it, res, ierr = fn(rp, ci, val, b, x0, 1e-30, k)
+        xr, itr, resr, ierrr = pyoracle.solve(kind, rp, ci, val, b, x0, 1e-30, k)
+        assert (it, ierr) == (itr, ierrr), k
+        assert abs(res - resr) <= bar * resr, (k, res, resr)
+        assert rel_l2(x, xr) <= bar, k
