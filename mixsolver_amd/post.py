@@ -23,6 +23,14 @@ a1 = (alpha / lam)(u_f(0) - u_p(0)), else continuity u_f(0) = u_p(0).
 The numerical check takes the u_x column at the middle x station: the first
 fluid cell above the interface (uF1) and the mean over the porous rows at
 least lam away from the bed wall and the interface (uD).
+
+``conduction_exact`` holds the closed-form solid temperatures of the
+reference's pure-conduction family (cases/heat_conduction/
+check_heat_conduction.py:50-90): a bar of length L between fixed ends
+(linear), one fixed wall with every other face adiabatic (uniform), an
+annulus between fixed radii (logarithmic) and a bar with a uniform source
+q on [a, b) (two lines joined by a parabola; slope jumps q (x - a)/k inside,
+ends fixed).
 """
 from __future__ import annotations
 
@@ -100,3 +108,27 @@ def bj_profile(xc, u, lx, ly, nx, ny, K):
     lam = np.sqrt(K)
     r0, r1 = int(np.ceil(lam / dy)), ny // 2 - int(np.ceil(lam / dy))
     return col, float(col[ny // 2]), float(col[r0:r1].mean())
+
+
+def conduction_exact(kind, x, y, t_cold=300.0, t_hot=500.0, length=0.1, k=15.0, q=1.0e6, a=0.03, b=0.07,
+                     ri=0.01, ro=0.03, t_uniform=400.0):
+    """Closed-form steady solid temperature at (x, y): kind 'linear',
+    'uniform', 'annulus' or 'source'."""
+    x = np.asarray(x, np.float64)
+    y = np.asarray(y, np.float64)
+    if kind == "linear":
+        return t_cold + (t_hot - t_cold) * x / length
+    if kind == "uniform":
+        return np.full_like(x, t_uniform)
+    if kind == "annulus":
+        return t_cold + (t_hot - t_cold) * np.log(np.hypot(x, y) / ri) / np.log(ro / ri)
+    qk = q / k
+    # T = Al x + Bl on [0, a), -qk x^2 / 2 + Am x + Bm on [a, b), Ar x + Br on [b, L]:
+    # value and slope continuous at a and b, T(0) = t_cold, T(L) = t_hot
+    bl = t_cold
+    bm = bl - 0.5 * qk * a * a
+    br = bm + 0.5 * qk * b * b
+    ar = (t_hot - br) / length
+    am = ar + qk * b
+    al = am - qk * a
+    return np.where(x < a, al * x + bl, np.where(x < b, -0.5 * qk * x * x + am * x + bm, ar * x + br))

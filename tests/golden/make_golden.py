@@ -10,6 +10,7 @@ reference sources exist; the fixtures are committed data.
 from __future__ import annotations
 
 import os
+import shutil
 import subprocess
 import sys
 import tempfile
@@ -500,6 +501,43 @@ def bj_known_answer():
     return m, out
 
 
+REF_HC = "/root/reference/cases/heat_conduction"
+HC_CASES = dict(c1=("c1_1d.cas", "c1_1d_fixedT"), c2=("c1_1d.cas", "c2_1d_adiabatic"),
+                c3=("c3_annulus.cas", "c3_annulus_fixedT"), c4=("c3_annulus.cas", "c4_annulus_adiabatic"),
+                c5=("c5_1d_src.cas", "c5_1d_source"))
+# cases/heat_conduction/README.md:80-84: max |T_s - T_exact|, RMS, tolerance
+HC_README = dict(c1=(0.0, 0.0, 1e-6), c2=(0.0, 0.0, 1e-6), c3=(3.912e-01, 2.869e-01, 1.0),
+                 c4=(0.0, 0.0, 1e-6), c5=(5.208e-02, 3.294e-02, 0.1))
+
+
+def heat_conduction_known_answer():
+    """The reference's pure-conduction family (cases/heat_conduction: all
+    cells solid, k_s = 15, fixed-temperature / adiabatic walls, a volumetric
+    source zone in c5): its shipped meshes and controls re-run by the
+    reference's simple_run (one outer iteration reaches the tolerance), with
+    the solid temperature kept beside the README's published maximum and RMS
+    errors against the closed-form profiles."""
+    import re
+    out = {}
+    for c, (mf, cf) in HC_CASES.items():
+        m = cas.read_cas(os.path.join(REF_HC, mf))
+        ctl = open(os.path.join(REF_HC, cf + ".control")).read()
+        with tempfile.TemporaryDirectory() as td:
+            shutil.copy(os.path.join(REF_HC, mf), os.path.join(td, "m.cas"))
+            open(os.path.join(td, "c.control"), "w").write(ctl)
+            log = run(["m.cas", "c.control", "simple", "out.bin", "", "3000"], td)
+            r = Reader(os.path.join(td, "out.bin"))
+            nc, nf = m.ncells, m.nfaces
+            ier = r.take(np.int32, 1)[0]
+            r.take(np.float64, 3 * nc + nc + nf + nc + 3 * nc + 9 * nc + nc + 3 * nc)
+            Ts = r.take(np.float64, nc)
+        it = int(re.search(r"CONVERGED after\s+(\d+)", log).group(1))
+        out.update({f"{c}_{k}": v for k, v in mesh_arrays(m).items()})
+        out.update({f"{c}_control": ctl, f"{c}_ref_ier": np.int64(ier), f"{c}_ref_iters": np.int64(it),
+                    f"{c}_ref_T_s": Ts, f"{c}_readme": np.array(HC_README[c])})
+    return out
+
+
 def main():
     if not os.path.exists(HARNESS):
         sys.exit("build the reference harness first: oracle/build_ref.sh")
@@ -542,6 +580,10 @@ def main():
         save("cavity_re1000_known_answer", control=ctlk, nsteps=np.int64(1000), **mesh_arrays(mk),
              out_ier=np.int64(ierk), ref_umax=umax, ref_ghia_l2=l2,
              readme_umax=0.911512, readme_ghia_l2=0.0289706)
+
+    # 3e. the reference's pure-conduction family and its published errors
+    if os.path.isdir(REF_HC):
+        save("heat_conduction_known_answer", **heat_conduction_known_answer())
 
     # 3d. the reference's shipped Beavers-Joseph porous channel and its published answers
     if os.path.isdir(REF_BJ):
@@ -622,6 +664,8 @@ def main():
 if __name__ == "__main__":
     if sys.argv[1:] == ["--porous"]:
         porous_fixtures()
+    elif sys.argv[1:] == ["--hc"]:
+        save("heat_conduction_known_answer", **heat_conduction_known_answer())
     elif sys.argv[1:] == ["--bj"]:
         mb, ob = bj_known_answer()
         save("bj_porous_known_answer", **mesh_arrays(mb), **ob, **BJ_README)

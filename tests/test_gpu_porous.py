@@ -181,3 +181,41 @@ def test_bj_published_answer(alpha, path):
     else:
         assert err <= 2e-6
         assert abs(uF1 - rF1) <= 1e-6 * abs(rF1) and abs(uD - rD) <= 1e-6 * abs(rD)
+
+
+# ------------------------------------------- pure conduction published answers
+HC_KIND = dict(c1="linear", c2="uniform", c3="annulus", c4="uniform", c5="source")
+
+
+@pytest.mark.parametrize("c", list(HC_KIND))
+@pytest.mark.parametrize("path", ["fp64", "fast"])
+def test_conduction_published_answer(c, path):
+    """The reference's pure-conduction family on device SIMPLE (solid block
+    type, k_s, zone heat source, fixed / adiabatic walls, non-orthogonal
+    annulus): one outer iteration as the reference, solid temperature within
+    1e-8 K of the reference's on the FP64 path (its solver, reduction order
+    aside) and within the README's 1e-6 K on the fast path (DILU stops at the
+    same 1e-12 residual by another route: the reference itself stops 5e-8 K
+    short of the exact uniform 400 K in c2), and the README's error against
+    the closed form (printed digits for c3 / c5, the 1e-6 K tolerance for the
+    exact cases)."""
+    from mixsolver_amd.post import conduction_exact
+    d = load_golden("heat_conduction_known_answer")
+    sub = {k[len(c) + 1:]: v for k, v in d.items() if k.startswith(c + "_")}
+    ctrl, groups = parse_control(str(sub["control"]), warn=False)
+    if path == "fast":
+        for k, v in dict(gpu_ppe_precond=2, gpu_mom_precond=2, gpu_precision=1).items():
+            setattr(ctrl, k, v)
+    cs = Case(mesh_from(sub), ctrl, groups)
+    ier, it = cs.simple_run(3000)
+    assert ier == 0 and it == int(sub["ref_iters"])
+    ts = cs.get("T_s")
+    assert np.abs(ts - sub["ref_T_s"]).max() <= (1e-8 if path == "fp64" else 1e-6)
+    xc = cs.get("xc").reshape(-1, 3)
+    err = ts - conduction_exact(HC_KIND[c], xc[:, 0], xc[:, 1])
+    emax, rms, tol = sub["readme"]
+    if emax > 0:
+        assert float("%.3e" % np.abs(err).max()) == emax
+        assert float("%.3e" % np.sqrt(np.mean(err ** 2))) == rms
+    else:
+        assert np.abs(err).max() <= tol

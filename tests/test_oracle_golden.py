@@ -275,3 +275,61 @@ def test_bj_oracle_reproduces_reference_run():
     h = oc.history()
     assert "%.3E" % h[0, 0] == "%.3E" % d["ref_it1_a1"][0]
     assert "%.2E" % h[0, 5] == "%.2E" % d["ref_it1_a1"][2]
+
+
+# ------------------------------------------- pure conduction published answers
+HC_KIND = dict(c1="linear", c2="uniform", c3="annulus", c4="uniform", c5="source")
+
+
+def _hc_mesh(d, c):
+    return mesh_from({k[len(c) + 1:]: v for k, v in d.items() if k.startswith(c + "_")})
+
+
+def test_conduction_closed_forms():
+    """The closed-form profiles satisfy what the README states of them:
+    T(a) = 400 K and T(b) = 480 K for the source bar, the energy balance
+    -k T'(b) + k T'(a) = q (b - a) = 4e4 W/m^2, T'' = -q/k inside the source,
+    end temperatures 300 / 500 K, and T(r = 0.02) = 426.19 K on the annulus."""
+    from mixsolver_amd.post import conduction_exact
+    f = lambda x: conduction_exact("source", x, 0.0)
+    assert float(f(0.03)) == pytest.approx(400.0, abs=1e-9)
+    assert float(f(0.07)) == pytest.approx(480.0, abs=1e-9)
+    assert float(f(0.0)) == 300.0 and float(f(0.1)) == pytest.approx(500.0, abs=1e-9)
+    for x in (0.03, 0.07):   # value continuity across the source edges (slopes ~5e3 K/m)
+        assert abs(float(f(x - 1e-9)) - float(f(x + 1e-9))) <= 2e-5
+    h = 1e-6
+    d1 = lambda x: (float(f(x + h)) - float(f(x - h))) / (2 * h)
+    assert -15.0 * d1(0.08) + 15.0 * d1(0.02) == pytest.approx(4.0e4, rel=1e-6)
+    h2 = 1e-4
+    assert (float(f(0.05 + h2)) - 2 * float(f(0.05)) + float(f(0.05 - h2))) / h2 ** 2 == pytest.approx(-1e6 / 15.0,
+                                                                                                         rel=1e-6)
+    assert float(conduction_exact("annulus", 0.02, 0.0)) == pytest.approx(426.19, abs=5e-3)
+    assert float(conduction_exact("annulus", 0.0, 0.03)) == pytest.approx(500.0, abs=1e-9)
+    assert float(conduction_exact("linear", 0.05, 0.0)) == 400.0
+
+
+@pytest.mark.parametrize("c", list(HC_KIND))
+def test_conduction_known_answer(c):
+    """The reference's pure-conduction family (all cells solid, fixed and
+    adiabatic walls, a source zone in c5): the reference's own solid
+    temperature has the README's maximum / RMS error against the closed form
+    (printed digits; c1, c2, c4 within the 1e-6 K tolerance), and the
+    oracle's simple_run reproduces it bit for bit."""
+    from mixsolver_amd.post import conduction_exact
+    d = load_golden("heat_conduction_known_answer")
+    m = _hc_mesh(d, c)
+    ctrl, groups = parse_control(str(d[f"{c}_control"]), warn=False)
+    oc = pyoracle.OracleCase(m, ctrl, groups)
+    xc = oc.geometry()["xc"].reshape(-1, 3)
+    ts = d[f"{c}_ref_T_s"]
+    err = ts - conduction_exact(HC_KIND[c], xc[:, 0], xc[:, 1])
+    emax, rms, tol = d[f"{c}_readme"]
+    assert int(d[f"{c}_ref_ier"]) == 0 and int(d[f"{c}_ref_iters"]) == 1
+    if emax > 0:
+        assert float("%.3e" % np.abs(err).max()) == emax
+        assert float("%.3e" % np.sqrt(np.mean(err ** 2))) == rms
+    else:
+        assert np.abs(err).max() <= tol
+    ier, it = oc.simple_run(3000, mpi=False)
+    assert ier == 0 and it == 1
+    assert np.array_equal(oc.get("T_s"), ts)
