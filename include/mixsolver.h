@@ -155,6 +155,24 @@ int ms_pimple_run(ms_system *s, int nsteps, int *ier);
  * imbal du_max dT_max|umax cfl lin_it_max lin_res_mean ppe_it_total ppe_res_last */
 int ms_get_history(ms_system *s, double *out, int maxrows, int *nrows);
 
+/* ---- wall-zone post-processing on the resident fields ------------------
+ * compute_forces (mod_uns_forces.f90:43-185): out14 = Fx Fy Fz CD CL u_inf
+ * A_ref p_inf, then the pressure parts Fx_p Fy_p Fz_p and the viscous parts
+ * Fx_v Fy_v Fz_v (the reference prints these).  Pressure force sum
+ * (p_c0 - p_inf) S_f and wall shear mu u_t / dn along n x z over the wall
+ * faces, reduced on the device; u_inf / p_inf of the first far-field group;
+ * A_ref = (x extent of the wall faces) x (z extent of the nodes). */
+int ms_compute_forces(ms_system *s, double *out14);
+/* write_cp_distribution (mod_uns_forces.f90:195-254) as rows of 6 doubles,
+ * one per wall face in face order: theta (deg, about the mean of the wall
+ * face centroids), Cp, 1 - 4 sin^2(theta), x, y, z.  *nrows = number of wall
+ * faces (0 without a far-field group); rows6 may be null to query it. */
+int ms_cp_distribution(ms_system *s, double *rows6, int maxrows, int *nrows);
+/* compute_separation (mod_uns_forces.f90:275-407): out5 = top separation
+ * angle (rad, from +x), found (1/0), bottom separation angle, found, wake
+ * length (farthest reverse-flow cell centroid behind the wall centre). */
+int ms_separation(ms_system *s, double *out5);
+
 /* ---- per-call level (copy in/out through ms_set/get_field) -------------
  * val/rhs/ap are host arrays in the reference CSR order (nnz / ncells). */
 int ms_compute_gradients(ms_system *s);                       /* mod_uns_fields.f90:164 */

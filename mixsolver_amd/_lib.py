@@ -55,6 +55,9 @@ def lib():
         "ms_piso_run": (C.c_int, [_V, C.c_int, _I]),
         "ms_pimple_run": (C.c_int, [_V, C.c_int, _I]),
         "ms_get_history": (C.c_int, [_V, _D, C.c_int, _I]),
+        "ms_compute_forces": (C.c_int, [_V, _D]),
+        "ms_cp_distribution": (C.c_int, [_V, _D, C.c_int, _I]),
+        "ms_separation": (C.c_int, [_V, _D]),
         "ms_compute_gradients": (C.c_int, [_V]),
         "ms_momentum_assembly": (C.c_int, [_V, C.c_int, _D, _D, _D]),
         "ms_temperature_assembly": (C.c_int, [_V, _D, _D, _D]),

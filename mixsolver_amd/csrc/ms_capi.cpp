@@ -522,4 +522,111 @@ int ms_server_stats(ms_server *h, long long *out8) {
   });
 }
 
+// ---- wall-zone post-processing (mod_uns_forces.f90) -----------------------
+static int wall_centre(System &s, double *xc, double *yc) {
+  double sx = 0.0, sy = 0.0;
+  for (int f : s.wall_faces) {
+    sx = sx + s.xf[3 * (size_t)f];
+    sy = sy + s.xf[3 * (size_t)f + 1];
+  }
+  const int n = (int)s.wall_faces.size();
+  if (n > 0) {
+    sx = sx / (double)n;
+    sy = sy / (double)n;
+  }
+  *xc = sx;
+  *yc = sy;
+  return n;
+}
+
+int ms_compute_forces(ms_system *h, double *out14) {
+  return guarded([&] {
+    if (!out14) fail(MS_EINVAL, "null output");
+    wall_forces(sys(h), out14);
+  });
+}
+
+int ms_cp_distribution(ms_system *h, double *rows6, int maxrows, int *nrows) {
+  return guarded([&] {
+    System &s = sys(h);
+    if (nrows) *nrows = 0;
+    double u_inf, p_inf;
+    free_stream(s, &u_inf, &p_inf);
+    std::vector<double> pu;
+    wall_samples(s, pu);
+    if (u_inf <= 0.0) return;
+    double xcyl, ycyl;
+    const int n = wall_centre(s, &xcyl, &ycyl);
+    if (nrows) *nrows = n;
+    if (!rows6 || n == 0) return;
+    if (maxrows < n) fail(MS_EINVAL, "ms_cp_distribution: rows6 holds fewer rows than wall faces");
+    const double q = 0.5 * s.ctrl.rho * (u_inf * u_inf);
+    const double pi = 3.141592653589793238462643;
+    for (int k = 0; k < n; k++) {
+      const double *xf = &s.xf[3 * (size_t)s.wall_faces[k]];
+      const double th = std::atan2(xf[1] - ycyl, xf[0] - xcyl);
+      const double sn = std::sin(th);
+      double *r = &rows6[6 * (size_t)k];
+      r[0] = th * 180.0 / pi;
+      r[1] = (pu[4 * (size_t)k] - p_inf) / q;
+      r[2] = 1.0 - 4.0 * (sn * sn);
+      r[3] = xf[0];
+      r[4] = xf[1];
+      r[5] = xf[2];
+    }
+  });
+}
+
+int ms_separation(ms_system *h, double *out5) {
+  return guarded([&] {
+    if (!out5) fail(MS_EINVAL, "null output");
+    System &s = sys(h);
+    for (int k = 0; k < 5; k++) out5[k] = 0.0;
+    std::vector<double> pu;
+    wall_samples(s, pu);
+    double xcyl, ycyl;
+    const int n = wall_centre(s, &xcyl, &ycyl);
+    if (n == 0) return;
+    std::vector<double> th(n), ua(n);
+    for (int k = 0; k < n; k++) {
+      const int f = s.wall_faces[k];
+      const double *sf = &s.sf[3 * (size_t)f], *xf = &s.xf[3 * (size_t)f];
+      const double nx = sf[0] / norm2_ref(sf, 3), ny = sf[1] / norm2_ref(sf, 3);
+      th[k] = std::atan2(xf[1] - ycyl, xf[0] - xcyl);
+      ua[k] = pu[4 * (size_t)k + 1] * ny + pu[4 * (size_t)k + 2] * (-nx);
+    }
+    for (int i = 1; i < n; i++) {   // insertion sort by theta, stable (as the reference)
+      const double t = th[i], u = ua[i];
+      int k = i - 1;
+      while (k >= 0 && th[k] > t) {
+        th[k + 1] = th[k];
+        ua[k + 1] = ua[k];
+        k--;
+      }
+      th[k + 1] = t;
+      ua[k + 1] = u;
+    }
+    double up = 0.0, tp = 0.0;
+    bool ftop = false, fbot = false;
+    for (int i = 0; i < n; i++) {
+      if (std::fabs(ua[i]) < 1.0e-4) continue;
+      if (ua[i] < 0.0 && up > 0.0) {
+        const double ts = tp + (0.0 - up) * (th[i] - tp) / (ua[i] - up);
+        if (th[i] < 0.0 && !fbot) {
+          out5[2] = ts;
+          fbot = true;
+        } else if (th[i] > 0.0 && !ftop) {
+          out5[0] = ts;
+          ftop = true;
+        }
+      }
+      up = ua[i];
+      tp = th[i];
+    }
+    out5[1] = ftop ? 1.0 : 0.0;
+    out5[3] = fbot ? 1.0 : 0.0;
+    out5[4] = (wake_end(s, xcyl) - xcyl) / 1.0;
+  });
+}
+
 }  // extern "C"

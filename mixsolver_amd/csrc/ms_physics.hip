@@ -19,9 +19,13 @@
 //   k_ppe_af/k_ppe   mod_uns_simple.f90:1552-1644, mod_uns_simple_mpi.f90:882-968
 //   k_correct_*      mod_uns_simple.f90:1649-1739
 //   k_diagnostics    mod_uns_simple_mpi.f90:502-526, nan_guard_mpi :1024-1074
+//   k_wall_forces    compute_forces, mod_uns_forces.f90:135-156
+//   k_wall_samples   owner values of write_cp_distribution (:238-246) and
+//                    compute_separation (:320-333); k_wake_end (:365-370)
 //   drivers          simple_run (mod_uns_simple.f90:88-288),
 //                    piso_run_mpi (mod_uns_simple_mpi.f90:298-571)
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstring>
 
@@ -829,6 +833,54 @@ __global__ void k_diagnostics(DMesh m, DFields F, const double *uref, const doub
   block_partials<OpMax, 5>(v, part);
 }
 
+// per wall face: pressure force (p - p_inf) S, and with mu > 0 the wall shear
+// tau = mu u_t / dn along t = n x z_hat times the face area
+__global__ void k_wall_forces(DMesh m, DFields F, const int *wf, int nw, double mu, double p_inf, double *part) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  double v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (t < nw) {
+    const int f = wf[t], c = m.c0[f];
+    const double *sf = &m.sf[3 * (size_t)f], *xf = &m.xf[3 * (size_t)f];
+    const double pg = F.p[c] - p_inf;
+    v[0] = pg * sf[0];
+    v[1] = pg * sf[1];
+    v[2] = pg * sf[2];
+    if (mu > 0.0) {
+      const double nx = sf[0] / dnorm3(sf), ny = sf[1] / dnorm3(sf);
+      const double tx = ny, ty = -nx, tz = 0.0;
+      const double d[3] = {xf[0] - m.xc[3 * (size_t)c], xf[1] - m.xc[3 * (size_t)c + 1],
+                           xf[2] - m.xc[3 * (size_t)c + 2]};
+      const double dn = dnorm3(d);
+      const double *u = &F.u[3 * (size_t)c];
+      const double ut = u[0] * tx + u[1] * ty + u[2] * tz;
+      const double tau = mu * ut / dn;
+      v[3] = tau * tx * m.area[f];
+      v[4] = tau * ty * m.area[f];
+      v[5] = tau * tz * m.area[f];
+    }
+  }
+  block_partials<OpSum, 6>(v, part);
+}
+
+// (p, u) of the owner cell of every wall face
+__global__ void k_wall_samples(DMesh m, DFields F, const int *wf, int nw, double *out) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= nw) return;
+  const int c = m.c0[wf[t]];
+  out[4 * (size_t)t] = F.p[c];
+  out[4 * (size_t)t + 1] = F.u[3 * (size_t)c];
+  out[4 * (size_t)t + 2] = F.u[3 * (size_t)c + 1];
+  out[4 * (size_t)t + 3] = F.u[3 * (size_t)c + 2];
+}
+
+// farthest reverse-flow cell behind x0: max of x0 and xc over cells with u_x < 0, xc > x0
+__global__ void k_wake_end(DMesh m, DFields F, double x0, double *part) {
+  const int r = blockIdx.x * kBlock + threadIdx.x;
+  double v[1] = {x0};
+  if (r < m.nc && F.u[3 * (size_t)r] < 0.0 && m.xc[3 * (size_t)r] > x0) v[0] = dmax(x0, m.xc[3 * (size_t)r]);
+  block_partials<OpMax, 1>(v, part);
+}
+
 }  // namespace
 
 // ======================================================================
@@ -1216,6 +1268,88 @@ void call_correct(System &s, const double *d_pp) {
   Driver D(s);
   hipLaunchKernelGGL(k_ppe_af, gc(s.nf), dim3(kBlock), 0, stream(), s.dmesh(), s.apc.p, s.ctrl.rho, s.af.p);
   D.correct(d_pp);
+}
+
+// ---------------------------------------------------------- wall zones
+static void ensure_walls(System &s) {
+  if (s.walls_ready) return;
+  s.wall_faces.clear();
+  for (int f = 0; f < s.nf; f++) {
+    const int g = s.fgrp[f];
+    if (g > 0 && s.groups[g - 1].btype == MS_BC_WALL) s.wall_faces.push_back(f);
+  }
+  if (!s.wall_faces.empty()) s.d_wall_faces.upload(s.wall_faces);
+  s.walls_ready = true;
+}
+
+void free_stream(const System &s, double *u_inf, double *p_inf) {
+  *u_inf = 0.0;
+  *p_inf = 0.0;
+  for (int g = 0; g < s.nbc; g++)
+    if (s.groups[g].btype == MS_BC_FARFIELD) {
+      *u_inf = norm2_ref(s.groups[g].uvel, 3);
+      *p_inf = s.groups[g].pval;
+      return;
+    }
+}
+
+void wall_forces(System &s, double *out) {
+  ensure_walls(s);
+  for (int k = 0; k < 14; k++) out[k] = 0.0;
+  double u_inf, p_inf;
+  free_stream(s, &u_inf, &p_inf);
+  out[5] = u_inf;
+  out[7] = p_inf;
+  const int nw = (int)s.wall_faces.size();
+  if (nw == 0) return;
+  double xmin = DBL_MAX, xmax = -DBL_MAX;
+  for (int f : s.wall_faces) {
+    xmin = std::min(xmin, s.xf[3 * (size_t)f]);
+    xmax = std::max(xmax, s.xf[3 * (size_t)f]);
+  }
+  const double a_ref = (xmax - xmin) * s.zspan;
+  hipLaunchKernelGGL(k_wall_forces, gc(nw), dim3(kBlock), 0, stream(), s.dmesh(), s.dfields(), s.d_wall_faces.p, nw,
+                     s.ctrl.mu, p_inf, s.part.p);
+  hipLaunchKernelGGL((k_finish<OpSum, 6, double, FinStore>), dim3(1), dim3(kFinThreads), 0, stream(), nblocks(nw),
+                     s.part.p, FinStore{s.tmpc.p, 6});
+  MS_CHECK_LAUNCH();
+  double f[6];
+  MS_HIP(hipMemcpyAsync(f, s.tmpc.p, 6 * sizeof(double), hipMemcpyDeviceToHost, stream()));
+  MS_HIP(hipStreamSynchronize(stream()));
+  out[0] = f[0] + f[3];
+  out[1] = f[1] + f[4];
+  out[2] = f[2] + f[5];
+  out[6] = a_ref;
+  for (int k = 0; k < 6; k++) out[8 + k] = f[k];
+  if (u_inf > 0.0 && a_ref > 0.0) {
+    const double q = 0.5 * s.ctrl.rho * (u_inf * u_inf);
+    out[3] = out[0] / (q * a_ref);
+    out[4] = out[1] / (q * a_ref);
+  }
+}
+
+void wall_samples(System &s, std::vector<double> &pu) {
+  ensure_walls(s);
+  const int nw = (int)s.wall_faces.size();
+  pu.assign(4 * (size_t)nw, 0.0);
+  if (nw == 0) return;
+  DBuf<double> d(4 * (size_t)nw);
+  hipLaunchKernelGGL(k_wall_samples, gc(nw), dim3(kBlock), 0, stream(), s.dmesh(), s.dfields(), s.d_wall_faces.p,
+                     nw, d.p);
+  MS_CHECK_LAUNCH();
+  d.download(pu.data(), pu.size());
+  MS_HIP(hipStreamSynchronize(stream()));
+}
+
+double wake_end(System &s, double xcyl) {
+  hipLaunchKernelGGL(k_wake_end, gc(s.nc), dim3(kBlock), 0, stream(), s.dmesh(), s.dfields(), xcyl, s.part.p);
+  hipLaunchKernelGGL((k_finish<OpMax, 1, double, FinStore>), dim3(1), dim3(kFinThreads), 0, stream(), nblocks(s.nc),
+                     s.part.p, FinStore{s.tmpc.p, 1});
+  MS_CHECK_LAUNCH();
+  double v = 0.0;
+  MS_HIP(hipMemcpyAsync(&v, s.tmpc.p, sizeof(double), hipMemcpyDeviceToHost, stream()));
+  MS_HIP(hipStreamSynchronize(stream()));
+  return v;
 }
 
 }  // namespace ms

@@ -15,6 +15,7 @@
 // the SELL-64 matrix pattern, per-cell face lists with matrix slot maps,
 // level schedules for natural-order factorizations and per-face constants.
 #include <algorithm>
+#include <cfloat>
 #include <climits>
 #include <cmath>
 #include <cstring>
@@ -631,6 +632,14 @@ System *create_system(int nnodes, const double *x, int nfaces, const long long *
   s.nn = nnodes;
   s.nbc = ctrl->nbc;
   s.ctrl = *ctrl;
+  {
+    double zmx = -DBL_MAX, zmn = DBL_MAX;
+    for (int i = 0; i < nnodes; i++) {
+      zmx = std::max(zmx, x[3 * (size_t)i + 2]);
+      zmn = std::min(zmn, x[3 * (size_t)i + 2]);
+    }
+    s.zspan = zmx - zmn;
+  }
   s.groups.assign(groups, groups + ctrl->nbc);
   const int nc = ncells, nf = nfaces;
   check_faces(nnodes, nf, fptr, fnodes, c0in, c1orig, nc);
@@ -901,8 +910,9 @@ System *create_system(int nnodes, const double *x, int nfaces, const long long *
     b->alloc(n3);
   for (DBuf<double> *b : {&s.p, &s.T, &s.T_s, &s.apc, &s.T_old, &s.T_old_old, &s.T_s_old, &s.T_s_old_old,
                           &s.porosity_d, &s.inertial_d, &s.k_s_d, &s.rho_s_d, &s.cp_s_d, &s.rhs, &s.ap, &s.pp,
-                          &s.ucomp, &s.diag8, &s.tmpc})
+                          &s.ucomp, &s.diag8})
     b->alloc(nc);
+  s.tmpc.alloc(std::max(nc, 8));   // also holds the reduced scalars (up to 6) of tiny meshes
   s.gu.alloc(9 * (size_t)nc);
   s.psi.alloc(4 * (size_t)nc);
   s.flux.alloc(nf);

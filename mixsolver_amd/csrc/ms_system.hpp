@@ -109,6 +109,12 @@ struct System {
   DBuf<unsigned int> ticket;
   std::vector<double> history;  // 8 per row
   int pin_cell = 0, has_pdir = 0, has_outflow = 0;
+  // wall-zone post-processing: node z range (the span of the reference area)
+  // and the wall faces in face order (built on first use)
+  double zspan = 0.0;
+  std::vector<int> wall_faces;
+  DBuf<int> d_wall_faces;
+  bool walls_ready = false;
 
   // the energy equations can have a non-zero solution (otherwise their
   // right-hand sides are zero and the solves return at once)
@@ -129,6 +135,15 @@ System *create_system(int nnodes, const double *x, int nfaces, const long long *
 // errors as create_system.
 void periodic_match(int nnodes, const double *x, int nfaces, const long long *fptr, const int *fnodes,
                     const int *c0, const int *c1, const int *zone, int ncells, const ms_ctrl_t *ctrl, int *partner);
+
+// Wall-zone post-processing on the device fields (mod_uns_forces.f90):
+// out14 = Fx Fy Fz CD CL u_inf A_ref p_inf, pressure parts, viscous parts;
+// samples = (p, u_x, u_y, u_z) of the owner cell of every wall face, face
+// order; wake_end = max(xcyl, max xc over cells with u_x < 0 behind xcyl).
+void wall_forces(System &s, double *out14);
+void wall_samples(System &s, std::vector<double> &pu);
+double wake_end(System &s, double xcyl);
+void free_stream(const System &s, double *u_inf, double *p_inf);
 
 // flang runtime NORM2 evaluation order (max-scaled accumulator), used for
 // every geometric length so the device coefficients equal the reference's.

@@ -24,6 +24,15 @@ The numerical check takes the u_x column at the middle x station: the first
 fluid cell above the interface (uF1) and the mean over the porous rows at
 least lam away from the bed wall and the interface (uD).
 
+``cylinder_check`` restates the reference's check of its Re=40 cylinder
+(cluster_cases/cylinder_Re40/check_cylinder.py:70-190) on cell values: the
+wall ring (innermost cell of each of 80 angular bins), Cp = p / (rho U^2 / 2)
+at the front stagnation point, at the base and its minimum, the near-wall
+u_x sign changes (separation, measured from the front stagnation point), the
+recirculation length on the wake axis (farthest x with u_x < 0 over 0.05-wide
+x bins, cell nearest y = 0), the pressure drag of the ring, and the mean /
+max |Cp - Cp_ref| against a tabulated Cp(theta) interpolated periodically.
+
 ``conduction_exact`` holds the closed-form solid temperatures of the
 reference's pure-conduction family (cases/heat_conduction/
 check_heat_conduction.py:50-90): a bar of length L between fixed ends
@@ -132,3 +141,56 @@ def conduction_exact(kind, x, y, t_cold=300.0, t_hot=500.0, length=0.1, k=15.0, 
     am = ar + qk * b
     al = am - qk * a
     return np.where(x < a, al * x + bl, np.where(x < b, -0.5 * qk * x * x + am * x + bm, ar * x + br))
+
+
+def cylinder_check(xc, u, p, r_cyl=0.5, u_inf=1.0, rho=1.0, nbins=80, cp_ref=None):
+    """Regression indicators of a 2-D cylinder in x-flow (dict)."""
+    xc = np.asarray(xc).reshape(-1, 3)
+    u = np.asarray(u).reshape(-1, 3)
+    p = np.asarray(p)
+    q = 0.5 * rho * u_inf ** 2
+    x, y, ux = xc[:, 0], xc[:, 1], u[:, 0]
+    th = np.degrees(np.arctan2(y, x))
+    rr = np.hypot(x, y)
+    b = np.clip(((th + 180.0) / 360.0 * nbins).astype(int), 0, nbins - 1)
+    sel = []
+    for k in range(nbins):
+        m = b == k
+        if m.any():
+            sel.append(np.nonzero(m)[0][np.argmin(rr[m])])
+    sel = np.array(sel, dtype=int)
+    o = np.argsort(th[sel])
+    th_w, ux_w, cp_w, r_w = th[sel][o], ux[sel][o], p[sel][o] / q, rr[sel][o]
+    phi = (th_w % 360.0) - 180.0
+    i_f = int(np.argmin(np.abs(phi)))
+    i_r = int(np.argmax(np.abs(phi)))
+    out = dict(ring_cells=len(th_w), ring_r_mean=float(r_w.mean()), cp_front=float(cp_w[i_f]),
+               cp_base=float(cp_w[i_r]), cp_min=float(cp_w.min()), phi_cp_min=float(phi[int(np.argmin(cp_w))]),
+               umax=float(np.hypot(u[:, 0], u[:, 1]).max() / u_inf))
+    oo = np.argsort(phi)
+    ph, uw = phi[oo], ux_w[oo]
+    sg = np.sign(uw)
+    chg = np.nonzero(sg[1:] * sg[:-1] < 0)[0]
+    out["separation_from_wake_axis"] = [float(180.0 - abs(ph[i])) for i in chg]
+    xs, us = [], []
+    for x0 in np.arange(0.5, 6.0, 0.05):
+        m = (x > x0 - 0.025) & (x < x0 + 0.025)
+        if m.any():
+            j = np.nonzero(m)[0][np.argmin(np.abs(y[m]))]
+            xs.append(x[j])
+            us.append(ux[j])
+    xs, us = np.array(xs), np.array(us)
+    neg = us < 0.0
+    out["wake_length_D"] = float(xs[neg].max() / (2 * r_cyl)) if neg.any() else 0.0
+    out["max_reverse_ux"] = float(us.min())
+    dth = np.diff(np.concatenate((th_w, th_w[:1] + 360.0)))
+    out["cd_pressure"] = float(-np.sum(cp_w * np.cos(np.radians(th_w)) * r_cyl * np.radians(dth)) / (2 * r_cyl))
+    if cp_ref is not None:
+        rth, rcp = np.asarray(cp_ref[0], float), np.asarray(cp_ref[1], float)
+        rth = np.where(rth > 180.0, rth - 360.0, rth)
+        oo = np.argsort(rth)
+        rth, rcp = rth[oo], rcp[oo]
+        d = cp_w - np.interp(th_w, np.concatenate((rth - 360.0, rth, rth + 360.0)), np.tile(rcp, 3))
+        out["dcp_mean"] = float(np.abs(d).mean())
+        out["dcp_max"] = float(np.abs(d).max())
+    return out

@@ -135,6 +135,35 @@ class Case:
         check(_lib.lib().ms_get_history(self.h, dp(out), n.value, C.byref(n)))
         return out[: n.value * 8].reshape(-1, 8)
 
+    FORCE_KEYS = ("Fx", "Fy", "Fz", "CD", "CL", "u_inf", "A_ref", "p_inf", "Fx_p", "Fy_p", "Fz_p", "Fx_v", "Fy_v",
+                  "Fz_v")
+
+    def compute_forces(self):
+        """Pressure and viscous force on the wall zones with the drag / lift
+        coefficients of compute_forces (mod_uns_forces.f90:43-185), reduced
+        on the device: dict over FORCE_KEYS."""
+        out = np.zeros(14)
+        check(_lib.lib().ms_compute_forces(self.h, dp(out)))
+        return dict(zip(self.FORCE_KEYS, out.tolist()))
+
+    def cp_distribution(self):
+        """Rows of write_cp_distribution (mod_uns_forces.f90:195-254), one per
+        wall face: (n, 6) theta deg, Cp, Cp_exact, x, y, z."""
+        n = C.c_int(0)
+        check(_lib.lib().ms_cp_distribution(self.h, None, 0, C.byref(n)))
+        rows = np.zeros(6 * max(1, n.value))
+        check(_lib.lib().ms_cp_distribution(self.h, dp(rows), n.value, C.byref(n)))
+        return rows[: 6 * n.value].reshape(-1, 6)
+
+    def separation(self):
+        """compute_separation (mod_uns_forces.f90:275-407): separation angles
+        (rad, from +x) of the upper and lower wall, whether found, and the
+        wake length."""
+        out = np.zeros(5)
+        check(_lib.lib().ms_separation(self.h, dp(out)))
+        return dict(theta_top=out[0], found_top=bool(out[1]), theta_bot=out[2], found_bot=bool(out[3]),
+                    wake_length=out[4])
+
     def time_kernel(self, fp32=True, reps=50):
         """(average ms, algorithmic bytes) of the PCG SpMV kernel on the
         current matrix (HIP events on the library stream)."""

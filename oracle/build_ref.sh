@@ -42,6 +42,7 @@ SRCS=(
   "$REF/unstructured/mod_uns_linsolver.f90"
   "$REF/unstructured/mod_uns_output.f90"
   "$REF/unstructured/mod_uns_simple.f90"
+  "$REF/unstructured/mod_uns_forces.f90"
 )
 OBJS=()
 cd "$TMP"

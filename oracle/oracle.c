@@ -50,6 +50,7 @@
 struct or_case {
   int nn, nf, nc, nbc, nnz;
   const double *x;               /* borrowed during create only */
+  double zspan;                  /* maxval(x(3,:)) - minval(x(3,:)) */
   int *c0, *c1, *zone, *fgrp;
   double *xf, *sf, *area, *vol, *xc, *lf;
   int *cf_ptr, *cf, *c2c_ptr, *c2c;
@@ -407,6 +408,14 @@ or_case *or_create(int nn, const double *x, int nf, const long long *fptr,
   *ier = 0;
   or_case *cs = xcalloc(1, sizeof(or_case));
   cs->nn = nn; cs->nf = nf; cs->nc = nc; cs->x = x;
+  {
+    double zmx = -DBL_MAX, zmn = DBL_MAX;
+    for (int i = 0; i < nn; i++) {
+      zmx = fmax0(zmx, x[3 * (size_t)i + 2]);
+      zmn = fmin0(zmn, x[3 * (size_t)i + 2]);
+    }
+    cs->zspan = nn > 0 ? zmx - zmn : 0.0;
+  }
   cs->ctrl = *ctrl;
   cs->nbc = ctrl->nbc;
   cs->inlet_ramp = 1.0; cs->pval_ramp = 1.0;
@@ -1807,4 +1816,195 @@ int or_pimple_run(or_case *cs, int nsteps, int mpi) {
   cs->t_total = now_s() - t_start;
   free(val); free(rhs); free(ap); free(pp); free(dpp); free(imb); free(ucomp);
   return ier;
+}
+
+
+/* ------------------------------------------------------------ wall forces */
+static int is_wall_face(const or_case *cs, int i) {
+  int gi = cs->fgrp[i];
+  return gi != 0 && cs->gb[gi - 1].btype == OR_BC_WALL;
+}
+
+/* free-stream (u_inf, p_inf) of the first far-field group, else (0, 0) */
+static void free_stream(const or_case *cs, double *u_inf, double *p_inf) {
+  *u_inf = 0.0;
+  *p_inf = 0.0;
+  for (int gi = 0; gi < cs->nbc; gi++)
+    if (cs->gb[gi].btype == OR_BC_FARFIELD) {
+      *u_inf = norm3(cs->gb[gi].uvel);
+      *p_inf = cs->gb[gi].pval;
+      return;
+    }
+}
+
+/* compute_forces (mod_uns_forces.f90:43-185).  out[14]: Fx Fy Fz CD CL u_inf
+ * A_ref p_inf, then the pressure parts Fx_p Fy_p Fz_p and the viscous parts
+ * Fx_v Fy_v Fz_v.  Per-group sums in face order, then the group sums in
+ * group order (Fortran SUM over the group arrays). */
+void or_compute_forces(or_case *cs, double *out) {
+  for (int k = 0; k < 14; k++) out[k] = 0.0;
+  double u_inf, p_inf;
+  free_stream(cs, &u_inf, &p_inf);
+  out[5] = u_inf;
+  out[7] = p_inf;
+  int nwall = 0;
+  for (int gi = 0; gi < cs->nbc; gi++)
+    if (cs->gb[gi].btype == OR_BC_WALL) nwall++;
+  if (nwall == 0) {
+    out[5] = u_inf; out[7] = p_inf;
+    return;
+  }
+  double xmin = DBL_MAX, xmax = -DBL_MAX;
+  for (int i = 0; i < cs->nf; i++) {
+    if (!is_wall_face(cs, i)) continue;
+    xmin = fmin0(xmin, cs->xf[3 * (size_t)i]);
+    xmax = fmax0(xmax, cs->xf[3 * (size_t)i]);
+  }
+  const double a_ref = (xmax - xmin) * cs->zspan;
+  double *z = xcalloc(6 * (size_t)cs->nbc, sizeof(double));   /* fx fy fz vfx vfy vfz per group */
+  const double mu = cs->ctrl.mu;
+  for (int i = 0; i < cs->nf; i++) {
+    if (!is_wall_face(cs, i)) continue;
+    double *zg = &z[6 * (size_t)(cs->fgrp[i] - 1)];
+    const int c0 = cs->c0[i] - 1;
+    const double *sf = &cs->sf[3 * (size_t)i], *xf = &cs->xf[3 * (size_t)i];
+    const double pg = cs->p[c0] - p_inf;
+    zg[0] = zg[0] + pg * sf[0];
+    zg[1] = zg[1] + pg * sf[1];
+    zg[2] = zg[2] + pg * sf[2];
+    if (mu > 0.0) {
+      const double nx = sf[0] / norm3(sf), ny = sf[1] / norm3(sf);
+      const double tx = ny, ty = -nx, tz = 0.0;
+      const double d[3] = {xf[0] - cs->xc[3 * (size_t)c0], xf[1] - cs->xc[3 * (size_t)c0 + 1],
+                           xf[2] - cs->xc[3 * (size_t)c0 + 2]};
+      const double dn = norm3(d);
+      const double *u = &cs->u[3 * (size_t)c0];
+      const double ut = u[0] * tx + u[1] * ty + u[2] * tz;
+      const double tau = mu * ut / dn;
+      zg[3] = zg[3] + tau * tx * cs->area[i];
+      zg[4] = zg[4] + tau * ty * cs->area[i];
+      zg[5] = zg[5] + tau * tz * cs->area[i];
+    }
+  }
+  double f[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int k = 0; k < 6; k++)
+    for (int gi = 0; gi < cs->nbc; gi++) f[k] = f[k] + z[6 * (size_t)gi + k];
+  free(z);
+  out[0] = f[0] + f[3];
+  out[1] = f[1] + f[4];
+  out[2] = f[2] + f[5];
+  out[6] = a_ref;
+  for (int k = 0; k < 6; k++) out[8 + k] = f[k];
+  if (u_inf > 0.0 && a_ref > 0.0) {
+    const double q = 0.5 * cs->ctrl.rho * (u_inf * u_inf);
+    out[3] = out[0] / (q * a_ref);
+    out[4] = out[1] / (q * a_ref);
+  }
+}
+
+/* centre of the wall faces in x-y (mean of their centroids); returns their count */
+static int wall_centre(const or_case *cs, double *xcyl, double *ycyl) {
+  double sx = 0.0, sy = 0.0;
+  int n = 0;
+  for (int i = 0; i < cs->nf; i++) {
+    if (!is_wall_face(cs, i)) continue;
+    sx = sx + cs->xf[3 * (size_t)i];
+    sy = sy + cs->xf[3 * (size_t)i + 1];
+    n++;
+  }
+  if (n > 0) {
+    sx = sx / (double)n;
+    sy = sy / (double)n;
+  }
+  *xcyl = sx;
+  *ycyl = sy;
+  return n;
+}
+
+/* write_cp_distribution (mod_uns_forces.f90:195-254) as arrays, one row per
+ * wall face in face order: theta (deg), Cp, Cp_exact = 1 - 4 sin^2(theta),
+ * x, y, z.  Returns the number of rows (0 without a far-field group or a
+ * wall). */
+int or_cp_distribution(or_case *cs, double *rows6) {
+  double u_inf, p_inf;
+  free_stream(cs, &u_inf, &p_inf);
+  if (u_inf <= 0.0) return 0;
+  const double q = 0.5 * cs->ctrl.rho * (u_inf * u_inf);
+  double xcyl, ycyl;
+  if (wall_centre(cs, &xcyl, &ycyl) == 0) return 0;
+  const double pi = 3.141592653589793238462643;
+  int k = 0;
+  for (int i = 0; i < cs->nf; i++) {
+    if (!is_wall_face(cs, i)) continue;
+    const int c0 = cs->c0[i] - 1;
+    const double *xf = &cs->xf[3 * (size_t)i];
+    const double th = atan2(xf[1] - ycyl, xf[0] - xcyl);
+    const double sn = sin(th);
+    double *r = &rows6[6 * (size_t)k++];
+    r[0] = th * 180.0 / pi;
+    r[1] = (cs->p[c0] - p_inf) / q;
+    r[2] = 1.0 - 4.0 * (sn * sn);
+    r[3] = xf[0];
+    r[4] = xf[1];
+    r[5] = xf[2];
+  }
+  return k;
+}
+
+/* compute_separation (mod_uns_forces.f90:275-407).  out[5]: top separation
+ * angle (rad, from +x), found_top, bottom separation angle, found_bot, wake
+ * length x_wake_end - xcyl. */
+void or_separation(or_case *cs, double *out) {
+  for (int k = 0; k < 5; k++) out[k] = 0.0;
+  double xcyl, ycyl;
+  const int n = wall_centre(cs, &xcyl, &ycyl);
+  if (n == 0) return;
+  double *th = xcalloc(n, sizeof(double)), *ua = xcalloc(n, sizeof(double));
+  int j = 0;
+  for (int i = 0; i < cs->nf; i++) {
+    if (!is_wall_face(cs, i)) continue;
+    const int c0 = cs->c0[i] - 1;
+    const double *sf = &cs->sf[3 * (size_t)i], *xf = &cs->xf[3 * (size_t)i];
+    const double nx = sf[0] / norm3(sf), ny = sf[1] / norm3(sf);
+    const double tx = ny, ty = -nx;
+    th[j] = atan2(xf[1] - ycyl, xf[0] - xcyl);
+    ua[j] = cs->u[3 * (size_t)c0] * tx + cs->u[3 * (size_t)c0 + 1] * ty;
+    j++;
+  }
+  for (int i = 1; i < n; i++) {   /* insertion sort by theta, stable */
+    const double t = th[i], u = ua[i];
+    int k = i - 1;
+    while (k >= 0 && th[k] > t) {
+      th[k + 1] = th[k];
+      ua[k + 1] = ua[k];
+      k--;
+    }
+    th[k + 1] = t;
+    ua[k + 1] = u;
+  }
+  double up = 0.0, tp = 0.0;
+  int ftop = 0, fbot = 0;
+  for (int i = 0; i < n; i++) {
+    if (fabs(ua[i]) < 1.0e-4) continue;
+    if (ua[i] < 0.0 && up > 0.0) {
+      const double ts = tp + (0.0 - up) * (th[i] - tp) / (ua[i] - up);
+      if (th[i] < 0.0 && !fbot) {
+        out[2] = ts;
+        fbot = 1;
+      } else if (th[i] > 0.0 && !ftop) {
+        out[0] = ts;
+        ftop = 1;
+      }
+    }
+    up = ua[i];
+    tp = th[i];
+  }
+  double xw = xcyl;
+  for (int k = 0; k < cs->nc; k++)
+    if (cs->u[3 * (size_t)k] < 0.0 && cs->xc[3 * (size_t)k] > xcyl) xw = fmax0(xw, cs->xc[3 * (size_t)k]);
+  out[1] = ftop;
+  out[3] = fbot;
+  out[4] = (xw - xcyl) / 1.0;
+  free(th);
+  free(ua);
 }

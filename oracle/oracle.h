@@ -116,6 +116,16 @@ const double *or_history(or_case *, int *nrows);
  * number of p' PCG iterations it performed (CPU-baseline extrapolation) */
 void or_timing(or_case *, double *out3);
 
+/* wall-zone post-processing (mod_uns_forces.f90) on the current fields:
+ * compute_forces -> out14 = Fx Fy Fz CD CL u_inf A_ref p_inf, pressure parts
+ * Fx_p Fy_p Fz_p, viscous parts Fx_v Fy_v Fz_v; write_cp_distribution ->
+ * rows6 (theta deg, Cp, Cp_exact, x, y, z) per wall face, returns the row
+ * count; compute_separation -> out5 = theta_top, found_top, theta_bot,
+ * found_bot, wake length */
+void or_compute_forces(or_case *, double *out14);
+int or_cp_distribution(or_case *, double *rows6);
+void or_separation(or_case *, double *out5);
+
 /* linear solvers on a 1-based CSR (csr_t), mod_uns_linsolver(_mpi).f90 */
 void or_cg_jacobi(int n, const int *rp, const int *ci, const double *v,
                   const double *b, double *x, double tol, int maxit,

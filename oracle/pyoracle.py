@@ -59,6 +59,9 @@ def lib():
         L.or_ic0_factor.argtypes = [C.c_int, _I, _I, _D]
         L.or_ilu0_factor.argtypes = [C.c_int, _I, _I, _D]
         L.or_csr_matvec.argtypes = [C.c_int, _I, _I, _D, _D, _D]
+        L.or_compute_forces.argtypes = [C.c_void_p, _D]
+        L.or_cp_distribution.argtypes = [C.c_void_p, _D]
+        L.or_separation.argtypes = [C.c_void_p, _D]
         L.or_norm2.restype = C.c_double
         L.or_norm2.argtypes = [_D, C.c_int]
         _lib = L
@@ -213,12 +216,35 @@ class OracleCase:
         lib().or_timing(self.h, _dp(out))
         return float(out[0]), float(out[1]), int(out[2])
 
+    def compute_forces(self):
+        """compute_forces: dict of Fx Fy Fz CD CL u_inf A_ref p_inf and the
+        pressure / viscous parts."""
+        out = np.zeros(14)
+        lib().or_compute_forces(self.h, _dp(out))
+        return dict(zip(FORCE_KEYS, out.tolist()))
+
+    def cp_distribution(self):
+        """write_cp_distribution rows: (n_wall, 6) theta deg, Cp, Cp_exact, x, y, z."""
+        rows = np.zeros(6 * self.nf)
+        n = lib().or_cp_distribution(self.h, _dp(rows))
+        return rows[: 6 * n].reshape(n, 6)
+
+    def separation(self):
+        out = np.zeros(5)
+        lib().or_separation(self.h, _dp(out))
+        return dict(theta_top=out[0], found_top=bool(out[1]), theta_bot=out[2], found_bot=bool(out[3]),
+                    wake_length=out[4])
+
     def history(self):
         n = C.c_int(0)
         p = lib().or_history(self.h, C.byref(n))
         if n.value == 0:
             return np.zeros((0, 6))
         return np.ctypeslib.as_array(p, shape=(n.value * 6,)).reshape(-1, 6).copy()
+
+
+FORCE_KEYS = ("Fx", "Fy", "Fz", "CD", "CL", "u_inf", "A_ref", "p_inf", "Fx_p", "Fy_p", "Fz_p", "Fx_v", "Fy_v",
+              "Fz_v")
 
 
 def solve(kind, row_ptr, col_idx, val, b, x0, tol, maxit, mpi=False):

@@ -15,6 +15,9 @@
 !   funcs2 : funcs with T_s / T_s_old / T_s_old_old seeded too, preceded by the
 !            resolved cell types and porous coefficients, with the solid-phase
 !            gradient and solid_temperature_assembly added
+!   forces : seed fields from in.bin (u,p,T), then compute_forces (written),
+!            write_cp_distribution (to cp.dat in the working directory) and
+!            compute_separation (printed)
 !   solve  : CSR system from in.bin; residual histories of cg_jacobi, cg_ic0,
 !            bicgstab_ilu0 for maxit = 1..n, IC0/ILU0 factors, final solutions
 ! All arrays are written as raw little-endian stream records in the order the
@@ -31,6 +34,7 @@ program ref_harness
    use mod_uns_fields
    use mod_uns_linsolver
    use mod_uns_simple
+   use mod_uns_forces, only: compute_forces, write_cp_distribution, compute_separation
    implicit none
 
    character(len=512) :: casf, ctlf, mode, outf, inf, narg
@@ -90,6 +94,9 @@ program ref_harness
       call pimple_run( m, c, g, ctrl, bcs, fld, '/tmp/ref_harness_pimple', ier )
       write(uo) ier
       call dump_state()
+   case ( 'forces' )
+      call seed_fields()
+      call run_forces()
    case ( 'funcs' )
       call run_funcs()
    case ( 'funcs2' )
@@ -131,6 +138,14 @@ contains
       close( ui )
       fld%T_s = fld%T
    end subroutine seed_fields
+
+   subroutine run_forces()
+      real(dp) :: Fx, Fy, Fz, CD, CL, u_inf, A_ref, p_inf
+      call compute_forces( m, g, ctrl, bcs, fld, Fx, Fy, Fz, CD, CL, u_inf, A_ref, p_inf )
+      write(uo) Fx, Fy, Fz, CD, CL, u_inf, A_ref, p_inf
+      call write_cp_distribution( m, g, ctrl, bcs, fld, 'cp.dat' )
+      call compute_separation( m, g, ctrl, bcs, fld )
+   end subroutine run_forces
 
    subroutine dump_state()
       write(uo) fld%u, fld%p, fld%flux, fld%T, fld%gp, fld%gu, fld%apc, fld%gt

@@ -538,6 +538,54 @@ def heat_conduction_known_answer():
     return out
 
 
+REF_CYL = "/root/reference/cluster_cases/cylinder_Re40"
+# cluster_cases/cylinder_Re40/README.md:48-56 (this bundle's column)
+CYL_README = dict(readme_cp_front=1.5018, readme_cp_base=-0.4925, readme_cp_min=-0.9863, readme_phi_cp_min=-82.9,
+                  readme_sep=np.array([51.7, 51.6]), readme_wake_D=2.3420, readme_umax=1.216703,
+                  readme_max_reverse=-0.10211, readme_cd_pressure=1.0766, readme_dcp_mean=0.0082,
+                  readme_dcp_max=0.1146)
+
+
+def cylinder_known_answer():
+    """The reference's shipped Re=40 cylinder (3928-cell O-grid, far field
+    at r = 10, limited second-order upwind, IC0 PCG, 8000 SIMPLE iterations:
+    the case stops at the iteration cap by design).  The harness re-runs
+    simple_run, then compute_forces / write_cp_distribution /
+    compute_separation on the final fields; the fixture keeps the final u
+    and p, the force outputs (full precision) and the pressure / viscous
+    parts (printed), the Cp table and separation lines as written, the
+    archived wall-Cp baseline shipped with the case and the README's
+    regression indicators."""
+    import re
+    m = cas.read_cas(os.path.join(REF_CYL, "unMesh.cas"))
+    ctl = open(os.path.join(REF_CYL, "unMesh.control")).read()
+    nc, nf = m.ncells, m.nfaces
+    with tempfile.TemporaryDirectory() as td:
+        shutil.copy(os.path.join(REF_CYL, "unMesh.cas"), os.path.join(td, "m.cas"))
+        open(os.path.join(td, "c.control"), "w").write(ctl)
+        run(["m.cas", "c.control", "simple", "out.bin", "", "8000"], td)
+        r = Reader(os.path.join(td, "out.bin"))
+        ier = r.take(np.int32, 1)[0]
+        u = r.take(np.float64, 3 * nc).reshape(nc, 3)
+        p = r.take(np.float64, nc)
+        write_fields(os.path.join(td, "in.bin"), u, p, np.zeros(nc))
+        log = run(["m.cas", "c.control", "forces", "fo.bin", "in.bin", "1"], td)
+        forces = np.fromfile(os.path.join(td, "fo.bin"), np.float64)
+        cp = np.loadtxt(os.path.join(td, "cp.dat"), comments="#")
+
+    def num(label):
+        return float(re.search(re.escape(label) + r"\s*:\s*(\S+)", log).group(1))
+
+    printed = np.array([num("Pressure  Fx (drag)"), num("Viscous   Fx (drag)"), num("CD_p (pressure)"),
+                        num("CD_v (viscous)")])
+    sep = np.array([num("Top separation theta"), num("Bot separation theta"),
+                    num("Wake length (x_wake - xcyl)/D")])
+    base = np.loadtxt(os.path.join(REF_CYL, "reference_cylinder_cp.dat"), comments="#")
+    return dict(**mesh_arrays(m), control=ctl, nsteps=np.int64(8000), ref_ier=np.int64(ier), ref_u=u, ref_p=p,
+                ref_forces=forces, ref_forces_printed=printed, ref_cp_table=cp, ref_separation_printed=sep,
+                archived_cp=base[:, :2], **CYL_README)
+
+
 def main():
     if not os.path.exists(HARNESS):
         sys.exit("build the reference harness first: oracle/build_ref.sh")
@@ -580,6 +628,10 @@ def main():
         save("cavity_re1000_known_answer", control=ctlk, nsteps=np.int64(1000), **mesh_arrays(mk),
              out_ier=np.int64(ierk), ref_umax=umax, ref_ghia_l2=l2,
              readme_umax=0.911512, readme_ghia_l2=0.0289706)
+
+    # 3f. the reference's shipped Re=40 cylinder: forces, Cp, separation, published indicators
+    if os.path.isdir(REF_CYL):
+        save("cylinder_re40_known_answer", **cylinder_known_answer())
 
     # 3e. the reference's pure-conduction family and its published errors
     if os.path.isdir(REF_HC):
@@ -664,6 +716,8 @@ def main():
 if __name__ == "__main__":
     if sys.argv[1:] == ["--porous"]:
         porous_fixtures()
+    elif sys.argv[1:] == ["--cyl"]:
+        save("cylinder_re40_known_answer", **cylinder_known_answer())
     elif sys.argv[1:] == ["--hc"]:
         save("heat_conduction_known_answer", **heat_conduction_known_answer())
     elif sys.argv[1:] == ["--bj"]:

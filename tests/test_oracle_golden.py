@@ -333,3 +333,96 @@ def test_conduction_known_answer(c):
     ier, it = oc.simple_run(3000, mpi=False)
     assert ier == 0 and it == 1
     assert np.array_equal(oc.get("T_s"), ts)
+
+
+# ------------------------------------------- Re=40 cylinder: forces and indicators
+def _cyl_case(d):
+    ctrl, groups = parse_control(str(d["control"]), warn=False)
+    oc = pyoracle.OracleCase(mesh_from(d), ctrl, groups)
+    oc.set_field("u", d["ref_u"])
+    oc.set_field("p", d["ref_p"])
+    return oc
+
+
+def test_cylinder_known_answer_fixture_matches_published():
+    """The reference's shipped Re=40 cylinder after its 8000 SIMPLE
+    iterations gives the README's regression indicators (its own check
+    script restated in post.cylinder_check): Cp at the front stagnation
+    point, the base and the suction peak, the separation angles, the
+    recirculation length, max|u|, the strongest reverse flow, the ring's
+    pressure drag and the Cp deviation from the archived wall baseline."""
+    from mixsolver_amd.post import cylinder_check
+    d = load_golden("cylinder_re40_known_answer")
+    assert int(d["ref_ier"]) == 0
+    xc = _cyl_case(d).geometry()["xc"].reshape(-1, 3)
+    a = d["archived_cp"]
+    c = cylinder_check(xc, d["ref_u"], d["ref_p"], cp_ref=(a[:, 0], a[:, 1]))
+    f4 = lambda v: float("%.4f" % v)
+    assert c["ring_cells"] == 80
+    assert f4(c["cp_front"]) == d["readme_cp_front"]
+    assert f4(c["cp_base"]) == d["readme_cp_base"]
+    assert f4(c["cp_min"]) == d["readme_cp_min"]
+    assert float("%.1f" % c["phi_cp_min"]) == d["readme_phi_cp_min"]
+    assert [float("%.1f" % v) for v in c["separation_from_wake_axis"]] == list(d["readme_sep"])
+    assert f4(c["wake_length_D"]) == d["readme_wake_D"]
+    assert float("%.6f" % c["umax"]) == d["readme_umax"]
+    assert float("%.5f" % c["max_reverse_ux"]) == d["readme_max_reverse"]
+    assert f4(c["cd_pressure"]) == d["readme_cd_pressure"]
+    assert f4(c["dcp_mean"]) == d["readme_dcp_mean"] and f4(c["dcp_max"]) == d["readme_dcp_max"]
+
+
+def test_cylinder_forces_oracle_matches_reference():
+    """compute_forces on the final fields: the oracle's Fx Fy Fz CD CL u_inf
+    A_ref p_inf equal the reference's bit for bit, its pressure / viscous
+    parts print as the reference printed them; the Cp rows equal the
+    reference's table to its printed digits (f10.3, es12.4) and the
+    separation angles and wake length to theirs (f8.2, f8.3)."""
+    d = load_golden("cylinder_re40_known_answer")
+    oc = _cyl_case(d)
+    f = oc.compute_forces()
+    ref = d["ref_forces"]
+    for i, k in enumerate(pyoracle.FORCE_KEYS[:8]):
+        assert f[k] == ref[i], k
+    q_a = 0.5 * 1.0 * f["u_inf"] ** 2 * f["A_ref"]
+    printed = ["%.6E" % v for v in d["ref_forces_printed"]]
+    assert ["%.6E" % v for v in (f["Fx_p"], f["Fx_v"], f["Fx_p"] / q_a, f["Fx_v"] / q_a)] == printed
+    cp, rt = oc.cp_distribution(), d["ref_cp_table"]
+    assert cp.shape == rt.shape == (80, 6)
+    assert ["%10.3f" % v for v in cp[:, 0]] == ["%10.3f" % v for v in rt[:, 0]]
+    for j in (1, 2):
+        assert ["%.4E" % v for v in cp[:, j]] == ["%.4E" % v for v in rt[:, j]]
+    s = oc.separation()
+    assert s["found_top"] and s["found_bot"]
+    got = [float("%.2f" % np.degrees(s["theta_top"])), float("%.2f" % np.degrees(s["theta_bot"])),
+           float("%.3f" % s["wake_length"])]
+    assert got == list(d["ref_separation_printed"])
+
+
+def test_forces_closed_forms():
+    """compute_forces on a box with one wall zone (the y = 0 side of a
+    channel): a uniform gauge pressure p0 gives F = p0 * sum of the wall
+    face vectors (they point out of the fluid, -y: Fy = -p0 * area), and a
+    uniform tangential velocity U at the wall cells gives the wall shear
+    mu * U / dn * area along t = n x z; the reference area is the x extent
+    of the wall face centroids times the z extent of the nodes."""
+    from mixsolver_amd import meshgen
+    from mixsolver_amd.control import parse_control as pc
+    m = meshgen.hex_box(np.linspace(0.0, 3.0, 7), np.linspace(0.0, 2.0, 5), np.linspace(0.0, 1.0, 3),
+                        zone_ids=dict(ymin=6))
+    ctl = "rho = 1.0\nmu = 0.1\nbc = 6 wall\nbc = 5 symmetry\nbc = 4 symmetry\n"
+    ctrl, groups = pc(ctl, warn=False)
+    oc = pyoracle.OracleCase(m, ctrl, groups)
+    nc = m.ncells
+    oc.set_field("p", np.full(nc, 2.5))
+    u = np.zeros((nc, 3))
+    u[:, 0] = 0.4
+    oc.set_field("u", u)
+    f = oc.compute_forces()
+    wall_area = 3.0 * 1.0
+    assert f["u_inf"] == 0.0 and f["CD"] == 0.0
+    assert f["Fy_p"] == pytest.approx(-2.5 * wall_area, rel=1e-14)
+    assert abs(f["Fx_p"]) <= 1e-14 and abs(f["Fz_p"]) <= 1e-14
+    # wall-adjacent cell centres sit dn = (2.0 / 4) / 2 above the wall; n = -y, t = n x z = (-1, 0, 0)
+    assert f["Fx_v"] == pytest.approx(0.1 * 0.4 * -1.0 / 0.25 * -1.0 * wall_area, rel=1e-13)
+    # A_ref = x extent of the wall face centroids (0.25 .. 2.75) times the node z extent
+    assert f["A_ref"] == pytest.approx(2.5 * 1.0, rel=1e-14)
