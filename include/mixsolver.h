@@ -53,7 +53,7 @@ typedef struct {
   int gpu_ppe_solver;    /* 0 = PCG (reference), 1 = BiCGSTAB                */
   double gpu_inner_tol;  /* relative tolerance of each FP32 inner solve      */
   int gpu_reorder;       /* 0 = natural order, 1 = RCM                       */
-  int gpu_coarse;        /* p' PCG coarse space: number of aggregates, 0 = off (max 256) */
+  int gpu_coarse;        /* p' PCG coarse space: number of aggregates, 0 = off (max 1024) */
   /* Periodic face pairs (no reference counterpart; the reference's BC list
    * has no periodic type).  Pair i joins every boundary face of zone
    * periodic_zone[i][0] to the face of zone periodic_zone[i][1] whose
@@ -90,6 +90,11 @@ typedef struct ms_system ms_system;
 int ms_init(int device);                 /* select device, create streams   */
 const char *ms_last_error(void);
 int ms_version(void);
+/* Replay the mixed-precision PCG's 16-iteration chunks from a captured HIP
+ * graph (1, default; the environment variable MIXSOLVER_GRAPHS=0 sets 0 at
+ * start-up) or launch their kernels one by one (0).  Results are identical
+ * either way.  No reference counterpart (library option). */
+int ms_set_graphs(int on);
 
 /* ---- system = one mesh + fields + matrices resident on the device ------
  * Replaces the setup half of uns_run_mpi / uns_solver_init

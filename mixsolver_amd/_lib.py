@@ -56,6 +56,7 @@ def lib():
         "ms_pimple_run": (C.c_int, [_V, C.c_int, _I]),
         "ms_get_history": (C.c_int, [_V, _D, C.c_int, _I]),
         "ms_compute_forces": (C.c_int, [_V, _D]),
+        "ms_set_graphs": (C.c_int, [C.c_int]),
         "ms_cp_distribution": (C.c_int, [_V, _D, C.c_int, _I]),
         "ms_separation": (C.c_int, [_V, _D]),
         "ms_compute_gradients": (C.c_int, [_V]),

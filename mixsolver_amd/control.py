@@ -15,7 +15,7 @@ stay compatible with the reference reader:
                    (default 0.1)
   gpu_reorder      none | rcm                     (default none)
   gpu_coarse       p' PCG two-level preconditioner: number of cell aggregates
-                   (recursive coordinate bisection), 0 = off, at most 256
+                   (recursive coordinate bisection), 0 = off, at most 1024
                    (default 0)
   periodic         zoneA zoneB [tx ty tz]: join the faces of zoneA to the faces
                    of zoneB translated by (tx, ty, tz) (default: the difference
@@ -322,7 +322,7 @@ def parse_control(text: str, warn=True):
                 ctrl.gpu_reorder = {"none": 0, "rcm": 1}[val.lower()]
             elif key == "gpu_coarse":
                 ctrl.gpu_coarse = int(val)
-                if not 0 <= ctrl.gpu_coarse <= 256:
+                if not 0 <= ctrl.gpu_coarse <= 1024:
                     raise ValueError(val)
             elif key == "porous_model":
                 v = val.lower()

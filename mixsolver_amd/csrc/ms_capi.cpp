@@ -116,6 +116,7 @@ extern "C" {
 int ms_init(int device) { return guarded([&] { ensure_runtime(device); }); }
 const char *ms_last_error(void) { return last_error(); }
 int ms_version(void) { return 100; }
+int ms_set_graphs(int on) { return guarded([&] { set_graphs(on != 0); }); }
 
 int ms_system_create(ms_system **out, int nnodes, const double *x, int nfaces, const long long *fptr,
                      const int *fnodes, const int *c0, const int *c1, const int *zone, int ncells,

@@ -30,6 +30,11 @@ void set_error(const std::string &m);
 
 hipStream_t stream();           // library stream (default priority)
 void ensure_runtime(int device); // select the device, create the library stream
+// replay the mixed-precision PCG chunk (16 iterations) from a captured HIP
+// graph instead of launching its kernels one by one (default on; the
+// environment variable MIXSOLVER_GRAPHS=0 or ms_set_graphs(0) turns it off)
+bool graphs_enabled();
+void set_graphs(bool on);
 
 // --------------------------------------------------------- device buffers
 template <class T>
@@ -89,7 +94,8 @@ inline int nblocks(long long n, int b = kBlock) { return (int)((n + b - 1) / b);
 // into chunks of at most kBlock rows that each lie in one aggregate; the
 // cell-wise PCG kernels run one workgroup per chunk, which turns Z^T r into
 // per-workgroup partial sums and Z nu into a broadcast.
-constexpr int kMaxAgg = 256;
+constexpr int kMaxAgg = 1024;
+constexpr int kInvRegMax = 256;   // largest E of the one-workgroup register inverter (request server)
 struct Coarse {
   int k = 0;                          // number of aggregates (0 = none)
   int nchunks = 0;
@@ -100,6 +106,7 @@ struct Coarse {
   std::vector<int> e_slot;            // matrix slots, grouped by key, ascending
   DBuf<int> d_chunk_ptr, d_chunk_agg, d_agg_cptr, d_agg_cidx, d_e_ptr, d_e_key, d_e_slot;
   DBuf<double> d_E, d_nu;             // dense E (k x k), coarse solution nu (k)
+  DBuf<double> d_W, d_rc;             // Gauss-Jordan work matrix (k x k), pivot row/column snapshots (4 k)
   DBuf<float> d_Einv;                 // dense E^-1 (k x k), FP32
   void upload();
   size_t device_bytes() const;

@@ -503,24 +503,31 @@ __device__ void coarse_ride(const CoWork<T> &cw, int g) {
   if (t == 0) cw.cnu_part[g] = (T)(((pr[0] + pr[1]) + pr[2]) + pr[3]);
 }
 
-// Coarse solve by the whole finisher workgroup (kFinThreads threads): four
-// threads per aggregate (quarter q = t / 256) each sum every fourth chunk
-// partial (the last of the NV values) and every fourth block of E^-1
-// columns; quarters are combined in q order.  Writes nu and co.st->cnu.
+// Coarse solve by the whole finisher workgroup (kFinThreads threads): Q
+// threads per aggregate (Q = 4, 2, 1 for k <= 256, 512, 1024; part q =
+// t / (kFinThreads / Q)) each sum every Q-th chunk partial (the last of the
+// NV values) and one Q-th of the E^-1 columns; the parts are combined in q
+// order.  Writes nu and co.st->cnu.
 template <int NV, class T>
 __device__ void coarse_solve(const T *partial, const CoDev &co) {
-  __shared__ double c[kMaxAgg], part4[4][kMaxAgg];
-  const int a = threadIdx.x & (kMaxAgg - 1), q = threadIdx.x / kMaxAgg;
+  __shared__ double c[kMaxAgg], partq[kFinThreads];
+  const int Q = co.k <= kFinThreads / 4 ? 4 : (co.k <= kFinThreads / 2 ? 2 : 1);
+  const int span = kFinThreads / Q;
+  const int a = threadIdx.x % span, q = threadIdx.x / span;
   if (a < co.k) {
     double s = 0.0;
-    for (int j = co.agg_cptr[a] + q; j < co.agg_cptr[a + 1]; j += 4)
+    for (int j = co.agg_cptr[a] + q; j < co.agg_cptr[a + 1]; j += Q)
       s = s + (double)partial[co.agg_cidx[j] * NV + NV - 1];
-    part4[q][a] = s;
+    partq[q * span + a] = s;
   }
   __syncthreads();
-  if (q == 0 && a < co.k) c[a] = ((part4[0][a] + part4[1][a]) + part4[2][a]) + part4[3][a];
+  if (q == 0 && a < co.k) {
+    double s = partq[a];
+    for (int qq = 1; qq < Q; qq++) s = s + partq[qq * span + a];
+    c[a] = s;
+  }
   __syncthreads();
-  const int kq = (co.k + 3) / 4, b0 = q * kq, b1 = min(co.k, b0 + kq);
+  const int kq = (co.k + Q - 1) / Q, b0 = q * kq, b1 = min(co.k, b0 + kq);
   if (a < co.k) {
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;    // E^-1 is symmetric: column a
     int b = b0;
@@ -533,12 +540,13 @@ __device__ void coarse_solve(const T *partial, const CoDev &co) {
       s3 = s3 + (double)e3 * c[b + 3];
     }
     for (; b < b1; b++) s0 = s0 + (double)co.Einv[(size_t)b * co.k + a] * c[b];
-    part4[q][a] = ((s0 + s1) + s2) + s3;
+    partq[q * span + a] = ((s0 + s1) + s2) + s3;
   }
   __syncthreads();
   double prod[1] = {0.0};
   if (q == 0 && a < co.k) {
-    const double nu = ((part4[0][a] + part4[1][a]) + part4[2][a]) + part4[3][a];
+    double nu = partq[a];
+    for (int qq = 1; qq < Q; qq++) nu = nu + partq[qq * span + a];
     co.nu[a] = nu;
     prod[0] = nu * c[a];
   }
@@ -574,6 +582,43 @@ __global__ void k_coarse_E(int nE, const int *e_ptr, const int *e_key, const int
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) s = s + __shfl_down(s, o, 64);
   if (lane == 0) E[e_key[e]] = s;
+}
+
+// Gauss-Jordan inversion of the SPD matrix E (no pivoting) in FP64 over the
+// whole grid, one launch per pivot j: W holds the current matrix; row j and
+// column j of it come from snapshots written by the previous launch (rowj,
+// colj), so every entry is read and written by its own thread only, and the
+// threads of row / column j + 1 write the next snapshots.
+//   a_jj <- 1/p,  a_jc <- a_jc/p,  a_rj <- -a_rj/p,  a_rc <- a_rc - a_rj a_jc/p.
+__global__ void k_gj_init(int k, const double *E, double *W, double *row0, double *col0) {
+  const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= (long long)k * k) return;
+  const int r = (int)(t / k), c = (int)(t % k);
+  const double v = E[t];
+  W[t] = v;
+  if (r == 0) row0[c] = v;
+  if (c == 0) col0[r] = v;
+}
+
+__global__ void k_gj_step(int k, int j, double *W, const double *rowj, const double *colj, double *rown,
+                          double *coln) {
+  const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= (long long)k * k) return;
+  const int r = (int)(t / k), c = (int)(t % k);
+  const double inv = 1.0 / rowj[j];
+  const double rowc = rowj[c] * inv;
+  double v;
+  if (r == j) v = (c == j) ? inv : rowc;
+  else if (c == j) v = -colj[r] * inv;
+  else v = W[t] - colj[r] * rowc;
+  W[t] = v;
+  if (r == j + 1) rown[c] = v;
+  if (c == j + 1) coln[r] = v;
+}
+
+__global__ void k_gj_out(long long kk, const double *W, float *Einv) {
+  const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (t < kk) Einv[t] = (float)W[t];
 }
 
 // In-place Gauss-Jordan inversion of the SPD matrix E (k <= 256, no

@@ -578,6 +578,7 @@ Krylov::Krylov(SellPattern *pat_) : pat(pat_), n(pat_->n) {
 }
 
 Krylov::~Krylov() {
+  if (chunk_exec_) (void)hipGraphExecDestroy(chunk_exec_);
   if (hpin_) (void)hipHostFree(hpin_);
 }
 
@@ -820,7 +821,18 @@ void Krylov::coarse_setup(const double *val) {
   hipLaunchKernelGGL(k_fill64, dim3(nblocks(kk)), dim3(kBlock), 0, stream(), kk, C.d_E.p, 0.0);
   hipLaunchKernelGGL(k_coarse_E, dim3(nblocks(nE, kBlock / 64)), dim3(kBlock), 0, stream(), nE, C.d_e_ptr.p,
                      C.d_e_key.p, C.d_e_slot.p, val, C.d_E.p);
-  hipLaunchKernelGGL(k_coarse_inv, dim3(1), dim3(kInvThreads), 0, stream(), C.k, C.d_E.p, C.d_Einv.p);
+  // FP64 Gauss-Jordan over the grid, one launch per pivot (k <= 1024)
+  if (C.d_W.n < (size_t)kk) C.d_W.alloc(kk);
+  if (C.d_rc.n < 4 * (size_t)C.k) C.d_rc.alloc(4 * (size_t)C.k);
+  double *rc[2][2] = {{C.d_rc.p, C.d_rc.p + C.k}, {C.d_rc.p + 2 * C.k, C.d_rc.p + 3 * C.k}};
+  const dim3 g(nblocks(kk));
+  hipLaunchKernelGGL(k_gj_init, g, dim3(kBlock), 0, stream(), C.k, C.d_E.p, C.d_W.p, rc[0][0], rc[0][1]);
+  for (int j = 0; j < C.k; j++) {
+    const int cur = j & 1, nxt = cur ^ 1;
+    hipLaunchKernelGGL(k_gj_step, g, dim3(kBlock), 0, stream(), C.k, j, C.d_W.p, rc[cur][0], rc[cur][1], rc[nxt][0],
+                       rc[nxt][1]);
+  }
+  hipLaunchKernelGGL(k_gj_out, g, dim3(kBlock), 0, stream(), kk, C.d_W.p, C.d_Einv.p);
   MS_CHECK_LAUNCH();
 }
 
@@ -1192,15 +1204,47 @@ SolveResult Krylov::mixed(const double *val, const double *b, double *x, double 
     MS_CHECK_LAUNCH();
   };
   constexpr int kChunk = 16;   // reliable updates are applied at most this many iterations late
-  hs = read_state<double>(st);
-  while (hs.done != 1 && hs.it < maxit) {
-    const int m = std::min(kChunk, maxit - hs.it);
+  auto chunk = [&](int m) {
     for (int it = 0; it < m; it++) {
       hipLaunchKernelGGL(k_pcg_spmv<float>, G, B, 0, S, n, A, v32, p, q, st, part32);
       finish<OpSum, 1>(nblk_, part32, FinPap{st});
       hipLaunchKernelGGL((k_pcg_update<float, float>), GR, B, 0, S, RW, p, q, y, r, z, dg, mode, c0end, st,
                          part32);
       tail(1);
+    }
+  };
+  hs = read_state<double>(st);
+  while (hs.done != 1 && hs.it < maxit) {
+    const int m = std::min(kChunk, maxit - hs.it);
+    if (m == kChunk && graphs_enabled()) {
+      // every argument of the chunk's launches is fixed by the key (device
+      // state lives in st, the iteration count in st->it), so one captured
+      // chunk replays for every full chunk of every solve with this set-up
+      const ChunkKey ck{v32, fac, dg, nu, (int)pc, (int)coarse, (int)fused};
+      if (!chunk_exec_ || !(chunk_key_ == ck)) {
+        if (chunk_exec_) MS_HIP(hipGraphExecDestroy(chunk_exec_));
+        chunk_exec_ = nullptr;
+        hipGraph_t g = nullptr;
+        MS_HIP(hipStreamBeginCapture(S, hipStreamCaptureModeRelaxed));
+        try {
+          chunk(kChunk);
+        } catch (...) {
+          (void)hipStreamEndCapture(S, &g);
+          if (g) (void)hipGraphDestroy(g);
+          throw;
+        }
+        MS_HIP(hipStreamEndCapture(S, &g));
+        const hipError_t e = hipGraphInstantiate(&chunk_exec_, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+          chunk_exec_ = nullptr;
+          fail(MS_EDEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        }
+        chunk_key_ = ck;
+      }
+      MS_HIP(hipGraphLaunch(chunk_exec_, S));
+    } else {
+      chunk(m);
     }
     MS_CHECK_LAUNCH();
     hs = read_state<double>(st);

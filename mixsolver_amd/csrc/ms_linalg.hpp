@@ -82,6 +82,17 @@ class Krylov {
       return val == o.val && pc == o.pc && fp32 == o.fp32 && coarse == o.coarse;
     }
   } last_;   // the set-up the preconditioner buffers currently hold
+  // captured 16-iteration chunk of the mixed PCG: valid while the kernels'
+  // arguments (buffers, preconditioner, coarse mode) are those of the key
+  struct ChunkKey {
+    const void *v32 = nullptr, *fac = nullptr, *dg = nullptr, *nu = nullptr;
+    int pc = -1, coarse = -1, fused = -1;
+    bool operator==(const ChunkKey &o) const {
+      return v32 == o.v32 && fac == o.fac && dg == o.dg && nu == o.nu && pc == o.pc && coarse == o.coarse &&
+             fused == o.fused;
+    }
+  } chunk_key_;
+  hipGraphExec_t chunk_exec_ = nullptr;
 };
 
 }  // namespace ms

@@ -460,7 +460,7 @@ class Server {
     if (o_.round_iters < 1) o_.round_iters = 16;
     if (o_.precond != MS_PC_JACOBI && o_.precond != MS_PC_DILU)
       fail(MS_EINVAL, "server preconditioner must be MS_PC_JACOBI or MS_PC_DILU");
-    if (o_.coarse < 0 || o_.coarse > kMaxAgg) fail(MS_EINVAL, "server coarse aggregates must be 0..256");
+    if (o_.coarse < 0 || o_.coarse > kInvRegMax) fail(MS_EINVAL, "server coarse aggregates must be 0..256");
     if (o_.budget <= 0) {
       size_t fr = 0, tot = 0;
       MS_HIP(hipMemGetInfo(&fr, &tot));

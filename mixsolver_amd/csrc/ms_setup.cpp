@@ -15,9 +15,11 @@
 // the SELL-64 matrix pattern, per-cell face lists with matrix slot maps,
 // level schedules for natural-order factorizations and per-face constants.
 #include <algorithm>
+#include <atomic>
 #include <cfloat>
 #include <climits>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <numeric>
@@ -53,6 +55,18 @@ hipStream_t stream() {
   if (!g_stream) ensure_runtime(-1);
   return g_stream;
 }
+
+static std::atomic<int> g_graphs{-1};   // -1: not yet read from the environment
+bool graphs_enabled() {
+  int v = g_graphs.load();
+  if (v < 0) {
+    const char *e = std::getenv("MIXSOLVER_GRAPHS");
+    v = (e && e[0] == '0') ? 0 : 1;
+    g_graphs.store(v);
+  }
+  return v == 1;
+}
+void set_graphs(bool on) { g_graphs.store(on ? 1 : 0); }
 
 // ------------------------------------------------------------------- norms
 double norm2_ref(const double *v, int n) {
@@ -402,7 +416,8 @@ void Coarse::upload() {
 
 size_t Coarse::device_bytes() const {
   return d_chunk_ptr.bytes() + d_chunk_agg.bytes() + d_agg_cptr.bytes() + d_agg_cidx.bytes() + d_e_ptr.bytes() +
-         d_e_key.bytes() + d_e_slot.bytes() + d_E.bytes() + d_nu.bytes() + d_Einv.bytes();
+         d_e_key.bytes() + d_e_slot.bytes() + d_E.bytes() + d_nu.bytes() + d_Einv.bytes() + d_W.bytes() +
+         d_rc.bytes();
 }
 
 // ---------------------------------------------------------------- system

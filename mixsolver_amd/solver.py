@@ -215,6 +215,13 @@ class Case:
         return dpp.reshape(-1, 3)
 
 
+def set_graphs(on: bool) -> None:
+    """Replay the mixed-precision PCG's 16-iteration chunks from a captured
+    HIP graph (default) or launch their kernels one by one; results are
+    identical either way."""
+    check(_lib.lib().ms_set_graphs(1 if on else 0))
+
+
 def coarse_aggregates(xc, k):
     """Recursive coordinate bisection of cell centres xc (n,3) into k
     aggregates of equal size (+-1): the aggregation of the p' coarse space
