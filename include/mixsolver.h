@@ -96,6 +96,18 @@ int ms_version(void);
  * either way.  No reference counterpart (library option). */
 int ms_set_graphs(int on);
 
+/* vtk_write (mod_uns_output.f90:81-240, tetrahedral fan) through vtu_write
+ * (mod_vtu_writer.f90): binary = 0 inline ASCII (Float64, ES24.16E3),
+ * 1 appended raw binary (Float32).  Mesh as for ms_system_create; xc (3,
+ * ncells) cell centroids; p, u (3,ncells), T cell fields; T_s written as
+ * temperature_solid when not NULL, q_src as heat_source_solid when not NULL
+ * (the reference writes them when LTNE exchange is active or a solid zone
+ * exists, and when a solid zone exists).  Host only; byte-identical to the
+ * reference's file for the same inputs. */
+int ms_vtu_write(const char *path, int binary, int nnodes, const double *x, int nfaces, const long long *fptr,
+                 const int *fnodes, const int *c0, const int *c1, int ncells, const double *xc, const double *p,
+                 const double *u, const double *T, const double *T_s, const double *q_src);
+
 /* ---- system = one mesh + fields + matrices resident on the device ------
  * Replaces the setup half of uns_run_mpi / uns_solver_init
  * (mod_uns_driver_mpi.f90:114-228, mod_uns_driver.f90:60-239):

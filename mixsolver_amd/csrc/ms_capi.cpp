@@ -118,6 +118,14 @@ const char *ms_last_error(void) { return last_error(); }
 int ms_version(void) { return 100; }
 int ms_set_graphs(int on) { return guarded([&] { set_graphs(on != 0); }); }
 
+int ms_vtu_write(const char *path, int binary, int nnodes, const double *x, int nfaces, const long long *fptr,
+                 const int *fnodes, const int *c0, const int *c1, int ncells, const double *xc, const double *p,
+                 const double *u, const double *T, const double *T_s, const double *q_src) {
+  return guarded([&] {
+    vtu_write(path, binary != 0, nnodes, x, nfaces, fptr, fnodes, c0, c1, ncells, xc, p, u, T, T_s, q_src);
+  });
+}
+
 int ms_system_create(ms_system **out, int nnodes, const double *x, int nfaces, const long long *fptr,
                      const int *fnodes, const int *c0, const int *c1, const int *zone, int ncells,
                      const ms_ctrl_t *ctrl, const ms_bcgroup_t *groups) {

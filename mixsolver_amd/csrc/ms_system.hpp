@@ -145,6 +145,11 @@ void wall_samples(System &s, std::vector<double> &pu);
 double wake_end(System &s, double xcyl);
 void free_stream(const System &s, double *u_inf, double *p_inf);
 
+// VTU file of the cell fields (host only; vtk_write + vtu_write layouts)
+void vtu_write(const char *path, bool binary, int nnodes, const double *x, int nfaces, const long long *fptr,
+               const int *fnodes, const int *c0, const int *c1, int ncells, const double *xc, const double *p,
+               const double *u, const double *T, const double *T_s, const double *q_src);
+
 // flang runtime NORM2 evaluation order (max-scaled accumulator), used for
 // every geometric length so the device coefficients equal the reference's.
 double norm2_ref(const double *v, int n);

@@ -65,6 +65,16 @@ class Case:
         if t is not None and t.size != self.nc:
             raise ValueError(f"cztype: expected {self.nc} values, got {t.size}")
         check(_lib.lib().ms_set_cell_types(self.h, None if t is None else ip(t)))
+        self.cztype = None if t is None else t.copy()
+
+    def write_vtu(self, path, binary=False):
+        """The current fields as the reference's vtk_write file (see
+        mixsolver_amd.vtu): ASCII Float64 or appended binary Float32."""
+        from .vtu import optional_arrays, write_vtu
+        with_ts, with_q = optional_arrays(getattr(self, "cztype", None), self.get("h_sf"), self.get("a_sf"))
+        write_vtu(path, self.mesh, self.get("xc"), self.get("p"), self.get("u"), self.get("T"),
+                  T_s=self.get("T_s") if with_ts else None, q_src=self.get("q_src") if with_q else None,
+                  binary=binary)
 
     @classmethod
     def from_control(cls, mesh, control_text, device=0, **overrides):

@@ -18,6 +18,9 @@
 !   forces : seed fields from in.bin (u,p,T), then compute_forces (written),
 !            write_cp_distribution (to cp.dat in the working directory) and
 !            compute_separation (printed)
+!   vtu    : seed fields from in.bin (u,p,T), T_s = 0.5 T + 1, then vtk_write
+!            to a.vtu (inline ASCII) and b.vtu (appended binary) in the
+!            working directory
 !   solve  : CSR system from in.bin; residual histories of cg_jacobi, cg_ic0,
 !            bicgstab_ilu0 for maxit = 1..n, IC0/ILU0 factors, final solutions
 ! All arrays are written as raw little-endian stream records in the order the
@@ -35,6 +38,7 @@ program ref_harness
    use mod_uns_linsolver
    use mod_uns_simple
    use mod_uns_forces, only: compute_forces, write_cp_distribution, compute_separation
+   use mod_uns_output, only: vtk_write
    implicit none
 
    character(len=512) :: casf, ctlf, mode, outf, inf, narg
@@ -97,6 +101,14 @@ program ref_harness
    case ( 'forces' )
       call seed_fields()
       call run_forces()
+   case ( 'vtu' )
+      call seed_fields()
+      fld%T_s = 0.5_dp * fld%T + 1.0_dp
+      call vtk_write( m, c, g, fld, 'a.vtu', ier )
+      call chk( ier, 'vtk_write ascii' )
+      call vtk_write( m, c, g, fld, 'b.vtu', ier, binary = .true. )
+      call chk( ier, 'vtk_write binary' )
+      write(uo) ier
    case ( 'funcs' )
       call run_funcs()
    case ( 'funcs2' )

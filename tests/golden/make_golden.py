@@ -586,6 +586,43 @@ def cylinder_known_answer():
                 archived_cp=base[:, :2], **CYL_README)
 
 
+def vtu_fixtures():
+    """The reference's vtk_write files (inline ASCII and appended binary) for
+    three small cases -- a cavity (quad faces, fluid only), a jittered prism
+    O-grid (triangle and quad faces) and the conduction bar with a source zone
+    (solid cells: temperature_solid and heat_source_solid written) -- with
+    seeded fields that include a negative zero and extreme exponents.  The
+    fixture keeps the inputs and the SHA-256 digest and size of each file."""
+    import hashlib
+    out = {}
+    mp = meshgen.cylinder_prisms(ntheta=12, nr=3, r_out=4.0, layers=1, lz=0.2, jitter=0.15, seed=7)
+    hc = os.path.join(REF_HC, "c5_1d_src.cas")
+    cases = [("cav", meshgen.cavity(8), {4: "symmetry", 5: "wall"},
+              CAVITY_CTL.format(blend=0.0, nonorth=1.0, precond="jacobi"), None),
+             ("prism", mp, {4: "symmetry", 5: "wall", 6: "pressure-far-field"}, PRISM_CTL, None)]
+    if os.path.exists(hc):
+        cases.append(("solid", cas.read_cas(hc), None, open(os.path.join(REF_HC, "c5_1d_source.control")).read(), hc))
+    for tag, m, zk, ctl, src in cases:
+        u, p, T = seeded_fields(m, 11)
+        p[0], p[1], p[2] = -0.0, 1.0e-300, -2.5e+200
+        u[0, 1] = 0.0
+        with tempfile.TemporaryDirectory() as td:
+            if src:
+                shutil.copy(src, os.path.join(td, "m.cas"))
+            else:
+                cas.write_cas(os.path.join(td, "m.cas"), m, zk)
+            open(os.path.join(td, "c.control"), "w").write(ctl)
+            write_fields(os.path.join(td, "in.bin"), u, p, T)
+            run(["m.cas", "c.control", "vtu", "o.bin", "in.bin", "1"], td)
+            for kind in ("a", "b"):
+                data = open(os.path.join(td, f"{kind}.vtu"), "rb").read()
+                out[f"{tag}_{kind}_sha256"] = hashlib.sha256(data).hexdigest()
+                out[f"{tag}_{kind}_size"] = np.int64(len(data))
+        out.update({f"{tag}_{k}": v for k, v in mesh_arrays(m).items()})
+        out.update({f"{tag}_control": ctl, f"{tag}_u": u, f"{tag}_p": p, f"{tag}_T": T})
+    return out
+
+
 def main():
     if not os.path.exists(HARNESS):
         sys.exit("build the reference harness first: oracle/build_ref.sh")
@@ -628,6 +665,9 @@ def main():
         save("cavity_re1000_known_answer", control=ctlk, nsteps=np.int64(1000), **mesh_arrays(mk),
              out_ier=np.int64(ierk), ref_umax=umax, ref_ghia_l2=l2,
              readme_umax=0.911512, readme_ghia_l2=0.0289706)
+
+    # 3g. the reference's VTU files (ASCII and binary) for three small cases
+    save("vtu_files", **vtu_fixtures())
 
     # 3f. the reference's shipped Re=40 cylinder: forces, Cp, separation, published indicators
     if os.path.isdir(REF_CYL):
@@ -716,6 +756,8 @@ def main():
 if __name__ == "__main__":
     if sys.argv[1:] == ["--porous"]:
         porous_fixtures()
+    elif sys.argv[1:] == ["--vtu"]:
+        save("vtu_files", **vtu_fixtures())
     elif sys.argv[1:] == ["--cyl"]:
         save("cylinder_re40_known_answer", **cylinder_known_answer())
     elif sys.argv[1:] == ["--hc"]:

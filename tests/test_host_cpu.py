@@ -360,3 +360,37 @@ end program t
         Lc[i, ci[rp[i] - 1:rp[i + 1] - 1] - 1] = d["ic0_L"][rp[i] - 1:rp[i + 1] - 1]
     Lc = np.tril(Lc)
     np.testing.assert_allclose(Lc @ Lc.T @ z_ic, A @ np.ones(n), rtol=1e-12, atol=1e-12)
+
+
+# ------------------------------------------------------------ VTU output
+@pytest.mark.parametrize("tag", ["cav", "prism", "solid"])
+@pytest.mark.parametrize("binary", [False, True])
+def test_vtu_byte_identical_to_reference(tag, binary, tmp_path):
+    """ms_vtu_write (host only) writes the reference's vtk_write file byte for
+    byte: tetrahedral fan of every face from its first node, reversed for the
+    neighbour cell, centroid points, ES24.16E3 ASCII numbers (negative zero,
+    1e-300, -2.5e200 among them) or appended Float32 binary with UInt32 block
+    headers; the solid bar adds temperature_solid and heat_source_solid."""
+    import hashlib
+    from conftest import load_golden, mesh_from
+    from mixsolver_amd.control import parse_control
+    from mixsolver_amd.vtu import optional_arrays, write_vtu
+    from mixsolver_amd.zones import resolve_cell_zones, setup_porous_fields
+    from oracle import pyoracle
+    d = load_golden("vtu_files")
+    sub = {k[len(tag) + 1:]: v for k, v in d.items() if k.startswith(tag + "_")}
+    m = mesh_from(sub)
+    ctrl, groups = parse_control(str(sub["control"]), warn=False)
+    xc = pyoracle.OracleCase(m, ctrl, groups).geometry()["xc"]
+    cz = resolve_cell_zones(m, ctrl.cell_zones) if m.czone is not None else None
+    f = setup_porous_fields(m, ctrl.cell_zones, cz) if cz is not None else None
+    with_ts, with_q = optional_arrays(cz, f["h_sf"] if f else [], f["a_sf"] if f else [])
+    assert (with_ts, with_q) == ((True, True) if tag == "solid" else (False, False))
+    T = sub["T"]
+    out = tmp_path / "x.vtu"
+    write_vtu(out, m, xc, sub["p"], sub["u"], T, T_s=0.5 * T + 1.0 if with_ts else None,
+              q_src=f["q_src"] if with_q else None, binary=binary)
+    data = out.read_bytes()
+    kind = "b" if binary else "a"
+    assert len(data) == int(sub[f"{kind}_size"])
+    assert hashlib.sha256(data).hexdigest() == str(sub[f"{kind}_sha256"])
