@@ -104,6 +104,21 @@ int ms_set_graphs(int on);
  * (the reference writes them when LTNE exchange is active or a solid zone
  * exists, and when a solid zone exists).  Host only; byte-identical to the
  * reference's file for the same inputs. */
+/* write_field_dump (mod_uns_restart.f90:77-152): field dump v1, ASCII,
+ * ES24.16, the header the reference writes ('# generated:' from `generated`
+ * when not NULL/empty, else the local time).  u, u_old, u_old_old (3,ncells);
+ * p, T, T_old, T_old_old (ncells).  Host only. */
+int ms_dump_write(const char *path, int ncells, int ts_order, const char *source_mesh, const char *generated,
+                  const double *u, const double *p, const double *T, const double *u_old, const double *u_old_old,
+                  const double *T_old, const double *T_old_old);
+/* read_field_dump (mod_uns_restart.f90:166-334, serial): the arrays as for
+ * ms_dump_write, *ts_order, the source_mesh header (truncated to
+ * source_len - 1 characters) and *ier with the reference's codes: 0 ok,
+ * 1 cannot open, 2 header ended early or malformed, 3 version mismatch,
+ * 4 ncells mismatch, 5 data.  Host only. */
+int ms_dump_read(const char *path, int ncells, double *u, double *p, double *T, double *u_old, double *u_old_old,
+                 double *T_old, double *T_old_old, int *ts_order, char *source_mesh, int source_len, int *ier);
+
 int ms_vtu_write(const char *path, int binary, int nnodes, const double *x, int nfaces, const long long *fptr,
                  const int *fnodes, const int *c0, const int *c1, int ncells, const double *xc, const double *p,
                  const double *u, const double *T, const double *T_s, const double *q_src);
@@ -151,6 +166,7 @@ int ms_periodic_match(int nnodes, const double *x, int nfaces, const long long *
 int ms_set_field(ms_system *s, const char *name, const double *host, long long n);
 int ms_get_field(ms_system *s, const char *name, double *host, long long n);
 int ms_set_ts_order(ms_system *s, int ts_order);
+int ms_get_ts_order(ms_system *s, int *ts_order);   /* fld%ts_order of the system */
 /* Block type of every cell (m%cztype after resolve_cell_zones,
  * mod_uns_control.f90:575-733): 1 fluid, 2 porous, 3 solid; NULL = all
  * fluid.  Porous coefficients come through ms_set_field. */

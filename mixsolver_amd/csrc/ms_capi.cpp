@@ -118,6 +118,31 @@ const char *ms_last_error(void) { return last_error(); }
 int ms_version(void) { return 100; }
 int ms_set_graphs(int on) { return guarded([&] { set_graphs(on != 0); }); }
 
+int ms_dump_write(const char *path, int ncells, int ts_order, const char *source_mesh, const char *generated,
+                  const double *u, const double *p, const double *T, const double *u_old, const double *u_old_old,
+                  const double *T_old, const double *T_old_old) {
+  return guarded([&] {
+    const double *f[7] = {u, p, T, u_old, u_old_old, T_old, T_old_old};
+    dump_write(path, ncells, ts_order, source_mesh, generated, f);
+  });
+}
+
+int ms_dump_read(const char *path, int ncells, double *u, double *p, double *T, double *u_old, double *u_old_old,
+                 double *T_old, double *T_old_old, int *ts_order, char *source_mesh, int source_len, int *ier) {
+  return guarded([&] {
+    if (!ier || !ts_order || !u || !p || !T || !u_old || !u_old_old || !T_old || !T_old_old || ncells <= 0)
+      fail(MS_EINVAL, "ms_dump_read: null argument or empty mesh");
+    double *const f[7] = {u, p, T, u_old, u_old_old, T_old, T_old_old};
+    std::string src;
+    *ier = dump_read(path, ncells, f, ts_order, src);
+    if (source_mesh && source_len > 0) {
+      const size_t n = std::min((size_t)source_len - 1, src.size());
+      std::memcpy(source_mesh, src.data(), n);
+      source_mesh[n] = '\0';
+    }
+  });
+}
+
 int ms_vtu_write(const char *path, int binary, int nnodes, const double *x, int nfaces, const long long *fptr,
                  const int *fnodes, const int *c0, const int *c1, int ncells, const double *xc, const double *p,
                  const double *u, const double *T, const double *T_s, const double *q_src) {
@@ -273,6 +298,13 @@ int ms_set_ts_order(ms_system *h, int t) {
   return guarded([&] {
     if (t < 0 || t > 2) fail(MS_EINVAL, "ts_order must be 0, 1 or 2");
     sys(h).ts_order = t;
+  });
+}
+
+int ms_get_ts_order(ms_system *h, int *t) {
+  return guarded([&] {
+    if (!t) fail(MS_EINVAL, "null output");
+    *t = sys(h).ts_order;
   });
 }
 

@@ -14,10 +14,19 @@
 // ASCII numbers use Fortran's ES24.16E3 edit descriptor (17 significant
 // digits, three-digit exponent, right-justified in 24 columns, one blank
 // before each); integers one per line.
+//
+// Also the field dump v1 of write_field_dump / read_field_dump
+// (mod_uns_restart.f90:77-152, :166-334): a '#' header (version, source
+// mesh, ncells, ts_order, generation time, field list), then u, p, T,
+// u_old, u_old_old, T_old, T_old_old one cell per line in ES24.16 (vectors
+// '(3(es24.16,1x))'), each block closed by a '# end <name>' line; read back
+// list-directed with the reference's error codes.
+#include <cctype>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <ctime>
 #include <memory>
 #include <string>
 #include <vector>
@@ -52,6 +61,49 @@ int es24(double v, char *out) {
   return 24;
 }
 
+// Fortran ES24.16 (no exponent width): E+dd for |exponent| <= 99, +ddd
+// (no letter) beyond, right-justified in 24 columns
+int es24_2(double v, char *out) {
+  char b[64];
+  if (std::isnan(v)) {
+    std::snprintf(b, sizeof b, "NaN");
+  } else if (std::isinf(v)) {
+    std::snprintf(b, sizeof b, v < 0 ? "-Infinity" : "Infinity");
+  } else {
+    char m[64];
+    std::snprintf(m, sizeof m, "%.16E", v);
+    char *e = std::strchr(m, 'E');
+    const int ex = std::atoi(e + 1);
+    *e = '\0';
+    const int a = ex < 0 ? -ex : ex;
+    if (a <= 99) std::snprintf(b, sizeof b, "%sE%c%02d", m, ex < 0 ? '-' : '+', a);
+    else std::snprintf(b, sizeof b, "%s%c%03d", m, ex < 0 ? '-' : '+', a);
+  }
+  const int len = (int)std::strlen(b);
+  if (len > 24) {
+    std::memset(out, '*', 24);
+    return 24;
+  }
+  std::memset(out, ' ', 24 - len);
+  std::memcpy(out + 24 - len, b, len);
+  return 24;
+}
+
+// one Fortran list-directed real: D/d exponents, and the letterless form
+// d.ddd+ddd that ES editing writes for three-digit exponents
+bool parse_real(const char *t, size_t n, double &v) {
+  std::string s(t, n);
+  for (char &c : s)
+    if (c == 'D' || c == 'd') c = 'E';
+  if (s.find_first_of("Ee") == std::string::npos) {
+    const size_t k = s.find_last_of("+-");
+    if (k != std::string::npos && k > 0 && std::isdigit((unsigned char)s[k - 1])) s.insert(k, "E");
+  }
+  char *end = nullptr;
+  v = std::strtod(s.c_str(), &end);
+  return end && *end == '\0' && !s.empty();
+}
+
 struct Out {
   FILE *f = nullptr;
   explicit Out(const char *path, bool bin) {
@@ -75,10 +127,20 @@ struct Out {
     std::fwrite(line, 1, k, f);
   }
   void ints(long long v) { std::fprintf(f, "%lld\n", v); }
+  void dump_line(const double *v, int n) {   // '(3(es24.16,1x))' or '(es24.16)'
+    char line[25 * 4 + 2];
+    int k = 0;
+    for (int i = 0; i < n; i++) {
+      k += es24_2(v[i], line + k);
+      if (n > 1) line[k++] = ' ';
+    }
+    line[k++] = '\n';
+    std::fwrite(line, 1, k, f);
+  }
   void close() {
     if (std::fclose(f) != 0) {
       f = nullptr;
-      fail(MS_EINVAL, "VTU write failed");
+      fail(MS_EINVAL, "output file write failed");
     }
     f = nullptr;
   }
@@ -258,6 +320,115 @@ void vtu_write(const char *path, bool binary, int nnodes, const double *x, int n
   o.s("\n  </AppendedData>\n");
   o.s("</VTKFile>\n");
   o.close();
+}
+
+// ------------------------------------------------------------ field dump v1
+void dump_write(const char *path, int ncells, int ts_order, const char *source_mesh, const char *generated,
+                const double *const f[7]) {
+  if (!path || ncells <= 0) fail(MS_EINVAL, "ms_dump_write: null path or empty mesh");
+  for (int k = 0; k < 7; k++)
+    if (!f[k]) fail(MS_EINVAL, "ms_dump_write: null field");
+  std::string gen;
+  if (generated && generated[0]) {
+    gen = generated;
+  } else {
+    std::time_t t = std::time(nullptr);
+    std::tm tm{};
+    localtime_r(&t, &tm);
+    char b[32];
+    std::strftime(b, sizeof b, "%Y-%m-%d %H:%M:%S", &tm);
+    gen = b;
+  }
+  std::string src = source_mesh ? source_mesh : "";
+  while (!src.empty() && src.back() == ' ') src.pop_back();
+  Out o(path, false);
+  o.s("# UNSolver field dump v1\n");
+  o.s("# source_mesh: " + src + "\n");
+  o.s("# ncells: " + std::to_string(ncells) + "\n");
+  o.s("# ts_order: " + std::to_string(ts_order) + "\n");
+  o.s("# generated: " + gen + "\n");
+  o.s("# fields: u p T u_old u_old_old T_old T_old_old\n");
+  static const char *names[7] = {"u", "p", "T", "u_old", "u_old_old", "T_old", "T_old_old"};
+  static const int width[7] = {3, 1, 1, 3, 3, 1, 1};
+  for (int k = 0; k < 7; k++) {
+    for (int c = 0; c < ncells; c++) o.dump_line(f[k] + (size_t)width[k] * c, width[k]);
+    o.s(std::string("# end ") + names[k] + "\n");
+  }
+  o.close();
+}
+
+int dump_read(const char *path, int ncells, double *const f[7], int *ts_order, std::string &src) {
+  src.clear();
+  FILE *fp = path ? std::fopen(path, "r") : nullptr;
+  if (!fp) return 1;
+  std::unique_ptr<FILE, int (*)(FILE *)> guard(fp, std::fclose);
+  std::string line;
+  auto getline = [&](std::string &l) -> bool {
+    l.clear();
+    int ch;
+    bool any = false;
+    while ((ch = std::fgetc(fp)) != EOF) {
+      any = true;
+      if (ch == '\n') return true;
+      l.push_back((char)ch);
+    }
+    return any;
+  };
+  auto ltrim = [](const std::string &x) {
+    size_t i = 0;
+    while (i < x.size() && (x[i] == ' ' || x[i] == '\t')) i++;
+    return x.substr(i);
+  };
+  auto after_colon_int = [&](const std::string &l, int &v) {
+    const size_t k = l.find(':');
+    v = (k == std::string::npos) ? 0 : std::atoi(l.c_str() + k + 1);
+  };
+  int ver = 0, n_in = 0, ts_in = 0;
+  const std::string vmark = "# UNSolver field dump v";
+  while (true) {
+    if (!getline(line)) return 2;          // ended in the header
+    line = ltrim(line);
+    while (!line.empty() && (line.back() == ' ' || line.back() == '\r')) line.pop_back();
+    if (line.empty()) continue;
+    if (line[0] != '#') return 2;          // malformed header
+    if (line.find(vmark) != std::string::npos) {
+      ver = std::atoi(line.c_str() + line.find(vmark) + vmark.size());
+    } else if (line.find("# source_mesh:") != std::string::npos) {
+      src = ltrim(line.substr(line.find(':') + 1));
+    } else if (line.find("# ncells:") != std::string::npos) {
+      after_colon_int(line, n_in);
+    } else if (line.find("# ts_order:") != std::string::npos) {
+      after_colon_int(line, ts_in);
+    } else if (line.find("# fields:") != std::string::npos) {
+      break;
+    }
+  }
+  if (ver != 1) return 3;
+  if (n_in != ncells) return 4;
+  static const int width[7] = {3, 1, 1, 3, 3, 1, 1};
+  for (int k = 0; k < 7; k++) {
+    // list-directed: values from successive lines until the block is full,
+    // the rest of the last line discarded; then the '# end' line is skipped
+    const size_t need = (size_t)width[k] * ncells;
+    size_t got = 0;
+    while (got < need) {
+      if (!getline(line)) return 5;
+      const char *p = line.c_str();
+      while (*p && got < need) {
+        while (*p == ' ' || *p == '\t' || *p == ',' || *p == '\r') p++;
+        if (!*p) break;
+        const char *q = p;
+        while (*q && *q != ' ' && *q != '\t' && *q != ',' && *q != '\r') q++;
+        double v;
+        if (!parse_real(p, (size_t)(q - p), v)) return 5;
+        f[k][got++] = v;
+        p = q;
+      }
+    }
+    getline(line);                           // '# end <name>'
+  }
+  *ts_order = ts_in;
+  return 0;
 }
 
 }  // namespace ms

@@ -150,6 +150,13 @@ void vtu_write(const char *path, bool binary, int nnodes, const double *x, int n
                const int *fnodes, const int *c0, const int *c1, int ncells, const double *xc, const double *p,
                const double *u, const double *T, const double *T_s, const double *q_src);
 
+// field dump v1 (write_field_dump / read_field_dump); f = u p T u_old
+// u_old_old T_old T_old_old; dump_read returns the reference's ier (0 ok, 1
+// open, 2 header, 3 version, 4 ncells, 5 data)
+void dump_write(const char *path, int ncells, int ts_order, const char *source_mesh, const char *generated,
+                const double *const f[7]);
+int dump_read(const char *path, int ncells, double *const f[7], int *ts_order, std::string &src);
+
 // flang runtime NORM2 evaluation order (max-scaled accumulator), used for
 // every geometric length so the device coefficients equal the reference's.
 double norm2_ref(const double *v, int n);

@@ -67,6 +67,28 @@ class Case:
         check(_lib.lib().ms_set_cell_types(self.h, None if t is None else ip(t)))
         self.cztype = None if t is None else t.copy()
 
+    def get_ts_order(self):
+        t = C.c_int(0)
+        check(_lib.lib().ms_get_ts_order(self.h, C.byref(t)))
+        return t.value
+
+    def write_dump(self, path, source_mesh="", generated=None):
+        """The system's u, p, T and time history as the reference's field
+        dump v1 (mixsolver_amd.restart)."""
+        from .restart import FIELDS, write_dump
+        write_dump(path, self.nc, {k: self.get(k) for k in FIELDS}, self.get_ts_order(), source_mesh, generated)
+
+    def read_dump(self, path):
+        """Restore u, p, T, the time history and ts_order from a field dump;
+        returns the reference's ier (0 ok) and the dump's source_mesh."""
+        from .restart import FIELDS, read_dump
+        ier, f, ts, src = read_dump(path, self.nc)
+        if ier == 0:
+            for k in FIELDS:
+                self.set_field(k, f[k])
+            self.set_ts_order(ts)
+        return ier, src
+
     def write_vtu(self, path, binary=False):
         """The current fields as the reference's vtk_write file (see
         mixsolver_amd.vtu): ASCII Float64 or appended binary Float32."""

@@ -623,6 +623,58 @@ def vtu_fixtures():
     return out
 
 
+FMT_PROGRAM = """program fmt_probe
+   implicit none
+   integer, parameter :: dp = kind(1.0d0)
+   integer :: n, i, u
+   real(dp), allocatable :: v(:)
+   open(newunit=u, file='v.bin', access='stream', form='unformatted', status='old')
+   read(u) n
+   allocate(v(n))
+   read(u) v
+   close(u)
+   open(newunit=u, file='es.txt', status='replace')
+   do i = 1, n
+      write(u,'(es24.16)') v(i)
+   end do
+   close(u)
+   open(newunit=u, file='es3.txt', status='replace')
+   do i = 1, n
+      write(u,'(1x,es24.16e3)') v(i)
+   end do
+   close(u)
+end program fmt_probe
+"""
+
+
+def fortran_formats():
+    """flang's own ES24.16 and ES24.16E3 output (the edit descriptors of the
+    field dump and of the VTU files) for a set of values that exercise the
+    rounding and exponent rules: signed zeros, denormals, the extremes,
+    values that round up to the next power of ten at the 17th digit, two-
+    and three-digit exponents on both sides.  Built from the program above
+    with the image's amdflang (no reference code involved)."""
+    fc = os.environ.get("FC", "/opt/rocm/llvm/bin/amdflang")
+    rng = np.random.default_rng(99)
+    v = np.concatenate([
+        [0.0, -0.0, 1.0, -1.0, 0.1, 1.0 / 3.0, 2.0 / 3.0, 5e-324, 2.2250738585072014e-308, 1.7976931348623157e308,
+         1e100, 1e-100, 1e99, 1e-99, 9.99999999999999999e99, 9.9999999999999999e-100, 9.999999999999999e99,
+         0.99999999999999999, 123456789.123456789, -9.5, 0.5, 1e16, 1e17, 9.87654321e-5],
+        rng.normal(size=40) * 10.0 ** rng.integers(-320, 300, size=40),
+        rng.uniform(-1, 1, size=20)])
+    with tempfile.TemporaryDirectory() as td:
+        open(os.path.join(td, "p.f90"), "w").write(FMT_PROGRAM)
+        subprocess.run([fc, "-O1", "-o", os.path.join(td, "p"), os.path.join(td, "p.f90")], check=True,
+                       capture_output=True)
+        with open(os.path.join(td, "v.bin"), "wb") as f:
+            np.array([len(v)], np.int32).tofile(f)
+            v.tofile(f)
+        subprocess.run([os.path.join(td, "p")], cwd=td, check=True, capture_output=True)
+        es = open(os.path.join(td, "es.txt")).read().split("\n")[: len(v)]
+        es3 = open(os.path.join(td, "es3.txt")).read().split("\n")[: len(v)]
+    return dict(values=v, es24_16=np.array(es), es24_16e3=np.array(es3))
+
+
 def main():
     if not os.path.exists(HARNESS):
         sys.exit("build the reference harness first: oracle/build_ref.sh")
@@ -665,6 +717,9 @@ def main():
         save("cavity_re1000_known_answer", control=ctlk, nsteps=np.int64(1000), **mesh_arrays(mk),
              out_ier=np.int64(ierk), ref_umax=umax, ref_ghia_l2=l2,
              readme_umax=0.911512, readme_ghia_l2=0.0289706)
+
+    # 3h. flang's ES24.16 / ES24.16E3 output (field dump and VTU numbers)
+    save("fortran_formats", **fortran_formats())
 
     # 3g. the reference's VTU files (ASCII and binary) for three small cases
     save("vtu_files", **vtu_fixtures())
@@ -756,6 +811,8 @@ def main():
 if __name__ == "__main__":
     if sys.argv[1:] == ["--porous"]:
         porous_fixtures()
+    elif sys.argv[1:] == ["--fmt"]:
+        save("fortran_formats", **fortran_formats())
     elif sys.argv[1:] == ["--vtu"]:
         save("vtu_files", **vtu_fixtures())
     elif sys.argv[1:] == ["--cyl"]:

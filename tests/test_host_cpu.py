@@ -394,3 +394,69 @@ def test_vtu_byte_identical_to_reference(tag, binary, tmp_path):
     kind = "b" if binary else "a"
     assert len(data) == int(sub[f"{kind}_size"])
     assert hashlib.sha256(data).hexdigest() == str(sub[f"{kind}_sha256"])
+
+
+# ------------------------------------------------------------ field dump v1
+def test_fortran_number_formats(tmp_path):
+    """The dump's ES24.16 numbers and the VTU's ES24.16E3 numbers equal
+    flang's own output for the same values (signed zero, denormals, the
+    extremes, rounding into the next power of ten, two- and three-digit
+    exponents: ES24.16 drops the letter for three-digit exponents)."""
+    from conftest import load_golden
+    from mixsolver_amd.restart import FIELDS, write_dump
+    from mixsolver_amd.vtu import write_vtu
+    d = load_golden("fortran_formats")
+    v = d["values"]
+    n = len(v)
+    f = {k: np.zeros((n, 3)) if k.startswith("u") else np.zeros(n) for k in FIELDS}
+    f["p"] = v.copy()
+    write_dump(tmp_path / "d.txt", n, f, 0, "m.cas", "2026-01-01 00:00:00")
+    lines = (tmp_path / "d.txt").read_text().split("\n")
+    i = lines.index("# end u") + 1
+    assert lines[i:i + n] == list(d["es24_16"])
+    assert lines[i + n] == "# end p"
+    m = meshgen.hex_box(np.linspace(0.0, 1.0, n + 1), np.array([0.0, 1.0]), np.array([0.0, 1.0]))
+    write_vtu(tmp_path / "a.vtu", m, np.zeros((n, 3)), v, np.zeros((n, 3)), np.zeros(n))
+    txt = (tmp_path / "a.vtu").read_text()
+    blk = txt.split('Name="pressure" NumberOfComponents="1" format="ascii">\n')[1].split("\n        </DataArray>")[0]
+    got = blk.split("\n")[::12]        # 6 quad faces -> 12 tetrahedra per cell
+    assert got == list(d["es24_16e3"])
+
+
+def test_dump_round_trip_and_errors(tmp_path):
+    """write_field_dump / read_field_dump restated: the seven arrays and
+    ts_order survive a round trip bit for bit (17 significant digits), the
+    reader takes list-directed input (values split or joined across lines,
+    D exponents, commas), and returns the reference's codes: 1 cannot open,
+    2 header ended or malformed, 3 version, 4 ncells, 5 data."""
+    from mixsolver_amd.restart import FIELDS, read_dump, write_dump
+    n = 7
+    rng = np.random.default_rng(3)
+    f = {k: rng.normal(size=(n, 3)) if k.startswith("u") else rng.normal(size=n) * 1e5 for k in FIELDS}
+    f["T"][2] = -0.0
+    path = tmp_path / "d.txt"
+    write_dump(path, n, f, 2, "case.cas", "2026-01-01 00:00:00")
+    ier, g, ts, src = read_dump(path, n)
+    assert (ier, ts, src) == (0, 2, "case.cas")
+    for k in FIELDS:
+        assert np.array_equal(g[k], f[k]), k
+    assert np.signbit(g["T"][2])
+    text = path.read_text()
+    lines = text.split("\n")
+    # list-directed: u's 21 values regrouped 7 per line with commas and D exponents
+    iu = lines.index("# fields: u p T u_old u_old_old T_old T_old_old") + 1
+    vals = [f"{x:.17e}".replace("e", "D") for x in f["u"].ravel()]
+    alt = lines[:iu] + [", ".join(vals[i:i + 7]) for i in range(0, 21, 7)] + lines[iu + n:]
+    (tmp_path / "alt.txt").write_text("\n".join(alt))
+    ier, g2, _, _ = read_dump(tmp_path / "alt.txt", n)
+    assert ier == 0 and np.array_equal(g2["u"], f["u"]) and np.array_equal(g2["p"], f["p"])
+    assert read_dump(tmp_path / "missing.txt", n)[0] == 1
+    (tmp_path / "h.txt").write_text("\n".join(lines[:3]) + "\n")
+    assert read_dump(tmp_path / "h.txt", n)[0] == 2
+    (tmp_path / "m.txt").write_text("\n".join(lines[:2] + ["1.0"] + lines[2:]))
+    assert read_dump(tmp_path / "m.txt", n)[0] == 2
+    (tmp_path / "v.txt").write_text(text.replace("field dump v1", "field dump v2"))
+    assert read_dump(tmp_path / "v.txt", n)[0] == 3
+    assert read_dump(path, n + 1)[0] == 4
+    (tmp_path / "t.txt").write_text("\n".join(lines[: iu + n + 3]))
+    assert read_dump(tmp_path / "t.txt", n)[0] == 5
