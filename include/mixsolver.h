@@ -167,6 +167,22 @@ int ms_set_field(ms_system *s, const char *name, const double *host, long long n
 int ms_get_field(ms_system *s, const char *name, double *host, long long n);
 int ms_set_ts_order(ms_system *s, int ts_order);
 int ms_get_ts_order(ms_system *s, int *ts_order);   /* fld%ts_order of the system */
+
+/* ---- coupling interface (mode = coupled callers) ------------------------
+ * Boundary faces of face zones whose condition or user name contains
+ * "interface" and that have no bc group form a BC_INTERFACE group (build_bc,
+ * mod_uns_bc.f90:282-323): pass it as a group with btype MS_BC_INTERFACE and
+ * the zone id.  Per-face data, natural face order, all faces (values of
+ * other faces are ignored), replacing set_interface_vel / _p / _T_ltne /
+ * _q / _T_mode / _slip (mod_uns_bc.f90:730-821): what = "vel" (3,nfaces),
+ * "p", "T", "Ts", "q", "slip_C", "peer_vel" (3,nfaces), "T_mode" (0
+ * Dirichlet iface T, 1 Neumann with q; set_interface_q sets 1 where it
+ * applies).  n = number of values. */
+int ms_set_interface(ms_system *s, const char *what, const double *values, long long n);
+/* set_iface_p_dirichlet / set_iface_slip_mode / set_iface_zhang_vel /
+ * set_iface_t_ltne_mode (mod_uns_bc.f90:823-857); per system here (module
+ * state in the reference). */
+int ms_set_interface_modes(ms_system *s, int p_dir, int slip, int zhang, int T_ltne);
 /* Block type of every cell (m%cztype after resolve_cell_zones,
  * mod_uns_control.f90:575-733): 1 fluid, 2 porous, 3 solid; NULL = all
  * fluid.  Porous coefficients come through ms_set_field. */

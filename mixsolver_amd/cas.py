@@ -17,7 +17,7 @@ from .meshgen import Mesh
 
 # Fluent boundary condition codes for section-13 face zones
 _COND = {"interior": 2, "wall": 3, "pressure-outlet": 5, "symmetry": 7,
-         "velocity-inlet": 10, "outflow": 36, "pressure-far-field": 9}
+         "velocity-inlet": 10, "outflow": 36, "pressure-far-field": 9, "interface": 24}
 
 
 def write_cas(path: str, m: Mesh, zone_kind: dict | None = None) -> None:
@@ -42,6 +42,10 @@ def write_cas(path: str, m: Mesh, zone_kind: dict | None = None) -> None:
         for z in ids:
             cond, name = m.czone_names.get(z, ("fluid", "fluid"))
             f.write(f"(45 ({z} {cond} {name})())\n")
+        # coupling-interface face zones (zone_kind 'interface') are named so
+        # that the reader recognises them (zone_is_interface)
+        for z in sorted(z for z, kind in zone_kind.items() if kind == "interface"):
+            f.write(f"(45 ({z} interface interface-{z})())\n")
         zones = m.zone
         start = 0
         while start < nf:
@@ -164,4 +168,5 @@ def read_cas(path: str) -> Mesh:
                 c1=np.array([r[2] for r in recs], np.int32),
                 zone=np.array([r[3] for r in recs], np.int32),
                 ncells=ncells, ctype=ctype, cell_zone=czone, czone=czarr,
-                czone_names={z: v for z, v in names.items() if any(z == c[0] for c in cranges)})
+                czone_names={z: v for z, v in names.items() if any(z == c[0] for c in cranges)},
+                zones={z: v for z, v in names.items() if not any(z == c[0] for c in cranges)})

@@ -32,6 +32,8 @@ from __future__ import annotations
 import ctypes as C
 import warnings
 
+import numpy as np
+
 BC_NONE, BC_WALL, BC_SYMMETRY, BC_VINLET, BC_POUTLET, BC_FARFIELD = 0, 1, 2, 3, 4, 5
 BC_SLIPWALL, BC_MASSINLET, BC_OUTFLOW, BC_INTERFACE, BC_VINLET_PARAB = 6, 7, 8, 9, 10
 
@@ -367,6 +369,34 @@ def parse_control(text: str, warn=True):
     ctrl.nbc = len(groups)
     ctrl.gpu_ppe_precond = ctrl.ppe_precond if ppe_gpu is None else ppe_gpu
     return ctrl, groups
+
+
+def interface_zones(mesh):
+    """Face zones whose condition or user name contains 'interface' (any
+    case): the coupling surfaces of zone_is_interface (mod_uns_bc.f90:
+    902-918)."""
+    out = []
+    for z, (cond, name) in sorted(getattr(mesh, "zones", {}).items()):
+        if "interface" in (str(cond).strip().lower() + " " + str(name).strip().lower()):
+            out.append(int(z))
+    return out
+
+
+def with_interface_groups(mesh, ctrl, groups):
+    """groups plus one BC_INTERFACE group per interface zone of the mesh that
+    has no bc line (build_bc, mod_uns_bc.f90:282-323, gives the boundary
+    faces of such zones a coupling-interface group after the control's
+    groups); sets ctrl.nbc."""
+    have = {int(g.zone) for g in groups}
+    bnd = set(int(z) for z in np.unique(mesh.zone[mesh.c1 == 0]))
+    out = list(groups)
+    for z in interface_zones(mesh):
+        if z in bnd and z not in have:
+            g = MsBcGroup()
+            g.zone, g.btype = z, BC_INTERFACE
+            out.append(g)
+    ctrl.nbc = len(out)
+    return out
 
 
 def groups_array(groups):

@@ -301,6 +301,46 @@ int ms_set_ts_order(ms_system *h, int t) {
   });
 }
 
+int ms_set_interface(ms_system *h, const char *what, const double *v, long long n) {
+  return guarded([&] {
+    System &s = sys(h);
+    if (!what || !v) fail(MS_EINVAL, "null argument");
+    if (!s.has_iface) fail(MS_EINVAL, "ms_set_interface: the system has no coupling-interface faces");
+    const std::string w(what);
+    std::vector<double> *dst = nullptr;
+    size_t width = 1;
+    if (w == "vel") dst = &s.iface_vel, width = 3;
+    else if (w == "p") dst = &s.iface_p;
+    else if (w == "T") dst = &s.iface_T;
+    else if (w == "Ts") dst = &s.iface_Ts;
+    else if (w == "q") dst = &s.iface_q;
+    else if (w == "slip_C") dst = &s.iface_slip_C;
+    else if (w == "peer_vel") dst = &s.iface_peer_vel, width = 3;
+    else if (w != "T_mode") fail(MS_EINVAL, "ms_set_interface: unknown item " + w);
+    if (n != (long long)(width * s.nf)) fail(MS_EINVAL, "ms_set_interface: " + w + " needs one value per face");
+    if (w == "T_mode") {
+      for (int f = 0; f < s.nf; f++) {
+        if (v[f] != 0.0 && v[f] != 1.0) fail(MS_EINVAL, "ms_set_interface: T_mode values are 0 or 1");
+        s.iface_T_mode[f] = (int)v[f];
+      }
+    } else {
+      dst->assign(v, v + n);
+    }
+    s.iface_dirty = true;
+  });
+}
+
+int ms_set_interface_modes(ms_system *h, int p_dir, int slip, int zhang, int T_ltne) {
+  return guarded([&] {
+    System &s = sys(h);
+    s.iface_p_dir = p_dir != 0;
+    s.iface_slip = slip != 0;
+    s.iface_zhang = zhang != 0;
+    s.iface_T_ltne = T_ltne != 0;
+    s.props_dirty = true;
+  });
+}
+
 int ms_get_ts_order(ms_system *h, int *t) {
   return guarded([&] {
     if (!t) fail(MS_EINVAL, "null output");

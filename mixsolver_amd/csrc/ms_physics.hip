@@ -117,6 +117,14 @@ __device__ void bc_face_vel(const DMesh &m, int f, const double *uP, double *uf,
       un = ddot3(uP, nv);
       for (int d = 0; d < 3; d++) uf[d] = uP[d] - un * nv[d];
       break;
+    case MS_BC_INTERFACE:   // :460-474: extrapolated under the interface pressure model
+      if (m.iface_pdir) {
+        uf[0] = uP[0]; uf[1] = uP[1]; uf[2] = uP[2];
+      } else {
+        const double *v = &m.iface_vel[3 * (size_t)f];
+        uf[0] = v[0]; uf[1] = v[1]; uf[2] = v[2];
+      }
+      break;
     default:
       uf[0] = uP[0]; uf[1] = uP[1]; uf[2] = uP[2];
   }
@@ -125,14 +133,27 @@ __device__ void bc_face_vel(const DMesh &m, int f, const double *uP, double *uf,
 __device__ __forceinline__ double bc_face_p(const DMesh &m, int f, double pP, const Ramp &rp) {
   const DevBC &g = m.bc[m.fgrp[f] - 1];
   if (g.btype == MS_BC_POUTLET || g.btype == MS_BC_FARFIELD) return g.pval * rp.pval;
+  if (g.btype == MS_BC_INTERFACE && m.iface_pdir) return m.iface_p[f];
   return pP;
 }
 
+// bc_face_p_dirichlet (mod_uns_bc.f90:879-893): pressure outlets, far field,
+// and interface faces under the interface pressure model
 __device__ __forceinline__ bool p_dirichlet(const DMesh &m, int f) {
   int gi = m.fgrp[f];
   if (gi <= 0) return false;
   int bt = m.bc[gi - 1].btype;
-  return bt == MS_BC_POUTLET || bt == MS_BC_FARFIELD;
+  return bt == MS_BC_POUTLET || bt == MS_BC_FARFIELD || (bt == MS_BC_INTERFACE && m.iface_pdir);
+}
+
+// the PPE's pressure-Dirichlet test: the serial ppe_assembly uses
+// bc_face_p_dirichlet (mod_uns_simple.f90:1585, :1624), ppe_assembly_mpi the
+// types alone (mod_uns_simple_mpi.f90:903-907, :942-946)
+__device__ __forceinline__ bool p_dirichlet_ppe(const DMesh &m, int f) {
+  int gi = m.fgrp[f];
+  if (gi <= 0) return false;
+  int bt = m.bc[gi - 1].btype;
+  return bt == MS_BC_POUTLET || bt == MS_BC_FARFIELD || (bt == MS_BC_INTERFACE && m.iface_ppe);
 }
 
 // bc_face_T (static part precomputed per face)
@@ -352,6 +373,27 @@ __global__ void k_momentum(DMesh m, DFields F, Ramp rp, MomParams q, const doubl
           rhs = rhs + D * uf[comp] - Ff * uf[comp];
         } else {
           ap = ap + Ff;
+        }
+        break;
+      case MS_BC_INTERFACE:   // :1144-1206
+        if (!m.iface_pdir) {
+          ap = ap + D + dmax(Ff, 0.0);
+          rhs = m.iface_zhang ? rhs + D * uf[comp] - dmin(Ff, 0.0) * uf[comp] : rhs + D * uf[comp] - Ff * uf[comp];
+        } else {
+          ap = ap + dmax(Ff, 0.0);
+          rhs = rhs - dmin(Ff, 0.0) * uP[comp];
+          if (m.iface_slip) {
+            const double ct = m.iface_slip_C[f];
+            if (ct > 0.0) {
+              double nh2[3];
+              for (int d = 0; d < 3; d++) nh2[d] = m.sf[3 * (size_t)f + d] / m.area[f];
+              const double *pv = &m.iface_peer_vel[3 * (size_t)f];
+              const double un_own = ddot3(uP, nh2), un_peer = ddot3(pv, nh2);
+              ap = ap + ct * (1.0 - nh2[comp] * nh2[comp]);
+              rhs = rhs + ct * (pv[comp] - un_peer * nh2[comp]);
+              rhs = rhs + ct * nh2[comp] * (un_own - uP[comp] * nh2[comp]);
+            }
+          }
         }
         break;
       default:
@@ -577,7 +619,7 @@ __global__ void k_solid_energy(DMesh m, DFields F, EnParams q, double *val, doub
     const int tm = m.tmode_s[f];
     if (tm == 1) {
       ap = ap + m.Dsb[f];
-      rhs = rhs + m.Dsb[f] * m.tval[f];
+      rhs = rhs + m.Dsb[f] * m.tval_s[f];
     } else {
       const double qf = tm == 2 ? m.qval[f] : 0.0;
       if (fabs(qf) > 0.0) {
@@ -707,7 +749,7 @@ __global__ void k_ppe(DMesh m, const double *flux, const double *af, double *val
   for (int k = k0; k < k1; k++) {
     const int e = m.cf_ent[k], f = e >> 1, side = e & 1;
     if (m.c1[f] < 0) {
-      if (p_dirichlet(m, f)) diag = diag + af[f];
+      if (p_dirichlet_ppe(m, f)) diag = diag + af[f];
       rhs = rhs - flux[f];
       continue;
     }
@@ -1040,6 +1082,7 @@ static void push_history(System &s, double a, double b, double c, double d, doub
 // simple_run (mod_uns_simple.f90:88-288) with the solver norms of the
 // partitioned path (sqrt of the global dot product).
 int run_simple(System &s, int nsteps, int *it_done) {
+  s.ppe_serial = true;   // simple_run assembles the serial ppe_assembly
   s.refresh_face_constants();
   Driver D(s);
   const ms_ctrl_t &c = s.ctrl;
@@ -1100,6 +1143,7 @@ int run_piso(System &s, int nsteps) {
   const ms_ctrl_t &c = s.ctrl;
   if (!c.transient) return 30;
   if (c.dt <= 0.0) return 31;
+  s.ppe_serial = false;  // piso_run_mpi: ppe_assembly_mpi
   s.refresh_face_constants();
   Driver D(s);
   const size_t n3 = 3 * (size_t)s.nc;
@@ -1167,6 +1211,7 @@ int run_pimple(System &s, int nsteps) {
   const ms_ctrl_t &c = s.ctrl;
   if (!c.transient) return 50;
   if (c.dt <= 0.0) return 51;
+  s.ppe_serial = false;  // pimple_run_mpi: ppe_assembly_mpi
   s.refresh_face_constants();
   Driver D(s);
   const size_t n3 = 3 * (size_t)s.nc;
@@ -1260,6 +1305,7 @@ void call_rhiechow(System &s) {
 }
 void call_outflow(System &s) { Driver(s).outflow(); }
 void call_ppe(System &s) {
+  s.ppe_serial = true;   // the per-call routine is the serial ppe_assembly
   s.refresh_face_constants();
   Driver(s).ppe();
 }

@@ -659,8 +659,7 @@ System *create_system(int nnodes, const double *x, int nfaces, const long long *
   const int nc = ncells, nf = nfaces;
   check_faces(nnodes, nf, fptr, fnodes, c0in, c1orig, nc);
   for (auto &g : s.groups)
-    if (g.btype == MS_BC_INTERFACE || g.btype < 1 || g.btype > 10)
-      fail(MS_EINVAL, "unsupported boundary type (coupling interface)");
+    if (g.btype < 1 || g.btype > 10) fail(MS_EINVAL, "unsupported boundary type");
 
   compute_geometry(s, x, nf, fptr, fnodes, c0in, c1orig, nc);
 
@@ -862,7 +861,15 @@ System *create_system(int nnodes, const double *x, int nfaces, const long long *
       int bt = s.groups[s.fgrp[f] - 1].btype;
       if (bt == MS_BC_POUTLET || bt == MS_BC_FARFIELD) s.has_pdir = 1;
       if (bt == MS_BC_OUTFLOW) s.has_outflow = 1;
+      if (bt == MS_BC_INTERFACE) s.has_iface = true;
     }
+  if (s.has_iface) {   // build_bc allocates the interface arrays zeroed (:325-330)
+    for (auto *v : {&s.iface_p, &s.iface_T, &s.iface_Ts, &s.iface_q, &s.iface_slip_C}) v->assign(nf, 0.0);
+    s.iface_vel.assign(3 * (size_t)nf, 0.0);
+    s.iface_peer_vel.assign(3 * (size_t)nf, 0.0);
+    s.iface_T_mode.assign(nf, 0);
+    s.iface_dirty = true;
+  }
   // flux_ref (simple_run :118-130)
   {
     double usc = 0.0, ab = 0.0;
@@ -965,6 +972,14 @@ System *create_system(int nnodes, const double *x, int nfaces, const long long *
 // per-face flags: fluid/porous faces (kink face pressure, Beavers-Joseph
 // slip), solid/fluid faces (conjugate heat transfer) and solid/solid faces.
 void System::refresh_face_constants() {
+  if (iface_dirty) {   // interface data: device copies; thermal face modes depend on them
+    d_iface_vel.upload(iface_vel);
+    d_iface_p.upload(iface_p);
+    d_iface_slip_C.upload(iface_slip_C);
+    d_iface_peer_vel.upload(iface_peer_vel);
+    iface_dirty = false;
+    props_dirty = true;
+  }
   if (!props_dirty) return;
   const double POROSITY_FLOOR = 1.0e-8;
   const ms_ctrl_t &c = ctrl;
@@ -989,7 +1004,8 @@ void System::refresh_face_constants() {
     return pidx[f] ? xc_nat[3 * (size_t)o + d] + psh[pidx[f] - 1][d] : xc_nat[3 * (size_t)o + d];
   };
   std::vector<double> dn(nf, 0.0), dvs(nf, 0.0), Dm(nf, 0.0), muf(nf, 0.0), tcor(3 * (size_t)nf, 0.0),
-      dw(nf, 0.0), Dmb(nf, 0.0), Dt(nf, 0.0), kfv(nf, 0.0), Dtb(nf, 0.0), tv(nf, 0.0), qv(nf, 0.0);
+      dw(nf, 0.0), Dmb(nf, 0.0), Dt(nf, 0.0), kfv(nf, 0.0), Dtb(nf, 0.0), tv(nf, 0.0), qv(nf, 0.0),
+      tvs(nf, 0.0);
   std::vector<signed char> tm(nf, 0), tms(nf, 0), ff(nf, 0);
   std::vector<double> Cbj(nf, 0.0), kch(nf, 0.0), Ds(nf, 0.0), ksfv(nf, 0.0), Dsb(nf, 0.0);
   bc_energy = false;
@@ -1103,9 +1119,19 @@ void System::refresh_face_constants() {
       } else if (g.btype == MS_BC_VINLET || g.btype == MS_BC_VINLET_PARAB || g.btype == MS_BC_MASSINLET) {
         tt = 1;
         tvv = g.tval;
+      } else if (g.btype == MS_BC_INTERFACE) {   // :671-701
+        if (iface_T_mode[f] == 1) {
+          tt = 2;
+          qvv = iface_q[f];
+        } else {
+          tt = 1;
+          tvv = iface_T[f];
+        }
+        if (tt == 1 && iface_T_ltne) tvs[f] = iface_Ts[f];
       }
       tm[f] = (signed char)tt;
       tv[f] = tt == 1 ? tvv : 0.0;
+      if (g.btype != MS_BC_INTERFACE || !(tt == 1 && iface_T_ltne)) tvs[f] = tv[f];
       qv[f] = tt == 2 ? qvv : 0.0;
       if ((tt == 1 && tvv != 0.0) || (tt == 2 && qvv != 0.0)) bc_energy = true;
       // the solid phase sees inlets as adiabatic (:2187-2193)
@@ -1158,6 +1184,7 @@ void System::refresh_face_constants() {
     d_ksf.upload(ksfv);
     d_Dsb.upload(Dsb);
     d_tmode_s.upload(tms);
+    if (has_iface && iface_T_ltne) d_tval_s.upload(tvs);
   }
   props_dirty = false;
 }
@@ -1168,7 +1195,15 @@ DMesh System::dmesh() const {
   m.nf = nf;
   m.nbc = nbc;
   m.pin_cell = pin_cell;
-  m.has_pdir = has_pdir;
+  m.iface_pdir = has_iface && iface_p_dir;
+  m.iface_ppe = m.iface_pdir && ppe_serial;
+  m.has_pdir = has_pdir || m.iface_ppe;
+  m.iface_slip = iface_slip;
+  m.iface_zhang = iface_zhang;
+  m.iface_vel = has_iface ? d_iface_vel.p : nullptr;
+  m.iface_p = has_iface ? d_iface_p.p : nullptr;
+  m.iface_slip_C = has_iface ? d_iface_slip_C.p : nullptr;
+  m.iface_peer_vel = has_iface ? d_iface_peer_vel.p : nullptr;
   m.nonorth = ctrl.nonorth_corr > 0.0;
   m.blend_on = ctrl.conv_blend > 0.0;
   m.c0 = d_c0.p;
@@ -1212,6 +1247,7 @@ DMesh System::dmesh() const {
   m.ksf = d_ksf.p;
   m.Dsb = d_Dsb.p;
   m.tmode_s = d_tmode_s.p;
+  m.tval_s = (has_iface && iface_T_ltne && d_tval_s.p) ? d_tval_s.p : d_tval.p;
   for (int k = 0; k < 4; k++)
     for (int d = 0; d < 3; d++) m.psh[k][d] = psh[k][d];
   return m;
@@ -1262,7 +1298,8 @@ size_t System::device_bytes() const {
                                 &ap, &pp, &dpp, &af, &ucomp, &part, &diag8, &tmpc, &disp_l_d, &disp_t_d, &h_sf_d,
                                 &a_sf_d, &q_src_d, &kdisp, &gp_prev, &d_Cbj, &d_kch, &d_Ds, &d_ksf, &d_Dsb})
     b += v->bytes();
-  b += d_fflag.bytes() + d_tmode_s.bytes() + d_cztype.bytes();
+  b += d_fflag.bytes() + d_tmode_s.bytes() + d_cztype.bytes() + d_iface_vel.bytes() + d_iface_p.bytes() +
+       d_iface_slip_C.bytes() + d_iface_peer_vel.bytes() + d_tval_s.bytes();
   b += d_c0.bytes() + d_c1.bytes() + d_fgrp.bytes() + d_pidx.bytes() + d_cf_ptr.bytes() + d_cf_ent.bytes() + d_cf_slot.bytes() +
        d_diag_slot.bytes();
   return b;

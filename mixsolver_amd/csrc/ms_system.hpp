@@ -38,6 +38,12 @@ struct DMesh {
   const signed char *fflag;
   const double *Cbj, *kch, *Ds, *ksf, *Dsb;
   const signed char *tmode_s;             // solid-phase boundary thermal mode
+  const double *tval_s;                   // solid-phase Dirichlet values (null: tval)
+  // coupling interface (BC_INTERFACE faces): per-face data in face order and
+  // the switches; iface_ppe = the PPE treats interface faces as
+  // pressure-Dirichlet (serial drivers under the interface pressure model)
+  int iface_pdir, iface_ppe, iface_slip, iface_zhang;
+  const double *iface_vel, *iface_p, *iface_slip_C, *iface_peer_vel;
 };
 
 enum { FF_KINK = 1, FF_BJ = 2, FF_CHT = 4, FF_SOLID2 = 8 };
@@ -109,6 +115,15 @@ struct System {
   DBuf<unsigned int> ticket;
   std::vector<double> history;  // 8 per row
   int pin_cell = 0, has_pdir = 0, has_outflow = 0;
+  // coupling interface: per-face data (face order, zero until set), the
+  // switches of set_iface_p_dirichlet / _slip_mode / _zhang_vel /
+  // _t_ltne_mode, and whether the current driver assembles the PPE the
+  // serial way (interface faces pressure-Dirichlet under the pressure model)
+  bool has_iface = false, iface_dirty = false, ppe_serial = true;
+  int iface_p_dir = 0, iface_slip = 0, iface_zhang = 0, iface_T_ltne = 0;
+  std::vector<double> iface_vel, iface_p, iface_T, iface_Ts, iface_q, iface_slip_C, iface_peer_vel;
+  std::vector<int> iface_T_mode;
+  DBuf<double> d_iface_vel, d_iface_p, d_iface_slip_C, d_iface_peer_vel, d_tval_s;
   // wall-zone post-processing: node z range (the span of the reference area)
   // and the wall faces in face order (built on first use)
   double zspan = 0.0;

@@ -38,7 +38,8 @@ class Case:
         self.mesh = mesh
         self.nc, self.nf = mesh.ncells, mesh.nfaces
         self.ctrl = ctrl
-        self._garr = groups_array(groups)
+        from .control import with_interface_groups
+        self._garr = groups_array(with_interface_groups(mesh, ctrl, groups))
         h = C.c_void_p()
         x = np.ascontiguousarray(mesh.x, np.float64)
         fptr = np.ascontiguousarray(mesh.fptr, np.int64)
@@ -66,6 +67,17 @@ class Case:
             raise ValueError(f"cztype: expected {self.nc} values, got {t.size}")
         check(_lib.lib().ms_set_cell_types(self.h, None if t is None else ip(t)))
         self.cztype = None if t is None else t.copy()
+
+    def set_interface(self, what, values):
+        """Coupling-interface data over all faces (natural face order):
+        vel (3,nf), p, T, Ts, q, slip_C, peer_vel (3,nf), T_mode (0/1); see
+        ms_set_interface."""
+        v = np.ascontiguousarray(values, np.float64).ravel()
+        check(_lib.lib().ms_set_interface(self.h, what.encode(), dp(v), v.size))
+
+    def set_interface_modes(self, p_dir=False, slip=False, zhang=False, T_ltne=False):
+        """The coupling driver's interface switches (set_iface_*)."""
+        check(_lib.lib().ms_set_interface_modes(self.h, int(p_dir), int(slip), int(zhang), int(T_ltne)))
 
     def get_ts_order(self):
         t = C.c_int(0)

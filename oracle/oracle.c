@@ -64,6 +64,11 @@ struct or_case {
   double *porosity, *perm_dir, *inertial, *k_s, *rho_s, *cp_s;
   double *disp_l, *disp_t, *bj_alpha, *h_sf, *a_sf, *q_src;
   int *cztype;                   /* m%cztype: 1 fluid, 2 porous, 3 solid */
+  /* coupling interface (bc_t iface_* arrays, mod_uns_bc.f90:300-330, and the
+   * module switches g_iface_p_dir / _slip / _zhang_vel / _T_ltne) */
+  double *iface_vel, *iface_p, *iface_T, *iface_Ts, *iface_q, *iface_slip_C, *iface_peer_vel;
+  int *iface_T_mode;
+  int iface_p_dir, iface_slip, iface_zhang, iface_T_ltne;
   int ts_order;
   double *hist;
   int nhist, caphist;
@@ -198,6 +203,10 @@ static void bc_face_vel(const or_case *cs, int f, const double *uP, double *uf) 
       un = dot3(uP, nvec);
       for (int d = 0; d < 3; d++) uf[d] = uP[d] - un * nvec[d];
       break;
+    case OR_BC_INTERFACE:   /* :460-474: extrapolated under the pressure model */
+      if (cs->iface_p_dir) { uf[0] = uP[0]; uf[1] = uP[1]; uf[2] = uP[2]; }
+      else { uf[0] = cs->iface_vel[3 * f]; uf[1] = cs->iface_vel[3 * f + 1]; uf[2] = cs->iface_vel[3 * f + 2]; }
+      break;
     default: /* POUTLET, OUTFLOW */
       uf[0] = uP[0]; uf[1] = uP[1]; uf[2] = uP[2];
   }
@@ -207,18 +216,34 @@ static void bc_face_vel(const or_case *cs, int f, const double *uP, double *uf) 
 static double bc_face_p(const or_case *cs, int f, double pP) {
   const or_bcgroup_t *g = &cs->gb[cs->fgrp[f] - 1];
   if (g->btype == OR_BC_POUTLET || g->btype == OR_BC_FARFIELD) return g->pval * cs->pval_ramp;
+  if (g->btype == OR_BC_INTERFACE && cs->iface_p_dir) return cs->iface_p[f];
   return pP;
 }
 
-static int bc_p_dirichlet(const or_case *cs, int f) {  /* :879-893 */
+/* bc_face_p_dirichlet (:879-893): POUTLET / FARFIELD, and interface faces
+ * under the interface pressure model */
+static int bc_p_dirichlet(const or_case *cs, int f) {
+  int gi = cs->fgrp[f];
+  if (gi <= 0) return 0;
+  int bt = cs->gb[gi - 1].btype;
+  return bt == OR_BC_POUTLET || bt == OR_BC_FARFIELD || (bt == OR_BC_INTERFACE && cs->iface_p_dir);
+}
+
+/* the pressure-Dirichlet test of ppe_assembly_mpi (mod_uns_simple_mpi.f90:
+ * 903-907, 942-946): POUTLET / FARFIELD by type only */
+static int bc_p_dirichlet_mpi(const or_case *cs, int f) {
   int gi = cs->fgrp[f];
   if (gi <= 0) return 0;
   int bt = cs->gb[gi - 1].btype;
   return bt == OR_BC_POUTLET || bt == OR_BC_FARFIELD;
 }
 
-/* bc_face_T, mod_uns_bc.f90:560-703 */
+/* bc_face_T, mod_uns_bc.f90:560-703; phase 2 = the solid-phase equation */
+static void bc_face_T_phase(const or_case *cs, int f, double Tc, double *Tf, double *qf, int *neum, int phase);
 static void bc_face_T(const or_case *cs, int f, double Tc, double *Tf, double *qf, int *neum) {
+  bc_face_T_phase(cs, f, Tc, Tf, qf, neum, 0);
+}
+static void bc_face_T_phase(const or_case *cs, int f, double Tc, double *Tf, double *qf, int *neum, int phase) {
   const or_bcgroup_t *g = &cs->gb[cs->fgrp[f] - 1];
   const double *xf = &cs->xf[3 * f];
   int tt;
@@ -243,6 +268,11 @@ static void bc_face_T(const or_case *cs, int f, double Tc, double *Tf, double *q
     case OR_BC_VINLET_PARAB:
     case OR_BC_MASSINLET:
       *neum = 0; *Tf = g->tval; *qf = 0.0;
+      break;
+    case OR_BC_INTERFACE:   /* :671-701 */
+      if (cs->iface_T_mode[f] == 1) { *neum = 1; *qf = cs->iface_q[f]; *Tf = Tc; break; }
+      *neum = 0; *qf = 0.0; *Tf = cs->iface_T[f];
+      if (cs->iface_T_ltne && phase == 2) *Tf = cs->iface_Ts[f];
       break;
     default: /* SYMMETRY, FARFIELD, SLIPWALL */
       *neum = 1; *qf = 0.0; *Tf = Tc;
@@ -452,7 +482,6 @@ or_case *or_create(int nn, const double *x, int nf, const long long *fptr,
     if (g->btype == OR_BC_MASSINLET) g->uspeed = g->mdot / ctrl->rho;
     if (g->has_lid) g->lid_tol = 1.0e-4 * cr;
     if (g->ntbc_plane > 0) g->tbc_tol = 1.0e-4 * cr;
-    if (g->btype == OR_BC_INTERFACE) *ier = 50;
     int nb = 0;
     for (int f = 0; f < nf; f++)
       if (cs->c1[f] == 0 && cs->zone[f] == g->zone) { nb++; cs->fgrp[f] = i + 1; }
@@ -468,6 +497,9 @@ or_case *or_create(int nn, const double *x, int nf, const long long *fptr,
   cs->apc = xcalloc(nc, 8); cs->u_old = xcalloc(n3, 8); cs->u_old_old = xcalloc(n3, 8);
   cs->T = xcalloc(nc, 8); cs->T_old = xcalloc(nc, 8); cs->T_old_old = xcalloc(nc, 8);
   cs->gt = xcalloc(n3, 8); cs->T_s = xcalloc(nc, 8); cs->T_s_old = xcalloc(nc, 8);
+  cs->iface_vel = xcalloc(3 * (size_t)nf, 8); cs->iface_peer_vel = xcalloc(3 * (size_t)nf, 8);
+  cs->iface_p = xcalloc(nf, 8); cs->iface_T = xcalloc(nf, 8); cs->iface_Ts = xcalloc(nf, 8);
+  cs->iface_q = xcalloc(nf, 8); cs->iface_slip_C = xcalloc(nf, 8); cs->iface_T_mode = xcalloc(nf, sizeof(int));
   cs->T_s_old_old = xcalloc(nc, 8); cs->gts = xcalloc(n3, 8);
   cs->porosity = xcalloc(nc, 8); cs->perm_dir = xcalloc(n3, 8); cs->inertial = xcalloc(nc, 8);
   cs->k_s = xcalloc(nc, 8); cs->rho_s = xcalloc(nc, 8); cs->cp_s = xcalloc(nc, 8);
@@ -499,7 +531,9 @@ void or_destroy(or_case *cs) {
                   cs->u_old, cs->u_old_old, cs->T, cs->T_old, cs->T_old_old, cs->gt, cs->T_s,
                   cs->T_s_old, cs->T_s_old_old, cs->gts, cs->porosity, cs->perm_dir,
                   cs->inertial, cs->k_s, cs->rho_s, cs->cp_s, cs->hist, cs->disp_l, cs->disp_t,
-                  cs->bj_alpha, cs->h_sf, cs->a_sf, cs->q_src, cs->cztype};
+                  cs->bj_alpha, cs->h_sf, cs->a_sf, cs->q_src, cs->cztype, cs->iface_vel,
+                  cs->iface_peer_vel, cs->iface_p, cs->iface_T, cs->iface_Ts, cs->iface_q,
+                  cs->iface_slip_C, cs->iface_T_mode};
   for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
   free(cs);
 }
@@ -801,6 +835,28 @@ void or_momentum_assembly(or_case *cs, int comp, double *val, double *rhs, doubl
           ap[c0 - 1] = ap[c0 - 1] + F;
         }
         break;
+      case OR_BC_INTERFACE:   /* :1144-1206 */
+        if (!cs->iface_p_dir) {
+          ap[c0 - 1] = ap[c0 - 1] + D + fmax0(F, 0.0);
+          if (cs->iface_zhang) rhs[c0 - 1] = rhs[c0 - 1] + D * uf[comp] - fmin0(F, 0.0) * uf[comp];
+          else rhs[c0 - 1] = rhs[c0 - 1] + D * uf[comp] - F * uf[comp];
+        } else {
+          ap[c0 - 1] = ap[c0 - 1] + fmax0(F, 0.0);
+          rhs[c0 - 1] = rhs[c0 - 1] - fmin0(F, 0.0) * U(c0, comp);
+          if (cs->iface_slip) {
+            const double ctc = cs->iface_slip_C[i];
+            if (ctc > 0.0) {
+              double nh2[3];
+              for (int d = 0; d < 3; d++) nh2[d] = SF(i, d) / cs->area[i];
+              const double un_own = dot3(&U(c0, 0), nh2);
+              const double un_peer = dot3(&cs->iface_peer_vel[3 * i], nh2);
+              ap[c0 - 1] = ap[c0 - 1] + ctc * (1.0 - nh2[comp] * nh2[comp]);
+              rhs[c0 - 1] = rhs[c0 - 1] + ctc * (cs->iface_peer_vel[3 * i + comp] - un_peer * nh2[comp]);
+              rhs[c0 - 1] = rhs[c0 - 1] + ctc * nh2[comp] * (un_own - U(c0, comp) * nh2[comp]);
+            }
+          }
+        }
+        break;
       default: break;
     }
     rhs[c0 - 1] = rhs[c0 - 1] - pf * SF(i, comp);
@@ -1078,7 +1134,7 @@ void or_solid_temperature_assembly(or_case *cs, double *val, double *rhs, double
     int c0 = cs->c0[i];
     double Tf, qf, r[3];
     int neum;
-    bc_face_T(cs, i, cs->T_s[c0 - 1], &Tf, &qf, &neum);
+    bc_face_T_phase(cs, i, cs->T_s[c0 - 1], &Tf, &qf, &neum, 2);
     int bt = cs->gb[gi - 1].btype;
     if (bt == OR_BC_VINLET || bt == OR_BC_VINLET_PARAB || bt == OR_BC_MASSINLET) { neum = 1; qf = 0.0; }
     for (int d = 0; d < 3; d++) r[d] = XF(i, d) - XC(c0, d);
@@ -1180,7 +1236,13 @@ void or_outflow_mass_rescale(or_case *cs) {
 
 /* ------------------------------------------------------------------ PPE */
 
-void or_ppe_assembly(or_case *cs, double *val, double *rhs) {
+/* ppe_assembly (serial, mod_uns_simple.f90:1552-1644: bc_face_p_dirichlet)
+ * or ppe_assembly_mpi at one partition (mod_uns_simple_mpi.f90:882-968:
+ * POUTLET / FARFIELD only); they differ on interface faces under the
+ * interface pressure model */
+static void ppe_assembly_impl(or_case *cs, double *val, double *rhs, int mpi);
+void or_ppe_assembly(or_case *cs, double *val, double *rhs) { ppe_assembly_impl(cs, val, rhs, 0); }
+static void ppe_assembly_impl(or_case *cs, double *val, double *rhs, int mpi) {
   const or_ctrl_t *ct = &cs->ctrl;
   for (int k = 0; k < cs->nnz; k++) val[k] = 0.0;
   for (int c = 0; c < cs->nc; c++) rhs[c] = 0.0;
@@ -1188,7 +1250,7 @@ void or_ppe_assembly(or_case *cs, double *val, double *rhs) {
   for (int i = 0; i < cs->nf; i++) {
     int c0 = cs->c0[i], c1 = cs->c1[i];
     if (c1 == 0) {
-      if (cs->fgrp[i] > 0 && bc_p_dirichlet(cs, i)) {
+      if (cs->fgrp[i] > 0 && (mpi ? bc_p_dirichlet_mpi(cs, i) : bc_p_dirichlet(cs, i))) {
         double r[3];
         for (int d = 0; d < 3; d++) r[d] = XF(i, d) - XC(c0, d);
         double dbf = cs->vol[c0 - 1] / cs->apc[c0 - 1];
@@ -1569,7 +1631,7 @@ static void momentum_solves(or_case *cs, double *val, double *rhs, double *ap, d
 static int ppe_solve(or_case *cs, double *val, double *rhs, double *pp, int mpi, int *itl) {
   const or_ctrl_t *ct = &cs->ctrl;
   int ierr; double resl;
-  or_ppe_assembly(cs, val, rhs);
+  ppe_assembly_impl(cs, val, rhs, mpi);
   for (int c = 0; c < cs->nc; c++) pp[c] = 0.0;
   double t0 = now_s();
   if (ct->ppe_precond == 1)
@@ -2007,4 +2069,40 @@ void or_separation(or_case *cs, double *out) {
   out[4] = (xw - xcyl) / 1.0;
   free(th);
   free(ua);
+}
+
+
+/* ---------------------------------------------------- coupling interface */
+/* set_interface_vel / _p / _T / _q / _T_ltne / _slip / _T_mode
+ * (mod_uns_bc.f90:730-869) over all faces: what = "vel" (3,nf), "p", "T",
+ * "Ts", "q", "slip_C",
+ * "peer_vel" (3,nf), "T_mode" (values 0/1 as doubles; 1 = Neumann with q,
+ * what set_interface_q switches on).  Faces outside the
+ * interface group keep the values but never read them. */
+int or_set_interface(or_case *cs, const char *what, const double *v) {
+  int nf = cs->nf;
+  double *dst = NULL;
+  int w = 1;
+  if (!strcmp(what, "vel")) { dst = cs->iface_vel; w = 3; }
+  else if (!strcmp(what, "p")) dst = cs->iface_p;
+  else if (!strcmp(what, "T")) dst = cs->iface_T;
+  else if (!strcmp(what, "Ts")) dst = cs->iface_Ts;
+  else if (!strcmp(what, "q")) dst = cs->iface_q;
+  else if (!strcmp(what, "slip_C")) dst = cs->iface_slip_C;
+  else if (!strcmp(what, "peer_vel")) { dst = cs->iface_peer_vel; w = 3; }
+  else if (!strcmp(what, "T_mode")) {
+    for (int f = 0; f < nf; f++) cs->iface_T_mode[f] = (int)v[f];
+    return 0;
+  } else return 1;
+  memcpy(dst, v, (size_t)w * nf * sizeof(double));
+  return 0;
+}
+
+/* set_iface_p_dirichlet / set_iface_slip_mode / set_iface_zhang_vel /
+ * set_iface_t_ltne_mode (mod_uns_bc.f90:823-857) */
+void or_set_interface_modes(or_case *cs, int p_dir, int slip, int zhang, int T_ltne) {
+  cs->iface_p_dir = p_dir != 0;
+  cs->iface_slip = slip != 0;
+  cs->iface_zhang = zhang != 0;
+  cs->iface_T_ltne = T_ltne != 0;
 }

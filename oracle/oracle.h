@@ -123,6 +123,9 @@ void or_timing(or_case *, double *out3);
  * count; compute_separation -> out5 = theta_top, found_top, theta_bot,
  * found_bot, wake length */
 void or_compute_forces(or_case *, double *out14);
+/* coupling interface data and switches (mod_uns_bc.f90:730-869) */
+int or_set_interface(or_case *, const char *what, const double *values);
+void or_set_interface_modes(or_case *, int p_dir, int slip, int zhang, int T_ltne);
 int or_cp_distribution(or_case *, double *rows6);
 void or_separation(or_case *, double *out5);
 

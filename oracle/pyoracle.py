@@ -60,6 +60,8 @@ def lib():
         L.or_ilu0_factor.argtypes = [C.c_int, _I, _I, _D]
         L.or_csr_matvec.argtypes = [C.c_int, _I, _I, _D, _D, _D]
         L.or_compute_forces.argtypes = [C.c_void_p, _D]
+        L.or_set_interface.argtypes = [C.c_void_p, C.c_char_p, _D]
+        L.or_set_interface_modes.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]
         L.or_cp_distribution.argtypes = [C.c_void_p, _D]
         L.or_separation.argtypes = [C.c_void_p, _D]
         L.or_norm2.restype = C.c_double
@@ -89,6 +91,8 @@ class OracleCase:
         self._keep = (x, fptr)
         ier = C.c_int(0)
         self.ctrl = ctrl
+        from mixsolver_amd.control import with_interface_groups
+        groups = with_interface_groups(mesh, ctrl, groups)
         garr = groups_array(groups)
         self.h = L.or_create(mesh.nnodes, _dp(x), self.nf, fptr.ctypes.data_as(C.POINTER(C.c_longlong)),
                              _ip(np.ascontiguousarray(mesh.fnodes, np.int32)),
@@ -215,6 +219,15 @@ class OracleCase:
         out = np.zeros(3)
         lib().or_timing(self.h, _dp(out))
         return float(out[0]), float(out[1]), int(out[2])
+
+    def set_interface(self, what, values):
+        """Per-face coupling-interface data over all faces (see or_set_interface)."""
+        v = np.ascontiguousarray(values, np.float64).ravel()
+        if lib().or_set_interface(self.h, what.encode(), _dp(v)) != 0:
+            raise ValueError(what)
+
+    def set_interface_modes(self, p_dir=False, slip=False, zhang=False, T_ltne=False):
+        lib().or_set_interface_modes(self.h, int(p_dir), int(slip), int(zhang), int(T_ltne))
 
     def compute_forces(self):
         """compute_forces: dict of Fx Fy Fz CD CL u_inf A_ref p_inf and the

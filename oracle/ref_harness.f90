@@ -21,6 +21,13 @@
 !   vtu    : seed fields from in.bin (u,p,T), T_s = 0.5 T + 1, then vtk_write
 !            to a.vtu (inline ASCII) and b.vtu (appended binary) in the
 !            working directory
+!   ifuncs : funcs2 with coupling-interface data: after the funcs2 inputs,
+!            in.bin holds the four interface switches (int p_dir, slip, zhang,
+!            T_ltne), then over all faces vel(3), p, T, Ts, q, slip_C,
+!            peer_vel(3) (doubles) and T_mode (int); they are applied to the
+!            faces of the BC_INTERFACE group through set_interface_* and the
+!            set_iface_* switches
+!   isimple: u, p, T then the interface block as in ifuncs; simple_run(n)
 !   solve  : CSR system from in.bin; residual histories of cg_jacobi, cg_ic0,
 !            bicgstab_ilu0 for maxit = 1..n, IC0/ILU0 factors, final solutions
 ! All arrays are written as raw little-endian stream records in the order the
@@ -113,6 +120,17 @@ program ref_harness
       call run_funcs()
    case ( 'funcs2' )
       call run_funcs2()
+   case ( 'ifuncs' )
+      call run_funcs2( .true. )
+   case ( 'isimple' )
+      open( newunit=ui, file=trim(inf), access='stream', form='unformatted', &
+            status='old', action='read' )
+      read(ui) fld%u, fld%p, fld%T
+      call read_iface( ui )
+      close( ui )
+      call simple_run( m, c, g, ctrl, bcs, fld, ier, nsteps )
+      write(uo) ier
+      call dump_state()
    case default
       write(*,*) 'unknown mode ', trim(mode)
       stop 2
@@ -197,7 +215,40 @@ contains
    end subroutine run_funcs
 
    ! in.bin: as run_funcs, then T_s(nc), T_s_old(nc), T_s_old_old(nc)
-   subroutine run_funcs2()
+   ! interface block of in.bin (see the usage notes above)
+   subroutine read_iface( un )
+      integer, intent(in) :: un
+      integer :: sw(4), ig, k, nfi, nq
+      real(dp), allocatable :: vel(:,:), pv(:), tv(:), tsv(:), qv(:), cv(:), peer(:,:)
+      integer, allocatable :: tmode(:), faces(:), qfaces(:)
+      allocate( vel(3,m%nfaces), pv(m%nfaces), tv(m%nfaces), tsv(m%nfaces), qv(m%nfaces), &
+                cv(m%nfaces), peer(3,m%nfaces), tmode(m%nfaces) )
+      read(un) sw
+      read(un) vel, pv, tv, tsv, qv, cv, peer, tmode
+      ig = 0
+      do k = 1, bcs%nb
+         if ( bcs%gb(k)%btype == BC_INTERFACE ) ig = k
+      end do
+      if ( ig == 0 ) stop 4
+      nfi = bcs%gb(ig)%nf
+      allocate( faces(nfi) )
+      faces = bcs%gb(ig)%faces(1:nfi)
+      call set_interface_vel( bcs, faces, vel(:, faces) )
+      call set_interface_p( bcs, faces, pv(faces) )
+      call set_interface_T_ltne( bcs, faces, tv(faces), tsv(faces) )
+      call set_interface_slip( bcs, faces, cv(faces), peer(:, faces) )
+      nq = count( tmode(faces) == 1 )
+      allocate( qfaces(nq) )
+      qfaces = pack( faces, tmode(faces) == 1 )
+      if ( nq > 0 ) call set_interface_q( bcs, qfaces, qv(qfaces) )
+      call set_iface_p_dirichlet( sw(1) /= 0 )
+      call set_iface_slip_mode( sw(2) /= 0 )
+      call set_iface_zhang_vel( sw(3) /= 0 )
+      call set_iface_t_ltne_mode( sw(4) /= 0 )
+   end subroutine read_iface
+
+   subroutine run_funcs2( with_iface )
+      logical, intent(in), optional :: with_iface
       type(csr_t) :: A
       real(dp), allocatable :: rhs(:), ap(:), pp(:), phif(:), dpp(:,:)
       integer :: comp, tso
@@ -208,6 +259,9 @@ contains
       read(ui) fld%u, fld%p, fld%T, fld%flux, fld%u_old, fld%u_old_old, &
                fld%T_old, fld%T_old_old, fld%apc, pp, tso
       read(ui) fld%T_s, fld%T_s_old, fld%T_s_old_old
+      if ( present(with_iface) ) then
+         if ( with_iface ) call read_iface( ui )
+      end if
       close( ui )
       fld%ts_order = tso
       write(uo) m%cztype

@@ -29,6 +29,24 @@ def mesh_from(d):
 
 
 POROUS_CASES = ("porous_plug_ltne", "porous_plug_unified", "porous_solid_cht")
+IFACE_CASES = ("iface_legacy", "iface_zhang", "iface_pdir_slip_ltne")
+IFACE_KEYS = ("vel", "p", "T", "Ts", "q", "slip_C", "peer_vel", "T_mode")
+
+
+def iface_mesh(d):
+    """Mesh of an interface fixture: zone 6 is the named coupling interface."""
+    m = mesh_from(d)
+    m.zones = {6: ("interface", "interface-6")}
+    return m
+
+
+def seed_interface(case, d):
+    """Apply an interface fixture's per-face data and switches to a device
+    Case or an OracleCase."""
+    for k in IFACE_KEYS:
+        case.set_interface(k, d["iface_" + k])
+    p_dir, slip, zhang, t_ltne = (bool(v) for v in d["switches"])
+    case.set_interface_modes(p_dir=p_dir, slip=slip, zhang=zhang, T_ltne=t_ltne)
 FUNCS2_FIELDS = ("u", "p", "T", "flux", "u_old", "u_old_old", "T_old", "T_old_old", "apc", "T_s", "T_s_old",
                  "T_s_old_old")
 

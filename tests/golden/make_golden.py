@@ -675,6 +675,120 @@ def fortran_formats():
     return dict(values=v, es24_16=np.array(es), es24_16e3=np.array(es3))
 
 
+IFACE_CASES = {   # name: (p_dir, slip, zhang, T_ltne)
+    "iface_legacy": (0, 0, 0, 0),
+    "iface_zhang": (0, 0, 1, 0),
+    "iface_pdir_slip_ltne": (1, 1, 0, 1),
+}
+
+
+def iface_data(m, seed):
+    """Seeded coupling-interface data over all faces."""
+    rng = np.random.default_rng(seed)
+    nf = m.nfaces
+    vel = np.zeros((nf, 3))
+    vel[:, 0] = 1.0 + 0.2 * rng.uniform(-1, 1, nf)
+    vel[:, 1:] = 0.1 * rng.normal(size=(nf, 2))
+    tmode = np.zeros(nf, np.int32)
+    tmode[::3] = 1
+    return dict(vel=vel, p=0.1 * rng.normal(size=nf), T=rng.uniform(0.5, 1.5, nf), Ts=rng.uniform(0.5, 1.5, nf),
+                q=rng.normal(size=nf), slip_C=rng.uniform(0.01, 0.2, nf), peer_vel=rng.normal(size=(nf, 3)),
+                T_mode=tmode)
+
+
+def write_iface(f, sw, d):
+    np.array(sw, np.int32).tofile(f)
+    for k in ("vel", "p", "T", "Ts", "q", "slip_C", "peer_vel"):
+        np.ascontiguousarray(d[k], np.float64).tofile(f)
+    np.ascontiguousarray(d["T_mode"], np.int32).tofile(f)
+
+
+def iface_fixtures():
+    """Coupling-interface faces (mod_uns_bc.f90:282-323, :460-474, :527-545,
+    :671-701, :879-893; mod_uns_simple.f90:1144-1206): the inlet zone of the
+    zoned LTNE channel becomes a named interface zone without a bc line; the
+    harness applies seeded per-face values through set_interface_* and the
+    four switches, then writes the per-function outputs (ts_order 0 and 2)
+    and a 3-iteration serial simple_run."""
+    m = zoned_channel()
+    zk = {4: "symmetry", 5: "wall", 6: "interface", 7: "outflow"}
+    ctl = "\n".join(l for l in POROUS_CTL.format(model=PLUG_LTNE[0], zones=PLUG_LTNE[1]).splitlines()
+                    if not l.startswith("bc = 6 "))
+    for name, sw in IFACE_CASES.items():
+        d = iface_data(m, 400 + len(name))
+        fx = {}
+        for tso in (0, 2):
+            rng = np.random.default_rng(500 + tso)
+            nc, nf = m.ncells, m.nfaces
+            u, p, T = seeded_fields(m, 500 + tso)
+            u[:, 0] += 1.0
+            flux = rng.normal(size=nf) * 0.1
+            u_old = u + 0.05 * rng.normal(size=u.shape)
+            u_oo = u + 0.05 * rng.normal(size=u.shape)
+            T_old = T + 0.05 * rng.normal(size=nc)
+            T_oo = T + 0.05 * rng.normal(size=nc)
+            apc = rng.uniform(1.0, 3.0, size=nc)
+            pp = rng.normal(size=nc) * 0.01
+            Ts = T + 0.2 * rng.normal(size=nc)
+            Ts_old = Ts + 0.05 * rng.normal(size=nc)
+            Ts_oo = Ts + 0.05 * rng.normal(size=nc)
+            ins = dict(u=u, p=p, T=T, flux=flux, u_old=u_old, u_old_old=u_oo, T_old=T_old, T_old_old=T_oo,
+                       apc=apc, pp=pp, T_s=Ts, T_s_old=Ts_old, T_s_old_old=Ts_oo)
+            with tempfile.TemporaryDirectory() as td:
+                cas.write_cas(os.path.join(td, "m.cas"), m, zk)
+                open(os.path.join(td, "c.control"), "w").write(ctl)
+                inf = os.path.join(td, "in.bin")
+                with open(inf, "wb") as f:
+                    for a in (u, p, T, flux, u_old, u_oo, T_old, T_oo, apc, pp, np.array([tso], np.int32), Ts,
+                              Ts_old, Ts_oo):
+                        np.ascontiguousarray(a).tofile(f)
+                    write_iface(f, sw, d)
+                run(["m.cas", "c.control", "ifuncs", "out.bin", inf], td)
+                r = Reader(os.path.join(td, "out.bin"))
+                g = geom_run(m, zk, ctl)
+                nnz = len(g["col_idx"])
+                r.take(np.int32, nc)
+                r.take(np.float64, 3 * nc + 11 * nc)
+                out = dict(gp=r.take(np.float64, 3 * nc).reshape(nc, 3), gu=r.take(np.float64, 9 * nc).reshape(nc, 9),
+                           gt=r.take(np.float64, 3 * nc).reshape(nc, 3), gts=r.take(np.float64, 3 * nc).reshape(nc, 3))
+                for comp in (1, 2, 3):
+                    out[f"mom{comp}_val"] = r.take(np.float64, nnz)
+                    out[f"mom{comp}_rhs"] = r.take(np.float64, nc)
+                    out[f"mom{comp}_ap"] = r.take(np.float64, nc)
+                for t in ("tem", "sol"):
+                    out[f"{t}_val"] = r.take(np.float64, nnz)
+                    out[f"{t}_rhs"] = r.take(np.float64, nc)
+                    out[f"{t}_ap"] = r.take(np.float64, nc)
+                out["flux_rc"] = r.take(np.float64, nf)
+                out["flux_out"] = r.take(np.float64, nf)
+                out["cor_u"] = r.take(np.float64, 3 * nc).reshape(nc, 3)
+                out["cor_p"] = r.take(np.float64, nc)
+                out["cor_flux"] = r.take(np.float64, nf)
+                out["cor_dpp"] = r.take(np.float64, 3 * nc).reshape(nc, 3)
+                out["cor_u_old"] = r.take(np.float64, 3 * nc).reshape(nc, 3)
+            fx.update({f"ts{tso}_in_{k}": v for k, v in ins.items()})
+            fx.update({f"ts{tso}_out_{k}": v for k, v in out.items()})
+        # serial simple_run, 3 iterations, from the ts_order-0 fields
+        with tempfile.TemporaryDirectory() as td:
+            cas.write_cas(os.path.join(td, "m.cas"), m, zk)
+            open(os.path.join(td, "c.control"), "w").write(ctl)
+            inf = os.path.join(td, "in.bin")
+            with open(inf, "wb") as f:
+                for a in (fx["ts0_in_u"], fx["ts0_in_p"], fx["ts0_in_T"]):
+                    np.ascontiguousarray(a).tofile(f)
+                write_iface(f, sw, d)
+            run(["m.cas", "c.control", "isimple", "out.bin", inf, "3"], td)
+            r = Reader(os.path.join(td, "out.bin"))
+            nc, nf = m.ncells, m.nfaces
+            sim = dict(ier=np.int64(r.take(np.int32, 1)[0]), u=r.take(np.float64, 3 * nc).reshape(nc, 3),
+                       p=r.take(np.float64, nc), flux=r.take(np.float64, nf), T=r.take(np.float64, nc))
+            r.take(np.float64, 3 * nc + 9 * nc + nc + 3 * nc)
+            sim["T_s"] = r.take(np.float64, nc)
+        save(name, control=ctl, switches=np.array(sw, np.int64), **mesh_arrays(m),
+             **{"iface_" + k: v for k, v in d.items()}, **fx,
+             **{"simple_" + k: v for k, v in sim.items()})
+
+
 def main():
     if not os.path.exists(HARNESS):
         sys.exit("build the reference harness first: oracle/build_ref.sh")
@@ -717,6 +831,9 @@ def main():
         save("cavity_re1000_known_answer", control=ctlk, nsteps=np.int64(1000), **mesh_arrays(mk),
              out_ier=np.int64(ierk), ref_umax=umax, ref_ghia_l2=l2,
              readme_umax=0.911512, readme_ghia_l2=0.0289706)
+
+    # 3i. coupling-interface faces
+    iface_fixtures()
 
     # 3h. flang's ES24.16 / ES24.16E3 output (field dump and VTU numbers)
     save("fortran_formats", **fortran_formats())
@@ -811,6 +928,8 @@ def main():
 if __name__ == "__main__":
     if sys.argv[1:] == ["--porous"]:
         porous_fixtures()
+    elif sys.argv[1:] == ["--iface"]:
+        iface_fixtures()
     elif sys.argv[1:] == ["--fmt"]:
         save("fortran_formats", **fortran_formats())
     elif sys.argv[1:] == ["--vtu"]:

@@ -426,3 +426,42 @@ def test_forces_closed_forms():
     assert f["Fx_v"] == pytest.approx(0.1 * 0.4 * -1.0 / 0.25 * -1.0 * wall_area, rel=1e-13)
     # A_ref = x extent of the wall face centroids (0.25 .. 2.75) times the node z extent
     assert f["A_ref"] == pytest.approx(2.5 * 1.0, rel=1e-14)
+
+
+# ------------------------------------------------------ coupling interface
+from conftest import IFACE_CASES, iface_mesh, seed_interface  # noqa: E402
+
+
+@pytest.mark.parametrize("name", IFACE_CASES)
+@pytest.mark.parametrize("tso", [0, 2])
+def test_interface_per_function_bitexact(name, tso):
+    """Coupling-interface faces (an interface-named zone without a bc line):
+    velocity-Dirichlet, Zhang viscous closure, pressure-Dirichlet with the
+    tangential stress jump, Dirichlet / Neumann temperature with the LTNE
+    phase split -- every assembled value, gradient, flux and correction
+    bit-exact against the reference's own routines fed through
+    set_interface_* and the set_iface_* switches."""
+    d = load_golden(name)
+    ctrl, groups = parse_control(str(d["control"]), warn=False)
+    oc = pyoracle.OracleCase(iface_mesh(d), ctrl, groups)
+    seed_interface(oc, d)
+    out = run_funcs2(oc, d, tso)
+    for k, v in out.items():
+        assert np.array_equal(np.ravel(v), np.ravel(d[f"ts{tso}_out_{k}"])), k
+
+
+@pytest.mark.parametrize("name", IFACE_CASES)
+def test_interface_simple_bitexact(name):
+    """Serial simple_run (3 iterations) with the coupling interface: the
+    serial PPE treats interface faces as pressure-Dirichlet under the
+    interface pressure model (no pinned cell)."""
+    d = load_golden(name)
+    ctrl, groups = parse_control(str(d["control"]), warn=False)
+    oc = pyoracle.OracleCase(iface_mesh(d), ctrl, groups)
+    seed_interface(oc, d)
+    for k in ("u", "p", "T"):
+        oc.set_field(k, d[f"ts0_in_{k}"])
+    ier, it = oc.simple_run(3, mpi=False)
+    assert ier == int(d["simple_ier"]) == 0
+    for k in ("u", "p", "flux", "T", "T_s"):
+        assert np.array_equal(oc.get(k).ravel(), d[f"simple_{k}"].ravel()), k
