@@ -178,7 +178,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--bc", choices=("periodic", "twin"), default="periodic",
                     help="periodic x/z channel, or its non-periodic twin")
-    ap.add_argument("--coarse", type=int, default=256, help="aggregates of the p' coarse space")
+    ap.add_argument("--coarse", type=int, default=512, help="aggregates of the p' coarse space")
     ap.add_argument("--variants", default="mixed,fp64_two_level,fp64",
                     help="comma list of variants to run (profiling runs may select one)")
     ap.add_argument("--serve-seconds", type=float, default=3.0, help="serving run length (0 = skip)")
@@ -289,7 +289,8 @@ def main():
                            "avg_launch_us": t_k * 1e3, "alg_bytes": b_k,
                            "fp64_kernel_GBps": b_k64 / (t_k64 * 1e-3) / 1e9, "fp64_avg_launch_us": t_k64 * 1e3}
     if args.serve_seconds > 0:
-        out["serving"] = serve_bench(args.serve_rate, args.serve_seconds, args.serve_pool, args.coarse)
+        # the request server's coarse inverse runs in one workgroup (k <= 256)
+        out["serving"] = serve_bench(args.serve_rate, args.serve_seconds, args.serve_pool, min(args.coarse, 256))
     if not args.no_cpu:
         # the oracle runs the non-periodic twin (the reference has no periodic
         # boundary); p' iterations its Jacobi PCG needs for the first step come

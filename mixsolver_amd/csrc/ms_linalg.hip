@@ -55,15 +55,16 @@ __global__ void k_rsum(Rows R, const T *r, const KS *st, T *part) {
   block_partials<OpSum, 3>(v, part);
 }
 
-template <class T>
+template <class T, int C16 = -1>
 __global__ void k_pcg_spmv(int n, Mat A, const T *val, const T *p, T *q, const KS *st, T *part) {
   if (st->done) return;
   int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   T v[1] = {T(0)};
   if (i < n) {
-    T qi = row_apply(i, A, val, p);
+    const T pi = p[i];   // read before the store to q, which may alias p for the compiler
+    const T qi = row_apply<T, C16>(i, A, val, p);
     q[i] = qi;
-    v[0] = p[i] * qi;
+    v[0] = pi * qi;
   }
   block_partials<OpSum, 1>(v, part);
 }
@@ -78,14 +79,19 @@ __global__ void k_pcg_update(Rows R, const T *p, const T *q, X *x, T *r, T *z, c
   int i, a;
   T v[3] = {T(0), T(0), T(0)};
   if (R.get(i, a)) {
+    // every load before the first store: the compiler cannot tell the
+    // pointers apart, and loads after a store would wait for it
     const T alpha = (T)st->alpha;
-    x[i] = x[i] + (X)(alpha * p[i]);
-    T ri = r[i] - alpha * q[i];
+    const T pi = p[i], qi = q[i], r0 = r[i];
+    const X xi = x[i];
+    const T di = (mode == 1 || (mode == 2 && i < c0end)) ? dg[i] : T(1);
+    x[i] = xi + (X)(alpha * pi);
+    const T ri = r0 - alpha * qi;
     r[i] = ri;
     v[0] = ri * ri;
     v[2] = ri;
-    if (mode == 1) v[1] = ri * (ri / dg[i]);
-    else if (mode == 2 && i < c0end) z[i] = ri / dg[i];
+    if (mode == 1) v[1] = ri * (ri / di);
+    else if (mode == 2 && i < c0end) z[i] = ri / di;
   }
   block_partials<OpSum, 3>(v, part);
 }
@@ -200,15 +206,16 @@ __global__ void k_bi_p(int n, const T *r, const T *v, T *p, const KS *st) {
   if (i < n) p[i] = r[i] + (T)st->beta * (p[i] - (T)st->omega * v[i]);
 }
 
-template <class T>
+template <class T, int C16 = -1>
 __global__ void k_bi_spmv_dot(int n, Mat A, const T *val, const T *ph, const T *r0, T *v, const KS *st, T *part) {
   if (st->done) return;
   int i = blockIdx.x * kBlock + threadIdx.x;
   T w[1] = {T(0)};
   if (i < n) {
-    T vi = row_apply(i, A, val, ph);
+    const T r0i = r0[i];
+    const T vi = row_apply<T, C16>(i, A, val, ph);
     v[i] = vi;
-    w[0] = r0[i] * vi;
+    w[0] = r0i * vi;
   }
   block_partials<OpSum, 1>(w, part);
 }
@@ -237,16 +244,17 @@ __global__ void k_done2_to_1(KS *st) {
   if (threadIdx.x == 0 && st->done == 2) st->done = 1;
 }
 
-template <class T>
+template <class T, int C16 = -1>
 __global__ void k_bi_spmv_omega(int n, Mat A, const T *val, const T *sh, const T *s, T *t, const KS *st, T *part) {
   if (st->done) return;
   int i = blockIdx.x * kBlock + threadIdx.x;
   T w[2] = {T(0), T(0)};
   if (i < n) {
-    T ti = row_apply(i, A, val, sh);
+    const T si = s[i];
+    const T ti = row_apply<T, C16>(i, A, val, sh);
     t[i] = ti;
     w[0] = ti * ti;
-    w[1] = ti * s[i];
+    w[1] = ti * si;
   }
   block_partials<OpSum, 2>(w, part);
 }
@@ -452,7 +460,7 @@ __global__ void k_dilu_first(int r0, int r1, const T *dg, const T *r, T *z, cons
 // forward sweep of one colour [r0, r1) over nbr row blocks; the first
 // cw.nwg blocks carry the coarse solve when a coarse space rides along (they
 // start first and overlap the sweep)
-template <class T>
+template <class T, int C16 = -1>
 __global__ void k_dilu_fwd(int r0, int r1, int nbr, Mat A, const T *val, const T *dg, const T *r, T *z, int last,
                            const int *done, T *part, CoWork<T> cw) {
   if (*done) return;
@@ -466,20 +474,21 @@ __global__ void k_dilu_fwd(int r0, int r1, int nbr, Mat A, const T *val, const T
   if (i < r1) {
     const long long bi = A.off[i >> 6] + (i & 63);
     const int dpi = A.diagk[i];
-    T s = r[i];
+    const T ri = r[i];   // kept: a load after the store to z would wait for it
+    T s = ri;
     for (int k = 0; k < dpi; k++) {
       const long long q = bi + (long long)k * kSlice;
-      const int c = sizeof(T) == 4 ? col_f32(A, q, i, i >> 6) : A.col[q];
+      const int c = sizeof(T) == 4 ? col_at<C16>(A, q, i, i >> 6) : A.col[q];
       s = s - val[q] * z[c];
     }
     const T zi = s / dg[i];
     z[i] = zi;
-    if (last) v[0] = r[i] * zi;
+    if (last) v[0] = ri * zi;
   }
   if (part) block_partials<OpSum, 1>(v, part, rb);
 }
 
-template <class T>
+template <class T, int C16 = -1>
 __global__ void k_dilu_bwd(int r0, int r1, Mat A, const T *val, const T *dg, const T *r, T *z, const int *done,
                            T *part) {
   if (*done) return;
@@ -488,15 +497,16 @@ __global__ void k_dilu_bwd(int r0, int r1, Mat A, const T *val, const T *dg, con
   if (i < r1) {
     const long long bi = A.off[i >> 6] + (i & 63);
     const int dpi = A.diagk[i], len = A.rowlen[i];
+    const T ri = r[i];   // loaded before the store to z
     T s = T(0);
     for (int k = dpi + 1; k < len; k++) {
       const long long q = bi + (long long)k * kSlice;
-      const int c = sizeof(T) == 4 ? col_f32(A, q, i, i >> 6) : A.col[q];
+      const int c = sizeof(T) == 4 ? col_at<C16>(A, q, i, i >> 6) : A.col[q];
       s = s + val[q] * z[c];
     }
     const T zi = z[i] - s / dg[i];
     z[i] = zi;
-    v[0] = r[i] * zi;
+    v[0] = ri * zi;
   }
   if (part) block_partials<OpSum, 1>(v, part);
 }
@@ -622,7 +632,10 @@ double Krylov::time_spmv(const double *val, bool fp32, int reps) {
   }
   auto launch = [&]() {
     if (fp32)
-      hipLaunchKernelGGL(k_pcg_spmv<float>, G, B, 0, S, n, A, val32_.p, p32_.p, q32_.p, st_.p, part32_.p);
+      if (pat->has16)
+        hipLaunchKernelGGL((k_pcg_spmv<float, 1>), G, B, 0, S, n, A, val32_.p, p32_.p, q32_.p, st_.p, part32_.p);
+      else
+        hipLaunchKernelGGL((k_pcg_spmv<float, 0>), G, B, 0, S, n, A, val32_.p, p32_.p, q32_.p, st_.p, part32_.p);
     else
       hipLaunchKernelGGL(k_pcg_spmv<double>, G, B, 0, S, n, A, val, p_.p, q_.p, st_.p, part_.p);
   };
@@ -701,6 +714,13 @@ void Krylov::setup_pc32(PcKind pc, const double *val) {
   MS_CHECK_LAUNCH();
 }
 
+// the column format of a SELL pattern as a template argument of the FP32
+// kernels (the FP64 kernels always read 32-bit columns)
+template <class T>
+static int c16_of(const SellPattern *pat) {
+  return sizeof(T) == 4 && pat->has16 ? 1 : 0;
+}
+
 template <class T>
 void Krylov::apply_pc(PcKind pc, const T *val, const T *fac, const T *dg, const T *in, T *out, const int *done) {
   Mat A = matof(pat);
@@ -715,13 +735,21 @@ void Krylov::apply_pc(PcKind pc, const T *val, const T *fac, const T *dg, const 
     for (int c = 1; c < nc; c++)
       if (cp[c + 1] > cp[c]) {
         const int gb = nblocks(cp[c + 1] - cp[c]);
-        hipLaunchKernelGGL(k_dilu_fwd<T>, dim3(gb), dim3(kBlock), 0, stream(), cp[c], cp[c + 1], gb, A, val, dg, in,
-                           out, (int)(c == nc - 1), done, (T *)nullptr, CoWork<T>{});
+        if (c16_of<T>(pat))
+          hipLaunchKernelGGL((k_dilu_fwd<T, 1>), dim3(gb), dim3(kBlock), 0, stream(), cp[c], cp[c + 1], gb, A, val, dg,
+                             in, out, (int)(c == nc - 1), done, (T *)nullptr, CoWork<T>{});
+        else
+          hipLaunchKernelGGL((k_dilu_fwd<T, 0>), dim3(gb), dim3(kBlock), 0, stream(), cp[c], cp[c + 1], gb, A, val, dg,
+                             in, out, (int)(c == nc - 1), done, (T *)nullptr, CoWork<T>{});
       }
     for (int c = nc - 2; c >= 0; c--)
       if (cp[c + 1] > cp[c])
-        hipLaunchKernelGGL(k_dilu_bwd<T>, dim3(nblocks(cp[c + 1] - cp[c])), dim3(kBlock), 0, stream(), cp[c],
-                           cp[c + 1], A, val, dg, in, out, done, (T *)nullptr);
+        if (c16_of<T>(pat))
+          hipLaunchKernelGGL((k_dilu_bwd<T, 1>), dim3(nblocks(cp[c + 1] - cp[c])), dim3(kBlock), 0, stream(), cp[c],
+                             cp[c + 1], A, val, dg, in, out, done, (T *)nullptr);
+        else
+          hipLaunchKernelGGL((k_dilu_bwd<T, 0>), dim3(nblocks(cp[c + 1] - cp[c])), dim3(kBlock), 0, stream(), cp[c],
+                             cp[c + 1], A, val, dg, in, out, done, (T *)nullptr);
   } else {
     const int ic0 = pc == PC_NATURAL_IC0;
     for (const auto &g : pat->seg_f) {
@@ -769,16 +797,24 @@ static int dilu_sweeps(const SellPattern *pat, Mat A, const T *val, const T *dg,
       const int gb = nblocks(cp[c + 1] - cp[c]);
       CoWork<T> cwl = cw;
       if (!first) cwl.nwg = 0;
-      hipLaunchKernelGGL(k_dilu_fwd<T>, dim3(gb + cwl.nwg), dim3(kBlock), 0, stream(), cp[c], cp[c + 1], gb, A, val,
-                         dg, r, z, (int)(c == ncol - 1), done, sec, cwl);
+      if (c16_of<T>(pat))
+        hipLaunchKernelGGL((k_dilu_fwd<T, 1>), dim3(gb + cwl.nwg), dim3(kBlock), 0, stream(), cp[c], cp[c + 1], gb, A,
+                           val, dg, r, z, (int)(c == ncol - 1), done, sec, cwl);
+      else
+        hipLaunchKernelGGL((k_dilu_fwd<T, 0>), dim3(gb + cwl.nwg), dim3(kBlock), 0, stream(), cp[c], cp[c + 1], gb, A,
+                           val, dg, r, z, (int)(c == ncol - 1), done, sec, cwl);
       first = false;
       sec += gb;
     }
   for (int c = ncol - 2; c >= 0; c--)
     if (cp[c + 1] > cp[c]) {
       const int gb = nblocks(cp[c + 1] - cp[c]);
-      hipLaunchKernelGGL(k_dilu_bwd<T>, dim3(gb), dim3(kBlock), 0, stream(), cp[c], cp[c + 1], A, val, dg, r, z, done,
-                         sec);
+      if (c16_of<T>(pat))
+        hipLaunchKernelGGL((k_dilu_bwd<T, 1>), dim3(gb), dim3(kBlock), 0, stream(), cp[c], cp[c + 1], A, val, dg, r, z,
+                           done, sec);
+      else
+        hipLaunchKernelGGL((k_dilu_bwd<T, 0>), dim3(gb), dim3(kBlock), 0, stream(), cp[c], cp[c + 1], A, val, dg, r, z,
+                           done, sec);
       sec += gb;
     }
   return total + (first ? 0 : cw.nwg);
@@ -1038,14 +1074,22 @@ SolveResult Krylov::bicg32(PcKind pc, int maxit) {
       finish<OpSum, 1>(nblk_, part, FinBiRho{st});
       hipLaunchKernelGGL(k_bi_p<float>, G, B, 0, S, n, r32_.p, v, p32_.p, st);
       apply_pc<float>(pc, val32_.p, fac32_.p, dg32_.p, p32_.p, ph32_.p, done);
-      hipLaunchKernelGGL(k_bi_spmv_dot<float>, G, B, 0, S, n, A, val32_.p, ph32_.p, r032_.p, v, st, part);
+      if (pat->has16)
+        hipLaunchKernelGGL((k_bi_spmv_dot<float, 1>), G, B, 0, S, n, A, val32_.p, ph32_.p, r032_.p, v, st, part);
+      else
+        hipLaunchKernelGGL((k_bi_spmv_dot<float, 0>), G, B, 0, S, n, A, val32_.p, ph32_.p, r032_.p, v, st, part);
       finish<OpSum, 1>(nblk_, part, FinBiAlpha{st});
       hipLaunchKernelGGL(k_bi_s<float>, G, B, 0, S, n, r32_.p, v, s32_.p, st, part);
       finish<OpSum, 1>(nblk_, part, FinBiS{st});
       hipLaunchKernelGGL(k_bi_early<float>, G, B, 0, S, n, ph32_.p, x32_.p, st);
       hipLaunchKernelGGL(k_done2_to_1, dim3(1), dim3(64), 0, S, st);
       apply_pc<float>(pc, val32_.p, fac32_.p, dg32_.p, s32_.p, sh32_.p, done);
-      hipLaunchKernelGGL(k_bi_spmv_omega<float>, G, B, 0, S, n, A, val32_.p, sh32_.p, s32_.p, t32_.p, st, part);
+      if (pat->has16)
+        hipLaunchKernelGGL((k_bi_spmv_omega<float, 1>), G, B, 0, S, n, A, val32_.p, sh32_.p, s32_.p, t32_.p, st,
+                           part);
+      else
+        hipLaunchKernelGGL((k_bi_spmv_omega<float, 0>), G, B, 0, S, n, A, val32_.p, sh32_.p, s32_.p, t32_.p, st,
+                           part);
       finish<OpSum, 2>(nblk_, part, FinBiOmega{st});
       hipLaunchKernelGGL(k_bi_xr<float>, G, B, 0, S, n, ph32_.p, sh32_.p, s32_.p, t32_.p, x32_.p, r32_.p, st, part);
       finish<OpSum, 1>(nblk_, part, FinBiXR{st});
@@ -1206,7 +1250,10 @@ SolveResult Krylov::mixed(const double *val, const double *b, double *x, double 
   constexpr int kChunk = 16;   // reliable updates are applied at most this many iterations late
   auto chunk = [&](int m) {
     for (int it = 0; it < m; it++) {
-      hipLaunchKernelGGL(k_pcg_spmv<float>, G, B, 0, S, n, A, v32, p, q, st, part32);
+      if (pat->has16)
+        hipLaunchKernelGGL((k_pcg_spmv<float, 1>), G, B, 0, S, n, A, v32, p, q, st, part32);
+      else
+        hipLaunchKernelGGL((k_pcg_spmv<float, 0>), G, B, 0, S, n, A, v32, p, q, st, part32);
       finish<OpSum, 1>(nblk_, part32, FinPap{st});
       hipLaunchKernelGGL((k_pcg_update<float, float>), GR, B, 0, S, RW, p, q, y, r, z, dg, mode, c0end, st,
                          part32);
