@@ -167,6 +167,12 @@ int ms_set_field(ms_system *s, const char *name, const double *host, long long n
 int ms_get_field(ms_system *s, const char *name, double *host, long long n);
 int ms_set_ts_order(ms_system *s, int ts_order);
 int ms_get_ts_order(ms_system *s, int *ts_order);   /* fld%ts_order of the system */
+/* nan_scan (mod_uns_simple.f90:2375-2426) on the current fields: the first
+ * non-finite value in cell order (caller's numbering), u_x u_y u_z p T T_s
+ * within a cell: *field 0..5, *cell 1-based, *val, *kind 1 NaN / 2 Inf,
+ * *ifin 1 when found.  The drivers return 60 (NAN_IER) when a step leaves a
+ * non-finite value; this names it for the report. */
+int ms_nan_scan(ms_system *s, int *field, int *cell, double *val, int *kind, int *ifin);
 
 /* ---- coupling interface (mode = coupled callers) ------------------------
  * Boundary faces of face zones whose condition or user name contains

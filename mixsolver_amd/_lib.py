@@ -58,6 +58,7 @@ def lib():
         "ms_compute_forces": (C.c_int, [_V, _D]),
         "ms_set_graphs": (C.c_int, [C.c_int]),
         "ms_get_ts_order": (C.c_int, [_V, _I]),
+        "ms_nan_scan": (C.c_int, [_V, _I, _I, _D, _I, _I]),
         "ms_set_interface": (C.c_int, [_V, C.c_char_p, _D, C.c_longlong]),
         "ms_set_interface_modes": (C.c_int, [_V, C.c_int, C.c_int, C.c_int, C.c_int]),
         "ms_dump_write": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_char_p, C.c_char_p, _D, _D, _D, _D, _D, _D, _D]),

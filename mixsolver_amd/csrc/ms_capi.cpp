@@ -341,6 +341,31 @@ int ms_set_interface_modes(ms_system *h, int p_dir, int slip, int zhang, int T_l
   });
 }
 
+int ms_nan_scan(ms_system *h, int *field, int *cell, double *val, int *kind, int *ifin) {
+  return guarded([&] {
+    if (!field || !cell || !val || !kind || !ifin) fail(MS_EINVAL, "null output");
+    System &s = sys(h);
+    std::vector<double> d, u(3 * (size_t)s.nc), p(s.nc), T(s.nc), Ts(s.nc);
+    download(s.u, d, 3 * (size_t)s.nc);
+    from_device_order(s, d, 3, u.data());
+    download(s.p, d, s.nc);
+    from_device_order(s, d, 1, p.data());
+    download(s.T, d, s.nc);
+    from_device_order(s, d, 1, T.data());
+    download(s.T_s, d, s.nc);
+    from_device_order(s, d, 1, Ts.data());
+    *field = -1, *cell = 0, *val = 0.0, *kind = 0, *ifin = 0;
+    for (int c = 0; c < s.nc && !*ifin; c++) {
+      const double v[6] = {u[3 * (size_t)c], u[3 * (size_t)c + 1], u[3 * (size_t)c + 2], p[c], T[c], Ts[c]};
+      for (int k = 0; k < 6; k++)
+        if (!std::isfinite(v[k])) {
+          *field = k, *cell = c + 1, *val = v[k], *kind = std::isnan(v[k]) ? 1 : 2, *ifin = 1;
+          break;
+        }
+    }
+  });
+}
+
 int ms_get_ts_order(ms_system *h, int *t) {
   return guarded([&] {
     if (!t) fail(MS_EINVAL, "null output");

@@ -869,7 +869,8 @@ __global__ void k_diagnostics(DMesh m, DFields F, const double *uref, const doub
     for (int d = 0; d < 3; d++) v[1] = dmax(v[1], fabs(u[d] - uref[3 * (size_t)r + d]));
     v[2] = fabs(F.T[r] - Tref[r]);
     v[3] = dnorm3(u);
-    bool bad = !isfinite(u[0]) || !isfinite(u[1]) || !isfinite(u[2]) || !isfinite(F.p[r]) || !isfinite(F.T[r]);
+    bool bad = !isfinite(u[0]) || !isfinite(u[1]) || !isfinite(u[2]) || !isfinite(F.p[r]) || !isfinite(F.T[r]) ||
+               !isfinite(F.T_s[r]);   // nan_scan covers T_s (mod_uns_simple.f90:2402-2404)
     v[4] = bad ? 1.0 : 0.0;
   }
   block_partials<OpMax, 5>(v, part);

@@ -79,6 +79,16 @@ class Case:
         """The coupling driver's interface switches (set_iface_*)."""
         check(_lib.lib().ms_set_interface_modes(self.h, int(p_dir), int(slip), int(zhang), int(T_ltne)))
 
+    def nan_scan(self):
+        """(ifin, field name, cell 1-based, value, 'NaN' | 'Inf') of
+        nan_scan over the current fields."""
+        f, c, k, i = C.c_int(0), C.c_int(0), C.c_int(0), C.c_int(0)
+        v = C.c_double(0.0)
+        check(_lib.lib().ms_nan_scan(self.h, C.byref(f), C.byref(c), C.byref(v), C.byref(k), C.byref(i)))
+        names = ("u_x", "u_y", "u_z", "p", "T", "T_s")
+        return (i.value, names[f.value] if i.value else "(none)", c.value, v.value,
+                {1: "NaN", 2: "Inf"}.get(k.value, "(none)"))
+
     def get_ts_order(self):
         t = C.c_int(0)
         check(_lib.lib().ms_get_ts_order(self.h, C.byref(t)))

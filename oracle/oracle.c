@@ -1665,7 +1665,7 @@ static int solid_temperature_solve(or_case *cs, double *val, double *rhs, double
 static int nonfinite(const or_case *cs) {
   for (int c = 0; c < cs->nc; c++) {
     for (int d = 0; d < 3; d++) if (!isfinite(cs->u[3 * c + d])) return 1;
-    if (!isfinite(cs->p[c]) || !isfinite(cs->T[c])) return 1;
+    if (!isfinite(cs->p[c]) || !isfinite(cs->T[c]) || !isfinite(cs->T_s[c])) return 1;
   }
   return 0;
 }
@@ -2105,4 +2105,21 @@ void or_set_interface_modes(or_case *cs, int p_dir, int slip, int zhang, int T_l
   cs->iface_slip = slip != 0;
   cs->iface_zhang = zhang != 0;
   cs->iface_T_ltne = T_ltne != 0;
+}
+
+
+/* nan_scan (mod_uns_simple.f90:2375-2426): the first non-finite value in
+ * cell order, u_x u_y u_z p T T_s within a cell.  Returns ifin (0 / 1);
+ * field = 0..5 in that order, cell 1-based, kind 1 NaN / 2 Inf. */
+int or_nan_scan(or_case *cs, int *field, int *cell, double *val, int *kind) {
+  *field = -1; *cell = 0; *val = 0.0; *kind = 0;
+  for (int c = 0; c < cs->nc; c++) {
+    const double v[6] = {cs->u[3 * c], cs->u[3 * c + 1], cs->u[3 * c + 2], cs->p[c], cs->T[c], cs->T_s[c]};
+    for (int k = 0; k < 6; k++)
+      if (!isfinite(v[k])) {
+        *field = k; *cell = c + 1; *val = v[k]; *kind = isnan(v[k]) ? 1 : 2;
+        return 1;
+      }
+  }
+  return 0;
 }

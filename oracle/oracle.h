@@ -125,6 +125,9 @@ void or_timing(or_case *, double *out3);
 void or_compute_forces(or_case *, double *out14);
 /* coupling interface data and switches (mod_uns_bc.f90:730-869) */
 int or_set_interface(or_case *, const char *what, const double *values);
+/* nan_scan: first non-finite value (field 0..5 = u_x u_y u_z p T T_s, cell
+ * 1-based, kind 1 NaN / 2 Inf); returns ifin */
+int or_nan_scan(or_case *, int *field, int *cell, double *val, int *kind);
 void or_set_interface_modes(or_case *, int p_dir, int slip, int zhang, int T_ltne);
 int or_cp_distribution(or_case *, double *rows6);
 void or_separation(or_case *, double *out5);

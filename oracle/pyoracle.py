@@ -61,6 +61,7 @@ def lib():
         L.or_csr_matvec.argtypes = [C.c_int, _I, _I, _D, _D, _D]
         L.or_compute_forces.argtypes = [C.c_void_p, _D]
         L.or_set_interface.argtypes = [C.c_void_p, C.c_char_p, _D]
+        L.or_nan_scan.argtypes = [C.c_void_p, _I, _I, _D, _I]
         L.or_set_interface_modes.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]
         L.or_cp_distribution.argtypes = [C.c_void_p, _D]
         L.or_separation.argtypes = [C.c_void_p, _D]
@@ -228,6 +229,15 @@ class OracleCase:
 
     def set_interface_modes(self, p_dir=False, slip=False, zhang=False, T_ltne=False):
         lib().or_set_interface_modes(self.h, int(p_dir), int(slip), int(zhang), int(T_ltne))
+
+    def nan_scan(self):
+        """(ifin, field name, cell 1-based, value, 'NaN' | 'Inf') of nan_scan."""
+        f, c, k = C.c_int(0), C.c_int(0), C.c_int(0)
+        v = C.c_double(0.0)
+        ifin = lib().or_nan_scan(self.h, C.byref(f), C.byref(c), C.byref(v), C.byref(k))
+        names = ("u_x", "u_y", "u_z", "p", "T", "T_s")
+        return (ifin, names[f.value] if ifin else "(none)", c.value, v.value,
+                {1: "NaN", 2: "Inf"}.get(k.value, "(none)"))
 
     def compute_forces(self):
         """compute_forces: dict of Fx Fy Fz CD CL u_inf A_ref p_inf and the

@@ -81,3 +81,33 @@ def run_funcs2(case, d, tso):
 @pytest.fixture
 def golden():
     return load_golden
+
+
+def nan_guard_groups(case, nc):
+    """The six groups of the reference's NaN-guard unit test
+    (src/unstructured/test_nan_guard.f90:72-121) on a Case or OracleCase:
+    kc1 = nc/2 + 1, kc2 = max(1, nc/4) (1-based)."""
+    kc1, kc2 = nc // 2 + 1, max(1, nc // 4)
+    if kc1 == kc2:
+        kc1 += 1
+    fields = {k: np.zeros(3 * nc if k == "u" else nc) for k in ("u", "p", "T", "T_s")}
+
+    def put(kv=None):
+        for k, v in fields.items():
+            case.set_field(k, v)
+        for (k, i), v in (kv or {}).items():
+            a = fields[k].copy()
+            a[i] = v
+            case.set_field(k, a)
+        return case.nan_scan()
+
+    assert put()[0] == 0                                                    # group 1: clean
+    ifin, f, c, v, kind = put({("u", 3 * (kc1 - 1) + 1): np.nan})        # group 2: NaN in u_y
+    assert (ifin, f, c, kind) == (1, "u_y", kc1, "NaN") and np.isnan(v)
+    ifin, f, c, v, kind = put({("p", kc2 - 1): np.inf})                   # group 3: Inf in p
+    assert (ifin, f, c, kind) == (1, "p", kc2, "Inf") and v == np.inf
+    ifin, f, c, v, kind = put({("T", kc2 - 1): np.nan, ("u", 3 * (kc1 - 1)): np.inf})
+    assert (ifin, f, c) == (1, "T", kc2)                                    # group 4: lowest cell first
+    ifin, f, c, v, kind = put({("T_s", kc1 - 1): np.nan})                 # group 5: T_s scanned
+    assert (ifin, f, c) == (1, "T_s", kc1)
+    assert put()[0] == 0                                                    # group 6: stateless

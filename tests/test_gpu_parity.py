@@ -362,3 +362,20 @@ def test_restart_through_field_dump(name, drv, tmp_path):
     assert run(a) == 0 and run(b) == 0
     for k in ("u", "p", "T", "flux", "u_old", "u_old_old"):
         assert np.array_equal(a.get(k), b.get(k)), k
+
+
+@pytest.mark.parametrize("reorder", [None, "dilu"])
+def test_nan_guard_groups_device(reorder):
+    """ms_nan_scan on the device fields (caller's numbering under the
+    colour-blocked renumbering): the reference NaN-guard unit test's groups;
+    and simple_run returns 60 when only T_s is non-finite."""
+    from conftest import nan_guard_groups
+    d = load_golden("cavity16_simple_jacobi")
+    over = dict(gpu_ppe_precond=2, gpu_mom_precond=2) if reorder else {}
+    cs = make(d, **over)
+    nan_guard_groups(cs, cs.nc)
+    cs2 = make(d, **over)
+    ts = np.zeros(cs2.nc)
+    ts[5] = np.nan
+    cs2.set_field("T_s", ts)
+    assert cs2.simple_run(2)[0] == 60

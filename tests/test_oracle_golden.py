@@ -465,3 +465,23 @@ def test_interface_simple_bitexact(name):
     assert ier == int(d["simple_ier"]) == 0
     for k in ("u", "p", "flux", "T", "T_s"):
         assert np.array_equal(oc.get(k).ravel(), d[f"simple_{k}"].ravel()), k
+
+
+# ------------------------------------------------------------ NaN guard
+from conftest import nan_guard_groups  # noqa: E402
+
+
+def test_nan_guard_groups_oracle():
+    """nan_scan restated: the reference unit test's groups (clean, NaN / Inf
+    detection with field, cell and kind, first offender by cell index, T_s
+    scanned, stateless), and the drivers' NAN_IER = 60 when T_s alone is
+    non-finite (the guard scans T_s too)."""
+    d = load_golden("cavity16_simple_jacobi")
+    ctrl, groups = parse_control(str(d["control"]), warn=False)
+    oc = pyoracle.OracleCase(mesh_from(d), ctrl, groups)
+    nan_guard_groups(oc, oc.nc)
+    oc2 = pyoracle.OracleCase(mesh_from(d), ctrl, groups)
+    ts = np.zeros(oc2.nc)
+    ts[5] = np.nan
+    oc2.set_field("T_s", ts)
+    assert oc2.simple_run(2, mpi=False)[0] == 60
