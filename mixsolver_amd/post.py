@@ -40,6 +40,16 @@ check_heat_conduction.py:50-90): a bar of length L between fixed ends
 annulus between fixed radii (logarithmic) and a bar with a uniform source
 q on [a, b) (two lines joined by a parabola; slope jumps q (x - a)/k inside,
 ends fixed).
+
+``channel_check`` restates the reference's check of its low-speed planar
+channel (cluster_cases/channel_lowspeed/plot_channel.py: layer_weights()
+:60-72, main() :74-139): per x layer of cell centres the bulk velocity and
+mean pressure are width-weighted sums over the rows (midpoint edges, end
+rows clipped to the walls), the developed profile at x = 299.5 is compared
+with u = 6 U eta (1 - eta), U = Re_H mu / (rho H), the entrance length is the
+first layer whose centreline u is within 1 % of 1.5 U, and dp/dx is the
+least-squares slope of the layer pressure over x in [200, 295]
+(exact -12 mu U / H^2).
 """
 from __future__ import annotations
 
@@ -194,3 +204,53 @@ def cylinder_check(xc, u, p, r_cyl=0.5, u_inf=1.0, rho=1.0, nbins=80, cp_ref=Non
         out["dcp_mean"] = float(np.abs(d).mean())
         out["dcp_max"] = float(np.abs(d).max())
     return out
+
+
+def _layer_widths(y, height):
+    """Control widths of cell-centre rows of one x layer: the midpoints
+    between neighbouring centres bound each row, the outermost rows extend
+    half their neighbour spacing and stop at the walls y = 0 and y = H."""
+    o = np.argsort(y)
+    ys = y[o]
+    e = np.concatenate(([max(ys[0] - 0.5 * (ys[1] - ys[0]), 0.0)], 0.5 * (ys[1:] + ys[:-1]),
+                        [min(ys[-1] + 0.5 * (ys[-1] - ys[-2]), height)]))
+    w = np.empty_like(ys)
+    w[o] = np.diff(e)
+    return w
+
+
+def channel_check(xc, u, p, height=10.0, re_h=100.0, rho=1.177, mu=1.846e-5, x_fit=(200.0, 295.0)):
+    """Regression indicators of a planar Poiseuille channel (dict)."""
+    xc = np.asarray(xc).reshape(-1, 3)
+    u = np.asarray(u).reshape(-1, 3)
+    p = np.asarray(p)
+    x, y, ux = xc[:, 0], xc[:, 1], u[:, 0]
+    ub = re_h * mu / (rho * height)
+    xr = np.round(x, 9)
+    lv = np.unique(xr)
+    bulk, pm, ucl = np.empty(len(lv)), np.empty(len(lv)), np.empty(len(lv))
+    rows = []
+    for k, x0 in enumerate(lv):
+        m = np.nonzero(np.abs(x - x0) < 1e-6)[0]
+        w = _layer_widths(y[m], height)
+        bulk[k] = np.dot(ux[m], w) / height
+        pm[k] = np.dot(p[m], w) / height
+        ucl[k] = ux[m][np.argmin(np.abs(y[m] - 0.5 * height))]
+        rows.append(m)
+    m = rows[int(np.argmin(np.abs(lv - 299.5)))]
+    eta = y[m] / height
+    err = np.abs(ux[m] - 6.0 * ub * eta * (1.0 - eta)) / ub
+    band = np.abs(ucl - 1.5 * ub) / ub < 0.01
+    f = (lv >= x_fit[0]) & (lv <= x_fit[1])
+    slope = np.polyfit(lv[f], pm[f], 1)[0]
+    mag = np.sqrt((u * u).sum(axis=1))
+    jm = int(np.argmax(mag))
+    return dict(u_bulk_target=ub, dpdx_exact=-12.0 * mu * ub / height ** 2, n_layers=len(lv),
+                u_bulk_in=float(bulk[0]), u_bulk_50=float(bulk[50]), u_bulk_out=float(bulk[-1]),
+                mass_drift=float((bulk[-1] - bulk[0]) / bulk[0]),
+                profile_err_max=float(err.max()), profile_err_l1=float(err.mean()),
+                umax_ratio_out=float(ux[m].max() / ub),
+                entrance_length=float(lv[int(np.argmax(band))]) if band.any() else float("nan"),
+                dpdx_fit=float(slope), p_rise=float(pm[0] - pm[-1]),
+                p_min=float(p.min()), p_max=float(p.max()),
+                umag_max=float(mag[jm]), umag_max_at=xc[jm].tolist())

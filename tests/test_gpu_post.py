@@ -132,3 +132,40 @@ def test_write_vtu_device_fields(binary, tmp_path):
                        int)[3::4] - m.nnodes
         assert len(off) == len(vals) == len(cen)
         assert np.array_equal(vals, d["ref_p"][cen])
+
+
+def test_channel_lowspeed_known_answer():
+    """The reference's low-speed planar channel (Re_H = 100, mass-flow inlet,
+    outflow, SIMPLE to an absolute du of 1e-10) on the device fast path
+    (multicolour DILU, FP32 inner iterations with FP64 refinement, coarse
+    space), rebuilt from the tensor grid the shipped mesh lies on.  Measured,
+    the device stops after the same 773 iterations and its indicators lie
+    within rel 1e-9 (bulk velocities), 1e-7 U (profile errors, u_max / U)
+    and rel 3e-7 (dp/dx, pressure) of the reference run's.  Bars: iterations
+    within 2 %, rel 1e-6 on the bulk velocities, 1e-5 U on the profile
+    errors, u_max / U and the mass drift, rel 1e-4 on dp/dx, the pressure
+    rise and p_min, and the same entrance-length layer (the 1 % band is
+    crossed with a 1.2e-4 U margin).
+    """
+    from mixsolver_amd import meshgen
+    from mixsolver_amd.post import channel_check
+    d = load_golden("channel_lowspeed_known_answer")
+    m = meshgen.hex_box(np.arange(301.0), d["ys"], np.array([0.0, 10.0]),
+                        zone_ids=dict(xmin=5, xmax=6, ymin=7, ymax=7, zmin=4, zmax=4))
+    ctrl, groups = parse_control(str(d["control"]), warn=False)
+    for k, v in dict(gpu_ppe_precond=2, gpu_mom_precond=2, gpu_precision=1, gpu_coarse=64).items():
+        setattr(ctrl, k, v)
+    cs = Case(m, ctrl, groups)
+    ier, it = cs.simple_run(8000)
+    c = channel_check(cs.get("xc").reshape(-1, 3), cs.get("u").reshape(-1, 3), cs.get("p"))
+    print("channel fast path:", it, c, flush=True)
+    assert ier == 0
+    assert abs(it - int(d["ref_iterations"])) <= 0.02 * int(d["ref_iterations"])
+    for k in ("u_bulk_in", "u_bulk_50", "u_bulk_out"):
+        assert c[k] == pytest.approx(float(d["ref_" + k]), rel=1e-6), k
+    for k in ("profile_err_max", "profile_err_l1", "umax_ratio_out", "mass_drift"):
+        assert c[k] == pytest.approx(float(d["ref_" + k]), abs=1e-5), k
+    for k in ("dpdx_fit", "p_rise", "p_min"):
+        assert c[k] == pytest.approx(float(d["ref_" + k]), rel=1e-4), k
+    assert c["entrance_length"] == float(d["ref_entrance_length"])
+    assert c["n_layers"] == 300

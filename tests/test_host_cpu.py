@@ -460,3 +460,49 @@ def test_dump_round_trip_and_errors(tmp_path):
     assert read_dump(path, n + 1)[0] == 4
     (tmp_path / "t.txt").write_text("\n".join(lines[: iu + n + 3]))
     assert read_dump(tmp_path / "t.txt", n)[0] == 5
+
+
+def _chan_cells(ys, nx=300):
+    xs = np.arange(nx + 1.0)
+    yc = 0.5 * (ys[1:] + ys[:-1])
+    X, Y = np.meshgrid(0.5 * (xs[1:] + xs[:-1]), yc, indexing="ij")
+    return np.stack([X.ravel(), Y.ravel(), np.full(X.size, 5.0)], 1)
+
+
+def test_channel_check_closed_forms():
+    """The channel indicators on fields with known answers, on wall-clustered
+    rows (spacing grows away from both walls, so the end rows' widths stop
+    at the walls): uniform u gives U_bulk = u exactly in every layer; the
+    Poiseuille profile sampled at the centres has zero profile error and
+    u_max / U = max 6 eta (1 - eta); a pressure linear in x gives its slope
+    and the rise -slope * 299; a centreline ramp crosses the 1 % band at the
+    layer it was built to; a dropped outer row or unclipped end width
+    changes the uniform-u bulk."""
+    from mixsolver_amd.post import channel_check
+    s = 0.5 * (1.0 - np.cos(np.pi * np.linspace(0.0, 1.0, 41)))
+    ys = 10.0 * s
+    xc = _chan_cells(ys)
+    n = len(xc)
+    ub = 100.0 * 1.846e-5 / (1.177 * 10.0)
+    u = np.zeros((n, 3))
+    u[:, 0] = 0.8 * ub
+    c = channel_check(xc, u, np.zeros(n))
+    assert c["u_bulk_in"] == pytest.approx(0.8 * ub, rel=1e-14)
+    assert c["u_bulk_out"] == pytest.approx(0.8 * ub, rel=1e-14)
+    assert c["mass_drift"] == pytest.approx(0.0, abs=1e-13)
+    assert c["n_layers"] == 300
+    eta = xc[:, 1] / 10.0
+    u[:, 0] = 6.0 * ub * eta * (1.0 - eta)
+    p = 3.0e-9 - 2.5e-10 * xc[:, 0]
+    c = channel_check(xc, u, p)
+    assert c["profile_err_max"] < 1e-14 and c["profile_err_l1"] < 1e-14
+    assert c["umax_ratio_out"] == pytest.approx((6 * eta * (1 - eta)).max(), rel=1e-14)
+    assert c["dpdx_fit"] == pytest.approx(-2.5e-10, rel=1e-10)
+    assert c["p_rise"] == pytest.approx(2.5e-10 * 299.0, rel=1e-12)
+    assert c["dpdx_exact"] == pytest.approx(-12 * 1.846e-5 * ub / 100.0, rel=1e-15)
+    assert 0.99 < c["u_bulk_in"] / ub < 1.01
+    # centreline ramp: reaches the 1 % band at x = 40.5
+    mid = np.abs(xc[:, 1] - 5.0) == np.abs(xc[:, 1] - 5.0).min()
+    u[:, 0] = 0.0
+    u[mid, 0] = 1.5 * ub * np.minimum(xc[mid, 0] / 40.5, 1.0) * 0.995 + 0.0075 * ub * (xc[mid, 0] >= 40.5)
+    assert channel_check(xc, u, p)["entrance_length"] == 40.5

@@ -485,3 +485,36 @@ def test_nan_guard_groups_oracle():
     ts[5] = np.nan
     oc2.set_field("T_s", ts)
     assert oc2.simple_run(2, mpi=False)[0] == 60
+
+
+def test_channel_reference_run_matches_readme():
+    """The reference's low-speed channel re-run by the fixture generator
+    (its own SIMPLE, IC0 PCG) and measured by the restated channel check
+    gives the README's table to its printed digits, converges after the
+    printed 773 iterations and prints the same first-iteration line; the
+    rebuilt tensor grid holds 101 y levels clustered at both walls."""
+    d = load_golden("channel_lowspeed_known_answer")
+    assert int(d["ref_ier"]) == 0
+    assert int(d["ref_iterations"]) == int(d["readme_iterations"])
+    assert np.array_equal(d["ref_it1"], d["readme_it1"]) and int(d["ref_it1_lin"]) == int(d["readme_it1_lin"])
+
+    def sig(a, n):
+        return float(f"{float(a):.{n - 1}e}")
+
+    for k, n in (("u_bulk_50", 7), ("u_bulk_out", 7), ("umag_max", 7), ("dpdx_fit", 7), ("p_min", 7),
+                 ("p_rise", 5)):
+        assert sig(d["ref_" + k], n) == float(d["readme_" + k]), k
+    assert round(float(d["ref_umax_ratio_out"]), 4) == float(d["readme_umax_ratio_out"])
+    for k in ("profile_err_max", "profile_err_l1", "mass_drift"):
+        assert round(100 * float(d["ref_" + k]), 3) == pytest.approx(100 * float(d["readme_" + k]), abs=1e-9), k
+    assert float(d["ref_entrance_length"]) == float(d["readme_entrance_length"])
+    assert np.allclose(np.round(d["ref_umag_max_at"], 3), d["readme_umag_max_at"], atol=1e-9)
+    ys = d["ys"]
+    assert len(ys) == 101 and ys[0] == 0.0 and ys[-1] == 10.0
+    assert np.diff(ys).min() == pytest.approx(0.01, rel=1e-9)
+    assert np.allclose(ys + ys[::-1], 10.0, atol=1e-12)
+    # the outlet profile is the developed Poiseuille profile within the printed max error
+    eta = d["ref_out_y"] / 10.0
+    ub = float(d["ref_u_bulk_target"])
+    assert np.abs(d["ref_out_u"] - 6 * ub * eta * (1 - eta)).max() / ub == pytest.approx(
+        float(d["ref_profile_err_max"]), rel=1e-12)
