@@ -90,6 +90,24 @@ __device__ __forceinline__ T row_dot(int r, const Mat &A, const T *val, const T 
 // algebraically equal to A p, but constants are annihilated exactly, so the
 // near-null space of the pressure-correction Laplacian survives the rounding
 // of its coefficients to FP32.
+// Slice of a fixed width W: every column is loaded first, then every value,
+// so the gathers of x start as soon as the first column arrives while the
+// value loads are in flight; accumulation runs k = 0..W-1 as in the loop.
+template <class T, int C16, int W>
+__device__ __forceinline__ T row_apply_w(int r, int s, long long base, const Mat &A, const T *val, const T *x,
+                                         T xi) {
+  int c[W];
+  T a[W];
+#pragma unroll
+  for (int k = 0; k < W; k++) c[k] = col_at<C16>(A, base + (long long)k * kSlice, r, s);
+#pragma unroll
+  for (int k = 0; k < W; k++) a[k] = val[base + (long long)k * kSlice];
+  T acc = T(0);
+#pragma unroll
+  for (int k = 0; k < W; k++) acc = acc + a[k] * (c[k] == r ? xi : x[c[k]] - xi);
+  return acc;
+}
+
 template <class T, int C16 = -1>
 __device__ __forceinline__ T row_apply(int r, const Mat &A, const T *val, const T *x) {
   if constexpr (sizeof(T) == 8) {
@@ -98,12 +116,17 @@ __device__ __forceinline__ T row_apply(int r, const Mat &A, const T *val, const 
     // every slot of the slice width contributes: a_k (x_col - x_i) off the
     // diagonal, s_i x_i in the diagonal slot (col = r, holding the row sum)
     // and 0 in padding (col = r, a = 0); the loop runs the slice width,
-    // uniform across the wavefront, and needs no diagonal position
+    // uniform across the wavefront, and needs no diagonal position.  The
+    // widths of tetrahedral, prismatic and hexahedral rows (5, 6, 7) take
+    // fully unrolled bodies.
     const int s = r >> 6;
     const long long o0 = A.off[s];
     const int w = (int)((A.off[s + 1] - o0) >> 6);
     const long long base = o0 + (r & 63);
     const T xi = x[r];
+    if (w == 7) return row_apply_w<T, C16, 7>(r, s, base, A, val, x, xi);
+    if (w == 6) return row_apply_w<T, C16, 6>(r, s, base, A, val, x, xi);
+    if (w == 5) return row_apply_w<T, C16, 5>(r, s, base, A, val, x, xi);
     T acc = T(0);
 #pragma unroll 4
     for (int k = 0; k < w; k++) {
