@@ -483,6 +483,7 @@ class Server {
     }
     cv_work_.notify_all();
     if (worker_.joinable()) worker_.join();
+    if (round_exec_) (void)hipGraphExecDestroy(round_exec_);
     (void)hipStreamSynchronize(setup_stream_);
     for (Setup &su : setups_) (void)hipEventDestroy(su.done);
     (void)hipStreamDestroy(setup_stream_);
@@ -652,6 +653,9 @@ class Server {
   std::map<long long, std::unique_ptr<Req>> reqs_;
   long long next_ticket_ = 1, n_sub_ = 0, n_done_ = 0;
   std::atomic<long long> rounds_{0}, launches_{0};
+  hipGraphExec_t round_exec_ = nullptr;   // captured round of the current set
+  long long round_launches_ = 0;          // kernel launches in that round
+  int stable_rounds_ = 0;                 // rounds since the set last changed
   std::vector<std::unique_ptr<Slot>> active_;   // iterating
   std::vector<std::unique_ptr<Slot>> staged_;   // set-up in flight or done, not yet started
   std::vector<Setup> setups_;
@@ -903,8 +907,7 @@ class Server {
     dirty_ = true;
   }
 
-  void round() {
-    const List &L = all_;
+  void round_body(const List &L) {
     for (int it = 0; it < o_.round_iters; it++) {
       launch(k_b_spmv, L, part32_.p);
       fin<1, float, FinPap>(L, part32_.p, 0);
@@ -912,9 +915,53 @@ class Server {
       fin<3, float, FinUpdateRU>(L, part32_.p, 1, jac(), 1);
       tail(L);
     }
+  }
+
+  // One round: with graphs enabled, a set that stayed unchanged over one
+  // round has its next round captured, and the graph is replayed while the
+  // set stays unchanged (a set that changes every round never pays for a
+  // capture).  Every argument of those launches (item lists, system table,
+  // partial buffers) is fixed until the set changes; the per-request state
+  // they read lives in device memory.  A change of the set (a request joins
+  // or retires) drops the graph.
+  void round() {
+    const List &L = all_;
+    if (graphs_enabled() && (round_exec_ || stable_rounds_ > 0)) {
+      if (!round_exec_) {
+        const long long l0 = launches_;
+        hipGraph_t g = nullptr;
+        MS_HIP(hipStreamBeginCapture(stream(), hipStreamCaptureModeRelaxed));
+        try {
+          round_body(L);
+        } catch (...) {
+          (void)hipStreamEndCapture(stream(), &g);
+          if (g) (void)hipGraphDestroy(g);
+          throw;
+        }
+        MS_HIP(hipStreamEndCapture(stream(), &g));
+        const hipError_t e = hipGraphInstantiate(&round_exec_, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+          round_exec_ = nullptr;
+          fail(MS_EDEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        }
+        round_launches_ = launches_ - l0;
+        launches_ -= round_launches_;
+      }
+      MS_HIP(hipGraphLaunch(round_exec_, stream()));
+      launches_ += round_launches_;
+    } else {
+      round_body(L);
+    }
     MS_CHECK_LAUNCH();
     rounds_++;
+    stable_rounds_++;
     read_states();
+  }
+  void drop_round_graph() {
+    stable_rounds_ = 0;
+    if (round_exec_) MS_HIP(hipGraphExecDestroy(round_exec_));
+    round_exec_ = nullptr;
   }
 
   void after_round() {
@@ -1030,6 +1077,7 @@ class Server {
         if (!ready.empty()) start(ready);
         if (active_.empty()) continue;
         if (dirty_) {
+          drop_round_graph();
           all_.build(active_ptrs());
           ensure_partials(all_.items.size());
           dirty_ = false;

@@ -84,3 +84,33 @@ def test_server_admits_by_budget():
     assert np.array_equal(out[0]["x"], ref["x"])
     # later requests waited for admission
     assert max(r["t_admit"] for r in out) > min(r["t_admit"] for r in out)
+
+
+@pytest.mark.parametrize("precond,coarse", [("dilu", 16), ("jacobi", 0)])
+def test_server_round_graphs_equal_launches(precond, coarse):
+    """Rounds replayed from a captured HIP graph (while the iterating set is
+    unchanged) give the same results, bit for bit, as rounds launched one
+    kernel at a time; requests joining and retiring change the set, so
+    graphs are dropped and captured again along the run."""
+    from mixsolver_amd.solver import set_graphs
+    pats, reqs = _problems()
+    out, stats = {}, {}
+    try:
+        for on in (False, True):
+            set_graphs(on)
+            srv = Server(precond=precond, coarse=coarse, round_iters=16)
+            pids = [srv.add_pattern(P.row_ptr, P.col_idx, P.xc) for P in pats]
+            ts = [srv.submit(pids[pi], val, b, tol=1e-10, maxit=5000) for pi, val, b in reqs]
+            out[on] = [srv.result(t, timeout_ms=-1) for t in ts]
+            stats[on] = srv.stats()
+            srv.close()
+    finally:
+        set_graphs(True)
+    for (pi, val, b), r in zip(reqs, out[True]):
+        _check(pats[pi], val, b, r)
+    for a, g in zip(out[False], out[True]):
+        assert np.array_equal(a["x"], g["x"])
+        assert a["it"] == g["it"] and a["res"] == g["res"]
+    assert stats[True]["completed"] == stats[False]["completed"] == len(reqs)
+    print("rounds/launches eager", stats[False]["rounds"], stats[False]["launches"], "graphs",
+          stats[True]["rounds"], stats[True]["launches"], flush=True)
