@@ -90,8 +90,9 @@ typedef struct ms_system ms_system;
 int ms_init(int device);                 /* select device, create streams   */
 const char *ms_last_error(void);
 int ms_version(void);
-/* Replay the mixed-precision PCG's 16-iteration chunks from a captured HIP
- * graph (1, default; the environment variable MIXSOLVER_GRAPHS=0 sets 0 at
+/* Replay the mixed-precision PCG's 16-iteration chunks, and the request
+ * server's rounds while its iterating set is unchanged, from captured HIP
+ * graphs (1, default; the environment variable MIXSOLVER_GRAPHS=0 sets 0 at
  * start-up) or launch their kernels one by one (0).  Results are identical
  * either way.  No reference counterpart (library option). */
 int ms_set_graphs(int on);
