@@ -3,7 +3,7 @@
  * SIMPLE/PISO pressure-velocity path (TEST INFRASTRUCTURE ONLY).
  *
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use
- * this library, and only as the checker / CPU baseline.  The product path
+ * this library, and only to check results and as the CPU baseline.  The product path
  * (libmixsolver_hip.so) never links or calls it.
  *
  * Every routine restates the arithmetic of the reference Fortran in the same

@@ -1,7 +1,7 @@
 """ctypes binding of the CPU oracle (liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
-bench.py's cpu_baseline leg, always as the checker / CPU baseline.
+bench.py's cpu_baseline leg, always to check results and as the CPU baseline.
 """
 from __future__ import annotations
 
