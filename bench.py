@@ -186,8 +186,8 @@ def main():
     ap.add_argument("--serve-pool", type=int, default=10)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU runs are not part "
-              "of this build", file=sys.stderr)
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}): the device library keeps a single "
+              "partition, and sharding over several GPUs is not implemented", file=sys.stderr)
         sys.exit(2)
     from mixsolver_amd import meshgen
     from mixsolver_amd.solver import Case
