@@ -210,6 +210,9 @@ def main():
         if name == "mixed":
             # roofline of the dominant kernel (FP32 SpMV of the mixed p' solve) on the last p' matrix
             runs["kernel"] = cs.time_kernel(fp32=True, reps=100) + cs.time_kernel(fp32=False, reps=50)
+            # the same kernels with the last-level cache evicted before each launch
+            runs["kernel_cold"] = (cs.time_kernel(fp32=True, reps=20, cold=True)[0],
+                                   cs.time_kernel(fp32=False, reps=20, cold=True)[0])
         cs.close()
         runs[name] = (args.steps / t, t, lat, h)
     cells = mesh.ncells
@@ -287,7 +290,21 @@ def main():
                            "frac": b_k / (t_k * 1e-3) / HBM_PEAK, "traffic": traffic,
                            "traffic_source": "profiles/pcg_mixed_channel2M.md (FETCH_SIZE, WRITE_SIZE, copy-calibrated)",
                            "avg_launch_us": t_k * 1e3, "alg_bytes": b_k,
-                           "fp64_kernel_GBps": b_k64 / (t_k64 * 1e-3) / 1e9, "fp64_avg_launch_us": t_k64 * 1e3}
+                           "fp64_kernel_GBps": b_k64 / (t_k64 * 1e-3) / 1e9, "fp64_avg_launch_us": t_k64 * 1e3,
+                           "copy_1GiB_GBps": 2.0 * (1 << 30) / (cp_ms.value * 1e-3) / 1e9}
+        # achievable HBM bandwidth on 4 GiB buffers (16x the last-level cache)
+        hbm = {}
+        for kind, name, moved in ((0, "copy", 2.0), (1, "copy_nt_stores", 2.0), (2, "read", 1.0)):
+            ms = ctypes.c_double(0.0)
+            _lib.check(_lib.lib().ms_time_stream(4 << 30, kind, 5, ctypes.byref(ms)))
+            hbm[name + "_GBps"] = moved * (4 << 30) / (ms.value * 1e-3) / 1e9
+        out["roofline"]["hbm_achievable_4GiB"] = hbm
+        if "kernel_cold" in runs:
+            c32, c64 = runs["kernel_cold"]
+            out["roofline"].update({"cold_avg_launch_us": c32 * 1e3, "cold_achieved": b_k / (c32 * 1e-3) / 1e9,
+                                    "cold_frac": b_k / (c32 * 1e-3) / HBM_PEAK,
+                                    "fp64_cold_avg_launch_us": c64 * 1e3,
+                                    "fp64_cold_GBps": b_k64 / (c64 * 1e-3) / 1e9})
     if args.serve_seconds > 0:
         # the request server's coarse inverse runs in one workgroup (k <= 256)
         out["serving"] = serve_bench(args.serve_rate, args.serve_seconds, args.serve_pool, min(args.coarse, 256))

@@ -307,7 +307,9 @@ int ms_server_stats(ms_server *s, long long *out8);
  * Average duration of the dominant solver kernel (PCG SpMV with its fused
  * p.q partials) on the system's current matrix, timed with HIP events on the
  * library stream over reps launches.  which: 0 = FP32 difference-form kernel
- * of the mixed path, 1 = FP64 kernel.  alg_bytes: nnz*(s+i) + 4(n+1) + 2ns with
+ * of the mixed path, 1 = FP64 kernel, back to back (the matrix stays in the
+ * last-level cache when it fits); 2 / 3 = the same kernels, each launch timed
+ * alone after a 512 MiB read that evicts the last-level cache (cold).  alg_bytes: nnz*(s+i) + 4(n+1) + 2ns with
  * s the value size and i the column-index size of the format in use (2 for the
  * FP32 kernel's 16-bit slice offsets, else 4). */
 int ms_time_kernel(ms_system *s, int which, int reps, double *avg_ms, double *alg_bytes);
@@ -315,6 +317,11 @@ int ms_time_kernel(ms_system *s, int which, int reps, double *avg_ms, double *al
  * `bytes` (read once, written once): the calibration of traffic counters and
  * the achievable-bandwidth reference beside the published peak. */
 int ms_time_copy(long long bytes, int reps, double *avg_ms);
+/* Average duration of a streaming kernel over `bytes`: kind 0 = the copy
+ * kernel above, 1 = the same copy with non-temporal stores, 2 = a read-only
+ * pass.  The achievable-HBM references beside the published peak (buffers
+ * larger than the 256 MB last-level cache).  No reference counterpart. */
+int ms_time_stream(long long bytes, int kind, int reps, double *avg_ms);
 
 #ifdef __cplusplus
 }

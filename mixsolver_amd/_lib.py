@@ -81,6 +81,7 @@ def lib():
                                C.c_double, _I, _D, _I]),
         "ms_time_kernel": (C.c_int, [_V, C.c_int, C.c_int, _D, _D]),
         "ms_time_copy": (C.c_int, [C.c_longlong, C.c_int, _D]),
+        "ms_time_stream": (C.c_int, [C.c_longlong, C.c_int, C.c_int, _D]),
         "ms_coarse_info": (C.c_int, [_V, _I, _I, _I]),
         "ms_coarse_aggregates": (C.c_int, [C.c_int, _D, C.c_int, _I]),
         "ms_periodic_match": (C.c_int, [C.c_int, _D, C.c_int, _L, _I, _I, _I, _I, C.c_int, _V, _I]),

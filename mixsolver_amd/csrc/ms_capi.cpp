@@ -543,12 +543,22 @@ int ms_time_copy(long long bytes, int reps, double *avg_ms) {
   });
 }
 
+int ms_time_stream(long long bytes, int kind, int reps, double *avg_ms) {
+  return guarded([&] {
+    if (bytes < 16 || kind < 0 || kind > 2 || reps <= 0)
+      fail(MS_EINVAL, "ms_time_stream: bytes >= 16, kind in {0,1,2}, reps > 0");
+    ensure_runtime(-1);
+    const double t = time_stream(bytes, kind, reps);
+    if (avg_ms) *avg_ms = t;
+  });
+}
+
 int ms_time_kernel(ms_system *h, int which, int reps, double *avg_ms, double *alg_bytes) {
   return guarded([&] {
     System &s = sys(h);
-    if (which < 0 || which > 1 || reps <= 0) fail(MS_EINVAL, "ms_time_kernel: which in {0,1}, reps > 0");
-    const bool fp32 = which == 0;
-    double t = s.kry->time_spmv(s.val.p, fp32, reps);
+    if (which < 0 || which > 3 || reps <= 0) fail(MS_EINVAL, "ms_time_kernel: which in {0,1,2,3}, reps > 0");
+    const bool fp32 = which == 0 || which == 2;
+    double t = s.kry->time_spmv(s.val.p, fp32, reps, which >= 2);
     const double sz = fp32 ? 4.0 : 8.0;
     const double idx = (fp32 && s.pat.has16) ? 2.0 : 4.0;   // column index bytes of the format in use
     if (avg_ms) *avg_ms = t;

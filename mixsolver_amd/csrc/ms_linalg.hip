@@ -547,7 +547,61 @@ __global__ void k_copy4(long long n4, const float4 *a, float4 *b) {
     if (base + u * kBlock < n4) b[base + u * kBlock] = v[u];
 }
 
+// streaming read of a zero-filled buffer: evicts the last-level cache before
+// a cold-cache timing (reads only, so no dirty lines drain during the timed
+// kernel); the store never happens for zeros but keeps the loads alive
+__global__ void k_touch(long long n4, const float4 *a, float *sink) {
+  float acc = 0.0f;
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n4; i += (long long)gridDim.x * kBlock) {
+    const float4 v = a[i];
+    acc += v.x + v.y + v.z + v.w;
+  }
+  if (acc == 1.0f) sink[0] = acc;
+}
+
+// copy with non-temporal stores (the destination bypasses the caches)
+typedef float f4v __attribute__((ext_vector_type(4)));
+__global__ void k_copy4_nt(long long n4, const float4 *a, float4 *b) {
+  const long long base = (long long)blockIdx.x * kBlock * 4 + threadIdx.x;
+  f4v v[4];
+#pragma unroll
+  for (int u = 0; u < 4; u++)
+    if (base + u * kBlock < n4) v[u] = *(const f4v *)&a[base + u * kBlock];
+#pragma unroll
+  for (int u = 0; u < 4; u++)
+    if (base + u * kBlock < n4) __builtin_nontemporal_store(v[u], (f4v *)&b[base + u * kBlock]);
+}
+
 }  // namespace
+
+// Average duration of a streaming kernel over `bytes`: kind 0 = k_copy4,
+// 1 = copy with non-temporal stores, 2 = read only (k_touch, 8192 blocks).
+double time_stream(long long bytes, int kind, int reps) {
+  const long long n4 = bytes / 16;
+  DBuf<float4> a(n4), b(kind == 2 ? 1 : n4);
+  DBuf<float> sink(1);
+  MS_HIP(hipMemsetAsync(a.p, 0, n4 * 16, stream()));
+  const int grid = (int)((n4 + 4 * kBlock - 1) / (4 * kBlock));
+  auto go = [&]() {
+    if (kind == 0) hipLaunchKernelGGL(k_copy4, dim3(grid), dim3(kBlock), 0, stream(), n4, a.p, b.p);
+    else if (kind == 1) hipLaunchKernelGGL(k_copy4_nt, dim3(grid), dim3(kBlock), 0, stream(), n4, a.p, b.p);
+    else hipLaunchKernelGGL(k_touch, dim3(8192), dim3(kBlock), 0, stream(), n4, a.p, sink.p);
+  };
+  for (int i = 0; i < 2; i++) go();
+  hipEvent_t e0, e1;
+  MS_HIP(hipEventCreate(&e0));
+  MS_HIP(hipEventCreate(&e1));
+  MS_HIP(hipEventRecord(e0, stream()));
+  for (int i = 0; i < reps; i++) go();
+  MS_HIP(hipEventRecord(e1, stream()));
+  MS_HIP(hipEventSynchronize(e1));
+  float ms = 0.0f;
+  MS_HIP(hipEventElapsedTime(&ms, e0, e1));
+  MS_HIP(hipEventDestroy(e0));
+  MS_HIP(hipEventDestroy(e1));
+  MS_CHECK_LAUNCH();
+  return (double)ms / reps;
+}
 
 double time_copy(long long bytes, int reps) {
   const long long n4 = bytes / 16;
@@ -615,7 +669,7 @@ void Krylov::spmv(const double *val, const double *x, double *y) {
   MS_CHECK_LAUNCH();
 }
 
-double Krylov::time_spmv(const double *val, bool fp32, int reps) {
+double Krylov::time_spmv(const double *val, bool fp32, int reps, bool cold) {
   Mat A = matof(pat);
   const dim3 G(nblk_), B(kBlock);
   hipStream_t S = stream();
@@ -640,6 +694,32 @@ double Krylov::time_spmv(const double *val, bool fp32, int reps) {
       hipLaunchKernelGGL(k_pcg_spmv<double>, G, B, 0, S, n, A, val, p_.p, q_.p, st_.p, part_.p);
   };
   for (int i = 0; i < 3; i++) launch();
+  if (cold) {
+    // each launch timed alone after a 512 MiB read (twice the 256 MB
+    // last-level cache), so its matrix and vectors come from HBM
+    const long long n4 = (512LL << 20) / 16;
+    DBuf<float4> flush(n4);
+    DBuf<float> sink(1);
+    MS_HIP(hipMemsetAsync(flush.p, 0, n4 * 16, S));
+    std::vector<hipEvent_t> ev(2 * (size_t)reps);
+    for (auto &e : ev) MS_HIP(hipEventCreate(&e));
+    for (int i = 0; i < reps; i++) {
+      hipLaunchKernelGGL(k_touch, dim3(8192), B, 0, S, n4, flush.p, sink.p);
+      MS_HIP(hipEventRecord(ev[2 * i], S));
+      launch();
+      MS_HIP(hipEventRecord(ev[2 * i + 1], S));
+    }
+    MS_HIP(hipEventSynchronize(ev.back()));
+    double tot = 0.0;
+    for (int i = 0; i < reps; i++) {
+      float ms = 0.0f;
+      MS_HIP(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]));
+      tot += ms;
+    }
+    for (auto &e : ev) MS_HIP(hipEventDestroy(e));
+    MS_CHECK_LAUNCH();
+    return tot / reps;
+  }
   hipEvent_t e0, e1;
   MS_HIP(hipEventCreate(&e0));
   MS_HIP(hipEventCreate(&e1));

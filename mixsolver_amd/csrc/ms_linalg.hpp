@@ -24,6 +24,7 @@ struct SolveResult {
 
 // average duration (ms) of a device-to-device copy of `bytes` (read + write)
 double time_copy(long long bytes, int reps);
+double time_stream(long long bytes, int kind, int reps);
 
 // Preconditioner kinds used internally.
 enum PcKind { PC_JACOBI = 0, PC_NATURAL_IC0 = 1, PC_NATURAL_ILU0 = 2, PC_DILU = 3 };
@@ -52,7 +53,7 @@ class Krylov {
   // average duration (ms) of the PCG SpMV+dot kernel on matrix val, measured
   // with HIP events on the library stream over `reps` launches; fp32 selects
   // the FP32 difference-form kernel of the mixed path.
-  double time_spmv(const double *val, bool fp32, int reps);
+  double time_spmv(const double *val, bool fp32, int reps, bool cold = false);
   long long launches = 0;
 
   SellPattern *pat;

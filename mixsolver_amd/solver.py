@@ -218,11 +218,13 @@ class Case:
         return dict(theta_top=out[0], found_top=bool(out[1]), theta_bot=out[2], found_bot=bool(out[3]),
                     wake_length=out[4])
 
-    def time_kernel(self, fp32=True, reps=50):
+    def time_kernel(self, fp32=True, reps=50, cold=False):
         """(average ms, algorithmic bytes) of the PCG SpMV kernel on the
-        current matrix (HIP events on the library stream)."""
+        current matrix (HIP events on the library stream); cold = each
+        launch timed alone after the last-level cache was evicted."""
         t, b = C.c_double(0.0), C.c_double(0.0)
-        check(_lib.lib().ms_time_kernel(self.h, 0 if fp32 else 1, int(reps), C.byref(t), C.byref(b)))
+        which = (0 if fp32 else 1) + (2 if cold else 0)
+        check(_lib.lib().ms_time_kernel(self.h, which, int(reps), C.byref(t), C.byref(b)))
         return t.value, b.value
 
     def coarse_info(self):
