@@ -1189,6 +1189,12 @@ SolveResult Krylov::bicg32(PcKind pc, int maxit) {
 // ---- mixed precision ------------------------------------------------------------
 SolveResult Krylov::mixed(const double *val, const double *b, double *x, double tol, int maxit, PcKind pc,
                           int solver, double inner_tol, bool coarse, bool reuse) {
+  // a system whose FP32 PCG stalled once is solved in FP64 from then on
+  if (solver == 0 && fp32_stalled_) {
+    SolveResult F = pcg(val, b, x, tol, maxit, pc, coarse, reuse);
+    F.stalled = 1;
+    return F;
+  }
   SolveResult R;
   Mat A = matof(pat);
   const dim3 G(nblk_), B(kBlock);
@@ -1341,7 +1347,23 @@ SolveResult Krylov::mixed(const double *val, const double *b, double *x, double 
     }
   };
   hs = read_state<double>(st);
+  // Stall guard: when the FP32 recurrence cannot resolve the system (its
+  // residual has not halved over kStall iterations), the iteration stops and
+  // the FP64 PCG with the same preconditioner continues from the current x.
+  // A solve that keeps converging never reaches it, so its result is
+  // unchanged.
+  constexpr int kStall = 1024;
+  double best = hs.res;
+  int it_best = hs.it;
+  bool stalled = false;
   while (hs.done != 1 && hs.it < maxit) {
+    if (hs.res < 0.5 * best) {
+      best = hs.res;
+      it_best = hs.it;
+    } else if (hs.it - it_best >= kStall) {
+      stalled = true;
+      break;
+    }
     const int m = std::min(kChunk, maxit - hs.it);
     if (m == kChunk && graphs_enabled()) {
       // every argument of the chunk's launches is fixed by the key (device
@@ -1395,6 +1417,15 @@ SolveResult Krylov::mixed(const double *val, const double *b, double *x, double 
   R.res = hs.res;
   R.ierr = hs.ierr;
   R.outer = hs.nru;
+  if (stalled && !(hs.res <= tol) && hs.it < maxit) {
+    fp32_stalled_ = true;
+    const SolveResult F = pcg(val, b, x, tol, maxit - hs.it, pc, coarse, false);
+    R.it += F.it;
+    R.res = F.res;
+    R.ierr = F.ierr;
+    R.stalled = 1;
+    return R;
+  }
   if (R.ierr == 0 && hs.done != 1) R.ierr = 1;
   return R;
 }

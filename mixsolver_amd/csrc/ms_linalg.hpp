@@ -20,6 +20,7 @@ struct SolveResult {
   int it = 0, ierr = 0;
   double res = 0.0;
   int outer = 0;          // refinement cycles / reliable updates (mixed precision)
+  int stalled = 0;        // mixed precision: the FP32 iteration stalled, FP64 PCG finished
 };
 
 // average duration (ms) of a device-to-device copy of `bytes` (read + write)
@@ -94,6 +95,7 @@ class Krylov {
     }
   } chunk_key_;
   hipGraphExec_t chunk_exec_ = nullptr;
+  bool fp32_stalled_ = false;   // the mixed PCG stalled on this system: FP64 PCG from then on
 };
 
 }  // namespace ms
