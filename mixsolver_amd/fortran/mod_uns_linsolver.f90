@@ -19,8 +19,7 @@
 !===============================================================================
 module mod_uns_linsolver
    use mod_precision, only: dp
-   use iso_c_binding, only: c_int, c_double
-   use mixsolver_c, only: ms_cg_jacobi, ms_cg_ic0, ms_bicgstab_ilu0, MS_OK
+   use mixsolver_dropin, only: dropin_solve, KIND_BICG, KIND_CGJ, KIND_CGIC
    implicit none
    private
    public :: csr_t, csr_matvec, ilu0_factor, ilu0_apply, bicgstab_ilu0, &
@@ -33,8 +32,6 @@ module mod_uns_linsolver
       real(dp), allocatable :: val(:)       ! nnz
       integer, allocatable :: diag_pos(:)   ! slot of A(i,i), set by the factorizations
    end type csr_t
-
-   integer, parameter :: KIND_BICG = 1, KIND_CGJ = 2, KIND_CGIC = 3
 
 contains
 
@@ -200,46 +197,6 @@ contains
       end do
    end subroutine ic0_apply
 
-   ! one device solve; x may be a strided section, so it goes through a copy
-   subroutine device_solve( kind, A, b, x, tol, maxit, it, res, ierr )
-      integer,     intent(in)    :: kind
-      type(csr_t), intent(in)    :: A
-      real(dp),    intent(in)    :: b(:)
-      real(dp),    intent(inout) :: x(:)
-      real(dp),    intent(in)    :: tol
-      integer,     intent(in)    :: maxit
-      integer,     intent(out)   :: it, ierr
-      real(dp),    intent(out)   :: res
-      real(c_double), allocatable :: xc(:), bc(:), vc(:)
-      integer(c_int), allocatable :: rp(:), ci(:)
-      integer(c_int) :: rc, it_c, ierr_c
-      real(c_double) :: res_c
-      xc = x
-      bc = b
-      vc = A%val(1:A%row_ptr(A%nrows+1)-1)
-      rp = A%row_ptr(1:A%nrows+1)
-      ci = A%col_idx(1:A%row_ptr(A%nrows+1)-1)
-      select case ( kind )
-      case ( KIND_BICG )
-         rc = ms_bicgstab_ilu0( int(A%nrows, c_int), rp, ci, vc, bc, xc, real(tol, c_double), &
-                                int(maxit, c_int), it_c, res_c, ierr_c )
-      case ( KIND_CGJ )
-         rc = ms_cg_jacobi( int(A%nrows, c_int), rp, ci, vc, bc, xc, real(tol, c_double), &
-                            int(maxit, c_int), it_c, res_c, ierr_c )
-      case default
-         rc = ms_cg_ic0( int(A%nrows, c_int), rp, ci, vc, bc, xc, real(tol, c_double), &
-                         int(maxit, c_int), it_c, res_c, ierr_c )
-      end select
-      if ( rc /= MS_OK ) then
-         write(*,'(a,i0)') 'mixsolver: device solver call failed, code ', rc
-         error stop 1
-      end if
-      x = xc
-      it = it_c
-      res = res_c
-      ierr = ierr_c
-   end subroutine device_solve
-
    subroutine bicgstab_ilu0( A, b, x, tol, maxit, it, res, ierr, verb )
       type(csr_t), intent(in)    :: A
       real(dp),    intent(in)    :: b(:)
@@ -250,7 +207,7 @@ contains
       real(dp),    intent(out)   :: res
       integer,     intent(out)   :: ierr
       integer,     intent(in), optional :: verb
-      call device_solve( KIND_BICG, A, b, x, tol, maxit, it, res, ierr )
+      call dropin_solve( KIND_BICG, A%nrows, A%row_ptr, A%col_idx, A%val, b, x, tol, maxit, it, res, ierr )
       if ( present(verb) ) then
          if ( verb > 0 ) write(*,'(a,i6,a,es12.4)') '  bicgstab_ilu0 (device): it=', it, '  res=', res
       end if
@@ -265,7 +222,7 @@ contains
       integer,     intent(out)   :: it
       real(dp),    intent(out)   :: res
       integer,     intent(out)   :: ierr
-      call device_solve( KIND_CGJ, A, b, x, tol, maxit, it, res, ierr )
+      call dropin_solve( KIND_CGJ, A%nrows, A%row_ptr, A%col_idx, A%val, b, x, tol, maxit, it, res, ierr )
    end subroutine cg_jacobi
 
    subroutine cg_ic0( A, b, x, tol, maxit, it, res, ierr )
@@ -277,7 +234,7 @@ contains
       integer,     intent(out)   :: it
       real(dp),    intent(out)   :: res
       integer,     intent(out)   :: ierr
-      call device_solve( KIND_CGIC, A, b, x, tol, maxit, it, res, ierr )
+      call dropin_solve( KIND_CGIC, A%nrows, A%row_ptr, A%col_idx, A%val, b, x, tol, maxit, it, res, ierr )
    end subroutine cg_ic0
 
 end module mod_uns_linsolver

@@ -55,23 +55,30 @@ done
 "$FC" -O2 -o "$OUT/ref_harness" "$TMP/ref_harness.o" "${OBJS[@]}"
 echo "built $OUT/ref_harness"
 
-# Drop-in check: the same chain with the reference's mod_uns_linsolver.f90
-# replaced by mixsolver_amd/fortran/mod_uns_linsolver.f90, whose solvers call
-# libmixsolver_hip.so: oracle/_ref/ref_dropin runs the reference's own
-# drivers with the device Krylov solvers.
+# Drop-in compile-and-link check: the same chain with the reference's
+# mod_uns_linsolver.f90 replaced by mixsolver_amd/fortran/mod_uns_linsolver.f90
+# (whose solvers call libmixsolver_hip.so), plus the one-partition
+# mod_uns_linsolver_mpi.f90 drop-in compiled against the reference's
+# mod_precision.  oracle/_ref/ref_dropin links the reference's own drivers to
+# the device solvers.  It is a reference-derived binary: it stays in this
+# container (oracle/_ref is listed in .gpurunignore) and no GPU test runs it;
+# tests/dropin/dropin_check is the caller the GPU tests run.
 LIBDIR="$(cd "$HERE/../mixsolver_amd" && pwd)"
 if [ -f "$LIBDIR/libmixsolver_hip.so" ]; then
   D="$TMP/dropin"
   mkdir -p "$D"
   cd "$D"
   "$FC" -cpp -O2 -c "$LIBDIR/fortran/mixsolver_c.f90" -o "$D/mixsolver_c.o"
-  DOBJS=("$D/mixsolver_c.o")
+  "$FC" -cpp -O2 -c "$LIBDIR/fortran/mixsolver_dropin.f90" -o "$D/mixsolver_dropin.o"
+  DOBJS=("$D/mixsolver_c.o" "$D/mixsolver_dropin.o")
   for s in "${SRCS[@]}"; do
     [ "$(basename "$s")" = mod_uns_linsolver.f90 ] && s="$LIBDIR/fortran/mod_uns_linsolver.f90"
     o="$D/$(basename "${s%.f90}").o"
     "$FC" -cpp -O2 -c "$s" -o "$o"
     DOBJS+=("$o")
   done
+  "$FC" -cpp -O2 -c "$LIBDIR/fortran/mod_uns_linsolver_mpi.f90" -o "$D/mod_uns_linsolver_mpi.o"
+  DOBJS+=("$D/mod_uns_linsolver_mpi.o")
   "$FC" -cpp -O2 -c "$HERE/ref_harness.f90" -o "$D/ref_harness.o"
   "$FC" -O2 -o "$OUT/ref_dropin" "$D/ref_harness.o" "${DOBJS[@]}" -L"$LIBDIR" -lmixsolver_hip \
     -Wl,-rpath,'$ORIGIN/../../mixsolver_amd'

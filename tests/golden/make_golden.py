@@ -606,7 +606,8 @@ def channel_known_answer():
     and mean pressure, the outlet-layer profile, the regression indicators of
     the case's check on the final fields and the README's table."""
     import re
-    from mixsolver_amd import post
+    sys.path.insert(0, os.path.dirname(HERE))
+    import casecheck
     from mixsolver_amd.control import parse_control
     from oracle import pyoracle
     m = cas.read_cas(os.path.join(REF_CHAN, "unMesh.cas"))
@@ -625,7 +626,7 @@ def channel_known_answer():
         p = r.take(np.float64, nc)
     ctrl, groups = parse_control(ctl, warn=False)
     xc = pyoracle.OracleCase(m, ctrl, groups).geometry()["xc"].reshape(-1, 3)
-    chk = post.channel_check(xc, u, p)
+    chk = casecheck.channel_check(xc, u, p)
     it = [int(k) for k in re.findall(r"CONVERGED after\s+(\d+)", log)]
     it1 = re.search(r"it=\s*1\s+mass-imbal=\s*(\S+)\s+du_max=\s*(\S+).*?lin-it=\s*(\d+)", log)
     lv = np.unique(np.round(xc[:, 0], 9))

@@ -156,7 +156,7 @@ def test_bj_published_answer(alpha, path):
     path (multicolour DILU, FP32 inner iterations with FP64 refinement,
     coarse space): the same steady state within the outer tolerance's reach,
     rel-L2 <= 2e-6 and uF1 / uD within 1e-6 relative of the reference's."""
-    from mixsolver_amd.post import bj_profile
+    from casecheck import bj_profile
     d = load_golden("bj_porous_known_answer")
     ctrl, groups = parse_control(str(d[f"control_a{alpha}"]), warn=False)
     if path == "fast":
@@ -199,7 +199,7 @@ def test_conduction_published_answer(c, path):
     short of the exact uniform 400 K in c2), and the README's error against
     the closed form (printed digits for c3 / c5, the 1e-6 K tolerance for the
     exact cases)."""
-    from mixsolver_amd.post import conduction_exact
+    from casecheck import conduction_exact
     d = load_golden("heat_conduction_known_answer")
     sub = {k[len(c) + 1:]: v for k, v in d.items() if k.startswith(c + "_")}
     ctrl, groups = parse_control(str(sub["control"]), warn=False)

@@ -84,7 +84,7 @@ def test_cylinder_published_indicators():
     within rel-L2 2e-6 of the reference's.  Bars: u, p rel-L2 <= 1e-5, the
     indicators of the reference's run (which give the README's numbers) to
     1e-5 (Cp, recirculation length, drag, Cp deviation) and 0.01 deg."""
-    from mixsolver_amd.post import cylinder_check
+    from casecheck import cylinder_check
     d = load_golden("cylinder_re40_known_answer")
     cs = make(d, gpu_ppe_precond=2, gpu_mom_precond=2, gpu_precision=1, gpu_coarse=16)
     ier, it = cs.simple_run(int(d["nsteps"]))
@@ -148,7 +148,7 @@ def test_channel_lowspeed_known_answer():
     crossed with a 1.2e-4 U margin).
     """
     from mixsolver_amd import meshgen
-    from mixsolver_amd.post import channel_check
+    from casecheck import channel_check
     d = load_golden("channel_lowspeed_known_answer")
     m = meshgen.hex_box(np.arange(301.0), d["ys"], np.array([0.0, 10.0]),
                         zone_ids=dict(xmin=5, xmax=6, ymin=7, ymax=7, zmin=4, zmax=4))

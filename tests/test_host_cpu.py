@@ -326,9 +326,10 @@ end program t
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     fdir = os.path.join(root, "mixsolver_amd", "fortran")
     libdir = os.path.join(root, "mixsolver_amd")
-    for src in (REF_PRECISION, os.path.join(fdir, "mixsolver_c.f90"), os.path.join(fdir, "mod_uns_linsolver.f90")):
+    for src in (REF_PRECISION, os.path.join(fdir, "mixsolver_c.f90"), os.path.join(fdir, "mixsolver_dropin.f90"),
+                os.path.join(fdir, "mod_uns_linsolver.f90")):
         subprocess.run([FLANG, "-O2", "-c", src], cwd=tmp_path, check=True)
-    subprocess.run([FLANG, "-O2", "t.f90", "mod_precision.o", "mixsolver_c.o", "mod_uns_linsolver.o",
+    subprocess.run([FLANG, "-O2", "t.f90", "mod_precision.o", "mixsolver_c.o", "mixsolver_dropin.o", "mod_uns_linsolver.o",
                     f"-L{libdir}", "-lmixsolver_hip", f"-Wl,-rpath,{libdir}", "-o", "t"], cwd=tmp_path, check=True)
     subprocess.run(["./t"], cwd=tmp_path, check=True)
     b = np.fromfile(tmp_path / "out.bin", dtype=np.uint8)
@@ -478,7 +479,7 @@ def test_channel_check_closed_forms():
     and the rise -slope * 299; a centreline ramp crosses the 1 % band at the
     layer it was built to; a dropped outer row or unclipped end width
     changes the uniform-u bulk."""
-    from mixsolver_amd.post import channel_check
+    from casecheck import channel_check
     s = 0.5 * (1.0 - np.cos(np.pi * np.linspace(0.0, 1.0, 41)))
     ys = 10.0 * s
     xc = _chan_cells(ys)
@@ -506,3 +507,61 @@ def test_channel_check_closed_forms():
     u[:, 0] = 0.0
     u[mid, 0] = 1.5 * ub * np.minimum(xc[mid, 0] / 40.5, 1.0) * 0.995 + 0.0075 * ub * (xc[mid, 0] >= 40.5)
     assert channel_check(xc, u, p)["entrance_length"] == 40.5
+
+
+@pytest.mark.skipif(FLANG is None, reason="needs a Fortran compiler")
+def test_dropin_mpi_module_host_helpers(tmp_path):
+    """The one-partition drop-in mod_uns_linsolver_mpi: owned_block keeps the
+    owned rows' columns <= nowned in stored order and matvec_owned is A x over
+    the owned rows (owned_block / matvec_owned, mod_uns_linsolver_mpi.f90:
+    56-149, host only), and a solver handed a partitioned matrix (halo rows)
+    stops with a message before any device call."""
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    fdir = os.path.join(root, "mixsolver_amd", "fortran")
+    libdir = os.path.join(root, "mixsolver_amd")
+    prog = tmp_path / "t.f90"
+    prog.write_text("""program t
+   use mod_precision, only: dp
+   use mod_uns_linsolver, only: csr_t
+   use mod_uns_linsolver_mpi
+   implicit none
+   type(csr_t) :: A, B
+   real(dp) :: x(4), y(3), res
+   integer :: it, ierr, halo, k
+   character(len=8) :: arg
+   ! 3 owned rows, one halo column (4) in rows 1 and 3
+   A%nrows = 4
+   A%row_ptr = [1, 4, 6, 9, 9]
+   A%col_idx = [1, 2, 4, 1, 2, 2, 3, 4]
+   A%val = [4.0_dp, -1.0_dp, -0.5_dp, -1.0_dp, 3.0_dp, -1.0_dp, 5.0_dp, -2.0_dp]
+   call owned_block(A, 3, B)
+   print '(4i3)', B%row_ptr
+   print '(6i3)', B%col_idx
+   print '(6f6.2)', B%val
+   x = [1.0_dp, 2.0_dp, 3.0_dp, 4.0_dp]
+   halo = 0
+   call matvec_owned(A, x, y, halo, 3)
+   print '(3f8.3)', y
+   call get_command_argument(1, arg)
+   if (trim(arg) == 'solve') then
+      x = 0.0_dp
+      call cg_jacobi_mpi(A, y, x, 3, halo, 1.0e-10_dp, 10, it, res, ierr)
+   end if
+end program t
+""")
+    srcs = [os.path.join(root, "tests", "dropin", "mod_precision.f90"), os.path.join(fdir, "mixsolver_c.f90"),
+            os.path.join(fdir, "mixsolver_dropin.f90"), os.path.join(fdir, "mod_uns_linsolver.f90"),
+            os.path.join(fdir, "mod_uns_linsolver_mpi.f90")]
+    for src in srcs:
+        subprocess.run([FLANG, "-O2", "-c", src], cwd=tmp_path, check=True)
+    objs = ["mod_precision.o", "mixsolver_c.o", "mixsolver_dropin.o", "mod_uns_linsolver.o", "mod_uns_linsolver_mpi.o"]
+    subprocess.run([FLANG, "-O2", "t.f90"] + objs + [f"-L{libdir}", "-lmixsolver_hip", f"-Wl,-rpath,{libdir}",
+                    "-o", "t"], cwd=tmp_path, check=True)
+    out = subprocess.run(["./t"], cwd=tmp_path, check=True, capture_output=True, text=True).stdout.split("\n")
+    assert out[0].split() == ["1", "3", "5", "7"]
+    assert out[1].split() == ["1", "2", "1", "2", "2", "3"]
+    assert [float(v) for v in out[2].split()] == [4.0, -1.0, -1.0, 3.0, -1.0, 5.0]
+    assert [float(v) for v in out[3].split()] == [4.0 - 2.0 - 2.0, -1.0 + 6.0, -2.0 + 15.0 - 8.0]
+    r = subprocess.run(["./t", "solve"], cwd=tmp_path, capture_output=True, text=True)
+    assert r.returncode != 0 and "one partition only" in r.stdout

@@ -210,7 +210,7 @@ def test_bj_two_layer_closed_form_published():
     uF1_ref, uD_ref and classical-slip columns to the printed digits, and
     satisfies its own ODEs and interface conditions: wall no-slip on both
     sides, equal shear stress, the alpha-weighted velocity jump."""
-    from mixsolver_amd.post import bj_classical_slip, bj_two_layer
+    from casecheck import bj_classical_slip, bj_two_layer
     d = load_golden("bj_porous_known_answer")
     H, K, mu, fb = (BJ_PHYS[k] for k in ("H", "K", "mu", "fb"))
     lam, dy = np.sqrt(K), 0.010 / 40
@@ -242,7 +242,7 @@ def test_bj_known_answer_fixture_matches_published():
     by the reference's own simple_run, gives its README's numbers: iteration
     counts 754 / 882 / 809, the it=1 log line of alpha = 1, and per alpha the
     uF1 / uD columns and the profile RMS error against the closed form."""
-    from mixsolver_amd.post import bj_profile, bj_two_layer
+    from casecheck import bj_profile, bj_two_layer
     d = load_golden("bj_porous_known_answer")
     xc = _bj_xc(d)
     assert np.array_equal(d["readme_it1_a1"], d["ref_it1_a1"])
@@ -290,7 +290,7 @@ def test_conduction_closed_forms():
     T(a) = 400 K and T(b) = 480 K for the source bar, the energy balance
     -k T'(b) + k T'(a) = q (b - a) = 4e4 W/m^2, T'' = -q/k inside the source,
     end temperatures 300 / 500 K, and T(r = 0.02) = 426.19 K on the annulus."""
-    from mixsolver_amd.post import conduction_exact
+    from casecheck import conduction_exact
     f = lambda x: conduction_exact("source", x, 0.0)
     assert float(f(0.03)) == pytest.approx(400.0, abs=1e-9)
     assert float(f(0.07)) == pytest.approx(480.0, abs=1e-9)
@@ -315,7 +315,7 @@ def test_conduction_known_answer(c):
     temperature has the README's maximum / RMS error against the closed form
     (printed digits; c1, c2, c4 within the 1e-6 K tolerance), and the
     oracle's simple_run reproduces it bit for bit."""
-    from mixsolver_amd.post import conduction_exact
+    from casecheck import conduction_exact
     d = load_golden("heat_conduction_known_answer")
     m = _hc_mesh(d, c)
     ctrl, groups = parse_control(str(d[f"{c}_control"]), warn=False)
@@ -347,11 +347,11 @@ def _cyl_case(d):
 def test_cylinder_known_answer_fixture_matches_published():
     """The reference's shipped Re=40 cylinder after its 8000 SIMPLE
     iterations gives the README's regression indicators (its own check
-    script restated in post.cylinder_check): Cp at the front stagnation
+    indicators computed by casecheck.cylinder_check): Cp at the front stagnation
     point, the base and the suction peak, the separation angles, the
     recirculation length, max|u|, the strongest reverse flow, the ring's
     pressure drag and the Cp deviation from the archived wall baseline."""
-    from mixsolver_amd.post import cylinder_check
+    from casecheck import cylinder_check
     d = load_golden("cylinder_re40_known_answer")
     assert int(d["ref_ier"]) == 0
     xc = _cyl_case(d).geometry()["xc"].reshape(-1, 3)
