@@ -7,7 +7,8 @@
 ! mesh_t) are not interoperable: callers pass their arrays (A%row_ptr,
 ! A%col_idx, A%val, fld%u, ...) as plain buffers.
 module mixsolver_c
-   use iso_c_binding, only: c_int, c_long_long, c_double, c_char, c_ptr, c_null_ptr, c_null_char
+   use iso_c_binding, only: c_int, c_long_long, c_double, c_char, c_ptr, c_null_ptr, c_null_char, &
+                            c_size_t, c_f_pointer, c_associated
    implicit none
    private
    public :: ms_ctrl_t, ms_bcgroup_t
@@ -20,7 +21,7 @@ module mixsolver_c
    public :: ms_compute_gradients, ms_momentum_assembly, ms_temperature_assembly
    public :: ms_flux_rhiechow, ms_outflow_mass_rescale, ms_ppe_assembly, ms_correct_fields
    public :: ms_cg_jacobi, ms_cg_ic0, ms_bicgstab_ilu0, ms_solve, ms_time_kernel, ms_time_copy
-   public :: c_str
+   public :: c_str, last_error
 
    integer(c_int), parameter :: MS_OK = 0, MS_EINVAL = -1, MS_EDEVICE = -2, MS_ENOMEM = -3
    integer(c_int), parameter :: MS_PC_JACOBI = 0, MS_PC_NATURAL = 1, MS_PC_DILU = 2
@@ -65,6 +66,13 @@ module mixsolver_c
       end function
       integer(c_int) function ms_version() bind(c, name='ms_version')
          import :: c_int
+      end function
+      type(c_ptr) function ms_last_error() bind(c, name='ms_last_error')
+         import :: c_ptr
+      end function
+      integer(c_size_t) function strlen(p) bind(c, name='strlen')
+         import :: c_size_t, c_ptr
+         type(c_ptr), value :: p
       end function
       integer(c_int) function ms_system_create(s, nnodes, x, nfaces, fptr, fnodes, c0, c1, zone, ncells, &
                                                ctrl, groups) bind(c, name='ms_system_create')
@@ -237,5 +245,24 @@ contains
       end do
       r(len_trim(s) + 1) = c_null_char
    end function c_str
+
+   ! the message of the library's last error (ms_last_error) as a Fortran string
+   function last_error() result(msg)
+      character(len=:), allocatable :: msg
+      type(c_ptr) :: p
+      character(kind=c_char), pointer :: chars(:)
+      integer :: i, n
+      p = ms_last_error()
+      if ( .not. c_associated(p) ) then
+         msg = ''
+         return
+      end if
+      n = int(strlen(p))
+      call c_f_pointer(p, chars, [n])
+      allocate(character(len=n) :: msg)
+      do i = 1, n
+         msg(i:i) = chars(i)
+      end do
+   end function last_error
 
 end module mixsolver_c

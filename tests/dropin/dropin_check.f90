@@ -98,13 +98,25 @@ contains
       type(ms_bcgroup_t), allocatable :: groups(:)
       type(c_ptr) :: s
       integer :: un
+      integer(c_int8_t), allocatable :: raw(:)
       open(newunit=un, file=fin, access='stream', form='unformatted', status='old')
       read(un) nnodes, nfaces, ncells, nbc, nsteps
       allocate(x(3*nnodes), fptr(nfaces+1), c0(nfaces), c1(nfaces), zone(nfaces))
       read(un) x, fptr
       allocate(fnodes(fptr(nfaces+1)), groups(max(nbc, 1)))
-      read(un) fnodes, c0, c1, zone, ctrl
-      if ( nbc > 0 ) read(un) groups(1:nbc)
+      read(un) fnodes, c0, c1, zone
+      ! The records are C struct images, padding included.  Unformatted I/O of a
+      ! derived type transfers its components only (no padding), so the bytes
+      ! are read as they lie and copied over the interoperable types.
+      allocate(raw(c_sizeof(ctrl)))
+      read(un) raw
+      ctrl = transfer(raw, ctrl)
+      deallocate(raw)
+      if ( nbc > 0 ) then
+         allocate(raw(c_sizeof(groups(1)) * nbc))
+         read(un) raw
+         groups(1:nbc) = transfer(raw, groups(1:nbc), nbc)
+      end if
       allocate(u(3*ncells), p(ncells), T(ncells), flux(nfaces))
       read(un) u, p, T
       close(un)
@@ -128,7 +140,7 @@ contains
       integer(c_int), intent(in) :: rc
       character(len=*), intent(in) :: what
       if ( rc /= MS_OK ) then
-         print '(a,a,i0)', what, ' failed, code ', rc
+         print '(a,a,i0,a,a)', what, ' failed, code ', rc, ': ', last_error()
          error stop 3
       end if
    end subroutine check
