@@ -5,17 +5,22 @@ Workload (one step = one PISO time step of piso_run_mpi at one partition):
   clustering, adjacent-spacing ratio 1.1), Re_tau = 180 (mu = 1/180,
   body force 1), periodic in x and z, walls in y; initial field = laminar
   parabola of bulk velocity U_b = 15.63 + seeded uniform noise of +-5% U_b
-  (seed 180); dt from CFL 0.5; BDF2; 2 PISO correctors; p' by PCG with
-  multicolour DILU to a 1e-10 relative residual, momentum by BiCGSTAB with
-  multicolour DILU to 1e-10.  --bc twin runs the non-periodic twin instead
-  (parabolic inlet / outflow in x, symmetry in z), the case the CPU oracle
-  times, since the reference has no periodic boundary.
-  * mixed: FP32 matrix / preconditioner / Krylov vectors with FP32
-    accumulation inside FP64 reliable updates (p') and FP64 iterative
-    refinement (momentum); every solve ends at a 1e-10 FP64 true residual.
-  * fp64: the same algorithms entirely in FP64.
-value = mixed PISO steps/s; fp64 steps/s, the ratio and p50/p99 step
-latency are reported beside it.  Inputs are resident in HBM before timing.
+  (seed 180); dt from CFL 0.5; BDF2; 2 PISO correctors; p' to a 1e-10
+  relative FP64 residual, momentum by BiCGSTAB with multicolour DILU to
+  1e-10.  --bc twin runs the non-periodic twin instead (parabolic inlet /
+  outflow in x, symmetry in z), the case the CPU oracle times, since the
+  reference has no periodic boundary.  Variants (all reach 1e-10):
+  * mixed_schur (value): the mesh's cell graph has two colours; colour 0 is
+    eliminated and PCG runs on the Schur complement of colour 1 with the
+    Jacobi preconditioner (the iterates of multicolour DILU-PCG) and the
+    aggregate coarse correction, FP32 inner iterations inside FP64 reliable
+    updates;
+  * fp64_schur: the same solver in FP64 (the precision ratio's reference);
+  * mixed / fp64_two_level: PCG with multicolour DILU and the coarse
+    correction on the full system, mixed and FP64;
+  * fp64: PCG with multicolour DILU, FP64, no coarse correction.
+p50/p99 step latency and the iteration counts are reported per variant.
+Inputs are resident in HBM before timing.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
 """
@@ -70,13 +75,24 @@ def initial_field(mesh, ny, nz, nx):
     return u0
 
 
-def run_variant(Case, mesh, ctl, u0, over, steps, warmup):
+def run_variant(Case, mesh, ctl, u0, over, steps, warmup, probe=False):
+    """Warm-up steps, then `steps` timed PISO steps.  probe=True times the
+    last FP32 SpMV launch of every 16-iteration chunk of the p' solves
+    inside the timed region (start / stop events on the library stream; on
+    from the warm-up on, so the chunk graphs are captured before the timed
+    region) and returns (launches, ms, rows) as a fifth element."""
+    from mixsolver_amd.solver import spmv_probe
     cs = Case.from_control(mesh, ctl, **over)
     cs.set_field("u", u0)
+    if probe:
+        spmv_probe(True)
     for _ in range(warmup):
         if cs.piso_run(1) != 0:
             raise RuntimeError("warmup step failed")
     lat, hist = [], []
+    cs.solver_stats(reset=True)       # counters of the timed steps only
+    if probe:
+        spmv_probe(True)
     t0 = time.perf_counter()
     for _ in range(steps):
         t1 = time.perf_counter()
@@ -86,31 +102,108 @@ def run_variant(Case, mesh, ctl, u0, over, steps, warmup):
             raise RuntimeError(f"PISO step failed, ier={ier}")
         hist.append(cs.history()[-1])
     total = time.perf_counter() - t0
+    if probe:
+        return cs, total, np.array(lat), np.array(hist), spmv_probe(False)
     return cs, total, np.array(lat), np.array(hist)
 
 
-def cpu_baseline(mesh, u0, n_needed):
-    """Oracle (C port of the reference path, 1 thread) on a bounded sample:
-    one PISO step of the 2M-cell non-periodic twin channel with the p'
-    PCG-Jacobi solves capped at 40 iterations; the full step is extrapolated
-    with the measured per-iteration p' cost to the iteration count the FP64
-    Jacobi device solve needed for the same step."""
+def step_bytes(mesh, periodic, coarse, n1, nnz_s, ppe_its, mom_its, correctors=2):
+    """Algorithmic HBM bytes of one PISO step on the red-black reduced p'
+    path (mixed precision), summed over its launches with the step's logged
+    iteration counts.  Each array is read once or written once per launch;
+    int32 indices; FP64 assembly, FP32 Krylov work.  n1 / nnz_s: rows and
+    nonzeros of the reduced matrix S."""
+    n = mesh.ncells
+    nper = int(np.count_nonzero(np.isin(mesh.zone, (6, 7, 8, 9)))) if periodic else 0
+    nfi = mesh.n_interior + nper // 2                 # couplings after joining the periodic pairs
+    nfb = mesh.nfaces - mesh.n_interior - nper
+    nf = nfi + nfb
+    nnz = n + 2 * nfi
+    spmv32 = nnz * 8 + 4 * (n + 1) + 2 * n * 4
+    spmv64 = nnz * 12 + 4 * (n + 1) + 3 * n * 8       # r = b - A x
+    dilu32 = (nnz - n) * 8 + 4 * n * 4
+    bicg_it = 2 * spmv32 + 2 * dilu32 + 20 * n * 4
+    mom_asm = nfi * 56 + nfb * 72 + n * 128 + nnz * 8 + 2 * n * 8
+    ppe_asm = nfi * 40 + nfb * 48 + 24 * n + nnz * 8 + 8 * n
+    grad = nf * 40 + 200 * n
+    rhie = nfi * 48 + 96 * n + nf * 8
+    corr = nf * 40 + 64 * n
+    spmv_s = nnz_s * 8 + 4 * (n1 + 1) + 2 * n1 * 4
+    red_it = spmv_s + 44 * n1                         # SpMV on S, update (8 arrays), p update (3)
+    schur_form = nnz * 12 + nnz_s * 8 + nnz_s * 12    # S from A, then its FP32 copy
+    coarse_setup = 16 * coarse ** 3 + 8 * coarse ** 2  # Gauss-Jordan: k launches over the k x k matrix
+    per_corr = ppe_asm + rhie + corr + grad + 3 * spmv64   # residual, reduced rhs, back substitution
+    parts = {
+        "momentum_assembly": 3 * mom_asm, "momentum_solves": mom_its * bicg_it,
+        "p_iterations": ppe_its * red_it, "correctors": correctors * per_corr,
+        "schur_and_coarse_setup": schur_form + coarse_setup, "gradients": grad,
+    }
+    return float(sum(parts.values())), {k: float(v) for k, v in parts.items()}
+
+
+def cpu_leg(cap):
+    """One PISO step of the oracle (the C restatement of the reference path)
+    on the 2M-cell non-periodic twin channel, p' PCG-Jacobi capped at `cap`
+    iterations per solve (0 = the control's 20000, a full step).  Prints
+    JSON: total and p' seconds, p' iterations.  Run in a child process so the
+    serial or the OpenMP oracle library (ORACLE_THREADS) can be chosen."""
+    from mixsolver_amd import meshgen
     from mixsolver_amd.control import parse_control
     from oracle import pyoracle
-    ctrl, groups = parse_control(control_text(400, lin_max=40, periodic=False), warn=False)
+    nx, ny, nz = 400, 100, 50
+    mesh = meshgen.channel(nx, ny, nz)
+    u0 = initial_field(mesh, ny, nz, nx)
+    ctrl, groups = parse_control(control_text(nx, lin_max=cap if cap > 0 else 20000, periodic=False), warn=False)
     ctrl.ppe_precond = 0
     oc = pyoracle.OracleCase(mesh, ctrl, groups)
     oc.set_field("u", u0)
     oc.piso_run(1, mpi=True)
     t_total, t_ppe, iters = oc.timing()
-    oc.close()
-    per_it = t_ppe / max(iters, 1)
-    t_step = (t_total - t_ppe) + per_it * n_needed
-    return dict(value=1.0 / t_step, unit="steps/s", cores=1, kind="port",
-                sample=(f"oracle PISO step on the 2M-cell non-periodic twin channel (parabolic inlet, outflow, "
-                        f"z symmetry) with p' PCG-Jacobi capped at 40 "
-                        f"iterations ({t_total:.1f} s, {iters} p' iterations at {per_it * 1e3:.1f} ms); "
-                        f"extrapolated to the {n_needed} p' iterations of the FP64 Jacobi device solve"))
+    print(json.dumps({"t_total": t_total, "t_ppe": t_ppe, "iters": iters}))
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(n_needed, full=False):
+    """The oracle on the GPU box's host cores, at 1 thread and on all the
+    cores this process may use, on one PISO step of the 2M-cell
+    non-periodic twin channel.  Bounded sample (default): the p' PCG-Jacobi
+    solves capped at 40 iterations each; the step is scaled with the
+    measured per-iteration p' cost to the iteration count the FP64 Jacobi
+    device solve needed for the same step.  full=True: the whole step, no
+    cap, no scaling."""
+    import subprocess
+    ncores = len(os.sched_getaffinity(0))
+    cap = 0 if full else 40
+    legs = {}
+    for threads in (1, ncores):
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads), ORACLE_THREADS="1" if threads > 1 else "0")
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-leg", str(cap)], env=env,
+                           capture_output=True, text=True, check=True)
+        leg = json.loads(r.stdout.strip().splitlines()[-1])
+        per_it = leg["t_ppe"] / max(leg["iters"], 1)
+        t_step = leg["t_total"] if full else (leg["t_total"] - leg["t_ppe"]) + per_it * n_needed
+        legs[threads] = (t_step, leg, per_it)
+        print(f"bench: cpu leg on {threads} thread(s): {t_step:.2f} s/step", file=sys.stderr, flush=True)
+    t1, leg1, pit1 = legs[1]
+    tn, legn, pitn = legs[ncores]
+    how = ("one full step, measured" if full else
+           f"p' PCG-Jacobi capped at 40 iterations per solve ({leg1['iters']} p' iterations at "
+           f"{pit1 * 1e3:.1f} ms on 1 thread, {pitn * 1e3:.1f} ms on {ncores}), the step scaled to the "
+           f"{n_needed} p' iterations of the FP64 Jacobi device solve")
+    return dict(value=1.0 / tn, unit="steps/s", cores=ncores, kind="port",
+                one_thread_steps_per_s=1.0 / t1, cpu_model=cpu_model(),
+                sample=(f"oracle (C restatement of the reference path) PISO step on the 2M-cell non-periodic twin "
+                        f"channel (parabolic inlet, outflow, z symmetry): {how}; {ncores} cores "
+                        f"(OpenMP in the Krylov vector work), 1-thread figure beside it"))
 
 
 def serve_bench(rate, seconds, pool, coarse, seed=4):
@@ -178,13 +271,21 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--bc", choices=("periodic", "twin"), default="periodic",
                     help="periodic x/z channel, or its non-periodic twin")
-    ap.add_argument("--coarse", type=int, default=512, help="aggregates of the p' coarse space")
-    ap.add_argument("--variants", default="mixed,fp64_two_level,fp64",
+    ap.add_argument("--coarse", type=int, default=256, help="aggregates of the p' coarse space")
+    ap.add_argument("--variants", default="mixed_schur,fp64_schur,mixed,fp64_two_level,fp64",
                     help="comma list of variants to run (profiling runs may select one)")
     ap.add_argument("--serve-seconds", type=float, default=3.0, help="serving run length (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-pool", type=int, default=10)
+    ap.add_argument("--probe", choices=("timed", "extra", "off"), default="timed",
+                    help="in-loop SpMV timing of the lead variant: inside its timed region, over as many extra "
+                         "steps after it, or none")
+    ap.add_argument("--cpu-full", action="store_true", help="CPU baseline on one full step (minutes), no scaling")
+    ap.add_argument("--cpu-leg", type=int, default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.cpu_leg is not None:
+        cpu_leg(args.cpu_leg)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}): the device library keeps a single "
               "partition, and sharding over several GPUs is not implemented", file=sys.stderr)
@@ -200,21 +301,49 @@ def main():
     two = dict(base, gpu_coarse=args.coarse)
 
     variants = set(args.variants.split(","))
-    runs = {}
-    for name, over in (("mixed", dict(two, gpu_precision=1, gpu_inner_tol=0.1)),
+    lead_name = next((k for k in ("mixed_schur", "mixed", "fp64_schur", "fp64_two_level", "fp64") if k in variants), None)
+    runs, fields = {}, {}
+    schur = dict(two, gpu_ppe_precond=0, gpu_ppe_solver=2)
+    for name, over in (("mixed_schur", dict(schur, gpu_precision=1, gpu_inner_tol=0.1)),
+                       ("fp64_schur", dict(schur, gpu_precision=0)),
+                       ("mixed", dict(two, gpu_precision=1, gpu_inner_tol=0.1)),
                        ("fp64_two_level", dict(two, gpu_precision=0)),
                        ("fp64", dict(base, gpu_precision=0))):
         if name not in variants:
             continue
-        cs, t, lat, h = run_variant(Case, mesh, ctl, u0, over, args.steps, args.warmup)
+        probe_here = name == lead_name and name.startswith("mixed") and args.probe == "timed"
+        res = run_variant(Case, mesh, ctl, u0, over, args.steps, args.warmup, probe=probe_here)
+        cs, t, lat, h = res[:4]
+        if probe_here:
+            runs["inloop"] = res[4] + ("timed region",)
+        runs[name + "_stats"] = cs.solver_stats()
+        if name in ("mixed", "fp64_two_level", "mixed_schur", "fp64_schur"):
+            # fields after the same steps: the mixed result's error against FP64
+            fields[name] = (cs.get("u"), cs.get("p"))
+        if name == "mixed_schur":
+            runs["kernel_schur"] = (cs.time_kernel(fp32=True, reps=100, reduced=True)
+                                    + cs.time_kernel(fp32=False, reps=50, reduced=True))
+            runs["kernel_schur_cold"] = (cs.time_kernel(fp32=True, reps=20, cold=True, reduced=True)[0],
+                                         cs.time_kernel(fp32=False, reps=20, cold=True, reduced=True)[0])
         if name == "mixed":
             # roofline of the dominant kernel (FP32 SpMV of the mixed p' solve) on the last p' matrix
             runs["kernel"] = cs.time_kernel(fp32=True, reps=100) + cs.time_kernel(fp32=False, reps=50)
             # the same kernels with the last-level cache evicted before each launch
             runs["kernel_cold"] = (cs.time_kernel(fp32=True, reps=20, cold=True)[0],
                                    cs.time_kernel(fp32=False, reps=20, cold=True)[0])
+        if name == lead_name and name.startswith("mixed") and args.probe == "extra":
+            from mixsolver_amd.solver import spmv_probe
+            spmv_probe(True)
+            tx = time.perf_counter()
+            for _ in range(args.steps):
+                if cs.piso_run(1) != 0:
+                    raise RuntimeError("PISO step failed")
+            tx = time.perf_counter() - tx
+            runs["inloop"] = spmv_probe(False) + (f"{args.steps} steps after the timed region "
+                                                  f"({args.steps / tx:.3f} steps/s with the probe)",)
         cs.close()
         runs[name] = (args.steps / t, t, lat, h)
+        print(f"bench: {name} {args.steps / t:.3f} steps/s", file=sys.stderr, flush=True)
     cells = mesh.ncells
     def pct(name, q):
         return float(np.percentile(runs[name][2], q) * 1e3) if name in runs else None
@@ -231,7 +360,27 @@ def main():
     def resmax(name):
         return float(np.max(runs[name][3][:, 7])) if name in runs else None
 
-    lead = "mixed" if "mixed" in runs else sorted(runs.keys() - {"kernel"})[0]
+    lead = next(k for k in ("mixed_schur", "mixed", "fp64_schur", "fp64_two_level", "fp64") if k in runs)
+    ref64 = "fp64_schur" if lead == "mixed_schur" else "fp64_two_level"
+
+    def field_err():
+        if lead not in fields or ref64 not in fields:
+            return None
+        (um, pm), (uf, pf) = fields[lead], fields[ref64]
+        pm, pf = pm - pm.mean(), pf - pf.mean()     # p is fixed up to a constant (periodic channel)
+        return {"u": float(np.linalg.norm(um - uf) / np.linalg.norm(uf)),
+                "p": float(np.linalg.norm(pm - pf) / np.linalg.norm(pf))}
+    solver_desc = {
+        "mixed_schur": "p' red-black reduced PCG (colour 0 eliminated, Jacobi on the Schur complement = "
+                       "multicolour DILU iterates) + {k}-aggregate coarse correction, FP32 inner / FP64 reliable "
+                       "updates to 1e-10",
+        "fp64_schur": "p' red-black reduced PCG + {k}-aggregate coarse correction, FP64, to 1e-10",
+        "mixed": "p' PCG + multicolour DILU + {k}-aggregate coarse correction, FP32 inner / FP64 reliable updates "
+                 "to 1e-10",
+        "fp64_two_level": "p' PCG + multicolour DILU + {k}-aggregate coarse correction, FP64, to 1e-10",
+        "fp64": "p' PCG + multicolour DILU, FP64, to 1e-10",
+    }
+    lead_stats = runs.get(lead + "_stats")
     out = {
         "metric": METRIC,
         "value": sps(lead),
@@ -243,33 +392,33 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 inner + f64 refinement" if lead == "mixed" else "f64",
+        "dtype": "f32 inner + f64 refinement" if lead.startswith("mixed") else "f64",
         "data": "synthetic",
         "config": {"workload": f"channel {nx}x{ny}x{nz} hex ({cells} cells), "
                                + ("periodic x/z" if periodic else "non-periodic twin (inlet/outflow, z symmetry)")
-                               + ", transient PISO, 2 correctors, "
-                               f"p' PCG + multicolour DILU + {args.coarse}-aggregate coarse correction to "
-                               "1e-10, momentum BiCGSTAB + DILU to 1e-10",
+                               + ", transient PISO, 2 correctors, " + solver_desc[lead].format(k=args.coarse)
+                               + ", momentum BiCGSTAB + DILU to 1e-10",
                    "cells": cells, "faces": mesh.nfaces, "time_scheme": "BDF2", "parallelism": "1 GPU",
                    "variant": lead},
-        # the plain FP64 device path: reference recurrences, one-level DILU
+        # precision gain at equal algorithm: the lead (mixed) against the same solver in FP64
+        "fp64_same_solver": ref64,
+        "speedup_precision": ratio(lead, ref64),
+        # rel-L2 of the lead's fields against the same solver in FP64 after the same steps
+        "mixed_vs_fp64_rel_l2": field_err(),
+        # p' solves of the lead run that ran or finished in FP64 (stall guard)
+        "mixed_fp64_fallback_solves": lead_stats["ppe_fp64_solves"] if lead_stats else None,
+        "mixed_fp32_stalled": lead_stats["fp32_stalled"] if lead_stats else None,
+        "variants": {k: {"steps_per_s": sps(k), "p50_step_ms": pct(k, 50), "p99_step_ms": pct(k, 99),
+                         "ppe_iters_per_step": iters(k), "ppe_final_res_max": resmax(k), "solver": solver_desc[k]
+                         .format(k=args.coarse) if k != "fp64" else solver_desc[k]}
+                     for k in ("mixed_schur", "fp64_schur", "mixed", "fp64_two_level", "fp64") if k in runs},
+        # the plain FP64 device path (one-level DILU) and the DILU two-level variants
         "fp64_steps_per_s": sps("fp64"),
-        "speedup_mixed_vs_fp64": ratio("mixed", "fp64"),
-        # the same two-level solver in FP64 (isolates the precision gain)
-        "fp64_two_level_steps_per_s": sps("fp64_two_level"),
-        "speedup_mixed_vs_fp64_two_level": ratio("mixed", "fp64_two_level"),
+        "speedup_vs_fp64_one_level": ratio(lead, "fp64"),
         "p50_step_ms": pct(lead, 50),
         "p99_step_ms": pct(lead, 99),
-        "fp64_p50_step_ms": pct("fp64", 50),
-        "fp64_p99_step_ms": pct("fp64", 99),
-        "fp64_two_level_p50_step_ms": pct("fp64_two_level", 50),
-        "fp64_two_level_p99_step_ms": pct("fp64_two_level", 99),
-        "ppe_iters_per_step_mixed": iters("mixed"),
-        "ppe_iters_per_step_fp64": iters("fp64"),
-        "ppe_iters_per_step_fp64_two_level": iters("fp64_two_level"),
-        "ppe_final_res_mixed_max": resmax("mixed"),
-        "ppe_final_res_fp64_max": resmax("fp64"),
-        "ppe_final_res_fp64_two_level_max": resmax("fp64_two_level"),
+        "ppe_iters_per_step": iters(lead),
+        "ppe_final_res_max": resmax(lead),
         "cell_iterations_per_s": cells * float(np.sum(runs[lead][3][:, 6])) / runs[lead][1],
     }
     # calibration kernel of the traffic counters (rocprofv3 --pmc runs of this
@@ -280,16 +429,28 @@ def main():
     _lib.check(_lib.lib().ms_time_copy(1 << 30, 20, ctypes.byref(cp_ms)))
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "pcg_spmv_traffic.json")
+    kname = "k_pcg_spmv<float> on the reduced matrix S" if "kernel_schur" in runs else "k_pcg_spmv<float>"
     if os.path.exists(tpath):      # HBM bytes per launch measured by the counter runs (committed summary)
         with open(tpath) as fh:
-            traffic = json.load(fh)["traffic_bytes"].get("k_pcg_spmv<float>")
-    if "kernel" in runs:
-        t_k, b_k, t_k64, b_k64 = runs["kernel"]
-        out["roofline"] = {"bound": "hbm", "kernel": "k_pcg_spmv<float> (SELL-64, difference form, fused p.q)",
-                           "achieved": b_k / (t_k * 1e-3) / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                           "frac": b_k / (t_k * 1e-3) / HBM_PEAK, "traffic": traffic,
-                           "traffic_source": "profiles/pcg_mixed_channel2M.md (FETCH_SIZE, WRITE_SIZE, copy-calibrated)",
-                           "avg_launch_us": t_k * 1e3, "alg_bytes": b_k,
+            traffic = json.load(fh)["traffic_bytes"].get(kname)
+    kkey = "kernel_schur" if "kernel_schur" in runs else ("kernel" if "kernel" in runs else None)
+    if kkey:
+        t_k, b_k, t_k64, b_k64 = runs[kkey]
+        # the kernel's average launch inside the PISO steps (probe), else the
+        # back-to-back launches on the last p' matrix
+        t_avg, src = t_k, "100 back-to-back launches on the last p' matrix, HIP events"
+        if "inloop" in runs and runs["inloop"][0] > 0:
+            nl, ms_l, rows_l, where = runs["inloop"]
+            t_avg = ms_l / nl
+            src = (f"in-loop: {nl} launches of the p' solves ({rows_l} rows; the last of each 16-iteration "
+                   f"chunk) over {where}, start/stop events of each launch (hipExtLaunchKernelGGL) on the "
+                   "library stream")
+        out["roofline"] = {"bound": "hbm", "kernel": kname + " (SELL-64, difference form, fused p.q)",
+                           "achieved": b_k / (t_avg * 1e-3) / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                           "frac": b_k / (t_avg * 1e-3) / HBM_PEAK, "traffic": traffic,
+                           "traffic_source": "profiles/ (FETCH_SIZE, WRITE_SIZE, copy-calibrated)",
+                           "avg_launch_us": t_avg * 1e3, "avg_launch_source": src,
+                           "back_to_back_avg_launch_us": t_k * 1e3, "alg_bytes": b_k,
                            "fp64_kernel_GBps": b_k64 / (t_k64 * 1e-3) / 1e9, "fp64_avg_launch_us": t_k64 * 1e3,
                            "copy_1GiB_GBps": 2.0 * (1 << 30) / (cp_ms.value * 1e-3) / 1e9}
         # achievable HBM bandwidth on 4 GiB buffers (16x the last-level cache)
@@ -299,15 +460,33 @@ def main():
             _lib.check(_lib.lib().ms_time_stream(4 << 30, kind, 5, ctypes.byref(ms)))
             hbm[name + "_GBps"] = moved * (4 << 30) / (ms.value * 1e-3) / 1e9
         out["roofline"]["hbm_achievable_4GiB"] = hbm
-        if "kernel_cold" in runs:
-            c32, c64 = runs["kernel_cold"]
+        if kkey + "_cold" in runs:
+            c32, c64 = runs[kkey + "_cold"]
             out["roofline"].update({"cold_avg_launch_us": c32 * 1e3, "cold_achieved": b_k / (c32 * 1e-3) / 1e9,
                                     "cold_frac": b_k / (c32 * 1e-3) / HBM_PEAK,
                                     "fp64_cold_avg_launch_us": c64 * 1e3,
                                     "fp64_cold_GBps": b_k64 / (c64 * 1e-3) / 1e9})
+        if kkey == "kernel_schur" and "kernel" in runs:
+            t_a, b_a = runs["kernel"][0], runs["kernel"][1]
+            out["roofline"]["full_matrix_spmv"] = {"avg_launch_us": t_a * 1e3, "alg_bytes": b_a,
+                                                   "frac": b_a / (t_a * 1e-3) / HBM_PEAK}
+    if lead == "mixed_schur" and "kernel_schur" in runs and lead_stats:
+        # whole-step roofline: algorithmic bytes of every launch of a step
+        # (logged iteration counts) over the measured step time
+        n1 = runs["inloop"][2] if "inloop" in runs and runs["inloop"][2] > 0 else cells // 2
+        nnz_s = (runs["kernel_schur"][3] - 4.0 * (n1 + 1) - 16.0 * n1) / 12.0
+        b_step, parts = step_bytes(mesh, periodic, args.coarse, n1, nnz_s, iters(lead),
+                                   lead_stats["mom_iters"] / args.steps)
+        t_step = runs[lead][1] / args.steps
+        out["roofline_step"] = {"bound": "hbm", "B_step": b_step, "t_step_ms": 1e3 * t_step,
+                                "achieved": b_step / t_step / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                                "frac": b_step / t_step / HBM_PEAK, "parts": parts,
+                                "ppe_iters_per_step": iters(lead),
+                                "mom_iters_per_step": lead_stats["mom_iters"] / args.steps}
     if args.serve_seconds > 0:
         # the request server's coarse inverse runs in one workgroup (k <= 256)
         out["serving"] = serve_bench(args.serve_rate, args.serve_seconds, args.serve_pool, min(args.coarse, 256))
+        print("bench: serving done", file=sys.stderr, flush=True)
     if not args.no_cpu:
         # the oracle runs the non-periodic twin (the reference has no periodic
         # boundary); p' iterations its Jacobi PCG needs for the first step come
@@ -316,7 +495,7 @@ def main():
         jac, _, _, h_jac = run_variant(Case, twin, control_text(nx, periodic=False), u0,
                                        dict(gpu_ppe_precond=0, gpu_mom_precond=2, gpu_precision=0), 1, 0)
         jac.close()
-        out["cpu_baseline"] = cpu_baseline(twin, u0, int(h_jac[0, 6]))
+        out["cpu_baseline"] = cpu_baseline(int(h_jac[0, 6]), full=args.cpu_full)
     print(json.dumps(out))
 
 

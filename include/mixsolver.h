@@ -32,7 +32,11 @@ enum { MS_BC_NONE = 0, MS_BC_WALL = 1, MS_BC_SYMMETRY = 2, MS_BC_VINLET = 3,
        MS_BC_VINLET_PARAB = 10 };
 
 /* preconditioner / precision selectors for the device-path keys */
-enum { MS_PC_JACOBI = 0, MS_PC_NATURAL = 1, MS_PC_DILU = 2 };
+/* MS_PC_NATURAL: ILU(0) / IC(0) in the caller's cell order (the reference's
+ * factors); MS_PC_MC: the same factorizations in the colour-blocked cell
+ * numbering, so the rows of one colour are factored and swept together
+ * (multicolour ILU(0) / IC(0)). */
+enum { MS_PC_JACOBI = 0, MS_PC_NATURAL = 1, MS_PC_DILU = 2, MS_PC_MC = 3 };
 enum { MS_FP64 = 0, MS_MIXED = 1 };
 
 /* Control values: the fields of ctrl_t used on the path
@@ -50,7 +54,11 @@ typedef struct {
   int gpu_ppe_precond;   /* MS_PC_*: p' preconditioner on the device        */
   int gpu_mom_precond;   /* MS_PC_*: momentum / energy preconditioner        */
   int gpu_precision;     /* MS_FP64 or MS_MIXED (FP32 inner, FP64 refinement) */
-  int gpu_ppe_solver;    /* 0 = PCG (reference), 1 = BiCGSTAB                */
+  int gpu_ppe_solver;    /* 0 = PCG (reference), 1 = BiCGSTAB (IC(0)-type
+                            preconditioners become ILU(0)), 2 = PCG on the
+                            red-black Schur complement of a two-colour cell
+                            graph (colour 0 eliminated; Jacobi on S, the
+                            coarse space of gpu_coarse on colour 1)        */
   double gpu_inner_tol;  /* relative tolerance of each FP32 inner solve      */
   int gpu_reorder;       /* 0 = natural order, 1 = RCM                       */
   int gpu_coarse;        /* p' PCG coarse space: number of aggregates, 0 = off (max 1024) */
@@ -148,6 +156,18 @@ int ms_coarse_info(ms_system *s, int *k, int *nchunks, int *agg);
  * in 0..k-1.  Host only (no device needed); no reference counterpart. */
 int ms_coarse_aggregates(int n, const double *xc, int k, int *agg);
 
+/* The device numbering ms_system_create uses (no reference counterpart;
+ * the reference keeps the mesh's order): n2o[new] = old cell (0-based) of a
+ * cell graph given as a 1-based CSR pattern (row_ptr, col_idx; the diagonal
+ * may be present).  rcm != 0: reverse Cuthill-McKee visiting order (from a
+ * minimum-degree cell of each component, ties by index), else natural;
+ * colored != 0: greedy colouring in the visiting order, colour classes
+ * ascending (color[old] = class, *ncolors classes); coarse = k > 0: k
+ * recursive-bisection aggregates of xc (3,n), each contiguous inside each
+ * class.  Host only (no device needed), deterministic. */
+int ms_cell_ordering(int n, const int *row_ptr, const int *col_idx, const double *xc, int rcm, int colored,
+                     int coarse, int *n2o, int *color, int *ncolors);
+
 /* Periodic face matching of ctrl's periodic pairs on a mesh given as for
  * ms_system_create: partner[f] (nfaces) = the face joined with boundary face
  * f (0-based, both directions), -1 elsewhere.  Host only (no device
@@ -207,9 +227,45 @@ int ms_piso_run(ms_system *s, int nsteps, int *ier);
 /* pimple_run_mpi at one partition (mod_uns_simple_mpi.f90:580): n_outer_iter
  * outer iterations per step; ier: 50-56 / 60 as the reference. */
 int ms_pimple_run(ms_system *s, int nsteps, int *ier);
+/* The next nsteps steps of the last ms_piso_run / ms_pimple_run on this
+ * system, as if that call had been given more steps: no re-initialisation
+ * (apc, gradients and fluxes stay as the last step left them) and the step
+ * count goes on (the first-order start applies to the run's first step
+ * only).  The same loop as piso_run_mpi / pimple_run_mpi, cut where a
+ * caller syncs fields (snapshots every n_out_every steps). */
+int ms_piso_continue(ms_system *s, int nsteps, int *ier);
+int ms_pimple_continue(ms_system *s, int nsteps, int *ier);
 /* per-iteration/step diagnostics of the last driver call, 8 doubles per row:
- * imbal du_max dT_max|umax cfl lin_it_max lin_res_mean ppe_it_total ppe_res_last */
+ * imbal du_max dT_max|umax cfl lin_it_max lin_res_mean ppe_it_total
+ * ppe_res_max (the largest final p' residual over the step's correctors) */
 int ms_get_history(ms_system *s, double *out, int maxrows, int *nrows);
+/* Solve counters of the system since creation or the last reset (no
+ * reference counterpart): out8 = p' solves, p' iterations run in FP32, p'
+ * iterations run in FP64, p' solves that ran or finished in FP64 on a
+ * mixed-precision system, 1 when the system's FP32 p' iteration has stalled
+ * (every later p' solve is FP64), the stall window in force, momentum
+ * component solves, their iterations (FP32 inner iterations on a
+ * mixed-precision system).  reset != 0 zeroes the counters after reading
+ * them (the stall switch stays). */
+int ms_solver_stats(ms_system *s, long long *out8, int reset);
+/* Stall guard of the mixed-precision PCG (no reference counterpart): when the
+ * FP32 residual has not halved over `iters` iterations (checked every 16
+ * iterations; 0 = the default 1024) the FP32 iteration stops, the FP64 PCG
+ * with the same preconditioner finishes from the current x with the
+ * remaining budget, and the system solves p' in FP64 from then on.  An
+ * exhausted mixed PCG reports it = maxit + 1 and ierr = 1, as the FP64 CG.
+ * Library-wide setting. */
+int ms_set_stall_window(int iters);
+/* In-loop timing of the mixed-precision PCG's FP32 SpMV (measurement, no
+ * reference counterpart).  The counters (launches timed, their summed
+ * duration in ms, rows of the last matrix timed) are read into the three
+ * pointers when they are all non-null, then on = 1 turns the probe on and
+ * zeroes them, on = 0 turns it off, on = -1 leaves it as it is.  With the
+ * probe on, the last SpMV of every 16-iteration chunk is launched with its
+ * own start / stop events (the kernel's dispatch timestamps) behind the
+ * chunk's captured graph, which then holds the other 15 iterations.
+ * Library-wide setting. */
+int ms_spmv_probe(int on, long long *launches, double *total_ms, long long *rows);
 
 /* ---- wall-zone post-processing on the resident fields ------------------
  * compute_forces (mod_uns_forces.f90:43-185): out14 = Fx Fy Fz CD CL u_inf
@@ -295,7 +351,10 @@ int ms_server_add_pattern(ms_server *s, int n, const int *row_ptr, const int *co
 int ms_server_submit(ms_server *s, int pattern, const double *val, const double *b,
                      double tol, int maxit, long long *ticket);
 /* timeout_ms < 0 waits, 0 polls.  *ready = 1 when x, it, res (FP64 true
- * relative residual), ierr (0 converged, 1 maxit, 2 breakdown) and the times
+ * relative residual), ierr (0 converged, 1 maxit, 2 breakdown, 3 FP32 stall:
+ * the residual did not halve over the stall window of ms_set_stall_window,
+ * and x is the iterate at that point, for the caller to finish in FP64,
+ * e.g. with ms_solve from x as the initial guess) and the times
  * from submission to admission and to completion (seconds) were written; the
  * ticket is then released. */
 int ms_server_result(ms_server *s, long long ticket, double timeout_ms, double *x, int *it,
@@ -312,6 +371,8 @@ int ms_server_stats(ms_server *s, long long *out8);
  * alone after a 512 MiB read that evicts the last-level cache (cold).  alg_bytes: nnz*(s+i) + 4(n+1) + 2ns with
  * s the value size and i the column-index size of the format in use (2 for the
  * FP32 kernel's 16-bit slice offsets, else 4). */
+/* which 4..7: the same four timings on the red-black reduced p' matrix S of a
+ * system with gpu_ppe_solver = 2, after its first p' solve. */
 int ms_time_kernel(ms_system *s, int which, int reps, double *avg_ms, double *alg_bytes);
 /* Average duration of a plain device-to-device copy kernel (k_copy4) moving
  * `bytes` (read once, written once): the calibration of traffic counters and

@@ -54,7 +54,12 @@ def lib():
         "ms_simple_run": (C.c_int, [_V, C.c_int, _I, _I]),
         "ms_piso_run": (C.c_int, [_V, C.c_int, _I]),
         "ms_pimple_run": (C.c_int, [_V, C.c_int, _I]),
+        "ms_piso_continue": (C.c_int, [_V, C.c_int, _I]),
+        "ms_pimple_continue": (C.c_int, [_V, C.c_int, _I]),
         "ms_get_history": (C.c_int, [_V, _D, C.c_int, _I]),
+        "ms_solver_stats": (C.c_int, [_V, _L, C.c_int]),
+        "ms_set_stall_window": (C.c_int, [C.c_int]),
+        "ms_spmv_probe": (C.c_int, [C.c_int, _L, _D, _L]),
         "ms_compute_forces": (C.c_int, [_V, _D]),
         "ms_set_graphs": (C.c_int, [C.c_int]),
         "ms_get_ts_order": (C.c_int, [_V, _I]),
@@ -84,6 +89,7 @@ def lib():
         "ms_time_stream": (C.c_int, [C.c_longlong, C.c_int, C.c_int, _D]),
         "ms_coarse_info": (C.c_int, [_V, _I, _I, _I]),
         "ms_coarse_aggregates": (C.c_int, [C.c_int, _D, C.c_int, _I]),
+        "ms_cell_ordering": (C.c_int, [C.c_int, _I, _I, _D, C.c_int, C.c_int, C.c_int, _I, _I, _I]),
         "ms_periodic_match": (C.c_int, [C.c_int, _D, C.c_int, _L, _I, _I, _I, _I, C.c_int, _V, _I]),
         "ms_server_create": (C.c_int, [C.POINTER(_V), _V]),
         "ms_server_destroy": (C.c_int, [_V]),

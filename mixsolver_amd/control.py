@@ -7,8 +7,16 @@ Keys, defaults and error codes follow ``read_control`` / ``parse_bc`` /
 warn, as in the reference (:540-541), which is how the device-path keys below
 stay compatible with the reference reader:
 
-  gpu_ppe_precond  jacobi | ic0 | dilu | ilu0     (default: follow ppe_precond)
-  gpu_mom_precond  ilu0 | jacobi | dilu           (default: ilu0, the reference)
+  gpu_ppe_precond  jacobi | ic0 | dilu | ilu0 | mc_ilu0 | mc_ic0
+                                                  (default: follow ppe_precond)
+  gpu_mom_precond  ilu0 | jacobi | dilu | mc_ilu0 (default: ilu0, the reference)
+  gpu_ppe_solver   pcg | bicgstab | schur         (default pcg, the reference's
+                   cg_jacobi / cg_ic0; bicgstab solves p' with BiCGSTAB and an
+                   ILU(0)-type preconditioner; schur eliminates one colour of a
+                   two-colour cell graph and runs PCG with the Jacobi
+                   preconditioner and the coarse space on the remaining Schur
+                   complement: the iterates of multicolour DILU-PCG at the cost
+                   of one matrix product per iteration)
   gpu_precision    fp64 | mixed                   (default fp64)
   gpu_inner_tol    mixed PCG: reliable-update trigger, the FP32 recursive residual
                    falls below this fraction of its value at the last FP64 update
@@ -25,7 +33,11 @@ stay compatible with the reference reader:
                    only.
 
 ``ilu0``/``ic0`` select natural-order factorizations (parity with the
-reference); ``dilu`` selects the multicolour DILU preconditioner.
+reference); ``dilu`` selects the multicolour DILU preconditioner;
+``mc_ilu0``/``mc_ic0`` select ILU(0) / IC(0) factorizations in the
+colour-blocked cell numbering (multicolour ILU(0): the rows of one colour are
+factored and swept together).  For p' the factorization is IC(0) under PCG
+and ILU(0) under BiCGSTAB.
 """
 from __future__ import annotations
 
@@ -37,7 +49,8 @@ import numpy as np
 BC_NONE, BC_WALL, BC_SYMMETRY, BC_VINLET, BC_POUTLET, BC_FARFIELD = 0, 1, 2, 3, 4, 5
 BC_SLIPWALL, BC_MASSINLET, BC_OUTFLOW, BC_INTERFACE, BC_VINLET_PARAB = 6, 7, 8, 9, 10
 
-PRECOND = {"jacobi": 0, "ic0": 1, "ilu0": 1, "dilu": 2}
+PRECOND = {"jacobi": 0, "ic0": 1, "ilu0": 1, "dilu": 2, "mc_ilu0": 3, "mc_ic0": 3}
+PPE_SOLVER = {"pcg": 0, "cg": 0, "bicgstab": 1, "schur": 2}
 PRECISION = {"fp64": 0, "mixed": 1}
 
 _BC_NAMES = {
@@ -318,6 +331,8 @@ def parse_control(text: str, warn=True):
                 ppe_gpu = PRECOND[val.lower()]
             elif key == "gpu_mom_precond":
                 ctrl.gpu_mom_precond = PRECOND[val.lower()]
+            elif key == "gpu_ppe_solver":
+                ctrl.gpu_ppe_solver = PPE_SOLVER[val.lower()]
             elif key == "gpu_precision":
                 ctrl.gpu_precision = PRECISION[val.lower()]
             elif key == "gpu_reorder":

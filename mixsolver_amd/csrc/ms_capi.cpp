@@ -13,8 +13,8 @@ namespace ms {
 void ensure_runtime(int device);
 const char *last_error();
 int run_simple(System &s, int nsteps, int *it_done);
-int run_piso(System &s, int nsteps);
-int run_pimple(System &s, int nsteps);
+int run_piso(System &s, int nsteps, bool resume);
+int run_pimple(System &s, int nsteps, bool resume);
 void call_gradients(System &s);
 void call_momentum(System &s, int comp);
 void call_energy(System &s);
@@ -215,6 +215,31 @@ int ms_coarse_aggregates(int n, const double *xc, int k, int *agg) {
   });
 }
 
+int ms_cell_ordering(int n, const int *row_ptr, const int *col_idx, const double *xc, int rcm, int colored,
+                     int coarse, int *n2o, int *color, int *ncolors) {
+  return guarded([&] {
+    if (n <= 0 || !row_ptr || !col_idx || !n2o || !color || !ncolors) fail(MS_EINVAL, "ms_cell_ordering: bad arguments");
+    if (coarse < 0 || coarse > n || (coarse > 0 && !xc)) fail(MS_EINVAL, "ms_cell_ordering: 0 <= coarse <= n, xc needed");
+    std::vector<int> ptr(n + 1, 0), adj;
+    for (int i = 0; i < n; i++) {
+      for (int k = row_ptr[i] - 1; k < row_ptr[i + 1] - 1; k++) {
+        const int j = col_idx[k] - 1;
+        if (j < 0 || j >= n) fail(MS_EINVAL, "ms_cell_ordering: column index out of range");
+        if (j != i) adj.push_back(j);
+      }
+      ptr[i + 1] = (int)adj.size();
+    }
+    std::vector<double> v;
+    if (xc) v.assign(xc, xc + 3 * (size_t)n);
+    std::vector<int> o, c, a;
+    int nc = 0;
+    cell_ordering(n, ptr, adj, v, rcm != 0, colored != 0, std::min(coarse, kMaxAgg), o, c, nc, a);
+    std::copy(o.begin(), o.end(), n2o);
+    std::copy(c.begin(), c.end(), color);
+    *ncolors = nc;
+  });
+}
+
 int ms_set_field(ms_system *h, const char *name, const double *host, long long n) {
   return guarded([&] {
     System &s = sys(h);
@@ -222,7 +247,7 @@ int ms_set_field(ms_system *h, const char *name, const double *host, long long n
     if (!host) fail(MS_EINVAL, "null host array");
     if (f.kind == FACE) {
       if (n != s.nf) fail(MS_EINVAL, "face field size mismatch");
-      f.buf->upload(host, n);
+      s.face_upload(*f.buf, std::vector<double>(host, host + n));
     } else {
       if (n != (long long)s.nc * f.width) fail(MS_EINVAL, "cell field size mismatch");
       std::vector<double> d;
@@ -268,10 +293,10 @@ int ms_get_field(ms_system *h, const char *name, double *host, long long n) {
     } else if (f.kind == FACE) {
       if (n != s.nf) fail(MS_EINVAL, "face field size mismatch");
       download(*f.buf, d, s.nf);
+      for (int g = 0; g < s.nf; g++) host[g] = d[s.fo2d[g]];   // device face order -> face order
       // a periodic shadow face carries its joined face's flux, oriented out of its own cell
       for (int g = 0; g < s.nf; g++)
-        if (s.shadow[g] >= 0) d[g] = -d[s.shadow[g]];
-      std::memcpy(host, d.data(), s.nf * sizeof(double));
+        if (s.shadow[g] >= 0) host[g] = -host[s.shadow[g]];
     } else {
       if (n != (long long)s.nc * f.width) fail(MS_EINVAL, "cell field size mismatch");
       download(*f.buf, d, (size_t)s.nc * f.width);
@@ -391,14 +416,28 @@ int ms_simple_run(ms_system *h, int nsteps, int *it_done, int *ier) {
 
 int ms_piso_run(ms_system *h, int nsteps, int *ier) {
   return guarded([&] {
-    int r = run_piso(sys(h), nsteps);
+    int r = run_piso(sys(h), nsteps, false);
+    if (ier) *ier = r;
+  });
+}
+
+int ms_piso_continue(ms_system *h, int nsteps, int *ier) {
+  return guarded([&] {
+    int r = run_piso(sys(h), nsteps, true);
     if (ier) *ier = r;
   });
 }
 
 int ms_pimple_run(ms_system *h, int nsteps, int *ier) {
   return guarded([&] {
-    int r = run_pimple(sys(h), nsteps);
+    int r = run_pimple(sys(h), nsteps, false);
+    if (ier) *ier = r;
+  });
+}
+
+int ms_pimple_continue(ms_system *h, int nsteps, int *ier) {
+  return guarded([&] {
+    int r = run_pimple(sys(h), nsteps, true);
     if (ier) *ier = r;
   });
 }
@@ -409,6 +448,40 @@ int ms_get_history(ms_system *h, double *out, int maxrows, int *nrows) {
     int rows = (int)(s.history.size() / 8);
     if (nrows) *nrows = rows;
     if (out) std::memcpy(out, s.history.data(), std::min(rows, maxrows) * 8 * sizeof(double));
+  });
+}
+
+int ms_solver_stats(ms_system *h, long long *out8, int reset) {
+  return guarded([&] {
+    System &s = sys(h);
+    if (out8) {
+      out8[0] = s.st_ppe_solves;
+      out8[1] = s.st_it32;
+      out8[2] = s.st_it64;
+      out8[3] = s.st_fallback;
+      const bool st_full = s.kry && s.kry->fp32_stalled();
+      const bool st_red = s.schur && s.schur->kry && s.schur->kry->fp32_stalled();
+      out8[4] = st_full || st_red ? 1 : 0;
+      out8[5] = stall_window();
+      out8[6] = s.st_mom_solves;
+      out8[7] = s.st_mom_its;
+    }
+    if (reset) s.st_ppe_solves = s.st_it32 = s.st_it64 = s.st_fallback = s.st_mom_solves = s.st_mom_its = 0;
+  });
+}
+
+int ms_set_stall_window(int iters) {
+  return guarded([&] {
+    if (iters < 0) fail(MS_EINVAL, "ms_set_stall_window: iters >= 0 (0 = default 1024)");
+    set_stall_window(iters);
+  });
+}
+
+int ms_spmv_probe(int on, long long *launches, double *total_ms, long long *rows) {
+  return guarded([&] {
+    ensure_runtime(-1);
+    if (launches && total_ms && rows) read_spmv_probe(launches, total_ms, rows);
+    if (on >= 0) set_spmv_probe(on != 0);
   });
 }
 
@@ -556,14 +629,21 @@ int ms_time_stream(long long bytes, int kind, int reps, double *avg_ms) {
 int ms_time_kernel(ms_system *h, int which, int reps, double *avg_ms, double *alg_bytes) {
   return guarded([&] {
     System &s = sys(h);
-    if (which < 0 || which > 3 || reps <= 0) fail(MS_EINVAL, "ms_time_kernel: which in {0,1,2,3}, reps > 0");
-    const bool fp32 = which == 0 || which == 2;
-    double t = s.kry->time_spmv(s.val.p, fp32, reps, which >= 2);
+    if (which < 0 || which > 7 || reps <= 0) fail(MS_EINVAL, "ms_time_kernel: which in 0..7, reps > 0");
+    // 4..7: the same four timings on the red-black reduced matrix S
+    const bool red = which >= 4;
+    if (red && (!s.schur || !s.schur->have_values))
+      fail(MS_EINVAL, "ms_time_kernel: no reduced p' matrix yet (gpu_ppe_solver = schur and a solve)");
+    const int w = which & 3;
+    const bool fp32 = w == 0 || w == 2;
+    Krylov &K = red ? *s.schur->kry : *s.kry;
+    const SellPattern &P = red ? s.schur->pat : s.pat;
+    double t = K.time_spmv(red ? s.schur->val.p : s.val.p, fp32, reps, w >= 2);
     const double sz = fp32 ? 4.0 : 8.0;
-    const double idx = (fp32 && s.pat.has16) ? 2.0 : 4.0;   // column index bytes of the format in use
+    const double idx = (fp32 && P.has16) ? 2.0 : 4.0;   // column index bytes of the format in use
     if (avg_ms) *avg_ms = t;
     // algorithmic bytes of one SpMV: nnz (value + column index) + one int per row + x and y
-    if (alg_bytes) *alg_bytes = (double)s.pat.nnz * (sz + idx) + 4.0 * (s.nc + 1) + 2.0 * s.nc * sz;
+    if (alg_bytes) *alg_bytes = (double)P.nnz * (sz + idx) + 4.0 * (P.n + 1) + 2.0 * P.n * sz;
   });
 }
 

@@ -554,9 +554,21 @@ __device__ void coarse_solve(const T *partial, const CoDev &co) {
   const int span = kFinThreads / Q;
   const int a = threadIdx.x % span, q = threadIdx.x / span;
   if (a < co.k) {
+    // four chunk indices, then their four partials, in flight at once; the
+    // sum runs in chunk order
     double s = 0.0;
-    for (int j = co.agg_cptr[a] + q; j < co.agg_cptr[a + 1]; j += Q)
-      s = s + (double)partial[co.agg_cidx[j] * NV + NV - 1];
+    int j = co.agg_cptr[a] + q;
+    const int je = co.agg_cptr[a + 1];
+    for (; j + 3 * Q < je; j += 4 * Q) {
+      const int i0 = co.agg_cidx[j], i1 = co.agg_cidx[j + Q], i2 = co.agg_cidx[j + 2 * Q], i3 = co.agg_cidx[j + 3 * Q];
+      const T v0 = partial[i0 * NV + NV - 1], v1 = partial[i1 * NV + NV - 1];
+      const T v2 = partial[i2 * NV + NV - 1], v3 = partial[i3 * NV + NV - 1];
+      s = s + (double)v0;
+      s = s + (double)v1;
+      s = s + (double)v2;
+      s = s + (double)v3;
+    }
+    for (; j < je; j += Q) s = s + (double)partial[co.agg_cidx[j] * NV + NV - 1];
     partq[q * span + a] = s;
   }
   __syncthreads();
@@ -570,6 +582,20 @@ __device__ void coarse_solve(const T *partial, const CoDev &co) {
   if (a < co.k) {
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;    // E^-1 is symmetric: column a
     int b = b0;
+    // 16 loads in flight per thread (the loop is latency-bound); accumulator
+    // u % 4 takes the entries b = u mod 4 in ascending order, as below
+    for (; b + 15 < b1; b += 16) {
+      float e[16];
+#pragma unroll
+      for (int u = 0; u < 16; u++) e[u] = co.Einv[(size_t)(b + u) * co.k + a];
+#pragma unroll
+      for (int u = 0; u < 16; u += 4) {
+        s0 = s0 + (double)e[u] * c[b + u];
+        s1 = s1 + (double)e[u + 1] * c[b + u + 1];
+        s2 = s2 + (double)e[u + 2] * c[b + u + 2];
+        s3 = s3 + (double)e[u + 3] * c[b + u + 3];
+      }
+    }
     for (; b + 3 < b1; b += 4) {
       const float e0 = co.Einv[(size_t)b * co.k + a], e1 = co.Einv[(size_t)(b + 1) * co.k + a];
       const float e2 = co.Einv[(size_t)(b + 2) * co.k + a], e3 = co.Einv[(size_t)(b + 3) * co.k + a];

@@ -19,9 +19,12 @@
 // Vectors are templated on the value type: double for the FP64 path, float
 // for the inner iterations of mixed precision (FP32 products and FP32
 // accumulation of dot-product partials; scalar recurrences in double).
+#include <atomic>
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+
+#include <hip/hip_ext.h>
 
 #include "ms_kernels.hpp"
 #include "ms_linalg.hpp"
@@ -1135,7 +1138,10 @@ SolveResult Krylov::bicgstab(const double *val, const double *b, double *x, doub
   return R;
 }
 
-SolveResult Krylov::bicg32(PcKind pc, int maxit) {
+// FP32 BiCGSTAB on the scaled residual.  An inner solve whose recursive
+// residual has not halved over `window` iterations ends early (the caller's
+// refinement restarts it from the FP64 residual).
+SolveResult Krylov::bicg32(PcKind pc, int maxit, int window) {
   SolveResult R;
   Mat A = matof(pat);
   const dim3 G(nblk_), B(kBlock);
@@ -1147,6 +1153,8 @@ SolveResult Krylov::bicg32(PcKind pc, int maxit) {
   hipLaunchKernelGGL(k_bi_init<float>, G, B, 0, S, n, r32_.p, r032_.p, p32_.p, v, st);
   int enq = 0, chunk = 4;
   KS hs{};
+  double best = HUGE_VAL;
+  int it_best = 0;
   while (true) {
     int m = std::min(chunk, maxit - enq);
     for (int k = 0; k < m; k++) {
@@ -1178,6 +1186,13 @@ SolveResult Krylov::bicg32(PcKind pc, int maxit) {
     enq += m;
     hs = read_state<float>(st);
     if (hs.done || enq >= maxit) break;
+    if (hs.res < 0.5 * best) {
+      best = hs.res;
+      it_best = enq;
+    } else if (enq - it_best >= window) {
+      R.stalled = 1;
+      break;
+    }
     chunk = std::min(chunk * 2, 32);
   }
   R.it = hs.it;
@@ -1186,13 +1201,62 @@ SolveResult Krylov::bicg32(PcKind pc, int maxit) {
   return R;
 }
 
+// ---- in-loop SpMV timing ---------------------------------------------------------
+// With the probe on, the mixed PCG launches the last FP32 SpMV of every
+// 16-iteration chunk through hipExtLaunchKernelGGL with a start / stop event
+// pair, which take the kernel's own dispatch timestamps (that iteration is
+// enqueued behind the captured graph of the other 15, so it runs back to
+// back with them); the pairs are read after each chunk (the host waits for
+// the chunk's state anyway).
+namespace {
+struct SpmvProbe {
+  bool on = false;
+  std::vector<hipEvent_t> ev;   // 2 per launch of one chunk
+  int used = 0;
+  long long launches = 0, rows = 0;
+  double ms = 0.0;
+};
+SpmvProbe g_probe;
+void probe_harvest() {
+  for (int i = 0; i < g_probe.used; i++) {
+    float t = 0.f;
+    MS_HIP(hipEventSynchronize(g_probe.ev[2 * i + 1]));
+    MS_HIP(hipEventElapsedTime(&t, g_probe.ev[2 * i], g_probe.ev[2 * i + 1]));
+    g_probe.ms += t;
+    g_probe.launches++;
+  }
+  g_probe.used = 0;
+}
+}  // namespace
+
+void set_spmv_probe(bool on) {
+  if (on && g_probe.ev.empty()) {
+    g_probe.ev.resize(64);
+    for (auto &e : g_probe.ev) MS_HIP(hipEventCreate(&e));
+  }
+  g_probe.on = on;
+  g_probe.used = 0;
+  g_probe.launches = g_probe.rows = 0;
+  g_probe.ms = 0.0;
+}
+void read_spmv_probe(long long *launches, double *ms, long long *rows) {
+  *launches = g_probe.launches;
+  *ms = g_probe.ms;
+  *rows = g_probe.rows;
+}
+
 // ---- mixed precision ------------------------------------------------------------
+static std::atomic<int> g_stall_window{1024};
+void set_stall_window(int iters) { g_stall_window.store(iters > 0 ? iters : 1024); }
+int stall_window() { return g_stall_window.load(); }
+
 SolveResult Krylov::mixed(const double *val, const double *b, double *x, double tol, int maxit, PcKind pc,
                           int solver, double inner_tol, bool coarse, bool reuse) {
-  // a system whose FP32 PCG stalled once is solved in FP64 from then on
-  if (solver == 0 && fp32_stalled_) {
-    SolveResult F = pcg(val, b, x, tol, maxit, pc, coarse, reuse);
+  // a system whose FP32 iteration stalled once is solved in FP64 from then on
+  if (fp32_stalled_) {
+    SolveResult F = solver == 0 ? pcg(val, b, x, tol, maxit, pc, coarse, reuse) : bicgstab(val, b, x, tol, maxit, pc);
     F.stalled = 1;
+    F.it64 = F.it;
     return F;
   }
   SolveResult R;
@@ -1232,12 +1296,17 @@ SolveResult Krylov::mixed(const double *val, const double *b, double *x, double 
   if (hs.done) return R;
 
   if (solver == 1) {
-    // BiCGSTAB: FP32 inner solves inside FP64 iterative refinement
+    // BiCGSTAB: FP32 inner solves inside FP64 iterative refinement.  An
+    // inner solve ends at its tolerance or when its residual stops falling
+    // (stall window); a refinement pass that does not halve the FP64
+    // residual hands the system to FP64 BiCGSTAB (the remaining budget, from
+    // the current x), and the system stays in FP64 from then on.
     int total = 0;
+    double prev = R.res;
     while (true) {
       int imax = maxit - total;
       hipLaunchKernelGGL(k_mp_scale_down, G, B, 0, S, n, r_.p, st, r32_.p, x32_.p, st32_.p, kRefineTol, imax);
-      SolveResult in = bicg32(pc, imax);
+      SolveResult in = bicg32(pc, imax, stall_window());
       total += std::max(in.it, 1);
       R.outer++;
       hipLaunchKernelGGL(k_mp_correct, G, B, 0, S, n, x32_.p, st, x);
@@ -1251,8 +1320,21 @@ SolveResult Krylov::mixed(const double *val, const double *b, double *x, double 
         R.ierr = 2;
         break;
       }
+      if (!(hs.res <= 0.5 * prev)) {
+        fp32_stalled_ = true;
+        const SolveResult F = bicgstab(val, b, x, tol, maxit - total, pc);
+        R.it = total + F.it;
+        R.it32 = total;
+        R.it64 = F.it;
+        R.res = F.res;
+        R.ierr = F.ierr;
+        R.stalled = 1;
+        return R;
+      }
+      prev = hs.res;
     }
     R.it = total;
+    R.it32 = total;
     if (R.ierr == 0 && !(R.res <= tol)) R.ierr = 1;
     return R;
   }
@@ -1334,12 +1416,22 @@ SolveResult Krylov::mixed(const double *val, const double *b, double *x, double 
     MS_CHECK_LAUNCH();
   };
   constexpr int kChunk = 16;   // reliable updates are applied at most this many iterations late
-  auto chunk = [&](int m) {
+  const bool probe = g_probe.on;
+  if (probe) g_probe.rows = n;
+  auto chunk = [&](int m, bool probe_last) {
     for (int it = 0; it < m; it++) {
-      if (pat->has16)
+      if (probe_last && it == m - 1) {
+        hipEvent_t e0 = g_probe.ev[2 * g_probe.used], e1 = g_probe.ev[2 * g_probe.used + 1];
+        g_probe.used++;
+        if (pat->has16)
+          hipExtLaunchKernelGGL((k_pcg_spmv<float, 1>), G, B, 0, S, e0, e1, 0, n, A, v32, p, q, st, part32);
+        else
+          hipExtLaunchKernelGGL((k_pcg_spmv<float, 0>), G, B, 0, S, e0, e1, 0, n, A, v32, p, q, st, part32);
+      } else if (pat->has16) {
         hipLaunchKernelGGL((k_pcg_spmv<float, 1>), G, B, 0, S, n, A, v32, p, q, st, part32);
-      else
+      } else {
         hipLaunchKernelGGL((k_pcg_spmv<float, 0>), G, B, 0, S, n, A, v32, p, q, st, part32);
+      }
       finish<OpSum, 1>(nblk_, part32, FinPap{st});
       hipLaunchKernelGGL((k_pcg_update<float, float>), GR, B, 0, S, RW, p, q, y, r, z, dg, mode, c0end, st,
                          part32);
@@ -1352,7 +1444,7 @@ SolveResult Krylov::mixed(const double *val, const double *b, double *x, double 
   // the FP64 PCG with the same preconditioner continues from the current x.
   // A solve that keeps converging never reaches it, so its result is
   // unchanged.
-  constexpr int kStall = 1024;
+  const int kStall = stall_window();
   double best = hs.res;
   int it_best = hs.it;
   bool stalled = false;
@@ -1368,15 +1460,17 @@ SolveResult Krylov::mixed(const double *val, const double *b, double *x, double 
     if (m == kChunk && graphs_enabled()) {
       // every argument of the chunk's launches is fixed by the key (device
       // state lives in st, the iteration count in st->it), so one captured
-      // chunk replays for every full chunk of every solve with this set-up
-      const ChunkKey ck{v32, fac, dg, nu, (int)pc, (int)coarse, (int)fused};
+      // chunk replays for every full chunk of every solve with this set-up;
+      // with the probe on, the chunk's last iteration runs after the graph
+      const int glen = probe ? kChunk - 1 : kChunk;
+      const ChunkKey ck{v32, fac, dg, nu, (int)pc, (int)coarse, (int)fused, glen};
       if (!chunk_exec_ || !(chunk_key_ == ck)) {
         if (chunk_exec_) MS_HIP(hipGraphExecDestroy(chunk_exec_));
         chunk_exec_ = nullptr;
         hipGraph_t g = nullptr;
         MS_HIP(hipStreamBeginCapture(S, hipStreamCaptureModeRelaxed));
         try {
-          chunk(kChunk);
+          chunk(glen, false);
         } catch (...) {
           (void)hipStreamEndCapture(S, &g);
           if (g) (void)hipGraphDestroy(g);
@@ -1392,11 +1486,13 @@ SolveResult Krylov::mixed(const double *val, const double *b, double *x, double 
         chunk_key_ = ck;
       }
       MS_HIP(hipGraphLaunch(chunk_exec_, S));
+      if (probe) chunk(1, true);
     } else {
-      chunk(m);
+      chunk(m, probe);
     }
     MS_CHECK_LAUNCH();
     hs = read_state<double>(st);
+    if (probe) probe_harvest();
     if (hs.ru == 1 && hs.done != 1) {
       reliable_update(hs.done == 2);
       hs = read_state<double>(st);
@@ -1414,19 +1510,26 @@ SolveResult Krylov::mixed(const double *val, const double *b, double *x, double 
     hs = read_state<double>(st);
   }
   R.it = hs.it;
+  R.it32 = hs.it;
   R.res = hs.res;
   R.ierr = hs.ierr;
   R.outer = hs.nru;
   if (stalled && !(hs.res <= tol) && hs.it < maxit) {
     fp32_stalled_ = true;
+    // the FP64 PCG gets the remaining budget; on exhaustion it reports its
+    // budget + 1, so the total is maxit + 1 as for any exhausted CG
     const SolveResult F = pcg(val, b, x, tol, maxit - hs.it, pc, coarse, false);
     R.it += F.it;
+    R.it64 = F.it;
     R.res = F.res;
     R.ierr = F.ierr;
     R.stalled = 1;
     return R;
   }
-  if (R.ierr == 0 && hs.done != 1) R.ierr = 1;
+  if (hs.done != 1) {     // exhaustion: it = maxit + 1, as the FP64 CG (Fortran DO-loop count)
+    if (R.ierr == 0) R.ierr = 1;
+    R.it = maxit + 1;
+  }
   return R;
 }
 

@@ -20,8 +20,19 @@ struct SolveResult {
   int it = 0, ierr = 0;
   double res = 0.0;
   int outer = 0;          // refinement cycles / reliable updates (mixed precision)
-  int stalled = 0;        // mixed precision: the FP32 iteration stalled, FP64 PCG finished
+  int stalled = 0;        // mixed precision: the solve ran or finished in FP64 after an FP32 stall
+  int it32 = 0, it64 = 0; // iterations run in FP32 and in FP64 (it32 + it64 = it, bar the exhaustion count)
 };
+
+// Stall guard of the mixed-precision PCG: the FP32 iteration gives way to
+// FP64 when its residual has not halved over this many iterations (default
+// 1024; checked at the end of each 16-iteration chunk).
+void set_stall_window(int iters);
+int stall_window();
+// in-loop timing of the mixed PCG's FP32 SpMV (the last launch of every
+// 16-iteration chunk; measurement only)
+void set_spmv_probe(bool on);
+void read_spmv_probe(long long *launches, double *ms, long long *rows);
 
 // average duration (ms) of a device-to-device copy of `bytes` (read + write)
 double time_copy(long long bytes, int reps);
@@ -56,6 +67,7 @@ class Krylov {
   // the FP32 difference-form kernel of the mixed path.
   double time_spmv(const double *val, bool fp32, int reps, bool cold = false);
   long long launches = 0;
+  bool fp32_stalled() const { return fp32_stalled_; }
 
   SellPattern *pat;
   int n;
@@ -66,7 +78,7 @@ class Krylov {
   template <class T> void apply_pc(PcKind pc, const T *val, const T *fac, const T *dg, const T *in, T *out,
                                    const int *done);
   template <class T> KS read_state(KS *d);
-  SolveResult bicg32(PcKind pc, int maxit);
+  SolveResult bicg32(PcKind pc, int maxit, int window);
   void ensure32();
   void coarse_setup(const double *val);   // E = Z^T A Z and its inverse
 
@@ -88,10 +100,10 @@ class Krylov {
   // arguments (buffers, preconditioner, coarse mode) are those of the key
   struct ChunkKey {
     const void *v32 = nullptr, *fac = nullptr, *dg = nullptr, *nu = nullptr;
-    int pc = -1, coarse = -1, fused = -1;
+    int pc = -1, coarse = -1, fused = -1, len = 0;
     bool operator==(const ChunkKey &o) const {
       return v32 == o.v32 && fac == o.fac && dg == o.dg && nu == o.nu && pc == o.pc && coarse == o.coarse &&
-             fused == o.fused;
+             fused == o.fused && len == o.len;
     }
   } chunk_key_;
   hipGraphExec_t chunk_exec_ = nullptr;

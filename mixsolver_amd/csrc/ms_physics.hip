@@ -931,10 +931,14 @@ __global__ void k_wake_end(DMesh m, DFields F, double x0, double *part) {
 // ======================================================================
 static dim3 gc(int n) { return dim3(nblocks(n)); }
 
-static PcKind pc_of(int sel, bool ppe) {
+// The preconditioner of a solve: natural and multicolour selectors give the
+// level-scheduled factorization in the system's numbering (IC(0) under PCG,
+// ILU(0) under BiCGSTAB); with the colour-blocked numbering its levels are
+// the colour classes.
+static PcKind pc_of(int sel, int solver) {
   if (sel == MS_PC_JACOBI) return PC_JACOBI;
   if (sel == MS_PC_DILU) return PC_DILU;
-  return ppe ? PC_NATURAL_IC0 : PC_NATURAL_ILU0;
+  return solver == 0 ? PC_NATURAL_IC0 : PC_NATURAL_ILU0;
 }
 
 struct Driver {
@@ -1032,13 +1036,30 @@ struct Driver {
   // corrector with unchanged apc), so the solver keeps its set-up
   SolveResult solve(bool ppe_eq, double *x, bool reuse = false) {
     const ms_ctrl_t &c = s.ctrl;
-    PcKind pc = pc_of(ppe_eq ? c.gpu_ppe_precond : c.gpu_mom_precond, ppe_eq);
     int solver = ppe_eq ? c.gpu_ppe_solver : 1;
+    PcKind pc = pc_of(ppe_eq ? c.gpu_ppe_precond : c.gpu_mom_precond, solver);
     const bool coarse = ppe_eq && s.pat.coarse.k > 0;   // two-level preconditioner for p'
-    if (c.gpu_precision == MS_MIXED)
-      return s.kry->mixed(s.val.p, s.rhs.p, x, c.lin_tol, c.lin_max, pc, solver, c.gpu_inner_tol, coarse, reuse);
-    if (solver == 0) return s.kry->pcg(s.val.p, s.rhs.p, x, c.lin_tol, c.lin_max, pc, coarse, reuse);
-    return s.kry->bicgstab(s.val.p, s.rhs.p, x, c.lin_tol, c.lin_max, pc);
+    SolveResult R;
+    if (ppe_eq && solver == 2) {   // red-black reduced system (ms_schur.hip)
+      R = schur_solve(s, s.val.p, s.rhs.p, x, c.lin_tol, c.lin_max, c.gpu_precision == MS_MIXED, c.gpu_inner_tol,
+                      reuse);
+    } else if (c.gpu_precision == MS_MIXED) {
+      R = s.kry->mixed(s.val.p, s.rhs.p, x, c.lin_tol, c.lin_max, pc, solver, c.gpu_inner_tol, coarse, reuse);
+    } else {
+      R = solver == 0 ? s.kry->pcg(s.val.p, s.rhs.p, x, c.lin_tol, c.lin_max, pc, coarse, reuse)
+                      : s.kry->bicgstab(s.val.p, s.rhs.p, x, c.lin_tol, c.lin_max, pc);
+      R.it64 = R.it;
+    }
+    if (ppe_eq) {
+      s.st_ppe_solves++;
+      s.st_it32 += R.it32;
+      s.st_it64 += R.it64;
+      s.st_fallback += R.stalled;
+    } else {
+      s.st_mom_solves++;
+      s.st_mom_its += R.it;
+    }
+    return R;
   }
   // momentum predictor over the 3 components; returns failing comp (1..3) or 0
   int momentum_solves(int &itl_max, double &ressum) {
@@ -1140,10 +1161,11 @@ int run_simple(System &s, int nsteps, int *it_done) {
 }
 
 // piso_run_mpi at one partition (mod_uns_simple_mpi.f90:298-571)
-int run_piso(System &s, int nsteps) {
+int run_piso(System &s, int nsteps, bool resume) {
   const ms_ctrl_t &c = s.ctrl;
   if (!c.transient) return 30;
   if (c.dt <= 0.0) return 31;
+  if (resume && s.steps_done <= 0) fail(MS_EINVAL, "ms_piso_continue: no earlier ms_piso_run on this system");
   s.ppe_serial = false;  // piso_run_mpi: ppe_assembly_mpi
   s.refresh_face_constants();
   Driver D(s);
@@ -1151,14 +1173,20 @@ int run_piso(System &s, int nsteps) {
   int n_steps = std::max(1, c.n_time_max);
   if (nsteps > 0) n_steps = nsteps;
   s.history.clear();
-  fill(s.apc.p, s.nc, 1.0);
-  D.gradients();
-  D.rhiechow();
+  // a continued run keeps apc, the gradients and the fluxes the last step
+  // left, as the reference's loop does from one step to the next
+  const int it0 = resume ? s.steps_done : 0;
+  if (!resume) {
+    s.steps_done = 0;
+    fill(s.apc.p, s.nc, 1.0);
+    D.gradients();
+    D.rhiechow();
+  }
   // CFL needs min over cells of vol^(1/3) (host pow, as the reference)
   double minv13 = INFINITY;
   for (double v : s.vol_nat) minv13 = std::min(minv13, std::pow(v, 1.0 / 3.0));
   int ier = 0;
-  for (int it = 1; it <= n_steps; it++) {
+  for (int it = it0 + 1; it <= it0 + n_steps; it++) {
     dcopy(s.u_old_old.p, s.u_old.p, n3);
     dcopy(s.u_old.p, s.u.p, n3);
     dcopy(s.T_old_old.p, s.T_old.p, s.nc);
@@ -1181,7 +1209,7 @@ int run_piso(System &s, int nsteps) {
       if (R.ierr == 2) { bad = true; break; }
       itl_max = std::max(itl_max, R.it);
       ppe_it += R.it;
-      ppe_res = R.res;
+      ppe_res = std::max(ppe_res, R.res);   // largest over the correctors
       D.correct(s.pp.p);
     }
     if (bad) { ier = 34; break; }
@@ -1199,6 +1227,7 @@ int run_piso(System &s, int nsteps) {
     const double cfl = umax * c.dt / minv13;
     if (dg[4] != 0.0) { ier = 60; break; }
     push_history(s, imbal, du, umax, cfl, itl_max, ressum / 3.0, ppe_it, ppe_res);
+    s.steps_done = it;
   }
   MS_HIP(hipStreamSynchronize(stream()));
   return ier;
@@ -1208,10 +1237,11 @@ int run_piso(System &s, int nsteps) {
 // iterations per time step, apc = ap / alpha_u (:706), corrections inside
 // each outer iteration, energy once per step (:751-760), ts_order kept at 2
 // on the first step when the state on entry is a BDF2 restart (:664-682).
-int run_pimple(System &s, int nsteps) {
+int run_pimple(System &s, int nsteps, bool resume) {
   const ms_ctrl_t &c = s.ctrl;
   if (!c.transient) return 50;
   if (c.dt <= 0.0) return 51;
+  if (resume && s.steps_done <= 0) fail(MS_EINVAL, "ms_pimple_continue: no earlier ms_pimple_run on this system");
   s.ppe_serial = false;  // pimple_run_mpi: ppe_assembly_mpi
   s.refresh_face_constants();
   Driver D(s);
@@ -1220,14 +1250,19 @@ int run_pimple(System &s, int nsteps) {
   if (nsteps > 0) n_steps = nsteps;
   const int n_outer = std::max(1, c.n_outer_iter);
   s.history.clear();
-  fill(s.apc.p, s.nc, 1.0);
-  D.gradients();
-  D.rhiechow();
-  const int ts_order_in = s.ts_order;
+  const int it0 = resume ? s.steps_done : 0;
+  if (!resume) {
+    s.steps_done = 0;
+    s.ts_order_in = s.ts_order;
+    fill(s.apc.p, s.nc, 1.0);
+    D.gradients();
+    D.rhiechow();
+  }
+  const int ts_order_in = s.ts_order_in;
   double minv13 = INFINITY;
   for (double v : s.vol_nat) minv13 = std::min(minv13, std::pow(v, 1.0 / 3.0));
   int ier = 0;
-  for (int it = 1; it <= n_steps; it++) {
+  for (int it = it0 + 1; it <= it0 + n_steps; it++) {
     dcopy(s.u_old_old.p, s.u_old.p, n3);
     dcopy(s.u_old.p, s.u.p, n3);
     dcopy(s.T_old_old.p, s.T_old.p, s.nc);
@@ -1254,7 +1289,7 @@ int run_pimple(System &s, int nsteps) {
         if (R.ierr == 2) { bad = true; break; }
         itl_max = std::max(itl_max, R.it);
         ppe_it += R.it;
-        ppe_res = R.res;
+        ppe_res = std::max(ppe_res, R.res);   // largest over the correctors
         D.correct(s.pp.p);
       }
     }
@@ -1274,6 +1309,7 @@ int run_pimple(System &s, int nsteps) {
     const double cfl = umax * c.dt / minv13;
     if (dg[4] != 0.0) { ier = 60; break; }
     push_history(s, imbal, du, umax, cfl, itl_max, ressum / (3.0 * n_outer), ppe_it, ppe_res);
+    s.steps_done = it;
   }
   MS_HIP(hipStreamSynchronize(stream()));
   return ier;
@@ -1325,7 +1361,11 @@ static void ensure_walls(System &s) {
     const int g = s.fgrp[f];
     if (g > 0 && s.groups[g - 1].btype == MS_BC_WALL) s.wall_faces.push_back(f);
   }
-  if (!s.wall_faces.empty()) s.d_wall_faces.upload(s.wall_faces);
+  if (!s.wall_faces.empty()) {   // face order, device face ids
+    std::vector<int> dev(s.wall_faces.size());
+    for (size_t k = 0; k < dev.size(); k++) dev[k] = s.fo2d[s.wall_faces[k]];
+    s.d_wall_faces.upload(dev);
+  }
   s.walls_ready = true;
 }
 

@@ -364,6 +364,10 @@ struct Slot {
   DBuf<float> v32, dg, r, z, p, q, y, Einv;
   BSys hsys{};
   KS hks{};
+  // stall guard: the best residual so far (halvings only) and its iteration
+  double best = HUGE_VAL;
+  int it_best = 0;
+  bool stalled = false;
 };
 
 // requests whose set-up runs on the set-up stream; ready when `done` fires
@@ -834,6 +838,9 @@ class Server {
       }
       MS_CHECK_LAUNCH();
       KS *dks = d_ks_.p + S->id;
+      S->best = HUGE_VAL;
+      S->it_best = 0;
+      S->stalled = false;
       S->hks = KS{};
       S->hks.tol = r->tol;
       S->hks.maxit = r->maxit;
@@ -967,9 +974,23 @@ class Server {
   void after_round() {
     std::vector<Slot *> slots = active_ptrs(), ru, paused;
     std::vector<char> exhausted(o_.max_active, 0);
+    // stall guard (as the mixed PCG of a system): a request whose FP32
+    // residual has not halved over the stall window is retired like an
+    // exhausted one, with its true residual and ierr = 3
+    const int window = stall_window();
+    for (Slot *s : slots) {
+      const KS &h = h_ks_[s->id];
+      if (h.done == 1) continue;
+      if (h.res < 0.5 * s->best) {
+        s->best = h.res;
+        s->it_best = h.it;
+      } else if (h.it - s->it_best >= window) {
+        s->stalled = true;
+      }
+    }
     for (Slot *s : slots) {
       KS &h = h_ks_[s->id];
-      const bool ex = h.done != 1 && h.it >= s->req->maxit;
+      const bool ex = h.done != 1 && (h.it >= s->req->maxit || s->stalled);
       if (((h.ru == 1 || h.done == 2) && h.done != 1) || ex) {
         ru.push_back(s);
         if (h.done == 2 && !ex) paused.push_back(s);
@@ -1024,7 +1045,7 @@ class Server {
         r->it = h.it;
         r->res = h.res;
         r->ierr = h.ierr;
-        if (r->ierr == 0 && !(h.done == 1 && h.res <= r->tol)) r->ierr = 1;
+        if (r->ierr == 0 && !(h.done == 1 && h.res <= r->tol)) r->ierr = s->stalled ? 3 : 1;
         r->t_done = t;
         r->done = true;
         n_done_++;

@@ -279,6 +279,38 @@ static std::vector<int> rcm_order(int n, const std::vector<int> &ptr, const std:
 
 // Greedy colouring visiting cells in `visit` order; returns colour per cell.
 std::vector<int> greedy_colors(int n, const std::vector<int> &ptr, const std::vector<int> &adj,
+                               const std::vector<int> &visit, int &ncol);
+
+// The device numbering of the cells (n2o[new] = old): natural or RCM visiting
+// order; with colouring, the greedy colour classes in ascending order, and
+// with k aggregates (agg[old]), each aggregate contiguous inside each class;
+// the visiting order decides ties.  ptr/adj: cell graph without self loops.  color[old] is
+// the colour of each cell (all 0 without colouring).  Deterministic.
+void cell_ordering(int nc, const std::vector<int> &ptr, const std::vector<int> &adj, const std::vector<double> &xc,
+                   bool rcm, bool colored, int nagg, std::vector<int> &n2o, std::vector<int> &color, int &ncolors,
+                   std::vector<int> &agg) {
+  std::vector<int> visit(nc);
+  std::iota(visit.begin(), visit.end(), 0);
+  if (rcm) visit = rcm_order(nc, ptr, adj);
+  color.assign(nc, 0);
+  ncolors = 1;
+  if (colored) color = greedy_colors(nc, ptr, adj, visit, ncolors);
+  agg.assign(nc, 0);
+  if (nagg > 0) agg = rcb_aggregates(nc, xc, nagg);
+  n2o = visit;
+  if (colored || nagg > 0) {
+    std::vector<int> pos(nc);
+    for (int i = 0; i < nc; i++) pos[visit[i]] = i;
+    std::stable_sort(n2o.begin(), n2o.end(), [&](int a, int b) {
+      if (color[a] != color[b]) return color[a] < color[b];
+      if (agg[a] != agg[b]) return agg[a] < agg[b];
+      return pos[a] < pos[b];
+    });
+  }
+}
+
+// Greedy colouring visiting cells in `visit` order; returns colour per cell.
+std::vector<int> greedy_colors(int n, const std::vector<int> &ptr, const std::vector<int> &adj,
                                       const std::vector<int> &visit, int &ncol) {
   std::vector<int> color(n, -1);
   std::vector<int> mark(64, -1);
@@ -761,29 +793,14 @@ System *create_system(int nnodes, const double *x, int nfaces, const long long *
 
   // ---- device numbering -----------------------------------------------------
   const int ppc = ctrl->gpu_ppe_precond, mpc = ctrl->gpu_mom_precond;
-  const bool want_color = ppc == MS_PC_DILU || mpc == MS_PC_DILU;
+  const bool want_schur = ctrl->gpu_ppe_solver == 2;
+  const bool want_color = ppc == MS_PC_DILU || mpc == MS_PC_DILU || ppc == MS_PC_MC || mpc == MS_PC_MC || want_schur;
   const bool want_rcm = ctrl->gpu_reorder == 1;
   if (ctrl->gpu_coarse < 0) fail(MS_EINVAL, "gpu_coarse must be >= 0");
+  if (ctrl->gpu_ppe_solver < 0 || ctrl->gpu_ppe_solver > 2) fail(MS_EINVAL, "gpu_ppe_solver must be 0, 1 or 2");
   const int nagg = std::min(std::min(ctrl->gpu_coarse, kMaxAgg), nc);
-  std::vector<int> visit(nc);
-  std::iota(visit.begin(), visit.end(), 0);
-  if (want_rcm) visit = rcm_order(nc, c2p, c2c);
-  std::vector<int> color(nc, 0);
-  s.ncolors = 1;
-  if (want_color) color = greedy_colors(nc, c2p, c2c, visit, s.ncolors);
-  std::vector<int> agg(nc, 0);
-  if (nagg > 0) agg = rcb_aggregates(nc, s.xc_nat, nagg);
-  s.n2o = visit;
-  if (want_color || nagg > 0) {
-    // colour classes, aggregates contiguous inside each class
-    std::vector<int> pos(nc);
-    for (int i = 0; i < nc; i++) pos[visit[i]] = i;
-    std::stable_sort(s.n2o.begin(), s.n2o.end(), [&](int a, int b) {
-      if (color[a] != color[b]) return color[a] < color[b];
-      if (agg[a] != agg[b]) return agg[a] < agg[b];
-      return pos[a] < pos[b];
-    });
-  }
+  std::vector<int> color, agg;
+  cell_ordering(nc, c2p, c2c, s.xc_nat, want_rcm, want_color, nagg, s.n2o, color, s.ncolors, agg);
   s.o2n.assign(nc, 0);
   for (int i = 0; i < nc; i++) s.o2n[s.n2o[i]] = i;
   s.renumbered = want_color || want_rcm || nagg > 0;
@@ -845,12 +862,13 @@ System *create_system(int nnodes, const double *x, int nfaces, const long long *
     for (int k = s.ref_row_ptr[i] - 1; k < s.ref_row_ptr[i + 1] - 1; k++)
       s.ref2dev[k] = find_slot(s.o2n[i], s.o2n[s.ref_col[k] - 1]);
   s.pat.upload();
-  if (ppc == MS_PC_NATURAL || mpc == MS_PC_NATURAL) s.pat.build_levels();
+  if (ppc == MS_PC_NATURAL || mpc == MS_PC_NATURAL || ppc == MS_PC_MC || mpc == MS_PC_MC) s.pat.build_levels();
+  std::vector<int> agg_dev(nc, 0);
   if (nagg > 0) {
-    std::vector<int> agg_dev(nc);
     for (int r = 0; r < nc; r++) agg_dev[r] = agg[s.n2o[r]];
     build_coarse(s.pat, agg_dev, nagg);
   }
+  if (want_schur) build_schur(s, agg_dev, nagg);
 
   // pin cell and pressure-Dirichlet flag (ppe_assembly :1612-1642)
   s.pin_cell = s.o2n[0];
@@ -884,24 +902,45 @@ System *create_system(int nnodes, const double *x, int nfaces, const long long *
     s.flux_ref = ctrl->rho * usc * ab;
   }
 
+  // ---- device face order ----------------------------------------------------
+  s.fd2o.resize(nf);
+  for (int f = 0; f < nf; f++) s.fd2o[f] = f;
+  {
+    auto lo = [&](int f) { return s.c1[f] >= 0 ? std::min(s.c0[f], s.c1[f]) : s.c0[f]; };
+    auto hi = [&](int f) { return s.c1[f] >= 0 ? std::max(s.c0[f], s.c1[f]) : s.c0[f]; };
+    std::stable_sort(s.fd2o.begin(), s.fd2o.end(), [&](int a, int b) {
+      const bool ba = s.c1[a] < 0, bb = s.c1[b] < 0;
+      if (ba != bb) return bb;   // interior faces first
+      const int la = lo(a), lb = lo(b);
+      if (la != lb) return la < lb;
+      return hi(a) < hi(b);
+    });
+  }
+  s.fo2d.resize(nf);
+  for (int f = 0; f < nf; f++) s.fo2d[s.fd2o[f]] = f;
+
   // ---- upload mesh data ---------------------------------------------------
   std::vector<double> vol(nc), xc(3 * (size_t)nc);
   for (int r = 0; r < nc; r++) {
     vol[r] = s.vol_nat[s.n2o[r]];
     for (int d = 0; d < 3; d++) xc[3 * r + d] = s.xc_nat[3 * (size_t)s.n2o[r] + d];
   }
-  s.d_c0.upload(s.c0);
-  s.d_c1.upload(s.c1);
-  s.d_fgrp.upload(s.fgrp);
-  if (s.nper > 0) s.d_pidx.upload(s.pidx);
+  s.face_upload(s.d_c0, s.c0);
+  s.face_upload(s.d_c1, s.c1);
+  s.face_upload(s.d_fgrp, s.fgrp);
+  if (s.nper > 0) s.face_upload(s.d_pidx, s.pidx);
   s.d_cf_ptr.upload(s.cf_ptr);
-  s.d_cf_ent.upload(s.cf_ent);
+  {
+    std::vector<int> ent(s.cf_ent.size());
+    for (size_t k = 0; k < ent.size(); k++) ent[k] = 2 * s.fo2d[s.cf_ent[k] >> 1] + (s.cf_ent[k] & 1);
+    s.d_cf_ent.upload(ent);
+  }
   s.d_cf_slot.upload(s.cf_slot);
   s.d_diag_slot.upload(s.diag_slot);
-  s.d_xf.upload(s.xf);
-  s.d_sf.upload(s.sf);
-  s.d_area.upload(s.area);
-  s.d_lf.upload(s.lf);
+  s.face_upload(s.d_xf, s.xf, 3);
+  s.face_upload(s.d_sf, s.sf, 3);
+  s.face_upload(s.d_area, s.area);
+  s.face_upload(s.d_lf, s.lf);
   s.d_vol.upload(vol);
   s.d_xc.upload(xc);
   std::vector<DevBC> dbc(s.nbc);
@@ -973,10 +1012,10 @@ System *create_system(int nnodes, const double *x, int nfaces, const long long *
 // slip), solid/fluid faces (conjugate heat transfer) and solid/solid faces.
 void System::refresh_face_constants() {
   if (iface_dirty) {   // interface data: device copies; thermal face modes depend on them
-    d_iface_vel.upload(iface_vel);
-    d_iface_p.upload(iface_p);
-    d_iface_slip_C.upload(iface_slip_C);
-    d_iface_peer_vel.upload(iface_peer_vel);
+    face_upload(d_iface_vel, iface_vel, 3);
+    face_upload(d_iface_p, iface_p);
+    face_upload(d_iface_slip_C, iface_slip_C);
+    face_upload(d_iface_peer_vel, iface_peer_vel, 3);
     iface_dirty = false;
     props_dirty = true;
   }
@@ -1141,23 +1180,23 @@ void System::refresh_face_constants() {
     }
   }
   has_face_flags = any_flag;
-  d_dn.upload(dn);
-  d_dvs.upload(dvs);
-  d_Dm.upload(Dm);
-  d_muf.upload(muf);
-  d_tcor.upload(tcor);
-  d_dw.upload(dw);
-  d_Dmb.upload(Dmb);
-  d_Dt.upload(Dt);
-  d_kf.upload(kfv);
-  d_Dtb.upload(Dtb);
-  d_tmode.upload(tm);
-  d_tval.upload(tv);
-  d_qval.upload(qv);
+  face_upload(d_dn, dn);
+  face_upload(d_dvs, dvs);
+  face_upload(d_Dm, Dm);
+  face_upload(d_muf, muf);
+  face_upload(d_tcor, tcor, 3);
+  face_upload(d_dw, dw);
+  face_upload(d_Dmb, Dmb);
+  face_upload(d_Dt, Dt);
+  face_upload(d_kf, kfv);
+  face_upload(d_Dtb, Dtb);
+  face_upload(d_tmode, tm);
+  face_upload(d_tval, tv);
+  face_upload(d_qval, qv);
   if (any_flag) {
-    d_fflag.upload(ff);
-    d_Cbj.upload(Cbj);
-    d_kch.upload(kch);
+    face_upload(d_fflag, ff);
+    face_upload(d_Cbj, Cbj);
+    face_upload(d_kch, kch);
   }
   // per-cell arrays the thermal kernels read, in device order
   auto dev_upload = [&](const std::vector<double> &h, DBuf<double> &d) {
@@ -1180,11 +1219,11 @@ void System::refresh_face_constants() {
     d_cztype.upload(t);
   }
   if (solid_phase) {
-    d_Ds.upload(Ds);
-    d_ksf.upload(ksfv);
-    d_Dsb.upload(Dsb);
-    d_tmode_s.upload(tms);
-    if (has_iface && iface_T_ltne) d_tval_s.upload(tvs);
+    face_upload(d_Ds, Ds);
+    face_upload(d_ksf, ksfv);
+    face_upload(d_Dsb, Dsb);
+    face_upload(d_tmode_s, tms);
+    if (has_iface && iface_T_ltne) face_upload(d_tval_s, tvs);
   }
   props_dirty = false;
 }

@@ -59,6 +59,17 @@ struct DFields {
   const double *gp_prev;                   // gp before the current gradient pass (kink faces)
 };
 
+// The red-black reduced p' system of a two-colour system (ms_schur.hip):
+// colour-1 rows of the Schur complement S = D1 - B^T D0^-1 B.
+struct SchurSystem {
+  int n0 = 0, n1 = 0;           // colour-0 rows [0,n0), colour-1 rows [n0,n0+n1)
+  SellPattern pat;              // pattern of S, colour-1 local numbering
+  std::unique_ptr<Krylov> kry;  // PCG on S (Jacobi, coarse space of the colour-1 aggregates)
+  DBuf<double> val, rhs, d, r64, part;
+  std::vector<double> hpart;
+  bool have_values = false;     // val holds S of the last matrix
+};
+
 struct System {
   // sizes and control
   int nc = 0, nf = 0, nn = 0, nbc = 0;
@@ -77,6 +88,23 @@ struct System {
   std::vector<signed char> pidx;
   std::vector<int> shadow;
   std::vector<long long> cf_slot, diag_slot;
+  // Device face order: interior faces sorted by their lower device cell
+  // (then the higher), boundary faces after them sorted by their cell, so
+  // the faces of consecutive device cells sit together in memory and the
+  // cell kernels' face loads coalesce.  fd2o[dev] = face, fo2d[face] = dev.
+  // Host arrays (and the API) stay in face order; every device face array is
+  // in device face order (face_upload), and the device cf_ent holds device
+  // face ids in each cell's ascending face order (the reference's
+  // accumulation order).
+  std::vector<int> fd2o, fo2d;
+  template <class T>
+  void face_upload(DBuf<T> &d, const std::vector<T> &h, int width = 1) const {
+    if (h.size() != (size_t)nf * width) fail(MS_EINVAL, "internal: face array size");
+    std::vector<T> t(h.size());
+    for (int f = 0; f < nf; f++)
+      for (int w = 0; w < width; w++) t[(size_t)width * f + w] = h[(size_t)width * fd2o[f] + w];
+    d.upload(t);
+  }
   // natural-order geometry (face arrays: face order; cell arrays: natural order)
   std::vector<double> xf, sf, area, lf, vol_nat, xc_nat;
   // reference CSR (1-based, natural) and its slot map to the device matrix
@@ -84,6 +112,7 @@ struct System {
   std::vector<long long> ref2dev;
   SellPattern pat;
   std::unique_ptr<Krylov> kry;
+  std::unique_ptr<SchurSystem> schur;   // gpu_ppe_solver = schur
   // per-cell properties (natural order, host copy)
   std::vector<double> porosity, perm_dir, inertial, k_s, rho_s, cp_s;
   std::vector<double> disp_l, disp_t, bj_alpha, h_sf, a_sf, q_src;
@@ -114,6 +143,13 @@ struct System {
   DBuf<double> val, rhs, ap, pp, dpp, af, ucomp, part, diag8, tmpc;
   DBuf<unsigned int> ticket;
   std::vector<double> history;  // 8 per row
+  // p' solve counters since creation: solves, iterations run in FP32 and in
+  // FP64, solves that ran or finished in FP64 on a mixed-precision system
+  long long st_ppe_solves = 0, st_it32 = 0, st_it64 = 0, st_fallback = 0;
+  long long st_mom_solves = 0, st_mom_its = 0;   // momentum component solves and their iterations
+  // transient drivers: steps taken since the last (re)started run, and the
+  // ts_order that run found on entry (a continued run goes on from them)
+  int steps_done = 0, ts_order_in = 0;
   int pin_cell = 0, has_pdir = 0, has_outflow = 0;
   // coupling interface: per-face data (face order, zero until set), the
   // switches of set_iface_p_dirichlet / _slip_mode / _zhang_vel /
@@ -176,6 +212,12 @@ int dump_read(const char *path, int ncells, double *const f[7], int *ts_order, s
 // every geometric length so the device coefficients equal the reference's.
 double norm2_ref(const double *v, int n);
 std::vector<int> rcb_aggregates(int n, const std::vector<double> &xc, int k);
+void build_schur(System &s, const std::vector<int> &agg_dev, int nagg);
+SolveResult schur_solve(System &s, const double *val, const double *b, double *x, double tol, int maxit, bool mixed,
+                        double inner_tol, bool reuse);
+void cell_ordering(int nc, const std::vector<int> &ptr, const std::vector<int> &adj, const std::vector<double> &xc,
+                   bool rcm, bool colored, int nagg, std::vector<int> &n2o, std::vector<int> &color, int &ncolors,
+                   std::vector<int> &agg);
 std::vector<int> greedy_colors(int n, const std::vector<int> &ptr, const std::vector<int> &adj,
                                const std::vector<int> &visit, int &ncol);
 void build_coarse(SellPattern &P, const std::vector<int> &agg_dev, int k);

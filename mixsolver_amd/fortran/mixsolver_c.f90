@@ -13,18 +13,19 @@ module mixsolver_c
    private
    public :: ms_ctrl_t, ms_bcgroup_t
    public :: MS_OK, MS_EINVAL, MS_EDEVICE, MS_ENOMEM
-   public :: MS_PC_JACOBI, MS_PC_NATURAL, MS_PC_DILU, MS_FP64, MS_MIXED
+   public :: MS_PC_JACOBI, MS_PC_NATURAL, MS_PC_DILU, MS_PC_MC, MS_FP64, MS_MIXED
    public :: ms_init, ms_version, ms_system_create, ms_system_destroy, ms_system_info
    public :: ms_coarse_info, ms_coarse_aggregates
    public :: ms_set_field, ms_get_field, ms_set_ts_order, ms_get_topology
-   public :: ms_simple_run, ms_piso_run, ms_pimple_run, ms_get_history
+   public :: ms_simple_run, ms_piso_run, ms_pimple_run, ms_piso_continue, ms_pimple_continue, ms_get_history
+   public :: ms_set_cell_types, ms_get_ts_order, ms_vtu_write, ms_nan_scan
    public :: ms_compute_gradients, ms_momentum_assembly, ms_temperature_assembly
    public :: ms_flux_rhiechow, ms_outflow_mass_rescale, ms_ppe_assembly, ms_correct_fields
    public :: ms_cg_jacobi, ms_cg_ic0, ms_bicgstab_ilu0, ms_solve, ms_time_kernel, ms_time_copy
    public :: c_str, last_error
 
    integer(c_int), parameter :: MS_OK = 0, MS_EINVAL = -1, MS_EDEVICE = -2, MS_ENOMEM = -3
-   integer(c_int), parameter :: MS_PC_JACOBI = 0, MS_PC_NATURAL = 1, MS_PC_DILU = 2
+   integer(c_int), parameter :: MS_PC_JACOBI = 0, MS_PC_NATURAL = 1, MS_PC_DILU = 2, MS_PC_MC = 3
    integer(c_int), parameter :: MS_FP64 = 0, MS_MIXED = 1
 
    type, bind(c) :: ms_ctrl_t
@@ -146,6 +147,44 @@ module mixsolver_c
          type(c_ptr), value :: s
          integer(c_int), value :: nsteps
          integer(c_int) :: ier
+      end function
+      integer(c_int) function ms_piso_continue(s, nsteps, ier) bind(c, name='ms_piso_continue')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: s
+         integer(c_int), value :: nsteps
+         integer(c_int) :: ier
+      end function
+      integer(c_int) function ms_pimple_continue(s, nsteps, ier) bind(c, name='ms_pimple_continue')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: s
+         integer(c_int), value :: nsteps
+         integer(c_int) :: ier
+      end function
+      integer(c_int) function ms_set_cell_types(s, cztype) bind(c, name='ms_set_cell_types')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: s
+         integer(c_int) :: cztype(*)
+      end function
+      integer(c_int) function ms_get_ts_order(s, t) bind(c, name='ms_get_ts_order')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: s
+         integer(c_int) :: t
+      end function
+      integer(c_int) function ms_nan_scan(s, field, cell, val, kind, ifin) bind(c, name='ms_nan_scan')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: s
+         integer(c_int) :: field, cell, kind, ifin
+         real(c_double) :: val
+      end function
+      integer(c_int) function ms_vtu_write(path, binary, nnodes, x, nfaces, fptr, fnodes, c0, c1, ncells, xc, p, u, &
+                                           T, T_s, q_src) bind(c, name='ms_vtu_write')
+         import :: c_int, c_char, c_double, c_long_long, c_ptr
+         character(kind=c_char) :: path(*)
+         integer(c_int), value :: binary, nnodes, nfaces, ncells
+         real(c_double) :: x(*), xc(*), p(*), u(*), T(*)
+         integer(c_long_long) :: fptr(*)
+         integer(c_int) :: fnodes(*), c0(*), c1(*)
+         type(c_ptr), value :: T_s, q_src
       end function
       integer(c_int) function ms_get_history(s, out, maxrows, nrows) bind(c, name='ms_get_history')
          import :: c_int, c_ptr, c_double

@@ -182,12 +182,37 @@ class Case:
         check(_lib.lib().ms_pimple_run(self.h, int(nsteps), C.byref(ier)))
         return ier.value
 
+    def piso_continue(self, nsteps):
+        """The next nsteps steps of the last piso_run, without restarting it."""
+        ier = C.c_int(0)
+        check(_lib.lib().ms_piso_continue(self.h, int(nsteps), C.byref(ier)))
+        return ier.value
+
+    def pimple_continue(self, nsteps):
+        """The next nsteps steps of the last pimple_run, without restarting it."""
+        ier = C.c_int(0)
+        check(_lib.lib().ms_pimple_continue(self.h, int(nsteps), C.byref(ier)))
+        return ier.value
+
     def history(self):
         n = C.c_int(0)
         check(_lib.lib().ms_get_history(self.h, None, 0, C.byref(n)))
         out = np.zeros(max(1, n.value) * 8)
         check(_lib.lib().ms_get_history(self.h, dp(out), n.value, C.byref(n)))
         return out[: n.value * 8].reshape(-1, 8)
+
+    STAT_KEYS = ("ppe_solves", "ppe_iters_fp32", "ppe_iters_fp64", "ppe_fp64_solves", "fp32_stalled",
+                 "stall_window", "mom_solves", "mom_iters")
+
+    def solver_stats(self, reset=False):
+        """Solve counters since creation or the last reset: p' solves, p'
+        iterations in FP32 and in FP64, p' solves that ran or finished in FP64
+        on a mixed-precision system, whether the FP32 p' iteration stalled
+        (sticky), the stall window in force, momentum component solves and
+        their iterations."""
+        out = np.zeros(8, np.int64)
+        check(_lib.lib().ms_solver_stats(self.h, out.ctypes.data_as(C.POINTER(C.c_longlong)), int(reset)))
+        return dict(zip(self.STAT_KEYS, (int(v) for v in out)))
 
     FORCE_KEYS = ("Fx", "Fy", "Fz", "CD", "CL", "u_inf", "A_ref", "p_inf", "Fx_p", "Fy_p", "Fz_p", "Fx_v", "Fy_v",
                   "Fz_v")
@@ -218,12 +243,13 @@ class Case:
         return dict(theta_top=out[0], found_top=bool(out[1]), theta_bot=out[2], found_bot=bool(out[3]),
                     wake_length=out[4])
 
-    def time_kernel(self, fp32=True, reps=50, cold=False):
+    def time_kernel(self, fp32=True, reps=50, cold=False, reduced=False):
         """(average ms, algorithmic bytes) of the PCG SpMV kernel on the
         current matrix (HIP events on the library stream); cold = each
-        launch timed alone after the last-level cache was evicted."""
+        launch timed alone after the last-level cache was evicted; reduced =
+        the red-black reduced p' matrix (gpu_ppe_solver = schur)."""
         t, b = C.c_double(0.0), C.c_double(0.0)
-        which = (0 if fp32 else 1) + (2 if cold else 0)
+        which = (0 if fp32 else 1) + (2 if cold else 0) + (4 if reduced else 0)
         check(_lib.lib().ms_time_kernel(self.h, which, int(reps), C.byref(t), C.byref(b)))
         return t.value, b.value
 
@@ -276,6 +302,35 @@ def set_graphs(on: bool) -> None:
     HIP graph (default) or launch their kernels one by one; results are
     identical either way."""
     check(_lib.lib().ms_set_graphs(1 if on else 0))
+
+
+def set_stall_window(iters: int) -> None:
+    """Iterations without a halving of the FP32 residual after which the
+    mixed-precision PCG hands over to FP64 (0 = the default 1024)."""
+    check(_lib.lib().ms_set_stall_window(int(iters)))
+
+
+def spmv_probe(on=None):
+    """In-loop timing of the mixed PCG's FP32 SpMV launches: returns
+    (launches, total_ms, rows) counted so far, then turns the probe on
+    (True, counters zeroed) or off (False), or leaves it (None)."""
+    n, ms, rows = C.c_longlong(0), C.c_double(0.0), C.c_longlong(0)
+    check(_lib.lib().ms_spmv_probe(-1 if on is None else int(bool(on)), C.byref(n), C.byref(ms), C.byref(rows)))
+    return n.value, ms.value, rows.value
+
+
+def cell_ordering(row_ptr, col_idx, xc=None, rcm=False, colored=False, coarse=0):
+    """The device numbering of a cell graph (1-based CSR pattern): returns
+    (n2o, color, ncolors) with n2o[new] = old (0-based), as ms_system_create
+    builds it for gpu_reorder / DILU colouring / gpu_coarse."""
+    rp = np.ascontiguousarray(row_ptr, np.int32)
+    ci = np.ascontiguousarray(col_idx, np.int32)
+    n = len(rp) - 1
+    n2o, color, nc = np.zeros(n, np.int32), np.zeros(n, np.int32), C.c_int(0)
+    x = None if xc is None else dp(np.ascontiguousarray(xc, np.float64))
+    check(_lib.lib().ms_cell_ordering(n, ip(rp), ip(ci), x, int(rcm), int(colored), int(coarse), ip(n2o), ip(color),
+                                      C.byref(nc)))
+    return n2o, color, nc.value
 
 
 def coarse_aggregates(xc, k):

@@ -79,6 +79,20 @@ if [ -f "$LIBDIR/libmixsolver_hip.so" ]; then
   done
   "$FC" -cpp -O2 -c "$LIBDIR/fortran/mod_uns_linsolver_mpi.f90" -o "$D/mod_uns_linsolver_mpi.o"
   DOBJS+=("$D/mod_uns_linsolver_mpi.o")
+  # the loop-level drop-in: the case adapter, compiled against the
+  # reference's modules, and the partitioned-driver glue, type-checked
+  # against the local-mesh components it reads (the reference's own
+  # mod_uns_local_mesh needs MPI); case_dump writes what the adapter hands
+  # to the library, for tests/test_oracle_golden.py
+  "$FC" -cpp -O2 -c "$LIBDIR/fortran/mixsolver_case.f90" -o "$D/mixsolver_case.o"
+  DOBJS+=("$D/mixsolver_case.o")
+  mkdir -p "$D/glue"
+  ( cd "$D/glue" && "$FC" -cpp -O2 -I"$D" -c "$HERE/../tests/dropin/local_mesh_iface.f90" \
+      && "$FC" -cpp -O2 -I"$D" -c "$LIBDIR/fortran/mod_uns_simple_mpi.f90" )
+  "$FC" -cpp -O2 -c "$HERE/../tests/dropin/case_dump.f90" -o "$D/case_dump.o"
+  "$FC" -O2 -o "$OUT/case_dump" "$D/case_dump.o" "${DOBJS[@]}" -L"$LIBDIR" -lmixsolver_hip \
+    -Wl,-rpath,'$ORIGIN/../../mixsolver_amd'
+  echo "built $OUT/case_dump"
   "$FC" -cpp -O2 -c "$HERE/ref_harness.f90" -o "$D/ref_harness.o"
   "$FC" -O2 -o "$OUT/ref_dropin" "$D/ref_harness.o" "${DOBJS[@]}" -L"$LIBDIR" -lmixsolver_hip \
     -Wl,-rpath,'$ORIGIN/../../mixsolver_amd'

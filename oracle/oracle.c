@@ -123,8 +123,21 @@ static double dot3(const double *a, const double *b) {
   return s;
 }
 
+/* OR_THREADS builds (liboracle_omp.so, the CPU baseline's all-core timing
+ * only): the matrix products, dot products and vector updates of the Krylov
+ * solvers run on OpenMP threads.  The dot products then sum in thread
+ * order, so only the serial build is bit-exact against the reference. */
+#ifdef OR_THREADS
+#define OR_PAR_FOR _Pragma("omp parallel for schedule(static)")
+#define OR_PAR_SUM_S _Pragma("omp parallel for schedule(static) reduction(+:s)")
+#else
+#define OR_PAR_FOR
+#define OR_PAR_SUM_S
+#endif
+
 static double dotn(const double *a, const double *b, int n) {
   double s = 0.0;
+  OR_PAR_SUM_S
   for (int i = 0; i < n; i++) s = s + a[i] * b[i];
   return s;
 }
@@ -1242,6 +1255,7 @@ void or_outflow_mass_rescale(or_case *cs) {
  * interface pressure model */
 static void ppe_assembly_impl(or_case *cs, double *val, double *rhs, int mpi);
 void or_ppe_assembly(or_case *cs, double *val, double *rhs) { ppe_assembly_impl(cs, val, rhs, 0); }
+void or_ppe_assembly_variant(or_case *cs, double *val, double *rhs, int mpi) { ppe_assembly_impl(cs, val, rhs, mpi); }
 static void ppe_assembly_impl(or_case *cs, double *val, double *rhs, int mpi) {
   const or_ctrl_t *ct = &cs->ctrl;
   for (int k = 0; k < cs->nnz; k++) val[k] = 0.0;
@@ -1334,6 +1348,7 @@ void or_correct_fields(or_case *cs, const double *pp, double *dpp) {
 
 void or_csr_matvec(int n, const int *rp, const int *ci, const double *v,
                    const double *x, double *y) {
+  OR_PAR_FOR
   for (int i = 0; i < n; i++) {
     y[i] = 0.0;
     for (int k = rp[i]; k <= rp[i + 1] - 1; k++) y[i] = y[i] + v[k - 1] * x[ci[k - 1] - 1];
@@ -1541,14 +1556,19 @@ static void pcg(int n, const int *rp, const int *ci, const double *v, const doub
     pap = dotn(p, ap, n);
     if (fabs(pap) < TINY) { *ierr = 2; break; }
     alpha = rz / pap;
+    OR_PAR_FOR
     for (int i = 0; i < n; i++) { x[i] = x[i] + alpha * p[i]; r[i] = r[i] - alpha * ap[i]; }
     *res = vnorm(r, n, mpi) / bnrm;
     if (*res <= tol) goto done;
     if (ic0) ic0_apply(n, rp, ci, L, dp, r, z, w);
-    else for (int i = 0; i < n; i++) z[i] = r[i] / diag[i];
+    else {
+      OR_PAR_FOR
+      for (int i = 0; i < n; i++) z[i] = r[i] / diag[i];
+    }
     rz_new = dotn(r, z, n);
     if (fabs(rz) < TINY) { *ierr = 2; break; }
     beta = rz_new / rz;
+    OR_PAR_FOR
     for (int i = 0; i < n; i++) p[i] = z[i] + beta * p[i];
     rz = rz_new;
   }
@@ -1745,6 +1765,11 @@ int or_piso_run(or_case *cs, int nsteps, int mpi) {
   int n_steps = ct->n_time_max > 1 ? ct->n_time_max : 1;
   if (nsteps > 0) n_steps = nsteps;
   double flux_ref = flux_reference(cs);
+  /* ts_order on entry: 2 after a restored BDF2 dump.  The serial piso_run
+   * keeps such a first step second order (mod_uns_simple.f90:380-414);
+   * piso_run_mpi takes a first-order first step either way
+   * (mod_uns_simple_mpi.f90:393-397). */
+  const int ts_order_in = cs->ts_order;
   for (int c = 0; c < nc; c++) cs->apc[c] = 1.0;
   or_compute_gradients(cs);
   or_flux_rhiechow(cs);
@@ -1756,7 +1781,9 @@ int or_piso_run(or_case *cs, int nsteps, int mpi) {
     memcpy(cs->T_old, cs->T, nc * 8);
     memcpy(cs->T_s_old_old, cs->T_s_old, nc * 8);
     memcpy(cs->T_s_old, cs->T_s, nc * 8);
-    cs->ts_order = (it == 1 || ct->time_scheme == 1) ? 1 : 2;
+    if (ct->time_scheme == 1) cs->ts_order = 1;
+    else if (it == 1 && (mpi || ts_order_in < 2)) cs->ts_order = 1;
+    else cs->ts_order = 2;
     int itl_max = 0, itl, brk;
     double ressum = 0.0;
     momentum_solves(cs, val, rhs, ap, ucomp, mpi, &itl_max, &ressum, &brk);

@@ -102,6 +102,8 @@ void or_set_cztype(or_case *, const int *cztype);
 void or_flux_rhiechow(or_case *);
 void or_outflow_mass_rescale(or_case *);
 void or_ppe_assembly(or_case *, double *val, double *rhs);
+/* mpi = 1: ppe_assembly_mpi at one partition (mod_uns_simple_mpi.f90:882-968) */
+void or_ppe_assembly_variant(or_case *, double *val, double *rhs, int mpi);
 void or_correct_fields(or_case *, const double *pp, double *dpp);
 
 /* drivers.  mpi = 1 selects the partitioned-path norms sqrt(gdot) and

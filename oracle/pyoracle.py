@@ -18,9 +18,13 @@ _I = C.POINTER(C.c_int)
 
 
 def lib():
+    """The oracle library; with ORACLE_THREADS=1 in the environment the
+    OpenMP build (liboracle_omp.so: the Krylov vector work on threads, for
+    the CPU baseline's all-core timing; not bit-exact)."""
     global _lib
     if _lib is None:
-        path = os.path.join(_HERE, "liboracle.so")
+        name = "liboracle_omp.so" if os.environ.get("ORACLE_THREADS") == "1" else "liboracle.so"
+        path = os.path.join(_HERE, name)
         if not os.path.exists(path):
             raise RuntimeError(f"oracle library missing: {path} (run make -C oracle)")
         L = C.CDLL(path)
@@ -46,6 +50,7 @@ def lib():
         L.or_solid_temperature_assembly.argtypes = [C.c_void_p, _D, _D, _D]
         L.or_set_cztype.argtypes = [C.c_void_p, _I]
         L.or_ppe_assembly.argtypes = [C.c_void_p, _D, _D]
+        L.or_ppe_assembly_variant.argtypes = [C.c_void_p, _D, _D, C.c_int]
         L.or_correct_fields.argtypes = [C.c_void_p, _D, _D]
         L.or_simple_run.argtypes = [C.c_void_p, C.c_int, C.c_int, _I]
         L.or_piso_run.argtypes = [C.c_void_p, C.c_int, C.c_int]
@@ -187,9 +192,10 @@ class OracleCase:
         lib().or_solid_temperature_assembly(self.h, _dp(val), _dp(rhs), _dp(ap))
         return val, rhs, ap
 
-    def ppe_assembly(self):
+    def ppe_assembly(self, mpi=False):
+        """ppe_assembly (serial) or ppe_assembly_mpi at one partition (mpi)."""
         val, rhs = np.zeros(self.nnz), np.zeros(self.nc)
-        lib().or_ppe_assembly(self.h, _dp(val), _dp(rhs))
+        lib().or_ppe_assembly_variant(self.h, _dp(val), _dp(rhs), int(mpi))
         return val, rhs
 
     def flux_rhiechow(self):

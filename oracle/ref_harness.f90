@@ -10,6 +10,9 @@
 !   simple : seed fields from in.bin (u,p,T), run simple_run(nsteps=n)
 !   piso   : seed fields from in.bin (u,p,T), run piso_run (n_time_max steps)
 !   pimple : seed fields from in.bin (u,p,T), run pimple_run (n_time_max steps)
+!   pisor  : a restored state from in.bin (u,p,T, u_old, u_old_old, T_old,
+!            T_old_old, then ts_order as int), T_s and its history = T's, then
+!            piso_run (n_time_max steps): the first step of a restarted run
 !   funcs  : seed fields from in.bin, then compute_gradients, momentum_assembly
 !            (comp 1..3), flux_rhiechow, temperature_assembly, correct_fields
 !   funcs2 : funcs with T_s / T_s_old / T_s_old_old seeded too, preceded by the
@@ -56,6 +59,7 @@ program ref_harness
    type(bc_t)     :: bcs
    type(fields_t) :: fld
    integer :: ier, nsteps, uo, ui, nargs
+   integer(8) :: t0, t1, trate
 
    nargs = command_argument_count()
    call get_command_argument(1, casf)
@@ -92,17 +96,35 @@ program ref_harness
       call dump_geom()
    case ( 'simple' )
       call seed_fields()
+      call system_clock( t0, trate )
       call simple_run( m, c, g, ctrl, bcs, fld, ier, nsteps )
+      call system_clock( t1 )
+      write(*,'(a,f12.3)') 'HARNESS_DRIVER_SECONDS ', real(t1 - t0, dp) / real(trate, dp)
       write(uo) ier
       call dump_state()
    case ( 'piso' )
       call seed_fields()
+      call system_clock( t0, trate )
       call piso_run( m, c, g, ctrl, bcs, fld, '/tmp/ref_harness_piso', ier )
+      call system_clock( t1 )
+      write(*,'(a,f12.3)') 'HARNESS_DRIVER_SECONDS ', real(t1 - t0, dp) / real(trate, dp)
       write(uo) ier
       call dump_state()
    case ( 'pimple' )
       call seed_fields()
       call pimple_run( m, c, g, ctrl, bcs, fld, '/tmp/ref_harness_pimple', ier )
+      write(uo) ier
+      call dump_state()
+   case ( 'pisor' )
+      open( newunit=ui, file=trim(inf), access='stream', form='unformatted', &
+            status='old', action='read' )
+      read(ui) fld%u, fld%p, fld%T, fld%u_old, fld%u_old_old, fld%T_old, fld%T_old_old
+      read(ui) fld%ts_order
+      close( ui )
+      fld%T_s = fld%T
+      fld%T_s_old = fld%T_old
+      fld%T_s_old_old = fld%T_old_old
+      call piso_run( m, c, g, ctrl, bcs, fld, '/tmp/ref_harness_pisor', ier )
       write(uo) ier
       call dump_state()
    case ( 'forces' )

@@ -1,4 +1,5 @@
 import os
+import re
 import sys
 
 import numpy as np
@@ -111,3 +112,71 @@ def nan_guard_groups(case, nc):
     ifin, f, c, v, kind = put({("T_s", kc1 - 1): np.nan})                 # group 5: T_s scanned
     assert (ifin, f, c) == (1, "T_s", kc1)
     assert put()[0] == 0                                                    # group 6: stateless
+
+
+# The per-step / per-iteration lines the reference's drivers print:
+# piso_run / pimple_run (mod_uns_simple.f90:519-523, 771-775) and
+# simple_run (:249-265).
+_STEP_RE = re.compile(r"step=\s*(\d+)\s+t=\s*(\S+)\s+mass-imbal=\s*(\S+)\s+du/dt=\s*(\S+)\s+max\|u\|=\s*(\S+)"
+                      r"\s+CFL~=\s*(\S+)\s+lin-it=\s*(\d+)\s+lin-res=\s*(\S+)")
+_IT_RE = re.compile(r"\bit=\s*(\d+)\s+mass-imbal=\s*(\S+)\s+du_max=\s*(\S+)\s+dT_max=\s*(\S+)"
+                    r"\s+lin-it=\s*(\d+)\s+lin-res=\s*(\S+)")
+
+
+def printed_history(log):
+    """Rows of the reference's printed diagnostics as (iteration, dict of
+    printed strings keyed by the history columns imbal, du, col2 (max|u|
+    for time steps, dT_max for SIMPLE), cfl, lin_it, lin_res, with their
+    Fortran edit descriptors)."""
+    rows = []
+    for ln in str(log).splitlines():
+        m = _STEP_RE.search(ln)
+        if m:
+            g = m.groups()
+            rows.append((int(g[0]), dict(imbal=(g[2], 9, 2), du=(g[3], 9, 2), col2=(g[4], 9, 2),
+                                         cfl=(g[5], 9, 2), lin_it=int(g[6]), lin_res=(g[7], 9, 2))))
+            continue
+        m = _IT_RE.search(ln)
+        if m:
+            g = m.groups()
+            rows.append((int(g[0]), dict(imbal=(g[1], 10, 3), du=(g[2], 10, 3), col2=(g[3], 10, 3),
+                                         lin_it=int(g[4]), lin_res=(g[5], 9, 2))))
+    return rows
+
+
+def fortran_es(v, w, d):
+    """Fortran ESw.d output of v (exponent of two digits, as flang writes for
+    |exponent| < 100)."""
+    return f"{v:.{d}E}".rjust(w).strip()
+
+
+HIST_COLS = dict(imbal=0, du=1, col2=2, cfl=3, lin_it=4, lin_res=5)
+
+
+def check_printed_history(hist, log, exact=True, skip=()):
+    """Compare a driver history (rows of imbal du col2 cfl lin_it lin_res ...)
+    with the printed lines: the same strings when exact, else within the
+    printed precision (one unit in the last printed digit) and lin-it within
+    two (the largest iteration count of the step's solves: BiCGSTAB's
+    residual is not monotone, so reduction-order differences near the
+    tolerance move its stopping iteration by up to two).  Returns the number
+    of printed rows checked."""
+    rows = printed_history(log)
+    assert rows, "no printed diagnostics in the fixture"
+    for it, vals in rows:
+        h = hist[it - 1]
+        for key, pv in vals.items():
+            if key in skip:
+                continue
+            got = h[HIST_COLS[key]]
+            if key == "lin_it":
+                assert (int(got) == pv) if exact else abs(int(got) - pv) <= 2, (it, key, got, pv)
+                continue
+            text, w, d = pv
+            if exact:
+                assert fortran_es(got, w, d) == text, (it, key, got, text)
+            else:
+                ref = float(text)
+                ulp = 10.0 ** (int(f"{ref:.{d}E}".split("E")[1]) - d) if ref != 0.0 else 0.0
+                assert abs(got - ref) <= 1.01 * ulp, (it, key, got, text)
+    return len(rows)

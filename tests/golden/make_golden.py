@@ -184,6 +184,13 @@ def write_fields(path, *arrs):
             np.ascontiguousarray(a).tofile(f)
 
 
+def driver_log(text):
+    """The diagnostic lines the reference's drivers print (SIMPLE '  it=',
+    PISO / PIMPLE '  step=' and the '    Tmin=' line after each step)."""
+    keep = ("  it=", "  step=", "    Tmin=")
+    return "\n".join(ln for ln in text.splitlines() if ln.startswith(keep))
+
+
 def case_run(name, m, zone_kind, ctl, mode, nsteps, fields=None, want_ts=False):
     with tempfile.TemporaryDirectory() as td:
         cas.write_cas(os.path.join(td, "m.cas"), m, zone_kind)
@@ -202,7 +209,30 @@ def case_run(name, m, zone_kind, ctl, mode, nsteps, fields=None, want_ts=False):
                    apc=r.take(np.float64, nc), gt=r.take(np.float64, 3 * nc).reshape(nc, 3))
         if want_ts:
             out["T_s"] = r.take(np.float64, nc)
+        # the reference's printed per-iteration / per-step diagnostics
+        out["log"] = np.array(driver_log(log))
         return out, log
+
+
+def restart_fixture():
+    """Serial piso_run from a restored BDF2 state (ts_order = 2 with u_old /
+    u_old_old and T_old / T_old_old given): mod_uns_simple.f90:408-414 keep
+    the first step second order."""
+    mc = channel_box(12, 8, 4)
+    zkc = {4: "symmetry", 5: "wall", 6: "velocity-inlet", 7: "outflow"}
+    ctl = CHANNEL_CTL.format(mu=0.05, dt=0.02, nsteps=2, blend=0.5, precond="ic0")
+    u0, p0, T0 = seeded_fields(mc, 181, umag=0.3)
+    u0[:, 0] += 1.0
+    rng = np.random.default_rng(182)
+    u1 = u0 + rng.uniform(-0.02, 0.02, u0.shape)
+    u2 = u1 + rng.uniform(-0.02, 0.02, u0.shape)
+    T1 = T0 + rng.uniform(-0.05, 0.05, T0.shape)
+    T2 = T1 + rng.uniform(-0.05, 0.05, T0.shape)
+    out, log = case_run("chanrs", mc, zkc, ctl, "pisor", 2,
+                        (u0, p0, T0, u1, u2, T1, T2, np.array([2], np.int32)), want_ts=True)
+    save("channel_piso_restart", control=ctl, nsteps=np.int64(2), **mesh_arrays(mc),
+         in_u=u0, in_p=p0, in_T=T0, in_u_old=u1, in_u_old_old=u2, in_T_old=T1, in_T_old_old=T2,
+         in_ts_order=np.int64(2), **{"out_" + k: v for k, v in out.items()})
 
 
 def geom_run(m, zone_kind, ctl):
@@ -455,6 +485,49 @@ def cavity_known_answer():
     xc = pyoracle.OracleCase(m, ctrl, groups).geometry()["xc"].reshape(-1, 3)
     l2, _ = ghia_compare(xc, u, 1000)
     return m, ctl, int(ier), float(np.linalg.norm(u, axis=1).max()), float(l2)
+
+
+def cavity128_fixture():
+    """Config 1: the Re = 1000 lid-driven cavity on 128 x 128 x 1 uniform hex
+    cells (16,384), plain FP64 steady SIMPLE to outer_tol = 1e-8 (PCG-Jacobi
+    p', BiCGSTAB-ILU(0) momentum, under-relaxation 0.7 / 0.3), run by the
+    reference's simple_run until it converges.  Keeps the iteration count,
+    the full final fields and the printed diagnostics."""
+    m = meshgen.cavity(128)
+    zk = {4: "symmetry", 5: "wall"}
+    ctl = CAVITY_CTL.format(blend=0.0, nonorth=1.0, precond="jacobi").replace("outer_max = 1000",
+                                                                             "outer_max = 20000")
+    out, log = case_run("cav128", m, zk, ctl, "simple", 20000)
+    its = [int(ln.split()[0]) for ln in (l.split("CONVERGED after ")[1] for l in log.splitlines()
+                                           if "CONVERGED after" in l)]
+    save("cavity128_simple_c1", control=ctl, **mesh_arrays(m), iters=np.int64(its[0] if its else -1),
+         **{"out_" + k: v for k, v in out.items() if k in ("ier", "u", "p", "flux", "log")})
+
+
+def channel2m_twin_fixture():
+    """Config 2's non-periodic twin at full size: the 400 x 100 x 50 channel
+    (2,000,000 hex cells, tanh wall clustering 1.1) with a parabolic inlet,
+    outflow and z symmetry, seeded with the benchmark's initial field
+    (laminar parabola + 5% uniform noise, seed 180), one PISO step of the
+    reference's piso_run (2 correctors, BDF1 first step, p' PCG-Jacobi and
+    momentum BiCGSTAB-ILU(0) to 1e-10).  The fields are too large to keep,
+    so the fixture holds a signature: the printed step line, the norms of
+    u, p and flux, and u, p at 1,000 seeded probe cells."""
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__)))))
+    import bench
+    nx, ny, nz = 400, 100, 50
+    m = meshgen.channel(nx, ny, nz)
+    u0 = bench.initial_field(m, ny, nz, nx)
+    ctl = bench.control_text(nx, periodic=False) + "n_out_every = 1000000\n"
+    zk = {4: "symmetry", 5: "wall", 6: "velocity-inlet", 7: "outflow"}
+    out, log = case_run("chan2m", m, zk, ctl, "piso", 1, (u0, np.zeros(m.ncells), np.zeros(m.ncells)))
+    t_ref = [float(ln.split()[1]) for ln in log.splitlines() if ln.startswith("HARNESS_DRIVER_SECONDS")]
+    probes = np.sort(np.random.default_rng(2000).choice(m.ncells, 1000, replace=False))
+    save("channel2m_twin_signature", control=ctl, nx=np.int64(nx), ny=np.int64(ny), nz=np.int64(nz),
+         out_ier=out["ier"], out_log=out["log"], probes=probes, probe_u=out["u"][probes], probe_p=out["p"][probes],
+         norm_u=np.float64(np.linalg.norm(out["u"])), norm_p=np.float64(np.linalg.norm(out["p"])),
+         norm_flux=np.float64(np.linalg.norm(out["flux"])), sum_p=np.float64(out["p"].sum()),
+         ref_driver_seconds=np.float64(t_ref[0] if t_ref else -1.0))
 
 
 REF_BJ = "/root/reference/cluster_cases/bj_porous"
@@ -993,9 +1066,16 @@ if __name__ == "__main__":
         save("cylinder_re40_known_answer", **cylinder_known_answer())
     elif sys.argv[1:] == ["--hc"]:
         save("heat_conduction_known_answer", **heat_conduction_known_answer())
+    elif sys.argv[1:] == ["--c2"]:
+        channel2m_twin_fixture()
+    elif sys.argv[1:] == ["--c1"]:
+        cavity128_fixture()
+    elif sys.argv[1:] == ["--restart"]:
+        restart_fixture()
     elif sys.argv[1:] == ["--bj"]:
         mb, ob = bj_known_answer()
         save("bj_porous_known_answer", **mesh_arrays(mb), **ob, **BJ_README)
     else:
         main()
         porous_fixtures()
+        restart_fixture()

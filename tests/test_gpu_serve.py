@@ -114,3 +114,41 @@ def test_server_round_graphs_equal_launches(precond, coarse):
     assert stats[True]["completed"] == stats[False]["completed"] == len(reqs)
     print("rounds/launches eager", stats[False]["rounds"], stats[False]["launches"], "graphs",
           stats[True]["rounds"], stats[True]["launches"], flush=True)
+
+
+def test_server_stall_guard_retires_request():
+    """With a stall window of one check (16 iterations), a Jacobi request on
+    a 195k-cell mesh, whose residual does not halve in every round (FP64 CG
+    on the host: the round ending at iteration 96 does not; 333 iterations
+    to 1e-10), is retired early with ierr = 3, its iterate and that
+    iterate's true residual; with the default window the same request
+    converges, and an FP64 finish from the returned iterate (ms_solve, x as
+    the initial guess) meets the tolerance."""
+    from mixsolver_amd import linsolver
+    from mixsolver_amd.solver import set_stall_window
+    P = LaplacianPattern(request_mesh(200000, "hex", 8))
+    rng = np.random.default_rng(42)
+    val, b = P.values(rng), P.rhs(rng)
+    A = sp.csr_matrix((val, P.col_idx - 1, P.row_ptr - 1), shape=(P.n, P.n))
+    out = {}
+    try:
+        for window in (1, 0):
+            set_stall_window(window)
+            srv = Server(precond="jacobi", coarse=0, round_iters=16)
+            pid = srv.add_pattern(P.row_ptr, P.col_idx, P.xc)
+            out[window] = srv.result(srv.submit(pid, val, b, tol=1e-10, maxit=5000), timeout_ms=-1)
+            srv.close()
+    finally:
+        set_stall_window(0)
+    st, ok = out[1], out[0]
+    print("stalled it", st["it"], "res", st["res"], "converged it", ok["it"], flush=True)
+    assert st["ierr"] == 3 and st["it"] < ok["it"]
+    true_res = np.linalg.norm(b - A @ st["x"]) / np.linalg.norm(b)
+    assert st["res"] > 1e-10 and abs(true_res - st["res"]) <= 1e-6 * true_res
+    # (no direct solve at this size: SuperLU takes minutes on a 195k-cell 3-D Laplacian)
+    assert ok["ierr"] == 0 and ok["res"] <= 1e-10
+    assert np.linalg.norm(b - A @ ok["x"]) / np.linalg.norm(b) <= 1e-10
+    x, it, res, ierr = linsolver.solve(P.row_ptr, P.col_idx, val, b, st["x"], 1e-10, 5000, solver="pcg",
+                                       precond="jacobi", precision="fp64")
+    assert ierr == 0 and res <= 1e-10
+    assert np.linalg.norm(b - A @ x) / np.linalg.norm(b) <= 1e-10

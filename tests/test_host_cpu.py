@@ -135,6 +135,120 @@ def test_coarse_aggregates_balanced_deterministic():
     assert ext.max() < 4.0 * (8.0 / 64)
 
 
+def _cell_graph(m):
+    """1-based CSR pattern (diagonal included) of a mesh's cell graph and the
+    interior face pairs (0-based)."""
+    inner = m.c1 > 0
+    a, b = m.c0[inner] - 1, m.c1[inner] - 1
+    n = m.ncells
+    r = np.concatenate([a, b, np.arange(n)])
+    c = np.concatenate([b, a, np.arange(n)])
+    key = np.unique(r.astype(np.int64) * n + c)
+    r, c = key // n, key % n
+    rp = np.zeros(n + 1, np.int64)
+    np.add.at(rp, r + 1, 1)
+    return (np.cumsum(rp) + 1).astype(np.int32), (c + 1).astype(np.int32), a, b
+
+
+def _bandwidth(a, b, o2n):
+    return int(np.abs(o2n[a] - o2n[b]).max())
+
+
+@pytest.mark.parametrize("mesh", ["hex", "prism"])
+@pytest.mark.parametrize("rcm", [False, True])
+@pytest.mark.parametrize("colored,coarse", [(False, 0), (True, 0), (True, 16), (False, 16)])
+def test_cell_ordering_invariants(mesh, rcm, colored, coarse):
+    """The device numbering: a permutation; with colouring no two neighbours
+    share a colour and the classes are contiguous and ascending; with a
+    coarse space every aggregate is contiguous inside each class (the same
+    aggregates as the bisection alone); deterministic."""
+    from mixsolver_amd.solver import cell_ordering, coarse_aggregates
+    m = (meshgen.channel(16, 8, 6) if mesh == "hex" else
+         meshgen.cylinder_prisms(ntheta=40, nr=12, layers=2, jitter=0.15, seed=3))
+    rp, ci, a, b = _cell_graph(m)
+    xc = _centroids(m)
+    n = m.ncells
+    n2o, color, ncol = cell_ordering(rp, ci, xc, rcm=rcm, colored=colored, coarse=coarse)
+    assert np.array_equal(np.sort(n2o), np.arange(n))
+    o2n = np.empty(n, np.int64)
+    o2n[n2o] = np.arange(n)
+    if colored:
+        assert ncol >= 2 and color.max() == ncol - 1
+        assert np.all(color[a] != color[b])
+        assert np.all(np.diff(color[n2o]) >= 0)
+    else:
+        assert ncol == 1 and not color.any()
+    if coarse:
+        agg = coarse_aggregates(xc, coarse)
+        cls = color[n2o]
+        for c in range(ncol):
+            seq = agg[n2o][cls == c]
+            # contiguous: each aggregate appears as one run inside the class
+            runs = seq[np.r_[True, seq[1:] != seq[:-1]]]
+            assert len(runs) == len(np.unique(seq)), c
+    again = cell_ordering(rp, ci, xc, rcm=rcm, colored=colored, coarse=coarse)
+    assert np.array_equal(again[0], n2o) and np.array_equal(again[1], color)
+
+
+def _centroids(m):
+    """Cell centroids as the mean of the cells' face centroids (enough for a
+    geometric split)."""
+    x = m.x.reshape(-1, 3)
+    fc = np.array([x[m.fnodes[m.fptr[f]:m.fptr[f + 1]] - 1].mean(axis=0) for f in range(m.nfaces)])
+    s = np.zeros((m.ncells, 3))
+    cnt = np.zeros(m.ncells)
+    for side in (m.c0, m.c1):
+        ok = side > 0
+        np.add.at(s, side[ok] - 1, fc[ok])
+        np.add.at(cnt, side[ok] - 1, 1)
+    return s / cnt[:, None]
+
+
+def test_greedy_colouring_of_a_box_is_the_checkerboard():
+    """A structured hex box is a bipartite cell graph: greedy colouring in the
+    natural order gives exactly two colours, (i + j + k) mod 2."""
+    from mixsolver_amd.solver import cell_ordering
+    nx, ny, nz = 7, 5, 4
+    m = meshgen.hex_box(np.linspace(0, 1, nx + 1), np.linspace(0, 1, ny + 1), np.linspace(0, 1, nz + 1),
+                        zone_ids=dict(xmin=5, xmax=5, ymin=5, ymax=5, zmin=5, zmax=5))
+    rp, ci, a, b = _cell_graph(m)
+    n2o, color, ncol = cell_ordering(rp, ci, colored=True)
+    idx = np.arange(m.ncells)
+    i, j, k = idx % nx, (idx // nx) % ny, idx // (nx * ny)
+    assert ncol == 2 and np.array_equal(color, (i + j + k) % 2)
+    # classes in ascending order, natural order inside each class
+    assert np.array_equal(n2o, np.r_[idx[color == 0], idx[color == 1]])
+
+
+def test_rcm_recovers_a_narrow_band_from_a_shuffled_mesh():
+    """RCM on a randomly shuffled channel mesh brings the matrix bandwidth
+    from about n back to the order of one cross-section (the structured
+    numbering's band is nx*ny = 128 cells here); it starts from a
+    minimum-degree cell (the last of the reversed order)."""
+    from mixsolver_amd.solver import cell_ordering
+    m = meshgen.channel(16, 8, 6)
+    n = m.ncells
+    rp0, ci0, a0, b0 = _cell_graph(m)
+    perm = np.random.default_rng(9).permutation(n)        # old -> shuffled id
+    a, b = perm[a0], perm[b0]
+    r = np.concatenate([a, b, np.arange(n)])
+    c = np.concatenate([b, a, np.arange(n)])
+    key = np.unique(r.astype(np.int64) * n + c)
+    rr, cc = key // n, key % n
+    rp = np.zeros(n + 1, np.int64)
+    np.add.at(rp, rr + 1, 1)
+    rp, ci = (np.cumsum(rp) + 1).astype(np.int32), (cc + 1).astype(np.int32)
+    ident = np.arange(n)
+    before = _bandwidth(a, b, ident)
+    n2o, _, _ = cell_ordering(rp, ci, rcm=True)
+    o2n = np.empty(n, np.int64)
+    o2n[n2o] = np.arange(n)
+    after = _bandwidth(a, b, o2n)
+    assert before > n // 2 and after <= 2 * 16 * 8, (before, after)
+    deg = np.diff(rp) - 1
+    assert deg[n2o[-1]] == deg.min()
+
+
 def test_coarse_aggregates_rejects_bad_k():
     from mixsolver_amd._lib import MixSolverError
     from mixsolver_amd.solver import coarse_aggregates

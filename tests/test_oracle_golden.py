@@ -5,10 +5,14 @@ Every fixture under tests/golden was written by the reference code itself
 oracle/ref_harness.f90).  The oracle restates the same arithmetic in the same
 order, so every comparison here is bit-exact (np.array_equal).
 """
+import ctypes as C
+import os
+
 import numpy as np
 import pytest
 
-from conftest import POROUS_CASES, load_golden, mesh_from, run_funcs2
+from conftest import IFACE_CASES, POROUS_CASES, check_printed_history, load_golden, mesh_from, run_funcs2, \
+    seed_interface, iface_mesh
 from mixsolver_amd.control import parse_control
 from oracle import pyoracle
 
@@ -52,6 +56,189 @@ def test_driver_bitexact(name, mode):
     assert ier == int(d["out_ier"])
     for k in ("u", "p", "flux", "T", "gp", "gu", "apc", "gt"):
         assert np.array_equal(oc.get(k).reshape(d["out_" + k].shape), d["out_" + k]), k
+    # the per-iteration / per-step diagnostics the reference printed, digit for digit
+    assert check_printed_history(oc.history(), d["out_log"], exact=True) >= 1
+
+
+def _seed_restart(case, d):
+    for k in ("u", "p", "T", "u_old", "u_old_old", "T_old", "T_old_old"):
+        case.set_field(k, d["in_" + k])
+    case.set_field("T_s", d["in_T"])
+    case.set_field("T_s_old", d["in_T_old"])
+    case.set_field("T_s_old_old", d["in_T_old_old"])
+
+
+def test_piso_restart_keeps_bdf2_serial_and_bdf1_partitioned():
+    """A restored BDF2 state (ts_order 2): the serial piso_run keeps the
+    first step second order (mod_uns_simple.f90:380-414), bit-exact against
+    the reference run of that state; piso_run_mpi takes a first-order first
+    step either way (mod_uns_simple_mpi.f90:393-397), so its first step
+    equals a cold first step of the same state, where the history levels are
+    not read."""
+    d = load_golden("channel_piso_restart")
+    oc = make_case(d)
+    _seed_restart(oc, d)
+    oc.ts_order = 2
+    assert oc.piso_run(int(d["nsteps"]), mpi=False) == int(d["out_ier"])
+    for k in ("u", "p", "flux", "T", "gp", "gu", "apc", "gt"):
+        assert np.array_equal(oc.get(k).reshape(d["out_" + k].shape), d["out_" + k]), k
+    check_printed_history(oc.history(), d["out_log"], exact=True)
+    # the partitioned driver: restored and cold first steps are identical,
+    # and differ from the serial restored step
+    a = make_case(d)
+    _seed_restart(a, d)
+    a.ts_order = 2
+    assert a.piso_run(1, mpi=True) == 0
+    b = make_case(d)
+    for k in ("u", "p", "T"):
+        b.set_field(k, d["in_" + k])
+    b.set_field("T_s", d["in_T"])
+    assert b.piso_run(1, mpi=True) == 0
+    c = make_case(d)
+    _seed_restart(c, d)
+    c.ts_order = 2
+    assert c.piso_run(1, mpi=False) == 0
+    for k in ("u", "p", "flux", "T"):
+        assert np.array_equal(a.get(k), b.get(k)), k
+    assert not np.array_equal(a.get("u"), c.get("u"))
+
+
+@pytest.mark.parametrize("name", IFACE_CASES)
+def test_ppe_assembly_mpi_reduces_to_pinned_serial(name):
+    """ppe_assembly_mpi (mod_uns_simple_mpi.f90:882-968) tests the pressure-
+    Dirichlet type of a face only (POUTLET / FARFIELD), where the serial
+    ppe_assembly (mod_uns_simple.f90:1552-1644) also counts interface faces
+    under the interface pressure model (bc_face_p_dirichlet,
+    mod_uns_bc.f90:879-893).  So the partitioned matrix with the model on
+    equals the serial matrix with the model off (interface faces Neumann,
+    cell 1 pinned when nothing else is Dirichlet), bit for bit; the serial
+    assembly is pinned by the reference's SIMPLE runs of these cases."""
+    d = load_golden(name)
+    ctrl, groups = parse_control(str(d["control"]), warn=False)
+    cases = []
+    for p_dir, mpi in ((None, True), (False, False), (None, False)):
+        oc = pyoracle.OracleCase(iface_mesh(d), ctrl, groups)
+        seed_interface(oc, d)
+        for k in ("u", "p", "T", "flux", "apc"):
+            oc.set_field(k, d["ts0_in_" + k])
+        if p_dir is not None:
+            sw = [bool(v) for v in d["switches"]]
+            oc.set_interface_modes(p_dir=p_dir, slip=sw[1], zhang=sw[2], T_ltne=sw[3])
+        cases.append(oc.ppe_assembly(mpi=mpi))
+    (vm, rm), (vs, rs), (v0, r0) = cases
+    assert np.array_equal(vm, vs) and np.array_equal(rm, rs)
+    if bool(d["switches"][0]):       # the model on: the serial variant differs
+        assert not np.array_equal(vm, v0)
+    else:
+        assert np.array_equal(vm, v0) and np.array_equal(rm, r0)
+
+
+@pytest.mark.parametrize("name", ["solvers_spd", "solvers_nonsym"])
+@pytest.mark.parametrize("kind", ["cg_jacobi", "cg_ic0", "bicgstab_ilu0"])
+def test_partitioned_solvers_follow_serial_histories(name, kind):
+    """The _mpi solvers at one partition (mod_uns_linsolver_mpi.f90:155-363)
+    run the serial recurrences with sqrt(dot_product) in place of norm2 and a
+    breakdown test on the IC0 diagonal: their residual histories equal the
+    reference's serial histories (maxit = k re-calls) to rounding, with the
+    same iteration counts and codes."""
+    d = load_golden(name)
+    key = kind
+    kmax = int(d["kmax"])
+    for k in range(1, kmax + 1):
+        x, it, res, ierr = pyoracle.solve(kind, d["row_ptr"], d["col_idx"], d["val"], d["b"], d["x0"],
+                                          float(d["tol"]), k, mpi=True)
+        ref = d[f"{key}_hist_res"][k - 1]
+        assert abs(res - ref) <= 1e-12 * abs(ref) + 1e-300, (k, res, ref)
+        assert it == int(d[f"{key}_hist_it"][k - 1]) and ierr == int(d[f"{key}_hist_ierr"][k - 1]), k
+    x, it, res, ierr = pyoracle.solve(kind, d["row_ptr"], d["col_idx"], d["val"], d["b"], d["x0"],
+                                      float(d["tol"]), int(d["maxit"]), mpi=True)
+    assert it == int(d[f"{key}_it"]) and ierr == int(d[f"{key}_ierr"])
+    assert np.linalg.norm(x - d[f"{key}_x"]) <= 1e-12 * np.linalg.norm(d[f"{key}_x"])
+
+
+def _piso_mpi_composed(oc, ctrl, nsteps):
+    """piso_run_mpi at one partition (mod_uns_simple_mpi.f90:376-497) written
+    out as the sequence of the per-function routines it calls, each pinned
+    bit for bit against the reference (momentum_assembly, the _mpi solvers,
+    compute_gradients, flux_rhiechow, outflow_mass_rescale, ppe_assembly_mpi,
+    correct_fields, temperature_assembly): apc = 1, gradients and Rhie-Chow
+    once; per step the history shift, a first-order first step (:393-397),
+    momentum solves with apc = the last ap, gradients, Rhie-Chow, outflow
+    rescale, n_correct PPE solves and corrections, the temperature solve, and
+    the solid phase under LTNE only (:483-495).  The driver has no
+    zero_solid_velocity call."""
+    t = oc.topology()
+    rp, ci = t["row_ptr"], t["col_idx"]
+    tol, mx = ctrl.lin_tol, ctrl.lin_max
+    oc.set_field("apc", np.ones(oc.nc))
+    oc.compute_gradients()
+    oc.flux_rhiechow()
+    for it in range(1, nsteps + 1):
+        for a, b in (("u_old_old", "u_old"), ("u_old", "u"), ("T_old_old", "T_old"), ("T_old", "T"),
+                     ("T_s_old_old", "T_s_old"), ("T_s_old", "T_s")):
+            oc.set_field(a, oc.get(b))
+        oc.ts_order = 1 if (it == 1 or ctrl.time_scheme == 1) else 2
+        for comp in (1, 2, 3):
+            val, rhs, ap = oc.momentum_assembly(comp)
+            u = oc.get("u").reshape(-1, 3).copy()
+            x, _, _, ierr = pyoracle.solve("bicgstab_ilu0", rp, ci, val, rhs, u[:, comp - 1], tol, mx, mpi=True)
+            assert ierr != 2
+            u[:, comp - 1] = x
+            oc.set_field("u", u)
+        oc.set_field("apc", ap)
+        oc.compute_gradients()
+        oc.flux_rhiechow()
+        oc.outflow_mass_rescale()
+        for _ in range(ctrl.n_correct):
+            val, rhs = oc.ppe_assembly(mpi=True)
+            kind = "cg_ic0" if ctrl.ppe_precond == 1 else "cg_jacobi"
+            pp, _, _, ierr = pyoracle.solve(kind, rp, ci, val, rhs, np.zeros(oc.nc), tol, mx, mpi=True)
+            assert ierr != 2
+            oc.correct_fields(pp)
+        val, rhs, ap = oc.temperature_assembly()
+        T, _, _, ierr = pyoracle.solve("bicgstab_ilu0", rp, ci, val, rhs, oc.get("T"), tol, mx, mpi=True)
+        oc.set_field("T", T)
+        if ctrl.thermal_model != 0:
+            val, rhs, ap = oc.solid_temperature_assembly()
+            Ts, _, _, ierr = pyoracle.solve("bicgstab_ilu0", rp, ci, val, rhs, oc.get("T_s"), tol, mx, mpi=True)
+            oc.set_field("T_s", Ts)
+
+
+@pytest.mark.parametrize("name", ["porous_solid_cht", "porous_plug_ltne"])
+def test_partitioned_piso_is_the_composition_of_pinned_routines(name):
+    """The oracle's piso_run_mpi equals, bit for bit, the sequence of pinned
+    per-function routines that mod_uns_simple_mpi.f90:376-497 calls.  On
+    porous_solid_cht (a solid block, no LTNE) this fixes the two lines where
+    the partitioned driver departs from the serial one: no
+    zero_solid_velocity, so the solid cells keep their momentum-solve
+    velocities until correct_fields, and no solid-phase solve, so T_s stays
+    as it was; the serial step (pinned by the reference run of this case)
+    does both.  porous_plug_ltne runs the LTNE solid phase."""
+    from mixsolver_amd import zones
+    d = load_golden(name)
+    ctrl, groups = parse_control(str(d["control"]), warn=False)
+
+    def fresh():
+        oc = pyoracle.OracleCase(mesh_from(d), ctrl, groups)
+        for k, f in (("u", "in_u"), ("p", "in_p"), ("T", "in_T"), ("T_s", "in_T")):
+            oc.set_field(k, d[f])
+        return oc
+
+    drv, comp = fresh(), fresh()
+    assert drv.piso_run(2, mpi=True) == 0
+    _piso_mpi_composed(comp, ctrl, 2)
+    for k in ("u", "p", "flux", "T", "T_s", "gp", "apc"):
+        assert np.array_equal(drv.get(k), comp.get(k)), k
+    if name == "porous_solid_cht":
+        assert ctrl.thermal_model == 0
+        solid = zones.resolve_cell_zones(mesh_from(d), ctrl.cell_zones) == 3
+        assert solid.any()
+        assert np.array_equal(drv.get("T_s"), d["in_T"])
+        ser = fresh()
+        assert ser.piso_run(2, mpi=False) == int(d["piso_ier"])
+        assert np.array_equal(ser.get("u").reshape(d["piso_u"].shape), d["piso_u"])
+        assert np.array_equal(ser.get("T_s"), d["piso_T_s"])
+        assert not np.array_equal(ser.get("T_s"), d["in_T"])
 
 
 @pytest.mark.parametrize("name", ["funcs_prism", "funcs_outlet", "funcs_channel"])
@@ -518,3 +705,90 @@ def test_channel_reference_run_matches_readme():
     ub = float(d["ref_u_bulk_target"])
     assert np.abs(d["ref_out_u"] - 6 * ub * eta * (1 - eta)).max() / ub == pytest.approx(
         float(d["ref_profile_err_max"]), rel=1e-12)
+
+
+# ------------------------------------------------ loop-level Fortran drop-in
+CASE_DUMP = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "_ref", "case_dump")
+
+
+@pytest.mark.skipif(not os.path.exists(CASE_DUMP), reason="the reference build (oracle/build_ref.sh) is absent")
+@pytest.mark.parametrize("name,zk", [
+    ("channel_piso_jacobi", {4: "symmetry", 5: "wall", 6: "velocity-inlet", 7: "outflow"}),
+    ("porous_solid_cht", {4: "symmetry", 5: "wall", 6: "velocity-inlet", 7: "outflow"}),
+    ("iface_pdir_slip_ltne", {4: "symmetry", 5: "wall", 6: "interface", 7: "outflow"}),
+    ("cavity16_simple_ic0_blend", {4: "symmetry", 5: "wall"})])
+def test_loop_dropin_case_conversion(name, zk, tmp_path):
+    """mixsolver_case.f90 (the loop-level drop-in's adapter) turns the
+    reference's own mesh_t / ctrl_t / bc_t / fields_t, set up by the
+    reference's read_cas, read_control, resolve_cell_zones, build_bc,
+    init_fields and setup_porous_fields, into the same ms_ctrl_t, boundary
+    groups, mesh arrays, cell types and porous coefficients the Python
+    front end builds from the same files (the device then recomputes
+    uspeed, lid_tol and tbc_tol from the mesh)."""
+    import subprocess
+    from mixsolver_amd import cas, zones
+    from mixsolver_amd.control import BC_VINLET_PARAB, MsBcGroup, MsCtrl, groups_array, with_interface_groups
+    d = load_golden(name)
+    m0 = mesh_from(d)
+    if name.startswith("iface"):
+        m0 = iface_mesh(d)
+    cas.write_cas(str(tmp_path / "m.cas"), m0, zk)
+    ctl = str(d["control"])
+    (tmp_path / "c.control").write_text(ctl)
+    r = subprocess.run([CASE_DUMP, "m.cas", "c.control", "out.bin"], cwd=tmp_path, capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    raw = (tmp_path / "out.bin").read_bytes()
+    nn, nf, nc, nb = np.frombuffer(raw, np.int32, 4)
+    o = 16
+    fc = MsCtrl.from_buffer_copy(raw[o:o + C.sizeof(MsCtrl)])
+    o += C.sizeof(MsCtrl)
+    fg = []
+    for _ in range(nb):
+        fg.append(MsBcGroup.from_buffer_copy(raw[o:o + C.sizeof(MsBcGroup)]))
+        o += C.sizeof(MsBcGroup)
+
+    def take(dt, n):
+        nonlocal o
+        a = np.frombuffer(raw, dt, n, o)
+        o += a.nbytes
+        return a
+
+    x, fptr = take(np.float64, 3 * nn), take(np.int64, nf + 1)
+    fnodes, c0, c1, zone = take(np.int32, int(fptr[-1])), take(np.int32, nf), take(np.int32, nf), take(np.int32, nf)
+    has_cz = take(np.int32, 1)[0]
+    cz = take(np.int32, nc) if has_cz else None
+    por, perm, inert, ks, qs = (take(np.float64, nc), take(np.float64, 3 * nc), take(np.float64, nc),
+                                take(np.float64, nc), take(np.float64, nc))
+    # the Python front end on the same files
+    m = cas.read_cas(str(tmp_path / "m.cas"))
+    ctrl, groups = parse_control(ctl, warn=False)
+    groups = with_interface_groups(m, ctrl, groups)
+    assert nb == len(groups) == ctrl.nbc == fc.nbc
+    for fname, _ in MsCtrl._fields_:
+        a, b = getattr(fc, fname), getattr(ctrl, fname)
+        if hasattr(a, "__len__"):
+            assert np.array_equal(np.ctypeslib.as_array(a), np.ctypeslib.as_array(b)), fname
+        else:
+            assert a == b, fname
+    garr = groups_array(groups)
+    for i in range(nb):
+        for fname, _ in MsBcGroup._fields_:
+            if fname in ("uspeed", "lid_tol", "tbc_tol"):
+                continue
+            if fname.startswith("prfl_") and garr[i].btype != BC_VINLET_PARAB:
+                continue      # read only for parabolic inlets
+            a, b = getattr(fg[i], fname), getattr(garr[i], fname)
+            if hasattr(a, "__len__"):
+                assert np.array_equal(np.ctypeslib.as_array(a), np.ctypeslib.as_array(b)), (i, fname)
+            else:
+                assert a == b, (i, fname)
+    assert np.array_equal(x, np.ravel(m.x)) and np.array_equal(fptr, m.fptr)
+    for a, b in ((fnodes, m.fnodes), (c0, m.c0), (c1, m.c1), (zone, m.zone)):
+        assert np.array_equal(a, b)
+    if ctrl.cell_zones:
+        czp = zones.resolve_cell_zones(m, ctrl.cell_zones)
+        f = zones.setup_porous_fields(m, ctrl.cell_zones, czp)
+        assert has_cz and np.array_equal(np.maximum(cz, 1), czp)
+        assert np.array_equal(por, f["porosity"]) and np.array_equal(perm, np.ravel(f["perm_dir"]))
+        assert np.array_equal(inert, f["inertial"]) and np.array_equal(ks, f["k_s"])
+        assert np.array_equal(qs, f["q_src"])
