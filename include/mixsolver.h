@@ -311,6 +311,19 @@ int ms_cg_ic0(int n, const int *row_ptr, const int *col_idx, const double *val,
 int ms_bicgstab_ilu0(int n, const int *row_ptr, const int *col_idx, const double *val,
                      const double *b, double *x, double tol, int maxit,
                      int *it, double *res, int *ierr);
+/* cg_jacobi that allocates nothing.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and results as ms_cg_jacobi (cg_jacobi_mpi recurrences,
+ * it = maxit+1 / ierr = 1 on exhaustion).  The matrix, vectors, reduction
+ * partials and solver scalars are staged in one fixed device array per GPU
+ * (about 20 MB, held until the process ends): n <= 131072 and
+ * nnz <= 1048576, larger systems return MS_EINVAL.  ms_init must have run on
+ * the current device.  Calls share that GPU's array: calls that overlap on
+ * one GPU race. */
+int ms_cg_jacobi_noalloc(int n, const int *row_ptr, const int *col_idx, const double *val,
+                         const double *b, double *x, double tol, int maxit,
+                         int *it, double *res, int *ierr);
 /* Extended solver entry (no reference counterpart: the north-star solver
  * options).  solver 0 PCG / 1 BiCGSTAB; precond MS_PC_*; precision
  * MS_FP64 / MS_MIXED; inner_tol for MS_MIXED.  res is the FP64 true

@@ -79,6 +79,7 @@ def lib():
         "ms_outflow_mass_rescale": (C.c_int, [_V]),
         "ms_ppe_assembly": (C.c_int, [_V, _D, _D]),
         "ms_correct_fields": (C.c_int, [_V, _D, _D]),
+        "ms_cg_jacobi_noalloc": (C.c_int, [C.c_int, _I, _I, _D, _D, _D, C.c_double, C.c_int, _I, _D, _I]),
         "ms_cg_jacobi": (C.c_int, [C.c_int, _I, _I, _D, _D, _D, C.c_double, C.c_int, _I, _D, _I]),
         "ms_cg_ic0": (C.c_int, [C.c_int, _I, _I, _D, _D, _D, C.c_double, C.c_int, _I, _D, _I]),
         "ms_bicgstab_ilu0": (C.c_int, [C.c_int, _I, _I, _D, _D, _D, C.c_double, C.c_int, _I, _D, _I]),

@@ -23,6 +23,8 @@ void call_rhiechow(System &s);
 void call_outflow(System &s);
 void call_ppe(System &s);
 void call_correct(System &s, const double *d_pp);
+void cg_jacobi_noalloc(int n, const int *row_ptr, const int *col_idx, const double *val, const double *b, double *x,
+                       double tol, int maxit, int *it, double *res, int *ierr);
 }  // namespace ms
 
 using namespace ms;
@@ -598,6 +600,13 @@ int ms_cg_ic0(int n, const int *rp, const int *ci, const double *v, const double
 int ms_bicgstab_ilu0(int n, const int *rp, const int *ci, const double *v, const double *b, double *x, double tol,
                      int maxit, int *it, double *res, int *ierr) {
   return guarded([&] { solve_csr(n, rp, ci, v, b, x, tol, maxit, 1, MS_PC_NATURAL, MS_FP64, 0.0, it, res, ierr); });
+}
+int ms_cg_jacobi_noalloc(int n, const int *rp, const int *ci, const double *v, const double *b, double *x,
+                         double tol, int maxit, int *it, double *res, int *ierr) {
+  return guarded([&] {
+    if (!it || !res || !ierr) fail(MS_EINVAL, "ms_cg_jacobi_noalloc: null output");
+    cg_jacobi_noalloc(n, rp, ci, v, b, x, tol, maxit, it, res, ierr);
+  });
 }
 int ms_solve(int n, const int *rp, const int *ci, const double *v, const double *b, double *x, double tol, int maxit,
              int solver, int precond, int precision, double inner_tol, int *it, double *res, int *ierr) {

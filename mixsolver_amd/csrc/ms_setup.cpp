@@ -37,6 +37,8 @@ static int g_device = -1;
 void set_error(const std::string &m) { g_err = m; }
 void fail(int code, const std::string &m) { throw Error{code, m}; }
 const char *last_error() { return g_err.c_str(); }
+void noalloc_register_device(int device);   // ms_noalloc.hip
+
 
 void ensure_runtime(int device) {
   if (g_stream && (device < 0 || device == g_device)) return;
@@ -49,6 +51,7 @@ void ensure_runtime(int device) {
   if (g_stream) MS_HIP(hipStreamDestroy(g_stream));
   MS_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
   g_device = device;
+  noalloc_register_device(device);
 }
 
 hipStream_t stream() {

@@ -40,6 +40,13 @@ def cg_jacobi(row_ptr, col_idx, val, b, x0, tol, maxit):
     return _call("ms_cg_jacobi", row_ptr, col_idx, val, b, x0, tol, maxit)
 
 
+def cg_jacobi_noalloc(row_ptr, col_idx, val, b, x0, tol, maxit):
+    """cg_jacobi through the entry point that allocates nothing: the system
+    is staged in the library's fixed per-GPU scratch array (at most
+    131072 rows and 1048576 nonzeros; larger systems raise)."""
+    return _call("ms_cg_jacobi_noalloc", row_ptr, col_idx, val, b, x0, tol, maxit)
+
+
 def cg_ic0(row_ptr, col_idx, val, b, x0, tol, maxit):
     return _call("ms_cg_ic0", row_ptr, col_idx, val, b, x0, tol, maxit)
 
