@@ -185,9 +185,11 @@ def cpu_baseline(n_needed, full=False):
     cap = 0 if full else 40
     legs = {}
     for threads in (1, ncores):
-        env = dict(os.environ, OMP_NUM_THREADS=str(threads), ORACLE_THREADS="1" if threads > 1 else "0")
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads), ORACLE_THREADS="1" if threads > 1 else "0",
+                   ORACLE_PROGRESS="1" if full else "0")
+        # the leg's stderr (progress of a full step) goes straight through
         r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-leg", str(cap)], env=env,
-                           capture_output=True, text=True, check=True)
+                           stdout=subprocess.PIPE, text=True, check=True)
         leg = json.loads(r.stdout.strip().splitlines()[-1])
         per_it = leg["t_ppe"] / max(leg["iters"], 1)
         t_step = leg["t_total"] if full else (leg["t_total"] - leg["t_ppe"]) + per_it * n_needed
@@ -311,6 +313,7 @@ def main():
                        ("fp64", dict(base, gpu_precision=0))):
         if name not in variants:
             continue
+        print(f"bench: {name}: set-up and {args.warmup} + {args.steps} steps", file=sys.stderr, flush=True)
         probe_here = name == lead_name and name.startswith("mixed") and args.probe == "timed"
         res = run_variant(Case, mesh, ctl, u0, over, args.steps, args.warmup, probe=probe_here)
         cs, t, lat, h = res[:4]
@@ -486,6 +489,12 @@ def main():
     if args.serve_seconds > 0:
         # the request server's coarse inverse runs in one workgroup (k <= 256)
         out["serving"] = serve_bench(args.serve_rate, args.serve_seconds, args.serve_pool, min(args.coarse, 256))
+        # the same stream at half the offered rate, below saturation: the
+        # steady-state latency (the offered-rate run above saturates one GPU,
+        # so its latency grows with the run length; its throughput is the
+        # saturation throughput)
+        out["serving_below_saturation"] = serve_bench(args.serve_rate / 2, args.serve_seconds, args.serve_pool,
+                                                      min(args.coarse, 256))
         print("bench: serving done", file=sys.stderr, flush=True)
     if not args.no_cpu:
         # the oracle runs the non-periodic twin (the reference has no periodic

@@ -177,6 +177,14 @@ struct FinResidual {
 // the coarse space is active (kept separate so the one-level path is
 // unchanged bit for bit).
 __device__ __forceinline__ double rz_total(const KS *st, double a) { return st->coarse ? a + st->cnu : a; }
+// r.z formed by k_pcg_p_split waits in st->rz_next (st->rzp = 1) until the
+// next finisher that reads st->rz takes it over.
+__device__ __forceinline__ void take_rz(KS *st) {
+  if (st->rzp) {
+    st->rz = st->rz_next;
+    st->rzp = 0;
+  }
+}
 
 struct FinNop {
   KS *st;
@@ -192,6 +200,7 @@ struct FinPap {
   KS *st;
   __device__ bool skip() const { return st->done != 0; }
   template <class T> __device__ void operator()(const T *a) const {
+    take_rz(st);
     st->it = st->it + 1;
     st->pap = a[0];
     if (fabs((double)a[0]) < kTiny) {
@@ -278,10 +287,39 @@ struct FinUpdateRU {
     }
   }
 };
+// FinUpdateRU for k_finish_split (Jacobi): the same residual checks; the
+// Jacobi part of r.z goes to st->rzd, and beta is formed by k_pcg_p_split
+// once the coarse parts c.nu are in.
+struct FinUpdateSplit {
+  KS *st;
+  int trigger;
+  __device__ bool skip() const { return st->done != 0; }
+  template <class T> __device__ void operator()(const T *a) const {
+    const double rn = sqrt((double)a[0]);
+    st->res = rn * st->sigma / st->bnrm;
+    if (trigger) {
+      if (st->res <= st->tol) {
+        st->ru = 1;
+        st->done = 2;
+        return;
+      }
+      if (rn < st->delta * st->rmax) st->ru = 1;
+    }
+    if (rn > st->rmax) st->rmax = rn;
+    if (!trigger) st->ru = 0;
+    if (fabs(st->rz) < kTiny) {
+      st->ierr = 2;
+      st->done = 1;
+    } else {
+      st->rzd = (double)a[1];
+    }
+  }
+};
 struct FinRU {         // after x += sigma*y and r = b - A x in FP64
   KS *st;
   __device__ bool skip() const { return st->done == 1 || st->ru != 1; }
   template <class T> __device__ void operator()(const T *a) const {
+    take_rz(st);
     const double rt = sqrt((double)a[0]);
     st->rr = a[0];
     st->res = rt / st->bnrm;
@@ -480,15 +518,6 @@ __global__ void k_dilu_setup(int r0, int r1, Mat A, const long long *tpos, const
 // the scalar sums as k_finish, then c_a = sum of the last partial over the
 // chunks of aggregate a (fixed order), nu = E^-1 c and c.nu, before the
 // functor runs on thread 0.
-// Workgroups are dispatched round-robin over the 8 XCDs (blockIdx % 8).
-// xcd_block maps them so that each XCD works on one contiguous range of
-// logical blocks; the neighbour gathers of its rows then mostly hit its own
-// L2 instead of being fetched again by every XCD.  A bijection of [0, nb).
-constexpr int kXcd = 8;
-__device__ __forceinline__ int xcd_block(int b, int nb) {
-  const int q = nb / kXcd, r = nb % kXcd, x = b % kXcd;
-  return x * q + min(x, r) + b / kXcd;
-}
 
 // Coarse solve riding along a DILU forward-sweep launch: nwg extra
 // workgroups of kBlock threads, each recomputing c = Z^T r from the chunk
@@ -547,9 +576,10 @@ __device__ void coarse_ride(const CoWork<T> &cw, int g) {
 // t / (kFinThreads / Q)) each sum every Q-th chunk partial (the last of the
 // NV values) and one Q-th of the E^-1 columns; the parts are combined in q
 // order.  Writes nu and co.st->cnu.
+// c = Z^T r into c[0, k) (LDS) by the whole workgroup, Q threads per
+// aggregate as described above; ends with the workgroup synchronised.
 template <int NV, class T>
-__device__ void coarse_solve(const T *partial, const CoDev &co) {
-  __shared__ double c[kMaxAgg], partq[kFinThreads];
+__device__ void coarse_c(const T *partial, const CoDev &co, double *c, double *partq) {
   const int Q = co.k <= kFinThreads / 4 ? 4 : (co.k <= kFinThreads / 2 ? 2 : 1);
   const int span = kFinThreads / Q;
   const int a = threadIdx.x % span, q = threadIdx.x / span;
@@ -578,6 +608,15 @@ __device__ void coarse_solve(const T *partial, const CoDev &co) {
     c[a] = s;
   }
   __syncthreads();
+}
+
+template <int NV, class T>
+__device__ void coarse_solve(const T *partial, const CoDev &co) {
+  __shared__ double c[kMaxAgg], partq[kFinThreads];
+  coarse_c<NV, T>(partial, co, c, partq);
+  const int Q = co.k <= kFinThreads / 4 ? 4 : (co.k <= kFinThreads / 2 ? 2 : 1);
+  const int span = kFinThreads / Q;
+  const int a = threadIdx.x % span, q = threadIdx.x / span;
   const int kq = (co.k + Q - 1) / Q, b0 = q * kq, b1 = min(co.k, b0 + kq);
   if (a < co.k) {
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;    // E^-1 is symmetric: column a
@@ -629,6 +668,50 @@ __global__ __launch_bounds__(kFinThreads) void k_finish_co(int nparts, const T *
   block_reduce<OpSum, NV, T, kFinThreads>(acc, lds);
   if (co.k > 0) coarse_solve<NV, T>(partial, co);
   if (threadIdx.x == 0) fin(acc);
+}
+
+// Finisher of the mixed Jacobi PCG with a coarse space, spread over 1 + G
+// workgroups (G = ceil(k / kSplitRows)) so the coarse solve no longer runs
+// in one workgroup: workgroup 0 reduces the update's scalar partials and
+// runs the functor; workgroup g >= 1 recomputes c = Z^T r from the chunk
+// partials, forms rows [kSplitRows (g-1), kSplitRows g) of nu = E^-1 c (one
+// wavefront per row, E^-1 read along its rows) and writes the c.nu sum of
+// its rows to cnu_part[g-1].  k_pcg_p_split adds the parts (fixed order).
+constexpr int kSplitRows = kFinThreads / 64;
+template <int NV, class T, class Fin>
+__global__ __launch_bounds__(kFinThreads) void k_finish_split(int nparts, const T *partial, Fin fin, CoDev co,
+                                                              double *cnu_part) {
+  if (fin.skip()) return;
+  if (blockIdx.x == 0) {
+    T acc[NV];
+    fin_gather<OpSum, NV, T>(nparts, partial, acc);
+    __shared__ T lds[NV * (kFinThreads / 64)];
+    block_reduce<OpSum, NV, T, kFinThreads>(acc, lds);
+    if (threadIdx.x == 0) fin(acc);
+    return;
+  }
+  __shared__ double c[kMaxAgg], partq[kFinThreads], pr[kSplitRows];
+  coarse_c<NV, T>(partial, co, c, partq);
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int a = (blockIdx.x - 1) * kSplitRows + w;
+  double prod = 0.0;
+  if (a < co.k) {
+    double s = 0.0;
+    for (int b = lane; b < co.k; b += 64) s = s + (double)co.Einv[(size_t)a * co.k + b] * c[b];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s = s + __shfl_down(s, o, 64);
+    if (lane == 0) {
+      co.nu[a] = s;
+      prod = s * c[a];
+    }
+  }
+  if (lane == 0) pr[w] = prod;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int j = 0; j < kSplitRows; j++) t = t + pr[j];
+    cnu_part[blockIdx.x - 1] = t;
+  }
 }
 
 __global__ void k_fill64(long long n, double *v, double x) {

@@ -112,6 +112,33 @@ __global__ void k_pcg_p(Rows R, const T *r, const T *z, const T *dg, int jac, T 
   }
 }
 
+// p = r/d + nu_agg + beta p after k_finish_split: every workgroup forms
+// r.z = rzd + the coarse parts (fixed order) and beta = r.z / rz; workgroup 0
+// leaves r.z in st->rz_next for the next finisher.
+template <class T>
+__global__ void k_pcg_p_split(Rows R, const T *r, const T *dg, T *p, const double *nu, KS *st,
+                              const double *cnu_part, int G) {
+  if (st->done) return;
+  __shared__ double beta_s;
+  if (threadIdx.x == 0) {
+    double rzn = st->rzd;
+    for (int g = 0; g < G; g++) rzn = rzn + cnu_part[g];
+    const double beta = rzn / st->rz;
+    beta_s = beta;
+    if (blockIdx.x == 0) {
+      st->rz_next = rzn;
+      st->rzp = 1;
+      st->beta = beta;
+    }
+  }
+  __syncthreads();
+  int i, a;
+  if (R.get(i, a)) {
+    const T zi = r[i] / dg[i] + (T)nu[a];
+    p[i] = zi + (T)beta_s * p[i];
+  }
+}
+
 template <class T>
 __global__ void k_dot(int n, const T *a, const T *b, const KS *st, T *part) {
   if (st->done) return;
@@ -1389,7 +1416,18 @@ SolveResult Krylov::mixed(const double *val, const double *b, double *x, double 
   // preconditioner, r.z and p update.  DILU finishes r.r and r.z in one
   // kernel after the sweeps (trigger: reliable-update checks active).
   const bool merged = mode == 2 && (!coarse || fused);
+  // Jacobi with a coarse space: the coarse solve is spread over the split
+  // finisher's workgroups and beta is formed in the p update
+  const bool split = coarse && !fused && mode == 1;
+  const int nsplit = split ? (pat->coarse.k + kSplitRows - 1) / kSplitRows : 0;
+  if (split && cnu_split_.n < (size_t)nsplit) cnu_split_.alloc(nsplit);
   auto tail = [&](int trigger) {
+    if (split) {
+      hipLaunchKernelGGL((k_finish_split<3, float, FinUpdateSplit>), dim3(1 + nsplit), dim3(kFinThreads), 0, S,
+                         (int)GR.x, part32, FinUpdateSplit{st, trigger}, co, cnu_split_.p);
+      hipLaunchKernelGGL(k_pcg_p_split<float>, GR, B, 0, S, RW, r, dg, p, nu, st, cnu_split_.p, nsplit);
+      return;
+    }
     if (merged) {
       const int np = dilu_sweeps<float>(pat, A, v32, dg, r, z, done, prz, cw);
       hipLaunchKernelGGL((k_finish2<float, FinUpdRzRU>), dim3(1), dim3(kFinThreads), 0, S, (int)GR.x, part32, np,

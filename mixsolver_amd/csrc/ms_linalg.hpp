@@ -10,6 +10,8 @@ struct KS {              // device-resident solver scalars (one per solve)
   double sigma, scale;   // FP32 iteration runs on r/sigma; scale = sigma_old/sigma_new at an update
   double cnu;            // coarse part of r.z: (Z^T r) . E^-1 (Z^T r)
   double tol;
+  double rzd, rz_next;   // split coarse finisher: Jacobi part of r.z, r.z waiting to become rz
+  int rzp;               // rz_next pending
   int it, ierr, done, maxit;  // done: 0 running, 1 finished, 2 paused for a reliable update
   int coarse;            // coarse correction active
   int ru;                // reliable update requested for this iteration
@@ -85,6 +87,7 @@ class Krylov {
   DBuf<double> r_, r0_, p_, q_, z_, s_, t_, ph_, sh_, fac_, dg_, part_, prz_;   // prz_: DILU sweep partials
   DBuf<float> val32_, r32_, r032_, p32_, q32_, z32_, s32_, t32_, ph32_, sh32_, x32_, fac32_, dg32_, part32_, prz32_;
   DBuf<KS> st_, st32_;
+  DBuf<double> cnu_split_;   // c.nu parts of k_finish_split
   DBuf<int> flag_;
   void *hpin_ = nullptr;
   int nblk_;

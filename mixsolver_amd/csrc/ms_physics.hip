@@ -190,19 +190,42 @@ __device__ __forceinline__ double kink_pf(const DMesh &m, const double *p, const
 __device__ __forceinline__ int face_flags(const DMesh &m, int f) { return m.has_face_flags ? m.fflag[f] : 0; }
 __device__ __forceinline__ bool solid_cell(const DFields &F, int r) { return F.cztype && F.cztype[r] == CZ_SOLID; }
 
+// Row of this thread in the cell-wise assembly kernels (-1: none).  On a
+// two-colour numbering each workgroup takes kBlock/2 rows of colour 0 and
+// the kBlock/2 rows of colour 1 at the same position of their colour's block
+// (the same aggregate, the same place in it), so most of a cell's neighbours
+// are rows of its own workgroup and its neighbour gathers hit the cache; the
+// workgroups are spread over the XCDs in contiguous ranges.  Each row's
+// arithmetic does not depend on the mapping.
+__device__ __forceinline__ int cell_row(const DMesh &m) {
+  const int b = xcd_block(blockIdx.x, gridDim.x), t = threadIdx.x;
+  if (m.pair_n0 > 0) {
+    constexpr int h = kBlock / 2;
+    if (t < h) {
+      const int i = b * h + t;
+      return i < m.pair_n0 ? i : -1;
+    }
+    const int i = b * h + (t - h);
+    return i < m.nc - m.pair_n0 ? m.pair_n0 + i : -1;
+  }
+  const int r = b * kBlock + t;
+  return r < m.nc ? r : -1;
+}
+
 // ------------------------------------------------------------- gradients
 __global__ void k_gradients(DMesh m, DFields F, Ramp rp, int do_T) {
-  const int r = blockIdx.x * kBlock + threadIdx.x;
-  if (r >= m.nc) return;
+  const int r = cell_row(m);
+  if (r < 0) return;
   double gp[3] = {0, 0, 0}, gu[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, gt[3] = {0, 0, 0}, gs[3] = {0, 0, 0};
   const double *uP = &F.u[3 * (size_t)r];
   for (int k = m.cf_ptr[r]; k < m.cf_ptr[r + 1]; k++) {
     const int e = m.cf_ent[k], f = e >> 1, side = e & 1;
-    const int a = m.c0[f], b = m.c1[f];
-    const double *sf = &m.sf[3 * (size_t)f];
+    const int o = m.e_oth[k];
+    const int a = side == 0 ? r : o, b = side == 0 ? o : r;   // c0 and c1 of the face (b < 0 on the boundary)
+    const double *sf = &m.e_sf[3 * (size_t)k];
     double phip, phiu[3], phit = 0.0, phis = 0.0;
     if (b >= 0) {
-      const double l = m.lf[f];
+      const double l = m.e_lf[k];
       if (face_flags(m, f) & FF_KINK) phip = kink_pf(m, F.p, F.gp_prev, f, a, b);
       else phip = l * F.p[a] + (1.0 - l) * F.p[b];
       for (int c = 0; c < 3; c++) phiu[c] = l * F.u[3 * (size_t)a + c] + (1.0 - l) * F.u[3 * (size_t)b + c];
@@ -247,8 +270,8 @@ __global__ void k_gradients(DMesh m, DFields F, Ramp rp, int do_T) {
 // ------------------------------------------------ Barth-Jespersen limiter
 // comp 0..2: velocity component (velocity_limiter); comp 3: temperature.
 __global__ void k_limiter(DMesh m, DFields F, Ramp rp, int comp, double *psi) {
-  const int r = blockIdx.x * kBlock + threadIdx.x;
-  if (r >= m.nc) return;
+  const int r = cell_row(m);
+  if (r < 0) return;
   const bool isT = comp == 3;
   const double uc = isT ? F.T[r] : F.u[3 * (size_t)r + comp];
   double pmin = uc, pmax = uc;
@@ -292,8 +315,8 @@ struct MomParams {
 
 __global__ void k_momentum(DMesh m, DFields F, Ramp rp, MomParams q, const double *psi, double *val, double *rhs_out,
                            double *ap_out) {
-  const int r = blockIdx.x * kBlock + threadIdx.x;
-  if (r >= m.nc) return;
+  const int r = cell_row(m);
+  if (r < 0) return;
   const int comp = q.comp;
   const int k0 = m.cf_ptr[r], k1 = m.cf_ptr[r + 1];
   double ap = 0.0, rhs = 0.0;
@@ -302,10 +325,10 @@ __global__ void k_momentum(DMesh m, DFields F, Ramp rp, MomParams q, const doubl
   // interior faces (:974-1120)
   for (int k = k0; k < k1; k++) {
     const int e = m.cf_ent[k], f = e >> 1, side = e & 1;
-    const int b = m.c1[f];
-    if (b < 0) continue;
-    const int a = m.c0[f];
-    const double Ff = F.flux[f], D = m.Dm[f], l = m.lf[f];
+    const int o = m.e_oth[k];
+    if (o < 0) continue;
+    const int a = side == 0 ? r : o, b = side == 0 ? o : r;   // c0 and c1 of the face
+    const double Ff = F.flux[f], D = m.e_Dm[k], l = m.e_lf[k];
     const long long sl = m.cf_slot[k];
     const int fl = face_flags(m, f);
     double Dface = D;
@@ -340,18 +363,18 @@ __global__ void k_momentum(DMesh m, DFields F, Ramp rp, MomParams q, const doubl
       double gf[3];
       for (int d = 0; d < 3; d++)
         gf[d] = l * F.gu[9 * (size_t)a + comp + 3 * d] + (1.0 - l) * F.gu[9 * (size_t)b + comp + 3 * d];
-      const double noc = q.nonorth * m.muf[f] * ddot3(gf, &m.tcor[3 * (size_t)f]);
+      const double noc = q.nonorth * m.e_muf[k] * ddot3(gf, &m.e_tcor[3 * (size_t)k]);
       rhs = side == 0 ? rhs + noc : rhs - noc;
     }
     const double pf = (fl & FF_KINK) ? kink_pf(m, F.p, F.gp, f, a, b) : l * F.p[a] + (1.0 - l) * F.p[b];
-    const double Sc = m.sf[3 * (size_t)f + comp];
+    const double Sc = m.e_sf[3 * (size_t)k + comp];
     rhs = side == 0 ? rhs - pf * Sc : rhs + pf * Sc;
   }
   // boundary faces (:1123-1227)
   const double *uP = &F.u[3 * (size_t)r];
   for (int k = k0; k < k1; k++) {
+    if (m.e_oth[k] >= 0) continue;
     const int f = m.cf_ent[k] >> 1;
-    if (m.c1[f] >= 0) continue;
     double uf[3];
     bc_face_vel(m, f, uP, uf, rp);
     const double pf = bc_face_p(m, f, F.p[r], rp);
@@ -466,8 +489,8 @@ __device__ __forceinline__ double disp_n(const DMesh &m, const DFields &F, int f
 // temperature_assembly (:1748-2037): LTE or the fluid phase of LTNE
 __global__ void k_energy(DMesh m, DFields F, EnParams q, const double *psi, double *val, double *rhs_out,
                          double *ap_out) {
-  const int r = blockIdx.x * kBlock + threadIdx.x;
-  if (r >= m.nc) return;
+  const int r = cell_row(m);
+  if (r < 0) return;
   const int k0 = m.cf_ptr[r], k1 = m.cf_ptr[r + 1];
   double ap = 0.0, rhs = 0.0;
   for (int k = k0; k < k1; k++)
@@ -581,8 +604,8 @@ __global__ void k_energy(DMesh m, DFields F, EnParams q, const double *psi, doub
 // solid_temperature_assembly (:2109-2296): the solid phase of LTNE and the
 // conduction in solid zones
 __global__ void k_solid_energy(DMesh m, DFields F, EnParams q, double *val, double *rhs_out, double *ap_out) {
-  const int r = blockIdx.x * kBlock + threadIdx.x;
-  if (r >= m.nc) return;
+  const int r = cell_row(m);
+  if (r < 0) return;
   const int k0 = m.cf_ptr[r], k1 = m.cf_ptr[r + 1];
   double ap = 0.0, rhs = 0.0;
   for (int k = k0; k < k1; k++)
@@ -740,15 +763,15 @@ __global__ void k_ppe_af(DMesh m, const double *apc, double rho, double *af) {
 }
 
 __global__ void k_ppe(DMesh m, const double *flux, const double *af, double *val, double *rhs_out) {
-  const int r = blockIdx.x * kBlock + threadIdx.x;
-  if (r >= m.nc) return;
+  const int r = cell_row(m);
+  if (r < 0) return;
   const int k0 = m.cf_ptr[r], k1 = m.cf_ptr[r + 1];
   for (int k = k0; k < k1; k++)
     if (m.cf_slot[k] >= 0) val[m.cf_slot[k]] = 0.0;
   double diag = 0.0, rhs = 0.0;
   for (int k = k0; k < k1; k++) {
     const int e = m.cf_ent[k], f = e >> 1, side = e & 1;
-    if (m.c1[f] < 0) {
+    if (m.e_oth[k] < 0) {
       if (p_dirichlet_ppe(m, f)) diag = diag + af[f];
       rhs = rhs - flux[f];
       continue;
@@ -766,9 +789,8 @@ __global__ void k_ppe(DMesh m, const double *flux, const double *af, double *val
       rhs = 0.0;
     } else {
       for (int k = k0; k < k1; k++) {
-        const int f = m.cf_ent[k] >> 1;
-        if (m.c1[f] < 0) continue;
-        const int o = (m.cf_ent[k] & 1) == 0 ? m.c1[f] : m.c0[f];
+        const int o = m.e_oth[k];
+        if (o < 0) continue;
         if (o == m.pin_cell) val[m.cf_slot[k]] = 0.0;
       }
     }
@@ -779,22 +801,23 @@ __global__ void k_ppe(DMesh m, const double *flux, const double *af, double *val
 
 // ------------------------------------------------------------- correction
 __global__ void k_correct_cells(DMesh m, DFields F, const double *pp, double *dpp_out, double palpha) {
-  const int r = blockIdx.x * kBlock + threadIdx.x;
-  if (r >= m.nc) return;
+  const int r = cell_row(m);
+  if (r < 0) return;
   double g[3] = {0.0, 0.0, 0.0};
   for (int k = m.cf_ptr[r]; k < m.cf_ptr[r + 1]; k++) {
     const int e = m.cf_ent[k], f = e >> 1, side = e & 1;
-    const int b = m.c1[f];
+    const int o = m.e_oth[k];
     double phi;
-    if (b >= 0) {
-      const double l = m.lf[f];
-      phi = l * pp[m.c0[f]] + (1.0 - l) * pp[b];
+    if (o >= 0) {
+      const double l = m.e_lf[k];
+      const int a = side == 0 ? r : o, b = side == 0 ? o : r;   // c0 and c1 of the face
+      phi = l * pp[a] + (1.0 - l) * pp[b];
     } else if (p_dirichlet(m, f)) {
       phi = 0.0;
     } else {
       phi = pp[r];
     }
-    const double *sf = &m.sf[3 * (size_t)f];
+    const double *sf = &m.e_sf[3 * (size_t)k];
     if (side == 0)
       for (int d = 0; d < 3; d++) g[d] = g[d] + phi * sf[d];
     else
@@ -930,6 +953,14 @@ __global__ void k_wake_end(DMesh m, DFields F, double x0, double *part) {
 // host drivers
 // ======================================================================
 static dim3 gc(int n) { return dim3(nblocks(n)); }
+// grid of the cell-wise assembly kernels (cell_row)
+static dim3 gcell(const System &s) {
+  if (s.pat.color_ptr.size() == 3) {
+    const int n0 = s.pat.color_ptr[1];
+    return dim3(nblocks(std::max(n0, s.nc - n0), kBlock / 2));
+  }
+  return gc(s.nc);
+}
 
 // The preconditioner of a solve: natural and multicolour selectors give the
 // level-scheduled factorization in the system's numbering (IC(0) under PCG,
@@ -950,7 +981,7 @@ struct Driver {
     if (s.has_kink)   // kink faces reconstruct with the gradient of the previous pass
       MS_HIP(hipMemcpyAsync(s.gp_prev.p, s.gp.p, 3 * (size_t)s.nc * sizeof(double), hipMemcpyDeviceToDevice,
                             stream()));
-    hipLaunchKernelGGL(k_gradients, gc(s.nc), dim3(kBlock), 0, stream(), s.dmesh(), s.dfields(), rp,
+    hipLaunchKernelGGL(k_gradients, gcell(s), dim3(kBlock), 0, stream(), s.dmesh(), s.dfields(), rp,
                        (int)s.energy_on());
     MS_CHECK_LAUNCH();
   }
@@ -969,7 +1000,7 @@ struct Driver {
   void momentum(int comp) {     // comp 0..2
     const ms_ctrl_t &c = s.ctrl;
     if (c.conv_blend > 0.0)
-      hipLaunchKernelGGL(k_limiter, gc(s.nc), dim3(kBlock), 0, stream(), s.dmesh(), s.dfields(), rp, comp, s.psi.p);
+      hipLaunchKernelGGL(k_limiter, gcell(s), dim3(kBlock), 0, stream(), s.dmesh(), s.dfields(), rp, comp, s.psi.p);
     MomParams q{};
     q.comp = comp;
     q.ts_order = s.ts_order;
@@ -986,25 +1017,25 @@ struct Driver {
     q.nonorth = c.nonorth_corr;
     q.grav = c.gravity[comp];
     q.body = c.body_force[comp];
-    hipLaunchKernelGGL(k_momentum, gc(s.nc), dim3(kBlock), 0, stream(), s.dmesh(), s.dfields(), rp, q, s.psi.p,
+    hipLaunchKernelGGL(k_momentum, gcell(s), dim3(kBlock), 0, stream(), s.dmesh(), s.dfields(), rp, q, s.psi.p,
                        s.val.p, s.rhs.p, s.ap.p);
     MS_CHECK_LAUNCH();
   }
   void energy() {
     const ms_ctrl_t &c = s.ctrl;
     if (c.conv_blend > 0.0)
-      hipLaunchKernelGGL(k_limiter, gc(s.nc), dim3(kBlock), 0, stream(), s.dmesh(), s.dfields(), rp, 3, s.psi.p);
+      hipLaunchKernelGGL(k_limiter, gcell(s), dim3(kBlock), 0, stream(), s.dmesh(), s.dfields(), rp, 3, s.psi.p);
     EnParams q{s.ts_order, c.pimple, c.rho, c.cp, c.alpha_u, c.dt, c.conv_blend, c.nonorth_corr, c.k_cond};
     if (s.has_disp)
       hipLaunchKernelGGL(k_kdisp, gc(s.nc), dim3(kBlock), 0, stream(), s.dfields(), s.nc, c.rho, c.cp);
-    hipLaunchKernelGGL(k_energy, gc(s.nc), dim3(kBlock), 0, stream(), s.dmesh(), s.dfields(), q, s.psi.p, s.val.p,
+    hipLaunchKernelGGL(k_energy, gcell(s), dim3(kBlock), 0, stream(), s.dmesh(), s.dfields(), q, s.psi.p, s.val.p,
                        s.rhs.p, s.ap.p);
     MS_CHECK_LAUNCH();
   }
   void solid_energy() {
     const ms_ctrl_t &c = s.ctrl;
     EnParams q{s.ts_order, c.pimple, c.rho, c.cp, c.alpha_u, c.dt, c.conv_blend, c.nonorth_corr, c.k_cond};
-    hipLaunchKernelGGL(k_solid_energy, gc(s.nc), dim3(kBlock), 0, stream(), s.dmesh(), s.dfields(), q, s.val.p,
+    hipLaunchKernelGGL(k_solid_energy, gcell(s), dim3(kBlock), 0, stream(), s.dmesh(), s.dfields(), q, s.val.p,
                        s.rhs.p, s.ap.p);
     MS_CHECK_LAUNCH();
   }
@@ -1022,12 +1053,12 @@ struct Driver {
   }
   void ppe() {
     hipLaunchKernelGGL(k_ppe_af, gc(s.nf), dim3(kBlock), 0, stream(), s.dmesh(), s.apc.p, s.ctrl.rho, s.af.p);
-    hipLaunchKernelGGL(k_ppe, gc(s.nc), dim3(kBlock), 0, stream(), s.dmesh(), s.flux.p, s.af.p, s.val.p, s.rhs.p);
+    hipLaunchKernelGGL(k_ppe, gcell(s), dim3(kBlock), 0, stream(), s.dmesh(), s.flux.p, s.af.p, s.val.p, s.rhs.p);
     MS_CHECK_LAUNCH();
   }
   void correct(const double *pp) {
     double palpha = s.ts_order > 0 ? -1.0 : s.ctrl.alpha_p;
-    hipLaunchKernelGGL(k_correct_cells, gc(s.nc), dim3(kBlock), 0, stream(), s.dmesh(), s.dfields(), pp, s.dpp.p,
+    hipLaunchKernelGGL(k_correct_cells, gcell(s), dim3(kBlock), 0, stream(), s.dmesh(), s.dfields(), pp, s.dpp.p,
                        palpha);
     hipLaunchKernelGGL(k_correct_flux, gc(s.nf), dim3(kBlock), 0, stream(), s.dmesh(), pp, s.af.p, s.flux.p);
     MS_CHECK_LAUNCH();

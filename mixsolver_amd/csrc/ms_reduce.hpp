@@ -100,4 +100,14 @@ __device__ __forceinline__ float tabs(float x) { return fabsf(x); }
 __device__ __forceinline__ double tsqrt(double x) { return sqrt(x); }
 __device__ __forceinline__ float tsqrt(float x) { return sqrtf(x); }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (blockIdx % 8).
+// xcd_block maps them so that each XCD works on one contiguous range of
+// logical blocks; the neighbour gathers of its rows then mostly hit its own
+// L2 instead of being fetched again by every XCD.  A bijection of [0, nb).
+inline constexpr int kXcd = 8;
+__device__ __forceinline__ inline int xcd_block(int b, int nb) {
+  const int q = nb / kXcd, r = nb % kXcd, x = b % kXcd;
+  return x * q + min(x, r) + b / kXcd;
+}
+
 }  // namespace ms

@@ -41,7 +41,7 @@ __device__ __forceinline__ long long sl(const Mat &M, int r, int k) {
 // column order) and every colour-1 neighbour j of k (in k's column order)
 // S(i, j) -= (a_rk / a_kk) a_kj.
 __global__ void k_schur_values(Mat A, const double *aval, int n0, int n1, Mat S, double *sval) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
+  const int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;   // XCD-local rows
   if (i >= n1) return;
   const int r = n0 + i;
   const int ls = S.rowlen[i];

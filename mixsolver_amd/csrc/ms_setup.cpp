@@ -944,6 +944,16 @@ System *create_system(int nnodes, const double *x, int nfaces, const long long *
   s.face_upload(s.d_sf, s.sf, 3);
   s.face_upload(s.d_area, s.area);
   s.face_upload(s.d_lf, s.lf);
+  {
+    std::vector<int> oth(s.cf_ent.size());
+    for (size_t k = 0; k < oth.size(); k++) {
+      const int f = s.cf_ent[k] >> 1;
+      oth[k] = s.c1[f] < 0 ? -1 : ((s.cf_ent[k] & 1) == 0 ? s.c1[f] : s.c0[f]);
+    }
+    s.d_e_oth.upload(oth);
+  }
+  s.entry_upload(s.d_e_lf, s.lf);
+  s.entry_upload(s.d_e_sf, s.sf, 3);
   s.d_vol.upload(vol);
   s.d_xc.upload(xc);
   std::vector<DevBC> dbc(s.nbc);
@@ -1188,6 +1198,9 @@ void System::refresh_face_constants() {
   face_upload(d_Dm, Dm);
   face_upload(d_muf, muf);
   face_upload(d_tcor, tcor, 3);
+  entry_upload(d_e_Dm, Dm);
+  entry_upload(d_e_muf, muf);
+  entry_upload(d_e_tcor, tcor, 3);
   face_upload(d_dw, dw);
   face_upload(d_Dmb, Dmb);
   face_upload(d_Dt, Dt);
@@ -1259,6 +1272,13 @@ DMesh System::dmesh() const {
   m.xc = d_xc.p;
   m.cf_ptr = d_cf_ptr.p;
   m.cf_ent = d_cf_ent.p;
+  m.pair_n0 = pat.color_ptr.size() == 3 ? pat.color_ptr[1] : 0;
+  m.e_oth = d_e_oth.p;
+  m.e_lf = d_e_lf.p;
+  m.e_sf = d_e_sf.p;
+  m.e_Dm = d_e_Dm.p;
+  m.e_muf = d_e_muf.p;
+  m.e_tcor = d_e_tcor.p;
   m.cf_slot = d_cf_slot.p;
   m.diag_slot = d_diag_slot.p;
   m.dn = d_dn.p;
@@ -1344,6 +1364,7 @@ size_t System::device_bytes() const {
        d_iface_slip_C.bytes() + d_iface_peer_vel.bytes() + d_tval_s.bytes();
   b += d_c0.bytes() + d_c1.bytes() + d_fgrp.bytes() + d_pidx.bytes() + d_cf_ptr.bytes() + d_cf_ent.bytes() + d_cf_slot.bytes() +
        d_diag_slot.bytes();
+  b += d_e_oth.bytes() + d_e_lf.bytes() + d_e_sf.bytes() + d_e_Dm.bytes() + d_e_muf.bytes() + d_e_tcor.bytes();
   return b;
 }
 

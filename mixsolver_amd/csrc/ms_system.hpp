@@ -17,9 +17,15 @@ struct DevBC {
 // the device numbering (0-based); face order is the mesh's face order.
 struct DMesh {
   int nc, nf, nbc, pin_cell, has_pdir, nonorth, blend_on;
+  int pair_n0;                             // > 0: two-colour numbering, rows of colour 0 (cell_row)
   const int *c0, *c1, *fgrp;               // c1 < 0 on boundary faces
   const double *xf, *sf, *area, *lf, *vol, *xc;
   const int *cf_ptr, *cf_ent;              // cell -> faces ascending, ent = 2*face + side
+  // the same per cell-face entry k (cf order), so a cell's face loads are
+  // adjacent in memory: the other cell (-1 on boundary faces), lf, sf (3),
+  // Dm, muf, tcor (3) of the entry's face
+  const int *e_oth;
+  const double *e_lf, *e_sf, *e_Dm, *e_muf, *e_tcor;
   const long long *cf_slot, *diag_slot;    // matrix slot of (cell, other) / (cell, cell)
   const double *dn, *dvs, *Dm, *muf, *tcor, *dw, *Dmb;   // momentum / PPE face constants
   const double *Dt, *kf, *Dtb;                           // energy face constants
@@ -129,6 +135,16 @@ struct System {
   // device mesh data
   DBuf<int> d_c0, d_c1, d_fgrp, d_cf_ptr, d_cf_ent;
   DBuf<long long> d_cf_slot, d_diag_slot;
+  DBuf<int> d_e_oth;
+  DBuf<double> d_e_lf, d_e_sf, d_e_Dm, d_e_muf, d_e_tcor;
+  // host face array (face order, width values per face) -> per cell-face entry
+  template <class T>
+  void entry_upload(DBuf<T> &d, const std::vector<T> &h, int width = 1) const {
+    std::vector<T> t((size_t)width * cf_ent.size());
+    for (size_t k = 0; k < cf_ent.size(); k++)
+      for (int w = 0; w < width; w++) t[(size_t)width * k + w] = h[(size_t)width * (cf_ent[k] >> 1) + w];
+    d.upload(t);
+  }
   DBuf<double> d_xf, d_sf, d_area, d_lf, d_vol, d_xc;
   DBuf<double> d_dn, d_dvs, d_Dm, d_muf, d_tcor, d_dw, d_Dmb, d_Dt, d_kf, d_Dtb, d_tval, d_qval;
   DBuf<signed char> d_tmode, d_pidx, d_fflag, d_tmode_s, d_cztype;

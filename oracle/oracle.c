@@ -42,6 +42,7 @@
 #include <float.h>
 #include <math.h>
 #include <time.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -1550,8 +1551,13 @@ static void pcg(int n, const int *rp, const int *ci, const double *v, const doub
   for (int i = 0; i < n; i++) p[i] = z[i];
   rz = dotn(r, z, n);
   int k;
+  /* ORACLE_PROGRESS=1: a line on stderr every 1000 iterations (long timed
+     runs print their own progress) */
+  const char *prog_env = getenv("ORACLE_PROGRESS");
+  const int progress = prog_env && prog_env[0] == '1';
   for (k = 1; k <= maxit; k++) {
     *it = k;
+    if (progress && k % 1000 == 0) fprintf(stderr, "oracle pcg: iteration %d, res %.3e\n", k, *res);
     or_csr_matvec(n, rp, ci, v, p, ap);
     pap = dotn(p, ap, n);
     if (fabs(pap) < TINY) { *ierr = 2; break; }
