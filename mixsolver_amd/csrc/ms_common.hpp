@@ -35,6 +35,9 @@ void ensure_runtime(int device); // select the device, create the library stream
 // environment variable MIXSOLVER_GRAPHS=0 or ms_set_graphs(0) turns it off)
 bool graphs_enabled();
 void set_graphs(bool on);
+// the mixed PCG queues its next chunk before reading the current chunk's
+// state (MIXSOLVER_PIPELINE=0 turns it off)
+bool pipeline_enabled();
 
 // --------------------------------------------------------- device buffers
 template <class T>

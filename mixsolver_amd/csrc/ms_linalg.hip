@@ -669,10 +669,13 @@ Krylov::Krylov(SellPattern *pat_) : pat(pat_), n(pat_->n) {
   st_.alloc(1);
   flag_.alloc(1);
   MS_HIP(hipHostMalloc(&hpin_, 4096, hipHostMallocDefault));
+  for (auto &e : state_ev_) MS_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
 }
 
 Krylov::~Krylov() {
   if (chunk_exec_) (void)hipGraphExecDestroy(chunk_exec_);
+  for (auto &e : state_ev_)
+    if (e) (void)hipEventDestroy(e);
   if (hpin_) (void)hipHostFree(hpin_);
 }
 
@@ -1238,31 +1241,29 @@ SolveResult Krylov::bicg32(PcKind pc, int maxit, int window) {
 namespace {
 struct SpmvProbe {
   bool on = false;
-  std::vector<hipEvent_t> ev;   // 2 per launch of one chunk
-  int used = 0;
+  std::vector<hipEvent_t> ev;   // a ring of start / stop pairs
+  int next = 0;
   long long launches = 0, rows = 0;
   double ms = 0.0;
 };
+constexpr int kProbePairs = 32;
 SpmvProbe g_probe;
-void probe_harvest() {
-  for (int i = 0; i < g_probe.used; i++) {
-    float t = 0.f;
-    MS_HIP(hipEventSynchronize(g_probe.ev[2 * i + 1]));
-    MS_HIP(hipEventElapsedTime(&t, g_probe.ev[2 * i], g_probe.ev[2 * i + 1]));
-    g_probe.ms += t;
-    g_probe.launches++;
-  }
-  g_probe.used = 0;
+void probe_harvest(int pair) {
+  float t = 0.f;
+  MS_HIP(hipEventSynchronize(g_probe.ev[2 * pair + 1]));
+  MS_HIP(hipEventElapsedTime(&t, g_probe.ev[2 * pair], g_probe.ev[2 * pair + 1]));
+  g_probe.ms += t;
+  g_probe.launches++;
 }
 }  // namespace
 
 void set_spmv_probe(bool on) {
   if (on && g_probe.ev.empty()) {
-    g_probe.ev.resize(64);
+    g_probe.ev.resize(2 * kProbePairs);
     for (auto &e : g_probe.ev) MS_HIP(hipEventCreate(&e));
   }
   g_probe.on = on;
-  g_probe.used = 0;
+  g_probe.next = 0;
   g_probe.launches = g_probe.rows = 0;
   g_probe.ms = 0.0;
 }
@@ -1456,11 +1457,13 @@ SolveResult Krylov::mixed(const double *val, const double *b, double *x, double 
   constexpr int kChunk = 16;   // reliable updates are applied at most this many iterations late
   const bool probe = g_probe.on;
   if (probe) g_probe.rows = n;
+  int probe_pair = -1;   // the probe pair of the chunk last queued
   auto chunk = [&](int m, bool probe_last) {
     for (int it = 0; it < m; it++) {
       if (probe_last && it == m - 1) {
-        hipEvent_t e0 = g_probe.ev[2 * g_probe.used], e1 = g_probe.ev[2 * g_probe.used + 1];
-        g_probe.used++;
+        probe_pair = g_probe.next;
+        g_probe.next = (g_probe.next + 1) % kProbePairs;
+        hipEvent_t e0 = g_probe.ev[2 * probe_pair], e1 = g_probe.ev[2 * probe_pair + 1];
         if (pat->has16)
           hipExtLaunchKernelGGL((k_pcg_spmv<float, 1>), G, B, 0, S, e0, e1, 0, n, A, v32, p, q, st, part32);
         else
@@ -1486,15 +1489,10 @@ SolveResult Krylov::mixed(const double *val, const double *b, double *x, double 
   double best = hs.res;
   int it_best = hs.it;
   bool stalled = false;
-  while (hs.done != 1 && hs.it < maxit) {
-    if (hs.res < 0.5 * best) {
-      best = hs.res;
-      it_best = hs.it;
-    } else if (hs.it - it_best >= kStall) {
-      stalled = true;
-      break;
-    }
-    const int m = std::min(kChunk, maxit - hs.it);
+  // Queue one chunk: the captured graph for a full chunk, else the
+  // launches; then its state goes to pinned slot `slot` and an event marks it.
+  auto queue_chunk = [&](int m, int slot) {
+    probe_pair = -1;
     if (m == kChunk && graphs_enabled()) {
       // every argument of the chunk's launches is fixed by the key (device
       // state lives in st, the iteration count in st->it), so one captured
@@ -1529,11 +1527,65 @@ SolveResult Krylov::mixed(const double *val, const double *b, double *x, double 
       chunk(m, probe);
     }
     MS_CHECK_LAUNCH();
-    hs = read_state<double>(st);
-    if (probe) probe_harvest();
-    if (hs.ru == 1 && hs.done != 1) {
-      reliable_update(hs.done == 2);
-      hs = read_state<double>(st);
+    MS_HIP(hipMemcpyAsync(state_slot(slot), st, sizeof(KS), hipMemcpyDeviceToHost, S));
+    MS_HIP(hipEventRecord(state_ev_[slot], S));
+  };
+  // Chunks run one ahead of the host: the next chunk is queued before the
+  // state after the current one is read, so the device does not idle while
+  // the host decides.  A solve that finished or paused makes the queued
+  // chunk's kernels return at once; a reliable update flagged in a chunk is
+  // applied after the chunk queued behind it (at most 2 x 16 iterations
+  // late), as the state then read is the latest.
+  struct Queued {
+    int slot, m, pair;
+  };
+  Queued inq[2];
+  int nq = 0, next_slot = 0, queued_it = hs.it;
+  const bool ahead = pipeline_enabled();
+  auto wait_front = [&]() {
+    const Queued f = inq[0];
+    inq[0] = inq[1];
+    nq--;
+    MS_HIP(hipEventSynchronize(state_ev_[f.slot]));
+    KS h;
+    std::memcpy(&h, state_slot(f.slot), sizeof(KS));
+    if (f.pair >= 0) probe_harvest(f.pair);
+    return h;
+  };
+  auto drain = [&]() {
+    while (nq > 0) hs = wait_front();
+  };
+  while (true) {
+    while (nq < (ahead ? 2 : 1) && queued_it < maxit) {
+      const int m = std::min(kChunk, maxit - queued_it);
+      queue_chunk(m, next_slot);
+      inq[nq++] = Queued{next_slot, m, probe_pair};
+      next_slot ^= 1;
+      queued_it += m;
+    }
+    if (nq == 0) break;
+    hs = wait_front();
+    if (hs.done == 1) {
+      drain();
+      break;
+    }
+    if (hs.res < 0.5 * best) {
+      best = hs.res;
+      it_best = hs.it;
+    } else if (hs.it - it_best >= kStall) {
+      drain();
+      if (hs.done != 1) stalled = true;
+      break;
+    }
+    if (hs.ru == 1) {   // reliable update flagged, or the solve paused for one
+      drain();
+      if (hs.done == 1) break;
+      if (hs.ru == 1) {
+        reliable_update(hs.done == 2);
+        hs = read_state<double>(st);
+      }
+      queued_it = hs.it;
+      if (hs.done == 1) break;
     }
   }
   if (hs.done != 1) {    // exhaustion: fold y and report the true residual

@@ -90,6 +90,8 @@ class Krylov {
   DBuf<double> cnu_split_;   // c.nu parts of k_finish_split
   DBuf<int> flag_;
   void *hpin_ = nullptr;
+  hipEvent_t state_ev_[2] = {nullptr, nullptr};   // the two pipelined chunk states (mixed PCG)
+  void *state_slot(int slot) const { return (char *)hpin_ + 1024 * (slot + 1); }
   int nblk_;
   bool have32_ = false;
   struct SetupKey {

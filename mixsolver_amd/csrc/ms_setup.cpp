@@ -70,6 +70,13 @@ bool graphs_enabled() {
   return v == 1;
 }
 void set_graphs(bool on) { g_graphs.store(on ? 1 : 0); }
+bool pipeline_enabled() {
+  static const bool on = [] {
+    const char *e = std::getenv("MIXSOLVER_PIPELINE");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
 
 // ------------------------------------------------------------------- norms
 double norm2_ref(const double *v, int n) {
