@@ -1454,7 +1454,7 @@ SolveResult Krylov::mixed(const double *val, const double *b, double *x, double 
     if (paused) tail(0);   // complete the interrupted iteration on the new r
     MS_CHECK_LAUNCH();
   };
-  constexpr int kChunk = 16;   // reliable updates are applied at most this many iterations late
+  constexpr int kChunk = 16;   // iterations between state reads
   const bool probe = g_probe.on;
   if (probe) g_probe.rows = n;
   int probe_pair = -1;   // the probe pair of the chunk last queued

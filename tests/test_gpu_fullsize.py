@@ -20,6 +20,9 @@ pytestmark = pytest.mark.gpu
 
 FAST = dict(gpu_ppe_precond=2, gpu_mom_precond=2, gpu_coarse=512, gpu_precision=1, gpu_inner_tol=0.1)
 FP64 = dict(gpu_ppe_precond=2, gpu_mom_precond=2, gpu_precision=0)
+# the benchmark's lead: p' through the red-black reduced system
+FAST_SCHUR = dict(gpu_ppe_precond=0, gpu_ppe_solver=2, gpu_mom_precond=2, gpu_coarse=256, gpu_precision=1,
+                  gpu_inner_tol=0.1)
 
 PRISM_CTL = """rho = 1.0
 mu = 0.005
@@ -194,7 +197,9 @@ def test_porous_ltne_16m_fast_path_equals_fp64():
     """16M hex cells (800x200x100) in one porous zone with a seeded porosity
     field in [0.4, 0.9] (smoothed Gaussian noise, correlation length 10
     cells) handed over per cell, Darcy-Forchheimer source, fluid and solid
-    energy equations coupled by h_sf a_sf (LTNE)."""
+    energy equations coupled by h_sf a_sf (LTNE).  The fast path solves p'
+    through the red-black reduced system in mixed precision and must need
+    no FP64 fallback at this size."""
     from scipy.ndimage import gaussian_filter
     from mixsolver_amd import zones
     from mixsolver_amd.control import parse_control
@@ -212,7 +217,7 @@ def test_porous_ltne_16m_fast_path_equals_fp64():
     u0 = np.zeros((m.ncells, 3))
     u0[:, 0] = 1.0
     out = {}
-    for name, over in (("fast", FAST), ("fp64", FP64)):
+    for name, over in (("fast", FAST_SCHUR), ("fp64", FP64)):
         c2, g2 = parse_control(POROUS_CTL, warn=False)
         c2.cell_zones = []
         for k, v in over.items():
@@ -224,10 +229,15 @@ def test_porous_ltne_16m_fast_path_equals_fp64():
         cs.set_field("u", u0)
         ier = cs.piso_run(1)
         h = cs.history()[-1]
+        st = cs.solver_stats()
+        print("16M", name, "p' its", h[6], "stats", st, flush=True)
+        if name == "fast":
+            assert st["ppe_fp64_solves"] == 0 and st["fp32_stalled"] == 0 and st["ppe_iters_fp32"] > 0
         out[name] = (ier, cs.get("u").reshape(-1, 3), cs.get("p"), h, cs.get("T"), cs.get("T_s"))
         cs.close()
-    # measured: u 4.8e-11, p 1.1e-10, T 1.5e-10, T_s 1.1e-9
-    _check({k: v[:4] for k, v in out.items()}, "porous LTNE 16M", 1e-8, 1e-8)
+    # the fast path is mixed precision to the 1e-10 residual: its fields
+    # agree with the FP64 path's to what that residual fixes
+    _check({k: v[:4] for k, v in out.items()}, "porous LTNE 16M", 1e-7, 1e-6)
     for i in (4, 5):
         a, b = out["fast"][i], out["fp64"][i]
         assert np.isfinite(a).all()

@@ -526,3 +526,61 @@ def test_rcm_numbering_matches_reference(name, mode):
         e = rel_l2(cs.get(k), d["out_" + k])
         print(name, "rcm", k, e, flush=True)
         assert e <= bar, (k, e)
+
+
+def _laplacian(nx, seed):
+    """1-based CSR of a 7-point finite-volume Laplacian on an nx^3 grid with
+    log-normal face coefficients and Dirichlet walls (SPD), and a seeded
+    right-hand side."""
+    rng = np.random.default_rng(seed)
+    n = nx ** 3
+    idx = np.arange(n).reshape(nx, nx, nx)
+    rows, cols, vals = [], [], []
+    diag = np.zeros(n)
+    for ax in range(3):
+        a = np.take(idx, np.arange(nx - 1), axis=ax).ravel()
+        b = np.take(idx, np.arange(1, nx), axis=ax).ravel()
+        k = np.exp(rng.standard_normal(a.size))
+        rows += [a, b]
+        cols += [b, a]
+        vals += [-k, -k]
+        np.add.at(diag, a, k)
+        np.add.at(diag, b, k)
+        for face in (0, nx - 1):
+            w = np.take(idx, [face], axis=ax).ravel()
+            np.add.at(diag, w, np.exp(rng.standard_normal(w.size)))
+    rows.append(np.arange(n))
+    cols.append(np.arange(n))
+    vals.append(diag)
+    r, c, v = np.concatenate(rows), np.concatenate(cols), np.concatenate(vals)
+    o = np.lexsort((c, r))
+    r, c, v = r[o], c[o], v[o]
+    rp = np.zeros(n + 1, np.int32)
+    np.add.at(rp, r + 1, 1)
+    return (np.cumsum(rp).astype(np.int32) + 1), (c + 1).astype(np.int32), v, rng.standard_normal(n)
+
+
+@pytest.mark.parametrize("case", ["solvers_spd", "solvers_nonsym", "laplace40"])
+def test_mixed_bicgstab_reaches_fp64_tolerance(case):
+    """BiCGSTAB with natural-order ILU(0) in mixed precision (FP32 inner
+    solves inside FP64 refinement) reaches the FP64 tolerance, with the
+    FP64 true residual recomputed on the host, in at most 3x the FP64
+    BiCGSTAB's iterations."""
+    import scipy.sparse as sp
+    if case == "laplace40":
+        rp, ci, val, b = _laplacian(40, 7)
+    else:
+        d = load_golden(case)
+        rp, ci, val, b = d["row_ptr"], d["col_idx"], d["val"], d["b"]
+    n = len(b)
+    A = sp.csr_matrix((val, ci - 1, rp - 1), shape=(n, n))
+    out = {}
+    for prec in ("fp64", "mixed"):
+        x, it, res, ierr = linsolver.solve(rp, ci, val, b, np.zeros_like(b), 1e-10, 20000, solver="bicgstab",
+                                           precond="natural", precision=prec)
+        out[prec] = (x, it, res, ierr)
+        print(case, prec, "it", it, "res", res, "ierr", ierr, flush=True)
+    x, it, res, ierr = out["mixed"]
+    assert ierr == 0 and res <= 1e-10
+    assert np.linalg.norm(b - A @ x) / np.linalg.norm(b) <= 1e-10
+    assert it <= 3 * max(out["fp64"][1], 10), (it, out["fp64"][1])
