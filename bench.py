@@ -173,16 +173,16 @@ def cpu_model():
 
 
 def cpu_baseline(n_needed, full=False):
-    """The oracle on the GPU box's host cores, at 1 thread and on all the
+    """The oracle on this host's cores, at 1 thread and on all the
     cores this process may use, on one PISO step of the 2M-cell
     non-periodic twin channel.  Bounded sample (default): the p' PCG-Jacobi
-    solves capped at 40 iterations each; the step is scaled with the
+    solves capped at 200 iterations each; the step is scaled with the
     measured per-iteration p' cost to the iteration count the FP64 Jacobi
     device solve needed for the same step.  full=True: the whole step, no
     cap, no scaling."""
     import subprocess
     ncores = len(os.sched_getaffinity(0))
-    cap = 0 if full else 40
+    cap = 0 if full else 200
     legs = {}
     for threads in (1, ncores):
         env = dict(os.environ, OMP_NUM_THREADS=str(threads), ORACLE_THREADS="1" if threads > 1 else "0",
@@ -198,7 +198,7 @@ def cpu_baseline(n_needed, full=False):
     t1, leg1, pit1 = legs[1]
     tn, legn, pitn = legs[ncores]
     how = ("one full step, measured" if full else
-           f"p' PCG-Jacobi capped at 40 iterations per solve ({leg1['iters']} p' iterations at "
+           f"p' PCG-Jacobi capped at 200 iterations per solve ({leg1['iters']} p' iterations at "
            f"{pit1 * 1e3:.1f} ms on 1 thread, {pitn * 1e3:.1f} ms on {ncores}), the step scaled to the "
            f"{n_needed} p' iterations of the FP64 Jacobi device solve")
     return dict(value=1.0 / tn, unit="steps/s", cores=ncores, kind="port",

@@ -324,8 +324,8 @@ int ms_bicgstab_ilu0(int n, const int *row_ptr, const int *col_idx, const double
 int ms_cg_jacobi_noalloc(int n, const int *row_ptr, const int *col_idx, const double *val,
                          const double *b, double *x, double tol, int maxit,
                          int *it, double *res, int *ierr);
-/* Extended solver entry (no reference counterpart: the north-star solver
- * options).  solver 0 PCG / 1 BiCGSTAB; precond MS_PC_*; precision
+/* Extended solver entry (no reference counterpart: the device path's
+ * solver options).  solver 0 PCG / 1 BiCGSTAB; precond MS_PC_*; precision
  * MS_FP64 / MS_MIXED; inner_tol for MS_MIXED.  res is the FP64 true
  * relative residual ||b-Ax||/||b|| for MS_MIXED. */
 int ms_solve(int n, const int *row_ptr, const int *col_idx, const double *val,

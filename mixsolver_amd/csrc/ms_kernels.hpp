@@ -295,6 +295,7 @@ struct FinUpdateSplit {
   int trigger;
   __device__ bool skip() const { return st->done != 0; }
   template <class T> __device__ void operator()(const T *a) const {
+    take_rz(st);   // the r.z this iteration's alpha used (k_pcg_update_fa)
     const double rn = sqrt((double)a[0]);
     st->res = rn * st->sigma / st->bnrm;
     if (trigger) {
@@ -678,6 +679,7 @@ __global__ __launch_bounds__(kFinThreads) void k_finish_co(int nparts, const T *
 // wavefront per row, E^-1 read along its rows) and writes the c.nu sum of
 // its rows to cnu_part[g-1].  k_pcg_p_split adds the parts (fixed order).
 constexpr int kSplitRows = kFinThreads / 64;
+constexpr int kSplitMaxParts = (kMaxAgg + kSplitRows - 1) / kSplitRows;   // coarse workgroups at most (64)
 template <int NV, class T, class Fin>
 __global__ __launch_bounds__(kFinThreads) void k_finish_split(int nparts, const T *partial, Fin fin, CoDev co,
                                                               double *cnu_part) {

@@ -112,6 +112,65 @@ __global__ void k_pcg_p(Rows R, const T *r, const T *z, const T *dg, int jac, T 
   }
 }
 
+// k_pcg_update (Jacobi) with the alpha finisher folded in, for the mixed
+// Jacobi PCG with a coarse space: every workgroup reduces the SpMV's p.q
+// partials itself, in the same fixed order, so all hold the same pap and
+// alpha (r.z is the value the last p update left pending, when it left one);
+// workgroup 0 keeps the solver state as FinPap does (iteration count, pap,
+// alpha, the breakdown test).  One launch and one kernel boundary less per
+// iteration.
+template <class T>
+__global__ void k_pcg_update_fa(Rows R, int np, const T *ppart, const T *p, const T *q, T *y, T *r, const T *dg,
+                                KS *st, T *part) {
+  if (st->done) return;
+  __shared__ T lds[kBlock / 64];
+  __shared__ double alpha_s;
+  __shared__ int bad_s;
+  T acc[1] = {T(0)};
+  int b = threadIdx.x;
+  for (; b + 3 * kBlock < np; b += 4 * kBlock) {   // four loads in flight, summed in index order
+    const T x0 = ppart[b], x1 = ppart[b + kBlock], x2 = ppart[b + 2 * kBlock], x3 = ppart[b + 3 * kBlock];
+    acc[0] = acc[0] + x0;
+    acc[0] = acc[0] + x1;
+    acc[0] = acc[0] + x2;
+    acc[0] = acc[0] + x3;
+  }
+  for (; b < np; b += kBlock) acc[0] = acc[0] + ppart[b];
+  block_reduce<OpSum, 1, T, kBlock>(acc, lds);
+  if (threadIdx.x == 0) {
+    const double pap = (double)acc[0];
+    const double rz = st->rzp ? st->rz_next : st->rz;
+    const int bad = fabs(pap) < kTiny;
+    bad_s = bad;
+    alpha_s = bad ? 0.0 : rz / pap;
+    if (blockIdx.x == 0) {
+      st->it = st->it + 1;
+      st->pap = pap;
+      if (bad) {
+        st->ierr = 2;
+        st->done = 1;
+      } else {
+        st->alpha = alpha_s;
+      }
+    }
+  }
+  __syncthreads();
+  if (bad_s) return;
+  int i, a;
+  T v[3] = {T(0), T(0), T(0)};
+  if (R.get(i, a)) {
+    const T alpha = (T)alpha_s;
+    const T pi = p[i], qi = q[i], r0 = r[i], yi = y[i], di = dg[i];
+    y[i] = yi + alpha * pi;
+    const T ri = r0 - alpha * qi;
+    r[i] = ri;
+    v[0] = ri * ri;
+    v[1] = ri * (ri / di);
+    v[2] = ri;
+  }
+  block_partials<OpSum, 3>(v, part);
+}
+
 // p = r/d + nu_agg + beta p after k_finish_split: every workgroup forms
 // r.z = rzd + the coarse parts (fixed order) and beta = r.z / rz; workgroup 0
 // leaves r.z in st->rz_next for the next finisher.
@@ -119,11 +178,18 @@ template <class T>
 __global__ void k_pcg_p_split(Rows R, const T *r, const T *dg, T *p, const double *nu, KS *st,
                               const double *cnu_part, int G) {
   if (st->done) return;
-  __shared__ double beta_s;
-  if (threadIdx.x == 0) {
-    double rzn = st->rzd;
-    for (int g = 0; g < G; g++) rzn = rzn + cnu_part[g];
-    const double beta = rzn / st->rz;
+  // the parts, rzd and rz are loaded by different threads at once (one load
+  // latency), then added by thread 0 in part order
+  __shared__ double parts[kSplitMaxParts + 2], beta_s;
+  const int t = threadIdx.x;
+  if (t < G) parts[t] = cnu_part[t];
+  else if (t == kSplitMaxParts) parts[t] = st->rzd;
+  else if (t == kSplitMaxParts + 1) parts[t] = st->rz;
+  __syncthreads();
+  if (t == 0) {
+    double rzn = parts[kSplitMaxParts];
+    for (int g = 0; g < G; g++) rzn = rzn + parts[g];
+    const double beta = rzn / parts[kSplitMaxParts + 1];
     beta_s = beta;
     if (blockIdx.x == 0) {
       st->rz_next = rzn;
@@ -1458,6 +1524,10 @@ SolveResult Krylov::mixed(const double *val, const double *b, double *x, double 
   const bool probe = g_probe.on;
   if (probe) g_probe.rows = n;
   int probe_pair = -1;   // the probe pair of the chunk last queued
+  // split path: the SpMV's p.q partials stay apart from the update's, which
+  // the update writes while other workgroups still read the SpMV's
+  if (split && pq32_.n < (size_t)nblk_) pq32_.alloc(nblk_);
+  float *spart = split ? pq32_.p : part32;
   auto chunk = [&](int m, bool probe_last) {
     for (int it = 0; it < m; it++) {
       if (probe_last && it == m - 1) {
@@ -1465,17 +1535,21 @@ SolveResult Krylov::mixed(const double *val, const double *b, double *x, double 
         g_probe.next = (g_probe.next + 1) % kProbePairs;
         hipEvent_t e0 = g_probe.ev[2 * probe_pair], e1 = g_probe.ev[2 * probe_pair + 1];
         if (pat->has16)
-          hipExtLaunchKernelGGL((k_pcg_spmv<float, 1>), G, B, 0, S, e0, e1, 0, n, A, v32, p, q, st, part32);
+          hipExtLaunchKernelGGL((k_pcg_spmv<float, 1>), G, B, 0, S, e0, e1, 0, n, A, v32, p, q, st, spart);
         else
-          hipExtLaunchKernelGGL((k_pcg_spmv<float, 0>), G, B, 0, S, e0, e1, 0, n, A, v32, p, q, st, part32);
+          hipExtLaunchKernelGGL((k_pcg_spmv<float, 0>), G, B, 0, S, e0, e1, 0, n, A, v32, p, q, st, spart);
       } else if (pat->has16) {
-        hipLaunchKernelGGL((k_pcg_spmv<float, 1>), G, B, 0, S, n, A, v32, p, q, st, part32);
+        hipLaunchKernelGGL((k_pcg_spmv<float, 1>), G, B, 0, S, n, A, v32, p, q, st, spart);
       } else {
-        hipLaunchKernelGGL((k_pcg_spmv<float, 0>), G, B, 0, S, n, A, v32, p, q, st, part32);
+        hipLaunchKernelGGL((k_pcg_spmv<float, 0>), G, B, 0, S, n, A, v32, p, q, st, spart);
       }
-      finish<OpSum, 1>(nblk_, part32, FinPap{st});
-      hipLaunchKernelGGL((k_pcg_update<float, float>), GR, B, 0, S, RW, p, q, y, r, z, dg, mode, c0end, st,
-                         part32);
+      if (split) {   // alpha reduced inside the update (the SpMV's partials go to a buffer of their own)
+        hipLaunchKernelGGL(k_pcg_update_fa<float>, GR, B, 0, S, RW, nblk_, pq32_.p, p, q, y, r, dg, st, part32);
+      } else {
+        finish<OpSum, 1>(nblk_, part32, FinPap{st});
+        hipLaunchKernelGGL((k_pcg_update<float, float>), GR, B, 0, S, RW, p, q, y, r, z, dg, mode, c0end, st,
+                           part32);
+      }
       tail(1);
     }
   };

@@ -88,6 +88,7 @@ class Krylov {
   DBuf<float> val32_, r32_, r032_, p32_, q32_, z32_, s32_, t32_, ph32_, sh32_, x32_, fac32_, dg32_, part32_, prz32_;
   DBuf<KS> st_, st32_;
   DBuf<double> cnu_split_;   // c.nu parts of k_finish_split
+  DBuf<float> pq32_;         // the SpMV's p.q partials on the split path
   DBuf<int> flag_;
   void *hpin_ = nullptr;
   hipEvent_t state_ev_[2] = {nullptr, nullptr};   // the two pipelined chunk states (mixed PCG)

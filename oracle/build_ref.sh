@@ -60,9 +60,10 @@ echo "built $OUT/ref_harness"
 # (whose solvers call libmixsolver_hip.so), plus the one-partition
 # mod_uns_linsolver_mpi.f90 drop-in compiled against the reference's
 # mod_precision.  oracle/_ref/ref_dropin links the reference's own drivers to
-# the device solvers.  It is a reference-derived binary: it stays in this
-# container (oracle/_ref is listed in .gpurunignore) and no GPU test runs it;
-# tests/dropin/dropin_check is the caller the GPU tests run.
+# the device solvers.  It is a reference-derived binary: it stays on the
+# machine that built it (oracle/_ref is never shipped with the GPU runs) and
+# no GPU test runs it; tests/dropin/dropin_check is the caller the GPU tests
+# run.
 LIBDIR="$(cd "$HERE/../mixsolver_amd" && pwd)"
 if [ -f "$LIBDIR/libmixsolver_hip.so" ]; then
   D="$TMP/dropin"
