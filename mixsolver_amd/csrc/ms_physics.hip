@@ -1214,8 +1214,7 @@ int run_piso(System &s, int nsteps, bool resume) {
     D.rhiechow();
   }
   // CFL needs min over cells of vol^(1/3) (host pow, as the reference)
-  double minv13 = INFINITY;
-  for (double v : s.vol_nat) minv13 = std::min(minv13, std::pow(v, 1.0 / 3.0));
+  const double minv13 = s.min_vol13();
   int ier = 0;
   for (int it = it0 + 1; it <= it0 + n_steps; it++) {
     dcopy(s.u_old_old.p, s.u_old.p, n3);
@@ -1290,8 +1289,7 @@ int run_pimple(System &s, int nsteps, bool resume) {
     D.rhiechow();
   }
   const int ts_order_in = s.ts_order_in;
-  double minv13 = INFINITY;
-  for (double v : s.vol_nat) minv13 = std::min(minv13, std::pow(v, 1.0 / 3.0));
+  const double minv13 = s.min_vol13();
   int ier = 0;
   for (int it = it0 + 1; it <= it0 + n_steps; it++) {
     dcopy(s.u_old_old.p, s.u_old.p, n3);

@@ -1,5 +1,7 @@
 // ms_system.hpp -- one mesh + fields + matrices resident on the device.
 #pragma once
+#include <algorithm>
+#include <cmath>
 #include <memory>
 
 #include "ms_common.hpp"
@@ -186,6 +188,17 @@ struct System {
   // the energy equations can have a non-zero solution (otherwise their
   // right-hand sides are zero and the solves return at once)
   bool energy_on() const { return T_nonzero || bc_energy || has_qsrc; }
+  // min over cells of vol^(1/3) (host pow, as the reference's CFL estimate),
+  // evaluated once: the volumes do not change
+  double minv13_ = -1.0;
+  double min_vol13() {
+    if (minv13_ < 0.0) {
+      double m = INFINITY;
+      for (double v : vol_nat) m = std::min(m, std::pow(v, 1.0 / 3.0));
+      minv13_ = m;
+    }
+    return minv13_;
+  }
   bool solid_phase() const { return ctrl.thermal_model != 0 || has_solid; }
   DMesh dmesh() const;
   DFields dfields();
