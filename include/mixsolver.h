@@ -375,6 +375,48 @@ int ms_server_result(ms_server *s, long long ticket, double timeout_ms, double *
 /* submitted completed active pending rounds launches peak_budget_bytes patterns */
 int ms_server_stats(ms_server *s, long long *out8);
 
+/* ---- parameter-sweep server (no reference counterpart) --------------------
+ * Serves whole transient cases of one mesh that differ in an inlet speed
+ * (a parameter sweep).  Every case is a fork of `base`: it reads the base's
+ * mesh, geometry, face constants, zone properties and matrix patterns in
+ * place and owns its fields (copied from the base's when it starts),
+ * matrices and solver state.  A case runs `nsteps` steps of the partitioned
+ * transient PISO driver at one partition (piso_run_mpi,
+ * mod_uns_simple_mpi.f90:298-578; the first step as ms_piso_run, the others
+ * as ms_piso_continue) with the base's control.  Cases are admitted by
+ * device-memory budget (budget_bytes; 0 = 90% of the free device memory at
+ * creation): the first case runs alone until its first step has allocated
+ * what a case allocates; then cases start while the footprints of the cases
+ * in flight plus one more fit the budget, at most max_active (1..64) at
+ * once, each on a host thread and HIP stream of its own.  A case's results
+ * do not depend on the mix (bit for bit).  While a sweep exists the base's
+ * zone properties and cell types cannot change and the base cannot be
+ * destroyed; do not run the base itself while cases are being admitted. */
+typedef struct ms_sweep ms_sweep;
+int ms_sweep_create(ms_sweep **sw, ms_system *base, long long budget_bytes, int max_active);
+/* waits for the cases in flight; queued cases are dropped */
+int ms_sweep_destroy(ms_sweep *sw);
+/* group: 1-based boundary group whose speed the case sets (0: none):
+ * velocity inlet / far field: uvel (3); parabolic inlet: mean speed `speed`;
+ * mass inlet: speed.  keep_fields: keep the final u (3,ncells), p, T, T_s
+ * for ms_sweep_result.  *id receives the case id. */
+int ms_sweep_submit(ms_sweep *sw, int group, const double *uvel, double speed, int nsteps, int keep_fields,
+                    long long *id);
+/* Waits up to wait_s seconds for case id to end.  *state: 0 queued,
+ * 1 running, 2 done, 3 failed (the call then returns the case's error code
+ * and ms_last_error its message).  When done: *steps the steps run, *ier
+ * the driver's ier, step_ms (nsteps) the wall time of each step from its
+ * launch to its completion, history (8 per step) the rows of
+ * ms_get_history, times3 = seconds since the sweep started at submission,
+ * admission and completion, stats8 as ms_solver_stats; u, p, T, T_s
+ * (natural order) when the case kept its fields.  Any output may be NULL. */
+int ms_sweep_result(ms_sweep *sw, long long id, double wait_s, int *state, int *steps, int *ier, double *step_ms,
+                    double *history, double *times3, long long *stats8, double *u, double *p, double *T,
+                    double *T_s);
+/* submitted done failed queued active peak_active footprint_bytes_per_case
+ * budget_bytes base_device_bytes steps_run */
+int ms_sweep_stats(ms_sweep *sw, long long *out10);
+
 /* ---- measurement (no reference counterpart) -----------------------------
  * Average duration of the dominant solver kernel (PCG SpMV with its fused
  * p.q partials) on the system's current matrix, timed with HIP events on the

@@ -98,6 +98,11 @@ def lib():
         "ms_server_submit": (C.c_int, [_V, C.c_int, _D, _D, C.c_double, C.c_int, _L]),
         "ms_server_result": (C.c_int, [_V, C.c_longlong, C.c_double, _D, _I, _D, _I, _D, _D, _I]),
         "ms_server_stats": (C.c_int, [_V, _L]),
+        "ms_sweep_create": (C.c_int, [C.POINTER(_V), _V, C.c_longlong, C.c_int]),
+        "ms_sweep_destroy": (C.c_int, [_V]),
+        "ms_sweep_submit": (C.c_int, [_V, C.c_int, _D, C.c_double, C.c_int, C.c_int, _L]),
+        "ms_sweep_result": (C.c_int, [_V, C.c_longlong, C.c_double, _I, _I, _I, _D, _D, _D, _L, _D, _D, _D, _D]),
+        "ms_sweep_stats": (C.c_int, [_V, _L]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmixsolver_hip.so")
 SOURCES = ["ms_setup.cpp", "ms_linalg.hip", "ms_physics.hip", "ms_serve.hip", "ms_output.cpp", "ms_noalloc.hip", "ms_schur.hip",
-           "ms_capi.cpp"]
+           "ms_sweep.cpp", "ms_capi.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-shared",
          # no FMA contraction: assembly and preconditioner rows stay bit-exact

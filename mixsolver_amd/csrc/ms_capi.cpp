@@ -7,6 +7,7 @@
 #include <memory>
 
 #include "ms_serve.hpp"
+#include "ms_sweep.hpp"
 #include "ms_system.hpp"
 
 namespace ms {
@@ -168,6 +169,7 @@ int ms_system_create(ms_system **out, int nnodes, const double *x, int nfaces, c
 int ms_system_destroy(ms_system *s) {
   return guarded([&] {
     if (!s) return;
+    if (s->s && s->s->nforks.load() > 0) fail(MS_EINVAL, "the system still has sweep cases (destroy the sweep first)");
     MS_HIP(hipStreamSynchronize(stream()));
     delete s;
   });
@@ -247,6 +249,8 @@ int ms_set_field(ms_system *h, const char *name, const double *host, long long n
     System &s = sys(h);
     FieldRef f = lookup(s, name);
     if (!host) fail(MS_EINVAL, "null host array");
+    if (f.kind == PROP && s.nforks.load() > 0)
+      fail(MS_EINVAL, std::string(name) + ": zone properties cannot change while sweep cases share this system");
     if (f.kind == FACE) {
       if (n != s.nf) fail(MS_EINVAL, "face field size mismatch");
       s.face_upload(*f.buf, std::vector<double>(host, host + n));
@@ -310,6 +314,7 @@ int ms_get_field(ms_system *h, const char *name, double *host, long long n) {
 int ms_set_cell_types(ms_system *h, const int *cztype) {
   return guarded([&] {
     System &s = sys(h);
+    if (s.nforks.load() > 0) fail(MS_EINVAL, "cell types cannot change while sweep cases share this system");
     if (!cztype) {
       s.cztype.clear();
     } else {
@@ -724,6 +729,75 @@ int ms_server_stats(ms_server *h, long long *out8) {
   return guarded([&] {
     if (!out8) fail(MS_EINVAL, "null output");
     server_stats(srv(h), out8);
+  });
+}
+
+// ---- parameter-sweep server ------------------------------------------------
+struct ms_sweep {
+  Sweep *s = nullptr;
+};
+
+int ms_sweep_create(ms_sweep **sw, ms_system *base, long long budget_bytes, int max_active) {
+  return guarded([&] {
+    if (!sw) fail(MS_EINVAL, "null output handle");
+    *sw = nullptr;
+    ensure_runtime(-1);
+    auto h = std::make_unique<ms_sweep>();
+    h->s = sweep_create(sys(base), budget_bytes, max_active);
+    *sw = h.release();
+  });
+}
+
+int ms_sweep_destroy(ms_sweep *sw) {
+  return guarded([&] {
+    if (!sw) return;
+    sweep_destroy(sw->s);
+    delete sw;
+  });
+}
+
+int ms_sweep_submit(ms_sweep *sw, int group, const double *uvel, double speed, int nsteps, int keep_fields,
+                    long long *id) {
+  return guarded([&] {
+    if (!sw || !sw->s) fail(MS_EINVAL, "null sweep handle");
+    const long long i = sweep_submit(sw->s, group, uvel, speed, nsteps, keep_fields);
+    if (id) *id = i;
+  });
+}
+
+int ms_sweep_result(ms_sweep *sw, long long id, double wait_s, int *state, int *steps, int *ier, double *step_ms,
+                    double *history, double *times3, long long *stats8, double *u, double *p, double *T,
+                    double *T_s) {
+  return guarded([&] {
+    if (!sw || !sw->s) fail(MS_EINVAL, "null sweep handle");
+    SweepCase c;
+    const int st = sweep_result(sw->s, id, wait_s, &c);
+    if (state) *state = st;
+    if (st < 2) return;
+    if (st == 3) fail(c.err ? c.err : MS_EINVAL, "sweep case " + std::to_string(id) + ": " + c.msg);
+    if (steps) *steps = (int)c.step_ms.size();
+    if (ier) *ier = c.ier;
+    if (step_ms) std::copy(c.step_ms.begin(), c.step_ms.end(), step_ms);
+    if (history) std::copy(c.history.begin(), c.history.end(), history);
+    if (times3) {
+      times3[0] = c.t_submit;
+      times3[1] = c.t_admit;
+      times3[2] = c.t_done;
+    }
+    if (stats8) std::copy(c.stats, c.stats + 8, stats8);
+    if (c.keep) {
+      if (u) std::copy(c.u.begin(), c.u.end(), u);
+      if (p) std::copy(c.p.begin(), c.p.end(), p);
+      if (T) std::copy(c.T.begin(), c.T.end(), T);
+      if (T_s) std::copy(c.T_s.begin(), c.T_s.end(), T_s);
+    }
+  });
+}
+
+int ms_sweep_stats(ms_sweep *sw, long long *out10) {
+  return guarded([&] {
+    if (!sw || !sw->s || !out10) fail(MS_EINVAL, "null sweep handle or output");
+    sweep_stats(sw->s, out10);
   });
 }
 

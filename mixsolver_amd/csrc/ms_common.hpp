@@ -4,6 +4,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -28,8 +29,22 @@ void set_error(const std::string &m);
 
 #define MS_CHECK_LAUNCH() MS_HIP(hipGetLastError())
 
-hipStream_t stream();           // library stream (default priority)
+hipStream_t stream();           // the calling thread's stream (default: the library stream)
 void ensure_runtime(int device); // select the device, create the library stream
+int runtime_device();            // device selected by ensure_runtime (-1 before)
+// A worker thread that drives systems of its own (the sweep server) gives
+// itself a stream of its own: every library call made on that thread then
+// launches and synchronises there.  Default priority; nullptr restores the
+// library stream.
+void set_thread_stream(hipStream_t s);
+// device bytes allocated minus freed by DBuf on the calling thread (the
+// sweep server measures a case's footprint with it)
+long long thread_device_bytes();
+void count_device_bytes(long long delta);
+// Held around device allocations and frees and around stream captures, so
+// that no thread allocates or frees (which may synchronise the device) while
+// another thread captures a graph.
+std::recursive_mutex &device_alloc_mutex();
 // replay the mixed-precision PCG chunk (16 iterations) from a captured HIP
 // graph instead of launching its kernels one by one (default on; the
 // environment variable MIXSOLVER_GRAPHS=0 or ms_set_graphs(0) turns it off)
@@ -44,13 +59,18 @@ template <class T>
 struct DBuf {
   T *p = nullptr;
   size_t n = 0;
+  bool own = true;   // false: a read-only view of another system's buffer (forked systems)
   DBuf() = default;
   explicit DBuf(size_t count) { alloc(count); }
   DBuf(const DBuf &) = delete;
   DBuf &operator=(const DBuf &) = delete;
-  DBuf(DBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DBuf(DBuf &&o) noexcept : p(o.p), n(o.n), own(o.own) { o.p = nullptr; o.n = 0; o.own = true; }
   DBuf &operator=(DBuf &&o) noexcept {
-    if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+    if (this != &o) {
+      release();
+      p = o.p; n = o.n; own = o.own;
+      o.p = nullptr; o.n = 0; o.own = true;
+    }
     return *this;
   }
   ~DBuf() { release(); }
@@ -58,15 +78,31 @@ struct DBuf {
     release();
     n = count;
     if (count) {
+      std::lock_guard<std::recursive_mutex> g(device_alloc_mutex());
       hipError_t e = hipMalloc(&p, count * sizeof(T));
       if (e != hipSuccess) { p = nullptr; n = 0; fail(MS_ENOMEM, "hipMalloc failed"); }
+      count_device_bytes((long long)(count * sizeof(T)));
     }
   }
+  // view of o's memory; never freed through this buffer
+  void alias(const DBuf &o) {
+    release();
+    p = o.p;
+    n = o.n;
+    own = false;
+  }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p && own) {
+      std::lock_guard<std::recursive_mutex> g(device_alloc_mutex());
+      (void)hipFree(p);
+      count_device_bytes(-(long long)(n * sizeof(T)));
+    }
     p = nullptr;
     n = 0;
+    own = true;
   }
+  // bytes this buffer owns (0 for a view)
+  size_t owned_bytes() const { return own ? n * sizeof(T) : 0; }
   void upload(const T *h, size_t count) {
     if (count > n) alloc(count);
     if (count) MS_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, stream()));
@@ -108,11 +144,27 @@ struct Coarse {
   std::vector<int> e_ptr, e_key;      // E gather map: entry key a*k+b, slots [e_ptr[i], e_ptr[i+1])
   std::vector<int> e_slot;            // matrix slots, grouped by key, ascending
   DBuf<int> d_chunk_ptr, d_chunk_agg, d_agg_cptr, d_agg_cidx, d_e_ptr, d_e_key, d_e_slot;
-  DBuf<double> d_E, d_nu;             // dense E (k x k), coarse solution nu (k)
-  DBuf<double> d_W, d_rc;             // Gauss-Jordan work matrix (k x k), pivot row/column snapshots (4 k)
-  DBuf<float> d_Einv;                 // dense E^-1 (k x k), FP32
+  // (the matrix-dependent E, its inverse and the coarse solution belong to
+  // each solver: CoarseMat)
   void upload();
   size_t device_bytes() const;
+};
+
+// Values of the coarse space for one matrix: dense E (k x k), FP64
+// Gauss-Jordan work matrix and pivot snapshots, E^-1 in FP32, and the
+// coarse solution nu (k).  Owned by the solver, so systems that share a
+// pattern (forked sweep cases) keep their own.
+struct CoarseMat {
+  DBuf<double> E, nu, W, rc;
+  DBuf<float> Einv;
+  void ensure(int k) {
+    const size_t kk = (size_t)k * k;
+    if (E.n < kk) E.alloc(kk);
+    if (Einv.n < kk) Einv.alloc(kk);
+    if (W.n < kk) W.alloc(kk);
+    if (rc.n < 4 * (size_t)k) rc.alloc(4 * (size_t)k);
+    if (nu.n < (size_t)k) nu.alloc(k);
+  }
 };
 
 // ------------------------------------------------- SELL-64 sparse pattern

@@ -727,18 +727,20 @@ static void finish(int nparts, const T *part, Fin fin) {
   hipLaunchKernelGGL((k_finish<Op, NV, T, Fin>), dim3(1), dim3(kFinThreads), 0, stream(), nparts, part, fin);
 }
 
-Krylov::Krylov(SellPattern *pat_) : pat(pat_), n(pat_->n) {
+Krylov::Krylov(const SellPattern *pat_) : pat(pat_), n(pat_->n) {
   nblk_ = nblocks(n);
   for (DBuf<double> *b : {&r_, &r0_, &p_, &q_, &z_, &s_, &t_, &ph_, &sh_, &dg_}) b->alloc(n);
   part_.alloc(4 * ((size_t)std::max(nblk_, pat_->coarse.nchunks) + 64) * 2);
   prz_.alloc(2 * ((size_t)nblk_ + 64) + kMaxAgg);
   st_.alloc(1);
   flag_.alloc(1);
+  std::lock_guard<std::recursive_mutex> lk(device_alloc_mutex());
   MS_HIP(hipHostMalloc(&hpin_, 4096, hipHostMallocDefault));
   for (auto &e : state_ev_) MS_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
 }
 
 Krylov::~Krylov() {
+  std::lock_guard<std::recursive_mutex> lk(device_alloc_mutex());
   if (chunk_exec_) (void)hipGraphExecDestroy(chunk_exec_);
   for (auto &e : state_ev_)
     if (e) (void)hipEventDestroy(e);
@@ -1004,13 +1006,13 @@ static Rows rows_of(const SellPattern *pat, bool coarse) {
   if (!coarse) return Rows{nullptr, nullptr, pat->n};
   return Rows{pat->coarse.d_chunk_ptr.p, pat->coarse.d_chunk_agg.p, pat->n};
 }
-static CoDev codev_of(const SellPattern *pat, bool coarse, KS *st) {
+static CoDev codev_of(const SellPattern *pat, const CoarseMat &cm, bool coarse, KS *st) {
   if (!coarse) return CoDev{0, nullptr, nullptr, nullptr, nullptr, st};
   const Coarse &C = pat->coarse;
-  return CoDev{C.k, C.d_agg_cptr.p, C.d_agg_cidx.p, C.d_Einv.p, C.d_nu.p, st};
+  return CoDev{C.k, C.d_agg_cptr.p, C.d_agg_cidx.p, cm.Einv.p, cm.nu.p, st};
 }
 template <class T>
-static CoWork<T> cowork_of(const SellPattern *pat, bool fused, const T *cpart) {
+static CoWork<T> cowork_of(const SellPattern *pat, const CoarseMat &cm, bool fused, const T *cpart) {
   CoWork<T> cw{};
   if (!fused) return cw;
   const Coarse &C = pat->coarse;
@@ -1018,8 +1020,8 @@ static CoWork<T> cowork_of(const SellPattern *pat, bool fused, const T *cpart) {
   cw.nwg = (C.k + kBlock / 64 - 1) / (kBlock / 64);
   cw.agg_cptr = C.d_agg_cptr.p;
   cw.agg_cidx = C.d_agg_cidx.p;
-  cw.Einv = C.d_Einv.p;
-  cw.nu = C.d_nu.p;
+  cw.Einv = cm.Einv.p;
+  cw.nu = cm.nu.p;
   cw.cpart = cpart;
   return cw;
 }
@@ -1030,24 +1032,24 @@ static void finish_co(int nparts, const T *part, Fin fin, const CoDev &co) {
 }
 
 void Krylov::coarse_setup(const double *val) {
-  Coarse &C = pat->coarse;
+  const Coarse &C = pat->coarse;
+  CoarseMat &M = cm_;
+  M.ensure(C.k);
   const long long kk = (long long)C.k * C.k;
   const int nE = (int)C.e_key.size();
-  hipLaunchKernelGGL(k_fill64, dim3(nblocks(kk)), dim3(kBlock), 0, stream(), kk, C.d_E.p, 0.0);
+  hipLaunchKernelGGL(k_fill64, dim3(nblocks(kk)), dim3(kBlock), 0, stream(), kk, M.E.p, 0.0);
   hipLaunchKernelGGL(k_coarse_E, dim3(nblocks(nE, kBlock / 64)), dim3(kBlock), 0, stream(), nE, C.d_e_ptr.p,
-                     C.d_e_key.p, C.d_e_slot.p, val, C.d_E.p);
+                     C.d_e_key.p, C.d_e_slot.p, val, M.E.p);
   // FP64 Gauss-Jordan over the grid, one launch per pivot (k <= 1024)
-  if (C.d_W.n < (size_t)kk) C.d_W.alloc(kk);
-  if (C.d_rc.n < 4 * (size_t)C.k) C.d_rc.alloc(4 * (size_t)C.k);
-  double *rc[2][2] = {{C.d_rc.p, C.d_rc.p + C.k}, {C.d_rc.p + 2 * C.k, C.d_rc.p + 3 * C.k}};
+  double *rc[2][2] = {{M.rc.p, M.rc.p + C.k}, {M.rc.p + 2 * C.k, M.rc.p + 3 * C.k}};
   const dim3 g(nblocks(kk));
-  hipLaunchKernelGGL(k_gj_init, g, dim3(kBlock), 0, stream(), C.k, C.d_E.p, C.d_W.p, rc[0][0], rc[0][1]);
+  hipLaunchKernelGGL(k_gj_init, g, dim3(kBlock), 0, stream(), C.k, M.E.p, M.W.p, rc[0][0], rc[0][1]);
   for (int j = 0; j < C.k; j++) {
     const int cur = j & 1, nxt = cur ^ 1;
-    hipLaunchKernelGGL(k_gj_step, g, dim3(kBlock), 0, stream(), C.k, j, C.d_W.p, rc[cur][0], rc[cur][1], rc[nxt][0],
+    hipLaunchKernelGGL(k_gj_step, g, dim3(kBlock), 0, stream(), C.k, j, M.W.p, rc[cur][0], rc[cur][1], rc[nxt][0],
                        rc[nxt][1]);
   }
-  hipLaunchKernelGGL(k_gj_out, g, dim3(kBlock), 0, stream(), kk, C.d_W.p, C.d_Einv.p);
+  hipLaunchKernelGGL(k_gj_out, g, dim3(kBlock), 0, stream(), kk, M.W.p, M.Einv.p);
   MS_CHECK_LAUNCH();
 }
 
@@ -1096,9 +1098,9 @@ SolveResult Krylov::pcg(const double *val, const double *b, double *x, double to
   const int *done = &st->done;
   const Rows RW = rows_of(pat, coarse);
   const dim3 GR(coarse ? pat->coarse.nchunks : nblk_);
-  const CoDev co = codev_of(pat, coarse, st);
-  const double *nu = coarse ? pat->coarse.d_nu.p : nullptr;
-  const CoWork<double> cw = cowork_of<double>(pat, fused, part);
+  const CoDev co = codev_of(pat, cm_, coarse, st);
+  const double *nu = coarse ? cm_.nu.p : nullptr;
+  const CoWork<double> cw = cowork_of<double>(pat, cm_, fused, part);
   double *prz = prz_.p;
   if (fused) {
     hipLaunchKernelGGL(k_rsum<double>, GR, B, 0, S, RW, r_.p, st, part);
@@ -1455,9 +1457,9 @@ SolveResult Krylov::mixed(const double *val, const double *b, double *x, double 
   last_ = key;
   const Rows RW = rows_of(pat, coarse);
   const dim3 GR(coarse ? pat->coarse.nchunks : nblk_);
-  const CoDev co = codev_of(pat, coarse && !fused, st);
-  const double *nu = coarse ? pat->coarse.d_nu.p : nullptr;
-  const CoWork<float> cw = cowork_of<float>(pat, fused, part32);
+  const CoDev co = codev_of(pat, cm_, coarse && !fused, st);
+  const double *nu = coarse ? cm_.nu.p : nullptr;
+  const CoWork<float> cw = cowork_of<float>(pat, cm_, fused, part32);
   float *prz = prz32_.p;
   MS_HIP(hipMemsetAsync(y, 0, n * sizeof(float), S));
   hipLaunchKernelGGL(k_set_sigma, dim3(1), dim3(64), 0, S, st);
@@ -1575,6 +1577,7 @@ SolveResult Krylov::mixed(const double *val, const double *b, double *x, double 
       const int glen = probe ? kChunk - 1 : kChunk;
       const ChunkKey ck{v32, fac, dg, nu, (int)pc, (int)coarse, (int)fused, glen};
       if (!chunk_exec_ || !(chunk_key_ == ck)) {
+        std::lock_guard<std::recursive_mutex> lk(device_alloc_mutex());
         if (chunk_exec_) MS_HIP(hipGraphExecDestroy(chunk_exec_));
         chunk_exec_ = nullptr;
         hipGraph_t g = nullptr;

@@ -45,7 +45,7 @@ enum PcKind { PC_JACOBI = 0, PC_NATURAL_IC0 = 1, PC_NATURAL_ILU0 = 2, PC_DILU = 
 
 class Krylov {
  public:
-  explicit Krylov(SellPattern *pat);
+  explicit Krylov(const SellPattern *pat);
   ~Krylov();
   // FP64 solvers with the reference recurrences (mod_uns_linsolver_mpi.f90):
   // x is the initial guess on entry.  Returns reference it/res/ierr.
@@ -71,7 +71,7 @@ class Krylov {
   long long launches = 0;
   bool fp32_stalled() const { return fp32_stalled_; }
 
-  SellPattern *pat;
+  const SellPattern *pat;
   int n;
 
  private:
@@ -90,6 +90,7 @@ class Krylov {
   DBuf<double> cnu_split_;   // c.nu parts of k_finish_split
   DBuf<float> pq32_;         // the SpMV's p.q partials on the split path
   DBuf<int> flag_;
+  CoarseMat cm_;             // E, E^-1 and nu of the coarse space for the current matrix
   void *hpin_ = nullptr;
   hipEvent_t state_ev_[2] = {nullptr, nullptr};   // the two pipelined chunk states (mixed PCG)
   void *state_slot(int slot) const { return (char *)hpin_ + 1024 * (slot + 1); }

@@ -955,8 +955,8 @@ __global__ void k_wake_end(DMesh m, DFields F, double x0, double *part) {
 static dim3 gc(int n) { return dim3(nblocks(n)); }
 // grid of the cell-wise assembly kernels (cell_row)
 static dim3 gcell(const System &s) {
-  if (s.pat.color_ptr.size() == 3) {
-    const int n0 = s.pat.color_ptr[1];
+  if (s.P->color_ptr.size() == 3) {
+    const int n0 = s.P->color_ptr[1];
     return dim3(nblocks(std::max(n0, s.nc - n0), kBlock / 2));
   }
   return gc(s.nc);
@@ -1069,7 +1069,7 @@ struct Driver {
     const ms_ctrl_t &c = s.ctrl;
     int solver = ppe_eq ? c.gpu_ppe_solver : 1;
     PcKind pc = pc_of(ppe_eq ? c.gpu_ppe_precond : c.gpu_mom_precond, solver);
-    const bool coarse = ppe_eq && s.pat.coarse.k > 0;   // two-level preconditioner for p'
+    const bool coarse = ppe_eq && s.P->coarse.k > 0;   // two-level preconditioner for p'
     SolveResult R;
     if (ppe_eq && solver == 2) {   // red-black reduced system (ms_schur.hip)
       R = schur_solve(s, s.val.p, s.rhs.p, x, c.lin_tol, c.lin_max, c.gpu_precision == MS_MIXED, c.gpu_inner_tol,

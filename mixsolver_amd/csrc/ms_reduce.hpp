@@ -105,7 +105,7 @@ __device__ __forceinline__ float tsqrt(float x) { return sqrtf(x); }
 // logical blocks; the neighbour gathers of its rows then mostly hit its own
 // L2 instead of being fetched again by every XCD.  A bijection of [0, nb).
 inline constexpr int kXcd = 8;
-__device__ __forceinline__ inline int xcd_block(int b, int nb) {
+__device__ __forceinline__ int xcd_block(int b, int nb) {
   const int q = nb / kXcd, r = nb % kXcd, x = b % kXcd;
   return x * q + min(x, r) + b / kXcd;
 }

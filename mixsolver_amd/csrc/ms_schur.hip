@@ -212,8 +212,8 @@ void build_schur(System &s, const std::vector<int> &agg_dev, int nagg) {
 SolveResult schur_solve(System &s, const double *val, const double *b, double *x, double tol, int maxit, bool mixed,
                         double inner_tol, bool reuse) {
   SchurSystem &S = *s.schur;
-  const int n = s.pat.n, n0 = S.n0, n1 = S.n1;
-  const Mat A = matof(&s.pat), Sm = matof(&S.pat);
+  const int n = s.P->n, n0 = S.n0, n1 = S.n1;
+  const Mat A = matof(s.P), Sm = matof(S.P);
   hipStream_t st = stream();
   SolveResult R;
   hipLaunchKernelGGL(k_schur_sumsq, gc(n), dim3(kBlock), 0, st, n, b, S.part.p);
@@ -230,7 +230,7 @@ SolveResult schur_solve(System &s, const double *val, const double *b, double *x
     S.have_values = true;
     reuse = false;
   }
-  const bool coarse = S.pat.coarse.k > 0;
+  const bool coarse = S.P->coarse.k > 0;
   int total = 0;
   // a pass: r = b - A x, the reduced solve to ||b|| tol, the back
   // substitution; a second pass only when rounding left the full residual

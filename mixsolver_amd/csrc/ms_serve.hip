@@ -937,6 +937,7 @@ class Server {
       if (!round_exec_) {
         const long long l0 = launches_;
         hipGraph_t g = nullptr;
+        std::lock_guard<std::recursive_mutex> lk(device_alloc_mutex());
         MS_HIP(hipStreamBeginCapture(stream(), hipStreamCaptureModeRelaxed));
         try {
           round_body(L);

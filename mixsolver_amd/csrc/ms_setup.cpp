@@ -54,10 +54,22 @@ void ensure_runtime(int device) {
   noalloc_register_device(device);
 }
 
+static thread_local hipStream_t t_stream = nullptr;
+static thread_local long long t_dev_bytes = 0;
+
 hipStream_t stream() {
+  if (t_stream) return t_stream;
   if (!g_stream) ensure_runtime(-1);
   return g_stream;
 }
+void set_thread_stream(hipStream_t s) { t_stream = s; }
+std::recursive_mutex &device_alloc_mutex() {
+  static std::recursive_mutex m;
+  return m;
+}
+int runtime_device() { return g_device; }
+long long thread_device_bytes() { return t_dev_bytes; }
+void count_device_bytes(long long delta) { t_dev_bytes += delta; }
 
 static std::atomic<int> g_graphs{-1};   // -1: not yet read from the environment
 bool graphs_enabled() {
@@ -451,15 +463,11 @@ void Coarse::upload() {
   d_e_ptr.upload(e_ptr);
   d_e_key.upload(e_key);
   d_e_slot.upload(e_slot);
-  d_E.alloc((size_t)k * k);
-  d_Einv.alloc((size_t)k * k);
-  d_nu.alloc(k);
 }
 
 size_t Coarse::device_bytes() const {
   return d_chunk_ptr.bytes() + d_chunk_agg.bytes() + d_agg_cptr.bytes() + d_agg_cidx.bytes() + d_e_ptr.bytes() +
-         d_e_key.bytes() + d_e_slot.bytes() + d_E.bytes() + d_nu.bytes() + d_Einv.bytes() + d_W.bytes() +
-         d_rc.bytes();
+         d_e_key.bytes() + d_e_slot.bytes();
 }
 
 // ---------------------------------------------------------------- system
@@ -674,6 +682,43 @@ void periodic_match(int nnodes, const double *x, int nfaces, const long long *fp
       partner[f] = s.shadow[f];
       partner[s.shadow[f]] = f;
     }
+}
+
+// the boundary groups as the kernels see them
+static void upload_bc(System &s) {
+  std::vector<DevBC> dbc(s.nbc);
+  for (int i = 0; i < s.nbc; i++) {
+    const ms_bcgroup_t &g = s.groups[i];
+    DevBC &b = dbc[i];
+    b.btype = g.btype;
+    b.prfl_axis = g.prfl_axis;
+    b.has_lid = g.has_lid;
+    b.lid_dir = g.lid_dir;
+    for (int d = 0; d < 3; d++) {
+      b.uvel[d] = g.uvel[d];
+      b.lid_vel[d] = g.lid_vel[d];
+    }
+    b.pval = g.pval;
+    b.uspeed = g.uspeed;
+    b.prfl_mean = g.prfl_mean;
+    b.prfl_span = g.prfl_span;
+    b.prfl_origin = g.prfl_origin;
+    b.lid_coord = g.lid_coord;
+    b.lid_tol = g.lid_tol;
+  }
+  s.d_bc.upload(dbc);
+}
+
+// flux_ref (simple_run :118-130): rho times the largest boundary speed times
+// the boundary area
+static void set_flux_ref(System &s) {
+  double usc = 0.0;
+  for (int i = 0; i < s.nbc; i++) {
+    const ms_bcgroup_t &g = s.groups[i];
+    usc = std::max(usc, std::max(norm2_ref(g.lid_vel, 3), std::max(norm2_ref(g.uvel, 3), g.uspeed)));
+  }
+  if (usc <= 0.0) usc = 1.0;
+  s.flux_ref = s.ctrl.rho * usc * (s.bnd_area > 0.0 ? s.bnd_area : 1.0);
 }
 
 System *create_system(int nnodes, const double *x, int nfaces, const long long *fptr, const int *fnodes,
@@ -898,18 +943,14 @@ System *create_system(int nnodes, const double *x, int nfaces, const long long *
     s.iface_T_mode.assign(nf, 0);
     s.iface_dirty = true;
   }
-  // flux_ref (simple_run :118-130)
+  // boundary area, summed group by group in face order (simple_run :118-130)
   {
-    double usc = 0.0, ab = 0.0;
-    for (int i = 0; i < s.nbc; i++) {
-      const ms_bcgroup_t &g = s.groups[i];
-      usc = std::max(usc, std::max(norm2_ref(g.lid_vel, 3), std::max(norm2_ref(g.uvel, 3), g.uspeed)));
+    double ab = 0.0;
+    for (int i = 0; i < s.nbc; i++)
       for (int f = 0; f < nf; f++)
         if (s.fgrp[f] == i + 1) ab = ab + s.area[f];
-    }
-    if (usc <= 0.0) usc = 1.0;
-    if (ab <= 0.0) ab = 1.0;
-    s.flux_ref = ctrl->rho * usc * ab;
+    s.bnd_area = ab;
+    set_flux_ref(s);
   }
 
   // ---- device face order ----------------------------------------------------
@@ -963,27 +1004,7 @@ System *create_system(int nnodes, const double *x, int nfaces, const long long *
   s.entry_upload(s.d_e_sf, s.sf, 3);
   s.d_vol.upload(vol);
   s.d_xc.upload(xc);
-  std::vector<DevBC> dbc(s.nbc);
-  for (int i = 0; i < s.nbc; i++) {
-    const ms_bcgroup_t &g = s.groups[i];
-    DevBC &b = dbc[i];
-    b.btype = g.btype;
-    b.prfl_axis = g.prfl_axis;
-    b.has_lid = g.has_lid;
-    b.lid_dir = g.lid_dir;
-    for (int d = 0; d < 3; d++) {
-      b.uvel[d] = g.uvel[d];
-      b.lid_vel[d] = g.lid_vel[d];
-    }
-    b.pval = g.pval;
-    b.uspeed = g.uspeed;
-    b.prfl_mean = g.prfl_mean;
-    b.prfl_span = g.prfl_span;
-    b.prfl_origin = g.prfl_origin;
-    b.lid_coord = g.lid_coord;
-    b.lid_tol = g.lid_tol;
-  }
-  s.d_bc.upload(dbc);
+  upload_bc(s);
 
   // ---- fields: init_fields (mod_uns_fields.f90:86-159) ---------------------
   const size_t n3 = 3 * (size_t)nc;
@@ -1031,6 +1052,9 @@ System *create_system(int nnodes, const double *x, int nfaces, const long long *
 // per-face flags: fluid/porous faces (kink face pressure, Beavers-Joseph
 // slip), solid/fluid faces (conjugate heat transfer) and solid/solid faces.
 void System::refresh_face_constants() {
+  if ((props_dirty || iface_dirty) && (nforks.load() > 0 || base))
+    fail(MS_EINVAL, "the zone properties or interface data of a system changed while sweep cases share its "
+                    "face constants");
   if (iface_dirty) {   // interface data: device copies; thermal face modes depend on them
     face_upload(d_iface_vel, iface_vel, 3);
     face_upload(d_iface_p, iface_p);
@@ -1251,6 +1275,127 @@ void System::refresh_face_constants() {
   props_dirty = false;
 }
 
+System *fork_system(System &b, const std::vector<ms_bcgroup_t> &groups) {
+  if (b.base) fail(MS_EINVAL, "fork of a forked system");
+  if (b.has_iface) fail(MS_EINVAL, "sweep cases of a system with coupling-interface data are not supported");
+  if ((int)groups.size() != b.nbc) fail(MS_EINVAL, "sweep case: boundary group count differs from the system's");
+  for (int i = 0; i < b.nbc; i++)
+    if (groups[i].btype != b.groups[i].btype) fail(MS_EINVAL, "sweep case: boundary group types differ");
+  b.refresh_face_constants();   // no-op unless properties changed since the last run
+  std::unique_ptr<System> F(new System);
+  System &s = *F;
+  s.base = &b;
+  s.P = b.P;
+  s.nc = b.nc;
+  s.nf = b.nf;
+  s.nn = b.nn;
+  s.nbc = b.nbc;
+  s.ctrl = b.ctrl;
+  s.groups = groups;
+  s.renumbered = b.renumbered;
+  s.ncolors = b.ncolors;
+  s.nper = b.nper;
+  for (int k = 0; k < 4; k++)
+    for (int d = 0; d < 3; d++) s.psh[k][d] = b.psh[k][d];
+  s.props_dirty = false;
+  s.iface_dirty = false;
+  s.has_face_flags = b.has_face_flags;
+  s.has_solid = b.has_solid;
+  s.has_disp = b.has_disp;
+  s.has_kink = b.has_kink;
+  s.solid_consts = b.solid_consts;
+  s.bc_energy = b.bc_energy;
+  s.has_qsrc = b.has_qsrc;
+  s.T_nonzero = b.T_nonzero;
+  s.ts_order = b.ts_order;
+  s.inlet_ramp = b.inlet_ramp;
+  s.pval_ramp = b.pval_ramp;
+  s.bnd_area = b.bnd_area;
+  set_flux_ref(s);
+  s.steps_done = b.steps_done;
+  s.ts_order_in = b.ts_order_in;
+  s.pin_cell = b.pin_cell;
+  s.has_pdir = b.has_pdir;
+  s.has_outflow = b.has_outflow;
+  s.ppe_serial = b.ppe_serial;
+  s.zspan = b.zspan;
+  s.minv13_ = b.min_vol13();
+  // views of the base's read-only device data
+  s.d_c0.alias(b.d_c0);
+  s.d_c1.alias(b.d_c1);
+  s.d_fgrp.alias(b.d_fgrp);
+  s.d_cf_ptr.alias(b.d_cf_ptr);
+  s.d_cf_ent.alias(b.d_cf_ent);
+  s.d_cf_slot.alias(b.d_cf_slot);
+  s.d_diag_slot.alias(b.d_diag_slot);
+  s.d_e_oth.alias(b.d_e_oth);
+  for (auto pr : {std::make_pair(&s.d_e_lf, &b.d_e_lf), std::make_pair(&s.d_e_sf, &b.d_e_sf),
+                  std::make_pair(&s.d_e_Dm, &b.d_e_Dm), std::make_pair(&s.d_e_muf, &b.d_e_muf),
+                  std::make_pair(&s.d_e_tcor, &b.d_e_tcor), std::make_pair(&s.d_xf, &b.d_xf),
+                  std::make_pair(&s.d_sf, &b.d_sf), std::make_pair(&s.d_area, &b.d_area),
+                  std::make_pair(&s.d_lf, &b.d_lf), std::make_pair(&s.d_vol, &b.d_vol),
+                  std::make_pair(&s.d_xc, &b.d_xc), std::make_pair(&s.d_dn, &b.d_dn),
+                  std::make_pair(&s.d_dvs, &b.d_dvs), std::make_pair(&s.d_Dm, &b.d_Dm),
+                  std::make_pair(&s.d_muf, &b.d_muf), std::make_pair(&s.d_tcor, &b.d_tcor),
+                  std::make_pair(&s.d_dw, &b.d_dw), std::make_pair(&s.d_Dmb, &b.d_Dmb),
+                  std::make_pair(&s.d_Dt, &b.d_Dt), std::make_pair(&s.d_kf, &b.d_kf),
+                  std::make_pair(&s.d_Dtb, &b.d_Dtb), std::make_pair(&s.d_tval, &b.d_tval),
+                  std::make_pair(&s.d_qval, &b.d_qval), std::make_pair(&s.d_Cbj, &b.d_Cbj),
+                  std::make_pair(&s.d_kch, &b.d_kch), std::make_pair(&s.d_Ds, &b.d_Ds),
+                  std::make_pair(&s.d_ksf, &b.d_ksf), std::make_pair(&s.d_Dsb, &b.d_Dsb),
+                  std::make_pair(&s.porosity_d, &b.porosity_d), std::make_pair(&s.perm_dir_d, &b.perm_dir_d),
+                  std::make_pair(&s.inertial_d, &b.inertial_d), std::make_pair(&s.k_s_d, &b.k_s_d),
+                  std::make_pair(&s.rho_s_d, &b.rho_s_d), std::make_pair(&s.cp_s_d, &b.cp_s_d),
+                  std::make_pair(&s.disp_l_d, &b.disp_l_d), std::make_pair(&s.disp_t_d, &b.disp_t_d),
+                  std::make_pair(&s.h_sf_d, &b.h_sf_d), std::make_pair(&s.a_sf_d, &b.a_sf_d),
+                  std::make_pair(&s.q_src_d, &b.q_src_d)})
+    pr.first->alias(*pr.second);
+  s.d_tmode.alias(b.d_tmode);
+  s.d_pidx.alias(b.d_pidx);
+  s.d_fflag.alias(b.d_fflag);
+  s.d_tmode_s.alias(b.d_tmode_s);
+  s.d_cztype.alias(b.d_cztype);
+  upload_bc(s);   // the case's own boundary groups
+  // owned: the fields (copies of the base's) and the work arrays
+  hipStream_t st = stream();
+  auto own = [&](DBuf<double> &d, const DBuf<double> &src) {
+    d.alloc(src.n);
+    if (src.n) MS_HIP(hipMemcpyAsync(d.p, src.p, src.n * sizeof(double), hipMemcpyDeviceToDevice, st));
+  };
+  for (auto pr : {std::make_pair(&s.u, &b.u), std::make_pair(&s.p, &b.p), std::make_pair(&s.T, &b.T),
+                  std::make_pair(&s.T_s, &b.T_s), std::make_pair(&s.flux, &b.flux), std::make_pair(&s.apc, &b.apc),
+                  std::make_pair(&s.u_old, &b.u_old), std::make_pair(&s.u_old_old, &b.u_old_old),
+                  std::make_pair(&s.T_old, &b.T_old), std::make_pair(&s.T_old_old, &b.T_old_old),
+                  std::make_pair(&s.T_s_old, &b.T_s_old), std::make_pair(&s.T_s_old_old, &b.T_s_old_old),
+                  std::make_pair(&s.gp, &b.gp), std::make_pair(&s.gu, &b.gu), std::make_pair(&s.gt, &b.gt),
+                  std::make_pair(&s.gts, &b.gts), std::make_pair(&s.psi, &b.psi), std::make_pair(&s.kdisp, &b.kdisp),
+                  std::make_pair(&s.gp_prev, &b.gp_prev), std::make_pair(&s.val, &b.val),
+                  std::make_pair(&s.rhs, &b.rhs), std::make_pair(&s.ap, &b.ap), std::make_pair(&s.pp, &b.pp),
+                  std::make_pair(&s.dpp, &b.dpp), std::make_pair(&s.af, &b.af), std::make_pair(&s.ucomp, &b.ucomp),
+                  std::make_pair(&s.part, &b.part), std::make_pair(&s.diag8, &b.diag8),
+                  std::make_pair(&s.tmpc, &b.tmpc)})
+    own(*pr.first, *pr.second);
+  s.ticket.alloc(b.ticket.n);
+  s.ticket.zero();
+  s.kry.reset(new Krylov(s.P));
+  if (b.schur) {
+    auto R = std::make_unique<SchurSystem>();
+    R->n0 = b.schur->n0;
+    R->n1 = b.schur->n1;
+    R->P = b.schur->P;
+    R->kry = std::make_unique<Krylov>(R->P);
+    R->val.alloc(R->P->nstore);
+    R->rhs.alloc(R->n1);
+    R->d.alloc(R->n1);
+    R->r64.alloc(s.nc);
+    R->part.alloc(nblocks(s.nc) + 64);
+    s.schur = std::move(R);
+  }
+  MS_HIP(hipStreamSynchronize(st));
+  b.nforks++;
+  return F.release();
+}
+
 DMesh System::dmesh() const {
   DMesh m{};
   m.nc = nc;
@@ -1279,7 +1424,7 @@ DMesh System::dmesh() const {
   m.xc = d_xc.p;
   m.cf_ptr = d_cf_ptr.p;
   m.cf_ent = d_cf_ent.p;
-  m.pair_n0 = pat.color_ptr.size() == 3 ? pat.color_ptr[1] : 0;
+  m.pair_n0 = P->color_ptr.size() == 3 ? P->color_ptr[1] : 0;
   m.e_oth = d_e_oth.p;
   m.e_lf = d_e_lf.p;
   m.e_sf = d_e_sf.p;

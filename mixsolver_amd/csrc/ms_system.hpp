@@ -1,6 +1,7 @@
 // ms_system.hpp -- one mesh + fields + matrices resident on the device.
 #pragma once
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <memory>
 
@@ -72,6 +73,7 @@ struct DFields {
 struct SchurSystem {
   int n0 = 0, n1 = 0;           // colour-0 rows [0,n0), colour-1 rows [n0,n0+n1)
   SellPattern pat;              // pattern of S, colour-1 local numbering
+  const SellPattern *P = &pat;  // the pattern in use (a forked system's points to its base's)
   std::unique_ptr<Krylov> kry;  // PCG on S (Jacobi, coarse space of the colour-1 aggregates)
   DBuf<double> val, rhs, d, r64, part;
   std::vector<double> hpart;
@@ -119,6 +121,12 @@ struct System {
   std::vector<int> ref_row_ptr, ref_col;
   std::vector<long long> ref2dev;
   SellPattern pat;
+  // the matrix pattern in use: pat, or the base system's on a forked system
+  // (fork_system), which shares its base's mesh, geometry, face constants,
+  // zone properties and patterns and owns its fields, matrices and solvers
+  const SellPattern *P = &pat;
+  System *base = nullptr;               // set on a forked system
+  std::atomic<int> nforks{0};           // live forks of this system
   std::unique_ptr<Krylov> kry;
   std::unique_ptr<SchurSystem> schur;   // gpu_ppe_solver = schur
   // per-cell properties (natural order, host copy)
@@ -134,6 +142,7 @@ struct System {
   int ts_order = 0;
   double inlet_ramp = 1.0, pval_ramp = 1.0;
   double flux_ref = 1.0;
+  double bnd_area = 0.0;   // boundary area (flux_ref)
   // device mesh data
   DBuf<int> d_c0, d_c1, d_fgrp, d_cf_ptr, d_cf_ent;
   DBuf<long long> d_cf_slot, d_diag_slot;
@@ -204,11 +213,26 @@ struct System {
   DFields dfields();
   void refresh_face_constants();
   size_t device_bytes() const;
+  System() = default;
+  System(const System &) = delete;
+  System &operator=(const System &) = delete;
+  ~System() {
+    if (base) base->nforks--;
+  }
 };
 
 System *create_system(int nnodes, const double *x, int nfaces, const long long *fptr, const int *fnodes,
                       const int *c0, const int *c1, const int *zone, int ncells, const ms_ctrl_t *ctrl,
                       const ms_bcgroup_t *groups);
+
+// A case of a parameter sweep: a new system on the calling thread's stream
+// that views base's mesh, geometry, face constants, zone properties and
+// matrix patterns (read-only, no copy) and owns its fields (a copy of
+// base's), matrices and solvers.  groups replaces base's boundary groups
+// (same count and types; a sweep changes inlet speeds).  Base must outlive
+// the fork and must not change its mesh-wide data while forks exist.  A fork
+// supports the drivers and field reads; the host topology stays with base.
+System *fork_system(System &base, const std::vector<ms_bcgroup_t> &groups);
 
 // Host only: partner[f] = the face joined with boundary face f by the
 // periodic pairs of ctrl (both directions), -1 elsewhere; same matching and

@@ -10,6 +10,14 @@ and an optional aggregate coarse correction.  Every admitted request is packed
 into every launch; a request's result does not depend on the other requests in
 flight.
 
+``Sweep`` wraps the parameter-sweep server (``ms_sweep_*``): whole
+transient PISO cases of one mesh that differ in an inlet speed, forked from
+one resident ``Case`` (they share its mesh, geometry, face constants, zone
+properties and matrix patterns on the device and own their fields, matrices
+and solver state), admitted by device-memory budget and run concurrently,
+one host thread and HIP stream per case in flight.  A case's results do not
+depend on the mix.
+
 ``LaplacianPattern`` builds the pressure-correction-like SPD systems of the
 workload on a jittered hex or prism mesh: face coefficients
 ``a_f = k_f |S_f| / |d_f|`` with log-normal face diffusivity ``k_f``,
@@ -26,6 +34,87 @@ from . import _lib, meshgen
 from ._lib import check, dp, ip
 
 PRECOND = {"jacobi": 0, "dilu": 2}
+
+
+SWEEP_STATS = ("submitted", "done", "failed", "queued", "active", "peak_active", "footprint_bytes",
+               "budget_bytes", "base_device_bytes", "steps_run")
+CASE_STATE = ("queued", "running", "done", "failed")
+
+
+class Sweep:
+    """Parameter sweep over whole transient cases forked from ``base`` (a
+    ``Case`` with a transient control; its fields are the cases' initial
+    fields).  ``budget_bytes`` 0: 90% of the free device memory."""
+
+    def __init__(self, base, budget_bytes=0, max_active=8):
+        self.base = base
+        self.h = C.c_void_p()
+        check(_lib.lib().ms_sweep_create(C.byref(self.h), base.h, int(budget_bytes), int(max_active)))
+        self._cases = {}
+
+    def submit(self, nsteps, group=0, uvel=None, speed=0.0, keep_fields=False, zone=None):
+        """Queue a case of ``nsteps`` PISO steps; ``group`` (1-based index of
+        the base's boundary groups, or ``zone``: the group of that boundary
+        zone id) is the inlet whose speed it sets: ``uvel`` for a velocity
+        inlet or far field, ``speed`` for a parabolic (mean speed) or mass
+        inlet.  Returns its id."""
+        if zone is not None:
+            ids = [int(g.zone) for g in self.base._garr]
+            if int(zone) not in ids:
+                raise ValueError(f"no boundary group for zone {zone}")
+            group = ids.index(int(zone)) + 1
+        uv = None
+        if uvel is not None:
+            uv = np.ascontiguousarray(uvel, np.float64).reshape(3)
+        i = C.c_longlong(0)
+        check(_lib.lib().ms_sweep_submit(self.h, int(group), dp(uv) if uv is not None else None, float(speed),
+                                         int(nsteps), int(bool(keep_fields)), C.byref(i)))
+        self._cases[i.value] = (int(nsteps), bool(keep_fields), uv)
+        return i.value
+
+    def result(self, case_id, wait_s=0.0):
+        """dict(state, steps, ier, step_ms, history, t_submit, t_admit, t_done,
+        stats, u, p, T, T_s) once the case is done; dict(state) before.  A
+        failed case raises MixSolverError with its message."""
+        nsteps, keep, _ = self._cases[case_id]
+        nc = self.base.nc
+        st, steps, ier = C.c_int(0), C.c_int(0), C.c_int(0)
+        step_ms, hist, times = np.zeros(nsteps), np.zeros(8 * nsteps), np.zeros(3)
+        stats = np.zeros(8, np.int64)
+        f = dict(u=np.zeros(3 * nc), p=np.zeros(nc), T=np.zeros(nc), T_s=np.zeros(nc)) if keep else {}
+        check(_lib.lib().ms_sweep_result(
+            self.h, C.c_longlong(case_id), float(wait_s), C.byref(st), C.byref(steps), C.byref(ier), dp(step_ms),
+            dp(hist), dp(times), stats.ctypes.data_as(C.POINTER(C.c_longlong)),
+            *(dp(f[k]) if keep else None for k in ("u", "p", "T", "T_s"))))
+        out = dict(state=CASE_STATE[st.value])
+        if st.value != 2:
+            return out
+        n = steps.value
+        out.update(steps=n, ier=ier.value, step_ms=step_ms[:n], history=hist[: 8 * n].reshape(-1, 8),
+                   t_submit=times[0], t_admit=times[1], t_done=times[2],
+                   stats=dict(zip(("ppe_solves", "ppe_iters_fp32", "ppe_iters_fp64", "ppe_fp64_solves",
+                                   "fp32_stalled", "stall_window", "mom_solves", "mom_iters"),
+                                  (int(v) for v in stats))))
+        if keep:
+            f["u"] = f["u"].reshape(-1, 3)
+            out.update(f)
+        return out
+
+    def stats(self):
+        out = np.zeros(10, np.int64)
+        check(_lib.lib().ms_sweep_stats(self.h, out.ctypes.data_as(C.POINTER(C.c_longlong))))
+        return dict(zip(SWEEP_STATS, (int(v) for v in out)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            check(_lib.lib().ms_sweep_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class ServerOpts(C.Structure):
