@@ -20,7 +20,11 @@ Workload (one step = one PISO time step of piso_run_mpi at one partition):
     correction on the full system, mixed and FP64;
   * fp64: PCG with multicolour DILU, FP64, no coarse correction.
 p50/p99 step latency and the iteration counts are reported per variant.
-Inputs are resident in HBM before timing.
+Inputs are resident in HBM before timing.  After the variants: the serving
+workload (Poisson arrivals of p' solves, at 200 and 100 req/s) and the
+parameter-sweep workload (32 porous LTNE cases of 16M cells, forked from one
+resident system, per-step p50/p99 latency; --no-sweep skips it,
+--sweep-active 32,4,1 compares in-flight limits).
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
 """
@@ -275,6 +279,118 @@ def serve_bench(rate, seconds, pool, coarse, seed=4):
     }
 
 
+SWEEP_CTL = """rho = 1.0
+mu = 0.05
+transient = true
+dt = 0.002
+n_time_max = 1
+n_correct = 2
+time_scheme = 2
+lin_tol = 1e-10
+lin_max = 20000
+conv_blend = 0.5
+nonorth_corr = 1.0
+ppe_precond = jacobi
+k_cond = 0.1
+cp = 1.0
+thermal_model = ltne
+bc = 6 velocity-inlet-parabolic 0.05 2.0 2 -1.0 1.0
+bc = 7 outflow
+bc = 5 wall
+bc = 4 symmetry
+tbc = 5 1 0.5
+cell_zone = 8 porous perm=0.01 perm_yy=0.02 inertial=0.5 porosity=0.4 disp_l=0.01 disp_t=0.002 bj_alpha=0.3 k_s=2.0 cp_s=0.8 rho_s=3.0 h_sf=5.0 a_sf=10.0
+"""
+
+
+def sweep_bench(size, ncases, nsteps, max_active, coarse, seed=7):
+    """Parameter-sweep workload: a porous channel (8 x 2 x 1, nx x ny x nz
+    hex cells, one porous zone: Darcy-Forchheimer source, anisotropic
+    permeability, dispersion, LTNE fluid and solid energy equations coupled
+    by h_sf a_sf, wall at fixed temperature) with a seeded porosity field in
+    [0.4, 0.9] (Gaussian noise smoothed over 10 cells); ncases cases with
+    seeded inlet mean speeds uniform in [0.01, 0.1] m/s (parabolic inlet),
+    starting from rest, nsteps PISO steps each (dt 0.002, 2 correctors,
+    BDF2), p' through the red-black reduced system in mixed precision to
+    1e-10; all cases submitted at once and admitted by device-memory budget
+    (90% of the free HBM after the base system)."""
+    from scipy.ndimage import gaussian_filter
+    from mixsolver_amd import meshgen, zones
+    from mixsolver_amd.control import parse_control
+    from mixsolver_amd.serve import Sweep
+    from mixsolver_amd.solver import Case
+    nx, ny, nz = (int(v) for v in size.split("x"))
+    t0 = time.perf_counter()
+    m = meshgen.hex_box(np.linspace(0.0, 8.0, nx + 1), np.linspace(-1.0, 1.0, ny + 1), np.linspace(0.0, 1.0, nz + 1),
+                        zone_ids=dict(xmin=6, xmax=7, ymin=5, ymax=5, zmin=4, zmax=4))
+    m = meshgen.with_cell_zones(m, np.full(m.ncells, 8), {8: ("porous", "bed")})
+    ctrl, _ = parse_control(SWEEP_CTL, warn=False)
+    cz = zones.resolve_cell_zones(m, ctrl.cell_zones)
+    f = zones.setup_porous_fields(m, ctrl.cell_zones, cz)
+    g = gaussian_filter(np.random.default_rng(seed).standard_normal((nz, ny, nx)), 10.0, mode="wrap").ravel()
+    f["porosity"] = 0.4 + 0.5 * (g - g.min()) / (g.max() - g.min())
+    c2, g2 = parse_control(SWEEP_CTL, warn=False)
+    c2.cell_zones = []
+    for k, v in dict(gpu_ppe_precond=0, gpu_ppe_solver=2, gpu_mom_precond=2, gpu_coarse=coarse, gpu_precision=1,
+                     gpu_inner_tol=0.1).items():
+        setattr(c2, k, v)
+    base = Case(m, c2, g2)
+    base.set_cell_types(cz)
+    for k, v in f.items():
+        base.set_field(k, v)
+    t_setup = time.perf_counter() - t0
+    print(f"bench: sweep base {m.ncells} cells set up in {t_setup:.1f} s", file=sys.stderr, flush=True)
+    speeds = np.random.default_rng(seed + 1).uniform(0.01, 0.1, ncases)
+    runs = []
+    for act in max_active:
+        sw = Sweep(base, budget_bytes=0, max_active=act)
+        t1 = time.perf_counter()
+        ids = [sw.submit(nsteps, zone=6, speed=float(v)) for v in speeds]
+        res = []
+        for i in ids:
+            res.append(sw.result(i, wait_s=3600.0))
+            print(f"bench: sweep (at most {act} in flight) case {len(res)}/{ncases} done at "
+                  f"{time.perf_counter() - t1:.1f} s", file=sys.stderr, flush=True)
+        span = time.perf_counter() - t1
+        st = sw.stats()
+        sw.close()
+        steps = np.concatenate([r["step_ms"] for r in res])
+        later = np.concatenate([r["step_ms"][1:] for r in res]) if nsteps > 1 else steps
+        first = min(res, key=lambda r: r["t_admit"])
+        bad = [dict(speed=float(v), ier=r["ier"], steps=r["steps"], ppe_res=r["history"][:, 7].tolist(),
+                    ppe_its=r["history"][:, 6].tolist())
+               for v, r in zip(speeds, res)
+               if not (r["ier"] == 0 and r["steps"] == nsteps and (r["history"][:, 7] <= 1e-10).all())]
+        nsteps_all = int(sum(r["steps"] for r in res))
+        lat = [r["t_done"] - r["t_submit"] for r in res]
+        runs.append({
+            "max_in_flight": act, "peak_cases_in_flight": st["peak_active"],
+            "converged_cases": ncases - len(bad), "unconverged": bad,
+            "ppe_fp64_solves": int(sum(r["stats"]["ppe_fp64_solves"] for r in res)),
+            "fp32_stalled_cases": int(sum(r["stats"]["fp32_stalled"] for r in res)),
+            "ppe_iters_per_step": float(np.mean(np.concatenate([r["history"][:, 6] for r in res]))),
+            "p50_step_ms": float(np.percentile(steps, 50)), "p99_step_ms": float(np.percentile(steps, 99)),
+            "p50_later_step_ms": float(np.percentile(later, 50)),
+            "p99_later_step_ms": float(np.percentile(later, 99)),
+            "first_step_alone_ms": float(first["step_ms"][0]),
+            "makespan_s": span, "case_steps_per_s": nsteps_all / span,
+            "cell_steps_per_s": nsteps_all * m.ncells / span,
+            "p50_case_latency_s": float(np.percentile(lat, 50)), "p99_case_latency_s": float(np.percentile(lat, 99)),
+            "footprint_GB_per_case": st["footprint_bytes"] / 1e9, "budget_GB": st["budget_bytes"] / 1e9,
+        })
+    nbytes = base.info["device_bytes"]
+    base.close()
+    return {
+        "workload": f"porous LTNE channel {size} hex ({m.ncells} cells), seeded porosity [0.4, 0.9] (smoothed, "
+                    f"correlation 10 cells), Darcy-Forchheimer, fluid + solid energy (h_sf a_sf); {ncases} cases, "
+                    f"seeded inlet mean speeds U(0.01, 0.1) m/s, {nsteps} PISO steps each from rest; p' reduced "
+                    f"system + {coarse}-aggregate coarse, FP32 inner / FP64 to 1e-10; all submitted at t = 0, "
+                    "admitted by HBM budget (90% of the free HBM)",
+        "cells": int(m.ncells), "cases": ncases, "steps_per_case": nsteps, "base_setup_s": t_setup,
+        "base_GB": nbytes / 1e9, "runs": runs,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -294,6 +410,12 @@ def main():
                     help="in-loop SpMV timing of the lead variant: inside its timed region, over as many extra "
                          "steps after it, or none")
     ap.add_argument("--cpu-full", action="store_true", help="CPU baseline on one full step (minutes), no scaling")
+    ap.add_argument("--no-sweep", action="store_true", help="skip the parameter-sweep workload")
+    ap.add_argument("--sweep-size", default="800x200x100")
+    ap.add_argument("--sweep-cases", type=int, default=32)
+    ap.add_argument("--sweep-steps", type=int, default=2)
+    ap.add_argument("--sweep-active", default="4",
+                    help="most cases in flight, one sweep per value (the HBM budget also bounds them)")
     ap.add_argument("--cpu-leg", type=int, default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.cpu_leg is not None:
@@ -507,6 +629,10 @@ def main():
         out["serving_below_saturation"] = serve_bench(args.serve_rate / 2, args.serve_seconds, args.serve_pool,
                                                       min(args.coarse, 256))
         print("bench: serving done", file=sys.stderr, flush=True)
+    if not args.no_sweep and args.sweep_cases > 0:
+        out["sweep"] = sweep_bench(args.sweep_size, args.sweep_cases, args.sweep_steps,
+                                   [int(v) for v in args.sweep_active.split(",")], args.coarse)
+        print("bench: sweep done", file=sys.stderr, flush=True)
     if not args.no_cpu:
         # the oracle runs the non-periodic twin (the reference has no periodic
         # boundary); p' iterations its Jacobi PCG needs for the first step come

@@ -247,7 +247,12 @@ SolveResult schur_solve(System &s, const double *val, const double *b, double *x
     hipLaunchKernelGGL(k_schur_rhs, gc(n1), dim3(kBlock), 0, st, A, val, n0, n1, S.r64.p, S.rhs.p, S.d.p, S.part.p);
     MS_CHECK_LAUNCH();
     const double rsn = std::sqrt(host_sum(S, nblocks(n1)));
-    const double tol_s = rsn > 0.0 ? tol * bnrm / rsn : tol;
+    double tol_s = rsn > 0.0 ? tol * bnrm / rsn : tol;
+    // A correction pass follows a pass whose reduced residual met its target
+    // while the full residual, evaluated with A rather than with the formed
+    // S, stayed just above tol by rounding; the correction then asks for a
+    // tenfold reduction at least, so that it does not end at the same margin.
+    if (pass > 0) tol_s = std::min(tol_s, 0.1);
     SolveResult I = mixed ? S.kry->mixed(S.val.p, S.rhs.p, S.d.p, tol_s, maxit - total, PC_JACOBI, 0, inner_tol,
                                          coarse, reuse || pass > 0)
                           : S.kry->pcg(S.val.p, S.rhs.p, S.d.p, tol_s, maxit - total, PC_JACOBI, coarse,
