@@ -104,6 +104,13 @@ int ms_version(void);
  * start-up) or launch their kernels one by one (0).  Results are identical
  * either way.  No reference counterpart (library option). */
 int ms_set_graphs(int on);
+/* The drivers assemble the three momentum components' interior faces in one
+ * pass (1, default; MIXSOLVER_MOM_FUSED=0 sets 0 at start-up) where that is
+ * exact: no Beavers-Joseph faces and, with the convection limiter on, no
+ * symmetry or slip-wall group; else (0) one pass per component as
+ * momentum_assembly (mod_uns_simple.f90:928-1338).  Results are identical
+ * either way.  No reference counterpart (library option). */
+int ms_set_fused_momentum(int on);
 
 /* vtk_write (mod_uns_output.f90:81-240, tetrahedral fan) through vtu_write
  * (mod_vtu_writer.f90): binary = 0 inline ASCII (Float64, ES24.16E3),

@@ -50,6 +50,11 @@ std::recursive_mutex &device_alloc_mutex();
 // environment variable MIXSOLVER_GRAPHS=0 or ms_set_graphs(0) turns it off)
 bool graphs_enabled();
 void set_graphs(bool on);
+// the drivers assemble the three momentum components' interior faces in one
+// pass where that is exact (default on; MIXSOLVER_MOM_FUSED=0 or
+// ms_set_fused_momentum(0) turns it off)
+bool momentum_fused_enabled();
+void set_momentum_fused(bool on);
 // the mixed PCG queues its next chunk before reading the current chunk's
 // state (MIXSOLVER_PIPELINE=0 turns it off)
 bool pipeline_enabled();

@@ -277,11 +277,9 @@ __global__ void k_limiter(DMesh m, DFields F, Ramp rp, int comp, double *psi) {
   double pmin = uc, pmax = uc;
   const int k0 = m.cf_ptr[r], k1 = m.cf_ptr[r + 1];
   for (int k = k0; k < k1; k++) {
-    const int e = m.cf_ent[k], f = e >> 1, side = e & 1;
-    const int b = m.c1[f];
+    const int f = m.cf_ent[k] >> 1, o = m.e_oth[k];
     double v;
-    if (b >= 0) {
-      const int o = side == 0 ? b : m.c0[f];
+    if (o >= 0) {
       v = isT ? F.T[o] : F.u[3 * (size_t)o + comp];
     } else if (isT) {
       v = bc_face_T(m, f, F.T[r]);
@@ -307,69 +305,60 @@ __global__ void k_limiter(DMesh m, DFields F, Ramp rp, int comp, double *psi) {
   psi[r] = dmax(0.0, dmin(1.0, ps));
 }
 
+// The three velocity components' limiters in one pass: psi[c * n + r].
+// Equal to three k_limiter launches when no boundary face value of one
+// component depends on another component (no symmetry or slip-wall group),
+// so that computing the limiters of components 1 and 2 before the solve of
+// component 0 changes nothing.
+__global__ void k_limiter3(DMesh m, DFields F, Ramp rp, double *psi) {
+  const int r = cell_row(m);
+  if (r < 0) return;
+  const double *uP = &F.u[3 * (size_t)r];
+  double uc[3], pmin[3], pmax[3];
+  for (int c = 0; c < 3; c++) uc[c] = pmin[c] = pmax[c] = uP[c];
+  const int k0 = m.cf_ptr[r], k1 = m.cf_ptr[r + 1];
+  for (int k = k0; k < k1; k++) {
+    const int o = m.e_oth[k];
+    double v[3];
+    if (o >= 0) {
+      for (int c = 0; c < 3; c++) v[c] = F.u[3 * (size_t)o + c];
+    } else {
+      bc_face_vel(m, m.cf_ent[k] >> 1, uP, v, rp);
+    }
+    for (int c = 0; c < 3; c++) {
+      pmin[c] = dmin(pmin[c], v[c]);
+      pmax[c] = dmax(pmax[c], v[c]);
+    }
+  }
+  double ps[3] = {1.0, 1.0, 1.0}, g[3][3];
+  for (int c = 0; c < 3; c++)
+    for (int d = 0; d < 3; d++) g[c][d] = F.gu[9 * (size_t)r + c + 3 * d];
+  for (int k = k0; k < k1; k++) {
+    const int e = m.cf_ent[k];
+    double rr[3];
+    face_offset(m, e >> 1, r, (e & 1) != 0, rr);
+    for (int c = 0; c < 3; c++) {
+      const double fr = uc[c] + ddot3(g[c], rr);
+      const double df = fr - uc[c];
+      if (df > 0.0) ps[c] = dmin(ps[c], (pmax[c] - uc[c]) / df);
+      else if (df < 0.0) ps[c] = dmin(ps[c], (pmin[c] - uc[c]) / df);
+    }
+  }
+  for (int c = 0; c < 3; c++) psi[(size_t)c * m.nc + r] = dmax(0.0, dmin(1.0, ps[c]));
+}
+
 // --------------------------------------------------------------- momentum
 struct MomParams {
   int comp, ts_order, pimple, boussinesq, body_on;
   double rho, mu, alpha_u, dt, beta, tref, blend, nonorth, grav, body;
 };
 
-__global__ void k_momentum(DMesh m, DFields F, Ramp rp, MomParams q, const double *psi, double *val, double *rhs_out,
-                           double *ap_out) {
-  const int r = cell_row(m);
-  if (r < 0) return;
+// The boundary faces, sources, time terms and the diagonal of one
+// component (:1123-1338), continuing ap and rhs from the interior faces.
+__device__ __forceinline__ void mom_tail(const DMesh &m, const DFields &F, const Ramp &rp, const MomParams &q, int r,
+                                         int k0, int k1, double ap, double rhs, double *val, double *rhs_out,
+                                         double *ap_out) {
   const int comp = q.comp;
-  const int k0 = m.cf_ptr[r], k1 = m.cf_ptr[r + 1];
-  double ap = 0.0, rhs = 0.0;
-  for (int k = k0; k < k1; k++)
-    if (m.cf_slot[k] >= 0) val[m.cf_slot[k]] = 0.0;
-  // interior faces (:974-1120)
-  for (int k = k0; k < k1; k++) {
-    const int e = m.cf_ent[k], f = e >> 1, side = e & 1;
-    const int o = m.e_oth[k];
-    if (o < 0) continue;
-    const int a = side == 0 ? r : o, b = side == 0 ? o : r;   // c0 and c1 of the face
-    const double Ff = F.flux[f], D = m.e_Dm[k], l = m.e_lf[k];
-    const long long sl = m.cf_slot[k];
-    const int fl = face_flags(m, f);
-    double Dface = D;
-    if (fl & FF_BJ) {   // Beavers-Joseph slip at a fluid/porous face (:1031-1059)
-      const double Cb = m.Cbj[f], ar = m.area[f];
-      double nh[3];
-      for (int d = 0; d < 3; d++) nh[d] = m.sf[3 * (size_t)f + d] / ar;
-      Dface = Cb + (D - Cb) * (nh[comp] * nh[comp]);
-      const double *u0 = &F.u[3 * (size_t)a], *u1 = &F.u[3 * (size_t)b];
-      const double rx = (D - Cb) * nh[comp] *
-                        ((u0[0] - u1[0]) * nh[0] + (u0[1] - u1[1]) * nh[1] + (u0[2] - u1[2]) * nh[2] -
-                         (u0[comp] - u1[comp]) * nh[comp]);
-      rhs = side == 0 ? rhs + rx : rhs - rx;
-    }
-    if (side == 0) {
-      ap = ap + dmax(Ff, 0.0) + Dface;
-      val[sl] = val[sl] - (Dface + dmax(-Ff, 0.0));
-    } else {
-      ap = ap + dmax(-Ff, 0.0) + Dface;
-      val[sl] = val[sl] - (Dface + dmax(Ff, 0.0));
-    }
-    if (m.blend_on) {
-      const int cu = Ff > 0.0 ? a : b;
-      double g[3], rr[3];
-      for (int d = 0; d < 3; d++) g[d] = F.gu[9 * (size_t)cu + comp + 3 * d];
-      face_offset(m, f, cu, !(Ff > 0.0), rr);
-      double dc = psi[cu] * ddot3(g, rr);
-      dc = q.blend * Ff * dc;
-      rhs = side == 0 ? rhs - dc : rhs + dc;
-    }
-    if (m.nonorth && !(fl & FF_BJ)) {
-      double gf[3];
-      for (int d = 0; d < 3; d++)
-        gf[d] = l * F.gu[9 * (size_t)a + comp + 3 * d] + (1.0 - l) * F.gu[9 * (size_t)b + comp + 3 * d];
-      const double noc = q.nonorth * m.e_muf[k] * ddot3(gf, &m.e_tcor[3 * (size_t)k]);
-      rhs = side == 0 ? rhs + noc : rhs - noc;
-    }
-    const double pf = (fl & FF_KINK) ? kink_pf(m, F.p, F.gp, f, a, b) : l * F.p[a] + (1.0 - l) * F.p[b];
-    const double Sc = m.e_sf[3 * (size_t)k + comp];
-    rhs = side == 0 ? rhs - pf * Sc : rhs + pf * Sc;
-  }
   // boundary faces (:1123-1227)
   const double *uP = &F.u[3 * (size_t)r];
   for (int k = k0; k < k1; k++) {
@@ -451,6 +440,141 @@ __global__ void k_momentum(DMesh m, DFields F, Ramp rp, MomParams q, const doubl
   }
   rhs_out[r] = rhs;
   ap_out[r] = ap;
+}
+
+__global__ void k_momentum(DMesh m, DFields F, Ramp rp, MomParams q, const double *psi, double *val, double *rhs_out,
+                           double *ap_out) {
+  const int r = cell_row(m);
+  if (r < 0) return;
+  const int comp = q.comp;
+  const int k0 = m.cf_ptr[r], k1 = m.cf_ptr[r + 1];
+  double ap = 0.0, rhs = 0.0;
+  for (int k = k0; k < k1; k++)
+    if (m.cf_slot[k] >= 0) val[m.cf_slot[k]] = 0.0;
+  // interior faces (:974-1120)
+  for (int k = k0; k < k1; k++) {
+    const int e = m.cf_ent[k], f = e >> 1, side = e & 1;
+    const int o = m.e_oth[k];
+    if (o < 0) continue;
+    const int a = side == 0 ? r : o, b = side == 0 ? o : r;   // c0 and c1 of the face
+    const double Ff = F.flux[f], D = m.e_Dm[k], l = m.e_lf[k];
+    const long long sl = m.cf_slot[k];
+    const int fl = face_flags(m, f);
+    double Dface = D;
+    if (fl & FF_BJ) {   // Beavers-Joseph slip at a fluid/porous face (:1031-1059)
+      const double Cb = m.Cbj[f], ar = m.area[f];
+      double nh[3];
+      for (int d = 0; d < 3; d++) nh[d] = m.sf[3 * (size_t)f + d] / ar;
+      Dface = Cb + (D - Cb) * (nh[comp] * nh[comp]);
+      const double *u0 = &F.u[3 * (size_t)a], *u1 = &F.u[3 * (size_t)b];
+      const double rx = (D - Cb) * nh[comp] *
+                        ((u0[0] - u1[0]) * nh[0] + (u0[1] - u1[1]) * nh[1] + (u0[2] - u1[2]) * nh[2] -
+                         (u0[comp] - u1[comp]) * nh[comp]);
+      rhs = side == 0 ? rhs + rx : rhs - rx;
+    }
+    if (side == 0) {
+      ap = ap + dmax(Ff, 0.0) + Dface;
+      val[sl] = val[sl] - (Dface + dmax(-Ff, 0.0));
+    } else {
+      ap = ap + dmax(-Ff, 0.0) + Dface;
+      val[sl] = val[sl] - (Dface + dmax(Ff, 0.0));
+    }
+    if (m.blend_on) {
+      const int cu = Ff > 0.0 ? a : b;
+      double g[3], rr[3];
+      for (int d = 0; d < 3; d++) g[d] = F.gu[9 * (size_t)cu + comp + 3 * d];
+      face_offset(m, f, cu, !(Ff > 0.0), rr);
+      double dc = psi[cu] * ddot3(g, rr);
+      dc = q.blend * Ff * dc;
+      rhs = side == 0 ? rhs - dc : rhs + dc;
+    }
+    if (m.nonorth && !(fl & FF_BJ)) {
+      double gf[3];
+      for (int d = 0; d < 3; d++)
+        gf[d] = l * F.gu[9 * (size_t)a + comp + 3 * d] + (1.0 - l) * F.gu[9 * (size_t)b + comp + 3 * d];
+      const double noc = q.nonorth * m.e_muf[k] * ddot3(gf, &m.e_tcor[3 * (size_t)k]);
+      rhs = side == 0 ? rhs + noc : rhs - noc;
+    }
+    const double pf = (fl & FF_KINK) ? kink_pf(m, F.p, F.gp, f, a, b) : l * F.p[a] + (1.0 - l) * F.p[b];
+    const double Sc = m.e_sf[3 * (size_t)k + comp];
+    rhs = side == 0 ? rhs - pf * Sc : rhs + pf * Sc;
+  }
+  mom_tail(m, F, rp, q, r, k0, k1, ap, rhs, val, rhs_out, ap_out);
+}
+
+// The interior faces of all three components in one pass (the momentum
+// matrix's off-diagonals are the same for the three; ap and rhs3 hold each
+// component's sums over the interior faces, which k_mom_tail continues).
+// Every component's operations are those of k_momentum in the same order,
+// so the result is the same bit for bit, provided no interior face term of
+// one component reads another component's velocity (no Beavers-Joseph
+// faces) and the limiters psi[c * n + r] of all three were computed before
+// the first solve (k_limiter3).  The face data, the flux, the neighbours'
+// gradients and the matrix slots are read once for the three components.
+__global__ void k_mom_interior3(DMesh m, DFields F, MomParams q, const double *psi, double *val, double *rhs3,
+                                double *ap1) {
+  const int r = cell_row(m);
+  if (r < 0) return;
+  const int k0 = m.cf_ptr[r], k1 = m.cf_ptr[r + 1];
+  double ap = 0.0, rhs[3] = {0.0, 0.0, 0.0};
+  for (int k = k0; k < k1; k++)
+    if (m.cf_slot[k] >= 0) val[m.cf_slot[k]] = 0.0;
+  for (int k = k0; k < k1; k++) {
+    const int e = m.cf_ent[k], f = e >> 1, side = e & 1;
+    const int o = m.e_oth[k];
+    if (o < 0) continue;
+    const int a = side == 0 ? r : o, b = side == 0 ? o : r;
+    const double Ff = F.flux[f], D = m.e_Dm[k], l = m.e_lf[k];
+    const long long sl = m.cf_slot[k];
+    const int fl = face_flags(m, f);
+    if (side == 0) {
+      ap = ap + dmax(Ff, 0.0) + D;
+      val[sl] = val[sl] - (D + dmax(-Ff, 0.0));
+    } else {
+      ap = ap + dmax(-Ff, 0.0) + D;
+      val[sl] = val[sl] - (D + dmax(Ff, 0.0));
+    }
+    if (m.blend_on) {
+      const int cu = Ff > 0.0 ? a : b;
+      double rr[3];
+      face_offset(m, f, cu, !(Ff > 0.0), rr);
+      const double *gc = &F.gu[9 * (size_t)cu];
+      for (int c = 0; c < 3; c++) {
+        double g[3];
+        for (int d = 0; d < 3; d++) g[d] = gc[c + 3 * d];
+        double dc = psi[(size_t)c * m.nc + cu] * ddot3(g, rr);
+        dc = q.blend * Ff * dc;
+        rhs[c] = side == 0 ? rhs[c] - dc : rhs[c] + dc;
+      }
+    }
+    if (m.nonorth) {
+      const double *ga = &F.gu[9 * (size_t)a], *gb = &F.gu[9 * (size_t)b];
+      const double mf = m.e_muf[k];
+      const double *tc = &m.e_tcor[3 * (size_t)k];
+      for (int c = 0; c < 3; c++) {
+        double gf[3];
+        for (int d = 0; d < 3; d++) gf[d] = l * ga[c + 3 * d] + (1.0 - l) * gb[c + 3 * d];
+        const double noc = q.nonorth * mf * ddot3(gf, tc);
+        rhs[c] = side == 0 ? rhs[c] + noc : rhs[c] - noc;
+      }
+    }
+    const double pf = (fl & FF_KINK) ? kink_pf(m, F.p, F.gp, f, a, b) : l * F.p[a] + (1.0 - l) * F.p[b];
+    for (int c = 0; c < 3; c++) {
+      const double Sc = m.e_sf[3 * (size_t)k + c];
+      rhs[c] = side == 0 ? rhs[c] - pf * Sc : rhs[c] + pf * Sc;
+    }
+  }
+  ap1[r] = ap;
+  for (int c = 0; c < 3; c++) rhs3[3 * (size_t)r + c] = rhs[c];
+}
+
+// component q.comp after k_mom_interior3: boundary faces, sources, diagonal
+__global__ void k_mom_tail(DMesh m, DFields F, Ramp rp, MomParams q, const double *rhs3, const double *ap1,
+                           double *val, double *rhs_out, double *ap_out) {
+  const int r = cell_row(m);
+  if (r < 0) return;
+  mom_tail(m, F, rp, q, r, m.cf_ptr[r], m.cf_ptr[r + 1], ap1[r], rhs3[3 * (size_t)r + q.comp], val, rhs_out,
+           ap_out);
 }
 
 // ----------------------------------------------------------------- energy
@@ -997,10 +1121,8 @@ struct Driver {
     hipLaunchKernelGGL(k_outflow_scale, gc(s.nf), dim3(kBlock), 0, stream(), s.dmesh(), s.flux.p, s.tmpc.p);
     MS_CHECK_LAUNCH();
   }
-  void momentum(int comp) {     // comp 0..2
+  MomParams mom_params(int comp) const {
     const ms_ctrl_t &c = s.ctrl;
-    if (c.conv_blend > 0.0)
-      hipLaunchKernelGGL(k_limiter, gcell(s), dim3(kBlock), 0, stream(), s.dmesh(), s.dfields(), rp, comp, s.psi.p);
     MomParams q{};
     q.comp = comp;
     q.ts_order = s.ts_order;
@@ -1017,8 +1139,38 @@ struct Driver {
     q.nonorth = c.nonorth_corr;
     q.grav = c.gravity[comp];
     q.body = c.body_force[comp];
-    hipLaunchKernelGGL(k_momentum, gcell(s), dim3(kBlock), 0, stream(), s.dmesh(), s.dfields(), rp, q, s.psi.p,
-                       s.val.p, s.rhs.p, s.ap.p);
+    return q;
+  }
+  void momentum(int comp) {     // comp 0..2
+    if (s.ctrl.conv_blend > 0.0)
+      hipLaunchKernelGGL(k_limiter, gcell(s), dim3(kBlock), 0, stream(), s.dmesh(), s.dfields(), rp, comp, s.psi.p);
+    hipLaunchKernelGGL(k_momentum, gcell(s), dim3(kBlock), 0, stream(), s.dmesh(), s.dfields(), rp, mom_params(comp),
+                       s.psi.p, s.val.p, s.rhs.p, s.ap.p);
+    MS_CHECK_LAUNCH();
+  }
+  // The three components' interior faces in one pass (k_mom_interior3) is
+  // exact when no interior or limiter term of one component reads another
+  // component's velocity: no Beavers-Joseph faces, and with the limiter on
+  // no symmetry or slip-wall group (their face velocity mixes components).
+  bool momentum_fused() const {
+    if (!momentum_fused_enabled() || s.has_bj) return false;
+    if (s.ctrl.conv_blend > 0.0)
+      for (const ms_bcgroup_t &g : s.groups)
+        if (g.btype == MS_BC_SYMMETRY || g.btype == MS_BC_SLIPWALL) return false;
+    return true;
+  }
+  void momentum_interior3() {
+    if (s.mom_rhs3.n < 3 * (size_t)s.nc) s.mom_rhs3.alloc(3 * (size_t)s.nc);
+    if (s.mom_ap1.n < (size_t)s.nc) s.mom_ap1.alloc(s.nc);
+    if (s.ctrl.conv_blend > 0.0)
+      hipLaunchKernelGGL(k_limiter3, gcell(s), dim3(kBlock), 0, stream(), s.dmesh(), s.dfields(), rp, s.psi.p);
+    hipLaunchKernelGGL(k_mom_interior3, gcell(s), dim3(kBlock), 0, stream(), s.dmesh(), s.dfields(), mom_params(0),
+                       s.psi.p, s.val.p, s.mom_rhs3.p, s.mom_ap1.p);
+    MS_CHECK_LAUNCH();
+  }
+  void momentum_tail(int comp) {
+    hipLaunchKernelGGL(k_mom_tail, gcell(s), dim3(kBlock), 0, stream(), s.dmesh(), s.dfields(), rp, mom_params(comp),
+                       s.mom_rhs3.p, s.mom_ap1.p, s.val.p, s.rhs.p, s.ap.p);
     MS_CHECK_LAUNCH();
   }
   void energy() {
@@ -1094,8 +1246,11 @@ struct Driver {
   }
   // momentum predictor over the 3 components; returns failing comp (1..3) or 0
   int momentum_solves(int &itl_max, double &ressum) {
+    const bool fused = momentum_fused();
+    if (fused) momentum_interior3();
     for (int comp = 0; comp < 3; comp++) {
-      momentum(comp);
+      if (fused) momentum_tail(comp);
+      else momentum(comp);
       hipLaunchKernelGGL(k_comp_get, gc(s.nc), dim3(kBlock), 0, stream(), s.nc, s.u.p, comp, s.ucomp.p);
       SolveResult R = solve(false, s.ucomp.p);
       hipLaunchKernelGGL(k_comp_set, gc(s.nc), dim3(kBlock), 0, stream(), s.nc, s.u.p, comp, s.ucomp.p);

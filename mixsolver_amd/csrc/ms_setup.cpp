@@ -82,6 +82,17 @@ bool graphs_enabled() {
   return v == 1;
 }
 void set_graphs(bool on) { g_graphs.store(on ? 1 : 0); }
+static std::atomic<int> g_mom_fused{-1};   // -1: not yet read from the environment
+bool momentum_fused_enabled() {
+  int v = g_mom_fused.load();
+  if (v < 0) {
+    const char *e = std::getenv("MIXSOLVER_MOM_FUSED");
+    v = (e && e[0] == '0') ? 0 : 1;
+    g_mom_fused.store(v);
+  }
+  return v != 0;
+}
+void set_momentum_fused(bool on) { g_mom_fused.store(on ? 1 : 0); }
 bool pipeline_enabled() {
   static const bool on = [] {
     const char *e = std::getenv("MIXSOLVER_PIPELINE");
@@ -1093,6 +1104,7 @@ void System::refresh_face_constants() {
   std::vector<double> Cbj(nf, 0.0), kch(nf, 0.0), Ds(nf, 0.0), ksfv(nf, 0.0), Dsb(nf, 0.0);
   bc_energy = false;
   has_kink = false;
+  has_bj = false;
   bool any_flag = false;
   for (int f = 0; f < nf; f++) {
     const int a = n2o[c0[f]];
@@ -1148,6 +1160,7 @@ void System::refresh_face_constants() {
             const double dPf = norm2_ref(r, 3);
             Cbj[f] = c.mu * area[f] * (bja / std::sqrt(Knn)) / (1.0 + bja * dPf / std::sqrt(Knn));
             ff[f] |= FF_BJ;
+            has_bj = true;
           }
         }
       }
@@ -1303,6 +1316,7 @@ System *fork_system(System &b, const std::vector<ms_bcgroup_t> &groups) {
   s.has_solid = b.has_solid;
   s.has_disp = b.has_disp;
   s.has_kink = b.has_kink;
+  s.has_bj = b.has_bj;
   s.solid_consts = b.solid_consts;
   s.bc_energy = b.bc_energy;
   s.has_qsrc = b.has_qsrc;

@@ -136,6 +136,7 @@ struct System {
   bool props_dirty = true;
   // porous / solid flags derived from the properties (refresh_face_constants)
   bool has_face_flags = false, has_solid = false, has_disp = false, has_kink = false;
+  bool has_bj = false;         // Beavers-Joseph faces
   bool solid_consts = false;   // keep solid-phase face constants (per-call assembly)
   bool bc_energy = false, has_qsrc = false;   // boundary / solid sources make T non-trivial
   bool T_nonzero = false;      // caller seeded a non-zero temperature field
@@ -168,6 +169,7 @@ struct System {
   DBuf<double> disp_l_d, disp_t_d, h_sf_d, a_sf_d, q_src_d, kdisp, gp_prev;
   // work
   DBuf<double> val, rhs, ap, pp, dpp, af, ucomp, part, diag8, tmpc;
+  DBuf<double> mom_rhs3, mom_ap1;   // the three momentum components' interior-face sums (fused assembly)
   DBuf<unsigned int> ticket;
   std::vector<double> history;  // 8 per row
   // p' solve counters since creation: solves, iterations run in FP32 and in

@@ -304,6 +304,13 @@ def set_graphs(on: bool) -> None:
     check(_lib.lib().ms_set_graphs(1 if on else 0))
 
 
+def set_fused_momentum(on: bool) -> None:
+    """Assemble the three momentum components' interior faces in one pass
+    where that is exact (default); off: one pass per component.  Results are
+    identical either way."""
+    check(_lib.lib().ms_set_fused_momentum(int(bool(on))))
+
+
 def set_stall_window(iters: int) -> None:
     """Iterations without a halving of the FP32 residual after which the
     mixed-precision PCG hands over to FP64 (0 = the default 1024)."""
