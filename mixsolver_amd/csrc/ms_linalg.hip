@@ -280,6 +280,30 @@ __global__ void k_set_sigma(KS *st) {
 }
 
 // -------------------------------------------------------------- BiCGSTAB
+// Additive coarse correction of a BiCGSTAB preconditioner application,
+// out += Z E^-1 Z^T in (the p' matrix and so E are symmetric): chunk sums of
+// in (FP64, one workgroup per aggregate chunk), nu = E^-1 c by one finisher
+// workgroup, then nu of each row's aggregate added to out.  All three skip a
+// finished solve, as the other BiCGSTAB kernels do.
+template <class T>
+__global__ void k_co_restrict(Rows RW, const T *in, double *cpart, const int *done) {
+  if (*done) return;
+  int i, a;
+  double v[1] = {0.0};
+  if (RW.get(i, a)) v[0] = (double)in[i];
+  block_partials<OpSum, 1, double>(v, cpart);
+}
+__global__ __launch_bounds__(kFinThreads) void k_co_solve(const double *cpart, CoDev co, const int *done) {
+  if (*done) return;
+  coarse_solve<1, double>(cpart, co);
+}
+template <class T>
+__global__ void k_co_prolong(Rows RW, const double *nu, T *out, const int *done) {
+  if (*done) return;
+  int i, a;
+  if (RW.get(i, a)) out[i] = out[i] + (T)nu[a];
+}
+
 template <class T>
 __global__ void k_bi_init(int n, const T *r, T *r0, T *p, T *v, KS *st) {
   int i = blockIdx.x * kBlock + threadIdx.x;
@@ -1053,6 +1077,17 @@ void Krylov::coarse_setup(const double *val) {
   MS_CHECK_LAUNCH();
 }
 
+template <class T>
+void Krylov::coarse_apply(const T *in, T *out, const int *done, KS *st) {
+  const Coarse &C = pat->coarse;
+  if (co_part_.n < (size_t)C.nchunks) co_part_.alloc(C.nchunks);
+  const Rows RW = rows_of(pat, true);
+  hipLaunchKernelGGL((k_co_restrict<T>), dim3(C.nchunks), dim3(kBlock), 0, stream(), RW, in, co_part_.p, done);
+  hipLaunchKernelGGL(k_co_solve, dim3(1), dim3(kFinThreads), 0, stream(), co_part_.p, codev_of(pat, cm_, true, st),
+                     done);
+  hipLaunchKernelGGL((k_co_prolong<T>), dim3(C.nchunks), dim3(kBlock), 0, stream(), RW, cm_.nu.p, out, done);
+}
+
 SolveResult Krylov::pcg(const double *val, const double *b, double *x, double tol, int maxit, PcKind pc,
                         bool coarse, bool reuse) {
   SolveResult R;
@@ -1166,7 +1201,8 @@ SolveResult Krylov::pcg(const double *val, const double *b, double *x, double to
 }
 
 // ---- BiCGSTAB -----------------------------------------------------------------
-SolveResult Krylov::bicgstab(const double *val, const double *b, double *x, double tol, int maxit, PcKind pc) {
+SolveResult Krylov::bicgstab(const double *val, const double *b, double *x, double tol, int maxit, PcKind pc,
+                             bool coarse) {
   SolveResult R;
   Mat A = matof(pat);
   const dim3 G(nblk_), B(kBlock);
@@ -1191,6 +1227,8 @@ SolveResult Krylov::bicgstab(const double *val, const double *b, double *x, doub
     R.ierr = fe + 10;
     return R;
   }
+  coarse = coarse && pat->coarse.k > 0;
+  if (coarse) coarse_setup(val);
   hipLaunchKernelGGL(k_residual<double>, G, B, 0, S, n, A, val, b, x, r_.p, part);
   finish<OpSum, 1>(nblk_, part, FinResidual{st});
   hipLaunchKernelGGL(k_bi_init<double>, G, B, 0, S, n, r_.p, r0_.p, p_.p, q_.p, st);
@@ -1208,6 +1246,7 @@ SolveResult Krylov::bicgstab(const double *val, const double *b, double *x, doub
       finish<OpSum, 1>(nblk_, part, FinBiRho{st});
       hipLaunchKernelGGL(k_bi_p<double>, G, B, 0, S, n, r_.p, v, p_.p, st);
       apply_pc<double>(pc, val, fac_.p, dg_.p, p_.p, ph_.p, done);
+      if (coarse) coarse_apply<double>(p_.p, ph_.p, done, st);
       hipLaunchKernelGGL(k_bi_spmv_dot<double>, G, B, 0, S, n, A, val, ph_.p, r0_.p, v, st, part);
       finish<OpSum, 1>(nblk_, part, FinBiAlpha{st});
       hipLaunchKernelGGL(k_bi_s<double>, G, B, 0, S, n, r_.p, v, s_.p, st, part);
@@ -1215,6 +1254,7 @@ SolveResult Krylov::bicgstab(const double *val, const double *b, double *x, doub
       hipLaunchKernelGGL(k_bi_early<double>, G, B, 0, S, n, ph_.p, x, st);
       hipLaunchKernelGGL(k_done2_to_1, dim3(1), dim3(64), 0, S, st);
       apply_pc<double>(pc, val, fac_.p, dg_.p, s_.p, sh_.p, done);
+      if (coarse) coarse_apply<double>(s_.p, sh_.p, done, st);
       hipLaunchKernelGGL(k_bi_spmv_omega<double>, G, B, 0, S, n, A, val, sh_.p, s_.p, t_.p, st, part);
       finish<OpSum, 2>(nblk_, part, FinBiOmega{st});
       hipLaunchKernelGGL(k_bi_xr<double>, G, B, 0, S, n, ph_.p, sh_.p, s_.p, t_.p, x, r_.p, st, part);
@@ -1239,7 +1279,7 @@ SolveResult Krylov::bicgstab(const double *val, const double *b, double *x, doub
 // FP32 BiCGSTAB on the scaled residual.  An inner solve whose recursive
 // residual has not halved over `window` iterations ends early (the caller's
 // refinement restarts it from the FP64 residual).
-SolveResult Krylov::bicg32(PcKind pc, int maxit, int window) {
+SolveResult Krylov::bicg32(PcKind pc, int maxit, int window, bool coarse) {
   SolveResult R;
   Mat A = matof(pat);
   const dim3 G(nblk_), B(kBlock);
@@ -1260,6 +1300,7 @@ SolveResult Krylov::bicg32(PcKind pc, int maxit, int window) {
       finish<OpSum, 1>(nblk_, part, FinBiRho{st});
       hipLaunchKernelGGL(k_bi_p<float>, G, B, 0, S, n, r32_.p, v, p32_.p, st);
       apply_pc<float>(pc, val32_.p, fac32_.p, dg32_.p, p32_.p, ph32_.p, done);
+      if (coarse) coarse_apply<float>(p32_.p, ph32_.p, done, st);
       if (pat->has16)
         hipLaunchKernelGGL((k_bi_spmv_dot<float, 1>), G, B, 0, S, n, A, val32_.p, ph32_.p, r032_.p, v, st, part);
       else
@@ -1270,6 +1311,7 @@ SolveResult Krylov::bicg32(PcKind pc, int maxit, int window) {
       hipLaunchKernelGGL(k_bi_early<float>, G, B, 0, S, n, ph32_.p, x32_.p, st);
       hipLaunchKernelGGL(k_done2_to_1, dim3(1), dim3(64), 0, S, st);
       apply_pc<float>(pc, val32_.p, fac32_.p, dg32_.p, s32_.p, sh32_.p, done);
+      if (coarse) coarse_apply<float>(s32_.p, sh32_.p, done, st);
       if (pat->has16)
         hipLaunchKernelGGL((k_bi_spmv_omega<float, 1>), G, B, 0, S, n, A, val32_.p, sh32_.p, s32_.p, t32_.p, st,
                            part);
@@ -1350,7 +1392,8 @@ SolveResult Krylov::mixed(const double *val, const double *b, double *x, double 
                           int solver, double inner_tol, bool coarse, bool reuse) {
   // a system whose FP32 iteration stalled once is solved in FP64 from then on
   if (fp32_stalled_) {
-    SolveResult F = solver == 0 ? pcg(val, b, x, tol, maxit, pc, coarse, reuse) : bicgstab(val, b, x, tol, maxit, pc);
+    SolveResult F = solver == 0 ? pcg(val, b, x, tol, maxit, pc, coarse, reuse)
+                                : bicgstab(val, b, x, tol, maxit, pc, coarse);
     F.stalled = 1;
     F.it64 = F.it;
     return F;
@@ -1376,13 +1419,14 @@ SolveResult Krylov::mixed(const double *val, const double *b, double *x, double 
     MS_HIP(hipMemsetAsync(x, 0, n * sizeof(double), S));
     return R;
   }
-  coarse = coarse && pat->coarse.k > 0 && solver == 0;
+  coarse = coarse && pat->coarse.k > 0;
   const SetupKey key{val, (int)pc, 1, (int)coarse};
   const bool reused = reuse && last_ == key;
   if (!reused) {
     last_ = SetupKey{};
     hipLaunchKernelGGL(k_mat32_diff, G, B, 0, S, n, A, val, val32_.p);
     setup_pc32(pc, val);
+    if (coarse && solver == 1) coarse_setup(val);
   }
   hipLaunchKernelGGL(k_residual<double>, G, B, 0, S, n, A, val, b, x, r_.p, part);
   finish<OpSum, 1>(nblk_, part, FinResidual{st});
@@ -1402,7 +1446,7 @@ SolveResult Krylov::mixed(const double *val, const double *b, double *x, double 
     while (true) {
       int imax = maxit - total;
       hipLaunchKernelGGL(k_mp_scale_down, G, B, 0, S, n, r_.p, st, r32_.p, x32_.p, st32_.p, kRefineTol, imax);
-      SolveResult in = bicg32(pc, imax, stall_window());
+      SolveResult in = bicg32(pc, imax, stall_window(), coarse);
       total += std::max(in.it, 1);
       R.outer++;
       hipLaunchKernelGGL(k_mp_correct, G, B, 0, S, n, x32_.p, st, x);
@@ -1418,7 +1462,7 @@ SolveResult Krylov::mixed(const double *val, const double *b, double *x, double 
       }
       if (!(hs.res <= 0.5 * prev)) {
         fp32_stalled_ = true;
-        const SolveResult F = bicgstab(val, b, x, tol, maxit - total, pc);
+        const SolveResult F = bicgstab(val, b, x, tol, maxit - total, pc, coarse);
         R.it = total + F.it;
         R.it32 = total;
         R.it64 = F.it;

@@ -56,7 +56,10 @@ class Krylov {
   // preconditioner, coarse inverse) is kept.
   SolveResult pcg(const double *val, const double *b, double *x, double tol, int maxit, PcKind pc,
                   bool coarse = false, bool reuse = false);
-  SolveResult bicgstab(const double *val, const double *b, double *x, double tol, int maxit, PcKind pc);
+  // coarse: add the aggregate coarse correction Z E^-1 Z^T to the
+  // preconditioner (the p' matrix is symmetric, so E is)
+  SolveResult bicgstab(const double *val, const double *b, double *x, double tol, int maxit, PcKind pc,
+                       bool coarse = false);
   // Mixed precision.  PCG: FP32 matrix, preconditioner and Krylov vectors
   // with FP32 accumulation, FP64 solution and reliable updates of the true
   // FP64 residual.  BiCGSTAB: FP32 inner solves inside FP64 iterative
@@ -80,7 +83,8 @@ class Krylov {
   template <class T> void apply_pc(PcKind pc, const T *val, const T *fac, const T *dg, const T *in, T *out,
                                    const int *done);
   template <class T> KS read_state(KS *d);
-  SolveResult bicg32(PcKind pc, int maxit, int window);
+  SolveResult bicg32(PcKind pc, int maxit, int window, bool coarse);
+  template <class T> void coarse_apply(const T *in, T *out, const int *done, KS *st);
   void ensure32();
   void coarse_setup(const double *val);   // E = Z^T A Z and its inverse
 
@@ -91,6 +95,7 @@ class Krylov {
   DBuf<float> pq32_;         // the SpMV's p.q partials on the split path
   DBuf<int> flag_;
   CoarseMat cm_;             // E, E^-1 and nu of the coarse space for the current matrix
+  DBuf<double> co_part_;     // chunk sums of a BiCGSTAB coarse correction
   void *hpin_ = nullptr;
   hipEvent_t state_ev_[2] = {nullptr, nullptr};   // the two pipelined chunk states (mixed PCG)
   void *state_slot(int slot) const { return (char *)hpin_ + 1024 * (slot + 1); }

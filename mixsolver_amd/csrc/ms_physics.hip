@@ -1230,7 +1230,7 @@ struct Driver {
       R = s.kry->mixed(s.val.p, s.rhs.p, x, c.lin_tol, c.lin_max, pc, solver, c.gpu_inner_tol, coarse, reuse);
     } else {
       R = solver == 0 ? s.kry->pcg(s.val.p, s.rhs.p, x, c.lin_tol, c.lin_max, pc, coarse, reuse)
-                      : s.kry->bicgstab(s.val.p, s.rhs.p, x, c.lin_tol, c.lin_max, pc);
+                      : s.kry->bicgstab(s.val.p, s.rhs.p, x, c.lin_tol, c.lin_max, pc, coarse);
       R.it64 = R.it;
     }
     if (ppe_eq) {

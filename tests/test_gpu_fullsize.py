@@ -79,7 +79,8 @@ def test_channel_2m_fast_path_equals_fp64():
     _check(out, "channel 2M", 1e-8, 1e-7)
 
 
-def test_prism_cylinder_1p2m_bicgstab_multicolour_ilu0():
+@pytest.mark.parametrize("coarse", [256, 0])
+def test_prism_cylinder_1p2m_bicgstab_multicolour_ilu0(coarse):
     """Config 3 as specified: 1.2M jittered prisms, p' by BiCGSTAB with
     multicolour ILU(0) (gpu_ppe_precond = mc_ilu0, gpu_ppe_solver =
     bicgstab), momentum by BiCGSTAB with multicolour ILU(0), dt for CFL 0.5
@@ -87,14 +88,16 @@ def test_prism_cylinder_1p2m_bicgstab_multicolour_ilu0():
     history: a first FP64 step at the free-stream estimate gives umax, dt is
     rescaled from it); mixed precision against FP64, both to 1e-10 on every
     p' solve; fields and lift / drag (compute_forces) of the two paths
-    agree."""
+    agree.  With the 256-aggregate coarse correction added to the
+    preconditioner the FP32 inner solves hold (no p' solve handed to FP64);
+    without it the stall guard hands the p' solves to FP64 BiCGSTAB."""
     mesh = meshgen.cylinder_prisms(ntheta=1000, nr=300, layers=2, jitter=0.15, seed=200)
     probe = Case.from_control(mesh, PRISM_CTL)
     h_min = float(probe.get("vol").min()) ** (1.0 / 3.0)
     probe.close()
     u0 = np.zeros((mesh.ncells, 3))
     u0[:, 0] = 1.0
-    c3 = dict(gpu_ppe_precond=3, gpu_ppe_solver=1, gpu_mom_precond=3)
+    c3 = dict(gpu_ppe_precond=3, gpu_ppe_solver=1, gpu_mom_precond=3, gpu_coarse=coarse)
 
     def step(dt, prec):
         cs = Case.from_control(mesh, PRISM_CTL.replace("dt = 0.001", f"dt = {float(dt)!r}"), gpu_precision=prec, **c3)
@@ -119,6 +122,8 @@ def test_prism_cylinder_1p2m_bicgstab_multicolour_ilu0():
     assert eu <= 5e-7 and ep <= 5e-7
     for k in ("CD", "CL"):
         assert abs(fa[k] - fb[k]) <= 1e-6 * abs(fb["CD"]), (k, fa[k], fb[k])
+    if coarse:
+        assert sa["ppe_fp64_solves"] == 0 and sa["fp32_stalled"] == 0 and sa["ppe_iters_fp32"] > 0, sa
 
 
 def test_channel_2m_twin_matches_reference_signature():

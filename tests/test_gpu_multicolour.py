@@ -81,13 +81,14 @@ bc = 6 pressure-far-field 0.0 1.0 0.0 0.0
 """
 
 
-@pytest.mark.parametrize("precision", [0, 1])
-def test_piso_bicgstab_multicolour_ilu0_on_prisms(precision):
+@pytest.mark.parametrize("precision,coarse", [(0, 0), (1, 0), (0, 32), (1, 32)])
+def test_piso_bicgstab_multicolour_ilu0_on_prisms(precision, coarse):
     """A jittered prism O-grid round a cylinder: p' by BiCGSTAB with
-    multicolour ILU(0) (FP64, or FP32 inner solves inside FP64 refinement)
-    ends every solve at a 1e-10 FP64 residual and leaves the state of the
-    reference path (PCG-Jacobi p', natural ILU(0) momentum) to the solver
-    tolerance; lift and drag agree."""
+    multicolour ILU(0) (FP64, or FP32 inner solves inside FP64 refinement),
+    alone or with the aggregate coarse correction added to the
+    preconditioner, ends every solve at a 1e-10 FP64 residual and leaves the
+    state of the reference path (PCG-Jacobi p', natural ILU(0) momentum) to
+    the solver tolerance; lift and drag agree."""
     from mixsolver_amd import meshgen
     m = meshgen.cylinder_prisms(ntheta=120, nr=40, layers=2, jitter=0.15, seed=200)
     ctl = PRISM_CTL.format(dt=0.002)
@@ -95,13 +96,15 @@ def test_piso_bicgstab_multicolour_ilu0_on_prisms(precision):
     u0[:, 0] = 1.0
     ref = Case.from_control(m, ctl)
     fast = Case.from_control(m, ctl, gpu_ppe_precond=3, gpu_ppe_solver=1, gpu_mom_precond=3,
-                             gpu_precision=precision)
+                             gpu_precision=precision, gpu_coarse=coarse)
     for cs in (ref, fast):
         cs.set_field("u", u0)
         assert cs.piso_run(2) == 0
     h = fast.history()
-    print("history", h.tolist(), flush=True)
+    print("history", h.tolist(), fast.solver_stats(), flush=True)
     assert np.all(h[:, 7] <= 1e-10)
+    if coarse:
+        assert fast.coarse_info()[0] > 0 and fast.solver_stats()["ppe_fp64_solves"] == 0
     for k in ("u", "p"):
         a, b = fast.get(k), ref.get(k)
         assert rel_l2(a, b) <= 1e-7, (k, rel_l2(a, b))
