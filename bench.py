@@ -122,12 +122,13 @@ def run_variant(Case, mesh, ctl, u0, over, steps, warmup, probe=False):
     return cs, total, np.array(lat), np.array(hist)
 
 
-def step_bytes(mesh, periodic, coarse, n1, nnz_s, ppe_its, mom_its, correctors=2):
+def step_bytes(mesh, periodic, coarse, n1, nnz_s, ppe_its, mom_its, correctors=2, spmv_red=None, matfree=False):
     """Algorithmic HBM bytes of one PISO step on the red-black reduced p'
     path (mixed precision), summed over its launches with the step's logged
     iteration counts.  Each array is read once or written once per launch;
     int32 indices; FP64 assembly, FP32 Krylov work.  n1 / nnz_s: rows and
-    nonzeros of the reduced matrix S."""
+    nonzeros of the reduced matrix S; spmv_red: algorithmic bytes of the
+    FP32 product with S in use (the matrix-free pair when matfree)."""
     n = mesh.ncells
     nper = int(np.count_nonzero(np.isin(mesh.zone, (6, 7, 8, 9)))) if periodic else 0
     nfi = mesh.n_interior + nper // 2                 # couplings after joining the periodic pairs
@@ -143,9 +144,11 @@ def step_bytes(mesh, periodic, coarse, n1, nnz_s, ppe_its, mom_its, correctors=2
     grad = nf * 40 + 200 * n
     rhie = nfi * 48 + 96 * n + nf * 8
     corr = nf * 40 + 64 * n
-    spmv_s = nnz_s * 8 + 4 * (n1 + 1) + 2 * n1 * 4
-    red_it = spmv_s + 44 * n1                         # SpMV on S, update (8 arrays), p update (3)
+    spmv_s = spmv_red if spmv_red else nnz_s * 8 + 4 * (n1 + 1) + 2 * n1 * 4
+    red_it = spmv_s + 44 * n1                         # product with S, update (8 arrays), p update (3)
     schur_form = nnz * 12 + nnz_s * 8 + nnz_s * 12    # S from A, then its FP32 copy
+    if matfree:
+        schur_form += nnz * 12 + nnz * 4              # the matrix-free weights from A
     coarse_setup = 16 * coarse ** 3 + 8 * coarse ** 2  # Gauss-Jordan: k launches over the k x k matrix
     per_corr = ppe_asm + rhie + corr + grad + 3 * spmv64   # residual, reduced rhs, back substitution
     parts = {
@@ -565,7 +568,9 @@ def main():
     _lib.check(_lib.lib().ms_time_copy(1 << 30, 20, ctypes.byref(cp_ms)))
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "pcg_spmv_traffic.json")
-    kname = "k_pcg_spmv<float> on the reduced matrix S" if "kernel_schur" in runs else "k_pcg_spmv<float>"
+    matfree = os.environ.get("MIXSOLVER_SCHUR_MF", "1") != "0"
+    kname = ("k_mf_stage1 + k_mf_stage2: the reduced matrix S applied matrix-free" if matfree else
+             "k_pcg_spmv<float> on the reduced matrix S") if "kernel_schur" in runs else "k_pcg_spmv<float>"
     if os.path.exists(tpath):      # HBM bytes per launch measured by the counter runs (committed summary)
         with open(tpath) as fh:
             traffic = json.load(fh)["traffic_bytes"].get(kname)
@@ -581,7 +586,9 @@ def main():
             src = (f"in-loop: {nl} launches of the p' solves ({rows_l} rows; the last of each 16-iteration "
                    f"chunk) over {where}, start/stop events of each launch (hipExtLaunchKernelGGL) on the "
                    "library stream")
-        out["roofline"] = {"bound": "hbm", "kernel": kname + " (SELL-64, difference form, fused p.q)",
+        kdesc = (" (two launches over the full matrix's SELL-64 rows, difference form, fused p.q; one product)"
+                 if kkey == "kernel_schur" and matfree else " (SELL-64, difference form, fused p.q)")
+        out["roofline"] = {"bound": "hbm", "kernel": kname + kdesc,
                            "achieved": b_k / (t_avg * 1e-3) / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                            "frac": b_k / (t_avg * 1e-3) / HBM_PEAK, "traffic": traffic,
                            "traffic_source": "profiles/ (FETCH_SIZE, WRITE_SIZE, copy-calibrated)",
@@ -612,7 +619,8 @@ def main():
         n1 = runs["inloop"][2] if "inloop" in runs and runs["inloop"][2] > 0 else cells // 2
         nnz_s = (runs["kernel_schur"][3] - 4.0 * (n1 + 1) - 16.0 * n1) / 12.0
         b_step, parts = step_bytes(mesh, periodic, args.coarse, n1, nnz_s, iters(lead),
-                                   lead_stats["mom_iters"] / args.steps)
+                                   lead_stats["mom_iters"] / args.steps, spmv_red=runs["kernel_schur"][1],
+                                   matfree=matfree)
         t_step = runs[lead][1] / args.steps
         out["roofline_step"] = {"bound": "hbm", "B_step": b_step, "t_step_ms": 1e3 * t_step,
                                 "achieved": b_step / t_step / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",

@@ -111,6 +111,13 @@ int ms_set_graphs(int on);
  * momentum_assembly (mod_uns_simple.f90:928-1338).  Results are identical
  * either way.  No reference counterpart (library option). */
 int ms_set_fused_momentum(int on);
+/* The FP32 iterations of the red-black reduced p' solve (gpu_ppe_solver =
+ * schur) apply S = D1 - B^T D0^-1 B matrix-free, in two launches over the
+ * full matrix's rows (1, default; MIXSOLVER_SCHUR_MF=0 sets 0 at start-up),
+ * or the formed S (0).  Both reach the same FP64 residual; the FP32
+ * rounding, and so the iteration count, differ.  No reference counterpart
+ * (library option). */
+int ms_set_schur_matfree(int on);
 
 /* vtk_write (mod_uns_output.f90:81-240, tetrahedral fan) through vtu_write
  * (mod_vtu_writer.f90): binary = 0 inline ASCII (Float64, ES24.16E3),
@@ -434,7 +441,10 @@ int ms_sweep_stats(ms_sweep *sw, long long *out10);
  * s the value size and i the column-index size of the format in use (2 for the
  * FP32 kernel's 16-bit slice offsets, else 4). */
 /* which 4..7: the same four timings on the red-black reduced p' matrix S of a
- * system with gpu_ppe_solver = 2, after its first p' solve. */
+ * system with gpu_ppe_solver = 2, after its first p' solve.  With the
+ * matrix-free S on (ms_set_schur_matfree), which 4 and 6 time its two
+ * launches, and alg_bytes = nnz_A (4 + i) + 4 (n + 1) + 8 n1 + 16 n0 + 4 n1
+ * over the full matrix A (n0 / n1 rows of colour 0 / 1). */
 int ms_time_kernel(ms_system *s, int which, int reps, double *avg_ms, double *alg_bytes);
 /* Average duration of a plain device-to-device copy kernel (k_copy4) moving
  * `bytes` (read once, written once): the calibration of traffic counters and

@@ -121,6 +121,7 @@ const char *ms_last_error(void) { return last_error(); }
 int ms_version(void) { return 100; }
 int ms_set_graphs(int on) { return guarded([&] { set_graphs(on != 0); }); }
 int ms_set_fused_momentum(int on) { return guarded([&] { set_momentum_fused(on != 0); }); }
+int ms_set_schur_matfree(int on) { return guarded([&] { set_schur_matfree(on != 0); }); }
 
 int ms_dump_write(const char *path, int ncells, int ts_order, const char *source_mesh, const char *generated,
                   const double *u, const double *p, const double *T, const double *u_old, const double *u_old_old,
@@ -659,6 +660,15 @@ int ms_time_kernel(ms_system *h, int which, int reps, double *avg_ms, double *al
     if (avg_ms) *avg_ms = t;
     // algorithmic bytes of one SpMV: nnz (value + column index) + one int per row + x and y
     if (alg_bytes) *alg_bytes = (double)P.nnz * (sz + idx) + 4.0 * (P.n + 1) + 2.0 * P.n * sz;
+    if (alg_bytes && red && fp32 && K.matfree_on()) {
+      // the matrix-free product: A's nonzeros (FP32 weight + column index)
+      // and one int per row of A, x read by each stage, the colour-0
+      // intermediate (h, x_ref) written and read, q written
+      const SellPattern &F = *s.P;
+      const double n0 = s.schur->n0, n1 = s.schur->n1;
+      *alg_bytes = (double)F.nnz * (4.0 + (F.has16 ? 2.0 : 4.0)) + 4.0 * (F.n + 1) + 2.0 * n1 * 4.0 +
+                   2.0 * n0 * 8.0 + n1 * 4.0;
+    }
   });
 }
 

@@ -22,6 +22,7 @@
 #include <atomic>
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include <hip/hip_ext.h>
@@ -68,6 +69,110 @@ __global__ void k_pcg_spmv(int n, Mat A, const T *val, const T *p, T *q, const K
     const T qi = row_apply<T, C16>(i, A, val, p);
     q[i] = qi;
     v[0] = pi * qi;
+  }
+  block_partials<OpSum, 1>(v, part);
+}
+
+// Matrix-free S p (MatFree): with g_k = sum_j (-a_kj / a_kk) x_j over the
+// colour-1 neighbours of colour-0 row k, (S x)_i = sum_k (-a_ik) (x_i - g_k)
+// + rowsum_i x_i.  Stage 1 forms, per colour-0 row, h_k = x_ref - g_k =
+// sum_j w_kj (x_ref - x_j) + rho_k x_ref with x_ref the row's first
+// colour-1 neighbour and rho_k = rowsum_k / a_kk; stage 2 forms
+// sum_k (-a_ik) ((x_i - x_ref_k) + h_k) + rowsum_i x_i.  Every term is a
+// difference or a row-sum term, as in row_apply's difference form, so a
+// constant vector gives the row sums' product exactly.  The slot loop runs
+// the slice width of A (diagonal and padding slots have col = row).
+template <int C16, int W>
+__device__ __forceinline__ float mf1_row(int k, int s, long long base, int n0, const Mat &A, const float *w,
+                                         const float *x, float xr) {
+  int c[W];
+  float a[W];
+#pragma unroll
+  for (int j = 0; j < W; j++) c[j] = col_at<C16>(A, base + (long long)j * kSlice, k, s);
+#pragma unroll
+  for (int j = 0; j < W; j++) a[j] = w[base + (long long)j * kSlice];
+  float acc = 0.f;
+#pragma unroll
+  for (int j = 0; j < W; j++) acc = acc + a[j] * (c[j] < n0 ? xr : xr - x[c[j] - n0]);
+  return acc;
+}
+
+template <int C16>
+__global__ void k_mf_stage1(int n0, Mat A, const float *w, const float *x, float2 *hx, const KS *st) {
+  if (st->done) return;
+  const int k = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (k >= n0) return;
+  const int s = k >> 6;
+  const long long o0 = A.off[s];
+  const int wd = (int)((A.off[s + 1] - o0) >> 6);
+  const long long base = o0 + (k & 63);
+  // slot 0 is the diagonal (colour-1 columns are above every colour-0 row)
+  const int c1 = wd > 1 ? col_at<C16>(A, base + kSlice, k, s) : k;
+  const float xr = c1 >= n0 ? x[c1 - n0] : 0.f;
+  float acc;
+  if (wd == 7) acc = mf1_row<C16, 7>(k, s, base, n0, A, w, x, xr);
+  else if (wd == 6) acc = mf1_row<C16, 6>(k, s, base, n0, A, w, x, xr);
+  else if (wd == 5) acc = mf1_row<C16, 5>(k, s, base, n0, A, w, x, xr);
+  else {
+    acc = 0.f;
+    for (int j = 0; j < wd; j++) {
+      const long long q = base + (long long)j * kSlice;
+      const int c = col_at<C16>(A, q, k, s);
+      acc = acc + w[q] * (c < n0 ? xr : xr - x[c - n0]);
+    }
+  }
+  hx[k] = make_float2(acc, xr);
+}
+
+template <int C16, int W>
+__device__ __forceinline__ float mf2_row(int r, int s, long long base, int n0, const Mat &A, const float *w,
+                                         const float2 *hx, float xi) {
+  int c[W];
+  float a[W];
+#pragma unroll
+  for (int j = 0; j < W; j++) c[j] = col_at<C16>(A, base + (long long)j * kSlice, r, s);
+#pragma unroll
+  for (int j = 0; j < W; j++) a[j] = w[base + (long long)j * kSlice];
+  float2 h[W];
+#pragma unroll
+  for (int j = 0; j < W; j++) h[j] = c[j] < n0 ? hx[c[j]] : make_float2(0.f, 0.f);
+  float acc = 0.f;
+#pragma unroll
+  for (int j = 0; j < W; j++) acc = acc + a[j] * (c[j] < n0 ? (xi - h[j].y) + h[j].x : xi);
+  return acc;
+}
+
+template <int C16>
+__global__ void k_mf_stage2(int n0, int n1, Mat A, const float *w, const float *x, const float2 *hx, float *q,
+                            const KS *st, float *part) {
+  if (st->done) return;
+  const int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  float v[1] = {0.f};
+  if (i < n1) {
+    const int r = n0 + i, s = r >> 6;
+    const long long o0 = A.off[s];
+    const int wd = (int)((A.off[s + 1] - o0) >> 6);
+    const long long base = o0 + (r & 63);
+    const float xi = x[i];
+    float acc;
+    if (wd == 7) acc = mf2_row<C16, 7>(r, s, base, n0, A, w, hx, xi);
+    else if (wd == 6) acc = mf2_row<C16, 6>(r, s, base, n0, A, w, hx, xi);
+    else if (wd == 5) acc = mf2_row<C16, 5>(r, s, base, n0, A, w, hx, xi);
+    else {
+      acc = 0.f;
+      for (int j = 0; j < wd; j++) {
+        const long long qq = base + (long long)j * kSlice;
+        const int c = col_at<C16>(A, qq, r, s);
+        if (c < n0) {
+          const float2 h = hx[c];
+          acc = acc + w[qq] * ((xi - h.y) + h.x);
+        } else {
+          acc = acc + w[qq] * xi;
+        }
+      }
+    }
+    q[i] = acc;
+    v[0] = xi * acc;
   }
   block_partials<OpSum, 1>(v, part);
 }
@@ -794,6 +899,51 @@ void Krylov::spmv(const double *val, const double *x, double *y) {
   MS_CHECK_LAUNCH();
 }
 
+void Krylov::spmv32(const double *val64, const float *v32, const float *p, float *q, KS *st, float *part,
+                    hipEvent_t e0, hipEvent_t e1) {
+  (void)val64;
+  const dim3 G(nblk_), B(kBlock);
+  hipStream_t S = stream();
+  if (matfree_on()) {
+    const MatFree &M = *mf_;
+    const Mat AM = matof(M.A);
+    const dim3 G0(nblocks(M.n0));
+    const bool c16 = M.A->has16;
+    if (e0) {
+      if (c16) hipExtLaunchKernelGGL((k_mf_stage1<1>), G0, B, 0, S, e0, nullptr, 0, M.n0, AM, M.w, p, M.hx, st);
+      else hipExtLaunchKernelGGL((k_mf_stage1<0>), G0, B, 0, S, e0, nullptr, 0, M.n0, AM, M.w, p, M.hx, st);
+    } else if (c16) {
+      hipLaunchKernelGGL((k_mf_stage1<1>), G0, B, 0, S, M.n0, AM, M.w, p, M.hx, st);
+    } else {
+      hipLaunchKernelGGL((k_mf_stage1<0>), G0, B, 0, S, M.n0, AM, M.w, p, M.hx, st);
+    }
+    if (e1) {
+      if (c16)
+        hipExtLaunchKernelGGL((k_mf_stage2<1>), G, B, 0, S, nullptr, e1, 0, M.n0, M.n1, AM, M.w, p, M.hx, q, st,
+                              part);
+      else
+        hipExtLaunchKernelGGL((k_mf_stage2<0>), G, B, 0, S, nullptr, e1, 0, M.n0, M.n1, AM, M.w, p, M.hx, q, st,
+                              part);
+    } else if (c16) {
+      hipLaunchKernelGGL((k_mf_stage2<1>), G, B, 0, S, M.n0, M.n1, AM, M.w, p, M.hx, q, st, part);
+    } else {
+      hipLaunchKernelGGL((k_mf_stage2<0>), G, B, 0, S, M.n0, M.n1, AM, M.w, p, M.hx, q, st, part);
+    }
+    return;
+  }
+  const Mat A = matof(pat);
+  if (e0 || e1) {
+    if (pat->has16)
+      hipExtLaunchKernelGGL((k_pcg_spmv<float, 1>), G, B, 0, S, e0, e1, 0, n, A, v32, p, q, st, part);
+    else
+      hipExtLaunchKernelGGL((k_pcg_spmv<float, 0>), G, B, 0, S, e0, e1, 0, n, A, v32, p, q, st, part);
+  } else if (pat->has16) {
+    hipLaunchKernelGGL((k_pcg_spmv<float, 1>), G, B, 0, S, n, A, v32, p, q, st, part);
+  } else {
+    hipLaunchKernelGGL((k_pcg_spmv<float, 0>), G, B, 0, S, n, A, v32, p, q, st, part);
+  }
+}
+
 double Krylov::time_spmv(const double *val, bool fp32, int reps, bool cold) {
   Mat A = matof(pat);
   const dim3 G(nblk_), B(kBlock);
@@ -811,10 +961,7 @@ double Krylov::time_spmv(const double *val, bool fp32, int reps, bool cold) {
   }
   auto launch = [&]() {
     if (fp32)
-      if (pat->has16)
-        hipLaunchKernelGGL((k_pcg_spmv<float, 1>), G, B, 0, S, n, A, val32_.p, p32_.p, q32_.p, st_.p, part32_.p);
-      else
-        hipLaunchKernelGGL((k_pcg_spmv<float, 0>), G, B, 0, S, n, A, val32_.p, p32_.p, q32_.p, st_.p, part32_.p);
+      spmv32(val, val32_.p, p32_.p, q32_.p, st_.p, part32_.p, nullptr, nullptr);
     else
       hipLaunchKernelGGL(k_pcg_spmv<double>, G, B, 0, S, n, A, val, p_.p, q_.p, st_.p, part_.p);
   };
@@ -1384,6 +1531,18 @@ void read_spmv_probe(long long *launches, double *ms, long long *rows) {
 }
 
 // ---- mixed precision ------------------------------------------------------------
+static std::atomic<int> g_schur_mf{-1};   // -1: not yet read from the environment
+bool schur_matfree_enabled() {
+  int v = g_schur_mf.load();
+  if (v < 0) {
+    const char *e = std::getenv("MIXSOLVER_SCHUR_MF");
+    v = (e && e[0] == '0') ? 0 : 1;
+    g_schur_mf.store(v);
+  }
+  return v != 0;
+}
+void set_schur_matfree(bool on) { g_schur_mf.store(on ? 1 : 0); }
+
 static std::atomic<int> g_stall_window{1024};
 void set_stall_window(int iters) { g_stall_window.store(iters > 0 ? iters : 1024); }
 int stall_window() { return g_stall_window.load(); }
@@ -1580,14 +1739,9 @@ SolveResult Krylov::mixed(const double *val, const double *b, double *x, double 
         probe_pair = g_probe.next;
         g_probe.next = (g_probe.next + 1) % kProbePairs;
         hipEvent_t e0 = g_probe.ev[2 * probe_pair], e1 = g_probe.ev[2 * probe_pair + 1];
-        if (pat->has16)
-          hipExtLaunchKernelGGL((k_pcg_spmv<float, 1>), G, B, 0, S, e0, e1, 0, n, A, v32, p, q, st, spart);
-        else
-          hipExtLaunchKernelGGL((k_pcg_spmv<float, 0>), G, B, 0, S, e0, e1, 0, n, A, v32, p, q, st, spart);
-      } else if (pat->has16) {
-        hipLaunchKernelGGL((k_pcg_spmv<float, 1>), G, B, 0, S, n, A, v32, p, q, st, spart);
+        spmv32(val, v32, p, q, st, spart, e0, e1);
       } else {
-        hipLaunchKernelGGL((k_pcg_spmv<float, 0>), G, B, 0, S, n, A, v32, p, q, st, spart);
+        spmv32(val, v32, p, q, st, spart, nullptr, nullptr);
       }
       if (split) {   // alpha reduced inside the update (the SpMV's partials go to a buffer of their own)
         hipLaunchKernelGGL(k_pcg_update_fa<float>, GR, B, 0, S, RW, nblk_, pq32_.p, p, q, y, r, dg, st, part32);
@@ -1619,7 +1773,7 @@ SolveResult Krylov::mixed(const double *val, const double *b, double *x, double 
       // chunk replays for every full chunk of every solve with this set-up;
       // with the probe on, the chunk's last iteration runs after the graph
       const int glen = probe ? kChunk - 1 : kChunk;
-      const ChunkKey ck{v32, fac, dg, nu, (int)pc, (int)coarse, (int)fused, glen};
+      const ChunkKey ck{v32, fac, dg, nu, (int)pc, (int)coarse, (int)fused, glen, (int)matfree_on()};
       if (!chunk_exec_ || !(chunk_key_ == ck)) {
         std::lock_guard<std::recursive_mutex> lk(device_alloc_mutex());
         if (chunk_exec_) MS_HIP(hipGraphExecDestroy(chunk_exec_));

@@ -40,6 +40,24 @@ void read_spmv_probe(long long *launches, double *ms, long long *rows);
 double time_copy(long long bytes, int reps);
 double time_stream(long long bytes, int kind, int reps);
 
+// Matrix-free product with the reduced matrix S = D1 - B^T D0^-1 B of a
+// two-colour system (rows [0,n0) colour 0, [n0,n0+n1) colour 1 of the full
+// matrix A), for the FP32 inner iterations of the reduced solve: two
+// launches over A's rows instead of one over the formed S.  w holds, in A's
+// SELL layout, -a_kj / a_kk off the diagonal and (row sum) / a_kk on it for
+// the colour-0 rows, -a_ij and the row sum for the colour-1 rows; hx holds
+// (h_k, x_ref_k) per colour-0 row between the two launches.
+struct MatFree {
+  int n0 = 0, n1 = 0;
+  const SellPattern *A = nullptr;
+  float *w = nullptr;
+  float2 *hx = nullptr;
+};
+// the reduced solve's FP32 iterations apply S matrix-free (default on;
+// MIXSOLVER_SCHUR_MF=0 or ms_set_schur_matfree(0) forms S for them too)
+bool schur_matfree_enabled();
+void set_schur_matfree(bool on);
+
 // Preconditioner kinds used internally.
 enum PcKind { PC_JACOBI = 0, PC_NATURAL_IC0 = 1, PC_NATURAL_ILU0 = 2, PC_DILU = 3 };
 
@@ -73,6 +91,9 @@ class Krylov {
   double time_spmv(const double *val, bool fp32, int reps, bool cold = false);
   long long launches = 0;
   bool fp32_stalled() const { return fp32_stalled_; }
+  // the FP32 SpMV of the mixed PCG applies this operator instead of val32
+  void set_matfree(const MatFree *m) { mf_ = m; }
+  bool matfree_on() const { return mf_ && schur_matfree_enabled(); }
 
   const SellPattern *pat;
   int n;
@@ -96,6 +117,11 @@ class Krylov {
   DBuf<int> flag_;
   CoarseMat cm_;             // E, E^-1 and nu of the coarse space for the current matrix
   DBuf<double> co_part_;     // chunk sums of a BiCGSTAB coarse correction
+  const MatFree *mf_ = nullptr;
+  // q = op p with the fused p.q partials; start / stop events (may be null)
+  // bracket the product's launches
+  void spmv32(const double *val64, const float *v32, const float *p, float *q, KS *st, float *part, hipEvent_t e0,
+              hipEvent_t e1);
   void *hpin_ = nullptr;
   hipEvent_t state_ev_[2] = {nullptr, nullptr};   // the two pipelined chunk states (mixed PCG)
   void *state_slot(int slot) const { return (char *)hpin_ + 1024 * (slot + 1); }
@@ -112,10 +138,10 @@ class Krylov {
   // arguments (buffers, preconditioner, coarse mode) are those of the key
   struct ChunkKey {
     const void *v32 = nullptr, *fac = nullptr, *dg = nullptr, *nu = nullptr;
-    int pc = -1, coarse = -1, fused = -1, len = 0;
+    int pc = -1, coarse = -1, fused = -1, len = 0, mf = 0;
     bool operator==(const ChunkKey &o) const {
       return v32 == o.v32 && fac == o.fac && dg == o.dg && nu == o.nu && pc == o.pc && coarse == o.coarse &&
-             fused == o.fused && len == o.len;
+             fused == o.fused && len == o.len && mf == o.mf;
     }
   } chunk_key_;
   hipGraphExec_t chunk_exec_ = nullptr;

@@ -143,6 +143,34 @@ __global__ void k_schur_zero(int n, double *x) {
   if (i < n) x[i] = 0.0;
 }
 
+// weights of the matrix-free S (MatFree) from the FP64 matrix A: colour-0
+// rows -a_kj / a_kk off the diagonal and (row sum) / a_kk on it, colour-1
+// rows -a_ij and the row sum; padding 0.  Quotients and sums in FP64.
+__global__ void k_mf_values(Mat A, const double *val, int n0, int n, float *w) {
+  const int r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= n) return;
+  const int s = r >> 6;
+  const long long o0 = A.off[s];
+  const int wd = (int)((A.off[s + 1] - o0) >> 6);
+  const long long base = o0 + (r & 63);
+  const int dk = A.diagk[r];
+  double d = 0.0, sum = 0.0;
+  for (int k = 0; k < wd; k++) {
+    const double a = val[base + (long long)k * kSlice];
+    sum = sum + a;
+    if (k == dk) d = a;
+  }
+  const bool c0 = r < n0;
+  for (int k = 0; k < wd; k++) {
+    const long long q = base + (long long)k * kSlice;
+    float v;
+    if (k == dk) v = c0 ? (d != 0.0 ? (float)(sum / d) : 0.f) : (float)sum;
+    else if (A.col[q] == r) v = 0.f;   // padding
+    else v = c0 ? (d != 0.0 ? (float)(-val[q] / d) : 0.f) : (float)(-val[q]);
+    w[q] = v;
+  }
+}
+
 // sum of the workgroup partials on the host, in index order
 double host_sum(SchurSystem &S, int nparts) {
   S.hpart.resize(nparts);
@@ -206,7 +234,15 @@ void build_schur(System &s, const std::vector<int> &agg_dev, int nagg) {
   S->d.alloc(n1);
   S->r64.alloc(n);
   S->part.alloc(nblocks(n) + 64);
+  attach_matfree(*S, s.P);
   s.schur = std::move(S);
+}
+
+void attach_matfree(SchurSystem &S, const SellPattern *A) {
+  S.mf_w.alloc(A->nstore);
+  S.mf_hx.alloc(std::max(S.n0, 1));
+  S.mf = MatFree{S.n0, S.n1, A, S.mf_w.p, S.mf_hx.p};
+  S.kry->set_matfree(&S.mf);
 }
 
 SolveResult schur_solve(System &s, const double *val, const double *b, double *x, double tol, int maxit, bool mixed,
@@ -226,6 +262,7 @@ SolveResult schur_solve(System &s, const double *val, const double *b, double *x
   }
   if (!(reuse && S.have_values)) {
     hipLaunchKernelGGL(k_schur_values, gc(n1), dim3(kBlock), 0, st, A, val, n0, n1, Sm, S.val.p);
+    if (mixed && S.kry->matfree_on()) hipLaunchKernelGGL(k_mf_values, gc(n), dim3(kBlock), 0, st, A, val, n0, n, S.mf_w.p);
     MS_CHECK_LAUNCH();
     S.have_values = true;
     reuse = false;

@@ -1403,6 +1403,7 @@ System *fork_system(System &b, const std::vector<ms_bcgroup_t> &groups) {
     R->d.alloc(R->n1);
     R->r64.alloc(s.nc);
     R->part.alloc(nblocks(s.nc) + 64);
+    attach_matfree(*R, s.P);
     s.schur = std::move(R);
   }
   MS_HIP(hipStreamSynchronize(st));

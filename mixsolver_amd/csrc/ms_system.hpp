@@ -76,6 +76,11 @@ struct SchurSystem {
   const SellPattern *P = &pat;  // the pattern in use (a forked system's points to its base's)
   std::unique_ptr<Krylov> kry;  // PCG on S (Jacobi, coarse space of the colour-1 aggregates)
   DBuf<double> val, rhs, d, r64, part;
+  // matrix-free S for the FP32 iterations: weights in A's layout and the
+  // colour-0 intermediate (MatFree)
+  DBuf<float> mf_w;
+  DBuf<float2> mf_hx;
+  MatFree mf;
   std::vector<double> hpart;
   bool have_values = false;     // val holds S of the last matrix
 };
@@ -268,6 +273,9 @@ int dump_read(const char *path, int ncells, double *const f[7], int *ts_order, s
 double norm2_ref(const double *v, int n);
 std::vector<int> rcb_aggregates(int n, const std::vector<double> &xc, int k);
 void build_schur(System &s, const std::vector<int> &agg_dev, int nagg);
+// the reduced system's matrix-free S over the full pattern A (buffers and
+// the Krylov hook)
+void attach_matfree(SchurSystem &S, const SellPattern *A);
 SolveResult schur_solve(System &s, const double *val, const double *b, double *x, double tol, int maxit, bool mixed,
                         double inner_tol, bool reuse);
 void cell_ordering(int nc, const std::vector<int> &ptr, const std::vector<int> &adj, const std::vector<double> &xc,

@@ -311,6 +311,12 @@ def set_fused_momentum(on: bool) -> None:
     check(_lib.lib().ms_set_fused_momentum(int(bool(on))))
 
 
+def set_schur_matfree(on: bool) -> None:
+    """FP32 iterations of the reduced p' solve apply S matrix-free (default)
+    or the formed S."""
+    check(_lib.lib().ms_set_schur_matfree(int(bool(on))))
+
+
 def set_stall_window(iters: int) -> None:
     """Iterations without a halving of the FP32 residual after which the
     mixed-precision PCG hands over to FP64 (0 = the default 1024)."""

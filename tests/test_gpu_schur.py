@@ -86,3 +86,36 @@ def test_schur_rejects_three_colour_graphs():
     refuses the system with a clear error."""
     with pytest.raises(MixSolverError, match="two-colour"):
         channel(41, 8, 4, True, gpu_ppe_precond=0, gpu_ppe_solver=2, gpu_mom_precond=2)
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_schur_matfree_equals_formed_s(periodic):
+    """The FP32 iterations of the reduced solve apply S = D1 - B^T D0^-1 B
+    matrix-free (two launches over A's rows) or as the formed S: both end
+    every p' solve at the FP64 1e-10 residual, take about the same number of
+    iterations (the FP32 rounding differs) and leave the same state to the
+    solver tolerance."""
+    from mixsolver_amd.solver import set_schur_matfree
+    dims = (48, 24, 12)
+    out = {}
+    try:
+        for mf in (False, True):
+            set_schur_matfree(mf)
+            cs = channel(*dims, periodic, gpu_ppe_precond=0, gpu_ppe_solver=2, gpu_mom_precond=2, gpu_precision=1,
+                         gpu_coarse=32)
+            assert cs.piso_run(2) == 0
+            out[mf] = (cs.history(), cs.get("u"), cs.get("p"), cs.solver_stats(), cs.time_kernel(reduced=True))
+            cs.close()
+    finally:
+        set_schur_matfree(True)
+    (h0, u0, p0, s0, k0), (h1, u1, p1, s1, k1) = out[False], out[True]
+    print("periodic", periodic, "its formed", h0[:, 6].tolist(), "matfree", h1[:, 6].tolist(), "kernel formed", k0,
+          "matfree", k1, flush=True)
+    assert np.all(h0[:, 7] <= 1e-10) and np.all(h1[:, 7] <= 1e-10)
+    assert abs(h1[:, 6].sum() - h0[:, 6].sum()) <= 0.05 * h0[:, 6].sum()
+    assert s1["ppe_fp64_solves"] == 0 and s1["ppe_iters_fp32"] > 0
+    assert rel_l2(u1, u0) <= 1e-8
+    if periodic:
+        p0, p1 = p0 - p0.mean(), p1 - p1.mean()
+    assert rel_l2(p1, p0) <= 1e-7
+    assert k0[0] > 0 and k1[0] > 0 and k1[1] > 0
