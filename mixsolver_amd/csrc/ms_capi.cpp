@@ -656,7 +656,9 @@ int ms_time_kernel(ms_system *h, int which, int reps, double *avg_ms, double *al
     const SellPattern &P = red ? s.schur->pat : s.pat;
     double t = K.time_spmv(red ? s.schur->val.p : s.val.p, fp32, reps, w >= 2);
     const double sz = fp32 ? 4.0 : 8.0;
-    const double idx = (fp32 && P.has16) ? 2.0 : 4.0;   // column index bytes of the format in use
+    // column index bytes of the format in use: 16-bit offsets plus the int32
+    // column of every far entry, or int32 columns
+    const double idx = (fp32 && P.has16) ? 2.0 + 4.0 * (double)P.nfar / (double)P.nnz : 4.0;
     if (avg_ms) *avg_ms = t;
     // algorithmic bytes of one SpMV: nnz (value + column index) + one int per row + x and y
     if (alg_bytes) *alg_bytes = (double)P.nnz * (sz + idx) + 4.0 * (P.n + 1) + 2.0 * P.n * sz;
@@ -666,7 +668,8 @@ int ms_time_kernel(ms_system *h, int which, int reps, double *avg_ms, double *al
       // intermediate (h, x_ref) written and read, q written
       const SellPattern &F = *s.P;
       const double n0 = s.schur->n0, n1 = s.schur->n1;
-      *alg_bytes = (double)F.nnz * (4.0 + (F.has16 ? 2.0 : 4.0)) + 4.0 * (F.n + 1) + 2.0 * n1 * 4.0 +
+      *alg_bytes = (double)F.nnz * 4.0 + (F.has16 ? 2.0 * F.nnz + 4.0 * F.nfar : 4.0 * F.nnz) + 4.0 * (F.n + 1) +
+                   2.0 * n1 * 4.0 +
                    2.0 * n0 * 8.0 + n1 * 4.0;
     }
   });

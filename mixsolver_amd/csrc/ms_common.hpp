@@ -55,6 +55,12 @@ void set_graphs(bool on);
 // ms_set_fused_momentum(0) turns it off)
 bool momentum_fused_enabled();
 void set_momentum_fused(bool on);
+// 16-bit column offsets with far columns read from the int32 array
+// (MIXSOLVER_COL16_FAR=1 at start-up; default off: 16-bit offsets only for
+// patterns without far columns, since the far branch and its dependent load
+// cost more than the 2 bytes per entry they save, 15.98 against 17.80 PISO
+// steps/s at 2M cells)
+bool col16_far_enabled();
 // the mixed PCG queues its next chunk before reading the current chunk's
 // state (MIXSOLVER_PIPELINE=0 turns it off)
 bool pipeline_enabled();
@@ -185,9 +191,10 @@ struct SellPattern {
   DBuf<long long> d_off, d_tpos;
   DBuf<int> d_rowlen, d_col, d_diagk;
   // 16-bit column offsets for the FP32 kernels: col = cbase[slice] + col16,
-  // 0xFFFF = the row itself (diagonal and padding); used when every slice's
-  // off-diagonal columns span fewer than 65535 (has16)
+  // 0xFFFF = the row itself (diagonal and padding), 0xFFFE = a far column
+  // outside the slice's window, read from col (has16; nfar such entries)
   std::vector<unsigned short> col16;
+  long long nfar = 0;
   std::vector<int> cbase;
   DBuf<unsigned short> d_col16;
   DBuf<int> d_cbase;

@@ -23,7 +23,7 @@ struct Mat {
 __device__ __forceinline__ int col_f32(const Mat &A, long long q, int r, int s) {
   if (A.col16) {
     const unsigned c = A.col16[q];
-    return c == 0xFFFFu ? r : A.cbase[s] + (int)c;
+    return c < 0xFFFEu ? A.cbase[s] + (int)c : (c == 0xFFFFu ? r : A.col[q]);
   }
   return A.col[q];
 }
@@ -36,7 +36,7 @@ template <int C16>
 __device__ __forceinline__ int col_at(const Mat &A, long long q, int r, int s) {
   if constexpr (C16 == 1) {
     const unsigned c = A.col16[q];
-    return c == 0xFFFFu ? r : A.cbase[s] + (int)c;
+    return c < 0xFFFEu ? A.cbase[s] + (int)c : (c == 0xFFFFu ? r : A.col[q]);
   } else if constexpr (C16 == 0) {
     return A.col[q];
   } else {

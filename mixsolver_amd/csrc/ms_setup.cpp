@@ -63,6 +63,13 @@ hipStream_t stream() {
   return g_stream;
 }
 void set_thread_stream(hipStream_t s) { t_stream = s; }
+bool col16_far_enabled() {
+  static const bool on = [] {
+    const char *e = std::getenv("MIXSOLVER_COL16_FAR");
+    return e && e[0] == '1';
+  }();
+  return on;
+}
 std::recursive_mutex &device_alloc_mutex() {
   static std::recursive_mutex m;
   return m;
@@ -177,22 +184,43 @@ void csr_to_sell(SellPattern &P, int n, const std::vector<int> &rp, const std::v
       P.tpos[P.slot(r, k)] = P.slot(c, (int)(f - b));
     }
   P.color_ptr = {0, n};
-  // 16-bit offsets per slice
+  // 16-bit offsets per slice: each slice takes the window of kCol16Span
+  // columns that holds most of its off-diagonal columns; columns outside it
+  // (far: neighbours across a periodic seam or an aggregate split plane)
+  // are marked and read from the int32 column array.  Used when at most a
+  // quarter of the off-diagonal entries are far.
+  constexpr int kCol16Span = 0xFFFD;   // offsets 0 .. 0xFFFD; 0xFFFE far, 0xFFFF the row itself
+  const bool allow_far = col16_far_enabled();
   P.cbase.assign(P.nslices, 0);
   P.has16 = true;
+  P.nfar = 0;
+  long long noff = 0;
+  std::vector<int> v;
+  std::vector<unsigned char> far_slice(P.nslices, 0);
   for (int s = 0; s < P.nslices && P.has16; s++) {
-    int lo = INT32_MAX, hi = -1;
+    v.clear();
     for (int r = s * kSlice; r < std::min(n, (s + 1) * kSlice); r++)
       for (int k = 0; k < P.rowlen[r]; k++) {
         const int c = P.col[P.slot(r, k)];
-        if (c == r) continue;
-        lo = std::min(lo, c);
-        hi = std::max(hi, c);
+        if (c != r) v.push_back(c);
       }
-    if (hi < 0) lo = hi = 0;
-    if (hi - lo >= 0xFFFF) P.has16 = false;
-    P.cbase[s] = lo;
+    noff += (long long)v.size();
+    if (v.empty()) continue;
+    std::sort(v.begin(), v.end());
+    size_t best = 0, bi = 0;
+    for (size_t i = 0, j = 0; j < v.size(); j++) {
+      while ((long long)v[j] - v[i] > kCol16Span) i++;
+      if (j - i + 1 > best) {
+        best = j - i + 1;
+        bi = i;
+      }
+    }
+    P.cbase[s] = v[bi];
+    const long long nf = (long long)(v.size() - best);
+    if (nf > 0 && !allow_far) P.has16 = false;
+    P.nfar += nf;
   }
+  if (P.has16 && 4 * P.nfar > noff) P.has16 = false;
   P.col16.clear();
   if (P.has16) {
     P.col16.assign(P.nstore, 0xFFFF);
@@ -200,8 +228,12 @@ void csr_to_sell(SellPattern &P, int n, const std::vector<int> &rp, const std::v
       for (int k = 0; k < P.rowlen[r]; k++) {
         const long long q = P.slot(r, k);
         const int c = P.col[q];
-        if (c != r) P.col16[q] = (unsigned short)(c - P.cbase[r / kSlice]);
+        if (c == r) continue;
+        const long long d = (long long)c - P.cbase[r / kSlice];
+        P.col16[q] = (d >= 0 && d <= kCol16Span) ? (unsigned short)d : (unsigned short)0xFFFE;
       }
+  } else {
+    P.nfar = 0;
   }
 }
 
