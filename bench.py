@@ -14,7 +14,8 @@ Workload (one step = one PISO time step of piso_run_mpi at one partition):
     eliminated and PCG runs on the Schur complement of colour 1 with the
     Jacobi preconditioner (the iterates of multicolour DILU-PCG) and the
     aggregate coarse correction, FP32 inner iterations inside FP64 reliable
-    updates;
+    updates; the FP32 iterations apply S matrix-free (two launches over the
+    full matrix's rows; MIXSOLVER_SCHUR_MF=0 uses the formed S);
   * fp64_schur: the same solver in FP64 (the precision ratio's reference);
   * mixed / fp64_two_level: PCG with multicolour DILU and the coarse
     correction on the full system, mixed and FP64;
@@ -511,8 +512,8 @@ def main():
                 "p": float(np.linalg.norm(pm - pf) / np.linalg.norm(pf))}
     solver_desc = {
         "mixed_schur": "p' red-black reduced PCG (colour 0 eliminated, Jacobi on the Schur complement = "
-                       "multicolour DILU iterates) + {k}-aggregate coarse correction, FP32 inner / FP64 reliable "
-                       "updates to 1e-10",
+                       "multicolour DILU iterates) + {k}-aggregate coarse correction, FP32 inner (S applied "
+                       "matrix-free) / FP64 reliable updates to 1e-10",
         "fp64_schur": "p' red-black reduced PCG + {k}-aggregate coarse correction, FP64, to 1e-10",
         "mixed": "p' PCG + multicolour DILU + {k}-aggregate coarse correction, FP32 inner / FP64 reliable updates "
                  "to 1e-10",
