@@ -118,6 +118,12 @@ int ms_set_fused_momentum(int on);
  * rounding, and so the iteration count, differ.  No reference counterpart
  * (library option). */
 int ms_set_schur_matfree(int on);
+/* With the matrix-free S, the reduced PCG's p update (p = z + nu + beta p)
+ * is applied inside the next product's two launches (1; MIXSOLVER_P_FUSED=1
+ * at start-up) or by a launch of its own (0, default: measured faster).
+ * Results are identical either way.  No reference counterpart (library
+ * option). */
+int ms_set_fused_p_update(int on);
 
 /* vtk_write (mod_uns_output.f90:81-240, tetrahedral fan) through vtu_write
  * (mod_vtu_writer.f90): binary = 0 inline ASCII (Float64, ES24.16E3),

@@ -64,6 +64,7 @@ def lib():
         "ms_set_graphs": (C.c_int, [C.c_int]),
         "ms_set_fused_momentum": (C.c_int, [C.c_int]),
         "ms_set_schur_matfree": (C.c_int, [C.c_int]),
+        "ms_set_fused_p_update": (C.c_int, [C.c_int]),
         "ms_get_ts_order": (C.c_int, [_V, _I]),
         "ms_nan_scan": (C.c_int, [_V, _I, _I, _D, _I, _I]),
         "ms_set_interface": (C.c_int, [_V, C.c_char_p, _D, C.c_longlong]),

@@ -122,6 +122,7 @@ int ms_version(void) { return 100; }
 int ms_set_graphs(int on) { return guarded([&] { set_graphs(on != 0); }); }
 int ms_set_fused_momentum(int on) { return guarded([&] { set_momentum_fused(on != 0); }); }
 int ms_set_schur_matfree(int on) { return guarded([&] { set_schur_matfree(on != 0); }); }
+int ms_set_fused_p_update(int on) { return guarded([&] { set_p_fused(on != 0); }); }
 
 int ms_dump_write(const char *path, int ncells, int ts_order, const char *source_mesh, const char *generated,
                   const double *u, const double *p, const double *T, const double *u_old, const double *u_old_old,

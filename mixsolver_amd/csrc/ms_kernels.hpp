@@ -313,6 +313,7 @@ struct FinUpdateSplit {
       st->done = 1;
     } else {
       st->rzd = (double)a[1];
+      st->pup = 1;   // the p update of this iteration is pending (k_pcg_p_split or the fused product)
     }
   }
 };

@@ -82,9 +82,48 @@ __global__ void k_pcg_spmv(int n, Mat A, const T *val, const T *p, T *q, const K
 // difference or a row-sum term, as in row_apply's difference form, so a
 // constant vector gives the row sums' product exactly.  The slot loop runs
 // the slice width of A (diagonal and padding slots have col = row).
-template <int C16, int W>
+// The pending p update of the split PCG (p = r/d + nu[agg] + beta p, as
+// k_pcg_p_split) can ride along the product (PU): both stages compute beta
+// from the coarse finisher's parts in every workgroup, stage 1 forms the
+// updated p of each neighbour it reads, stage 2 that of its own row and
+// stores it (stage 1 has finished reading the old p by then).  The values
+// are those k_pcg_p_split would store, bit for bit.
+__device__ __forceinline__ float pu_beta(KS *st, const double *cnu, int G, bool writer) {
+  __shared__ double parts[kSplitMaxParts + 2];
+  __shared__ float beta_f;
+  const int t = threadIdx.x;
+  if (t < G) parts[t] = cnu[t];
+  else if (t == kSplitMaxParts) parts[t] = st->rzd;
+  else if (t == kSplitMaxParts + 1) parts[t] = st->rz;
+  __syncthreads();
+  if (t == 0) {
+    double rzn = parts[kSplitMaxParts];
+    for (int g = 0; g < G; g++) rzn = rzn + parts[g];
+    const double beta = rzn / parts[kSplitMaxParts + 1];
+    beta_f = (float)beta;
+    if (writer) {
+      st->rz_next = rzn;
+      st->rzp = 1;
+      st->beta = beta;
+    }
+  }
+  __syncthreads();
+  return beta_f;
+}
+template <bool PU>
+__device__ __forceinline__ float pu_val(int j, const float *p, const PUp &u, float beta, bool up) {
+  if constexpr (PU) {
+    if (up) {
+      const float zj = u.r[j] / u.dg[j] + (float)u.nu[u.agg1[j]];
+      return zj + beta * p[j];
+    }
+  }
+  return p[j];
+}
+
+template <int C16, int W, bool PU>
 __device__ __forceinline__ float mf1_row(int k, int s, long long base, int n0, const Mat &A, const float *w,
-                                         const float *x, float xr) {
+                                         const float *x, float xr, const PUp &u, float beta, bool up) {
   int c[W];
   float a[W];
 #pragma unroll
@@ -93,13 +132,19 @@ __device__ __forceinline__ float mf1_row(int k, int s, long long base, int n0, c
   for (int j = 0; j < W; j++) a[j] = w[base + (long long)j * kSlice];
   float acc = 0.f;
 #pragma unroll
-  for (int j = 0; j < W; j++) acc = acc + a[j] * (c[j] < n0 ? xr : xr - x[c[j] - n0]);
+  for (int j = 0; j < W; j++) acc = acc + a[j] * (c[j] < n0 ? xr : xr - pu_val<PU>(c[j] - n0, x, u, beta, up));
   return acc;
 }
 
-template <int C16>
-__global__ void k_mf_stage1(int n0, Mat A, const float *w, const float *x, float2 *hx, const KS *st) {
+template <int C16, bool PU>
+__global__ void k_mf_stage1(int n0, Mat A, const float *w, const float *x, float2 *hx, KS *st, PUp u) {
   if (st->done) return;
+  float beta = 0.f;
+  bool up = false;
+  if constexpr (PU) {
+    up = st->pup != 0;   // the same for every workgroup: pup changes only in k_pcg_update_fa
+    if (up) beta = pu_beta(st, u.cnu, u.G, false);
+  }
   const int k = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (k >= n0) return;
   const int s = k >> 6;
@@ -108,17 +153,17 @@ __global__ void k_mf_stage1(int n0, Mat A, const float *w, const float *x, float
   const long long base = o0 + (k & 63);
   // slot 0 is the diagonal (colour-1 columns are above every colour-0 row)
   const int c1 = wd > 1 ? col_at<C16>(A, base + kSlice, k, s) : k;
-  const float xr = c1 >= n0 ? x[c1 - n0] : 0.f;
+  const float xr = c1 >= n0 ? pu_val<PU>(c1 - n0, x, u, beta, up) : 0.f;
   float acc;
-  if (wd == 7) acc = mf1_row<C16, 7>(k, s, base, n0, A, w, x, xr);
-  else if (wd == 6) acc = mf1_row<C16, 6>(k, s, base, n0, A, w, x, xr);
-  else if (wd == 5) acc = mf1_row<C16, 5>(k, s, base, n0, A, w, x, xr);
+  if (wd == 7) acc = mf1_row<C16, 7, PU>(k, s, base, n0, A, w, x, xr, u, beta, up);
+  else if (wd == 6) acc = mf1_row<C16, 6, PU>(k, s, base, n0, A, w, x, xr, u, beta, up);
+  else if (wd == 5) acc = mf1_row<C16, 5, PU>(k, s, base, n0, A, w, x, xr, u, beta, up);
   else {
     acc = 0.f;
     for (int j = 0; j < wd; j++) {
       const long long q = base + (long long)j * kSlice;
       const int c = col_at<C16>(A, q, k, s);
-      acc = acc + w[q] * (c < n0 ? xr : xr - x[c - n0]);
+      acc = acc + w[q] * (c < n0 ? xr : xr - pu_val<PU>(c - n0, x, u, beta, up));
     }
   }
   hx[k] = make_float2(acc, xr);
@@ -142,10 +187,16 @@ __device__ __forceinline__ float mf2_row(int r, int s, long long base, int n0, c
   return acc;
 }
 
-template <int C16>
-__global__ void k_mf_stage2(int n0, int n1, Mat A, const float *w, const float *x, const float2 *hx, float *q,
-                            const KS *st, float *part) {
+template <int C16, bool PU>
+__global__ void k_mf_stage2(int n0, int n1, Mat A, const float *w, float *x, const float2 *hx, float *q, KS *st,
+                            float *part, PUp u) {
   if (st->done) return;
+  float beta = 0.f;
+  bool up = false;
+  if constexpr (PU) {
+    up = st->pup != 0;
+    if (up) beta = pu_beta(st, u.cnu, u.G, blockIdx.x == 0);
+  }
   const int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   float v[1] = {0.f};
   if (i < n1) {
@@ -153,7 +204,8 @@ __global__ void k_mf_stage2(int n0, int n1, Mat A, const float *w, const float *
     const long long o0 = A.off[s];
     const int wd = (int)((A.off[s + 1] - o0) >> 6);
     const long long base = o0 + (r & 63);
-    const float xi = x[i];
+    const float xi = pu_val<PU>(i, x, u, beta, up);
+    if (up) x[i] = xi;
     float acc;
     if (wd == 7) acc = mf2_row<C16, 7>(r, s, base, n0, A, w, hx, xi);
     else if (wd == 6) acc = mf2_row<C16, 6>(r, s, base, n0, A, w, hx, xi);
@@ -251,6 +303,7 @@ __global__ void k_pcg_update_fa(Rows R, int np, const T *ppart, const T *p, cons
     if (blockIdx.x == 0) {
       st->it = st->it + 1;
       st->pap = pap;
+      st->pup = 0;   // the product before this kernel applied any pending p update
       if (bad) {
         st->ierr = 2;
         st->done = 1;
@@ -281,8 +334,9 @@ __global__ void k_pcg_update_fa(Rows R, int np, const T *ppart, const T *p, cons
 // leaves r.z in st->rz_next for the next finisher.
 template <class T>
 __global__ void k_pcg_p_split(Rows R, const T *r, const T *dg, T *p, const double *nu, KS *st,
-                              const double *cnu_part, int G) {
+                              const double *cnu_part, int G, int flush = 0) {
   if (st->done) return;
+  if (flush && !st->pup) return;   // flush: apply a pending update only
   // the parts, rzd and rz are loaded by different threads at once (one load
   // latency), then added by thread 0 in part order
   __shared__ double parts[kSplitMaxParts + 2], beta_s;
@@ -308,6 +362,10 @@ __global__ void k_pcg_p_split(Rows R, const T *r, const T *dg, T *p, const doubl
     const T zi = r[i] / dg[i] + (T)nu[a];
     p[i] = zi + (T)beta_s * p[i];
   }
+}
+
+__global__ void k_clear_pup(KS *st) {
+  if (threadIdx.x == 0) st->pup = 0;
 }
 
 template <class T>
@@ -899,8 +957,8 @@ void Krylov::spmv(const double *val, const double *x, double *y) {
   MS_CHECK_LAUNCH();
 }
 
-void Krylov::spmv32(const double *val64, const float *v32, const float *p, float *q, KS *st, float *part,
-                    hipEvent_t e0, hipEvent_t e1) {
+void Krylov::spmv32(const double *val64, const float *v32, float *p, float *q, KS *st, float *part,
+                    hipEvent_t e0, hipEvent_t e1, const PUp *pu) {
   (void)val64;
   const dim3 G(nblk_), B(kBlock);
   hipStream_t S = stream();
@@ -908,26 +966,22 @@ void Krylov::spmv32(const double *val64, const float *v32, const float *p, float
     const MatFree &M = *mf_;
     const Mat AM = matof(M.A);
     const dim3 G0(nblocks(M.n0));
+    const PUp u = pu ? *pu : PUp{};
+    // stage 1 then stage 2; the start event on the first launch, the stop
+    // event on the second
+    auto launch = [&](auto k1, auto k2) {
+      if (e0) hipExtLaunchKernelGGL(k1, G0, B, 0, S, e0, nullptr, 0, M.n0, AM, M.w, p, M.hx, st, u);
+      else hipLaunchKernelGGL(k1, G0, B, 0, S, M.n0, AM, M.w, p, M.hx, st, u);
+      if (e1) hipExtLaunchKernelGGL(k2, G, B, 0, S, nullptr, e1, 0, M.n0, M.n1, AM, M.w, p, M.hx, q, st, part, u);
+      else hipLaunchKernelGGL(k2, G, B, 0, S, M.n0, M.n1, AM, M.w, p, M.hx, q, st, part, u);
+    };
     const bool c16 = M.A->has16;
-    if (e0) {
-      if (c16) hipExtLaunchKernelGGL((k_mf_stage1<1>), G0, B, 0, S, e0, nullptr, 0, M.n0, AM, M.w, p, M.hx, st);
-      else hipExtLaunchKernelGGL((k_mf_stage1<0>), G0, B, 0, S, e0, nullptr, 0, M.n0, AM, M.w, p, M.hx, st);
-    } else if (c16) {
-      hipLaunchKernelGGL((k_mf_stage1<1>), G0, B, 0, S, M.n0, AM, M.w, p, M.hx, st);
+    if (pu) {
+      if (c16) launch(k_mf_stage1<1, true>, k_mf_stage2<1, true>);
+      else launch(k_mf_stage1<0, true>, k_mf_stage2<0, true>);
     } else {
-      hipLaunchKernelGGL((k_mf_stage1<0>), G0, B, 0, S, M.n0, AM, M.w, p, M.hx, st);
-    }
-    if (e1) {
-      if (c16)
-        hipExtLaunchKernelGGL((k_mf_stage2<1>), G, B, 0, S, nullptr, e1, 0, M.n0, M.n1, AM, M.w, p, M.hx, q, st,
-                              part);
-      else
-        hipExtLaunchKernelGGL((k_mf_stage2<0>), G, B, 0, S, nullptr, e1, 0, M.n0, M.n1, AM, M.w, p, M.hx, q, st,
-                              part);
-    } else if (c16) {
-      hipLaunchKernelGGL((k_mf_stage2<1>), G, B, 0, S, M.n0, M.n1, AM, M.w, p, M.hx, q, st, part);
-    } else {
-      hipLaunchKernelGGL((k_mf_stage2<0>), G, B, 0, S, M.n0, M.n1, AM, M.w, p, M.hx, q, st, part);
+      if (c16) launch(k_mf_stage1<1, false>, k_mf_stage2<1, false>);
+      else launch(k_mf_stage1<0, false>, k_mf_stage2<0, false>);
     }
     return;
   }
@@ -1542,6 +1596,17 @@ bool schur_matfree_enabled() {
   return v != 0;
 }
 void set_schur_matfree(bool on) { g_schur_mf.store(on ? 1 : 0); }
+static std::atomic<int> g_p_fused{-1};
+bool p_fused_enabled() {
+  int v = g_p_fused.load();
+  if (v < 0) {
+    const char *e = std::getenv("MIXSOLVER_P_FUSED");
+    v = (e && e[0] == '1') ? 1 : 0;
+    g_p_fused.store(v);
+  }
+  return v != 0;
+}
+void set_p_fused(bool on) { g_p_fused.store(on ? 1 : 0); }
 
 static std::atomic<int> g_stall_window{1024};
 void set_stall_window(int iters) { g_stall_window.store(iters > 0 ? iters : 1024); }
@@ -1693,11 +1758,15 @@ SolveResult Krylov::mixed(const double *val, const double *b, double *x, double 
   const bool split = coarse && !fused && mode == 1;
   const int nsplit = split ? (pat->coarse.k + kSplitRows - 1) / kSplitRows : 0;
   if (split && cnu_split_.n < (size_t)nsplit) cnu_split_.alloc(nsplit);
+  // the split path's p update rides along the next matrix-free product
+  const bool pfused = split && matfree_on() && mf_->agg1 && p_fused_enabled();
+  const PUp pu{r, dg, mf_ ? mf_->agg1 : nullptr, nu, cnu_split_.p, nsplit};
   auto tail = [&](int trigger) {
     if (split) {
       hipLaunchKernelGGL((k_finish_split<3, float, FinUpdateSplit>), dim3(1 + nsplit), dim3(kFinThreads), 0, S,
                          (int)GR.x, part32, FinUpdateSplit{st, trigger}, co, cnu_split_.p);
-      hipLaunchKernelGGL(k_pcg_p_split<float>, GR, B, 0, S, RW, r, dg, p, nu, st, cnu_split_.p, nsplit);
+      if (!pfused)
+        hipLaunchKernelGGL(k_pcg_p_split<float>, GR, B, 0, S, RW, r, dg, p, nu, st, cnu_split_.p, nsplit, 0);
       return;
     }
     if (merged) {
@@ -1718,6 +1787,10 @@ SolveResult Krylov::mixed(const double *val, const double *b, double *x, double 
     hipLaunchKernelGGL(k_pcg_p<float>, GR, B, 0, S, RW, r, z, dg, jac, p, nu, st);
   };
   auto reliable_update = [&](bool paused) {
+    if (pfused) {   // apply a pending p update before r, p and z are rescaled
+      hipLaunchKernelGGL(k_pcg_p_split<float>, GR, B, 0, S, RW, r, dg, p, nu, st, cnu_split_.p, nsplit, 1);
+      hipLaunchKernelGGL(k_clear_pup, dim3(1), dim3(64), 0, S, st);
+    }
     hipLaunchKernelGGL(k_fold, G, B, 0, S, n, y, x, st);
     hipLaunchKernelGGL(k_true_res, G, B, 0, S, n, A, val, b, x, r_.p, st, part);
     finish<OpSum, 1>(nblk_, part, FinRU{st});
@@ -1739,9 +1812,9 @@ SolveResult Krylov::mixed(const double *val, const double *b, double *x, double 
         probe_pair = g_probe.next;
         g_probe.next = (g_probe.next + 1) % kProbePairs;
         hipEvent_t e0 = g_probe.ev[2 * probe_pair], e1 = g_probe.ev[2 * probe_pair + 1];
-        spmv32(val, v32, p, q, st, spart, e0, e1);
+        spmv32(val, v32, p, q, st, spart, e0, e1, pfused ? &pu : nullptr);
       } else {
-        spmv32(val, v32, p, q, st, spart, nullptr, nullptr);
+        spmv32(val, v32, p, q, st, spart, nullptr, nullptr, pfused ? &pu : nullptr);
       }
       if (split) {   // alpha reduced inside the update (the SpMV's partials go to a buffer of their own)
         hipLaunchKernelGGL(k_pcg_update_fa<float>, GR, B, 0, S, RW, nblk_, pq32_.p, p, q, y, r, dg, st, part32);
@@ -1773,7 +1846,7 @@ SolveResult Krylov::mixed(const double *val, const double *b, double *x, double 
       // chunk replays for every full chunk of every solve with this set-up;
       // with the probe on, the chunk's last iteration runs after the graph
       const int glen = probe ? kChunk - 1 : kChunk;
-      const ChunkKey ck{v32, fac, dg, nu, (int)pc, (int)coarse, (int)fused, glen, (int)matfree_on()};
+      const ChunkKey ck{v32, fac, dg, nu, (int)pc, (int)coarse, (int)fused, glen, (int)matfree_on(), (int)pfused};
       if (!chunk_exec_ || !(chunk_key_ == ck)) {
         std::lock_guard<std::recursive_mutex> lk(device_alloc_mutex());
         if (chunk_exec_) MS_HIP(hipGraphExecDestroy(chunk_exec_));

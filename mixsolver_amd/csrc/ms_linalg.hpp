@@ -16,6 +16,7 @@ struct KS {              // device-resident solver scalars (one per solve)
   int coarse;            // coarse correction active
   int ru;                // reliable update requested for this iteration
   int nru;               // reliable updates performed
+  int pup;               // a p update is pending for the next product (fused p update)
 };
 
 struct SolveResult {
@@ -52,6 +53,21 @@ struct MatFree {
   const SellPattern *A = nullptr;
   float *w = nullptr;
   float2 *hx = nullptr;
+  const int *agg1 = nullptr;   // aggregate of every colour-1 row (coarse space), for the fused p update
+};
+// the pending p update p = r/d + nu[agg] + beta p, applied inside the next
+// matrix-free product instead of its own launch (MIXSOLVER_P_FUSED=1;
+// default off: the neighbours' r, d, aggregate and p gathered by the product
+// cost more than the launch they save, 15.87 against 17.68 PISO steps/s at
+// 2M cells)
+bool p_fused_enabled();
+void set_p_fused(bool on);
+// arguments of the fused p update (beta from the coarse finisher's parts)
+struct PUp {
+  const float *r, *dg;
+  const int *agg1;
+  const double *nu, *cnu;
+  int G;
 };
 // the reduced solve's FP32 iterations apply S matrix-free (default on;
 // MIXSOLVER_SCHUR_MF=0 or ms_set_schur_matfree(0) forms S for them too)
@@ -120,8 +136,8 @@ class Krylov {
   const MatFree *mf_ = nullptr;
   // q = op p with the fused p.q partials; start / stop events (may be null)
   // bracket the product's launches
-  void spmv32(const double *val64, const float *v32, const float *p, float *q, KS *st, float *part, hipEvent_t e0,
-              hipEvent_t e1);
+  void spmv32(const double *val64, const float *v32, float *p, float *q, KS *st, float *part, hipEvent_t e0,
+              hipEvent_t e1, const PUp *pu = nullptr);
   void *hpin_ = nullptr;
   hipEvent_t state_ev_[2] = {nullptr, nullptr};   // the two pipelined chunk states (mixed PCG)
   void *state_slot(int slot) const { return (char *)hpin_ + 1024 * (slot + 1); }
@@ -138,10 +154,10 @@ class Krylov {
   // arguments (buffers, preconditioner, coarse mode) are those of the key
   struct ChunkKey {
     const void *v32 = nullptr, *fac = nullptr, *dg = nullptr, *nu = nullptr;
-    int pc = -1, coarse = -1, fused = -1, len = 0, mf = 0;
+    int pc = -1, coarse = -1, fused = -1, len = 0, mf = 0, pf = 0;
     bool operator==(const ChunkKey &o) const {
       return v32 == o.v32 && fac == o.fac && dg == o.dg && nu == o.nu && pc == o.pc && coarse == o.coarse &&
-             fused == o.fused && len == o.len && mf == o.mf;
+             fused == o.fused && len == o.len && mf == o.mf && pf == o.pf;
     }
   } chunk_key_;
   hipGraphExec_t chunk_exec_ = nullptr;

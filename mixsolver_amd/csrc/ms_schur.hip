@@ -241,7 +241,15 @@ void build_schur(System &s, const std::vector<int> &agg_dev, int nagg) {
 void attach_matfree(SchurSystem &S, const SellPattern *A) {
   S.mf_w.alloc(A->nstore);
   S.mf_hx.alloc(std::max(S.n0, 1));
-  S.mf = MatFree{S.n0, S.n1, A, S.mf_w.p, S.mf_hx.p};
+  S.mf = MatFree{S.n0, S.n1, A, S.mf_w.p, S.mf_hx.p, nullptr};
+  const Coarse &C = S.P->coarse;
+  if (C.k > 0) {   // aggregate of every colour-1 row, from the aggregate chunks
+    std::vector<int> agg(S.n1, 0);
+    for (int c = 0; c < C.nchunks; c++)
+      for (int i = C.chunk_ptr[c]; i < C.chunk_ptr[c + 1]; i++) agg[i] = C.chunk_agg[c];
+    S.mf_agg.upload(agg);
+    S.mf.agg1 = S.mf_agg.p;
+  }
   S.kry->set_matfree(&S.mf);
 }
 

@@ -80,6 +80,7 @@ struct SchurSystem {
   // colour-0 intermediate (MatFree)
   DBuf<float> mf_w;
   DBuf<float2> mf_hx;
+  DBuf<int> mf_agg;   // aggregate of every colour-1 row (the fused p update)
   MatFree mf;
   std::vector<double> hpart;
   bool have_values = false;     // val holds S of the last matrix

@@ -317,6 +317,12 @@ def set_schur_matfree(on: bool) -> None:
     check(_lib.lib().ms_set_schur_matfree(int(bool(on))))
 
 
+def set_fused_p_update(on: bool) -> None:
+    """With the matrix-free S, apply the reduced PCG's p update inside the
+    next product or by a launch of its own (default); results identical."""
+    check(_lib.lib().ms_set_fused_p_update(int(bool(on))))
+
+
 def set_stall_window(iters: int) -> None:
     """Iterations without a halving of the FP32 residual after which the
     mixed-precision PCG hands over to FP64 (0 = the default 1024)."""

@@ -119,3 +119,28 @@ def test_schur_matfree_equals_formed_s(periodic):
         p0, p1 = p0 - p0.mean(), p1 - p1.mean()
     assert rel_l2(p1, p0) <= 1e-7
     assert k0[0] > 0 and k1[0] > 0 and k1[1] > 0
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_fused_p_update_equals_separate_launch(periodic):
+    """The reduced PCG's p update applied inside the next matrix-free product
+    gives the same iterates as its own launch: histories and fields equal bit
+    for bit, including solves with reliable updates (tight inner tolerance)."""
+    from mixsolver_amd.solver import set_fused_p_update
+    dims = (48, 24, 12)
+    out = {}
+    try:
+        for on in (False, True):
+            set_fused_p_update(on)
+            cs = channel(*dims, periodic, gpu_ppe_precond=0, gpu_ppe_solver=2, gpu_mom_precond=2, gpu_precision=1,
+                         gpu_coarse=32, gpu_inner_tol=0.01)
+            assert cs.piso_run(2) == 0
+            out[on] = (cs.history(), cs.get("u"), cs.get("p"), cs.solver_stats())
+            cs.close()
+    finally:
+        set_fused_p_update(False)
+    (h0, u0, p0, s0), (h1, u1, p1, s1) = out[False], out[True]
+    print("periodic", periodic, "its", h0[:, 6].tolist(), h1[:, 6].tolist(), s1, flush=True)
+    assert np.array_equal(h0, h1)
+    assert np.array_equal(u0, u1) and np.array_equal(p0, p1)
+    assert s1["ppe_iters_fp32"] > 0 and s1["ppe_fp64_solves"] == 0
