@@ -121,18 +121,23 @@ __device__ __forceinline__ float pu_val(int j, const float *p, const PUp &u, flo
   return p[j];
 }
 
+// x_ref (slot 1's column, the row's first colour-1 neighbour) is taken from
+// the columns already loaded, so its gather is issued with the others
 template <int C16, int W, bool PU>
 __device__ __forceinline__ float mf1_row(int k, int s, long long base, int n0, const Mat &A, const float *w,
-                                         const float *x, float xr, const PUp &u, float beta, bool up) {
+                                         const float *x, float &xr, const PUp &u, float beta, bool up) {
   int c[W];
-  float a[W];
+  float a[W], xv[W];
 #pragma unroll
   for (int j = 0; j < W; j++) c[j] = col_at<C16>(A, base + (long long)j * kSlice, k, s);
 #pragma unroll
   for (int j = 0; j < W; j++) a[j] = w[base + (long long)j * kSlice];
+#pragma unroll
+  for (int j = 0; j < W; j++) xv[j] = c[j] < n0 ? 0.f : pu_val<PU>(c[j] - n0, x, u, beta, up);
+  xr = c[1] >= n0 ? xv[1] : 0.f;
   float acc = 0.f;
 #pragma unroll
-  for (int j = 0; j < W; j++) acc = acc + a[j] * (c[j] < n0 ? xr : xr - pu_val<PU>(c[j] - n0, x, u, beta, up));
+  for (int j = 0; j < W; j++) acc = acc + a[j] * (c[j] < n0 ? xr : xr - xv[j]);
   return acc;
 }
 
@@ -152,13 +157,13 @@ __global__ void k_mf_stage1(int n0, Mat A, const float *w, const float *x, float
   const int wd = (int)((A.off[s + 1] - o0) >> 6);
   const long long base = o0 + (k & 63);
   // slot 0 is the diagonal (colour-1 columns are above every colour-0 row)
-  const int c1 = wd > 1 ? col_at<C16>(A, base + kSlice, k, s) : k;
-  const float xr = c1 >= n0 ? pu_val<PU>(c1 - n0, x, u, beta, up) : 0.f;
-  float acc;
+  float xr = 0.f, acc;
   if (wd == 7) acc = mf1_row<C16, 7, PU>(k, s, base, n0, A, w, x, xr, u, beta, up);
   else if (wd == 6) acc = mf1_row<C16, 6, PU>(k, s, base, n0, A, w, x, xr, u, beta, up);
   else if (wd == 5) acc = mf1_row<C16, 5, PU>(k, s, base, n0, A, w, x, xr, u, beta, up);
   else {
+    const int c1 = wd > 1 ? col_at<C16>(A, base + kSlice, k, s) : k;
+    xr = c1 >= n0 ? pu_val<PU>(c1 - n0, x, u, beta, up) : 0.f;
     acc = 0.f;
     for (int j = 0; j < wd; j++) {
       const long long q = base + (long long)j * kSlice;
