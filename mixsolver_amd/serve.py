@@ -46,7 +46,7 @@ class Sweep:
     ``Case`` with a transient control; its fields are the cases' initial
     fields).  ``budget_bytes`` 0: 90% of the free device memory."""
 
-    def __init__(self, base, budget_bytes=0, max_active=8):
+    def __init__(self, base, budget_bytes=0, max_active=4):
         self.base = base
         self.h = C.c_void_p()
         check(_lib.lib().ms_sweep_create(C.byref(self.h), base.h, int(budget_bytes), int(max_active)))
