@@ -166,7 +166,7 @@ struct Coarse {
 // coarse solution nu (k).  Owned by the solver, so systems that share a
 // pattern (forked sweep cases) keep their own.
 struct CoarseMat {
-  DBuf<double> E, nu, W, rc;
+  DBuf<double> E, nu, W, rc, Eparts;
   DBuf<float> Einv;
   void ensure(int k) {
     const size_t kk = (size_t)k * k;

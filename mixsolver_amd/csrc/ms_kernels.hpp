@@ -735,6 +735,32 @@ __global__ void k_coarse_E(int nE, const int *e_ptr, const int *e_key, const int
   if (lane == 0) E[e_key[e]] = s;
 }
 
+// E from the slot map in two passes, for entries with many slots: kEParts
+// wavefronts per entry each sum an interleaved share of its slots (lane
+// order, then shuffles), and one thread per entry adds the parts in part
+// order; the fixed order keeps E deterministic, and the entry's slots are
+// spread over kEParts times more wavefronts than in k_coarse_E.
+constexpr int kEParts = 16;
+__global__ void k_coarse_E_parts(int nE, const int *e_ptr, const int *e_slot, const double *val, double *parts) {
+  const long long w = ((long long)blockIdx.x * kBlock + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  const long long e = w / kEParts;
+  const int part = (int)(w % kEParts);
+  if (e >= nE) return;
+  double s = 0.0;
+  for (int j = e_ptr[e] + part * 64 + lane; j < e_ptr[e + 1]; j += 64 * kEParts) s = s + val[e_slot[j]];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s = s + __shfl_down(s, o, 64);
+  if (lane == 0) parts[e * kEParts + part] = s;
+}
+__global__ void k_coarse_E_sum(int nE, const int *e_key, const double *parts, double *E) {
+  const int e = blockIdx.x * kBlock + threadIdx.x;
+  if (e >= nE) return;
+  double s = parts[(long long)e * kEParts];
+  for (int q = 1; q < kEParts; q++) s = s + parts[(long long)e * kEParts + q];
+  E[e_key[e]] = s;
+}
+
 // Gauss-Jordan inversion of the SPD matrix E (no pivoting) in FP64 over the
 // whole grid, one launch per pivot j: W holds the current matrix; row j and
 // column j of it come from snapshots written by the previous launch (rowj,

@@ -1269,8 +1269,11 @@ void Krylov::coarse_setup(const double *val) {
   const long long kk = (long long)C.k * C.k;
   const int nE = (int)C.e_key.size();
   hipLaunchKernelGGL(k_fill64, dim3(nblocks(kk)), dim3(kBlock), 0, stream(), kk, M.E.p, 0.0);
-  hipLaunchKernelGGL(k_coarse_E, dim3(nblocks(nE, kBlock / 64)), dim3(kBlock), 0, stream(), nE, C.d_e_ptr.p,
-                     C.d_e_key.p, C.d_e_slot.p, val, M.E.p);
+  if (M.Eparts.n < (size_t)nE * kEParts) M.Eparts.alloc((size_t)nE * kEParts);
+  hipLaunchKernelGGL(k_coarse_E_parts, dim3(nblocks((long long)nE * kEParts, kBlock / 64)), dim3(kBlock), 0,
+                     stream(), nE, C.d_e_ptr.p, C.d_e_slot.p, val, M.Eparts.p);
+  hipLaunchKernelGGL(k_coarse_E_sum, dim3(nblocks(nE)), dim3(kBlock), 0, stream(), nE, C.d_e_key.p, M.Eparts.p,
+                     M.E.p);
   if (C.k <= kInvRegMax && coarse_inv32_enabled()) {
     // one workgroup, E held in registers, Gauss-Jordan in FP32
     hipLaunchKernelGGL(k_coarse_inv, dim3(1), dim3(kInvThreads), 0, stream(), C.k, M.E.p, M.Einv.p);
